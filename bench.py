@@ -1,0 +1,189 @@
+"""Benchmark: g2 pixel*frames/s of the dense batched i8-MFMA path on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload headline]
+
+A step is one dense correlate of the whole workload: qmask gather of every bin into
+i8 slabs, one grouped i8-MFMA Gram launch over all bins with the fused tau
+reduction, exact denominators and g2.  Frames are generated on the GPU (seeded AR(1)
+speckle) before the timed region, so inputs are resident in HBM; g2, the int64
+numerators, sums and FP64 denominators stay in HBM.
+
+Printed as one JSON line.  Besides the metric it reports
+  roofline      the Gram kernel's achieved int8 ops/s (algorithmic ops: the upper
+                triangle with the diagonal, 2 * sum_b n_b * N (N+1)/2, per launch)
+                over its average duration from HIP events recorded around each
+                launch inside the timed region, against the 5.0 POP/s dense int8 peak;
+  cpu_baseline  the reference's numpy MatMul correlator (float32 sgemm + diagonal
+                loop, restated in oracle/) on one bin of the same stack, all host cores;
+  plain_fp32    this library's one-bin-per-launch FP32 comparator on the same stack.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("g2 pixel·frames/s at 514×1030×10k uint8, 20 bins; "
+          "p50/p99 mixed-job latency")
+INT8_PEAK_TOPS = 5000.0   # MI355X dense int8 MFMA, 5.0 POP/s
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def read_traffic(name):
+    """HBM bytes per launch measured with rocprofv3 --pmc, committed under profiles/."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    return None
+
+
+def cpu_baseline(frames_dev, qmask, target_s=20.0):
+    """Reference-equivalent numpy MatMul correlator on one bin of the same stack."""
+    import numpy as np
+    import torch
+    from oracle import g2 as og
+    n = frames_dev.shape[0]
+    counts = np.bincount(qmask.ravel())[1:]
+    # the bin whose size is closest to the mean bin size
+    b = int(np.argmin(np.abs(counts - counts.mean()))) + 1
+    idx = torch.from_numpy(np.flatnonzero(qmask.ravel() == b)).cuda()
+    a = frames_dev.view(n, -1).index_select(1, idx).cpu().numpy()   # (N, n_b) uint8
+    sub = a.reshape(n, 1, -1)
+    mask = np.ones((1, a.shape[1]), dtype=bool)
+    cores = len(os.sched_getaffinity(0))
+    t0 = time.perf_counter()
+    og.py_dense_correlator_f32(sub, mask)
+    dt = time.perf_counter() - t0
+    return {"value": a.shape[1] * n / dt, "unit": "pixel*frames/s", "cores": cores,
+            "kind": "port",
+            "sample": "numpy MatMulCorrelator (float32 sgemm + per-lag diagonal loop, "
+                      "oracle/g2.py) on bin %d (%d px) x %d frames of the same stack, "
+                      "%.1f s, OpenBLAS threads = %d" % (b, a.shape[1], n, dt, cores)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="headline")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--seed", type=int, default=2026)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build runs the whole workload on a "
+              "single MI355X (multi-GPU sharding is a design note, not a measured path).",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from dynamix_amd import _lib, synth
+    from dynamix_amd.correlator.common import Plan
+
+    torch.cuda.set_device(0)
+    wl = synth.WORKLOADS[args.workload]
+    shape, n = wl["shape"], wl["nframes"]
+    q = synth.workload_qmask(args.workload)
+    log("generating %s frames %s x %d on the GPU" % (args.workload, shape, n))
+    frames = synth.speckle_frames_device(shape, n, wl["mean"], wl["tau_c"], args.seed,
+                                         np.dtype(wl["dtype"]))
+    torch.cuda.synchronize()
+    plan = Plan(shape, n, q, np.dtype(wl["dtype"]))
+    nb = plan.n_bins
+    num = torch.empty((nb, n), dtype=torch.int64, device="cuda")
+    sums = torch.empty((nb, n), dtype=torch.int64, device="cuda")
+    den = torch.empty((nb, n), dtype=torch.float64, device="cuda")
+    g2 = torch.empty((nb, n), dtype=torch.float32, device="cuda")
+    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    lib = plan.lib
+
+    def step():
+        _lib.check(lib.dxg2_correlate_dense(plan.handle, _lib.ptr(frames), _lib.ptr(num),
+                                            _lib.ptr(sums), _lib.ptr(den), _lib.ptr(g2), stream))
+
+    log("warmup %d" % args.warmup)
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    plan.set_profiling(True)
+    log("timing %d steps" % args.steps)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    gather_ms, gram_ms, fin_ms, total_ms, ncalls = plan.stage_times()
+    plan.set_profiling(False)
+    ms = wall * 1e3 / args.steps
+    npix = int(plan.pixel_counts[:nb].sum())
+    pf = npix * n
+    ops = 2.0 * npix * n * (n + 1) / 2.0
+    achieved = ops / (gram_ms * 1e-3) / 1e12
+    log("ms/step %.3f  gather %.3f gram %.3f finalize %.3f  (%d calls)" %
+        (ms, gather_ms, gram_ms, fin_ms, ncalls))
+
+    # sanity: g2 finite where denominators are positive
+    assert torch.isfinite(g2[den > 0]).all().item()
+
+    out = {
+        "metric": METRIC,
+        "value": pf / (ms * 1e-3),
+        "unit": "pixel*frames/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "i8",
+        "data": "synthetic",
+        "config": {"workload": "%s: %dx%d x %d frames %s, %d annular bins, %d binned px"
+                               % (args.workload, shape[0], shape[1], n, wl["dtype"], nb, npix),
+                   "path": "dense batched i8-MFMA Gram (gather + gram + finalise)",
+                   "stage_ms": {"gather": gather_ms, "gram": gram_ms, "finalise": fin_ms}},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": INT8_PEAK_TOPS,
+                     "unit": "TOP/s", "frac": achieved / INT8_PEAK_TOPS,
+                     "traffic": read_traffic("gram_i8_%s_traffic.txt" % args.workload),
+                     "kernel": "gram_i8_kernel", "ops_per_launch": ops},
+    }
+
+    if not args.no_plain:
+        from dynamix_amd.correlator.dense import PlainMatMulCorrelator
+        log("plain fp32 comparator")
+        pc = PlainMatMulCorrelator(shape, n, q, extra_options={"return_device": True})
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        pg2 = pc.correlate(frames)
+        torch.cuda.synchronize()
+        pms = (time.perf_counter() - t0) * 1e3
+        fin = torch.isfinite(g2) & torch.isfinite(pg2)
+        rel = ((pg2[fin].double() - g2[fin].double()).abs() / g2[fin].double().abs()).max().item()
+        out["plain_fp32"] = {"ms": pms, "value": pf / (pms * 1e-3),
+                             "speedup_of_i8": pms / ms, "max_rel_diff_vs_i8": rel}
+        del pc, pg2
+        torch.cuda.empty_cache()
+
+    if not args.no_cpu:
+        log("cpu baseline")
+        out["cpu_baseline"] = cpu_baseline(frames, q)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,140 @@
+"""Dense correlators on the MI355X: drop-ins for dynamix/correlator/dense.py.
+
+MatMulCorrelator  -- dense.py:118-142.  Same constructor and correlate(frames,
+                     calc_std=False); the numerator is the grouped i8-MFMA Gram of
+                     every bin in one launch instead of one float32 sgemm per bin.
+DenseCorrelator   -- dense.py:145-247.  Same constructor (OpenCL device-selection
+                     arguments are accepted and ignored); same HIP path.
+PlainMatMulCorrelator -- the one-bin-per-launch FP32 form of
+                     CublasMatMulCorrelator (cuda.py:19-125), kept as the comparator.
+
+Results: float32 (n_bins, nframes), tau = 0 in column 0, NaN where the denominator is
+zero (an empty bin, or frames with no counts).  g2 = n_b * Num / Den with exact integer
+Num and Den and one FP64 division, i.e. within float32 rounding of the FP64 value.
+After each call `last_numerators`, `last_sums`, `last_denominators` (torch tensors on
+the GPU, int64/int64/float64) hold the exact intermediate results.
+"""
+from collections import namedtuple
+
+import numpy as np
+
+from .. import _lib
+from .common import BaseCorrelator, Plan, _torch, current_stream, to_device_frames
+
+CorrelationResult = namedtuple("CorrelationResult", "res dev")
+
+
+class _DenseHIP(BaseCorrelator):
+    """Shared machinery: one dxg2 plan per frame dtype, device outputs, host result."""
+
+    def _init_dense(self):
+        self._plans = {}
+        self.last_numerators = None
+        self.last_sums = None
+        self.last_denominators = None
+        self.last_g2 = None
+
+    def _plan(self, dtype):
+        dtype = np.dtype(dtype)
+        p = self._plans.get(dtype)
+        if p is None:
+            q = self.qmask if self._qmask_given else None
+            p = Plan(self.shape, self.nframes, q, dtype, scale=None)
+            self._plans[dtype] = p
+        return p
+
+    def _run_dense(self, frames):
+        torch = _torch()
+        d_frames, dtype = to_device_frames(frames, self.shape, self.nframes, torch)
+        if dtype not in (np.dtype(np.uint8), np.dtype(np.int8)):
+            raise NotImplementedError("the dense i8 path takes uint8 or int8 frames, got %s" % dtype)
+        plan = self._plan(dtype)
+        nb, n = max(self.n_bins, 0), self.nframes
+        num = torch.empty((nb, n), dtype=torch.int64, device="cuda")
+        sums = torch.empty((nb, n), dtype=torch.int64, device="cuda")
+        den = torch.empty((nb, n), dtype=torch.float64, device="cuda")
+        g2 = torch.empty((nb, n), dtype=torch.float32, device="cuda")
+        if nb:
+            _lib.check(plan.lib.dxg2_correlate_dense(plan.handle, _lib.ptr(d_frames), _lib.ptr(num),
+                                                     _lib.ptr(sums), _lib.ptr(den), _lib.ptr(g2),
+                                                     current_stream(torch)),
+                       "dxg2_correlate_dense")
+        self.last_numerators, self.last_sums, self.last_denominators, self.last_g2 = num, sums, den, g2
+        if self.extra_options.get("return_device", False):
+            return g2
+        return g2.cpu().numpy()
+
+
+class MatMulCorrelator(_DenseHIP):
+    """Drop-in for MatMulCorrelator (dense.py:118-142).  scale_factor is accepted and,
+    as in the reference, not used: g2 is normalised by the pixel count of each bin."""
+
+    def __init__(self, shape, nframes, qmask=None, scale_factor=None, extra_options={}):
+        super().__init__()
+        super()._set_parameters(shape, nframes, qmask, scale_factor, extra_options)
+        self._init_dense()
+
+    def correlate(self, frames, calc_std=False):
+        if calc_std:
+            raise NotImplementedError("calc_std is not implemented on the HIP path yet")
+        return self._run_dense(frames)
+
+
+class DenseCorrelator(_DenseHIP):
+    """Drop-in for the OpenCL DenseCorrelator (dense.py:145-247).
+
+    `dtype` is the frames dtype; correlate() asserts the frames match it, as the
+    reference's _set_data does (common.py:178-180).  ctx/devicetype/platformid/
+    deviceid/block_size/memory are OpenCL device selection and are ignored; `profile`
+    turns on per-stage HIP event timing (see `stage_times`).
+    """
+
+    def __init__(self, shape, nframes, qmask=None, dtype="f", weights=None, extra_options={},
+                 ctx=None, devicetype="all", platformid=None, deviceid=None,
+                 block_size=None, memory=None, profile=False):
+        super().__init__()
+        super()._set_parameters(shape, nframes, qmask, None, extra_options)
+        self._set_weights(weights)
+        self.dtype = np.dtype(dtype)
+        if self.dtype not in (np.dtype(np.uint8), np.dtype(np.int8)):
+            raise NotImplementedError("DenseCorrelator on the HIP path takes uint8 or int8 frames")
+        self.profile = bool(profile)
+        self._init_dense()
+        if self.profile:
+            self._plan(self.dtype).set_profiling(True)
+
+    def correlate(self, frames):
+        dt = frames.dtype if isinstance(frames, np.ndarray) else None
+        if dt is not None:
+            assert np.dtype(dt) == self.dtype, "frames dtype %s != %s" % (dt, self.dtype)
+        return self._run_dense(frames)
+
+    def stage_times(self):
+        """Average [gather, gram, finalise, total] ms and the number of calls averaged,
+        over the calls since the last read (profile=True)."""
+        return self._plan(self.dtype).stage_times()
+
+
+class PlainMatMulCorrelator(_DenseHIP):
+    """One bin per launch, FP32 Gram: the structure of CublasMatMulCorrelator
+    (cuda.py:101-125) written as this library's own HIP kernels.  Used as the
+    comparator of the batched integer path."""
+
+    def __init__(self, shape, nframes, qmask=None, scale_factor=None, extra_options={}):
+        super().__init__()
+        super()._set_parameters(shape, nframes, qmask, scale_factor, extra_options)
+        self._init_dense()
+
+    def correlate(self, frames):
+        torch = _torch()
+        d_frames, dtype = to_device_frames(frames, self.shape, self.nframes, torch)
+        plan = self._plan(dtype)
+        g2 = torch.empty((max(self.n_bins, 0), self.nframes), dtype=torch.float32, device="cuda")
+        if self.n_bins:
+            _lib.check(plan.lib.dxg2_correlate_dense_plain(plan.handle, _lib.ptr(d_frames),
+                                                           _lib.ptr(g2), current_stream(torch)),
+                       "dxg2_correlate_dense_plain")
+        self.last_g2 = g2
+        if self.extra_options.get("return_device", False):
+            return g2
+        return g2.cpu().numpy()

@@ -1,0 +1,420 @@
+// Plan construction (host) and the extern "C" entry points of libdxg2.so.
+//
+// A plan turns (shape, nframes, qmask, dtype, scale factors) into: per-bin raster-
+// ordered pixel index lists padded to k-blocks of 64, the i8 slab layout, the gather
+// work list and the Gram tile work list (one grouped launch for all bins), plus the
+// device workspaces.  Bins follow BaseCorrelator._set_qmask (correlator/common.py:36-46):
+// qmask NULL is one bin of every pixel, otherwise bins are 1..max(qmask); scale factors
+// follow _set_scale_factor (common.py:56-72): default pixel count per bin.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "dxg2_internal.h"
+
+using namespace dxg2;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int set_error(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+#define DXG2_HIP(call)                                                                     \
+    do {                                                                                   \
+        hipError_t e_ = (call);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return set_error(DXG2_EHIP, "%s failed: %s", #call, hipGetErrorString(e_));     \
+    } while (0)
+
+// Order in which the hardware hands workgroups to the 8 XCDs of an MI355X: workgroup
+// b runs on XCD b % 8.  remap_for_xcds() permutes a list so that XCD x receives the
+// chunks x, x+8, x+16, ... of `group` consecutive items of the original order: items
+// that share frame panels then run concurrently on one XCD and share its L2.
+constexpr int N_XCD = 8;
+
+template <typename T>
+std::vector<T> remap_for_xcds(const std::vector<T>& in, int group, const T& pad) {
+    const size_t n = in.size();
+    if (n == 0) return {};
+    const size_t n_groups = (n + group - 1) / group;
+    std::vector<std::vector<T>> per(N_XCD);
+    for (size_t g = 0; g < n_groups; ++g) {
+        auto& dst = per[g % N_XCD];
+        for (size_t i = g * group; i < std::min(n, (g + 1) * (size_t)group); ++i)
+            dst.push_back(in[i]);
+    }
+    size_t len = 0;
+    for (auto& v : per) len = std::max(len, v.size());
+    std::vector<T> out(len * N_XCD, pad);
+    for (int x = 0; x < N_XCD; ++x)
+        for (size_t r = 0; r < per[x].size(); ++r) out[r * N_XCD + x] = per[x][r];
+    return out;
+}
+
+}  // namespace
+
+struct dxg2_plan {
+    int H = 0, W = 0, N = 0, dtype = 0, n_bins = 0;
+    int device = 0;
+    int gram_tile = 128;
+    std::vector<int64_t> npix;
+    std::vector<double> scale;
+    std::vector<int32_t> pix_idx;
+    std::vector<BinDev> bins;
+    std::vector<GatherWork> gather_work;
+    std::vector<GramWork> gram_work;
+    size_t slab_bytes = 0;
+
+    // device (owned)
+    int32_t* d_pix = nullptr;
+    BinDev* d_bins = nullptr;
+    GatherWork* d_gwork = nullptr;
+    GramWork* d_mwork = nullptr;
+    int8_t* d_slab = nullptr;
+    int64_t* d_ugram = nullptr;
+    int64_t* d_sums = nullptr;
+    float* d_pa = nullptr;
+    float* d_pg = nullptr;
+    float* d_pm = nullptr;
+    size_t pa_elems = 0;
+
+    // Profiling: each correlate call records 4 events on its stream (no host sync);
+    // dxg2_plan_stage_times() waits for the last one and averages every recorded call.
+    bool profiling = false;
+    std::vector<std::array<hipEvent_t, 4>> ev_pool;
+    size_t ev_used = 0;
+};
+
+namespace {
+
+void build_gram_work(dxg2_plan* p) {
+    const int T = p->gram_tile;
+    const int nT = (p->N + T - 1) / T;
+    const int band = 8;
+    std::vector<int> order(p->n_bins);
+    for (int b = 0; b < p->n_bins; ++b) order[b] = b;
+    // Largest bins first so their long tiles start early (LPT).
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return p->bins[a].nkb > p->bins[b].nkb; });
+    std::vector<GramWork> raster;
+    for (int b : order) {
+        const int nkb = p->bins[b].nkb;
+        if (nkb == 0) continue;
+        for (int k0 = 0; k0 < nkb; k0 += MAX_KBLOCKS_PER_ITEM) {
+            const int k1 = std::min(nkb, k0 + MAX_KBLOCKS_PER_ITEM);
+            for (int i0 = 0; i0 < nT; i0 += band)
+                for (int j = i0; j < nT; ++j)
+                    for (int i = i0; i < std::min(nT, i0 + band) && i <= j; ++i)
+                        raster.push_back(GramWork{b, i, j, k0, k1, {0, 0, 0}});
+        }
+    }
+    GramWork pad{-1, 0, 0, 0, 0, {0, 0, 0}};
+    p->gram_work = remap_for_xcds(raster, 64, pad);
+}
+
+// Next unused event quad of the profiling pool (created on first use).
+int next_events(dxg2_plan* p, hipEvent_t** out) {
+    if (p->ev_used == p->ev_pool.size()) {
+        std::array<hipEvent_t, 4> q{};
+        for (auto& e : q) DXG2_HIP(hipEventCreate(&e));
+        p->ev_pool.push_back(q);
+    }
+    *out = p->ev_pool[p->ev_used++].data();
+    return DXG2_OK;
+}
+
+int ensure_dense_buffers(dxg2_plan* p) {
+    if (!p->d_slab && p->slab_bytes) {
+        if (hipMalloc(&p->d_slab, p->slab_bytes) != hipSuccess) {
+            p->d_slab = nullptr;
+            return set_error(DXG2_ENOMEM, "cannot allocate %zu bytes of slab", p->slab_bytes);
+        }
+    }
+    return DXG2_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* dxg2_last_error(void) { return g_last_error.c_str(); }
+
+const char* dxg2_version(void) { return "dxg2 0.1.0 gfx950"; }
+
+int dxg2_plan_create(const dxg2_plan_desc* desc, dxg2_plan** out) {
+    if (!desc || !out) return set_error(DXG2_EINVAL, "null argument");
+    *out = nullptr;
+    if (desc->height <= 0 || desc->width <= 0)
+        return set_error(DXG2_EINVAL, "bad frame shape (%d, %d)", desc->height, desc->width);
+    if (desc->nframes <= 0) return set_error(DXG2_EINVAL, "nframes must be > 0");
+    if (desc->dtype != DXG2_U8 && desc->dtype != DXG2_I8 && desc->dtype != DXG2_U16)
+        return set_error(DXG2_EINVAL, "unsupported dtype code %d", desc->dtype);
+    const int64_t hw = (int64_t)desc->height * desc->width;
+    if (hw > (int64_t)INT32_MAX) return set_error(DXG2_EINVAL, "frame too large");
+
+    dxg2_plan* p = new dxg2_plan();
+    p->H = desc->height;
+    p->W = desc->width;
+    p->N = desc->nframes;
+    p->dtype = desc->dtype;
+
+    // qmask -> bins (common.py:36-46)
+    std::vector<std::vector<int32_t>> lists;
+    if (!desc->qmask) {
+        p->n_bins = 1;
+        lists.resize(1);
+        lists[0].resize(hw);
+        for (int64_t i = 0; i < hw; ++i) lists[0][i] = (int32_t)i;
+    } else {
+        int32_t mx = 0;
+        for (int64_t i = 0; i < hw; ++i) mx = std::max(mx, desc->qmask[i]);
+        p->n_bins = mx;
+        lists.resize(mx);
+        for (int64_t i = 0; i < hw; ++i) {
+            const int32_t q = desc->qmask[i];
+            if (q > 0) lists[q - 1].push_back((int32_t)i);
+        }
+    }
+    // scale factors (common.py:56-72)
+    if (desc->scale_factors) {
+        if (desc->n_scale_factors != p->n_bins) {
+            const int nb = p->n_bins;
+            delete p;
+            return set_error(DXG2_EINVAL, "got %d scale factors for %d bins",
+                             desc->n_scale_factors, nb);
+        }
+    }
+    int64_t idx_off = 0, slab_off = 0;
+    for (int b = 0; b < p->n_bins; ++b) {
+        const int64_t n = (int64_t)lists[b].size();
+        const int nkb = (int)((n + KB - 1) / KB);
+        BinDev bd;
+        bd.slab_off = slab_off;
+        bd.idx_off = idx_off;
+        bd.npix = (int32_t)n;
+        bd.nkb = nkb;
+        bd.scale = desc->scale_factors ? desc->scale_factors[b] : (double)n;
+        p->bins.push_back(bd);
+        p->npix.push_back(n);
+        p->scale.push_back(bd.scale);
+        for (int64_t i = 0; i < (int64_t)nkb * KB; ++i)
+            p->pix_idx.push_back(i < n ? lists[b][i] : -1);
+        for (int kb0 = 0; kb0 < nkb; kb0 += 16)
+            p->gather_work.push_back(GatherWork{b, kb0, std::min(16, nkb - kb0), 0});
+        idx_off += (int64_t)nkb * KB;
+        slab_off += (int64_t)nkb * KB * p->N;
+    }
+    p->slab_bytes = (size_t)slab_off;
+    build_gram_work(p);
+
+    hipError_t e = hipGetDevice(&p->device);
+    if (e != hipSuccess) {
+        delete p;
+        return set_error(DXG2_EHIP, "hipGetDevice: %s", hipGetErrorString(e));
+    }
+    auto upload = [&](void** dst, const void* src, size_t bytes) -> int {
+        if (bytes == 0) return DXG2_OK;
+        if (hipMalloc(dst, bytes) != hipSuccess)
+            return set_error(DXG2_ENOMEM, "cannot allocate %zu bytes", bytes);
+        if (hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice) != hipSuccess)
+            return set_error(DXG2_EHIP, "upload of %zu bytes failed", bytes);
+        return DXG2_OK;
+    };
+    int rc = DXG2_OK;
+    if (!rc) rc = upload((void**)&p->d_pix, p->pix_idx.data(), p->pix_idx.size() * 4);
+    if (!rc) rc = upload((void**)&p->d_bins, p->bins.data(), p->bins.size() * sizeof(BinDev));
+    if (!rc)
+        rc = upload((void**)&p->d_gwork, p->gather_work.data(),
+                    p->gather_work.size() * sizeof(GatherWork));
+    if (!rc)
+        rc = upload((void**)&p->d_mwork, p->gram_work.data(),
+                    p->gram_work.size() * sizeof(GramWork));
+    const size_t nb_n = (size_t)std::max(p->n_bins, 1) * p->N * sizeof(int64_t);
+    if (!rc && hipMalloc(&p->d_ugram, nb_n) != hipSuccess)
+        rc = set_error(DXG2_ENOMEM, "cannot allocate numerator workspace");
+    if (!rc && hipMalloc(&p->d_sums, nb_n) != hipSuccess)
+        rc = set_error(DXG2_ENOMEM, "cannot allocate sums workspace");
+    if (rc) {
+        dxg2_plan_destroy(p);
+        return rc;
+    }
+    *out = p;
+    return DXG2_OK;
+}
+
+int dxg2_plan_destroy(dxg2_plan* p) {
+    if (!p) return DXG2_OK;
+    void* bufs[] = {p->d_pix, p->d_bins, p->d_gwork, p->d_mwork, p->d_slab,
+                    p->d_ugram, p->d_sums, p->d_pa, p->d_pg, p->d_pm};
+    for (void* b : bufs)
+        if (b) (void)hipFree(b);
+    for (auto& q : p->ev_pool)
+        for (auto e : q)
+            if (e) (void)hipEventDestroy(e);
+    delete p;
+    return DXG2_OK;
+}
+
+int dxg2_plan_query(const dxg2_plan* p, int32_t* n_bins, int64_t* pixel_counts, double* scale) {
+    if (!p) return set_error(DXG2_EINVAL, "null plan");
+    if (n_bins) *n_bins = p->n_bins;
+    for (int b = 0; b < p->n_bins; ++b) {
+        if (pixel_counts) pixel_counts[b] = p->npix[b];
+        if (scale) scale[b] = p->scale[b];
+    }
+    return DXG2_OK;
+}
+
+int dxg2_plan_footprint(const dxg2_plan* p, int32_t path, size_t* bytes) {
+    if (!p || !bytes) return set_error(DXG2_EINVAL, "null argument");
+    const size_t nb_n = (size_t)std::max(p->n_bins, 1) * p->N * sizeof(int64_t);
+    size_t common = p->pix_idx.size() * 4 + p->bins.size() * sizeof(BinDev) + 2 * nb_n;
+    switch (path) {
+        case 0:
+            *bytes = common + p->slab_bytes + p->gather_work.size() * sizeof(GatherWork) +
+                     p->gram_work.size() * sizeof(GramWork);
+            return DXG2_OK;
+        case 1: {
+            int64_t mx = 0;
+            for (auto n : p->npix) mx = std::max(mx, n);
+            *bytes = common + ((size_t)p->N * ((mx + 7) / 8 * 8) + (size_t)p->N * p->N + p->N) * 4;
+            return DXG2_OK;
+        }
+        default:
+            return set_error(DXG2_EINVAL, "unknown path %d", path);
+    }
+}
+
+int dxg2_plan_set_profiling(dxg2_plan* p, int32_t on) {
+    if (!p) return set_error(DXG2_EINVAL, "null plan");
+    p->profiling = on != 0;
+    p->ev_used = 0;
+    return DXG2_OK;
+}
+
+int dxg2_plan_stage_times(dxg2_plan* p, float* times, int32_t n) {
+    if (!p || !times) return set_error(DXG2_EINVAL, "null argument");
+    float acc[5] = {0, 0, 0, 0, 0};
+    if (p->ev_used) {
+        DXG2_HIP(hipEventSynchronize(p->ev_pool[p->ev_used - 1][3]));
+        for (size_t c = 0; c < p->ev_used; ++c) {
+            auto& q = p->ev_pool[c];
+            float ms;
+            for (int k = 0; k < 3; ++k) {
+                DXG2_HIP(hipEventElapsedTime(&ms, q[k], q[k + 1]));
+                acc[k] += ms;
+            }
+            DXG2_HIP(hipEventElapsedTime(&ms, q[0], q[3]));
+            acc[3] += ms;
+        }
+        for (int k = 0; k < 4; ++k) acc[k] /= (float)p->ev_used;
+        acc[4] = (float)p->ev_used;
+    }
+    for (int i = 0; i < n && i < 5; ++i) times[i] = acc[i];
+    p->ev_used = 0;
+    return DXG2_OK;
+}
+
+int dxg2_correlate_dense(dxg2_plan* p, const void* d_frames, int64_t* d_num, int64_t* d_sums,
+                         double* d_den, float* d_g2, void* stream) {
+    if (!p) return set_error(DXG2_EINVAL, "null plan");
+    if (!d_frames || !d_g2) return set_error(DXG2_EINVAL, "frames and g2 must be non-null");
+    if (p->dtype != DXG2_U8 && p->dtype != DXG2_I8)
+        return set_error(DXG2_EUNSUPPORTED, "dense i8 path supports uint8/int8 frames");
+    if (p->n_bins == 0) return DXG2_OK;
+    int rc = ensure_dense_buffers(p);
+    if (rc) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    int64_t* sums = d_sums ? d_sums : p->d_sums;
+    const int64_t nbn = (int64_t)p->n_bins * p->N;
+    (void)hipGetLastError();
+    hipEvent_t* ev = nullptr;
+    if (p->profiling && (rc = next_events(p, &ev))) return rc;
+    if (ev) DXG2_HIP(hipEventRecord(ev[0], s));
+    launch_fill_i64(sums, nbn, s);
+    launch_fill_i64(p->d_ugram, nbn, s);
+    launch_gather_i8((const uint8_t*)d_frames, p->dtype, (int64_t)p->H * p->W, p->N, p->d_pix,
+                     p->d_bins, p->d_gwork, (int)p->gather_work.size(), p->d_slab, sums, s);
+    if (ev) DXG2_HIP(hipEventRecord(ev[1], s));
+    launch_gram_i8(p->gram_tile, p->d_slab, p->d_bins, p->d_mwork, (int)p->gram_work.size(),
+                   p->N, p->d_ugram, s);
+    if (ev) DXG2_HIP(hipEventRecord(ev[2], s));
+    launch_finalize(sums, p->d_ugram, p->d_bins, p->n_bins, p->N, p->dtype == DXG2_U8 ? 1 : 0,
+                    d_num, d_den, d_g2, s);
+    if (ev) DXG2_HIP(hipEventRecord(ev[3], s));
+    DXG2_HIP(hipGetLastError());
+    return DXG2_OK;
+}
+
+int dxg2_correlate_dense_plain(dxg2_plan* p, const void* d_frames, float* d_g2, void* stream) {
+    if (!p) return set_error(DXG2_EINVAL, "null plan");
+    if (!d_frames || !d_g2) return set_error(DXG2_EINVAL, "frames and g2 must be non-null");
+    hipStream_t s = (hipStream_t)stream;
+    int64_t mx = 0;
+    for (auto n : p->npix) mx = std::max(mx, n);
+    const size_t lda = (size_t)((mx + 7) / 8 * 8);
+    if (!p->d_pa && lda > 0) {
+        p->pa_elems = (size_t)p->N * lda;
+        if (hipMalloc(&p->d_pa, p->pa_elems * 4) != hipSuccess ||
+            hipMalloc(&p->d_pg, (size_t)p->N * p->N * 4) != hipSuccess ||
+            hipMalloc(&p->d_pm, (size_t)p->N * 4) != hipSuccess)
+            return set_error(DXG2_ENOMEM, "cannot allocate plain-path workspace");
+    }
+    (void)hipGetLastError();
+    hipEvent_t* ev = nullptr;
+    int rc;
+    if (p->profiling && (rc = next_events(p, &ev))) return rc;
+    if (ev) DXG2_HIP(hipEventRecord(ev[0], s));
+    for (int b = 0; b < p->n_bins; ++b) {
+        float* row = d_g2 + (int64_t)b * p->N;
+        if (p->npix[b] == 0) {
+            std::vector<float> nan(p->N, std::nanf(""));
+            DXG2_HIP(hipMemcpyAsync(row, nan.data(), p->N * 4, hipMemcpyHostToDevice, s));
+            DXG2_HIP(hipStreamSynchronize(s));
+            continue;
+        }
+        launch_plain_bin((const uint8_t*)d_frames, p->dtype, (int64_t)p->H * p->W, p->N,
+                         p->d_pix + p->bins[b].idx_off, (int)p->npix[b], p->d_pa, p->d_pg,
+                         p->d_pm, row, s);
+    }
+    if (ev) {
+        DXG2_HIP(hipEventRecord(ev[1], s));
+        DXG2_HIP(hipEventRecord(ev[2], s));
+        DXG2_HIP(hipEventRecord(ev[3], s));
+    }
+    DXG2_HIP(hipGetLastError());
+    return DXG2_OK;
+}
+
+int dxg2_synth_speckle(void* d_frames, int32_t dtype, int32_t nframes, int32_t height,
+                       int32_t width, const float* d_mean_map, double tau_c, uint64_t seed,
+                       void* stream) {
+    if (!d_frames || !d_mean_map) return set_error(DXG2_EINVAL, "null argument");
+    if (nframes <= 0 || height <= 0 || width <= 0) return set_error(DXG2_EINVAL, "bad sizes");
+    if (dtype != DXG2_U8 && dtype != DXG2_I8 && dtype != DXG2_U16)
+        return set_error(DXG2_EINVAL, "unsupported dtype %d", dtype);
+    const float rho = tau_c > 0 ? (float)std::exp(-1.0 / tau_c) : 0.f;
+    (void)hipGetLastError();
+    launch_synth(d_frames, dtype, nframes, (int64_t)height * width, d_mean_map, rho, seed,
+                 (hipStream_t)stream);
+    DXG2_HIP(hipGetLastError());
+    return DXG2_OK;
+}
+
+}  // extern "C"

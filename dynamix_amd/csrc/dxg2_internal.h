@@ -1,0 +1,64 @@
+// Internal declarations shared by the dxg2 translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/dxg2.h"
+
+namespace dxg2 {
+
+// Slab geometry: every bin's gathered pixels are stored as i8, grouped in k-blocks of
+// KB pixels; a k-block holds all frames: slab[kb][t][KB].  A (rows x KB) panel of one
+// k-block is therefore one contiguous run of bytes.
+constexpr int KB = 64;
+// Largest number of k-blocks one Gram work item accumulates in int32 before it
+// flushes: |a*b| <= 2^14 for i8 operands, 2^14 * 2047 * 64 < 2^31.
+constexpr int MAX_KBLOCKS_PER_ITEM = 2047;
+
+struct BinDev {
+    int64_t slab_off;   // byte offset of the bin's slab
+    int64_t idx_off;    // offset of the bin's pixel index list (padded to kpad)
+    int32_t npix;       // n_b
+    int32_t nkb;        // k-blocks (kpad / KB)
+    double scale;       // g2 scale factor
+};
+
+struct GatherWork {      // one workgroup of the gather kernel (x dimension)
+    int32_t bin;
+    int32_t kb0;         // first k-block
+    int32_t nkb;         // k-blocks handled (<= 16)
+    int32_t pad;
+};
+
+struct GramWork {        // one output tile (i, j >= i) over a k-block range
+    int32_t bin;
+    int32_t ti;          // row tile
+    int32_t tj;          // column tile
+    int32_t kb0;
+    int32_t kb1;         // exclusive
+    int32_t pad[3];
+};
+
+// ------------------------------------------------------------------ launchers
+void launch_fill_i64(int64_t* p, int64_t n, hipStream_t s);
+void launch_gather_i8(const uint8_t* frames, int dtype, int64_t hw, int nframes,
+                      const int32_t* pix_idx, const BinDev* bins, const GatherWork* work,
+                      int n_work, int8_t* slab, int64_t* sums, hipStream_t s);
+void launch_gram_i8(int tile, const int8_t* slab, const BinDev* bins, const GramWork* work,
+                    int n_work, int nframes, int64_t* ugram, hipStream_t s);
+void launch_finalize(const int64_t* sums, const int64_t* ugram, const BinDev* bins,
+                     int n_bins, int nframes, int biased, int64_t* num, double* den,
+                     float* g2, hipStream_t s);
+void launch_plain_bin(const uint8_t* frames, int dtype, int64_t hw, int nframes,
+                      const int32_t* pix, int npix, float* a, float* gram, float* means,
+                      float* g2_row, hipStream_t s);
+void launch_synth(void* frames, int dtype, int nframes, int64_t hw, const float* mean_map,
+                  float rho, uint64_t seed, hipStream_t s);
+
+int gram_tile_size(int variant);
+
+}  // namespace dxg2

@@ -1,0 +1,114 @@
+/*
+ * dxg2 -- MI355X (gfx950) XPCS g2 correlators: C ABI.
+ *
+ * One shared library, libdxg2.so, built from dynamix_amd/csrc with hipcc.  Every entry
+ * point takes plain pointers and sizes; device pointers are borrowed (never freed) and
+ * every call is ordered on the HIP stream passed as `stream` (a hipStream_t, NULL for the
+ * default stream).  Functions return 0 on success or a DXG2_E* status; dxg2_last_error()
+ * then holds a thread-local message.  A plan is used by one host thread at a time.
+ *
+ * Output layout of every correlate call (device memory, row-major):
+ *   num  int64  (n_bins, nframes)   Num_b(tau), exact
+ *   sums int64  (n_bins, nframes)   S_b(t) per-frame bin sums, exact
+ *   den  double (n_bins, nframes)   Den_b(tau) = sum_t S_b(t) S_b(t+tau) (exact below 2^53)
+ *   g2   float  (n_bins, nframes)   scale_b * Num_b(tau) / Den_b(tau), tau = 0 in column 0
+ * Any of num/sums/den may be NULL.
+ *
+ * Reference interfaces replaced (silx-kit/dynamix 0.2.0, paths under dynamix/):
+ *   see the comment above each declaration.
+ */
+#ifndef DXG2_H
+#define DXG2_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define DXG2_OK 0
+#define DXG2_EINVAL 1      /* bad argument: maps to AssertionError/ValueError in Python */
+#define DXG2_ENOMEM 2      /* device allocation failed */
+#define DXG2_EHIP 3        /* HIP runtime error */
+#define DXG2_EUNSUPPORTED 4
+#define DXG2_ECAPACITY 5   /* more events than the plan's capacity (event.py:110-115) */
+
+/* element types of frames / event counts */
+#define DXG2_U8 1
+#define DXG2_I8 2
+#define DXG2_U16 3
+
+/* plan flags */
+#define DXG2_FLAG_NONE 0u
+
+typedef struct dxg2_plan dxg2_plan;
+
+typedef struct dxg2_plan_desc {
+    int32_t height;               /* frame rows (H) */
+    int32_t width;                /* frame columns (W) */
+    int32_t nframes;              /* N */
+    int32_t dtype;                /* DXG2_U8 / DXG2_I8 / DXG2_U16 */
+    const int32_t* qmask;         /* host, H*W; bins 1..max, <= 0 excluded; NULL = one bin */
+    const double* scale_factors;  /* host, n_bins values; NULL = pixel count of each bin */
+    int32_t n_scale_factors;      /* length of scale_factors (must equal n_bins if given) */
+    uint32_t flags;
+} dxg2_plan_desc;
+
+/* Plan: qmask -> bins, per-bin pixel lists, slab layout, tile work lists, workspaces.
+ * Replaces BaseCorrelator._set_parameters/_set_qmask/_set_scale_factor
+ * (correlator/common.py:22-72) and the constructors MatMulCorrelator.__init__
+ * (correlator/dense.py:120-126), DenseCorrelator.__init__ (dense.py:149-213),
+ * FFTCorrelator.__init__ (dense.py:277-300), EventCorrelator.__init__ (event.py:15-101). */
+int dxg2_plan_create(const dxg2_plan_desc* desc, dxg2_plan** out);
+int dxg2_plan_destroy(dxg2_plan* plan);
+
+/* Host-only queries (no device work).  pixel_counts / scale may be NULL.
+ * Mirrors the n_bins / bins / scale_factors attributes (common.py:36-72). */
+int dxg2_plan_query(const dxg2_plan* plan, int32_t* n_bins, int64_t* pixel_counts,
+                    double* scale);
+
+/* Device bytes the plan holds for a path (0 dense, 1 plain, 2 fft, 3 events), for
+ * admission control by memory footprint.  No reference counterpart. */
+int dxg2_plan_footprint(const dxg2_plan* plan, int32_t path, size_t* bytes);
+
+/* Dense batched path: qmask gather into per-bin i8 slabs (+ S_b(t)), one grouped
+ * i8-MFMA Gram launch over every bin with the diagonal (tau) reduction fused, exact
+ * denominator and FP64 normalisation.  d_frames: (nframes, H, W) of the plan dtype.
+ * Replaces MatMulCorrelator.correlate (dense.py:128-142, numerator dense.py:32-70) and
+ * DenseCorrelator.correlate (dense.py:223-272 with densecorrelator.cl:6-176). */
+int dxg2_correlate_dense(dxg2_plan* plan, const void* d_frames, int64_t* d_num,
+                         int64_t* d_sums, double* d_den, float* d_g2, void* stream);
+
+/* Plain comparator: one bin per launch, FP32 gather, FP32 full-square Gram, FP32
+ * diagonal sums and FP32 mean outer-product denominator.
+ * Replaces CublasMatMulCorrelator.correlate (cuda.py:101-125). */
+int dxg2_correlate_dense_plain(dxg2_plan* plan, const void* d_frames, float* d_g2,
+                               void* stream);
+
+/* Stage timing with HIP events recorded on the call's stream (no host sync per call).
+ * dxg2_plan_stage_times waits for the last recorded call and returns the averages over
+ * every call since profiling was switched on or last read, in ms: times[0] gather,
+ * [1] gram, [2] denominator+finalise, [3] total, [4] number of calls averaged; then
+ * it resets the record.  Counterpart of the reference's OpenCL event profiling
+ * (profile=True, profile_add in correlator/dense.py:244,261,272). */
+int dxg2_plan_set_profiling(dxg2_plan* plan, int32_t on);
+int dxg2_plan_stage_times(dxg2_plan* plan, float* times, int32_t n);
+
+/* Synthetic speckle frames on the device: per-pixel AR(1) complex Gaussian field with
+ * correlation time tau_c frames, counts ~ Poisson(mean_map[p] * |E|^2) clipped to the
+ * dtype, counter-based Philox-4x32-10 random numbers keyed by seed.  Test-data
+ * generator; no reference counterpart. d_mean_map: float, H*W (device). */
+int dxg2_synth_speckle(void* d_frames, int32_t dtype, int32_t nframes, int32_t height,
+                       int32_t width, const float* d_mean_map, double tau_c,
+                       uint64_t seed, void* stream);
+
+const char* dxg2_last_error(void);
+const char* dxg2_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DXG2_H */

@@ -1,0 +1,59 @@
+"""CPU checks of the C ABI surface and host-side logic (no GPU compute)."""
+import numpy as np
+import pytest
+
+import dynamix_amd._lib as L
+from dynamix_amd import synth
+
+
+def test_library_loads_and_exports_header_symbols():
+    lib = L.load()
+    names = L.header_symbols()
+    assert len(names) >= 10
+    for n in names:
+        assert hasattr(lib, n), n
+    assert lib.dxg2_version().startswith(b"dxg2")
+
+
+def test_signatures_cover_header():
+    assert set(L.header_symbols()) <= set(L._SIGNATURES), \
+        set(L.header_symbols()) - set(L._SIGNATURES)
+
+
+def test_error_mapping_on_bad_desc():
+    """Invalid plan descriptions fail on the host before any device call."""
+    import ctypes
+    lib = L.load()
+    d = L.PlanDesc()
+    d.height, d.width, d.nframes, d.dtype = 0, 4, 10, L.DXG2_U8
+    h = ctypes.c_void_p()
+    rc = lib.dxg2_plan_create(ctypes.byref(d), ctypes.byref(h))
+    assert rc == L.DXG2_EINVAL
+    with pytest.raises(ValueError, match="shape"):
+        L.check(rc)
+    d.height, d.dtype = 4, 99
+    rc = lib.dxg2_plan_create(ctypes.byref(d), ctypes.byref(h))
+    with pytest.raises(ValueError, match="dtype"):
+        L.check(rc)
+
+
+def test_headline_qmask_geometry():
+    q = synth.workload_qmask("headline")
+    assert q.shape == (514, 1030)
+    assert q.max() == 20
+    assert (q > 0).sum() == 479980          # 466 unmasked rows x 1030
+    assert (q == 0).all(axis=1).sum() == 48  # six 8-row gap bands
+
+
+def test_annulus_widths():
+    q = synth.annular_qmask((101, 101), 5, center=(50, 50), r_max=50.0)
+    assert q[50, 50] == 1 and q[50, 99] == 5 and q[50, 100] == 0 and q[0, 0] == 0
+    ql = synth.annular_qmask((201, 201), 4, center=(100, 100), r_max=100.0, widths="linear")
+    # linear widths: edges at 10, 30, 60, 100
+    assert ql[100, 109] == 1 and ql[100, 111] == 2 and ql[100, 131] == 3 and ql[100, 161] == 4
+
+
+def test_host_speckle_statistics():
+    f = synth.speckle_frames((32, 32), 400, 2.0, 10.0, seed=3)
+    assert f.dtype == np.uint8
+    assert abs(f.mean() - 2.0) < 0.1

@@ -1,0 +1,200 @@
+"""Pins the CPU oracle: golden vectors of the reference, brute force, closed forms."""
+import numpy as np
+import pytest
+
+from oracle import events as ev
+from oracle import g2 as og
+
+DENSE_CASES = ["u8_small", "u8_big_values", "i8_sparse", "u16_mixed", "u8_onepix"]
+
+
+def _case(golden, name):
+    return (golden["dense_%s_frames" % name], golden["dense_%s_qmask" % name],
+            golden["dense_%s_res" % name], golden["dense_%s_dev" % name])
+
+
+# ------------------------------------------------------------ reference goldens
+@pytest.mark.parametrize("name", DENSE_CASES)
+def test_matmul_f32_bit_exact_vs_reference(golden, name):
+    frames, q, res, _ = _case(golden, name)
+    with np.errstate(all="ignore"):
+        mine = og.matmul_correlator_f32(frames, q)
+    np.testing.assert_array_equal(mine, res)
+
+
+@pytest.mark.parametrize("name", DENSE_CASES)
+def test_exact_g2_matches_reference_within_f32(golden, name):
+    """The exact FP64 g2 and the reference's float32 g2 agree to float32 rounding."""
+    frames, q, res, _ = _case(golden, name)
+    out = og.dense_g2(frames, q)
+    g = out["g2"]
+    fin = np.isfinite(res)
+    assert np.array_equal(fin, np.isfinite(g))
+    rel = np.abs(g[fin] - res[fin]) / np.maximum(np.abs(g[fin]), 1e-30)
+    assert rel.max() < 2e-5
+    # the reference's own test tolerance (test_dense.py:86)
+    assert np.abs(g[fin] - res[fin]).max() < 5e-3
+
+
+def test_qmask_and_scale_conventions(golden):
+    q = golden["base_qmask"]
+    qq, bins = og.qmask_bins(q.shape, q)
+    np.testing.assert_array_equal(bins, golden["base_bins"])
+    assert len(bins) == int(golden["base_n_bins"])
+    np.testing.assert_array_equal(og.scale_factors(qq, bins), golden["base_scale"])
+    np.testing.assert_array_equal(og.scale_factors(qq, bins, [10.0, 20.0, 30.0]),
+                                  golden["base_scale_given"])
+    qn, bn = og.qmask_bins((3, 4), None)
+    np.testing.assert_array_equal(bn, golden["base_none_bins"])
+    np.testing.assert_array_equal(qn, golden["base_none_qmask"])
+    np.testing.assert_array_equal(og.scale_factors(qn, bn), golden["base_none_scale"])
+    qs, bs = og.qmask_bins(4, None)
+    assert qs.shape == tuple(golden["base_scalar_shape"])
+    np.testing.assert_array_equal(og.scale_factors(qs, bs, 7.5), golden["base_scalar_scale"])
+
+
+def test_fft_length_rule(golden):
+    for n, v in zip(golden["nextpow_n"], golden["nextpow_val"]):
+        assert og.next_fft_length(int(n)) == int(v)
+
+
+def test_fft_length_closed_form():
+    def ok(m):
+        a = 0
+        while m % 2 == 0:
+            m //= 2
+            a += 1
+        for p in (3, 5, 7):
+            while m % p == 0:
+                m //= p
+        return m == 1 and a >= 2
+    for n in [1, 5, 37, 1001, 3001]:
+        v = og.next_fft_length(n)
+        assert v >= n and ok(v)
+        assert not any(ok(m) for m in range(n, v))
+
+
+@pytest.mark.parametrize("name", ["evt_u8", "evt_i8"])
+def test_compaction_vs_reference(golden, name):
+    frames = golden["%s_frames" % name]
+    d, t, o = ev.compress_all_stack(frames)
+    np.testing.assert_array_equal(d, golden["%s_data" % name])
+    np.testing.assert_array_equal(t, golden["%s_times" % name])
+    np.testing.assert_array_equal(o, golden["%s_offsets" % name])
+    fc = ev.FramesCompressorOracle(frames.shape[1:], frames.shape[0],
+                                   int((frames > 0).sum(axis=0).max()), frames.dtype)
+    for f in frames:
+        fc.process_frame(f)
+    d2, t2, o2 = fc.get_compacted_events()
+    np.testing.assert_array_equal(d2, golden["%s_prog_data" % name])
+    np.testing.assert_array_equal(t2, golden["%s_prog_times" % name])
+    np.testing.assert_array_equal(o2, golden["%s_prog_offsets" % name])
+
+
+# ---------------------------------------------------------------- brute force
+def _tiny(seed, n=7, shape=(3, 4), hi=6, dtype=np.uint8, nbins=2):
+    rng = np.random.default_rng(seed)
+    f = rng.integers(0, hi, size=(n,) + shape).astype(dtype)
+    q = rng.integers(0, nbins + 1, size=shape).astype(np.int32)
+    for b in range(1, nbins + 1):
+        q.flat[b] = b
+    return f, q
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_numerators_vs_brute_force(seed):
+    f, q = _tiny(seed, hi=256 if seed % 2 else 4)
+    num, s, den = og.brute_force(f, q)
+    out = og.dense_g2(f, q)
+    assert [[int(x) for x in r] for r in out["num"]] == [[int(x) for x in r] for r in num]
+    assert [[int(x) for x in r] for r in out["sums"]] == [[int(x) for x in r] for r in s]
+    assert [[int(x) for x in r] for r in out["den"]] == [[int(x) for x in r] for r in den]
+
+
+def test_int8_negative_values_vs_brute_force():
+    f, q = _tiny(9, hi=5)
+    f = (f.astype(np.int16) - 2).astype(np.int8)
+    num, s, den = og.brute_force(f, q)
+    out = og.dense_g2(f, q)
+    assert [[int(x) for x in r] for r in out["num"]] == [[int(x) for x in r] for r in num]
+    assert [[int(x) for x in r] for r in out["den"]] == [[int(x) for x in r] for r in den]
+
+
+def test_denominator_python_int_path():
+    s = np.array([[2 ** 31 - 1, 2 ** 31 - 5, 7, 2 ** 30]], dtype=np.int64)
+    d = og.denominators(s)
+    n = s.shape[1]
+    for tau in range(n):
+        assert int(d[0, tau]) == sum(int(s[0, t]) * int(s[0, t + tau]) for t in range(n - tau))
+
+
+def test_invariants():
+    rng = np.random.default_rng(5)
+    f = rng.integers(0, 9, size=(20, 5, 6)).astype(np.uint8)
+    q = rng.integers(1, 4, size=(5, 6)).astype(np.int32)
+    out = og.dense_g2(f, q)
+    # Num(0) = sum I^2 over the bin
+    for i, b in enumerate(out["bins"]):
+        a = og.gather_bin(f, q, b).astype(np.int64)
+        assert out["num"][i, 0] == (a * a).sum()
+    # permutation of pixels leaves everything unchanged
+    perm = rng.permutation(30)
+    f2 = f.reshape(20, -1)[:, perm].reshape(f.shape)
+    q2 = q.ravel()[perm].reshape(q.shape)
+    out2 = og.dense_g2(f2, q2)
+    np.testing.assert_array_equal(out["num"], out2["num"])
+    np.testing.assert_array_equal(out["den"], out2["den"])
+    # time-constant frames: g2 = n * sum c^2 / (sum c)^2 exactly
+    c = rng.integers(1, 9, size=(5, 6)).astype(np.uint8)
+    fc = np.broadcast_to(c, (11, 5, 6)).copy()
+    o = og.dense_g2(fc, q)
+    for i, b in enumerate(o["bins"]):
+        cb = c[q == b].astype(np.float64)
+        np.testing.assert_allclose(o["g2"][i], cb.size * (cb ** 2).sum() / cb.sum() ** 2,
+                                   rtol=1e-14)
+    # single-pixel bin: g2 == 1 wherever defined
+    q1 = np.zeros((5, 6), np.int32)
+    q1[2, 3] = 1
+    o1 = og.dense_g2(f, q1)
+    fin = np.isfinite(o1["g2"])
+    np.testing.assert_allclose(o1["g2"][fin], 1.0, rtol=1e-14)
+
+
+def test_fft_correlator_close_to_exact():
+    rng = np.random.default_rng(3)
+    f = rng.poisson(0.5, size=(64, 6, 7)).astype(np.uint8)
+    q = rng.integers(0, 4, size=(6, 7)).astype(np.int32)
+    out = og.dense_g2(f, q)
+    r = og.fft_correlator(f, q)
+    fin = np.isfinite(out["g2"])
+    assert np.allclose(r[fin], out["g2"][fin], rtol=1e-3, atol=1e-4)
+
+
+# ----------------------------------------------------------------- event path
+@pytest.mark.parametrize("seed", range(3))
+def test_event_numerators_equal_dense(seed):
+    rng = np.random.default_rng(seed)
+    f = (rng.random((30, 5, 6)) < 0.2).astype(np.uint8) * rng.integers(1, 5, size=(30, 5, 6)).astype(np.uint8)
+    q = rng.integers(0, 4, size=(5, 6)).astype(np.int32)
+    for b in range(1, 4):
+        q.flat[b] = b
+    d, t, o = ev.compress_all_stack(f)
+    e = ev.event_g2(t, d, o, q, 30)
+    dn = og.dense_g2(f, q)
+    np.testing.assert_array_equal(e["num"], dn["num"])
+    np.testing.assert_array_equal(e["sums"], dn["sums"])
+    bnum, bs = ev.event_brute_force(t, d, o, q, 30)
+    assert [[int(x) for x in r] for r in e["num"]] == [[int(x) for x in r] for r in bnum]
+    assert [[int(x) for x in r] for r in e["sums"]] == [[int(x) for x in r] for r in bs]
+
+
+def test_coo_csr_round_trip():
+    rng = np.random.default_rng(11)
+    f = (rng.random((25, 4, 5)) < 0.3).astype(np.uint8) * 3
+    d, t, o = ev.compress_all_stack(f)
+    pix, fr, cnt = ev.csr_to_coo(t, d, o)
+    assert np.all(np.diff(fr.astype(np.int64)) >= 0)
+    t2, d2, o2 = ev.coo_to_csr(pix, fr, cnt, 20)
+    np.testing.assert_array_equal(t2, t)
+    np.testing.assert_array_equal(d2, d)
+    np.testing.assert_array_equal(o2, o)
