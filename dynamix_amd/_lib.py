@@ -20,6 +20,8 @@ DXG2_EHIP = 3
 DXG2_EUNSUPPORTED = 4
 DXG2_ECAPACITY = 5
 
+DXG2_FLAG_TILE128 = 1
+
 DXG2_U8 = 1
 DXG2_I8 = 2
 DXG2_U16 = 3

@@ -93,7 +93,7 @@ class BaseCorrelator(object):
 class Plan(object):
     """Owns one dxg2 plan (device work lists and workspaces)."""
 
-    def __init__(self, shape, nframes, qmask, dtype, scale=None):
+    def __init__(self, shape, nframes, qmask, dtype, scale=None, flags=0):
         self.lib = _lib.load()
         self.dtype = np.dtype(dtype)
         if self.dtype not in _lib.DTYPE_CODES:
@@ -104,6 +104,7 @@ class Plan(object):
         desc.height, desc.width = int(shape[0]), int(shape[1])
         desc.nframes = int(nframes)
         desc.dtype = _lib.DTYPE_CODES[self.dtype]
+        desc.flags = int(flags)
         if qmask is not None:
             q = np.ascontiguousarray(qmask, dtype=np.int32)
             self._keep.append(q)

@@ -39,7 +39,8 @@ class _DenseHIP(BaseCorrelator):
         p = self._plans.get(dtype)
         if p is None:
             q = self.qmask if self._qmask_given else None
-            p = Plan(self.shape, self.nframes, q, dtype, scale=None)
+            flags = _lib.DXG2_FLAG_TILE128 if self.extra_options.get("tile") == 128 else 0
+            p = Plan(self.shape, self.nframes, q, dtype, scale=None, flags=flags)
             self._plans[dtype] = p
         return p
 

@@ -72,7 +72,7 @@ std::vector<T> remap_for_xcds(const std::vector<T>& in, int group, const T& pad)
 struct dxg2_plan {
     int H = 0, W = 0, N = 0, dtype = 0, n_bins = 0;
     int device = 0;
-    int gram_tile = 128;
+    int gram_tile = 256;
     std::vector<int64_t> npix;
     std::vector<double> scale;
     std::vector<int32_t> pix_idx;
@@ -173,6 +173,7 @@ int dxg2_plan_create(const dxg2_plan_desc* desc, dxg2_plan** out) {
     p->W = desc->width;
     p->N = desc->nframes;
     p->dtype = desc->dtype;
+    p->gram_tile = (desc->flags & DXG2_FLAG_TILE128) ? 128 : 256;
 
     // qmask -> bins (common.py:36-46)
     std::vector<std::vector<int32_t>> lists;
@@ -350,10 +351,10 @@ int dxg2_correlate_dense(dxg2_plan* p, const void* d_frames, int64_t* d_num, int
     launch_fill_i64(sums, nbn, s);
     launch_fill_i64(p->d_ugram, nbn, s);
     launch_gather_i8((const uint8_t*)d_frames, p->dtype, (int64_t)p->H * p->W, p->N, p->d_pix,
-                     p->d_bins, p->d_gwork, (int)p->gather_work.size(), p->d_slab, sums, s);
+                     p->d_bins, p->d_gwork, (int)p->gather_work.size(), p->d_slab, s);
     if (ev) DXG2_HIP(hipEventRecord(ev[1], s));
     launch_gram_i8(p->gram_tile, p->d_slab, p->d_bins, p->d_mwork, (int)p->gram_work.size(),
-                   p->N, p->d_ugram, s);
+                   p->N, p->dtype == DXG2_U8 ? 1 : 0, p->d_ugram, sums, s);
     if (ev) DXG2_HIP(hipEventRecord(ev[2], s));
     launch_finalize(sums, p->d_ugram, p->d_bins, p->n_bins, p->N, p->dtype == DXG2_U8 ? 1 : 0,
                     d_num, d_den, d_g2, s);

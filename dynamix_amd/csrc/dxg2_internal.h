@@ -47,9 +47,10 @@ struct GramWork {        // one output tile (i, j >= i) over a k-block range
 void launch_fill_i64(int64_t* p, int64_t n, hipStream_t s);
 void launch_gather_i8(const uint8_t* frames, int dtype, int64_t hw, int nframes,
                       const int32_t* pix_idx, const BinDev* bins, const GatherWork* work,
-                      int n_work, int8_t* slab, int64_t* sums, hipStream_t s);
+                      int n_work, int8_t* slab, hipStream_t s);
 void launch_gram_i8(int tile, const int8_t* slab, const BinDev* bins, const GramWork* work,
-                    int n_work, int nframes, int64_t* ugram, hipStream_t s);
+                    int n_work, int nframes, int biased, int64_t* ugram, int64_t* sums,
+                    hipStream_t s);
 void launch_finalize(const int64_t* sums, const int64_t* ugram, const BinDev* bins,
                      int n_bins, int nframes, int biased, int64_t* num, double* den,
                      float* g2, hipStream_t s);

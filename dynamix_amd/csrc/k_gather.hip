@@ -1,11 +1,15 @@
-// qmask gather: frames (N, H, W) -> per-bin i8 slabs + exact per-frame bin sums S_b(t).
+// qmask gather: frames (N, H, W) -> per-bin i8 slabs slab[kb][t][64].
 //
 // The reference gathers one bin at a time on the host (correlator/dense.py:48-49,
-// dense.py:317-319) and sums frames per bin in a separate pass
-// (resources/opencl/densecorrelator.cl:65-150).  Here one launch covers every bin:
-// each workgroup owns (bin, up to 16 k-blocks of 64 pixels, 64 frames), reads the
-// pixels through the bin's raster-ordered index list, writes 1 KiB contiguous slab
-// runs per wave and folds the per-frame sums into one 64-bit atomic per frame.
+// dense.py:317-319).  Here one launch covers every bin.  A wave owns one k-block (64
+// consecutive entries of a bin's raster-ordered pixel list) and a 64-frame tile:
+//   * each load instruction reads entry `lane` of one frame, so the 64 lanes read 64
+//     consecutive list entries -- runs of neighbouring pixels, few cache lines;
+//   * after four frames each lane holds 4 frames x 1 pixel; a 4x4 byte transpose
+//     inside every lane quad (DPP quad broadcast + v_perm) gives lane 4q+i the dword
+//     of pixels 4q..4q+3 for frame t+i, so one store instruction writes the 256
+//     contiguous slab bytes of rows t..t+3.
+// The per-frame bin sums are formed later by the Gram kernel's diagonal tiles.
 #include "dxg2_internal.h"
 
 namespace dxg2 {
@@ -23,99 +27,71 @@ void launch_fill_i64(int64_t* p, int64_t n, hipStream_t s) {
     fill_i64_kernel<<<dim3((unsigned)blocks), dim3(256), 0, s>>>(p, n);
 }
 
+constexpr int GATHER_TF = 64;  // frames per workgroup tile
+
+// Broadcast lane j of every quad to the whole quad (DPP quad_perm [j, j, j, j]).
+template <int J>
+__device__ __forceinline__ uint32_t quad_bcast(uint32_t x) {
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, J * 0x55, 0xF, 0xF, false);
+}
+
 // Work layout: blockIdx.x -> GatherWork (bin, first k-block, k-block count <= 16),
-// blockIdx.y -> 64-frame tile.  Wave w takes k-blocks kb0+w, kb0+w+4, ...; in a
-// k-block, lane l takes pixels (l&3)*16..+16 and frames t0 + 16*it + (l>>2), it=0..3,
-// and stores its 16 bytes at slab[kb][t][(l&3)*16] -- a wave writes 1 KiB runs.
-// BIAS (uint8 input): store v ^ 0x80 = v - 128 as i8, so every byte fits the signed
-// MFMA operand; padded pixels (index -1) read as 0 and become -128 the same way.
+// blockIdx.y -> 64-frame tile.  Wave w handles k-blocks kb0 + w, kb0 + w + 4, ...
+// BIAS (uint8 input): bytes are stored as v ^ 0x80 = v - 128 (i8 MFMA operands);
+// padded list entries (index -1) read as 0 and become -128 the same way.
 template <bool BIAS>
 __global__ __launch_bounds__(256) void gather_i8_kernel(
     const uint8_t* __restrict__ frames, int64_t hw, int nframes,
     const int32_t* __restrict__ pix_idx, const BinDev* __restrict__ bins,
-    const GatherWork* __restrict__ work, int8_t* __restrict__ slab,
-    int64_t* __restrict__ sums) {
-    __shared__ int s_part[64];
+    const GatherWork* __restrict__ work, int8_t* __restrict__ slab) {
     const GatherWork w = work[blockIdx.x];
     const BinDev bd = bins[w.bin];
-    const int t0 = blockIdx.y * 64;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const int g = lane & 3;
-    const int f = lane >> 2;
-    if (threadIdx.x < 64) s_part[threadIdx.x] = 0;
-    __syncthreads();
+    const int qi = lane & 3;        // frame offset this lane stores after the transpose
+    const int q4 = (lane >> 2) * 4; // first pixel of this lane's stored dword
+    // byte selectors for v_perm: lo = (y0[qi], y1[qi], 0, 0), hi = (0, 0, y2[qi], y3[qi])
+    const uint32_t sel_lo = (uint32_t)qi | ((uint32_t)(4 + qi) << 8) | 0x0c0c0000u;
+    const uint32_t sel_hi = 0x00000c0cu | ((uint32_t)qi << 16) | ((uint32_t)(4 + qi) << 24);
+    const int t0 = blockIdx.y * GATHER_TF;
+    const int tend = min(nframes, t0 + GATHER_TF);
 
-    int acc[4] = {0, 0, 0, 0};
     for (int kb = w.kb0 + wave; kb < w.kb0 + w.nkb; kb += 4) {
-        const int4* idx4 =
-            reinterpret_cast<const int4*>(pix_idx + bd.idx_off + (int64_t)kb * KB + g * 16);
-        int ix[16];
+        const int p = pix_idx[bd.idx_off + (int64_t)kb * KB + lane];
+        const int pc = p >= 0 ? p : 0;
+        const uint32_t keep = p >= 0 ? 0xFFu : 0u;
+        int8_t* dst = slab + bd.slab_off + (int64_t)kb * nframes * KB + q4;
+        for (int t = t0; t < tend; t += 4) {
+            uint32_t x = 0;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int4 v = idx4[q];
-            ix[4 * q + 0] = v.x;
-            ix[4 * q + 1] = v.y;
-            ix[4 * q + 2] = v.z;
-            ix[4 * q + 3] = v.w;
-        }
-        int8_t* dst = slab + bd.slab_off + (int64_t)kb * nframes * KB + g * 16;
-#pragma unroll
-        for (int it = 0; it < 4; ++it) {
-            const int t = t0 + it * 16 + f;
-            if (t < nframes) {
-                const uint8_t* fr = frames + (int64_t)t * hw;
-                uint32_t word[4];
-                int s = 0;
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    uint32_t wd = 0;
-#pragma unroll
-                    for (int b = 0; b < 4; ++b) {
-                        const int p = ix[4 * q + b];
-                        const uint32_t v = p >= 0 ? (uint32_t)fr[p] : 0u;
-                        s += BIAS ? (int)v : (int)(int8_t)v;
-                        wd |= v << (8 * b);
-                    }
-                    word[q] = BIAS ? (wd ^ 0x80808080u) : wd;
-                }
-                *reinterpret_cast<uint4*>(dst + (int64_t)t * KB) =
-                    make_uint4(word[0], word[1], word[2], word[3]);
-                acc[it] += s;
+            for (int j = 0; j < 4; ++j) {
+                const int tj = t + j < nframes ? t + j : nframes - 1;
+                const uint32_t b = (uint32_t)frames[(int64_t)tj * hw + pc] & keep;
+                x |= b << (8 * j);
             }
+            const uint32_t y0 = quad_bcast<0>(x);
+            const uint32_t y1 = quad_bcast<1>(x);
+            const uint32_t y2 = quad_bcast<2>(x);
+            const uint32_t y3 = quad_bcast<3>(x);
+            uint32_t out = __builtin_amdgcn_perm(y1, y0, sel_lo) | __builtin_amdgcn_perm(y3, y2, sel_hi);
+            if (BIAS) out ^= 0x80808080u;
+            if (t + qi < nframes)
+                *reinterpret_cast<uint32_t*>(dst + (int64_t)(t + qi) * KB) = out;
         }
-    }
-#pragma unroll
-    for (int it = 0; it < 4; ++it) {
-        acc[it] += __shfl_xor(acc[it], 1);
-        acc[it] += __shfl_xor(acc[it], 2);
-    }
-    if (g == 0) {
-#pragma unroll
-        for (int it = 0; it < 4; ++it)
-            if (acc[it] != 0) atomicAdd(&s_part[it * 16 + f], acc[it]);
-    }
-    __syncthreads();
-    if (threadIdx.x < 64) {
-        const int t = t0 + threadIdx.x;
-        const int v = s_part[threadIdx.x];
-        if (t < nframes && v != 0)
-            atomicAdd(reinterpret_cast<unsigned long long*>(sums + (int64_t)w.bin * nframes + t),
-                      (unsigned long long)(int64_t)v);
     }
 }
 
 void launch_gather_i8(const uint8_t* frames, int dtype, int64_t hw, int nframes,
                       const int32_t* pix_idx, const BinDev* bins, const GatherWork* work,
-                      int n_work, int8_t* slab, int64_t* sums, hipStream_t s) {
+                      int n_work, int8_t* slab, hipStream_t s) {
     if (n_work <= 0 || nframes <= 0) return;
-    dim3 grid((unsigned)n_work, (unsigned)((nframes + 63) / 64));
+    dim3 grid((unsigned)n_work, (unsigned)((nframes + GATHER_TF - 1) / GATHER_TF));
     if (dtype == DXG2_U8)
         gather_i8_kernel<true><<<grid, dim3(256), 0, s>>>(frames, hw, nframes, pix_idx, bins,
-                                                         work, slab, sums);
+                                                         work, slab);
     else
         gather_i8_kernel<false><<<grid, dim3(256), 0, s>>>(frames, hw, nframes, pix_idx, bins,
-                                                          work, slab, sums);
+                                                          work, slab);
 }
 
 }  // namespace dxg2

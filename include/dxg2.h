@@ -42,6 +42,7 @@ extern "C" {
 
 /* plan flags */
 #define DXG2_FLAG_NONE 0u
+#define DXG2_FLAG_TILE128 1u   /* Gram tiles of 128 frames (default 256) */
 
 typedef struct dxg2_plan dxg2_plan;
 

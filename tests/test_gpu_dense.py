@@ -51,20 +51,22 @@ def _speckle(seed, n, shape, mean, tau_c=20.0, dtype=np.uint8):
     return speckle_frames(shape, n, mean, tau_c, seed, dtype)
 
 
+@pytest.mark.parametrize("tile", [256, 128])
 @pytest.mark.parametrize("n,shape,mean,nbins", [
     (7, (5, 6), 2.0, 2),          # fewer frames than one tile
     (130, (9, 11), 1.0, 3),       # frames straddle a 128 tile boundary
     (300, (16, 20), 0.3, 4),      # several tiles, several bins
     (257, (33, 40), 0.05, 5),     # 256 + 1 frames, sparse
+    (600, (40, 50), 0.1, 3),      # several 256 tiles, odd k-block counts
 ])
-def test_dense_u8_vs_oracle(n, shape, mean, nbins):
-    _announce("dense u8 n=%d shape=%s" % (n, shape))
+def test_dense_u8_vs_oracle(n, shape, mean, nbins, tile):
+    _announce("dense u8 n=%d shape=%s tile=%d" % (n, shape, tile))
     rng = np.random.default_rng(n)
     f = _speckle(n, n, shape, mean)
     q = rng.integers(0, nbins + 1, size=shape).astype(np.int32)
     for b in range(1, nbins + 1):
         q.flat[b] = b
-    c, g2 = _run(f, q)
+    c, g2 = _run(f, q, extra_options={"tile": tile})
     _check_exact(c, g2, og.dense_g2(f, q))
 
 
@@ -113,13 +115,14 @@ def test_dense_single_pixel_bin_is_one():
     np.testing.assert_allclose(g2[fin], 1.0, rtol=1e-6)
 
 
-def test_dense_k_chunks_over_int32_limit():
+@pytest.mark.parametrize("tile", [256, 128])
+def test_dense_k_chunks_over_int32_limit(tile):
     """One bin of 160,000 pixels: the work list splits K into chunks of 2047 k-blocks
     so every int32 MFMA accumulator stays below 2^31 (values up to 255)."""
-    _announce("dense k-chunk split")
+    _announce("dense k-chunk split tile=%d" % tile)
     rng = np.random.default_rng(5)
     f = rng.integers(0, 256, size=(24, 400, 400)).astype(np.uint8)
-    c, g2 = _run(f, None)
+    c, g2 = _run(f, None, extra_options={"tile": tile})
     _check_exact(c, g2, og.dense_g2(f, None))
 
 
