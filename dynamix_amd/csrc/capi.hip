@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -71,6 +72,7 @@ std::vector<T> remap_for_xcds(const std::vector<T>& in, int group, const T& pad)
 
 struct dxg2_plan {
     int H = 0, W = 0, N = 0, dtype = 0, n_bins = 0;
+    int nrows = 0;  // slab rows per k-block: N rounded up to SLAB_ROW_ALIGN
     int device = 0;
     int gram_tile = 256;
     std::vector<int64_t> npix;
@@ -103,10 +105,21 @@ struct dxg2_plan {
 
 namespace {
 
+int env_int(const char* name, int dflt) {
+    const char* v = getenv(name);
+    return (v && *v) ? atoi(v) : dflt;
+}
+
+// Tile order of the grouped Gram launch.  Within a bin (largest bins first) the
+// upper-triangle tiles are walked in bands of `band` tile rows, column by column, so
+// consecutive items share row and column frame panels; the list is then dealt to the
+// XCDs in chunks of `group` items (0: no XCD remap).  Both are tuning parameters
+// (environment DXG2_GRAM_BAND, DXG2_GRAM_GROUP).
 void build_gram_work(dxg2_plan* p) {
     const int T = p->gram_tile;
     const int nT = (p->N + T - 1) / T;
-    const int band = 8;
+    const int band = std::max(1, env_int("DXG2_GRAM_BAND", 8));
+    const int group = std::max(0, env_int("DXG2_GRAM_GROUP", 64));
     std::vector<int> order(p->n_bins);
     for (int b = 0; b < p->n_bins; ++b) order[b] = b;
     // Largest bins first so their long tiles start early (LPT).
@@ -125,7 +138,7 @@ void build_gram_work(dxg2_plan* p) {
         }
     }
     GramWork pad{-1, 0, 0, 0, 0, {0, 0, 0}};
-    p->gram_work = remap_for_xcds(raster, 64, pad);
+    p->gram_work = group > 0 ? remap_for_xcds(raster, group, pad) : raster;
 }
 
 // Next unused event quad of the profiling pool (created on first use).
@@ -145,6 +158,8 @@ int ensure_dense_buffers(dxg2_plan* p) {
             p->d_slab = nullptr;
             return set_error(DXG2_ENOMEM, "cannot allocate %zu bytes of slab", p->slab_bytes);
         }
+        // rows past the last frame stay zero for the life of the plan
+        DXG2_HIP(hipMemset(p->d_slab, 0, p->slab_bytes));
     }
     return DXG2_OK;
 }
@@ -172,6 +187,7 @@ int dxg2_plan_create(const dxg2_plan_desc* desc, dxg2_plan** out) {
     p->H = desc->height;
     p->W = desc->width;
     p->N = desc->nframes;
+    p->nrows = (desc->nframes + SLAB_ROW_ALIGN - 1) / SLAB_ROW_ALIGN * SLAB_ROW_ALIGN;
     p->dtype = desc->dtype;
     p->gram_tile = (desc->flags & DXG2_FLAG_TILE128) ? 128 : 256;
 
@@ -219,7 +235,7 @@ int dxg2_plan_create(const dxg2_plan_desc* desc, dxg2_plan** out) {
         for (int kb0 = 0; kb0 < nkb; kb0 += 16)
             p->gather_work.push_back(GatherWork{b, kb0, std::min(16, nkb - kb0), 0});
         idx_off += (int64_t)nkb * KB;
-        slab_off += (int64_t)nkb * KB * p->N;
+        slab_off += (int64_t)nkb * KB * p->nrows;
     }
     p->slab_bytes = (size_t)slab_off;
     build_gram_work(p);
@@ -350,11 +366,11 @@ int dxg2_correlate_dense(dxg2_plan* p, const void* d_frames, int64_t* d_num, int
     if (ev) DXG2_HIP(hipEventRecord(ev[0], s));
     launch_fill_i64(sums, nbn, s);
     launch_fill_i64(p->d_ugram, nbn, s);
-    launch_gather_i8((const uint8_t*)d_frames, p->dtype, (int64_t)p->H * p->W, p->N, p->d_pix,
-                     p->d_bins, p->d_gwork, (int)p->gather_work.size(), p->d_slab, s);
+    launch_gather_i8((const uint8_t*)d_frames, p->dtype, (int64_t)p->H * p->W, p->N, p->nrows,
+                     p->d_pix, p->d_bins, p->d_gwork, (int)p->gather_work.size(), p->d_slab, s);
     if (ev) DXG2_HIP(hipEventRecord(ev[1], s));
     launch_gram_i8(p->gram_tile, p->d_slab, p->d_bins, p->d_mwork, (int)p->gram_work.size(),
-                   p->N, p->dtype == DXG2_U8 ? 1 : 0, p->d_ugram, sums, s);
+                   p->N, p->nrows, p->dtype == DXG2_U8 ? 1 : 0, p->d_ugram, sums, s);
     if (ev) DXG2_HIP(hipEventRecord(ev[2], s));
     launch_finalize(sums, p->d_ugram, p->d_bins, p->n_bins, p->N, p->dtype == DXG2_U8 ? 1 : 0,
                     d_num, d_den, d_g2, s);

@@ -12,9 +12,11 @@
 namespace dxg2 {
 
 // Slab geometry: every bin's gathered pixels are stored as i8, grouped in k-blocks of
-// KB pixels; a k-block holds all frames: slab[kb][t][KB].  A (rows x KB) panel of one
-// k-block is therefore one contiguous run of bytes.
+// KB pixels; a k-block holds all frames: slab[kb][t][KB] with t < nrows, where nrows is
+// nframes rounded up to SLAB_ROW_ALIGN and rows t >= nframes are zero.  A (rows x KB)
+// panel of one k-block is therefore one contiguous run of bytes.
 constexpr int KB = 64;
+constexpr int SLAB_ROW_ALIGN = 256;
 // Largest number of k-blocks one Gram work item accumulates in int32 before it
 // flushes: |a*b| <= 2^14 for i8 operands, 2^14 * 2047 * 64 < 2^31.
 constexpr int MAX_KBLOCKS_PER_ITEM = 2047;
@@ -45,12 +47,12 @@ struct GramWork {        // one output tile (i, j >= i) over a k-block range
 
 // ------------------------------------------------------------------ launchers
 void launch_fill_i64(int64_t* p, int64_t n, hipStream_t s);
-void launch_gather_i8(const uint8_t* frames, int dtype, int64_t hw, int nframes,
+void launch_gather_i8(const uint8_t* frames, int dtype, int64_t hw, int nframes, int nrows,
                       const int32_t* pix_idx, const BinDev* bins, const GatherWork* work,
                       int n_work, int8_t* slab, hipStream_t s);
 void launch_gram_i8(int tile, const int8_t* slab, const BinDev* bins, const GramWork* work,
-                    int n_work, int nframes, int biased, int64_t* ugram, int64_t* sums,
-                    hipStream_t s);
+                    int n_work, int nframes, int nrows, int biased, int64_t* ugram,
+                    int64_t* sums, hipStream_t s);
 void launch_finalize(const int64_t* sums, const int64_t* ugram, const BinDev* bins,
                      int n_bins, int nframes, int biased, int64_t* num, double* den,
                      float* g2, hipStream_t s);

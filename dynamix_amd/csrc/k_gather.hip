@@ -41,7 +41,7 @@ __device__ __forceinline__ uint32_t quad_bcast(uint32_t x) {
 // padded list entries (index -1) read as 0 and become -128 the same way.
 template <bool BIAS>
 __global__ __launch_bounds__(256) void gather_i8_kernel(
-    const uint8_t* __restrict__ frames, int64_t hw, int nframes,
+    const uint8_t* __restrict__ frames, int64_t hw, int nframes, int nrows,
     const int32_t* __restrict__ pix_idx, const BinDev* __restrict__ bins,
     const GatherWork* __restrict__ work, int8_t* __restrict__ slab) {
     const GatherWork w = work[blockIdx.x];
@@ -60,7 +60,7 @@ __global__ __launch_bounds__(256) void gather_i8_kernel(
         const int p = pix_idx[bd.idx_off + (int64_t)kb * KB + lane];
         const int pc = p >= 0 ? p : 0;
         const uint32_t keep = p >= 0 ? 0xFFu : 0u;
-        int8_t* dst = slab + bd.slab_off + (int64_t)kb * nframes * KB + q4;
+        int8_t* dst = slab + bd.slab_off + (int64_t)kb * nrows * KB + q4;
         for (int t = t0; t < tend; t += 4) {
             uint32_t x = 0;
 #pragma unroll
@@ -81,17 +81,17 @@ __global__ __launch_bounds__(256) void gather_i8_kernel(
     }
 }
 
-void launch_gather_i8(const uint8_t* frames, int dtype, int64_t hw, int nframes,
+void launch_gather_i8(const uint8_t* frames, int dtype, int64_t hw, int nframes, int nrows,
                       const int32_t* pix_idx, const BinDev* bins, const GatherWork* work,
                       int n_work, int8_t* slab, hipStream_t s) {
     if (n_work <= 0 || nframes <= 0) return;
     dim3 grid((unsigned)n_work, (unsigned)((nframes + GATHER_TF - 1) / GATHER_TF));
     if (dtype == DXG2_U8)
-        gather_i8_kernel<true><<<grid, dim3(256), 0, s>>>(frames, hw, nframes, pix_idx, bins,
-                                                         work, slab);
+        gather_i8_kernel<true><<<grid, dim3(256), 0, s>>>(frames, hw, nframes, nrows, pix_idx,
+                                                         bins, work, slab);
     else
-        gather_i8_kernel<false><<<grid, dim3(256), 0, s>>>(frames, hw, nframes, pix_idx, bins,
-                                                          work, slab);
+        gather_i8_kernel<false><<<grid, dim3(256), 0, s>>>(frames, hw, nframes, nrows, pix_idx,
+                                                          bins, work, slab);
 }
 
 }  // namespace dxg2

@@ -10,277 +10,258 @@
 // into its 2T-1 diagonals tau = (tj-ti)*T + c - r in LDS before one 64-bit global
 // atomic per diagonal.  Nothing of size N x N is ever stored.
 //
-// LDS diagonal sums stay in 32-bit words: every int32 value v is split as
-// v = hi * 2^16 + lo with lo in [0, 2^16) and hi in [-2^15, 2^15); a diagonal of a
-// T <= 256 tile collects at most 256 values, so sum(lo) < 2^24 and |sum(hi)| < 2^23
-// are exact, and the 64-bit total hi_sum * 2^16 + lo_sum is formed once per slot.
+// Data movement: frame panels go from HBM straight into LDS with
+// global_load_lds_dwordx4 (no register staging) through a ring of 4 one-k-block
+// stages, so three stages are in flight while the MFMAs consume the fourth.  Slab rows
+// are padded with zero rows to a multiple of 256 frames, so every load is in bounds
+// and rows past the last frame add nothing.  In LDS a panel row is 64 bytes = four
+// 16-byte quads, stored at quad (q ^ ((row >> 2) & 3)): the MFMA fragment reads (16
+// consecutive rows, one quad) and the DMA writes (contiguous 1 KiB) are both free of
+// bank conflicts.
 //
 // Diagonal tiles (ti == tj) read each frame row of their panel exactly once per
 // k-range, so they also produce the per-frame bin sums S_b(t) (densecorrelator.cl:
 // 65-150) from the staged bytes with v_dot4 -- no extra pass over the data.
 //
-// Pipeline: a stage is 2 k-blocks (128 bytes of every panel row).  Global loads for
-// stage s+1 are issued into registers before the MFMAs of stage s, written to the
-// other LDS buffer after them, and one barrier per stage orders the two buffers.
+// LDS diagonal sums stay in 32-bit words: every int32 value v is split as
+// v = hi * 2^16 + lo with lo in [0, 2^16) and hi in [-2^15, 2^15); a diagonal of a
+// T <= 256 tile collects at most 256 values, so sum(lo) < 2^24 and |sum(hi)| < 2^23
+// are exact, and the 64-bit total hi_sum * 2^16 + lo_sum is formed once per slot.
 #include "dxg2_internal.h"
 
 namespace dxg2 {
 
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v16i __attribute__((ext_vector_type(16)));
+typedef __attribute__((address_space(3))) void lds_void;
 
 template <int WM, int WN>
 struct GramShape {
     static constexpr int TM = 2 * WM * 32;          // 2 x 2 waves of WM x WN MFMA blocks
     static constexpr int TN = 2 * WN * 32;
-    static constexpr int SKB = 2;                   // k-blocks per stage
-    static constexpr int SK = SKB * KB;             // bytes of a panel row per stage
-    static constexpr int STRIDE = SK + 16;          // padded LDS row (144 B)
-    static constexpr int PANEL = TM * STRIDE;       // bytes of one staged panel
+    static constexpr int PANEL = TM * KB;           // bytes of one staged panel (1 k-block)
     static constexpr int NDIAG = TM + TN - 1;
-    static constexpr int ROUNDS = TM * SK / (256 * 16);  // 16-byte loads per thread per panel
-    static constexpr int ROWS_PER_ROUND = 256 * 16 / KB; // 64 rows of one k-block
+    static constexpr int LOADS = TM / 64;           // 1 KiB DMA instructions per wave per panel
 };
 
-// Global -> registers for one panel and one stage.  Round r covers k-block part
-// (r & 1) of rows (r >> 1) * 64 .. +64; thread tid loads 16 bytes at row tid/4,
-// chunk tid%4, so a round is one contiguous 4 KiB run of the slab.  Rows past the
-// last frame and k-blocks past the item's range are clamped to valid addresses and
-// zeroed by value (an i8 zero adds nothing to the Gram).
-template <int WM, int WN, int R>
-__device__ __forceinline__ void load_stage(int4 (&r)[R],
-                                           const int8_t* __restrict__ slab_bin, int kb, int kb1,
-                                           int row0, int nframes, int tid) {
+// Physical LDS offset of (row, logical 16-byte quad q) in a staged panel.
+__device__ __forceinline__ int swz(int row, int q) { return row * KB + ((q ^ (row >> 2)) & 3) * 16; }
+
+// Issue the DMA of one panel (rows row0 .. row0+TM of k-block kb) into LDS `dst`.
+// Instruction j of wave w fills LDS rows (w*LOADS + j)*16 .. +16; lane i writes the
+// 16 bytes at dst + 1 KiB*(w*LOADS+j) + 16 i, i.e. row +i/4, physical quad i%4, so it
+// fetches the logical quad (i%4) ^ ((row >> 2) & 3) of that row.
+template <int WM, int WN>
+__device__ __forceinline__ void dma_panel(const int8_t* __restrict__ slab_bin, int kb, int row0,
+                                          int nrows, int8_t* dst, int tid) {
     using S = GramShape<WM, WN>;
+    const int w = tid >> 6;
+    const int i = tid & 63;
 #pragma unroll
-    for (int q = 0; q < S::ROUNDS; ++q) {
-        const int part = q & 1;
-        const int row = (q >> 1) * S::ROWS_PER_ROUND + (tid >> 2);
-        const int t = row0 + row;
-        const int kbp = kb + part;
-        const bool ok = (t < nframes) && (kbp < kb1);
-        const int tc = t < nframes ? t : nframes - 1;
-        const int kc = kbp < kb1 ? kbp : kb;
-        int4 v = *reinterpret_cast<const int4*>(slab_bin + ((int64_t)kc * nframes + tc) * KB +
-                                                (tid & 3) * 16);
-        if (!ok) v = make_int4(0, 0, 0, 0);
-        r[q] = v;
+    for (int j = 0; j < S::LOADS; ++j) {
+        const int g = w * S::LOADS + j;
+        const int row = g * 16 + (i >> 2);
+        const int q = ((i & 3) ^ (row >> 2)) & 3;
+        const int8_t* src = slab_bin + ((int64_t)kb * nrows + row0 + row) * KB + q * 16;
+        __builtin_amdgcn_global_load_lds((const void*)src, (lds_void*)(dst + g * 1024), 16, 0, 0);
     }
 }
 
-template <int WM, int WN, int R>
-__device__ __forceinline__ void store_stage(const int4 (&r)[R],
-                                            int8_t* s, int tid) {
-    using S = GramShape<WM, WN>;
-#pragma unroll
-    for (int q = 0; q < S::ROUNDS; ++q) {
-        const int row = (q >> 1) * S::ROWS_PER_ROUND + (tid >> 2);
-        *reinterpret_cast<int4*>(s + row * S::STRIDE + (q & 1) * KB + (tid & 3) * 16) = r[q];
-    }
+// s_waitcnt vmcnt(N), other counters left at their maximum.
+template <int N>
+__device__ __forceinline__ void wait_vm() {
+    __builtin_amdgcn_s_waitcnt((N & 0xF) | ((N >> 4) << 14) | (0x7 << 4) | (0xF << 8));
 }
 
-// Per-frame sums of the staged rows this thread loaded (diagonal tiles only):
-// rowsum[j] accumulates row (tid/4) + 64 j over this thread's 16-byte chunks.
-template <int WM, int WN, int R>
-__device__ __forceinline__ void sum_stage(const int4 (&r)[R], int (&rowsum)[R / 2]) {
+// LDS of one workgroup: a ring of 4 stages for each operand, as separate arrays so
+// every access names its stage statically (the compiler then waits only for the DMA
+// that wrote the stage being read).
+template <int WM, int WN>
+struct GramLds {
     using S = GramShape<WM, WN>;
-#pragma unroll
-    for (int q = 0; q < S::ROUNDS; ++q) {
-        int s = rowsum[q >> 1];
-        s = __builtin_amdgcn_sdot4(r[q].x, 0x01010101, s, false);
-        s = __builtin_amdgcn_sdot4(r[q].y, 0x01010101, s, false);
-        s = __builtin_amdgcn_sdot4(r[q].z, 0x01010101, s, false);
-        s = __builtin_amdgcn_sdot4(r[q].w, 0x01010101, s, false);
-        rowsum[q >> 1] = s;
-    }
-}
+    int8_t* a[4];
+    int8_t* b[4];
+    unsigned int* diag;
+};
 
-// One tile.  DIAG: ti == tj, a single panel feeds both operands, the MFMA blocks
-// strictly below the main diagonal (which only feed tau < 0) are skipped, and the
-// per-frame sums of the panel rows are accumulated.
 template <int WM, int WN, bool DIAG>
-__device__ __forceinline__ void gram_tile(const int8_t* __restrict__ slab_bin, int kb0, int kb1,
-                                          int row0, int col0, int nframes, int biased,
-                                          int8_t* lds, unsigned int* sDiag,
-                                          int64_t* __restrict__ out, int64_t* __restrict__ sums) {
+struct GramTile {
     using S = GramShape<WM, WN>;
-    const int tid = threadIdx.x;
-    const int lane = tid & 63;
-    const int wave = tid >> 6;
-    const int wr = wave >> 1;
-    const int wc = wave & 1;
-    // LDS: buffer b holds A at lds + b * 2 * PANEL, B right after it.
-    auto sA = [&](int b) { return lds + b * 2 * S::PANEL; };
-    auto sB = [&](int b) { return DIAG ? sA(b) : lds + b * 2 * S::PANEL + S::PANEL; };
-
-    for (int i = tid; i < 2 * S::NDIAG; i += 256) sDiag[i] = 0u;
-
+    static constexpr int L = DIAG ? S::LOADS : 2 * S::LOADS;  // DMA per thread per stage
+    const int8_t* __restrict__ slab_bin;
+    int kb0, kb1, row0, col0, nframes, nrows;
+    int tid, lane, wr, wc;
+    GramLds<WM, WN> lds;
     v16i acc[WM][WN];
-#pragma unroll
-    for (int mi = 0; mi < WM; ++mi)
-#pragma unroll
-        for (int ni = 0; ni < WN; ++ni)
-#pragma unroll
-            for (int v = 0; v < 16; ++v) acc[mi][ni][v] = 0;
+    int rowsum;
 
-    int rowsum[S::ROUNDS / 2];
-#pragma unroll
-    for (int j = 0; j < S::ROUNDS / 2; ++j) rowsum[j] = 0;
+    __device__ __forceinline__ static bool live(int wr, int wc, int mi, int ni) {
+        return !DIAG || (wc * WN * 32 + ni * 32 + 31 >= wr * WM * 32 + mi * 32);
+    }
 
-    const int frag_row = lane & 31;
-    const int frag_k = (lane >> 5) * 16;
+    template <int SLOT>
+    __device__ __forceinline__ void issue(int kb) {
+        dma_panel<WM, WN>(slab_bin, kb, row0, nrows, lds.a[SLOT], tid);
+        if (!DIAG) dma_panel<WM, WN>(slab_bin, kb, col0, nrows, lds.b[SLOT], tid);
+    }
 
-    int4 ra[S::ROUNDS], rb[S::ROUNDS];
-    load_stage<WM, WN>(ra, slab_bin, kb0, kb1, row0, nframes, tid);
-    if (!DIAG) load_stage<WM, WN>(rb, slab_bin, kb0, kb1, col0, nframes, tid);
-    if (DIAG) sum_stage<WM, WN>(ra, rowsum);
-    store_stage<WM, WN>(ra, sA(0), tid);
-    if (!DIAG) store_stage<WM, WN>(rb, sB(0), tid);
-    __syncthreads();
-
-    int buf = 0;
-    for (int kb = kb0; kb < kb1; kb += S::SKB) {
-        const bool more = kb + S::SKB < kb1;
-        if (more) {
-            load_stage<WM, WN>(ra, slab_bin, kb + S::SKB, kb1, row0, nframes, tid);
-            if (!DIAG) load_stage<WM, WN>(rb, slab_bin, kb + S::SKB, kb1, col0, nframes, tid);
-        }
-        const int8_t* a_s = sA(buf);
-        const int8_t* b_s = sB(buf);
-        // Fragments of k-step ks+1 are read while the MFMAs of k-step ks run; the
-        // scheduling barrier keeps the compiler from hoisting further LDS reads.
-        v4i fa[2][WM], fb[2][WN];
+    // Consume stage kb from ring slot SLOT; refill slot (SLOT + 3) % 4 with kb + 3.
+    template <int SLOT>
+    __device__ __forceinline__ void stage(int kb) {
+        const int later = kb1 - 1 - kb;  // stages issued after kb that may still be in flight
+        if (later >= 2)
+            wait_vm<2 * L>();
+        else if (later == 1)
+            wait_vm<L>();
+        else
+            wait_vm<0>();
+        // The barrier orders (a) every wave's completed DMA of this stage before any
+        // read of it and (b) every read of the previous stage before its slot is refilled.
+        __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
+        __builtin_amdgcn_s_barrier();
+        if (kb + 3 < kb1) issue<(SLOT + 3) & 3>(kb + 3);
+        const int8_t* a_s = lds.a[SLOT];
+        const int8_t* b_s = DIAG ? lds.a[SLOT] : lds.b[SLOT];
+        if (DIAG && tid < S::TM) {
 #pragma unroll
-        for (int mi = 0; mi < WM; ++mi)
-            fa[0][mi] = *reinterpret_cast<const v4i*>(
-                a_s + (wr * WM * 32 + mi * 32 + frag_row) * S::STRIDE + frag_k);
-#pragma unroll
-        for (int ni = 0; ni < WN; ++ni)
-            fb[0][ni] = *reinterpret_cast<const v4i*>(
-                b_s + (wc * WN * 32 + ni * 32 + frag_row) * S::STRIDE + frag_k);
-#pragma unroll
-        for (int ks = 0; ks < S::SK / 32; ++ks) {
-            const int cur = ks & 1;
-            if (ks + 1 < S::SK / 32) {
-#pragma unroll
-                for (int mi = 0; mi < WM; ++mi)
-                    fa[cur ^ 1][mi] = *reinterpret_cast<const v4i*>(
-                        a_s + (wr * WM * 32 + mi * 32 + frag_row) * S::STRIDE + (ks + 1) * 32 +
-                        frag_k);
-#pragma unroll
-                for (int ni = 0; ni < WN; ++ni)
-                    fb[cur ^ 1][ni] = *reinterpret_cast<const v4i*>(
-                        b_s + (wc * WN * 32 + ni * 32 + frag_row) * S::STRIDE + (ks + 1) * 32 +
-                        frag_k);
+            for (int q = 0; q < 4; ++q) {
+                const v4i x = *reinterpret_cast<const v4i*>(a_s + swz(tid, q));
+                rowsum = __builtin_amdgcn_sdot4(x.x, 0x01010101, rowsum, false);
+                rowsum = __builtin_amdgcn_sdot4(x.y, 0x01010101, rowsum, false);
+                rowsum = __builtin_amdgcn_sdot4(x.z, 0x01010101, rowsum, false);
+                rowsum = __builtin_amdgcn_sdot4(x.w, 0x01010101, rowsum, false);
             }
+        }
+        const int frag_row = lane & 31;
+        const int frag_q = lane >> 5;
+#pragma unroll
+        for (int ks = 0; ks < KB / 32; ++ks) {
+            v4i a[WM], b[WN];
+#pragma unroll
+            for (int mi = 0; mi < WM; ++mi)
+                a[mi] = *reinterpret_cast<const v4i*>(
+                    a_s + swz(wr * WM * 32 + mi * 32 + frag_row, ks * 2 + frag_q));
+#pragma unroll
+            for (int ni = 0; ni < WN; ++ni)
+                b[ni] = *reinterpret_cast<const v4i*>(
+                    b_s + swz(wc * WN * 32 + ni * 32 + frag_row, ks * 2 + frag_q));
 #pragma unroll
             for (int mi = 0; mi < WM; ++mi)
 #pragma unroll
-                for (int ni = 0; ni < WN; ++ni) {
-                    const bool live =
-                        !DIAG || (wc * WN * 32 + ni * 32 + 31 >= wr * WM * 32 + mi * 32);
-                    if (live)
-                        acc[mi][ni] = __builtin_amdgcn_mfma_i32_32x32x32_i8(
-                            fa[cur][mi], fb[cur][ni], acc[mi][ni], 0, 0, 0);
-                }
-            __builtin_amdgcn_sched_barrier(0);
+                for (int ni = 0; ni < WN; ++ni)
+                    if (live(wr, wc, mi, ni))
+                        acc[mi][ni] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[mi], b[ni],
+                                                                            acc[mi][ni], 0, 0, 0);
         }
-        if (more) {
-            if (DIAG) sum_stage<WM, WN>(ra, rowsum);
-            store_stage<WM, WN>(ra, sA(buf ^ 1), tid);
-            if (!DIAG) store_stage<WM, WN>(rb, sB(buf ^ 1), tid);
+    }
+
+    __device__ __forceinline__ void run(int biased, int64_t* __restrict__ out,
+                                        int64_t* __restrict__ sums) {
+        unsigned int* sDiag = lds.diag;
+        for (int i = tid; i < 2 * S::NDIAG; i += 256) sDiag[i] = 0u;
+#pragma unroll
+        for (int mi = 0; mi < WM; ++mi)
+#pragma unroll
+            for (int ni = 0; ni < WN; ++ni)
+#pragma unroll
+                for (int v = 0; v < 16; ++v) acc[mi][ni][v] = 0;
+        rowsum = 0;
+
+        if (kb0 < kb1) issue<0>(kb0);
+        if (kb0 + 1 < kb1) issue<1>(kb0 + 1);
+        if (kb0 + 2 < kb1) issue<2>(kb0 + 2);
+        for (int kb = kb0; kb < kb1; kb += 4) {
+            stage<0>(kb);
+            if (kb + 1 < kb1) stage<1>(kb + 1);
+            if (kb + 2 < kb1) stage<2>(kb + 2);
+            if (kb + 3 < kb1) stage<3>(kb + 3);
         }
+
+        if (DIAG && tid < S::TM) {
+            const int t = row0 + tid;
+            const int64_t v = (int64_t)rowsum + (biased ? (int64_t)128 * (kb1 - kb0) * KB : 0);
+            if (t < nframes && v != 0)
+                atomicAdd(reinterpret_cast<unsigned long long*>(sums + t), (unsigned long long)v);
+        }
+
+        // Epilogue.  32x32 i32 accumulator layout: lane holds column (lane & 31) and
+        // rows 8*(v/4) + 4*(lane/32) + (v%4), v = 0..15.  Diagonal slot c - r + TM - 1.
         __syncthreads();
-        buf ^= 1;
-    }
-
-    if (DIAG) {
-        // rows (tid/4) + 64 j: combine the 4 chunks of a row, add the bias back.
-        const int64_t kitems = (int64_t)(kb1 - kb0) * KB;
 #pragma unroll
-        for (int j = 0; j < S::ROUNDS / 2; ++j) {
-            int s = rowsum[j];
-            s += __shfl_xor(s, 1);
-            s += __shfl_xor(s, 2);
-            const int t = row0 + j * S::ROWS_PER_ROUND + (tid >> 2);
-            if ((tid & 3) == 0 && t < nframes) {
-                const int64_t v = (int64_t)s + (biased ? 128 * kitems : 0);
-                if (v != 0)
-                    atomicAdd(reinterpret_cast<unsigned long long*>(sums + t),
-                              (unsigned long long)v);
-            }
-        }
-    }
-
-    // Epilogue.  32x32 i32 accumulator layout: lane holds column (lane & 31) and rows
-    // 8*(v/4) + 4*(lane/32) + (v%4), v = 0..15.  Diagonal slot c - r + TM - 1.
-    // One block at a time keeps the accumulator reads from being hoisted together.
+        for (int mi = 0; mi < WM; ++mi)
 #pragma unroll
-    for (int mi = 0; mi < WM; ++mi)
+            for (int ni = 0; ni < WN; ++ni) {
+                if (live(wr, wc, mi, ni)) {
+                    const int cbase = wc * WN * 32 + ni * 32 + (lane & 31);
+                    const int rbase = wr * WM * 32 + mi * 32 + 4 * (lane >> 5);
 #pragma unroll
-        for (int ni = 0; ni < WN; ++ni) {
-            const bool live = !DIAG || (wc * WN * 32 + ni * 32 + 31 >= wr * WM * 32 + mi * 32);
-            if (live) {
-                const int cbase = wc * WN * 32 + ni * 32 + (lane & 31);
-                const int rbase = wr * WM * 32 + mi * 32 + 4 * (lane >> 5);
-#pragma unroll
-                for (int v = 0; v < 16; ++v) {
-                    const int r = rbase + 8 * (v >> 2) + (v & 3);
-                    const int val = acc[mi][ni][v];
-                    const int slot = cbase - r + S::TM - 1;
-                    atomicAdd(&sDiag[slot], (unsigned)(val & 0xFFFF));
-                    atomicAdd(&sDiag[S::NDIAG + slot], (unsigned)(val >> 16));
+                    for (int v = 0; v < 16; ++v) {
+                        const int r = rbase + 8 * (v >> 2) + (v & 3);
+                        const int val = acc[mi][ni][v];
+                        const int slot = cbase - r + S::TM - 1;
+                        atomicAdd(&sDiag[slot], (unsigned)(val & 0xFFFF));
+                        atomicAdd(&sDiag[S::NDIAG + slot], (unsigned)(val >> 16));
+                    }
                 }
+                __builtin_amdgcn_sched_barrier(0);
             }
-            __builtin_amdgcn_sched_barrier(0);
+        __syncthreads();
+        for (int i = tid; i < S::NDIAG; i += 256) {
+            const long long v =
+                (long long)(int)sDiag[S::NDIAG + i] * 65536 + (long long)sDiag[i];
+            const int tau = col0 - row0 + i - (S::TM - 1);
+            if (v != 0 && tau >= 0 && tau < nframes)
+                atomicAdd(reinterpret_cast<unsigned long long*>(out + tau), (unsigned long long)v);
         }
-    __syncthreads();
-    for (int i = tid; i < S::NDIAG; i += 256) {
-        const long long v = (long long)(int)sDiag[S::NDIAG + i] * 65536 + (long long)sDiag[i];
-        const int tau = col0 - row0 + i - (S::TM - 1);
-        if (v != 0 && tau >= 0 && tau < nframes)
-            atomicAdd(reinterpret_cast<unsigned long long*>(out + tau), (unsigned long long)v);
     }
-}
+};
 
 template <int WM, int WN>
 __global__ __launch_bounds__(256) void gram_i8_kernel(const int8_t* __restrict__ slab,
                                                       const BinDev* __restrict__ bins,
                                                       const GramWork* __restrict__ work,
-                                                      int nframes, int biased,
+                                                      int nframes, int nrows, int biased,
                                                       int64_t* __restrict__ ugram,
                                                       int64_t* __restrict__ sums) {
     using S = GramShape<WM, WN>;
     static_assert(S::TM == S::TN, "square tiles");
-    __shared__ __attribute__((aligned(16))) int8_t lds[4 * S::PANEL];
+    __shared__ __attribute__((aligned(1024))) int8_t a0[S::PANEL], a1[S::PANEL], a2[S::PANEL],
+        a3[S::PANEL];
+    __shared__ __attribute__((aligned(1024))) int8_t b0[S::PANEL], b1[S::PANEL], b2[S::PANEL],
+        b3[S::PANEL];
     __shared__ unsigned int sDiag[2 * S::NDIAG];
 
     const GramWork w = work[blockIdx.x];
     if (w.bin < 0) return;  // padding item
     const BinDev bd = bins[w.bin];
-    const int8_t* slab_bin = slab + bd.slab_off;
+    const GramLds<WM, WN> lds{{a0, a1, a2, a3}, {b0, b1, b2, b3}, sDiag};
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6;
     int64_t* out = ugram + (int64_t)w.bin * nframes;
     int64_t* srow = sums + (int64_t)w.bin * nframes;
-    if (w.ti == w.tj)
-        gram_tile<WM, WN, true>(slab_bin, w.kb0, w.kb1, w.ti * S::TM, w.tj * S::TN, nframes,
-                                biased, lds, sDiag, out, srow);
-    else
-        gram_tile<WM, WN, false>(slab_bin, w.kb0, w.kb1, w.ti * S::TM, w.tj * S::TN, nframes,
-                                 biased, lds, sDiag, out, srow);
+    if (w.ti == w.tj) {
+        GramTile<WM, WN, true> t{slab + bd.slab_off, w.kb0, w.kb1, w.ti * S::TM, w.tj * S::TN,
+                                 nframes, nrows, tid, tid & 63, wave >> 1, wave & 1, lds};
+        t.run(biased, out, srow);
+    } else {
+        GramTile<WM, WN, false> t{slab + bd.slab_off, w.kb0, w.kb1, w.ti * S::TM, w.tj * S::TN,
+                                  nframes, nrows, tid, tid & 63, wave >> 1, wave & 1, lds};
+        t.run(biased, out, srow);
+    }
 }
 
 int gram_tile_size(int variant) { return variant == 1 ? 128 : 256; }
 
 void launch_gram_i8(int tile, const int8_t* slab, const BinDev* bins, const GramWork* work,
-                    int n_work, int nframes, int biased, int64_t* ugram, int64_t* sums,
-                    hipStream_t s) {
+                    int n_work, int nframes, int nrows, int biased, int64_t* ugram,
+                    int64_t* sums, hipStream_t s) {
     if (n_work <= 0) return;
     if (tile == 128)
         gram_i8_kernel<2, 2><<<dim3((unsigned)n_work), dim3(256), 0, s>>>(
-            slab, bins, work, nframes, biased, ugram, sums);
+            slab, bins, work, nframes, nrows, biased, ugram, sums);
     else
         gram_i8_kernel<4, 4><<<dim3((unsigned)n_work), dim3(256), 0, s>>>(
-            slab, bins, work, nframes, biased, ugram, sums);
+            slab, bins, work, nframes, nrows, biased, ugram, sums);
 }
 
 }  // namespace dxg2
