@@ -50,6 +50,11 @@ _SIGNATURES = {
     "dxg2_plan_footprint": (c_int, [c_void_p, c_int, ctypes.POINTER(c_size_t)]),
     "dxg2_correlate_dense": (c_int, [c_void_p] * 7),
     "dxg2_correlate_dense_plain": (c_int, [c_void_p] * 4),
+    "dxg2_correlate_events_csr": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_i64,
+                                          c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "dxg2_correlate_events_coo": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_i64,
+                                          c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "dxg2_plan_event_status": (c_int, [c_void_p, ctypes.POINTER(c_int), c_void_p]),
     "dxg2_plan_set_profiling": (c_int, [c_void_p, c_int]),
     "dxg2_plan_stage_times": (c_int, [c_void_p, c_void_p, c_int]),
     "dxg2_synth_speckle": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_double,

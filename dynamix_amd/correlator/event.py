@@ -1,0 +1,193 @@
+"""Event (sparse) correlator on the MI355X: drop-in for dynamix/correlator/event.py.
+
+EventCorrelator   -- event.py:7-204.  Same constructor and correlate(vol_times,
+                     vol_data, offsets) on the pixel-major event layout; also
+                     correlate_coo(pixel, frame, count) for frame-sorted event lists.
+FramesCompressor  -- event.py:207-321.  Host-side producer of that layout.
+
+Differences from the reference, all deliberate:
+  * numerators are recomputed from zero on every call (the reference zeroes res_int
+    only at allocation, event.py:98 vs 170-171, so repeated calls accumulate);
+  * max_events_count is accepted but not a limit (the reference copies a pixel's
+    events into private arrays of that size without checking, evtcorrelator.cl:29-34);
+  * unsorted or out-of-range frames raise ValueError instead of indexing out of bounds;
+  * numerators are exact 64-bit integers and the normalisation is exact with one FP64
+    division (the reference accumulates it in float32, evtcorrelator.cl:70-73).
+"""
+import ctypes
+
+import numpy as np
+
+from .. import _lib
+from .common import BaseCorrelator, Plan, _torch, current_stream
+
+_FLAG_TEXT = {1: "frames out of [0, nframes) or not strictly increasing within a pixel",
+              2: "offsets decreasing or past the number of events",
+              4: "pixel index past the frame size"}
+
+
+def _dtype_of(dtype):
+    dt = np.dtype(dtype)
+    if dt not in _lib.DTYPE_CODES:
+        raise NotImplementedError("event counts of dtype %s are not supported "
+                                  "(uint8, int8, uint16)" % dt)
+    return dt
+
+
+class EventCorrelator(BaseCorrelator):
+    """Drop-in for EventCorrelator (event.py:15-204).
+
+    ctx/devicetype/platformid/deviceid/block_size/memory/profile are OpenCL options and
+    are accepted and ignored.  scale_factor is honoured as in the reference (a scalar
+    for one bin, or one value per bin; default: pixel count of each bin).
+    """
+
+    def __init__(self, shape, nframes, max_events_count, total_events_count=None,
+                 allow_reallocate=True, qmask=None, dtype="f", weights=None, scale_factor=None,
+                 extra_options={}, ctx=None, devicetype="all", platformid=None, deviceid=None,
+                 block_size=None, memory=None, profile=False):
+        super().__init__()
+        super()._set_parameters(shape, nframes, qmask, scale_factor, extra_options)
+        self._set_weights(weights)
+        self.dtype = _dtype_of(dtype)
+        self._set_events_size(max_events_count, total_events_count)
+        self.allow_reallocate = allow_reallocate
+        q = self.qmask if self._qmask_given else None
+        scale = np.array([self.scale_factors[b] for b in self.bins], dtype=np.float64) \
+            if self.n_bins > 0 else None
+        self._plan = Plan(self.shape, self.nframes, q, self.dtype, scale=scale)
+        self.last_numerators = None
+        self.last_sums = None
+        self.last_denominators = None
+        self.last_g2 = None
+
+    def _set_events_size(self, max_events_count, total_events_count):  # event.py:61-63
+        self.max_events_count = max_events_count
+        self.total_events_count = total_events_count or max_events_count * int(np.prod(self.shape))
+
+    def _check_event_arrays(self, vol_times, vol_data, offsets):  # event.py:104-115
+        for arr in [vol_times, vol_data, offsets]:
+            assert arr.ndim == 1
+        assert vol_times.shape[0] == vol_data.shape[0]
+        assert offsets.shape[0] == int(np.prod(self.shape)) + 1
+        numels = vol_data.shape[0]
+        if numels > self.total_events_count:
+            if self.allow_reallocate:
+                self.total_events_count = numels
+            else:
+                raise ValueError("Too many events and allow_reallocate was set to False")
+
+    def _outputs(self, torch):
+        nb, n = max(self.n_bins, 0), self.nframes
+        return (torch.empty((nb, n), dtype=torch.int64, device="cuda"),
+                torch.empty((nb, n), dtype=torch.int64, device="cuda"),
+                torch.empty((nb, n), dtype=torch.float64, device="cuda"),
+                torch.empty((nb, n), dtype=torch.float32, device="cuda"))
+
+    def _finish(self, torch, outs):
+        num, sums, den, g2 = outs
+        flags = ctypes.c_int32()
+        _lib.check(self._plan.lib.dxg2_plan_event_status(self._plan.handle, ctypes.byref(flags),
+                                                         current_stream(torch)))
+        if flags.value:
+            msg = "; ".join(t for b, t in _FLAG_TEXT.items() if flags.value & b)
+            raise ValueError("invalid event input: " + msg)
+        self.last_numerators, self.last_sums, self.last_denominators, self.last_g2 = outs
+        if self.extra_options.get("return_device", False):
+            return g2
+        return g2.cpu().numpy()
+
+    @staticmethod
+    def _dev(torch, a, dtype):
+        if isinstance(a, np.ndarray):
+            return torch.from_numpy(np.ascontiguousarray(a, dtype=dtype)).to("cuda")
+        if isinstance(a, torch.Tensor):
+            return a.to(device="cuda").contiguous()
+        raise ValueError("Unknown array type %s" % str(type(a)))
+
+    def correlate(self, vol_times, vol_data, offsets):
+        """g2 from pixel-major events (times int32, data, offsets uint32)."""
+        torch = _torch()
+        self._check_event_arrays(vol_times, vol_data, offsets)
+        if isinstance(vol_data, np.ndarray):
+            assert np.dtype(vol_data.dtype) == self.dtype, \
+                "vol_data dtype %s != %s" % (vol_data.dtype, self.dtype)
+        t = self._dev(torch, vol_times, np.int32)
+        d = self._dev(torch, vol_data, self.dtype)
+        o = self._dev(torch, offsets, np.uint32)
+        outs = self._outputs(torch)
+        if self.n_bins:
+            _lib.check(self._plan.lib.dxg2_correlate_events_csr(
+                self._plan.handle, _lib.ptr(t), _lib.ptr(d), _lib.ptr(o),
+                ctypes.c_int64(int(vol_data.shape[0])), *[_lib.ptr(x) for x in outs],
+                current_stream(torch)), "dxg2_correlate_events_csr")
+        return self._finish(torch, outs)
+
+    def correlate_coo(self, pixel, frame, count):
+        """g2 from a frame-sorted event list (uint32 pixel, uint32 frame, uint8 count)."""
+        torch = _torch()
+        assert pixel.shape == frame.shape == count.shape and pixel.ndim == 1
+        if pixel.shape[0] > self.total_events_count:
+            if self.allow_reallocate:
+                self.total_events_count = pixel.shape[0]
+            else:
+                raise ValueError("Too many events and allow_reallocate was set to False")
+        px = self._dev(torch, pixel, np.uint32)
+        fr = self._dev(torch, frame, np.uint32)
+        ct = self._dev(torch, count, self.dtype)
+        outs = self._outputs(torch)
+        if self.n_bins:
+            _lib.check(self._plan.lib.dxg2_correlate_events_coo(
+                self._plan.handle, _lib.ptr(px), _lib.ptr(fr), _lib.ptr(ct),
+                ctypes.c_int64(int(pixel.shape[0])), *[_lib.ptr(x) for x in outs],
+                current_stream(torch)), "dxg2_correlate_events_coo")
+        return self._finish(torch, outs)
+
+
+class FramesCompressor(object):
+    """Dense frames -> (data, times, offsets) event layout (event.py:207-321).
+
+    For every pixel in raster order, its non-zero samples sorted by frame are
+    concatenated; offsets[p] .. offsets[p+1] index pixel p's events.
+    """
+
+    def __init__(self, shape, nframes, max_nnz, dtype=np.int8):
+        self.shape = shape
+        self.nframes = nframes
+        self.dtype = dtype
+        self.max_nnz = max_nnz
+        self.npix = int(np.prod(self.shape))
+        self.events_counter = np.zeros(self.npix, dtype=np.int32)
+        self.events = np.zeros((self.npix, self.max_nnz), dtype=self.dtype)
+        self.times = np.zeros((self.npix, self.max_nnz), dtype=np.int32)
+        self.frames_counter = 0
+
+    @staticmethod
+    def compress_all_stack(frames):
+        """Whole stack at once (event.py:267-284)."""
+        assert frames.ndim == 3
+        n = frames.shape[0]
+        per_pixel = np.moveaxis(frames, 0, -1).reshape(-1, n)
+        pix, t = np.nonzero(per_pixel > 0)
+        offsets = np.zeros(per_pixel.shape[0] + 1, dtype=np.uint32)
+        offsets[1:] = np.cumsum(np.bincount(pix, minlength=per_pixel.shape[0]))
+        return per_pixel[pix, t], t.astype(np.int32), offsets
+
+    def process_frame(self, frame):
+        """Append one frame's events (event.py:287-297)."""
+        f = frame.ravel()
+        m = f > 0
+        self.events[m, self.events_counter[m]] = f[m]
+        self.times[m, self.events_counter[m]] = self.frames_counter
+        self.events_counter[m] += 1
+        self.frames_counter += 1
+
+    def get_compacted_events(self, wait_for_all_frames=True):
+        """(data, times, offsets) of the frames processed so far (event.py:300-321)."""
+        if self.frames_counter < self.nframes - 1 and wait_for_all_frames:
+            raise RuntimeError("Not all frames were compressed yet (%d/%d)"
+                               % (self.frames_counter, self.nframes))
+        offsets = np.zeros(self.npix + 1, dtype=np.uint32)
+        offsets[1:] = np.cumsum(self.events_counter)
+        m = self.events.ravel() > 0
+        return self.events.ravel()[m], self.times.ravel()[m], offsets

@@ -96,6 +96,17 @@ struct dxg2_plan {
     float* d_pm = nullptr;
     size_t pa_elems = 0;
 
+    // event path
+    std::vector<EventWork> event_work;
+    EventWork* d_ework = nullptr;
+    int* d_err = nullptr;
+    uint32_t* d_cnt = nullptr;     // per-pixel counts / scatter cursors (COO input)
+    uint32_t* d_scan = nullptr;    // scan block totals
+    uint32_t* d_off = nullptr;     // CSR offsets built from COO input
+    int32_t* d_etimes = nullptr;   // CSR frames built from COO input
+    void* d_edata = nullptr;       // CSR counts built from COO input
+    int64_t ev_cap = 0;            // capacity of d_etimes / d_edata in events
+
     // Profiling: each correlate call records 4 events on its stream (no host sync);
     // dxg2_plan_stage_times() waits for the last one and averages every recorded call.
     bool profiling = false;
@@ -120,6 +131,8 @@ void build_gram_work(dxg2_plan* p) {
     const int nT = (p->N + T - 1) / T;
     const int band = std::max(1, env_int("DXG2_GRAM_BAND", 8));
     const int group = std::max(0, env_int("DXG2_GRAM_GROUP", 64));
+    const int kchunk = std::min(MAX_KBLOCKS_PER_ITEM,
+                                std::max(1, env_int("DXG2_GRAM_KCHUNK", MAX_KBLOCKS_PER_ITEM)));
     std::vector<int> order(p->n_bins);
     for (int b = 0; b < p->n_bins; ++b) order[b] = b;
     // Largest bins first so their long tiles start early (LPT).
@@ -129,8 +142,8 @@ void build_gram_work(dxg2_plan* p) {
     for (int b : order) {
         const int nkb = p->bins[b].nkb;
         if (nkb == 0) continue;
-        for (int k0 = 0; k0 < nkb; k0 += MAX_KBLOCKS_PER_ITEM) {
-            const int k1 = std::min(nkb, k0 + MAX_KBLOCKS_PER_ITEM);
+        for (int k0 = 0; k0 < nkb; k0 += kchunk) {
+            const int k1 = std::min(nkb, k0 + kchunk);
             for (int i0 = 0; i0 < nT; i0 += band)
                 for (int j = i0; j < nT; ++j)
                     for (int i = i0; i < std::min(nT, i0 + band) && i <= j; ++i)
@@ -149,6 +162,47 @@ int next_events(dxg2_plan* p, hipEvent_t** out) {
         p->ev_pool.push_back(q);
     }
     *out = p->ev_pool[p->ev_used++].data();
+    return DXG2_OK;
+}
+
+// Pixels of one bin's list handled by one event work item.
+constexpr int EVENT_ITEM_PIXELS = 1024;
+
+void build_event_work(dxg2_plan* p) {
+    for (int b = 0; b < p->n_bins; ++b)
+        for (int k0 = 0; k0 < p->bins[b].npix; k0 += EVENT_ITEM_PIXELS)
+            p->event_work.push_back(
+                EventWork{b, k0, std::min<int>(p->bins[b].npix, k0 + EVENT_ITEM_PIXELS), 0});
+}
+
+int ensure_event_buffers(dxg2_plan* p, int64_t nnz_coo) {
+    if (!p->d_err) {
+        DXG2_HIP(hipMalloc(&p->d_err, sizeof(int)));
+        if (!p->event_work.empty()) {
+            DXG2_HIP(hipMalloc(&p->d_ework, p->event_work.size() * sizeof(EventWork)));
+            DXG2_HIP(hipMemcpy(p->d_ework, p->event_work.data(),
+                               p->event_work.size() * sizeof(EventWork), hipMemcpyHostToDevice));
+        }
+    }
+    if (nnz_coo >= 0) {
+        const int64_t npix = (int64_t)p->H * p->W;
+        if (!p->d_cnt) {
+            DXG2_HIP(hipMalloc(&p->d_cnt, npix * 4));
+            DXG2_HIP(hipMalloc(&p->d_off, (npix + 1) * 4));
+            DXG2_HIP(hipMalloc(&p->d_scan, ((npix + 1023) / 1024 + 1) * 4));
+        }
+        if (nnz_coo > p->ev_cap) {
+            if (p->d_etimes) (void)hipFree(p->d_etimes);
+            if (p->d_edata) (void)hipFree(p->d_edata);
+            p->d_etimes = nullptr;
+            p->d_edata = nullptr;
+            const int64_t cap = std::max<int64_t>(nnz_coo, 1);
+            if (hipMalloc(&p->d_etimes, cap * 4) != hipSuccess ||
+                hipMalloc(&p->d_edata, cap * 2) != hipSuccess)
+                return set_error(DXG2_ENOMEM, "cannot allocate %lld events", (long long)cap);
+            p->ev_cap = cap;
+        }
+    }
     return DXG2_OK;
 }
 
@@ -239,6 +293,7 @@ int dxg2_plan_create(const dxg2_plan_desc* desc, dxg2_plan** out) {
     }
     p->slab_bytes = (size_t)slab_off;
     build_gram_work(p);
+    build_event_work(p);
 
     hipError_t e = hipGetDevice(&p->device);
     if (e != hipSuccess) {
@@ -277,8 +332,10 @@ int dxg2_plan_create(const dxg2_plan_desc* desc, dxg2_plan** out) {
 
 int dxg2_plan_destroy(dxg2_plan* p) {
     if (!p) return DXG2_OK;
-    void* bufs[] = {p->d_pix, p->d_bins, p->d_gwork, p->d_mwork, p->d_slab,
-                    p->d_ugram, p->d_sums, p->d_pa, p->d_pg, p->d_pm};
+    void* bufs[] = {p->d_pix,   p->d_bins, p->d_gwork, p->d_mwork, p->d_slab,
+                    p->d_ugram, p->d_sums, p->d_pa,    p->d_pg,    p->d_pm,
+                    p->d_ework, p->d_err,  p->d_cnt,   p->d_scan,  p->d_off,
+                    p->d_etimes, p->d_edata};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     for (auto& q : p->ev_pool)
@@ -311,6 +368,13 @@ int dxg2_plan_footprint(const dxg2_plan* p, int32_t path, size_t* bytes) {
             int64_t mx = 0;
             for (auto n : p->npix) mx = std::max(mx, n);
             *bytes = common + ((size_t)p->N * ((mx + 7) / 8 * 8) + (size_t)p->N * p->N + p->N) * 4;
+            return DXG2_OK;
+        }
+        case 3: {
+            const int64_t npix = (int64_t)p->H * p->W;
+            *bytes = common + p->event_work.size() * sizeof(EventWork) + sizeof(int) +
+                     (size_t)(2 * npix + 1 + (npix + 1023) / 1024 + 1) * 4 +
+                     (size_t)p->ev_cap * 6;
             return DXG2_OK;
         }
         default:
@@ -416,6 +480,75 @@ int dxg2_correlate_dense_plain(dxg2_plan* p, const void* d_frames, float* d_g2, 
         DXG2_HIP(hipEventRecord(ev[3], s));
     }
     DXG2_HIP(hipGetLastError());
+    return DXG2_OK;
+}
+
+// Shared tail of both event entry points: pairs + sums from pixel-major CSR, then the
+// exact denominator and g2 (no bias: counts are used as they are).
+static int events_from_csr(dxg2_plan* p, const int32_t* times, const void* data,
+                           const uint32_t* offsets, int64_t nnz, int64_t* d_num, int64_t* d_sums,
+                           double* d_den, float* d_g2, hipStream_t s) {
+    int64_t* sums = d_sums ? d_sums : p->d_sums;
+    launch_event_pairs(p->dtype, times, data, offsets, p->d_pix, p->d_bins, p->d_ework,
+                       (int)p->event_work.size(), p->N, nnz, p->d_ugram, sums, p->d_err, s);
+    launch_finalize(sums, p->d_ugram, p->d_bins, p->n_bins, p->N, 0, d_num, d_den, d_g2, s);
+    DXG2_HIP(hipGetLastError());
+    return DXG2_OK;
+}
+
+int dxg2_correlate_events_csr(dxg2_plan* p, const int32_t* d_times, const void* d_data,
+                              const uint32_t* d_offsets, int64_t nnz, int64_t* d_num,
+                              int64_t* d_sums, double* d_den, float* d_g2, void* stream) {
+    if (!p) return set_error(DXG2_EINVAL, "null plan");
+    if (!d_offsets || !d_g2 || (nnz > 0 && (!d_times || !d_data)))
+        return set_error(DXG2_EINVAL, "times, data, offsets and g2 must be non-null");
+    if (nnz < 0 || nnz > (int64_t)UINT32_MAX)
+        return set_error(DXG2_EINVAL, "event count %lld out of range", (long long)nnz);
+    if (p->n_bins == 0) return DXG2_OK;
+    int rc = ensure_event_buffers(p, -1);
+    if (rc) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    (void)hipGetLastError();
+    const int64_t nbn = (int64_t)p->n_bins * p->N;
+    launch_fill_i64(d_sums ? d_sums : p->d_sums, nbn, s);
+    launch_fill_i64(p->d_ugram, nbn, s);
+    launch_fill_i32(p->d_err, 1, s);
+    return events_from_csr(p, d_times, d_data, d_offsets, nnz, d_num, d_sums, d_den, d_g2, s);
+}
+
+int dxg2_correlate_events_coo(dxg2_plan* p, const uint32_t* d_pixel, const uint32_t* d_frame,
+                              const void* d_count, int64_t nnz, int64_t* d_num, int64_t* d_sums,
+                              double* d_den, float* d_g2, void* stream) {
+    if (!p) return set_error(DXG2_EINVAL, "null plan");
+    if (!d_g2 || (nnz > 0 && (!d_pixel || !d_frame || !d_count)))
+        return set_error(DXG2_EINVAL, "pixel, frame, count and g2 must be non-null");
+    if (nnz < 0 || nnz > (int64_t)UINT32_MAX)
+        return set_error(DXG2_EINVAL, "event count %lld out of range", (long long)nnz);
+    if (p->dtype == DXG2_I8)
+        return set_error(DXG2_EUNSUPPORTED, "COO event counts are unsigned (uint8 / uint16)");
+    if (p->n_bins == 0) return DXG2_OK;
+    int rc = ensure_event_buffers(p, nnz);
+    if (rc) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    (void)hipGetLastError();
+    const int64_t nbn = (int64_t)p->n_bins * p->N;
+    launch_fill_i64(d_sums ? d_sums : p->d_sums, nbn, s);
+    launch_fill_i64(p->d_ugram, nbn, s);
+    launch_fill_i32(p->d_err, 1, s);
+    launch_coo_to_csr(p->dtype, d_pixel, d_frame, d_count, nnz, (uint32_t)((int64_t)p->H * p->W),
+                      p->d_cnt, p->d_scan, p->d_off, p->d_etimes, p->d_edata, p->d_err, s);
+    return events_from_csr(p, p->d_etimes, p->d_edata, p->d_off, nnz, d_num, d_sums, d_den,
+                           d_g2, s);
+}
+
+int dxg2_plan_event_status(dxg2_plan* p, int32_t* flags, void* stream) {
+    if (!p || !flags) return set_error(DXG2_EINVAL, "null argument");
+    *flags = 0;
+    if (!p->d_err) return DXG2_OK;
+    int v = 0;
+    DXG2_HIP(hipMemcpyAsync(&v, p->d_err, sizeof(int), hipMemcpyDeviceToHost, (hipStream_t)stream));
+    DXG2_HIP(hipStreamSynchronize((hipStream_t)stream));
+    *flags = v;
     return DXG2_OK;
 }
 

@@ -45,6 +45,13 @@ struct GramWork {        // one output tile (i, j >= i) over a k-block range
     int32_t pad[3];
 };
 
+struct EventWork {       // one workgroup of the event pair kernel (x dimension)
+    int32_t bin;
+    int32_t k0;          // range [k0, k1) of the bin's pixel list
+    int32_t k1;
+    int32_t pad;
+};
+
 // ------------------------------------------------------------------ launchers
 void launch_fill_i64(int64_t* p, int64_t n, hipStream_t s);
 void launch_gather_i8(const uint8_t* frames, int dtype, int64_t hw, int nframes, int nrows,
@@ -59,6 +66,14 @@ void launch_finalize(const int64_t* sums, const int64_t* ugram, const BinDev* bi
 void launch_plain_bin(const uint8_t* frames, int dtype, int64_t hw, int nframes,
                       const int32_t* pix, int npix, float* a, float* gram, float* means,
                       float* g2_row, hipStream_t s);
+void launch_event_pairs(int dtype, const int32_t* times, const void* data,
+                        const uint32_t* offsets, const int32_t* pix_idx, const BinDev* bins,
+                        const EventWork* work, int n_work, int nframes, int64_t nnz,
+                        int64_t* num, int64_t* sums, int* err, hipStream_t s);
+void launch_coo_to_csr(int dtype, const uint32_t* pix, const uint32_t* frame, const void* cnt,
+                       int64_t nnz, uint32_t npix, uint32_t* count, uint32_t* scan_tot,
+                       uint32_t* offsets, int32_t* times, void* data, int* err, hipStream_t s);
+void launch_fill_i32(int* p, int64_t n, hipStream_t s);
 void launch_synth(void* frames, int dtype, int nframes, int64_t hw, const float* mean_map,
                   float rho, uint64_t seed, hipStream_t s);
 

@@ -1,0 +1,295 @@
+// Event (sparse) correlator kernels.
+//
+// Reference: EventCorrelator (correlator/event.py:162-204) with event_correlator
+// (resources/opencl/evtcorrelator.cl:3-46): one work-item per pixel walks every ordered
+// pair j <= i of that pixel's frame-sorted events and does a global atomic
+// res[bin][t_i - t_j] += d_i d_j, plus sums[bin][t_i] += d_i.
+//
+// Here:
+//  * work items are (bin, range of the bin's pixel list) x (lag window of EW lags);
+//    a wave takes one pixel at a time, stages its events in LDS, and each lane takes
+//    events i = lane, lane+64, ...; a binary search finds the first partner j > i
+//    whose lag t_j - t_i reaches the window, and the pairs up to the window's end go
+//    to a 64-bit LDS histogram (exact for any count type), flushed once per work
+//    item with one global 64-bit atomic per non-zero lag;
+//  * S_b(t) and the self pairs (lag 0, d_i^2) are added in the first window;
+//  * the input is validated on the device (frames in [0, N), strictly increasing per
+//    pixel, as compress_all_stack produces, event.py:267-284); a violation sets a
+//    flag the host turns into an error instead of reading out of bounds.
+// Frame-sorted COO input (pixel, frame, count) is first turned into this pixel-major
+// layout on the device: per-pixel counts, an exclusive scan, a scatter through atomic
+// cursors, and a per-pixel insertion sort of the (nearly ordered) segments.
+#include "dxg2_internal.h"
+
+namespace dxg2 {
+
+constexpr int EW = 16384;       // lags per LDS window (128 KiB of 64-bit counters)
+constexpr int ECAP = 512;       // events of one pixel staged in LDS per wave
+
+template <typename T>
+__device__ __forceinline__ long long as_count(T v) { return (long long)v; }
+
+template <typename T>
+__global__ __launch_bounds__(256) void event_pairs_kernel(
+    const int32_t* __restrict__ times, const T* __restrict__ data,
+    const uint32_t* __restrict__ offsets, const int32_t* __restrict__ pix_idx,
+    const BinDev* __restrict__ bins, const EventWork* __restrict__ work, int nframes,
+    int64_t nnz, int64_t* __restrict__ num, int64_t* __restrict__ sums, int* __restrict__ err) {
+    __shared__ unsigned long long hist[EW];
+    __shared__ int st_t[4][ECAP];
+    __shared__ long long st_d[4][ECAP];
+    const EventWork w = work[blockIdx.x];
+    const BinDev bd = bins[w.bin];
+    const int w0 = blockIdx.y * EW;
+    if (w0 >= nframes) return;
+    const int w1 = min(nframes, w0 + EW);
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    for (int k = threadIdx.x; k < EW; k += 256) hist[k] = 0ull;
+    __syncthreads();
+    int64_t* srow = sums + (int64_t)w.bin * nframes;
+
+    for (int pi = w.k0 + wave; pi < w.k1; pi += 4) {
+        const int p = pix_idx[bd.idx_off + pi];
+        if (p < 0) continue;
+        const uint32_t beg = offsets[p];
+        const uint32_t end = offsets[p + 1];
+        if (end <= beg || end > (uint64_t)nnz) {
+            if ((end < beg || end > (uint64_t)nnz) && lane == 0) atomicOr(err, 2);
+            continue;
+        }
+        const int n = (int)(end - beg);
+        const bool staged = n <= ECAP;
+        const int* tt = staged ? st_t[wave] : times + beg;
+        // stage + validate
+        int bad = 0;
+        for (int k = lane; k < n; k += 64) {
+            const int t = times[beg + k];
+            if (t < 0 || t >= nframes) bad = 1;
+            if (k + 1 < n && times[beg + k + 1] <= t) bad = 1;
+            if (staged) {
+                st_t[wave][k] = t;
+                st_d[wave][k] = as_count<T>(data[beg + k]);
+            }
+        }
+        if (bad) atomicOr(err, 1);
+        if (__any(bad)) continue;
+        __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): staged values visible to the wave
+        __builtin_amdgcn_wave_barrier();
+        for (int i = lane; i < n; i += 64) {
+            const int ti = tt[i];
+            const long long di = staged ? st_d[wave][i] : as_count<T>(data[beg + i]);
+            if (w0 == 0) {
+                atomicAdd(&hist[0], (unsigned long long)(di * di));
+                if (di != 0)
+                    atomicAdd(reinterpret_cast<unsigned long long*>(srow + ti),
+                              (unsigned long long)di);
+            }
+            // first j > i with t_j - t_i >= max(w0, 1)
+            const int target = ti + (w0 > 0 ? w0 : 1);
+            int lo = i + 1, hi = n;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (tt[mid] < target) lo = mid + 1; else hi = mid;
+            }
+            for (int j = lo; j < n; ++j) {
+                const int tau = tt[j] - ti;
+                if (tau >= w1) break;
+                const long long dj = staged ? st_d[wave][j] : as_count<T>(data[beg + j]);
+                const long long prod = di * dj;
+                if (prod != 0) atomicAdd(&hist[tau - w0], (unsigned long long)prod);
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+    __syncthreads();
+    int64_t* nrow = num + (int64_t)w.bin * nframes;
+    for (int k = threadIdx.x; k < w1 - w0; k += 256) {
+        const unsigned long long v = hist[k];
+        if (v != 0ull) atomicAdd(reinterpret_cast<unsigned long long*>(nrow + w0 + k), v);
+    }
+}
+
+void launch_event_pairs(int dtype, const int32_t* times, const void* data,
+                        const uint32_t* offsets, const int32_t* pix_idx, const BinDev* bins,
+                        const EventWork* work, int n_work, int nframes, int64_t nnz,
+                        int64_t* num, int64_t* sums, int* err, hipStream_t s) {
+    if (n_work <= 0 || nframes <= 0) return;
+    dim3 grid((unsigned)n_work, (unsigned)((nframes + EW - 1) / EW));
+    if (dtype == DXG2_U16)
+        event_pairs_kernel<uint16_t><<<grid, 256, 0, s>>>(times, (const uint16_t*)data, offsets,
+                                                          pix_idx, bins, work, nframes, nnz,
+                                                          num, sums, err);
+    else if (dtype == DXG2_I8)
+        event_pairs_kernel<int8_t><<<grid, 256, 0, s>>>(times, (const int8_t*)data, offsets,
+                                                        pix_idx, bins, work, nframes, nnz, num,
+                                                        sums, err);
+    else
+        event_pairs_kernel<uint8_t><<<grid, 256, 0, s>>>(times, (const uint8_t*)data, offsets,
+                                                         pix_idx, bins, work, nframes, nnz,
+                                                         num, sums, err);
+}
+
+// ----------------------------------------------------------------- COO -> CSR
+__global__ void fill_u32_kernel(uint32_t* __restrict__ p, int64_t n, uint32_t v) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+__global__ void coo_count_kernel(const uint32_t* __restrict__ pix, int64_t nnz, uint32_t npix,
+                                 uint32_t* __restrict__ count, int* __restrict__ err) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < nnz;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t p = pix[e];
+        if (p >= npix) {
+            atomicOr(err, 4);
+            continue;
+        }
+        atomicAdd(&count[p], 1u);
+    }
+}
+
+// Exclusive scan of n counts into out[0..n] (out[n] = total), three passes over
+// blocks of 1024: per-block totals, a scan of the totals, then the block scans.
+constexpr int SCAN_B = 1024;
+
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* sh, uint32_t* total) {
+    // 256 threads x 4 values handled by the caller; here one value per thread
+    const int tid = threadIdx.x;
+    sh[tid] = v;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        const uint32_t x = tid >= off ? sh[tid - off] : 0u;
+        __syncthreads();
+        sh[tid] += x;
+        __syncthreads();
+    }
+    const uint32_t incl = sh[tid];
+    *total = sh[255];
+    __syncthreads();
+    return incl - v;
+}
+
+__global__ __launch_bounds__(256) void scan_block_totals(const uint32_t* __restrict__ in,
+                                                         int64_t n, uint32_t* __restrict__ tot) {
+    __shared__ uint32_t sh[256];
+    const int64_t base = (int64_t)blockIdx.x * SCAN_B + threadIdx.x * 4;
+    uint32_t s = 0;
+    for (int k = 0; k < 4; ++k)
+        if (base + k < n) s += in[base + k];
+    uint32_t total;
+    block_excl_scan(s, sh, &total);
+    if (threadIdx.x == 0) tot[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(256) void scan_totals(uint32_t* __restrict__ tot, int nb) {
+    __shared__ uint32_t sh[256];
+    uint32_t carry = 0;
+    for (int b0 = 0; b0 < nb; b0 += 256) {
+        const int b = b0 + threadIdx.x;
+        const uint32_t v = b < nb ? tot[b] : 0u;
+        uint32_t total;
+        const uint32_t ex = block_excl_scan(v, sh, &total);
+        if (b < nb) tot[b] = carry + ex;
+        carry += total;
+    }
+}
+
+__global__ __launch_bounds__(256) void scan_apply(const uint32_t* __restrict__ in, int64_t n,
+                                                  const uint32_t* __restrict__ tot,
+                                                  uint32_t* __restrict__ out) {
+    __shared__ uint32_t sh[256];
+    const int64_t base = (int64_t)blockIdx.x * SCAN_B + threadIdx.x * 4;
+    uint32_t v[4], s = 0;
+    for (int k = 0; k < 4; ++k) {
+        v[k] = base + k < n ? in[base + k] : 0u;
+        s += v[k];
+    }
+    uint32_t total;
+    uint32_t ex = block_excl_scan(s, sh, &total) + tot[blockIdx.x];
+    for (int k = 0; k < 4; ++k) {
+        if (base + k < n) out[base + k] = ex;
+        ex += v[k];
+    }
+    if (base <= n - 1 && n - 1 < base + 4) out[n] = ex;  // grand total
+}
+
+__global__ void copy_u32_kernel(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst,
+                                int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+template <typename T>
+__global__ void coo_scatter_kernel(const uint32_t* __restrict__ pix, const uint32_t* __restrict__ frame,
+                                   const T* __restrict__ cnt, int64_t nnz, uint32_t npix,
+                                   uint32_t* __restrict__ cursor, int32_t* __restrict__ times,
+                                   T* __restrict__ data) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < nnz;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t p = pix[e];
+        if (p >= npix) continue;
+        const uint32_t pos = atomicAdd(&cursor[p], 1u);
+        times[pos] = (int32_t)frame[e];
+        data[pos] = cnt[e];
+    }
+}
+
+// Per-pixel insertion sort of (time, value) by time; segments arrive nearly sorted.
+template <typename T>
+__global__ void segment_sort_kernel(const uint32_t* __restrict__ offsets, uint32_t npix,
+                                    int32_t* __restrict__ times, T* __restrict__ data) {
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < npix;
+         p += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t b = offsets[p], e = offsets[p + 1];
+        for (uint32_t i = b + 1; i < e; ++i) {
+            const int32_t t = times[i];
+            const T d = data[i];
+            uint32_t j = i;
+            while (j > b && times[j - 1] > t) {
+                times[j] = times[j - 1];
+                data[j] = data[j - 1];
+                --j;
+            }
+            times[j] = t;
+            data[j] = d;
+        }
+    }
+}
+
+static unsigned grid_for(int64_t n, int64_t cap = 65536) {
+    int64_t g = (n + 255) / 256;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (unsigned)g;
+}
+
+void launch_coo_to_csr(int dtype, const uint32_t* pix, const uint32_t* frame, const void* cnt,
+                       int64_t nnz, uint32_t npix, uint32_t* count, uint32_t* scan_tot,
+                       uint32_t* offsets, int32_t* times, void* data, int* err, hipStream_t s) {
+    fill_u32_kernel<<<grid_for(npix), 256, 0, s>>>(count, npix, 0u);
+    if (nnz > 0) coo_count_kernel<<<grid_for(nnz), 256, 0, s>>>(pix, nnz, npix, count, err);
+    const int nb = (int)((npix + SCAN_B - 1) / SCAN_B);
+    scan_block_totals<<<nb, 256, 0, s>>>(count, npix, scan_tot);
+    scan_totals<<<1, 256, 0, s>>>(scan_tot, nb);
+    scan_apply<<<nb, 256, 0, s>>>(count, npix, scan_tot, offsets);
+    // cursors start at the offsets (reuse the count buffer)
+    copy_u32_kernel<<<grid_for(npix), 256, 0, s>>>(offsets, count, npix);
+    if (nnz > 0) {
+        if (dtype == DXG2_U16) {
+            coo_scatter_kernel<uint16_t><<<grid_for(nnz), 256, 0, s>>>(
+                pix, frame, (const uint16_t*)cnt, nnz, npix, count, times, (uint16_t*)data);
+            segment_sort_kernel<uint16_t><<<grid_for(npix), 256, 0, s>>>(offsets, npix, times,
+                                                                         (uint16_t*)data);
+        } else {
+            coo_scatter_kernel<uint8_t><<<grid_for(nnz), 256, 0, s>>>(
+                pix, frame, (const uint8_t*)cnt, nnz, npix, count, times, (uint8_t*)data);
+            segment_sort_kernel<uint8_t><<<grid_for(npix), 256, 0, s>>>(offsets, npix, times,
+                                                                        (uint8_t*)data);
+        }
+    }
+}
+
+}  // namespace dxg2

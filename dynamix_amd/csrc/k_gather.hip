@@ -27,6 +27,19 @@ void launch_fill_i64(int64_t* p, int64_t n, hipStream_t s) {
     fill_i64_kernel<<<dim3((unsigned)blocks), dim3(256), 0, s>>>(p, n);
 }
 
+__global__ void fill_i32_kernel(int* __restrict__ p, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0;
+}
+
+void launch_fill_i32(int* p, int64_t n, hipStream_t s) {
+    if (n <= 0) return;
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    fill_i32_kernel<<<dim3((unsigned)blocks), dim3(256), 0, s>>>(p, n);
+}
+
 constexpr int GATHER_TF = 64;  // frames per workgroup tile
 
 // Broadcast lane j of every quad to the whole quad (DPP quad_perm [j, j, j, j]).

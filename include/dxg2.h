@@ -88,6 +88,35 @@ int dxg2_correlate_dense(dxg2_plan* plan, const void* d_frames, int64_t* d_num,
 int dxg2_correlate_dense_plain(dxg2_plan* plan, const void* d_frames, float* d_g2,
                                void* stream);
 
+/* Event (sparse) path, pixel-major CSR exactly as the reference's EventCorrelator takes
+ * it: d_times int32[nnz] frame indices, strictly increasing within a pixel; d_data
+ * [nnz] counts of the plan dtype; d_offsets uint32[H*W+1], pixel p owns events
+ * [offsets[p], offsets[p+1]).  Numerators are exact (64-bit), sums/denominators as for
+ * the dense path, g2 uses the plan's scale factors (EventCorrelator honours
+ * scale_factor, evtcorrelator.cl:74).  Input errors are recorded on the device and
+ * reported by dxg2_plan_event_status.  Numerators are recomputed from zero on every
+ * call (the reference keeps accumulating res_int across calls, event.py:98,170-171).
+ * Replaces EventCorrelator.correlate (correlator/event.py:162-204) with
+ * event_correlator/normalize_correlation (resources/opencl/evtcorrelator.cl:3-76). */
+int dxg2_correlate_events_csr(dxg2_plan* plan, const int32_t* d_times, const void* d_data,
+                              const uint32_t* d_offsets, int64_t nnz, int64_t* d_num,
+                              int64_t* d_sums, double* d_den, float* d_g2, void* stream);
+
+/* Event path from frame-sorted COO: d_pixel uint32 raster pixel index, d_frame uint32,
+ * d_count uint8 (or uint16 for a DXG2_U16 plan), nnz events.  Converted to the CSR
+ * layout above on the device (count, scan, scatter, per-pixel sort), then as above.
+ * The layout FramesCompressor.compress_all_stack produces (correlator/event.py:267-284)
+ * is the CSR form; this COO form has no reference counterpart. */
+int dxg2_correlate_events_coo(dxg2_plan* plan, const uint32_t* d_pixel, const uint32_t* d_frame,
+                              const void* d_count, int64_t nnz, int64_t* d_num, int64_t* d_sums,
+                              double* d_den, float* d_g2, void* stream);
+
+/* Waits for `stream` and returns the input-error flags of the last event call: bit 0
+ * frame out of [0, nframes) or not strictly increasing within a pixel, bit 1 offsets
+ * decreasing or past nnz, bit 2 COO pixel index past H*W.  0 = input valid.  The
+ * reference raises AssertionError/ValueError on bad shapes (event.py:104-115). */
+int dxg2_plan_event_status(dxg2_plan* plan, int32_t* flags, void* stream);
+
 /* Stage timing with HIP events recorded on the call's stream (no host sync per call).
  * dxg2_plan_stage_times waits for the last recorded call and returns the averages over
  * every call since profiling was switched on or last read, in ms: times[0] gather,

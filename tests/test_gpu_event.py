@@ -1,0 +1,123 @@
+"""GPU parity of the event (sparse) HIP path against the CPU oracle and the dense path."""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import events as ev
+from oracle import g2 as og
+
+pytestmark = pytest.mark.gpu
+
+
+def _announce(msg):
+    print("[case] " + msg, flush=True)
+    sys.stdout.flush()
+
+
+def _sparse_stack(seed, n, shape, p=0.05, hi=4, dtype=np.uint8):
+    from dynamix_amd.synth import speckle_frames
+    f = speckle_frames(shape, n, p, 15.0, seed, dtype)
+    return f
+
+
+def _check(c, g2, ref):
+    np.testing.assert_array_equal(c.last_numerators.cpu().numpy(), ref["num"])
+    np.testing.assert_array_equal(c.last_sums.cpu().numpy(), ref["sums"])
+    np.testing.assert_array_equal(c.last_denominators.cpu().numpy(),
+                                  np.asarray(ref["den"], dtype=np.float64))
+    g = ref["g2"]
+    fin = np.isfinite(g)
+    assert np.array_equal(fin, np.isfinite(g2))
+    nz = fin & (g != 0)
+    assert (g2[fin & (g == 0)] == 0).all()
+    rel = np.abs(g2[nz] - g[nz]) / np.abs(g[nz])
+    assert rel.max(initial=0) <= 1e-6
+
+
+@pytest.mark.parametrize("n,shape,mean,nbins", [
+    (50, (6, 7), 0.3, 3),
+    (400, (20, 25), 0.05, 5),
+    (17000, (4, 5), 0.02, 2),      # more frames than one 16384-lag window
+])
+def test_events_csr_vs_oracle_and_dense(n, shape, mean, nbins):
+    _announce("events csr n=%d shape=%s" % (n, shape))
+    from dynamix_amd.correlator.event import EventCorrelator
+    f = _sparse_stack(n, n, shape, mean)
+    q = np.random.default_rng(n).integers(0, nbins + 1, size=shape).astype(np.int32)
+    for b in range(1, nbins + 1):
+        q.flat[b] = b
+    data, times, offsets = ev.compress_all_stack(f)
+    c = EventCorrelator(shape, n, max_events_count=int(np.diff(offsets).max()), qmask=q,
+                        dtype=np.uint8)
+    g2 = c.correlate(times, data, offsets)
+    ref = ev.event_g2(times, data, offsets, q, n)
+    _check(c, g2, ref)
+    dense = og.dense_g2(f, q)
+    np.testing.assert_array_equal(c.last_numerators.cpu().numpy(), dense["num"])
+
+
+def test_events_coo_matches_csr():
+    _announce("events coo")
+    from dynamix_amd.correlator.event import EventCorrelator
+    f = _sparse_stack(3, 300, (12, 14), 0.1)
+    q = np.random.default_rng(3).integers(0, 4, size=(12, 14)).astype(np.int32)
+    data, times, offsets = ev.compress_all_stack(f)
+    pix, fr, cnt = ev.csr_to_coo(times, data, offsets)
+    c = EventCorrelator((12, 14), 300, 10, qmask=q, dtype=np.uint8)
+    g2 = c.correlate_coo(pix, fr, cnt.astype(np.uint8))
+    _check(c, g2, ev.event_g2(times, data, offsets, q, 300))
+
+
+def test_events_scale_factor_and_int8_and_goldens(golden):
+    _announce("events int8 + scale")
+    from dynamix_amd.correlator.event import EventCorrelator
+    for name in ["evt_u8", "evt_i8"]:
+        f = golden["%s_frames" % name]
+        data, times, offsets = golden["%s_data" % name], golden["%s_times" % name], \
+            golden["%s_offsets" % name]
+        n = f.shape[0]
+        c = EventCorrelator(f.shape[1:], n, 10, dtype=f.dtype, scale_factor=1234.0)
+        g2 = c.correlate(times, data, offsets)
+        ref = ev.event_g2(times, data, offsets, np.ones(f.shape[1:], np.int32), n,
+                          scale_factor=1234.0)
+        _check(c, g2, ref)
+
+
+def test_events_repeat_calls_do_not_accumulate():
+    _announce("events repeat")
+    from dynamix_amd.correlator.event import EventCorrelator
+    f = _sparse_stack(5, 120, (8, 8), 0.2)
+    data, times, offsets = ev.compress_all_stack(f)
+    c = EventCorrelator((8, 8), 120, 20, dtype=np.uint8)
+    a = c.correlate(times, data, offsets)
+    n1 = c.last_numerators.clone()
+    b = c.correlate(times, data, offsets)
+    assert (n1 == c.last_numerators).all().item()
+    np.testing.assert_array_equal(a, b)
+
+
+def test_events_invalid_input_raises():
+    _announce("events invalid input")
+    from dynamix_amd.correlator.event import EventCorrelator
+    f = _sparse_stack(6, 60, (5, 5), 0.3)
+    data, times, offsets = ev.compress_all_stack(f)
+    c = EventCorrelator((5, 5), 60, 20, dtype=np.uint8, allow_reallocate=False,
+                        total_events_count=data.size)
+    bad = times.copy()
+    k = int(offsets[1])  # first event of pixel 1: make pixel 0's last frame repeat
+    if k >= 2:
+        bad[k - 1] = bad[k - 2]
+        with pytest.raises(ValueError, match="increasing"):
+            c.correlate(bad, data, offsets)
+    bad = times.copy()
+    bad[0] = 60          # frame index out of range
+    with pytest.raises(ValueError):
+        c.correlate(bad, data, offsets)
+    with pytest.raises(ValueError, match="allow_reallocate"):
+        c.correlate(np.concatenate([times, times]), np.concatenate([data, data]), offsets)
+    with pytest.raises(AssertionError):
+        c.correlate(times, data, offsets[:-1])
+    # a valid call after the errors is exact again
+    g2 = c.correlate(times, data, offsets)
+    _check(c, g2, ev.event_g2(times, data, offsets, np.ones((5, 5), np.int32), 60))
