@@ -50,6 +50,7 @@ _SIGNATURES = {
     "dxg2_plan_footprint": (c_int, [c_void_p, c_int, ctypes.POINTER(c_size_t)]),
     "dxg2_correlate_dense": (c_int, [c_void_p] * 7),
     "dxg2_correlate_dense_plain": (c_int, [c_void_p] * 4),
+    "dxg2_correlate_fft": (c_int, [c_void_p] * 7),
     "dxg2_correlate_events_csr": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_i64,
                                           c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "dxg2_correlate_events_coo": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_i64,

@@ -7,6 +7,10 @@ DenseCorrelator   -- dense.py:145-247.  Same constructor (OpenCL device-selectio
                      arguments are accepted and ignored); same HIP path.
 PlainMatMulCorrelator -- the one-bin-per-launch FP32 form of
                      CublasMatMulCorrelator (cuda.py:19-125), kept as the comparator.
+FFTCorrelator     -- dense.py:275-369 (FFTWCorrelator) and cuda.py:129-219
+                     (CUFFTCorrelator): numerators from FP32 power spectra along the
+                     time axis (FFTWCorrelator is the same class under the reference's
+                     name).
 
 Results: float32 (n_bins, nframes), tau = 0 in column 0, NaN where the denominator is
 zero (an empty bin, or frames with no counts).  g2 = n_b * Num / Den with exact integer
@@ -139,3 +143,58 @@ class PlainMatMulCorrelator(_DenseHIP):
         if self.extra_options.get("return_device", False):
             return g2
         return g2.cpu().numpy()
+
+
+def fft_length(nframes):
+    """Nf = 2 L, L the smallest power of two >= max(nframes, 1024); Nf >= 2 N - 1 is all the
+    linear autocorrelation needs (the reference uses get_next_power(2N), utils.py:50-60)."""
+    L = 1024
+    while L < nframes:
+        L *= 2
+    return 2 * L
+
+
+class FFTCorrelator(_DenseHIP):
+    """Drop-in for FFTWCorrelator / CUFFTCorrelator (dense.py:275-369, cuda.py:129-219).
+
+    uint8 frames, nframes <= 16384.  `weights` and `precompute_fft_plans` are accepted;
+    weights are ignored as in the reference FFT correlators (dense.py:277-285), and the
+    FFT needs no plan.  scale_factor is ignored (the reference divides by the pixel count,
+    dense.py:366).  Numerators carry FP32 FFT rounding: |Num - Num_exact| <= 2e-7 * Num(0)
+    (asserted by the tests; measured <= 4.8e-8 for N up to 10,000); the denominator is
+    exact.
+    After each call `last_numerators` holds the FP64 numerators.
+    """
+
+    def __init__(self, shape, nframes, qmask=None, weights=None, scale_factor=None,
+                 precompute_fft_plans=False, extra_options={}):
+        super().__init__()
+        super()._set_parameters(shape, nframes, qmask, scale_factor, extra_options)
+        self._init_dense()
+        self.precompute_fft_plans = precompute_fft_plans
+        self.Nf = fft_length(self.nframes)
+        self.n_pix = {int(b): int(np.sum(self.qmask == b)) for b in self.bins}
+
+    def correlate(self, frames):
+        torch = _torch()
+        d_frames, dtype = to_device_frames(frames, self.shape, self.nframes, torch)
+        if dtype != np.dtype(np.uint8):
+            raise NotImplementedError("the FFT path takes uint8 frames, got %s" % dtype)
+        plan = self._plan(dtype)
+        nb, n = max(self.n_bins, 0), self.nframes
+        num = torch.empty((nb, n), dtype=torch.float64, device="cuda")
+        sums = torch.empty((nb, n), dtype=torch.int64, device="cuda")
+        den = torch.empty((nb, n), dtype=torch.float64, device="cuda")
+        g2 = torch.empty((nb, n), dtype=torch.float32, device="cuda")
+        if nb:
+            _lib.check(plan.lib.dxg2_correlate_fft(plan.handle, _lib.ptr(d_frames), _lib.ptr(num),
+                                                   _lib.ptr(sums), _lib.ptr(den), _lib.ptr(g2),
+                                                   current_stream(torch)),
+                       "dxg2_correlate_fft")
+        self.last_numerators, self.last_sums, self.last_denominators, self.last_g2 = num, sums, den, g2
+        if self.extra_options.get("return_device", False):
+            return g2
+        return g2.cpu().numpy()
+
+
+FFTWCorrelator = FFTCorrelator

@@ -107,6 +107,15 @@ struct dxg2_plan {
     void* d_edata = nullptr;       // CSR counts built from COO input
     int64_t ev_cap = 0;            // capacity of d_etimes / d_edata in events
 
+    // FFT path
+    std::vector<KbWork> kb_work;   // every k-block of every bin
+    KbWork* d_kbwork = nullptr;
+    uint8_t* d_xs = nullptr;       // pixel-major bytes, same size as the slab
+    float2* d_tw = nullptr;        // exp(-2 pi i m / L)
+    double* d_power = nullptr;     // (n_bins, L + 1)
+    double* d_fnum = nullptr;      // (n_bins, N)
+    int fft_L = 0;
+
     // Profiling: each correlate call records 4 events on its stream (no host sync);
     // dxg2_plan_stage_times() waits for the last one and averages every recorded call.
     bool profiling = false;
@@ -167,6 +176,47 @@ int next_events(dxg2_plan* p, hipEvent_t** out) {
 
 // Pixels of one bin's list handled by one event work item.
 constexpr int EVENT_ITEM_PIXELS = 1024;
+
+// FFT length: Nf = 2L with L the smallest power of two >= max(N, 1024) (any Nf >= 2N - 1
+// gives the linear autocorrelation); the LDS holds L complex points, so L <= 16384.
+constexpr int FFT_MIN_L = 1024;
+constexpr int FFT_MAX_L = 16384;
+
+int fft_length(int n) {
+    int L = FFT_MIN_L;
+    while (L < n) L *= 2;
+    return L;
+}
+
+int ensure_fft_buffers(dxg2_plan* p) {
+    if (p->d_power) return DXG2_OK;
+    const int L = fft_length(p->N);
+    if (L > FFT_MAX_L)
+        return set_error(DXG2_EUNSUPPORTED, "FFT path supports up to %d frames (got %d)",
+                         FFT_MAX_L, p->N);
+    p->fft_L = L;
+    for (int b = 0; b < p->n_bins; ++b)
+        for (int kb = 0; kb < p->bins[b].nkb; ++kb) p->kb_work.push_back(KbWork{b, kb});
+    std::vector<float2> tw(L);
+    for (int m = 0; m < L; ++m) {
+        const double a = -2.0 * M_PI * (double)m / (double)L;
+        tw[m] = make_float2((float)std::cos(a), (float)std::sin(a));
+    }
+    if (hipMalloc(&p->d_xs, std::max<size_t>(p->slab_bytes, 1)) != hipSuccess ||
+        hipMalloc(&p->d_tw, (size_t)L * sizeof(float2)) != hipSuccess ||
+        hipMalloc(&p->d_power, (size_t)std::max(p->n_bins, 1) * (L + 1) * sizeof(double)) !=
+            hipSuccess ||
+        hipMalloc(&p->d_fnum, (size_t)std::max(p->n_bins, 1) * p->N * sizeof(double)) !=
+            hipSuccess ||
+        (!p->kb_work.empty() &&
+         hipMalloc(&p->d_kbwork, p->kb_work.size() * sizeof(KbWork)) != hipSuccess))
+        return set_error(DXG2_ENOMEM, "cannot allocate FFT workspace");
+    DXG2_HIP(hipMemcpy(p->d_tw, tw.data(), (size_t)L * sizeof(float2), hipMemcpyHostToDevice));
+    if (!p->kb_work.empty())
+        DXG2_HIP(hipMemcpy(p->d_kbwork, p->kb_work.data(), p->kb_work.size() * sizeof(KbWork),
+                           hipMemcpyHostToDevice));
+    return DXG2_OK;
+}
 
 void build_event_work(dxg2_plan* p) {
     for (int b = 0; b < p->n_bins; ++b)
@@ -335,7 +385,8 @@ int dxg2_plan_destroy(dxg2_plan* p) {
     void* bufs[] = {p->d_pix,   p->d_bins, p->d_gwork, p->d_mwork, p->d_slab,
                     p->d_ugram, p->d_sums, p->d_pa,    p->d_pg,    p->d_pm,
                     p->d_ework, p->d_err,  p->d_cnt,   p->d_scan,  p->d_off,
-                    p->d_etimes, p->d_edata};
+                    p->d_etimes, p->d_edata, p->d_kbwork, p->d_xs, p->d_tw,
+                    p->d_power, p->d_fnum};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     for (auto& q : p->ev_pool)
@@ -368,6 +419,12 @@ int dxg2_plan_footprint(const dxg2_plan* p, int32_t path, size_t* bytes) {
             int64_t mx = 0;
             for (auto n : p->npix) mx = std::max(mx, n);
             *bytes = common + ((size_t)p->N * ((mx + 7) / 8 * 8) + (size_t)p->N * p->N + p->N) * 4;
+            return DXG2_OK;
+        }
+        case 2: {
+            const int L = fft_length(p->N);
+            *bytes = common + p->slab_bytes + (size_t)L * sizeof(float2) +
+                     (size_t)std::max(p->n_bins, 1) * ((L + 1) + p->N) * sizeof(double);
             return DXG2_OK;
         }
         case 3: {
@@ -539,6 +596,37 @@ int dxg2_correlate_events_coo(dxg2_plan* p, const uint32_t* d_pixel, const uint3
                       p->d_cnt, p->d_scan, p->d_off, p->d_etimes, p->d_edata, p->d_err, s);
     return events_from_csr(p, p->d_etimes, p->d_edata, p->d_off, nnz, d_num, d_sums, d_den,
                            d_g2, s);
+}
+
+int dxg2_correlate_fft(dxg2_plan* p, const void* d_frames, double* d_num, int64_t* d_sums,
+                       double* d_den, float* d_g2, void* stream) {
+    if (!p) return set_error(DXG2_EINVAL, "null plan");
+    if (!d_frames || !d_g2) return set_error(DXG2_EINVAL, "frames and g2 must be non-null");
+    if (p->dtype != DXG2_U8)
+        return set_error(DXG2_EUNSUPPORTED, "FFT path takes uint8 frames");
+    if (p->n_bins == 0) return DXG2_OK;
+    int rc = ensure_fft_buffers(p);
+    if (rc) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    int64_t* sums = d_sums ? d_sums : p->d_sums;
+    (void)hipGetLastError();
+    hipEvent_t* ev = nullptr;
+    if (p->profiling && (rc = next_events(p, &ev))) return rc;
+    if (ev) DXG2_HIP(hipEventRecord(ev[0], s));
+    launch_fill_i64(reinterpret_cast<int64_t*>(p->d_power), (int64_t)p->n_bins * (p->fft_L + 1), s);
+    launch_gather_pm((const uint8_t*)d_frames, p->dtype, (int64_t)p->H * p->W, p->N, p->nrows,
+                     p->d_pix, p->d_bins, p->d_kbwork, (int)p->kb_work.size(), p->d_xs, s);
+    launch_pm_sums(p->d_xs, p->d_bins, p->n_bins, p->N, p->nrows, sums, s);
+    if (ev) DXG2_HIP(hipEventRecord(ev[1], s));
+    if (launch_fft_power(p->fft_L, p->d_xs, p->d_bins, p->d_kbwork, (int)p->kb_work.size(), p->N,
+                         p->nrows, p->d_tw, p->d_power, s))
+        return set_error(DXG2_EUNSUPPORTED, "no FFT kernel for length %d", p->fft_L);
+    launch_fft_inverse(p->d_power, p->fft_L, p->n_bins, p->N, p->d_fnum, s);
+    if (ev) DXG2_HIP(hipEventRecord(ev[2], s));
+    launch_finalize_f64(sums, p->d_fnum, p->d_bins, p->n_bins, p->N, d_num, d_den, d_g2, s);
+    if (ev) DXG2_HIP(hipEventRecord(ev[3], s));
+    DXG2_HIP(hipGetLastError());
+    return DXG2_OK;
 }
 
 int dxg2_plan_event_status(dxg2_plan* p, int32_t* flags, void* stream) {

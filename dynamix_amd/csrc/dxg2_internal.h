@@ -52,6 +52,11 @@ struct EventWork {       // one workgroup of the event pair kernel (x dimension)
     int32_t pad;
 };
 
+struct KbWork {          // one k-block (64 pixels) of one bin
+    int32_t bin;
+    int32_t kb;
+};
+
 // ------------------------------------------------------------------ launchers
 void launch_fill_i64(int64_t* p, int64_t n, hipStream_t s);
 void launch_gather_i8(const uint8_t* frames, int dtype, int64_t hw, int nframes, int nrows,
@@ -74,6 +79,18 @@ void launch_coo_to_csr(int dtype, const uint32_t* pix, const uint32_t* frame, co
                        int64_t nnz, uint32_t npix, uint32_t* count, uint32_t* scan_tot,
                        uint32_t* offsets, int32_t* times, void* data, int* err, hipStream_t s);
 void launch_fill_i32(int* p, int64_t n, hipStream_t s);
+void launch_finalize_f64(const int64_t* sums, const double* fnum, const BinDev* bins, int n_bins,
+                         int nframes, double* num, double* den, float* g2, hipStream_t s);
+void launch_gather_pm(const uint8_t* frames, int dtype, int64_t hw, int nframes, int nrows,
+                      const int32_t* pix_idx, const BinDev* bins, const KbWork* work, int n_work,
+                      uint8_t* xs, hipStream_t s);
+void launch_pm_sums(const uint8_t* xs, const BinDev* bins, int n_bins, int nframes, int nrows,
+                    int64_t* sums, hipStream_t s);
+int launch_fft_power(int L, const uint8_t* xs, const BinDev* bins, const KbWork* work,
+                     int n_work, int nframes, int nrows, const float2* tw, double* power,
+                     hipStream_t s);
+void launch_fft_inverse(const double* power, int L, int n_bins, int nframes, double* num,
+                        hipStream_t s);
 void launch_synth(void* frames, int dtype, int nframes, int64_t hw, const float* mean_map,
                   float rho, uint64_t seed, hipStream_t s);
 

@@ -1,0 +1,298 @@
+// FFT correlator: numerator from power spectra along the time axis.
+//
+// Reference: FFTWCorrelator._correlate_fft (correlator/dense.py:347-369) and
+// CUFFTCorrelator (cuda.py:185-219): per bin, a float32 FFT of every pixel series
+// zero-padded to Nf >= 2N (dense.py:289), times the FFT of the reversed series, one
+// inverse FFT per pixel, a sum over pixels; the denominator is np.correlate of the
+// integer frame sums (dense.py:365-366).
+//
+// Here, per bin:
+//   Num(tau) = IFFT_Nf( sum_p |X_p(f)|^2 )[tau],   X_p = FFT_Nf(x_p zero-padded),
+// (the product with the reversed series is the same autocorrelation; the inverse is
+// linear, so one inverse per bin replaces one per pixel).
+//  * gather_pm_kernel: frames -> pixel-major bytes xs[k][t] (k in the bin's raster
+//    list order, t < nrows, zero past the last frame), 64 pixels x 256 frames per
+//    workgroup through an LDS transpose.
+//  * pm_sums_kernel: exact S_b(t) from xs.
+//  * fft_power_kernel: one workgroup per 64 pixels of a bin; each pixel's series is
+//    packed as z[n] = x[2n] + i x[2n+1] (L = Nf/2 complex points), transformed by an
+//    FP32 Stockham FFT in LDS (Govindaraju et al., "High performance discrete Fourier
+//    transforms on graphics processors", SC 2008: radix-4 passes, one radix-2 pass when
+//    log2 L is odd), unpacked to X[k], k = 0..L, and |X[k]|^2 is accumulated in FP64.
+//  * fft_inverse_kernel: Num(tau) = (P_0 + (-1)^tau P_L + 2 sum_{k=1}^{L-1} P_k
+//    cos(2 pi k tau / Nf)) / Nf in FP64, the cosine by an FP64 rotation recurrence.
+// Twiddle tables are computed on the host in FP64 and rounded to FP32.
+#include "dxg2_internal.h"
+
+namespace dxg2 {
+
+// threads per FFT workgroup: L/4 butterflies per radix-4 pass, at most 1024 threads
+template <int L>
+struct FftCfg {
+    static constexpr int THREADS = L / 4 < 1024 ? L / 4 : 1024;
+};
+
+// --------------------------------------------------------------- pixel-major gather
+// grid (bin k-blocks, frame tiles of 256).  Lane = pixel j of the k-block; each wave
+// stages 64 frames, the LDS tile is [64 pixels][256 frames + 4].
+__global__ __launch_bounds__(256) void gather_pm_kernel(const uint8_t* __restrict__ frames,
+                                                        int dtype, int64_t hw, int nframes,
+                                                        int nrows, const int32_t* __restrict__ pix_idx,
+                                                        const BinDev* __restrict__ bins,
+                                                        const KbWork* __restrict__ work,
+                                                        uint8_t* __restrict__ xs) {
+    __shared__ uint8_t tile[64][256 + 4];
+    const KbWork w = work[blockIdx.x];
+    const BinDev bd = bins[w.bin];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int t0 = blockIdx.y * 256;
+    const int p = pix_idx[bd.idx_off + (int64_t)w.kb * KB + lane];
+    const int pc = p >= 0 ? p : 0;
+    for (int f = wave; f < 256; f += 4) {
+        const int t = t0 + f;
+        uint8_t v = 0;
+        if (t < nframes && p >= 0) {
+            if (dtype == DXG2_U16) {
+                const uint16_t x = reinterpret_cast<const uint16_t*>(frames)[(int64_t)t * hw + pc];
+                v = x > 255 ? 255 : (uint8_t)x;   // not reached: the FFT path takes 8-bit input
+            } else {
+                v = frames[(int64_t)t * hw + pc];
+            }
+        }
+        tile[lane][f] = v;
+    }
+    __syncthreads();
+    // write: pixel row j gets 256 contiguous bytes; 64 threads per 1 KiB? -> thread
+    // writes 16 bytes: pixel j = tid / 16, bytes (tid % 16) * 16 .. +16 of the tile row
+    for (int c = threadIdx.x; c < 64 * 16; c += 256) {
+        const int j = c >> 4;
+        const int o = (c & 15) * 16;
+        uint32_t wd[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            wd[q] = (uint32_t)tile[j][o + 4 * q] | ((uint32_t)tile[j][o + 4 * q + 1] << 8) |
+                    ((uint32_t)tile[j][o + 4 * q + 2] << 16) |
+                    ((uint32_t)tile[j][o + 4 * q + 3] << 24);
+        uint8_t* dst = xs + bd.slab_off + ((int64_t)w.kb * KB + j) * nrows + t0 + o;
+        *reinterpret_cast<uint4*>(dst) = make_uint4(wd[0], wd[1], wd[2], wd[3]);
+    }
+}
+
+// S_b(t) = sum over the bin's pixels of xs[k][t]; grid (nrows / 256, n_bins).
+__global__ __launch_bounds__(256) void pm_sums_kernel(const uint8_t* __restrict__ xs,
+                                                      const BinDev* __restrict__ bins, int nframes,
+                                                      int nrows, int64_t* __restrict__ sums) {
+    const int b = blockIdx.y;
+    const BinDev bd = bins[b];
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= nframes) return;
+    const uint8_t* base = xs + bd.slab_off + t;
+    int64_t s = 0;
+    for (int k = 0; k < bd.npix; ++k) s += base[(int64_t)k * nrows];
+    sums[(int64_t)b * nframes + t] = s;
+}
+
+// --------------------------------------------------------------------- forward FFT
+struct c32 {
+    float x, y;
+};
+__device__ __forceinline__ c32 cmul(c32 a, c32 b) {
+    return c32{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+__device__ __forceinline__ c32 cadd(c32 a, c32 b) { return c32{a.x + b.x, a.y + b.y}; }
+__device__ __forceinline__ c32 csub(c32 a, c32 b) { return c32{a.x - b.x, a.y - b.y}; }
+// -i * a
+__device__ __forceinline__ c32 cmuli_neg(c32 a) { return c32{a.y, -a.x}; }
+
+// grid: one workgroup per (bin, k-block) of 64 pixels.  LDS holds L complex points.
+// tw[m] = exp(-2 pi i m / L), m < L (host FP64, rounded); the unpacking factor
+// exp(-pi i k / L) comes from sincospif.
+template <int L>
+__global__ __launch_bounds__(FftCfg<L>::THREADS) void fft_power_kernel(
+    const uint8_t* __restrict__ xs, const BinDev* __restrict__ bins,
+    const KbWork* __restrict__ work, int nframes, int nrows, const float2* __restrict__ tw,
+    double* __restrict__ power) {
+    __shared__ c32 z[L];
+    constexpr int FFT_THREADS = FftCfg<L>::THREADS;
+    constexpr int LOG2 = __builtin_ctz(L);
+    constexpr int PER = L / FFT_THREADS;          // complex points per thread (load)
+    constexpr int BF4 = L / 4 / FFT_THREADS;      // radix-4 butterflies per thread per pass
+    constexpr int NK = (L + 1 + FFT_THREADS - 1) / FFT_THREADS;  // power bins per thread
+    const KbWork w = work[blockIdx.x];
+    const BinDev bd = bins[w.bin];
+    const int tid = threadIdx.x;
+    double acc[NK];
+#pragma unroll
+    for (int m = 0; m < NK; ++m) acc[m] = 0.0;
+
+    const int kend = min(KB, bd.npix - w.kb * KB);
+    for (int j = 0; j < kend; ++j) {
+        const uint8_t* x = xs + bd.slab_off + ((int64_t)w.kb * KB + j) * nrows;
+        // load z[n] = x[2n] + i x[2n+1] (zero past the last frame)
+#pragma unroll
+        for (int m = 0; m < PER; ++m) {
+            const int n = tid + m * FFT_THREADS;
+            const int t = 2 * n;
+            float re = 0.f, im = 0.f;
+            if (t + 1 < nrows) {
+                const uint16_t two = *reinterpret_cast<const uint16_t*>(x + t);
+                re = t < nframes ? (float)(two & 0xFF) : 0.f;
+                im = t + 1 < nframes ? (float)(two >> 8) : 0.f;
+            }
+            z[n] = c32{re, im};
+        }
+        __syncthreads();
+        int ns = 1;
+        if (LOG2 & 1) {  // one radix-2 pass, Ns = 1
+            c32 v[PER / 2][2];
+#pragma unroll
+            for (int m = 0; m < PER / 2; ++m) {
+                const int jj = tid + m * FFT_THREADS;
+                v[m][0] = z[jj];
+                v[m][1] = z[jj + L / 2];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int m = 0; m < PER / 2; ++m) {
+                const int jj = tid + m * FFT_THREADS;
+                z[2 * jj] = cadd(v[m][0], v[m][1]);
+                z[2 * jj + 1] = csub(v[m][0], v[m][1]);
+            }
+            __syncthreads();
+            ns = 2;
+        }
+        // kept rolled: unrolled, the twiddle loads of every pass would be hoisted above
+        // the barriers and overflow the register file
+#pragma unroll 1
+        for (; ns < L; ns *= 4) {
+            c32 v[BF4][4];
+            const int tstep = L / (4 * ns);   // twiddle index step: exp(-2 pi i r k / (4 ns))
+#pragma unroll
+            for (int m = 0; m < BF4; ++m) {
+                const int jj = tid + m * FFT_THREADS;
+                const int k = jj % ns;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    c32 a = z[jj + r * (L / 4)];
+                    if (r) {
+                        const float2 t = tw[(r * k * tstep) & (L - 1)];
+                        a = cmul(a, c32{t.x, t.y});
+                    }
+                    v[m][r] = a;
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int m = 0; m < BF4; ++m) {
+                const int jj = tid + m * FFT_THREADS;
+                const int k = jj % ns;
+                const c32 s02 = cadd(v[m][0], v[m][2]), d02 = csub(v[m][0], v[m][2]);
+                const c32 s13 = cadd(v[m][1], v[m][3]), d13 = cmuli_neg(csub(v[m][1], v[m][3]));
+                const int o = (jj / ns) * ns * 4 + k;
+                z[o] = cadd(s02, s13);
+                z[o + ns] = cadd(d02, d13);
+                z[o + 2 * ns] = csub(s02, s13);
+                z[o + 3 * ns] = csub(d02, d13);
+            }
+            __syncthreads();
+        }
+        // unpack X[k] = (Z[k] + conj Z[L-k]) / 2 - i W^k (Z[k] - conj Z[L-k]) / 2,
+        // W = exp(-pi i / L); accumulate |X[k]|^2
+#pragma unroll
+        for (int m = 0; m < NK; ++m) {
+            const int k = tid + m * FFT_THREADS;
+            if (k <= L) {
+                const c32 a = z[k & (L - 1)];
+                const c32 b = z[(L - k) & (L - 1)];
+                const c32 s = c32{a.x + b.x, a.y - b.y};   // Z[k] + conj Z[L-k]
+                const c32 d = c32{a.x - b.x, a.y + b.y};   // Z[k] - conj Z[L-k]
+                float sn, cs;
+                sincospif((float)k / (float)L, &sn, &cs);   // W^k = cos - i sin (pi k / L)
+                const c32 wd = cmul(c32{cs, -sn}, d);
+                const float xr = 0.5f * (s.x + wd.y);      // s/2 - i wd/2
+                const float xi = 0.5f * (s.y - wd.x);
+                acc[m] += (double)xr * xr + (double)xi * xi;
+            }
+        }
+        __syncthreads();
+    }
+    double* prow = power + (int64_t)w.bin * (L + 1);
+#pragma unroll
+    for (int m = 0; m < NK; ++m) {
+        const int k = tid + m * FFT_THREADS;
+        if (k <= L && acc[m] != 0.0) atomicAdd(prow + k, acc[m]);
+    }
+}
+
+// Num(tau) from the power spectrum, FP64; grid (ceil(N / 256), n_bins).  Every
+// thread reads the same ps[k] in step, so the spectrum is read as uniform loads.
+__global__ __launch_bounds__(256) void fft_inverse_kernel(const double* __restrict__ power,
+                                                          int L, int nframes,
+                                                          double* __restrict__ num) {
+    const int b = blockIdx.y;
+    const double* ps = power + (int64_t)b * (L + 1);
+    const int tau = blockIdx.x * 256 + threadIdx.x;
+    if (tau >= nframes) return;
+    const double nf = 2.0 * L;
+    double sn, cs;
+    sincospi(2.0 * tau / nf, &sn, &cs);      // rotation by 2 pi tau / Nf
+    double c = cs, s = sn;                   // cos, sin of 2 pi k tau / Nf at k = 1
+    double acc = 0.0;
+    for (int k = 1; k < L; ++k) {
+        acc += ps[k] * c;
+        const double cn = c * cs - s * sn;
+        s = s * cs + c * sn;
+        c = cn;
+        if ((k & 1023) == 0) {               // re-anchor the recurrence
+            sincospi(2.0 * (double)(((int64_t)(k + 1) * tau) % (int64_t)(2 * L)) / nf, &s, &c);
+        }
+    }
+    const double last = (tau & 1) ? -ps[L] : ps[L];
+    num[(int64_t)b * nframes + tau] = (ps[0] + last + 2.0 * acc) / nf;
+}
+
+void launch_gather_pm(const uint8_t* frames, int dtype, int64_t hw, int nframes, int nrows,
+                      const int32_t* pix_idx, const BinDev* bins, const KbWork* work, int n_work,
+                      uint8_t* xs, hipStream_t s) {
+    if (n_work <= 0) return;
+    gather_pm_kernel<<<dim3((unsigned)n_work, (unsigned)(nrows / 256)), 256, 0, s>>>(
+        frames, dtype, hw, nframes, nrows, pix_idx, bins, work, xs);
+}
+
+void launch_pm_sums(const uint8_t* xs, const BinDev* bins, int n_bins, int nframes, int nrows,
+                    int64_t* sums, hipStream_t s) {
+    if (n_bins <= 0) return;
+    pm_sums_kernel<<<dim3((unsigned)((nframes + 255) / 256), (unsigned)n_bins), 256, 0, s>>>(
+        xs, bins, nframes, nrows, sums);
+}
+
+int launch_fft_power(int L, const uint8_t* xs, const BinDev* bins, const KbWork* work,
+                     int n_work, int nframes, int nrows, const float2* tw, double* power,
+                     hipStream_t s) {
+    if (n_work <= 0) return 0;
+#define DXG2_FFT_CASE(LL)                                                                     \
+    case LL:                                                                                  \
+        fft_power_kernel<LL><<<dim3((unsigned)n_work), FftCfg<LL>::THREADS, 0, s>>>(          \
+            xs, bins, work, nframes, nrows, tw, power);                                       \
+        return 0;
+    switch (L) {
+        DXG2_FFT_CASE(1024)
+        DXG2_FFT_CASE(2048)
+        DXG2_FFT_CASE(4096)
+        DXG2_FFT_CASE(8192)
+        DXG2_FFT_CASE(16384)
+        default:
+            return -1;
+    }
+#undef DXG2_FFT_CASE
+}
+
+void launch_fft_inverse(const double* power, int L, int n_bins, int nframes, double* num,
+                        hipStream_t s) {
+    if (n_bins <= 0) return;
+    fft_inverse_kernel<<<dim3((unsigned)((nframes + 255) / 256), (unsigned)n_bins), 256, 0, s>>>(
+        power, L, nframes, num);
+}
+
+}  // namespace dxg2

@@ -26,12 +26,14 @@ __device__ __forceinline__ double i128_to_double(__int128 x) {
     return (double)hi * 18446744073709551616.0 + (double)lo;
 }
 
-// grid (ceil(N/256), n_bins), 256 threads: one thread per lag tau.
+// grid (ceil(N/256), n_bins), 256 threads: one thread per lag tau.  NumT is int64_t
+// (exact numerators of the Gram and event paths) or double (FFT path, no bias).
+template <typename NumT>
 __global__ __launch_bounds__(256) void finalize_kernel(const int64_t* __restrict__ sums,
-                                                       const int64_t* __restrict__ ugram,
+                                                       const NumT* __restrict__ ugram,
                                                        const BinDev* __restrict__ bins,
                                                        int nframes, int biased,
-                                                       int64_t* __restrict__ num,
+                                                       NumT* __restrict__ num,
                                                        double* __restrict__ den,
                                                        float* __restrict__ g2) {
     __shared__ int64_t sa[FIN_CH];
@@ -68,9 +70,10 @@ __global__ __launch_bounds__(256) void finalize_kernel(const int64_t* __restrict
     }
     if (tau >= nframes) return;
     const BinDev bd = bins[b];
-    int64_t n = ugram[(int64_t)b * nframes + tau];
+    NumT n = ugram[(int64_t)b * nframes + tau];
     if (biased)
-        n += 128 * (lo_sum + hi_sum) - (int64_t)16384 * ((int64_t)bd.nkb * KB) * (nframes - tau);
+        n += (NumT)(128 * (lo_sum + hi_sum) -
+                    (int64_t)16384 * ((int64_t)bd.nkb * KB) * (nframes - tau));
     const double d = i128_to_double(acc);
     const int64_t o = (int64_t)b * nframes + tau;
     if (num) num[o] = n;
@@ -83,7 +86,15 @@ void launch_finalize(const int64_t* sums, const int64_t* ugram, const BinDev* bi
                      float* g2, hipStream_t s) {
     if (n_bins <= 0 || nframes <= 0) return;
     dim3 grid((unsigned)((nframes + 255) / 256), (unsigned)n_bins);
-    finalize_kernel<<<grid, dim3(256), 0, s>>>(sums, ugram, bins, nframes, biased, num, den, g2);
+    finalize_kernel<int64_t><<<grid, dim3(256), 0, s>>>(sums, ugram, bins, nframes, biased, num,
+                                                        den, g2);
+}
+
+void launch_finalize_f64(const int64_t* sums, const double* fnum, const BinDev* bins, int n_bins,
+                         int nframes, double* num, double* den, float* g2, hipStream_t s) {
+    if (n_bins <= 0 || nframes <= 0) return;
+    dim3 grid((unsigned)((nframes + 255) / 256), (unsigned)n_bins);
+    finalize_kernel<double><<<grid, dim3(256), 0, s>>>(sums, fnum, bins, nframes, 0, num, den, g2);
 }
 
 }  // namespace dxg2

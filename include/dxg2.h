@@ -88,6 +88,16 @@ int dxg2_correlate_dense(dxg2_plan* plan, const void* d_frames, int64_t* d_num,
 int dxg2_correlate_dense_plain(dxg2_plan* plan, const void* d_frames, float* d_g2,
                                void* stream);
 
+/* FFT path for uint8 frames: per bin, the power spectra of every pixel's time series
+ * (FP32 Stockham FFT of length Nf = 2L, L = smallest power of two >= max(N, 1024),
+ * L <= 16384) summed in FP64, one FP64 inverse per bin, exact integer denominators.
+ * d_num receives the FP64 numerators (n_bins, N) (may be NULL); other outputs as for
+ * the dense path.  Numerators carry FP32 FFT rounding: see the documented tolerance.
+ * Replaces FFTWCorrelator.correlate / _correlate_fft (correlator/dense.py:314-369) and
+ * CUFFTCorrelator (correlator/cuda.py:129-219). */
+int dxg2_correlate_fft(dxg2_plan* plan, const void* d_frames, double* d_num, int64_t* d_sums,
+                       double* d_den, float* d_g2, void* stream);
+
 /* Event (sparse) path, pixel-major CSR exactly as the reference's EventCorrelator takes
  * it: d_times int32[nnz] frame indices, strictly increasing within a pixel; d_data
  * [nnz] counts of the plan dtype; d_offsets uint32[H*W+1], pixel p owns events
