@@ -15,7 +15,11 @@ Printed as one JSON line.  Besides the metric it reports
                 launch inside the timed region, against the 5.0 POP/s dense int8 peak;
   cpu_baseline  the reference's numpy MatMul correlator (float32 sgemm + diagonal
                 loop, restated in oracle/) on one bin of the same stack, all host cores;
-  plain_fp32    this library's one-bin-per-launch FP32 comparator on the same stack.
+  plain_fp32    this library's one-bin-per-launch FP32 comparator on the same stack;
+  fft           the FFT path (FP32 Stockham power spectra) on the same stack;
+  events        the event path on 1030 x 1065 x 100,000 frames at 1e-3 counts/pixel/frame
+                (generated as events on the GPU): pixel-major input, and frame-sorted
+                (pixel, frame, count) input including its conversion.
 """
 import argparse
 import ctypes
@@ -77,6 +81,98 @@ def cpu_baseline(frames_dev, qmask, target_s=15.0):
                       % (", ".join(done), n, spent, cores)}
 
 
+def _timed(fn, reps, torch):
+    fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) * 1e3 / reps
+
+
+def bench_fft(plan, frames, nb, n, pf):
+    """FFT path on the same stack; its stage times come from the library's HIP events."""
+    import torch
+    from dynamix_amd import _lib
+    num = torch.empty((nb, n), dtype=torch.float64, device="cuda")
+    sums = torch.empty((nb, n), dtype=torch.int64, device="cuda")
+    den = torch.empty((nb, n), dtype=torch.float64, device="cuda")
+    g2 = torch.empty((nb, n), dtype=torch.float32, device="cuda")
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+    def step():
+        _lib.check(plan.lib.dxg2_correlate_fft(plan.handle, _lib.ptr(frames), _lib.ptr(num),
+                                               _lib.ptr(sums), _lib.ptr(den), _lib.ptr(g2), st))
+    step()
+    torch.cuda.synchronize()
+    plan.set_profiling(True)
+    ms = _timed(step, 2, torch)
+    t = plan.stage_times()
+    plan.set_profiling(False)
+    L = 1024
+    while L < n:
+        L *= 2
+    npix = int(plan.pixel_counts[:nb].sum())
+    flop = npix * 5.0 * L * (L.bit_length() - 1)   # one complex FFT of L points per pixel
+    return {"ms": ms, "value": pf / (ms * 1e-3), "unit": "pixel*frames/s",
+            "stage_ms": {"gather_and_sums": t[0], "spectra_and_inverse": t[1], "finalise": t[2]},
+            "fft_length": 2 * L, "spectra_gflops": flop / (t[1] * 1e-3) / 1e9}
+
+
+def bench_events(seed):
+    """Event path on the events workload: pixel-major and frame-sorted COO input."""
+    import numpy as np
+    import torch
+    from dynamix_amd import _lib, synth
+    from dynamix_amd.correlator.common import Plan
+    wl = synth.WORKLOADS["events"]
+    shape, n = wl["shape"], wl["nframes"]
+    q = synth.workload_qmask("events")
+    t0 = time.perf_counter()
+    times, data, offsets = synth.speckle_events_device(shape, n, wl["mean"], wl["tau_c"], seed)
+    torch.cuda.synchronize()
+    gen_s = time.perf_counter() - t0
+    nnz = int(times.shape[0])
+    counts = np.diff(offsets.cpu().numpy().astype(np.int64))
+    binned = q.ravel() > 0
+    pairs = int((counts[binned] * (counts[binned] + 1) // 2).sum())
+    plan = Plan(shape, n, q, np.uint8, scale=np.bincount(q.ravel())[1:].astype(np.float64))
+    nb = plan.n_bins
+    num = torch.empty((nb, n), dtype=torch.int64, device="cuda")
+    sums = torch.empty_like(num)
+    den = torch.empty((nb, n), dtype=torch.float64, device="cuda")
+    g2 = torch.empty((nb, n), dtype=torch.float32, device="cuda")
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    outs = [_lib.ptr(x) for x in (num, sums, den, g2)]
+
+    def csr():
+        _lib.check(plan.lib.dxg2_correlate_events_csr(plan.handle, _lib.ptr(times), _lib.ptr(data),
+                                                      _lib.ptr(offsets), nnz, *outs, st))
+    ms_csr = _timed(csr, 2, torch)
+    num_csr = num.clone()
+    flags = ctypes.c_int32()
+    _lib.check(plan.lib.dxg2_plan_event_status(plan.handle, ctypes.byref(flags), st))
+    pix, fr, cnt = synth.events_to_coo_device(times, data, offsets, shape, n)
+    del times, data
+    torch.cuda.empty_cache()
+
+    def coo():
+        _lib.check(plan.lib.dxg2_correlate_events_coo(plan.handle, _lib.ptr(pix), _lib.ptr(fr),
+                                                      _lib.ptr(cnt), nnz, *outs, st))
+    ms_coo = _timed(coo, 2, torch)
+    same = bool((num == num_csr).all().item())
+    hw = shape[0] * shape[1]
+    conv_bytes = 9 * nnz + 2 * 5 * nnz + 4 * (hw + 1)
+    plan.close()
+    return {"workload": "events: %dx%d x %d frames, mean %g, %d bins, %d events, %d pairs"
+                        % (shape[0], shape[1], n, wl["mean"], nb, nnz, pairs),
+            "csr_ms": ms_csr, "coo_ms": ms_coo, "pairs_per_s": pairs / (ms_csr * 1e-3),
+            "events_per_s": nnz / (ms_csr * 1e-3), "pixel_frames_per_s": int(binned.sum()) * n / (ms_csr * 1e-3),
+            "coo_to_csr_GBps": conv_bytes / ((ms_coo - ms_csr) * 1e-3) / 1e9 if ms_coo > ms_csr else None,
+            "coo_equals_csr": same, "input_flags": flags.value, "generate_s": gen_s}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -85,6 +181,8 @@ def main():
     ap.add_argument("--workload", default="headline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-fft", action="store_true")
+    ap.add_argument("--no-events", action="store_true")
     ap.add_argument("--seed", type=int, default=2026)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -183,9 +281,18 @@ def main():
         del pc, pg2
         torch.cuda.empty_cache()
 
+    if not args.no_fft:
+        log("fft path")
+        out["fft"] = bench_fft(plan, frames, nb, n, pf)
     if not args.no_cpu:
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(frames, q)
+    if not args.no_events:
+        del frames, num, sums, den, g2
+        plan.close()
+        torch.cuda.empty_cache()
+        log("event path")
+        out["events"] = bench_events(args.seed)
     print(json.dumps(out), flush=True)
 
 

@@ -60,6 +60,11 @@ _SIGNATURES = {
     "dxg2_plan_stage_times": (c_int, [c_void_p, c_void_p, c_int]),
     "dxg2_synth_speckle": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_double,
                                    c_u64, c_void_p]),
+    "dxg2_synth_events": (c_int, [c_void_p, c_int, c_int, c_int, c_double, c_u64, c_int,
+                                  c_void_p, c_void_p, c_void_p, c_i64, ctypes.POINTER(c_i64),
+                                  c_void_p]),
+    "dxg2_events_to_coo": (c_int, [c_void_p, c_void_p, c_void_p, c_i64, c_int, c_int, c_int,
+                                   c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     "dxg2_last_error": (ctypes.c_char_p, []),
     "dxg2_version": (ctypes.c_char_p, []),
 }

@@ -656,3 +656,70 @@ int dxg2_synth_speckle(void* d_frames, int32_t dtype, int32_t nframes, int32_t h
 }
 
 }  // extern "C"
+
+extern "C" {
+
+int dxg2_synth_events(const float* d_mean_map, int32_t height, int32_t width, int32_t nframes,
+                      double tau_c, uint64_t seed, int32_t dtype, uint32_t* d_offsets,
+                      int32_t* d_times, void* d_data, int64_t capacity, int64_t* nnz,
+                      void* stream) {
+    if (!d_mean_map || !d_offsets || !nnz) return set_error(DXG2_EINVAL, "null argument");
+    if (height <= 0 || width <= 0 || nframes <= 0) return set_error(DXG2_EINVAL, "bad sizes");
+    if (dtype != DXG2_U8 && dtype != DXG2_U16)
+        return set_error(DXG2_EINVAL, "event counts are uint8 or uint16");
+    const int64_t hw = (int64_t)height * width;
+    hipStream_t s = (hipStream_t)stream;
+    const float rho = tau_c > 0 ? (float)std::exp(-1.0 / tau_c) : 0.f;
+    uint32_t *count = nullptr, *tot = nullptr;
+    DXG2_HIP(hipMalloc(&count, hw * 4));
+    DXG2_HIP(hipMalloc(&tot, ((hw + 1023) / 1024 + 1) * 4));
+    (void)hipGetLastError();
+    launch_synth_events(0, dtype, nframes, hw, d_mean_map, rho, seed, count, nullptr, nullptr,
+                        nullptr, s);
+    launch_excl_scan(count, hw, tot, d_offsets, s);
+    uint32_t total = 0;
+    hipError_t e = hipMemcpyAsync(&total, d_offsets + hw, 4, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    (void)hipFree(count);
+    (void)hipFree(tot);
+    if (e != hipSuccess) return set_error(DXG2_EHIP, "event count failed: %s", hipGetErrorString(e));
+    *nnz = total;
+    if (d_times) {
+        if (capacity < (int64_t)total)
+            return set_error(DXG2_ECAPACITY, "capacity %lld < %u events", (long long)capacity,
+                             total);
+        launch_synth_events(1, dtype, nframes, hw, d_mean_map, rho, seed, nullptr, d_offsets,
+                            d_times, d_data, s);
+    }
+    DXG2_HIP(hipGetLastError());
+    return DXG2_OK;
+}
+
+int dxg2_events_to_coo(const int32_t* d_times, const void* d_data, const uint32_t* d_offsets,
+                       int64_t nnz, int32_t height, int32_t width, int32_t nframes, int32_t dtype,
+                       uint32_t* d_pixel, uint32_t* d_frame, void* d_count, void* stream) {
+    if (!d_offsets || (nnz > 0 && (!d_times || !d_data || !d_pixel || !d_frame || !d_count)))
+        return set_error(DXG2_EINVAL, "null argument");
+    if (nframes <= 0) return set_error(DXG2_EINVAL, "bad frame count");
+    hipStream_t s = (hipStream_t)stream;
+    const int64_t hw = (int64_t)height * width;
+    uint32_t *fcount = nullptr, *fcursor = nullptr, *tot = nullptr;
+    DXG2_HIP(hipMalloc(&fcount, (size_t)nframes * 4));
+    DXG2_HIP(hipMalloc(&fcursor, (size_t)(nframes + 1) * 4));
+    DXG2_HIP(hipMalloc(&tot, (size_t)((nframes + 1023) / 1024 + 1) * 4));
+    (void)hipGetLastError();
+    launch_fill_u32(fcount, nframes, s);
+    launch_frame_count(d_times, nnz, nframes, fcount, s);
+    launch_excl_scan(fcount, nframes, tot, fcursor, s);
+    launch_to_coo(dtype, d_times, d_data, d_offsets, hw, nframes, fcursor, d_pixel, d_frame,
+                  d_count, s);
+    hipError_t e = hipStreamSynchronize(s);
+    (void)hipFree(fcount);
+    (void)hipFree(fcursor);
+    (void)hipFree(tot);
+    if (e != hipSuccess) return set_error(DXG2_EHIP, "to_coo failed: %s", hipGetErrorString(e));
+    DXG2_HIP(hipGetLastError());
+    return DXG2_OK;
+}
+
+}  // extern "C"

@@ -91,6 +91,17 @@ int launch_fft_power(int L, const uint8_t* xs, const BinDev* bins, const KbWork*
                      hipStream_t s);
 void launch_fft_inverse(const double* power, int L, int n_bins, int nframes, double* num,
                         hipStream_t s);
+void launch_excl_scan(const uint32_t* in, int64_t n, uint32_t* tot, uint32_t* out, hipStream_t s);
+void launch_copy_u32(const uint32_t* src, uint32_t* dst, int64_t n, hipStream_t s);
+void launch_fill_u32(uint32_t* p, int64_t n, hipStream_t s);
+void launch_synth_events(int pass, int dtype, int nframes, int64_t hw, const float* mean_map,
+                         float rho, uint64_t seed, uint32_t* count, const uint32_t* offsets,
+                         int32_t* times, void* data, hipStream_t s);
+void launch_frame_count(const int32_t* times, int64_t nnz, int nframes, uint32_t* fcount,
+                        hipStream_t s);
+void launch_to_coo(int dtype, const int32_t* times, const void* data, const uint32_t* offsets,
+                   int64_t npix, int nframes, uint32_t* fcursor, uint32_t* pix, uint32_t* frame,
+                   void* cnt, hipStream_t s);
 void launch_synth(void* frames, int dtype, int nframes, int64_t hw, const float* mean_map,
                   float rho, uint64_t seed, hipStream_t s);
 

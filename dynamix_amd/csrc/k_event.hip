@@ -259,7 +259,25 @@ __global__ void segment_sort_kernel(const uint32_t* __restrict__ offsets, uint32
     }
 }
 
-static unsigned grid_for(int64_t n, int64_t cap = 65536) {
+static unsigned grid_for(int64_t n, int64_t cap = 65536);
+
+void launch_excl_scan(const uint32_t* in, int64_t n, uint32_t* tot, uint32_t* out, hipStream_t s) {
+    if (n <= 0) return;
+    const int nb = (int)((n + SCAN_B - 1) / SCAN_B);
+    scan_block_totals<<<nb, 256, 0, s>>>(in, n, tot);
+    scan_totals<<<1, 256, 0, s>>>(tot, nb);
+    scan_apply<<<nb, 256, 0, s>>>(in, n, tot, out);
+}
+
+void launch_copy_u32(const uint32_t* src, uint32_t* dst, int64_t n, hipStream_t s) {
+    if (n > 0) copy_u32_kernel<<<grid_for(n), 256, 0, s>>>(src, dst, n);
+}
+
+void launch_fill_u32(uint32_t* p, int64_t n, hipStream_t s) {
+    if (n > 0) fill_u32_kernel<<<grid_for(n), 256, 0, s>>>(p, n, 0u);
+}
+
+static unsigned grid_for(int64_t n, int64_t cap) {
     int64_t g = (n + 255) / 256;
     if (g < 1) g = 1;
     if (g > cap) g = cap;

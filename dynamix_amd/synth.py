@@ -113,3 +113,46 @@ def speckle_frames_device(shape, nframes, mean, tau_c, seed, dtype=np.uint8, mea
                                       int(seed) & 0xFFFFFFFFFFFFFFFF, stream),
                "dxg2_synth_speckle")
     return frames
+
+
+def speckle_events_device(shape, nframes, mean, tau_c, seed, dtype=np.uint8, mean_map_=None):
+    """GPU events of the same walk as speckle_frames_device, in the pixel-major layout:
+    returns torch tensors (times int32, data, offsets uint32)."""
+    import torch
+    from . import _lib
+    lib = _lib.load()
+    dtype = np.dtype(dtype)
+    tdt = {np.dtype(np.uint8): torch.uint8, np.dtype(np.uint16): torch.uint16}[dtype]
+    mm = mean_map(shape, mean) if mean_map_ is None else np.asarray(mean_map_, np.float32)
+    d_mean = torch.from_numpy(np.ascontiguousarray(mm, dtype=np.float32)).to("cuda")
+    hw = int(shape[0]) * int(shape[1])
+    offsets = torch.empty(hw + 1, dtype=torch.int32, device="cuda")
+    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    nnz = ctypes.c_int64()
+    args = (_lib.ptr(d_mean), shape[0], shape[1], nframes, float(tau_c),
+            int(seed) & 0xFFFFFFFFFFFFFFFF, _lib.DTYPE_CODES[dtype], _lib.ptr(offsets))
+    _lib.check(lib.dxg2_synth_events(*args, None, None, 0, ctypes.byref(nnz), stream),
+               "dxg2_synth_events")
+    n = nnz.value
+    times = torch.empty(max(n, 1), dtype=torch.int32, device="cuda")
+    data = torch.empty(max(n, 1), dtype=tdt, device="cuda")
+    _lib.check(lib.dxg2_synth_events(*args, _lib.ptr(times), _lib.ptr(data), max(n, 1),
+                                     ctypes.byref(nnz), stream), "dxg2_synth_events")
+    return times[:n], data[:n], offsets
+
+
+def events_to_coo_device(times, data, offsets, shape, nframes):
+    """Pixel-major events (torch, on the GPU) -> frame-sorted COO (pixel, frame, count)."""
+    import torch
+    from . import _lib
+    lib = _lib.load()
+    n = int(times.shape[0])
+    pix = torch.empty(max(n, 1), dtype=torch.int32, device="cuda")
+    fr = torch.empty(max(n, 1), dtype=torch.int32, device="cuda")
+    cnt = torch.empty(max(n, 1), dtype=data.dtype, device="cuda")
+    code = _lib.DXG2_U16 if data.dtype == torch.uint16 else _lib.DXG2_U8
+    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    _lib.check(lib.dxg2_events_to_coo(_lib.ptr(times), _lib.ptr(data), _lib.ptr(offsets), n,
+                                      shape[0], shape[1], nframes, code, _lib.ptr(pix),
+                                      _lib.ptr(fr), _lib.ptr(cnt), stream), "dxg2_events_to_coo")
+    return pix[:n], fr[:n], cnt[:n]

@@ -144,6 +144,22 @@ int dxg2_synth_speckle(void* d_frames, int32_t dtype, int32_t nframes, int32_t h
                        int32_t width, const float* d_mean_map, double tau_c,
                        uint64_t seed, void* stream);
 
+/* Synthetic events: the same per-pixel walk as dxg2_synth_speckle, emitting only the
+ * non-zero samples in the pixel-major layout (so they equal the compaction of the dense
+ * stack with the same seed).  Always writes d_offsets (H*W+1) and *nnz (waits for the
+ * stream); writes d_times/d_data too when d_times is non-NULL and capacity >= *nnz.
+ * Test-data generator; no reference counterpart. */
+int dxg2_synth_events(const float* d_mean_map, int32_t height, int32_t width, int32_t nframes,
+                      double tau_c, uint64_t seed, int32_t dtype, uint32_t* d_offsets,
+                      int32_t* d_times, void* d_data, int64_t capacity, int64_t* nnz,
+                      void* stream);
+
+/* Pixel-major events -> frame-sorted COO (pixel, frame, count), pixel order within a
+ * frame unspecified.  Waits for the stream.  Test-data utility; no reference counterpart. */
+int dxg2_events_to_coo(const int32_t* d_times, const void* d_data, const uint32_t* d_offsets,
+                       int64_t nnz, int32_t height, int32_t width, int32_t nframes, int32_t dtype,
+                       uint32_t* d_pixel, uint32_t* d_frame, void* d_count, void* stream);
+
 const char* dxg2_last_error(void);
 const char* dxg2_version(void);
 

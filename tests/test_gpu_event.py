@@ -121,3 +121,30 @@ def test_events_invalid_input_raises():
     # a valid call after the errors is exact again
     g2 = c.correlate(times, data, offsets)
     _check(c, g2, ev.event_g2(times, data, offsets, np.ones((5, 5), np.int32), 60))
+
+
+def test_device_event_generator_matches_dense_compaction():
+    """dxg2_synth_events emits exactly the non-zero samples of dxg2_synth_speckle (same
+    seed); the dense and event paths then give identical numerators."""
+    _announce("device events vs dense")
+    from dynamix_amd.correlator.dense import MatMulCorrelator
+    from dynamix_amd.correlator.event import EventCorrelator
+    from dynamix_amd.synth import (events_to_coo_device, speckle_events_device,
+                                   speckle_frames_device)
+    shape, n = (24, 20), 700
+    f = speckle_frames_device(shape, n, 0.05, 40.0, seed=99)
+    t, d, o = speckle_events_device(shape, n, 0.05, 40.0, seed=99)
+    data, times, offsets = ev.compress_all_stack(f.cpu().numpy())
+    np.testing.assert_array_equal(o.cpu().numpy().astype(np.uint32), offsets)
+    np.testing.assert_array_equal(t.cpu().numpy(), times)
+    np.testing.assert_array_equal(d.cpu().numpy(), data)
+    q = np.random.default_rng(9).integers(0, 5, size=shape).astype(np.int32)
+    cd = MatMulCorrelator(shape, n, q)
+    cd.correlate(f)
+    ce = EventCorrelator(shape, n, 50, qmask=q, dtype=np.uint8)
+    ce.correlate(t, d, o)
+    assert (cd.last_numerators == ce.last_numerators).all().item()
+    pix, fr, cnt = events_to_coo_device(t, d, o, shape, n)
+    assert (fr[1:] >= fr[:-1]).all().item()
+    ce.correlate_coo(pix, fr, cnt)
+    assert (cd.last_numerators == ce.last_numerators).all().item()
