@@ -184,6 +184,9 @@ def main():
     ap.add_argument("--no-fft", action="store_true")
     ap.add_argument("--no-events", action="store_true")
     ap.add_argument("--seed", type=int, default=2026)
+    ap.add_argument("--pmc-copy-gib", type=float, default=0.0,
+                    help="after the run, copy this many GiB device to device (a kernel of "
+                         "known traffic that scales rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build runs the whole workload on a "
@@ -293,6 +296,15 @@ def main():
         torch.cuda.empty_cache()
         log("event path")
         out["events"] = bench_events(args.seed)
+    if args.pmc_copy_gib > 0:
+        nbytes = int(args.pmc_copy_gib * (1 << 30))
+        src = torch.ones(nbytes, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("copied %d bytes device to device" % nbytes)
+        del src, dst
     print(json.dumps(out), flush=True)
 
 

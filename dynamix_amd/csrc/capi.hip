@@ -91,6 +91,8 @@ struct dxg2_plan {
     int8_t* d_slab = nullptr;
     int64_t* d_ugram = nullptr;
     int64_t* d_sums = nullptr;
+    int64_t* d_prefix = nullptr;  // per bin: prefix sums of S(t), N + 1 values
+    int64_t* d_smax = nullptr;    // per bin: max |S(t)|
     float* d_pa = nullptr;
     float* d_pg = nullptr;
     float* d_pm = nullptr;
@@ -372,6 +374,11 @@ int dxg2_plan_create(const dxg2_plan_desc* desc, dxg2_plan** out) {
         rc = set_error(DXG2_ENOMEM, "cannot allocate numerator workspace");
     if (!rc && hipMalloc(&p->d_sums, nb_n) != hipSuccess)
         rc = set_error(DXG2_ENOMEM, "cannot allocate sums workspace");
+    if (!rc && (hipMalloc(&p->d_prefix, (size_t)std::max(p->n_bins, 1) * (p->N + 1) *
+                                            sizeof(int64_t)) != hipSuccess ||
+                hipMalloc(&p->d_smax, (size_t)std::max(p->n_bins, 1) * sizeof(int64_t)) !=
+                    hipSuccess))
+        rc = set_error(DXG2_ENOMEM, "cannot allocate denominator workspace");
     if (rc) {
         dxg2_plan_destroy(p);
         return rc;
@@ -383,7 +390,7 @@ int dxg2_plan_create(const dxg2_plan_desc* desc, dxg2_plan** out) {
 int dxg2_plan_destroy(dxg2_plan* p) {
     if (!p) return DXG2_OK;
     void* bufs[] = {p->d_pix,   p->d_bins, p->d_gwork, p->d_mwork, p->d_slab,
-                    p->d_ugram, p->d_sums, p->d_pa,    p->d_pg,    p->d_pm,
+                    p->d_ugram, p->d_sums, p->d_prefix, p->d_smax, p->d_pa, p->d_pg, p->d_pm,
                     p->d_ework, p->d_err,  p->d_cnt,   p->d_scan,  p->d_off,
                     p->d_etimes, p->d_edata, p->d_kbwork, p->d_xs, p->d_tw,
                     p->d_power, p->d_fnum};
@@ -409,7 +416,8 @@ int dxg2_plan_query(const dxg2_plan* p, int32_t* n_bins, int64_t* pixel_counts, 
 int dxg2_plan_footprint(const dxg2_plan* p, int32_t path, size_t* bytes) {
     if (!p || !bytes) return set_error(DXG2_EINVAL, "null argument");
     const size_t nb_n = (size_t)std::max(p->n_bins, 1) * p->N * sizeof(int64_t);
-    size_t common = p->pix_idx.size() * 4 + p->bins.size() * sizeof(BinDev) + 2 * nb_n;
+    size_t common = p->pix_idx.size() * 4 + p->bins.size() * sizeof(BinDev) + 2 * nb_n +
+                    (size_t)std::max(p->n_bins, 1) * (p->N + 2) * sizeof(int64_t);
     switch (path) {
         case 0:
             *bytes = common + p->slab_bytes + p->gather_work.size() * sizeof(GatherWork) +
@@ -494,7 +502,7 @@ int dxg2_correlate_dense(dxg2_plan* p, const void* d_frames, int64_t* d_num, int
                    p->N, p->nrows, p->dtype == DXG2_U8 ? 1 : 0, p->d_ugram, sums, s);
     if (ev) DXG2_HIP(hipEventRecord(ev[2], s));
     launch_finalize(sums, p->d_ugram, p->d_bins, p->n_bins, p->N, p->dtype == DXG2_U8 ? 1 : 0,
-                    d_num, d_den, d_g2, s);
+                    p->d_prefix, p->d_smax, d_num, d_den, d_g2, s);
     if (ev) DXG2_HIP(hipEventRecord(ev[3], s));
     DXG2_HIP(hipGetLastError());
     return DXG2_OK;
@@ -548,7 +556,8 @@ static int events_from_csr(dxg2_plan* p, const int32_t* times, const void* data,
     int64_t* sums = d_sums ? d_sums : p->d_sums;
     launch_event_pairs(p->dtype, times, data, offsets, p->d_pix, p->d_bins, p->d_ework,
                        (int)p->event_work.size(), p->N, nnz, p->d_ugram, sums, p->d_err, s);
-    launch_finalize(sums, p->d_ugram, p->d_bins, p->n_bins, p->N, 0, d_num, d_den, d_g2, s);
+    launch_finalize(sums, p->d_ugram, p->d_bins, p->n_bins, p->N, 0, p->d_prefix, p->d_smax,
+                    d_num, d_den, d_g2, s);
     DXG2_HIP(hipGetLastError());
     return DXG2_OK;
 }
@@ -623,7 +632,8 @@ int dxg2_correlate_fft(dxg2_plan* p, const void* d_frames, double* d_num, int64_
         return set_error(DXG2_EUNSUPPORTED, "no FFT kernel for length %d", p->fft_L);
     launch_fft_inverse(p->d_power, p->fft_L, p->n_bins, p->N, p->d_fnum, s);
     if (ev) DXG2_HIP(hipEventRecord(ev[2], s));
-    launch_finalize_f64(sums, p->d_fnum, p->d_bins, p->n_bins, p->N, d_num, d_den, d_g2, s);
+    launch_finalize_f64(sums, p->d_fnum, p->d_bins, p->n_bins, p->N, p->d_prefix, p->d_smax,
+                        d_num, d_den, d_g2, s);
     if (ev) DXG2_HIP(hipEventRecord(ev[3], s));
     DXG2_HIP(hipGetLastError());
     return DXG2_OK;

@@ -6,19 +6,28 @@
 // normalize_correlation in the event path (evtcorrelator.cl:58-76); the ratio
 // res = Num / (Den / n_b) (dense.py:64, densecorrelator.cl:54).
 //
-// Here Den_b(tau) = sum_t S(t) S(t+tau) is accumulated in 128-bit integers (exact for
-// every input size), Num_b(tau) is completed from the raw i8 Gram diagonals with the
-// uint8 bias correction, and g2 = scale_b * Num / Den is formed once in FP64.
+// Here Den_b(tau) = sum_t S(t) S(t+tau) is exact, Num_b(tau) is completed from the raw i8
+// Gram diagonals with the uint8 bias correction, and g2 = scale_b * Num / Den is formed
+// once in FP64.
+//
+// Exactness: a per-bin prep pass forms the prefix sums P(t) = sum_{u<t} S(u) and
+// M = max_t |S(t)|.  When N M^2 < 2^52 every product and every partial sum of Den is an
+// integer below 2^53, so FP64 fused multiply-adds are exact; otherwise the lag loop
+// accumulates in 128-bit integers.  Both give the same Den.
 //
 // Bias correction (uint8 frames are stored as s = v - 128):
 //   G(t,t') = U(t,t') + 128 (S(t) + S(t')) - 2^14 K_pad
 // summed over t = 0..N-1-tau:
-//   Num(tau) = U(tau) + 128 (sum_{t<N-tau} S(t) + sum_{t>=tau} S(t)) - 2^14 K_pad (N-tau).
+//   Num(tau) = U(tau) + 128 (P(N-tau) + P(N) - P(tau)) - 2^14 K_pad (N-tau).
 #include "dxg2_internal.h"
 
 namespace dxg2 {
 
-constexpr int FIN_CH = 1024;  // frames of S staged per pass
+constexpr int FIN_CH = 1024;       // frames of S staged per pass
+constexpr int FIN_THREADS = 256;
+constexpr int FIN_LAGS = 4;        // lags per thread: tau0 + tid + 256 j, j < FIN_LAGS
+constexpr int FIN_SPAN = FIN_THREADS * FIN_LAGS;  // lags per block
+constexpr int PREP_THREADS = 1024;
 
 __device__ __forceinline__ double i128_to_double(__int128 x) {
     const long long hi = (long long)(x >> 64);
@@ -26,75 +35,155 @@ __device__ __forceinline__ double i128_to_double(__int128 x) {
     return (double)hi * 18446744073709551616.0 + (double)lo;
 }
 
-// grid (ceil(N/256), n_bins), 256 threads: one thread per lag tau.  NumT is int64_t
-// (exact numerators of the Gram and event paths) or double (FFT path, no bias).
-template <typename NumT>
-__global__ __launch_bounds__(256) void finalize_kernel(const int64_t* __restrict__ sums,
-                                                       const NumT* __restrict__ ugram,
-                                                       const BinDev* __restrict__ bins,
-                                                       int nframes, int biased,
-                                                       NumT* __restrict__ num,
-                                                       double* __restrict__ den,
-                                                       float* __restrict__ g2) {
-    __shared__ int64_t sa[FIN_CH];
-    __shared__ int64_t sb[FIN_CH + 256];
-    const int b = blockIdx.y;
-    const int tau0 = blockIdx.x * 256;
-    const int tau = tau0 + threadIdx.x;
+// One block per bin: prefix[b][t] = sum_{u<t} S(u) for t = 0..N, smax[b] = max |S|.
+__global__ __launch_bounds__(PREP_THREADS) void sums_prep_kernel(const int64_t* __restrict__ sums,
+                                                                 int nframes,
+                                                                 int64_t* __restrict__ prefix,
+                                                                 int64_t* __restrict__ smax) {
+    __shared__ int64_t wave_tot[PREP_THREADS / 64];
+    __shared__ int64_t wave_max[PREP_THREADS / 64];
+    const int b = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
     const int64_t* S = sums + (int64_t)b * nframes;
+    int64_t* P = prefix + (int64_t)b * (nframes + 1);
+    int64_t carry = 0, mx = 0;
+    for (int t0 = 0; t0 < nframes; t0 += PREP_THREADS) {
+        const int t = t0 + tid;
+        const int64_t v = t < nframes ? S[t] : 0;
+        mx = max(mx, v < 0 ? -v : v);
+        // inclusive scan within the wave
+        int64_t x = v;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int64_t y = __shfl_up(x, d, 64);
+            if (lane >= d) x += y;
+        }
+        if (lane == 63) wave_tot[wave] = x;
+        __syncthreads();
+        int64_t before = carry;
+        for (int w = 0; w < wave; ++w) before += wave_tot[w];
+        int64_t total = carry;
+        for (int w = 0; w < PREP_THREADS / 64; ++w) total += wave_tot[w];
+        if (t < nframes) P[t] = before + x - v;  // exclusive prefix
+        carry = total;
+        __syncthreads();
+    }
+    if (tid == 0) P[nframes] = carry;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) mx = max(mx, __shfl_xor(mx, d, 64));
+    if (lane == 0) wave_max[wave] = mx;
+    __syncthreads();
+    if (tid == 0) {
+        int64_t m = 0;
+        for (int w = 0; w < PREP_THREADS / 64; ++w) m = max(m, wave_max[w]);
+        smax[b] = m;
+    }
+}
 
-    __int128 acc = 0;
-    int64_t lo_sum = 0;  // sum_{t < N - tau} S(t)
-    int64_t hi_sum = 0;  // sum_{t < N - tau} S(t + tau)
+// Accumulate Den for the FIN_LAGS lags of this thread over the frames of the block.
+// Stage element type E (double or int64) and accumulator A (double or __int128).
+template <typename E, typename A>
+__device__ __forceinline__ void den_lags(const int64_t* __restrict__ S, int nframes, int tau0,
+                                         E* sa, E* sb, A (&acc)[FIN_LAGS]) {
+    const int tid = threadIdx.x;
     const int tmax = nframes - tau0;  // frames t needed by the smallest lag of the block
     for (int t0 = 0; t0 < tmax; t0 += FIN_CH) {
         __syncthreads();
-        for (int i = threadIdx.x; i < FIN_CH; i += 256) {
+        for (int i = tid; i < FIN_CH; i += FIN_THREADS) {
             const int t = t0 + i;
-            sa[i] = t < nframes ? S[t] : 0;
+            sa[i] = (E)(t < nframes ? S[t] : 0);
         }
-        for (int i = threadIdx.x; i < FIN_CH + 256; i += 256) {
-            const int t = t0 + tau0 + i;
-            sb[i] = t < nframes ? S[t] : 0;
+        for (int i = tid; i < FIN_CH + FIN_SPAN; i += FIN_THREADS) {
+            const int64_t t = (int64_t)t0 + tau0 + i;
+            sb[i] = (E)(t < nframes ? S[t] : 0);
         }
         __syncthreads();
-        int lim = nframes - tau - t0;
+        // Terms past the end read the zero padding of sb, so one bound serves all lags.
+        int lim = nframes - (tau0 + tid) - t0;
         if (lim > FIN_CH) lim = FIN_CH;
         for (int k = 0; k < lim; ++k) {
-            const int64_t x = sa[k];
-            const int64_t y = sb[k + threadIdx.x];
-            acc += (__int128)x * (__int128)y;
-            lo_sum += x;
-            hi_sum += y;
+            const E x = sa[k];
+#pragma unroll
+            for (int j = 0; j < FIN_LAGS; ++j) {
+                const E y = sb[k + tid + FIN_THREADS * j];
+                if constexpr (sizeof(A) == 16)
+                    acc[j] += (__int128)x * (__int128)y;
+                else
+                    acc[j] = __builtin_fma(x, y, acc[j]);
+            }
         }
     }
-    if (tau >= nframes) return;
+}
+
+// grid (ceil(N / FIN_SPAN), n_bins), FIN_THREADS threads.  NumT is int64_t (exact
+// numerators of the Gram and event paths) or double (FFT path, no bias).
+template <typename NumT>
+__global__ __launch_bounds__(FIN_THREADS) void finalize_kernel(
+    const int64_t* __restrict__ sums, const NumT* __restrict__ ugram,
+    const BinDev* __restrict__ bins, int nframes, int biased,
+    const int64_t* __restrict__ prefix, const int64_t* __restrict__ smax,
+    NumT* __restrict__ num, double* __restrict__ den, float* __restrict__ g2) {
+    __shared__ __attribute__((aligned(16))) unsigned char stage[(2 * FIN_CH + FIN_SPAN) * 8];
+    const int b = blockIdx.y;
+    const int tau0 = blockIdx.x * FIN_SPAN;
+    const int tid = threadIdx.x;
+    const int64_t* S = sums + (int64_t)b * nframes;
+    const int64_t* P = prefix + (int64_t)b * (nframes + 1);
+
+    double d[FIN_LAGS];
+    const double m = (double)smax[b];
+    if ((double)nframes * m * m < 4503599627370496.0) {  // 2^52: FP64 sums stay exact
+        double acc[FIN_LAGS] = {};
+        den_lags<double, double>(S, nframes, tau0, reinterpret_cast<double*>(stage),
+                                 reinterpret_cast<double*>(stage) + FIN_CH, acc);
+#pragma unroll
+        for (int j = 0; j < FIN_LAGS; ++j) d[j] = acc[j];
+    } else {
+        __int128 acc[FIN_LAGS] = {};
+        den_lags<int64_t, __int128>(S, nframes, tau0, reinterpret_cast<int64_t*>(stage),
+                                    reinterpret_cast<int64_t*>(stage) + FIN_CH, acc);
+#pragma unroll
+        for (int j = 0; j < FIN_LAGS; ++j) d[j] = i128_to_double(acc[j]);
+    }
+
     const BinDev bd = bins[b];
-    NumT n = ugram[(int64_t)b * nframes + tau];
-    if (biased)
-        n += (NumT)(128 * (lo_sum + hi_sum) -
-                    (int64_t)16384 * ((int64_t)bd.nkb * KB) * (nframes - tau));
-    const double d = i128_to_double(acc);
-    const int64_t o = (int64_t)b * nframes + tau;
-    if (num) num[o] = n;
-    if (den) den[o] = d;
-    g2[o] = (float)(bd.scale * ((double)n / d));
+#pragma unroll
+    for (int j = 0; j < FIN_LAGS; ++j) {
+        const int tau = tau0 + tid + FIN_THREADS * j;
+        if (tau >= nframes) continue;
+        NumT n = ugram[(int64_t)b * nframes + tau];
+        if (biased)
+            n += (NumT)(128 * (P[nframes - tau] + P[nframes] - P[tau]) -
+                        (int64_t)16384 * ((int64_t)bd.nkb * KB) * (nframes - tau));
+        const int64_t o = (int64_t)b * nframes + tau;
+        if (num) num[o] = n;
+        if (den) den[o] = d[j];
+        g2[o] = (float)(bd.scale * ((double)n / d[j]));
+    }
 }
 
 void launch_finalize(const int64_t* sums, const int64_t* ugram, const BinDev* bins,
-                     int n_bins, int nframes, int biased, int64_t* num, double* den,
-                     float* g2, hipStream_t s) {
+                     int n_bins, int nframes, int biased, int64_t* prefix, int64_t* smax,
+                     int64_t* num, double* den, float* g2, hipStream_t s) {
     if (n_bins <= 0 || nframes <= 0) return;
-    dim3 grid((unsigned)((nframes + 255) / 256), (unsigned)n_bins);
-    finalize_kernel<int64_t><<<grid, dim3(256), 0, s>>>(sums, ugram, bins, nframes, biased, num,
-                                                        den, g2);
+    sums_prep_kernel<<<dim3((unsigned)n_bins), dim3(PREP_THREADS), 0, s>>>(sums, nframes,
+                                                                          prefix, smax);
+    dim3 grid((unsigned)((nframes + FIN_SPAN - 1) / FIN_SPAN), (unsigned)n_bins);
+    finalize_kernel<int64_t><<<grid, dim3(FIN_THREADS), 0, s>>>(sums, ugram, bins, nframes,
+                                                                biased, prefix, smax, num, den,
+                                                                g2);
 }
 
 void launch_finalize_f64(const int64_t* sums, const double* fnum, const BinDev* bins, int n_bins,
-                         int nframes, double* num, double* den, float* g2, hipStream_t s) {
+                         int nframes, int64_t* prefix, int64_t* smax, double* num, double* den,
+                         float* g2, hipStream_t s) {
     if (n_bins <= 0 || nframes <= 0) return;
-    dim3 grid((unsigned)((nframes + 255) / 256), (unsigned)n_bins);
-    finalize_kernel<double><<<grid, dim3(256), 0, s>>>(sums, fnum, bins, nframes, 0, num, den, g2);
+    sums_prep_kernel<<<dim3((unsigned)n_bins), dim3(PREP_THREADS), 0, s>>>(sums, nframes,
+                                                                          prefix, smax);
+    dim3 grid((unsigned)((nframes + FIN_SPAN - 1) / FIN_SPAN), (unsigned)n_bins);
+    finalize_kernel<double><<<grid, dim3(FIN_THREADS), 0, s>>>(sums, fnum, bins, nframes, 0,
+                                                               prefix, smax, num, den, g2);
 }
 
 }  // namespace dxg2

@@ -80,6 +80,18 @@ def test_dense_u8_full_range_values():
     _check_exact(c, g2, og.dense_g2(f, q))
 
 
+def test_dense_denominator_int128_path():
+    """N max(S)^2 >= 2^52: the denominator takes the 128-bit accumulation path."""
+    _announce("dense denominator int128 path")
+    rng = np.random.default_rng(7)
+    f = rng.integers(200, 256, size=(64, 256, 256)).astype(np.uint8)
+    q = np.ones((256, 256), dtype=np.int32)
+    s_max = float(f.reshape(64, -1).sum(axis=1, dtype=np.int64).max())
+    assert 64 * s_max ** 2 >= 2.0 ** 52
+    c, g2 = _run(f, q)
+    _check_exact(c, g2, og.dense_g2(f, q))
+
+
 def test_dense_i8_signed_values():
     _announce("dense i8 signed")
     rng = np.random.default_rng(2)
