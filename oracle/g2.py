@@ -11,7 +11,7 @@ This is what every reference correlator computes (README.md:11-15;
 dynamix/correlator/dense.py:48-64 MatMul; dense.py:364-368 FFT;
 dynamix/resources/opencl/densecorrelator.cl:54 and evtcorrelator.cl:70-75).
 
-Parity pinning of each function (tests/test_oracle.py, tests/test_golden.py):
+Parity pinning of each function (tests/test_oracle.py):
   qmask_bins, scale_factors   -- golden vectors of the reference's BaseCorrelator
                                  semantics + closed form (bins 1..max, None -> 1 bin).
   frame_sums, numerators      -- brute-force triple loops on tiny inputs; invariants
