@@ -51,8 +51,9 @@ class _DenseHIP(BaseCorrelator):
     def _run_dense(self, frames):
         torch = _torch()
         d_frames, dtype = to_device_frames(frames, self.shape, self.nframes, torch)
-        if dtype not in (np.dtype(np.uint8), np.dtype(np.int8)):
-            raise NotImplementedError("the dense i8 path takes uint8 or int8 frames, got %s" % dtype)
+        if dtype not in (np.dtype(np.uint8), np.dtype(np.int8), np.dtype(np.uint16)):
+            raise NotImplementedError("the dense i8 path takes uint8, int8 or uint16 frames, "
+                                      "got %s" % dtype)
         plan = self._plan(dtype)
         nb, n = max(self.n_bins, 0), self.nframes
         num = torch.empty((nb, n), dtype=torch.int64, device="cuda")

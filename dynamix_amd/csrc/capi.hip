@@ -80,7 +80,9 @@ struct dxg2_plan {
     std::vector<int32_t> pix_idx;
     std::vector<BinDev> bins;
     std::vector<GatherWork> gather_work;
-    std::vector<GramWork> gram_work;
+    std::vector<GramWork> gram_work;    // plain kernel: every item of uint8/int8 plans;
+                                        // HH and LL items of uint16 plans
+    std::vector<GramWork> gram_xwork;   // cross kernel: [H|L][L|H]^T items of uint16 plans
     size_t slab_bytes = 0;
 
     // device (owned)
@@ -88,10 +90,12 @@ struct dxg2_plan {
     BinDev* d_bins = nullptr;
     GatherWork* d_gwork = nullptr;
     GramWork* d_mwork = nullptr;
+    GramWork* d_xwork = nullptr;
     int8_t* d_slab = nullptr;
     int64_t* d_ugram = nullptr;
     int64_t* d_sums = nullptr;
-    int64_t* d_prefix = nullptr;  // per bin: prefix sums of S(t), N + 1 values
+    int64_t* d_psums = nullptr;   // digit planes: S_H rows then S_L rows (n_bins each)
+    int64_t* d_prefix = nullptr;  // per bin and plane: prefix sums of S(t), N + 1 values
     int64_t* d_smax = nullptr;    // per bin: max |S(t)|
     float* d_pa = nullptr;
     float* d_pg = nullptr;
@@ -149,20 +153,34 @@ void build_gram_work(dxg2_plan* p) {
     // Largest bins first so their long tiles start early (LPT).
     std::stable_sort(order.begin(), order.end(),
                      [&](int a, int b) { return p->bins[a].nkb > p->bins[b].nkb; });
-    std::vector<GramWork> raster;
-    for (int b : order) {
-        const int nkb = p->bins[b].nkb;
-        if (nkb == 0) continue;
-        for (int k0 = 0; k0 < nkb; k0 += kchunk) {
-            const int k1 = std::min(nkb, k0 + kchunk);
+    std::vector<GramWork> raster, xraster;
+    // One pass of upper-triangle tiles over k-blocks [ka, kend) of bin b, adding into
+    // output row `out`.
+    auto triangle = [&](int b, int ka, int kend, int out, int cross) {
+        auto& dst = cross ? xraster : raster;
+        for (int k0 = ka; k0 < kend; k0 += kchunk) {
+            const int k1 = std::min(kend, k0 + kchunk);
             for (int i0 = 0; i0 < nT; i0 += band)
                 for (int j = i0; j < nT; ++j)
                     for (int i = i0; i < std::min(nT, i0 + band) && i <= j; ++i)
-                        raster.push_back(GramWork{b, i, j, k0, k1, {0, 0, 0}});
+                        dst.push_back(GramWork{b, i, j, k0, k1, out, cross, 0});
+        }
+    };
+    const int nb = p->n_bins;
+    for (int b : order) {
+        const int nkb = p->bins[b].nkb;
+        if (nkb == 0) continue;
+        if (slab_planes(p->dtype) == 1) {
+            triangle(b, 0, nkb, b, 0);
+        } else {  // digit planes: H H^T, L L^T, [H|L][L|H]^T
+            triangle(b, 0, nkb, b, 0);
+            triangle(b, nkb, 2 * nkb, nb + b, 0);
+            triangle(b, 0, 2 * nkb, 2 * nb + b, 1);
         }
     }
-    GramWork pad{-1, 0, 0, 0, 0, {0, 0, 0}};
+    GramWork pad{-1, 0, 0, 0, 0, 0, 0, 0};
     p->gram_work = group > 0 ? remap_for_xcds(raster, group, pad) : raster;
+    p->gram_xwork = group > 0 ? remap_for_xcds(xraster, group, pad) : xraster;
 }
 
 // Next unused event quad of the profiling pool (created on first use).
@@ -341,7 +359,7 @@ int dxg2_plan_create(const dxg2_plan_desc* desc, dxg2_plan** out) {
         for (int kb0 = 0; kb0 < nkb; kb0 += 16)
             p->gather_work.push_back(GatherWork{b, kb0, std::min(16, nkb - kb0), 0});
         idx_off += (int64_t)nkb * KB;
-        slab_off += (int64_t)nkb * KB * p->nrows;
+        slab_off += (int64_t)slab_planes(p->dtype) * nkb * KB * p->nrows;
     }
     p->slab_bytes = (size_t)slab_off;
     build_gram_work(p);
@@ -369,13 +387,20 @@ int dxg2_plan_create(const dxg2_plan_desc* desc, dxg2_plan** out) {
     if (!rc)
         rc = upload((void**)&p->d_mwork, p->gram_work.data(),
                     p->gram_work.size() * sizeof(GramWork));
+    if (!rc)
+        rc = upload((void**)&p->d_xwork, p->gram_xwork.data(),
+                    p->gram_xwork.size() * sizeof(GramWork));
     const size_t nb_n = (size_t)std::max(p->n_bins, 1) * p->N * sizeof(int64_t);
-    if (!rc && hipMalloc(&p->d_ugram, nb_n) != hipSuccess)
+    const int planes = slab_planes(p->dtype);
+    // numerator rows: one per bin, or HH, LL and cross rows for digit planes
+    if (!rc && hipMalloc(&p->d_ugram, (planes == 2 ? 3 : 1) * nb_n) != hipSuccess)
         rc = set_error(DXG2_ENOMEM, "cannot allocate numerator workspace");
     if (!rc && hipMalloc(&p->d_sums, nb_n) != hipSuccess)
         rc = set_error(DXG2_ENOMEM, "cannot allocate sums workspace");
-    if (!rc && (hipMalloc(&p->d_prefix, (size_t)std::max(p->n_bins, 1) * (p->N + 1) *
-                                            sizeof(int64_t)) != hipSuccess ||
+    if (!rc && planes == 2 && hipMalloc(&p->d_psums, 2 * nb_n) != hipSuccess)
+        rc = set_error(DXG2_ENOMEM, "cannot allocate digit-plane sums workspace");
+    if (!rc && (hipMalloc(&p->d_prefix, (size_t)planes * std::max(p->n_bins, 1) *
+                                            (p->N + 1) * sizeof(int64_t)) != hipSuccess ||
                 hipMalloc(&p->d_smax, (size_t)std::max(p->n_bins, 1) * sizeof(int64_t)) !=
                     hipSuccess))
         rc = set_error(DXG2_ENOMEM, "cannot allocate denominator workspace");
@@ -389,8 +414,9 @@ int dxg2_plan_create(const dxg2_plan_desc* desc, dxg2_plan** out) {
 
 int dxg2_plan_destroy(dxg2_plan* p) {
     if (!p) return DXG2_OK;
-    void* bufs[] = {p->d_pix,   p->d_bins, p->d_gwork, p->d_mwork, p->d_slab,
-                    p->d_ugram, p->d_sums, p->d_prefix, p->d_smax, p->d_pa, p->d_pg, p->d_pm,
+    void* bufs[] = {p->d_pix,   p->d_bins, p->d_gwork, p->d_mwork, p->d_xwork, p->d_slab,
+                    p->d_ugram, p->d_sums, p->d_psums, p->d_prefix, p->d_smax, p->d_pa,
+                    p->d_pg, p->d_pm,
                     p->d_ework, p->d_err,  p->d_cnt,   p->d_scan,  p->d_off,
                     p->d_etimes, p->d_edata, p->d_kbwork, p->d_xs, p->d_tw,
                     p->d_power, p->d_fnum};
@@ -416,12 +442,14 @@ int dxg2_plan_query(const dxg2_plan* p, int32_t* n_bins, int64_t* pixel_counts, 
 int dxg2_plan_footprint(const dxg2_plan* p, int32_t path, size_t* bytes) {
     if (!p || !bytes) return set_error(DXG2_EINVAL, "null argument");
     const size_t nb_n = (size_t)std::max(p->n_bins, 1) * p->N * sizeof(int64_t);
-    size_t common = p->pix_idx.size() * 4 + p->bins.size() * sizeof(BinDev) + 2 * nb_n +
-                    (size_t)std::max(p->n_bins, 1) * (p->N + 2) * sizeof(int64_t);
+    const int planes = slab_planes(p->dtype);
+    size_t common = p->pix_idx.size() * 4 + p->bins.size() * sizeof(BinDev) +
+                    (planes == 2 ? 6 : 2) * nb_n +
+                    (size_t)std::max(p->n_bins, 1) * (planes * (p->N + 1) + 1) * sizeof(int64_t);
     switch (path) {
         case 0:
             *bytes = common + p->slab_bytes + p->gather_work.size() * sizeof(GatherWork) +
-                     p->gram_work.size() * sizeof(GramWork);
+                     (p->gram_work.size() + p->gram_xwork.size()) * sizeof(GramWork);
             return DXG2_OK;
         case 1: {
             int64_t mx = 0;
@@ -481,28 +509,33 @@ int dxg2_correlate_dense(dxg2_plan* p, const void* d_frames, int64_t* d_num, int
                          double* d_den, float* d_g2, void* stream) {
     if (!p) return set_error(DXG2_EINVAL, "null plan");
     if (!d_frames || !d_g2) return set_error(DXG2_EINVAL, "frames and g2 must be non-null");
-    if (p->dtype != DXG2_U8 && p->dtype != DXG2_I8)
-        return set_error(DXG2_EUNSUPPORTED, "dense i8 path supports uint8/int8 frames");
     if (p->n_bins == 0) return DXG2_OK;
     int rc = ensure_dense_buffers(p);
     if (rc) return rc;
     hipStream_t s = (hipStream_t)stream;
     int64_t* sums = d_sums ? d_sums : p->d_sums;
+    const bool wide = p->dtype == DXG2_U16;
+    // the Gram's diagonal tiles write S (uint8/int8) or S_H, S_L (uint16) here
+    int64_t* gsums = wide ? p->d_psums : sums;
     const int64_t nbn = (int64_t)p->n_bins * p->N;
     (void)hipGetLastError();
     hipEvent_t* ev = nullptr;
     if (p->profiling && (rc = next_events(p, &ev))) return rc;
     if (ev) DXG2_HIP(hipEventRecord(ev[0], s));
-    launch_fill_i64(sums, nbn, s);
-    launch_fill_i64(p->d_ugram, nbn, s);
+    launch_fill_i64(gsums, (wide ? 2 : 1) * nbn, s);
+    launch_fill_i64(p->d_ugram, (wide ? 3 : 1) * nbn, s);
     launch_gather_i8((const uint8_t*)d_frames, p->dtype, (int64_t)p->H * p->W, p->N, p->nrows,
                      p->d_pix, p->d_bins, p->d_gwork, (int)p->gather_work.size(), p->d_slab, s);
     if (ev) DXG2_HIP(hipEventRecord(ev[1], s));
-    launch_gram_i8(p->gram_tile, p->d_slab, p->d_bins, p->d_mwork, (int)p->gram_work.size(),
-                   p->N, p->nrows, p->dtype == DXG2_U8 ? 1 : 0, p->d_ugram, sums, s);
+    const int biased = p->dtype == DXG2_I8 ? 0 : 1;
+    launch_gram_i8(p->gram_tile, false, p->d_slab, p->d_bins, p->d_mwork,
+                   (int)p->gram_work.size(), p->N, p->nrows, biased, p->d_ugram, gsums, s);
+    launch_gram_i8(p->gram_tile, true, p->d_slab, p->d_bins, p->d_xwork,
+                   (int)p->gram_xwork.size(), p->N, p->nrows, biased, p->d_ugram, gsums, s);
     if (ev) DXG2_HIP(hipEventRecord(ev[2], s));
-    launch_finalize(sums, p->d_ugram, p->d_bins, p->n_bins, p->N, p->dtype == DXG2_U8 ? 1 : 0,
-                    p->d_prefix, p->d_smax, d_num, d_den, d_g2, s);
+    launch_finalize(gsums, sums, p->d_ugram, p->d_bins, p->n_bins, p->N,
+                    wide ? 2 : (p->dtype == DXG2_U8 ? 1 : 0), p->d_prefix, p->d_smax, d_num,
+                    d_den, d_g2, s);
     if (ev) DXG2_HIP(hipEventRecord(ev[3], s));
     DXG2_HIP(hipGetLastError());
     return DXG2_OK;
@@ -556,8 +589,8 @@ static int events_from_csr(dxg2_plan* p, const int32_t* times, const void* data,
     int64_t* sums = d_sums ? d_sums : p->d_sums;
     launch_event_pairs(p->dtype, times, data, offsets, p->d_pix, p->d_bins, p->d_ework,
                        (int)p->event_work.size(), p->N, nnz, p->d_ugram, sums, p->d_err, s);
-    launch_finalize(sums, p->d_ugram, p->d_bins, p->n_bins, p->N, 0, p->d_prefix, p->d_smax,
-                    d_num, d_den, d_g2, s);
+    launch_finalize(sums, sums, p->d_ugram, p->d_bins, p->n_bins, p->N, 0, p->d_prefix,
+                    p->d_smax, d_num, d_den, d_g2, s);
     DXG2_HIP(hipGetLastError());
     return DXG2_OK;
 }

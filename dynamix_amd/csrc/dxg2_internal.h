@@ -42,8 +42,15 @@ struct GramWork {        // one output tile (i, j >= i) over a k-block range
     int32_t tj;          // column tile
     int32_t kb0;
     int32_t kb1;         // exclusive
-    int32_t pad[3];
+    int32_t out;         // row of the numerator (and frame-sum) outputs it adds into
+    int32_t cross;       // 1: B operand is the other digit plane ([H|L] x [L|H]^T)
+    int32_t pad;
 };
+
+// Digit planes of a bin's slab: uint16 frames are split into a high-byte plane H
+// (k-blocks 0..nkb-1) and a low-byte plane L (k-blocks nkb..2nkb-1), each stored
+// biased by -128 like uint8.  Numerators combine as 65536 HH + 256 (HL + LH) + LL.
+inline int slab_planes(int dtype) { return dtype == DXG2_U16 ? 2 : 1; }
 
 struct EventWork {       // one workgroup of the event pair kernel (x dimension)
     int32_t bin;
@@ -62,12 +69,16 @@ void launch_fill_i64(int64_t* p, int64_t n, hipStream_t s);
 void launch_gather_i8(const uint8_t* frames, int dtype, int64_t hw, int nframes, int nrows,
                       const int32_t* pix_idx, const BinDev* bins, const GatherWork* work,
                       int n_work, int8_t* slab, hipStream_t s);
-void launch_gram_i8(int tile, const int8_t* slab, const BinDev* bins, const GramWork* work,
-                    int n_work, int nframes, int nrows, int biased, int64_t* ugram,
-                    int64_t* sums, hipStream_t s);
-void launch_finalize(const int64_t* sums, const int64_t* ugram, const BinDev* bins,
-                     int n_bins, int nframes, int biased, int64_t* prefix, int64_t* smax,
-                     int64_t* num, double* den, float* g2, hipStream_t s);
+// cross: the work items are cross-plane ([H|L] [L|H]^T) items of uint16 frames.
+void launch_gram_i8(int tile, bool cross, const int8_t* slab, const BinDev* bins,
+                    const GramWork* work, int n_work, int nframes, int nrows, int biased,
+                    int64_t* ugram, int64_t* sums, hipStream_t s);
+// mode 0: numerators as accumulated; 1: uint8 bias correction; 2: uint16 digit planes
+// (ugram holds HH, LL, cross rows; plane_sums the per-plane S_H, S_L rows, and `sums`
+// receives S = 256 S_H + S_L).
+void launch_finalize(const int64_t* plane_sums, int64_t* sums, const int64_t* ugram,
+                     const BinDev* bins, int n_bins, int nframes, int mode, int64_t* prefix,
+                     int64_t* smax, int64_t* num, double* den, float* g2, hipStream_t s);
 void launch_plain_bin(const uint8_t* frames, int dtype, int64_t hw, int nframes,
                       const int32_t* pix, int npix, float* a, float* gram, float* means,
                       float* g2_row, hipStream_t s);

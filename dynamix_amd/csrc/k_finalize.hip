@@ -15,10 +15,15 @@
 // integer below 2^53, so FP64 fused multiply-adds are exact; otherwise the lag loop
 // accumulates in 128-bit integers.  Both give the same Den.
 //
-// Bias correction (uint8 frames are stored as s = v - 128):
+// Bias correction (uint8 frames are stored as s = v - 128, padded pixels as -128):
 //   G(t,t') = U(t,t') + 128 (S(t) + S(t')) - 2^14 K_pad
 // summed over t = 0..N-1-tau:
 //   Num(tau) = U(tau) + 128 (P(N-tau) + P(N) - P(tau)) - 2^14 K_pad (N-tau).
+// uint16 frames v = 256 h + l are stored as two biased digit planes; with the plane
+// sums S_H, S_L (prefix sums P_H, P_L) the same identity gives
+//   HH = U_HH + 128 E_H - 2^14 K_pad (N-tau),  LL = U_LL + 128 E_L - 2^14 K_pad (N-tau),
+//   X  = U_X + 128 (E_H + E_L) - 2^15 K_pad (N-tau),   E = P(N-tau) + P(N) - P(tau),
+// and Num = 65536 HH + 256 X + LL (formed in 128 bits), S = 256 S_H + S_L.
 #include "dxg2_internal.h"
 
 namespace dxg2 {
@@ -35,41 +40,66 @@ __device__ __forceinline__ double i128_to_double(__int128 x) {
     return (double)hi * 18446744073709551616.0 + (double)lo;
 }
 
-// One block per bin: prefix[b][t] = sum_{u<t} S(u) for t = 0..N, smax[b] = max |S|.
-__global__ __launch_bounds__(PREP_THREADS) void sums_prep_kernel(const int64_t* __restrict__ sums,
-                                                                 int nframes,
-                                                                 int64_t* __restrict__ prefix,
-                                                                 int64_t* __restrict__ smax) {
+// Exclusive prefix scan of one value per thread over the block; returns the prefix and
+// adds the block total to `carry`.  wave_tot: PREP_THREADS / 64 words of LDS.
+__device__ __forceinline__ int64_t block_scan(int64_t v, int64_t* wave_tot, int64_t& carry) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    int64_t x = v;  // inclusive scan within the wave
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int64_t y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+    }
+    if (lane == 63) wave_tot[wave] = x;
+    __syncthreads();
+    int64_t before = carry, total = carry;
+    for (int w = 0; w < PREP_THREADS / 64; ++w) {
+        if (w < wave) before += wave_tot[w];
+        total += wave_tot[w];
+    }
+    __syncthreads();
+    carry = total;
+    return before + x - v;
+}
+
+// One block per bin.  S(t) are the rows of plane_sums; WIDE (uint16 digit planes): rows
+// b (S_H) and n_bins + b (S_L), combined into sums[b][t] = 256 S_H + S_L.  Writes the
+// exclusive prefix sums of each plane, prefix[plane * n_bins + b][t] for t = 0..N, and
+// smax[b] = max |S|.
+template <bool WIDE>
+__global__ __launch_bounds__(PREP_THREADS) void sums_prep_kernel(
+    const int64_t* __restrict__ plane_sums, int64_t* __restrict__ sums, int nframes,
+    int64_t* __restrict__ prefix, int64_t* __restrict__ smax) {
     __shared__ int64_t wave_tot[PREP_THREADS / 64];
     __shared__ int64_t wave_max[PREP_THREADS / 64];
-    const int b = blockIdx.x;
-    const int tid = threadIdx.x;
-    const int lane = tid & 63, wave = tid >> 6;
-    const int64_t* S = sums + (int64_t)b * nframes;
-    int64_t* P = prefix + (int64_t)b * (nframes + 1);
-    int64_t carry = 0, mx = 0;
+    const int b = blockIdx.x, nb = gridDim.x;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t* S0 = plane_sums + (int64_t)b * nframes;
+    const int64_t* S1 = plane_sums + (int64_t)(nb + b) * nframes;
+    int64_t* P0 = prefix + (int64_t)b * (nframes + 1);
+    int64_t* P1 = prefix + (int64_t)(nb + b) * (nframes + 1);
+    int64_t c0 = 0, c1 = 0, mx = 0;
     for (int t0 = 0; t0 < nframes; t0 += PREP_THREADS) {
         const int t = t0 + tid;
-        const int64_t v = t < nframes ? S[t] : 0;
-        mx = max(mx, v < 0 ? -v : v);
-        // inclusive scan within the wave
-        int64_t x = v;
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            const int64_t y = __shfl_up(x, d, 64);
-            if (lane >= d) x += y;
+        const int64_t v0 = t < nframes ? S0[t] : 0;
+        const int64_t e0 = block_scan(v0, wave_tot, c0);
+        if (t < nframes) P0[t] = e0;
+        int64_t v = v0;
+        if (WIDE) {
+            const int64_t v1 = t < nframes ? S1[t] : 0;
+            const int64_t e1 = block_scan(v1, wave_tot, c1);
+            if (t < nframes) {
+                P1[t] = e1;
+                v = 256 * v0 + v1;
+                sums[(int64_t)b * nframes + t] = v;
+            }
         }
-        if (lane == 63) wave_tot[wave] = x;
-        __syncthreads();
-        int64_t before = carry;
-        for (int w = 0; w < wave; ++w) before += wave_tot[w];
-        int64_t total = carry;
-        for (int w = 0; w < PREP_THREADS / 64; ++w) total += wave_tot[w];
-        if (t < nframes) P[t] = before + x - v;  // exclusive prefix
-        carry = total;
-        __syncthreads();
+        mx = max(mx, v < 0 ? -v : v);
     }
-    if (tid == 0) P[nframes] = carry;
+    if (tid == 0) {
+        P0[nframes] = c0;
+        if (WIDE) P1[nframes] = c1;
+    }
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) mx = max(mx, __shfl_xor(mx, d, 64));
     if (lane == 0) wave_max[wave] = mx;
@@ -116,12 +146,19 @@ __device__ __forceinline__ void den_lags(const int64_t* __restrict__ S, int nfra
     }
 }
 
+// sum_{t < N - tau} X(t) + sum_{t >= tau} X(t) from the exclusive prefix sums P of X.
+__device__ __forceinline__ int64_t edge_sums(const int64_t* P, int nframes, int tau) {
+    return P[nframes - tau] + P[nframes] - P[tau];
+}
+
 // grid (ceil(N / FIN_SPAN), n_bins), FIN_THREADS threads.  NumT is int64_t (exact
-// numerators of the Gram and event paths) or double (FFT path, no bias).
+// numerators of the Gram and event paths) or double (FFT path).  mode 0: numerators as
+// accumulated; 1: uint8 bias correction; 2: uint16 digit planes (ugram rows b, n_bins + b,
+// 2 n_bins + b hold the biased HH, LL and cross sums).
 template <typename NumT>
 __global__ __launch_bounds__(FIN_THREADS) void finalize_kernel(
     const int64_t* __restrict__ sums, const NumT* __restrict__ ugram,
-    const BinDev* __restrict__ bins, int nframes, int biased,
+    const BinDev* __restrict__ bins, int nframes, int mode,
     const int64_t* __restrict__ prefix, const int64_t* __restrict__ smax,
     NumT* __restrict__ num, double* __restrict__ den, float* __restrict__ g2) {
     __shared__ __attribute__((aligned(16))) unsigned char stage[(2 * FIN_CH + FIN_SPAN) * 8];
@@ -148,39 +185,59 @@ __global__ __launch_bounds__(FIN_THREADS) void finalize_kernel(
     }
 
     const BinDev bd = bins[b];
+    const int nb = gridDim.y;
+    const int64_t kpad = (int64_t)bd.nkb * KB;  // pixels per digit plane, with padding
 #pragma unroll
     for (int j = 0; j < FIN_LAGS; ++j) {
         const int tau = tau0 + tid + FIN_THREADS * j;
         if (tau >= nframes) continue;
-        NumT n = ugram[(int64_t)b * nframes + tau];
-        if (biased)
-            n += (NumT)(128 * (P[nframes - tau] + P[nframes] - P[tau]) -
-                        (int64_t)16384 * ((int64_t)bd.nkb * KB) * (nframes - tau));
         const int64_t o = (int64_t)b * nframes + tau;
+        const int64_t pairs = kpad * (nframes - tau);  // padded pixel pairs on the diagonal
+        double nd;
+        NumT n;
+        if (mode == 2) {
+            // biased digit products: (h-128)(h'-128) etc.; padding entries are -128 too
+            const int64_t* P1 = prefix + (int64_t)(nb + b) * (nframes + 1);
+            const int64_t eh = edge_sums(P, nframes, tau), el = edge_sums(P1, nframes, tau);
+            const int64_t stride = (int64_t)nb * nframes;
+            const int64_t hh = (int64_t)ugram[o] + 128 * eh - 16384 * pairs;
+            const int64_t ll = (int64_t)ugram[o + stride] + 128 * el - 16384 * pairs;
+            const int64_t x = (int64_t)ugram[o + 2 * stride] + 128 * (eh + el) - 32768 * pairs;
+            const __int128 v = (__int128)hh * 65536 + (__int128)x * 256 + ll;
+            n = (NumT)(int64_t)v;
+            nd = i128_to_double(v);
+        } else {
+            n = ugram[o];
+            if (mode == 1) n += (NumT)(128 * edge_sums(P, nframes, tau) - 16384 * pairs);
+            nd = (double)n;
+        }
         if (num) num[o] = n;
         if (den) den[o] = d[j];
-        g2[o] = (float)(bd.scale * ((double)n / d[j]));
+        g2[o] = (float)(bd.scale * (nd / d[j]));
     }
 }
 
-void launch_finalize(const int64_t* sums, const int64_t* ugram, const BinDev* bins,
-                     int n_bins, int nframes, int biased, int64_t* prefix, int64_t* smax,
-                     int64_t* num, double* den, float* g2, hipStream_t s) {
+void launch_finalize(const int64_t* plane_sums, int64_t* sums, const int64_t* ugram,
+                     const BinDev* bins, int n_bins, int nframes, int mode, int64_t* prefix,
+                     int64_t* smax, int64_t* num, double* den, float* g2, hipStream_t s) {
     if (n_bins <= 0 || nframes <= 0) return;
-    sums_prep_kernel<<<dim3((unsigned)n_bins), dim3(PREP_THREADS), 0, s>>>(sums, nframes,
-                                                                          prefix, smax);
+    if (mode == 2)
+        sums_prep_kernel<true><<<dim3((unsigned)n_bins), dim3(PREP_THREADS), 0, s>>>(
+            plane_sums, sums, nframes, prefix, smax);
+    else
+        sums_prep_kernel<false><<<dim3((unsigned)n_bins), dim3(PREP_THREADS), 0, s>>>(
+            sums, sums, nframes, prefix, smax);
     dim3 grid((unsigned)((nframes + FIN_SPAN - 1) / FIN_SPAN), (unsigned)n_bins);
-    finalize_kernel<int64_t><<<grid, dim3(FIN_THREADS), 0, s>>>(sums, ugram, bins, nframes,
-                                                                biased, prefix, smax, num, den,
-                                                                g2);
+    finalize_kernel<int64_t><<<grid, dim3(FIN_THREADS), 0, s>>>(sums, ugram, bins, nframes, mode,
+                                                                prefix, smax, num, den, g2);
 }
 
 void launch_finalize_f64(const int64_t* sums, const double* fnum, const BinDev* bins, int n_bins,
                          int nframes, int64_t* prefix, int64_t* smax, double* num, double* den,
                          float* g2, hipStream_t s) {
     if (n_bins <= 0 || nframes <= 0) return;
-    sums_prep_kernel<<<dim3((unsigned)n_bins), dim3(PREP_THREADS), 0, s>>>(sums, nframes,
-                                                                          prefix, smax);
+    sums_prep_kernel<false><<<dim3((unsigned)n_bins), dim3(PREP_THREADS), 0, s>>>(
+        sums, const_cast<int64_t*>(sums), nframes, prefix, smax);
     dim3 grid((unsigned)((nframes + FIN_SPAN - 1) / FIN_SPAN), (unsigned)n_bins);
     finalize_kernel<double><<<grid, dim3(FIN_THREADS), 0, s>>>(sums, fnum, bins, nframes, 0,
                                                                prefix, smax, num, den, g2);

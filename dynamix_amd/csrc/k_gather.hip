@@ -1,4 +1,5 @@
-// qmask gather: frames (N, H, W) -> per-bin i8 slabs slab[kb][t][64].
+// qmask gather: frames (N, H, W) -> per-bin i8 slabs slab[kb][t][64] (uint16 frames:
+// two digit planes).
 //
 // The reference gathers one bin at a time on the host (correlator/dense.py:48-49,
 // dense.py:317-319).  Here one launch covers every bin.  A wave owns one k-block (64
@@ -48,15 +49,28 @@ __device__ __forceinline__ uint32_t quad_bcast(uint32_t x) {
     return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, J * 0x55, 0xF, 0xF, false);
 }
 
+// 4x4 byte transpose inside a lane quad: lane 4q+i holds bytes of frames t..t+3 of pixel
+// 4q+i in x; returns the dword of pixels 4q..4q+3 for frame t+i (i = lane & 3).
+__device__ __forceinline__ uint32_t quad_transpose(uint32_t x, uint32_t sel_lo, uint32_t sel_hi) {
+    const uint32_t y0 = quad_bcast<0>(x);
+    const uint32_t y1 = quad_bcast<1>(x);
+    const uint32_t y2 = quad_bcast<2>(x);
+    const uint32_t y3 = quad_bcast<3>(x);
+    return __builtin_amdgcn_perm(y1, y0, sel_lo) | __builtin_amdgcn_perm(y3, y2, sel_hi);
+}
+
 // Work layout: blockIdx.x -> GatherWork (bin, first k-block, k-block count <= 16),
 // blockIdx.y -> 64-frame tile.  Wave w handles k-blocks kb0 + w, kb0 + w + 4, ...
-// BIAS (uint8 input): bytes are stored as v ^ 0x80 = v - 128 (i8 MFMA operands);
-// padded list entries (index -1) read as 0 and become -128 the same way.
-template <bool BIAS>
+// BIAS (uint8 and uint16 input): bytes are stored as v ^ 0x80 = v - 128 (i8 MFMA
+// operands); padded list entries (index -1) read as 0 and become -128 the same way.
+// T = uint16_t writes the high bytes to k-block kb and the low bytes to k-block
+// kb + nkb of the bin (two digit planes).
+template <typename T, bool BIAS>
 __global__ __launch_bounds__(256) void gather_i8_kernel(
-    const uint8_t* __restrict__ frames, int64_t hw, int nframes, int nrows,
+    const T* __restrict__ frames, int64_t hw, int nframes, int nrows,
     const int32_t* __restrict__ pix_idx, const BinDev* __restrict__ bins,
     const GatherWork* __restrict__ work, int8_t* __restrict__ slab) {
+    constexpr bool WIDE = sizeof(T) == 2;
     const GatherWork w = work[blockIdx.x];
     const BinDev bd = bins[w.bin];
     const int lane = threadIdx.x & 63;
@@ -68,28 +82,33 @@ __global__ __launch_bounds__(256) void gather_i8_kernel(
     const uint32_t sel_hi = 0x00000c0cu | ((uint32_t)qi << 16) | ((uint32_t)(4 + qi) << 24);
     const int t0 = blockIdx.y * GATHER_TF;
     const int tend = min(nframes, t0 + GATHER_TF);
+    const int64_t plane = (int64_t)bd.nkb * nrows * KB;  // bytes between digit planes
 
     for (int kb = w.kb0 + wave; kb < w.kb0 + w.nkb; kb += 4) {
         const int p = pix_idx[bd.idx_off + (int64_t)kb * KB + lane];
         const int pc = p >= 0 ? p : 0;
-        const uint32_t keep = p >= 0 ? 0xFFu : 0u;
+        const uint32_t keep = p >= 0 ? 0xFFFFu : 0u;
         int8_t* dst = slab + bd.slab_off + (int64_t)kb * nrows * KB + q4;
         for (int t = t0; t < tend; t += 4) {
-            uint32_t x = 0;
+            uint32_t xl = 0, xh = 0;
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 const int tj = t + j < nframes ? t + j : nframes - 1;
-                const uint32_t b = (uint32_t)frames[(int64_t)tj * hw + pc] & keep;
-                x |= b << (8 * j);
+                const uint32_t v = (uint32_t)frames[(int64_t)tj * hw + pc] & keep;
+                xl |= (v & 0xFFu) << (8 * j);
+                if (WIDE) xh |= (v >> 8) << (8 * j);
             }
-            const uint32_t y0 = quad_bcast<0>(x);
-            const uint32_t y1 = quad_bcast<1>(x);
-            const uint32_t y2 = quad_bcast<2>(x);
-            const uint32_t y3 = quad_bcast<3>(x);
-            uint32_t out = __builtin_amdgcn_perm(y1, y0, sel_lo) | __builtin_amdgcn_perm(y3, y2, sel_hi);
-            if (BIAS) out ^= 0x80808080u;
-            if (t + qi < nframes)
-                *reinterpret_cast<uint32_t*>(dst + (int64_t)(t + qi) * KB) = out;
+            uint32_t lo = quad_transpose(xl, sel_lo, sel_hi);
+            if (BIAS) lo ^= 0x80808080u;
+            if (WIDE) {
+                const uint32_t hi = quad_transpose(xh, sel_lo, sel_hi) ^ 0x80808080u;
+                if (t + qi < nframes) {
+                    *reinterpret_cast<uint32_t*>(dst + (int64_t)(t + qi) * KB) = hi;
+                    *reinterpret_cast<uint32_t*>(dst + plane + (int64_t)(t + qi) * KB) = lo;
+                }
+            } else if (t + qi < nframes) {
+                *reinterpret_cast<uint32_t*>(dst + (int64_t)(t + qi) * KB) = lo;
+            }
         }
     }
 }
@@ -99,12 +118,16 @@ void launch_gather_i8(const uint8_t* frames, int dtype, int64_t hw, int nframes,
                       int n_work, int8_t* slab, hipStream_t s) {
     if (n_work <= 0 || nframes <= 0) return;
     dim3 grid((unsigned)n_work, (unsigned)((nframes + GATHER_TF - 1) / GATHER_TF));
-    if (dtype == DXG2_U8)
-        gather_i8_kernel<true><<<grid, dim3(256), 0, s>>>(frames, hw, nframes, nrows, pix_idx,
-                                                         bins, work, slab);
+    if (dtype == DXG2_U16)
+        gather_i8_kernel<uint16_t, true><<<grid, dim3(256), 0, s>>>(
+            reinterpret_cast<const uint16_t*>(frames), hw, nframes, nrows, pix_idx, bins, work,
+            slab);
+    else if (dtype == DXG2_U8)
+        gather_i8_kernel<uint8_t, true><<<grid, dim3(256), 0, s>>>(frames, hw, nframes, nrows,
+                                                                  pix_idx, bins, work, slab);
     else
-        gather_i8_kernel<false><<<grid, dim3(256), 0, s>>>(frames, hw, nframes, nrows, pix_idx,
-                                                          bins, work, slab);
+        gather_i8_kernel<uint8_t, false><<<grid, dim3(256), 0, s>>>(frames, hw, nframes, nrows,
+                                                                   pix_idx, bins, work, slab);
 }
 
 }  // namespace dxg2

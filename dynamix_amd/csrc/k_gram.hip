@@ -23,6 +23,18 @@
 // k-range, so they also produce the per-frame bin sums S_b(t) (densecorrelator.cl:
 // 65-150) from the staged bytes with v_dot4 -- no extra pass over the data.
 //
+// uint16 frames arrive as two biased digit planes H, L (k-blocks [0, nkb) and
+// [nkb, 2 nkb) of the bin).  Work items then come in three kinds, each adding into its
+// own output row: H H^T over [0, nkb), L L^T over [nkb, 2 nkb) (both in the plain
+// kernel), and the cross product [H|L] [L|H]^T = H L^T + L H^T over [0, 2 nkb) in the
+// CROSS kernel, whose B operand reads k-block kb +- nkb.  The cross Gram is symmetric,
+// so the same upper-triangle tiles cover it; its diagonal tiles run as full tiles
+// (A != B) and the epilogue drops their lags below the diagonal.
+//
+// The loop is kept free of stores and of extra control flow: the compiler's tracking of
+// which LDS stage each DMA writes is easily lost (it then waits for every DMA in flight
+// before an LDS read), so the cross-plane variant is a separate kernel.
+//
 // LDS diagonal sums stay in 32-bit words: every int32 value v is split as
 // v = hi * 2^16 + lo with lo in [0, 2^16) and hi in [-2^15, 2^15); a diagonal of a
 // T <= 256 tile collects at most 256 values, so sum(lo) < 2^24 and |sum(hi)| < 2^23
@@ -84,13 +96,14 @@ struct GramLds {
     unsigned int* diag;
 };
 
-template <int WM, int WN, bool DIAG>
+template <int WM, int WN, bool DIAG, bool CROSS>
 struct GramTile {
     using S = GramShape<WM, WN>;
     static constexpr int L = DIAG ? S::LOADS : 2 * S::LOADS;  // DMA per thread per stage
     const int8_t* __restrict__ slab_bin;
     int kb0, kb1, row0, col0, nframes, nrows;
     int tid, lane, wr, wc;
+    int nkbp;  // CROSS: k-blocks per digit plane (B reads kb + nkbp or kb - nkbp)
     GramLds<WM, WN> lds;
     v16i acc[WM][WN];
     int rowsum;
@@ -102,7 +115,9 @@ struct GramTile {
     template <int SLOT>
     __device__ __forceinline__ void issue(int kb) {
         dma_panel<WM, WN>(slab_bin, kb, row0, nrows, lds.a[SLOT], tid);
-        if (!DIAG) dma_panel<WM, WN>(slab_bin, kb, col0, nrows, lds.b[SLOT], tid);
+        if (!DIAG)
+            dma_panel<WM, WN>(slab_bin, CROSS ? (kb < nkbp ? kb + nkbp : kb - nkbp) : kb, col0,
+                              nrows, lds.b[SLOT], tid);
     }
 
     // Consume stage kb from ring slot SLOT; refill slot (SLOT + 3) % 4 with kb + 3.
@@ -216,7 +231,7 @@ struct GramTile {
     }
 };
 
-template <int WM, int WN>
+template <int WM, int WN, bool CROSS>
 __global__ __launch_bounds__(256) void gram_i8_kernel(const int8_t* __restrict__ slab,
                                                       const BinDev* __restrict__ bins,
                                                       const GramWork* __restrict__ work,
@@ -237,31 +252,43 @@ __global__ __launch_bounds__(256) void gram_i8_kernel(const int8_t* __restrict__
     const GramLds<WM, WN> lds{{a0, a1, a2, a3}, {b0, b1, b2, b3}, sDiag};
     const int tid = threadIdx.x;
     const int wave = tid >> 6;
-    int64_t* out = ugram + (int64_t)w.bin * nframes;
-    int64_t* srow = sums + (int64_t)w.bin * nframes;
-    if (w.ti == w.tj) {
-        GramTile<WM, WN, true> t{slab + bd.slab_off, w.kb0, w.kb1, w.ti * S::TM, w.tj * S::TN,
-                                 nframes, nrows, tid, tid & 63, wave >> 1, wave & 1, lds};
+    int64_t* out = ugram + (int64_t)w.out * nframes;
+    int64_t* srow = sums + (int64_t)w.out * nframes;
+    if (!CROSS && w.ti == w.tj) {
+        GramTile<WM, WN, true, false> t{slab + bd.slab_off, w.kb0, w.kb1, w.ti * S::TM,
+                                        w.tj * S::TN, nframes, nrows, tid, tid & 63,
+                                        wave >> 1, wave & 1, 0, lds};
         t.run(biased, out, srow);
     } else {
-        GramTile<WM, WN, false> t{slab + bd.slab_off, w.kb0, w.kb1, w.ti * S::TM, w.tj * S::TN,
-                                  nframes, nrows, tid, tid & 63, wave >> 1, wave & 1, lds};
+        GramTile<WM, WN, false, CROSS> t{slab + bd.slab_off, w.kb0, w.kb1, w.ti * S::TM,
+                                         w.tj * S::TN, nframes, nrows, tid, tid & 63,
+                                         wave >> 1, wave & 1, bd.nkb, lds};
         t.run(biased, out, srow);
     }
 }
 
 int gram_tile_size(int variant) { return variant == 1 ? 128 : 256; }
 
-void launch_gram_i8(int tile, const int8_t* slab, const BinDev* bins, const GramWork* work,
-                    int n_work, int nframes, int nrows, int biased, int64_t* ugram,
-                    int64_t* sums, hipStream_t s) {
+void launch_gram_i8(int tile, bool cross, const int8_t* slab, const BinDev* bins,
+                    const GramWork* work, int n_work, int nframes, int nrows, int biased,
+                    int64_t* ugram, int64_t* sums, hipStream_t s) {
     if (n_work <= 0) return;
-    if (tile == 128)
-        gram_i8_kernel<2, 2><<<dim3((unsigned)n_work), dim3(256), 0, s>>>(
-            slab, bins, work, nframes, nrows, biased, ugram, sums);
-    else
-        gram_i8_kernel<4, 4><<<dim3((unsigned)n_work), dim3(256), 0, s>>>(
-            slab, bins, work, nframes, nrows, biased, ugram, sums);
+    const dim3 grid((unsigned)n_work), block(256);
+    if (tile == 128) {
+        if (cross)
+            gram_i8_kernel<2, 2, true><<<grid, block, 0, s>>>(slab, bins, work, nframes, nrows,
+                                                              biased, ugram, sums);
+        else
+            gram_i8_kernel<2, 2, false><<<grid, block, 0, s>>>(slab, bins, work, nframes, nrows,
+                                                               biased, ugram, sums);
+    } else {
+        if (cross)
+            gram_i8_kernel<4, 4, true><<<grid, block, 0, s>>>(slab, bins, work, nframes, nrows,
+                                                              biased, ugram, sums);
+        else
+            gram_i8_kernel<4, 4, false><<<grid, block, 0, s>>>(slab, bins, work, nframes, nrows,
+                                                               biased, ugram, sums);
+    }
 }
 
 }  // namespace dxg2

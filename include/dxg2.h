@@ -77,6 +77,9 @@ int dxg2_plan_footprint(const dxg2_plan* plan, int32_t path, size_t* bytes);
 /* Dense batched path: qmask gather into per-bin i8 slabs (+ S_b(t)), one grouped
  * i8-MFMA Gram launch over every bin with the diagonal (tau) reduction fused, exact
  * denominator and FP64 normalisation.  d_frames: (nframes, H, W) of the plan dtype.
+ * uint16 frames are byte-split into two i8 digit planes and the numerator is combined
+ * as 65536 HH + 256 (HL + LH) + LL from three Gram passes in the same launch; num is
+ * exact while it stays below 2^63 (g2 is formed from the exact 128-bit value).
  * Replaces MatMulCorrelator.correlate (dense.py:128-142, numerator dense.py:32-70) and
  * DenseCorrelator.correlate (dense.py:223-272 with densecorrelator.cl:6-176). */
 int dxg2_correlate_dense(dxg2_plan* plan, const void* d_frames, int64_t* d_num,

@@ -92,6 +92,37 @@ def test_dense_denominator_int128_path():
     _check_exact(c, g2, og.dense_g2(f, q))
 
 
+@pytest.mark.parametrize("tile", [256, 128])
+@pytest.mark.parametrize("n,shape,hi", [
+    (7, (5, 6), 65535),       # fewer frames than one tile, full 16-bit range
+    (300, (16, 20), 700),     # several tiles, values across the byte boundary
+    (520, (24, 30), 65535),   # 256 * 2 + 8 frames, full range
+])
+def test_dense_u16_byte_split_vs_oracle(n, shape, hi, tile):
+    """uint16 frames: two biased digit planes, HH + cross + LL Gram passes."""
+    _announce("dense u16 n=%d shape=%s hi=%d tile=%d" % (n, shape, hi, tile))
+    rng = np.random.default_rng(n + hi)
+    f = rng.integers(0, hi + 1, size=(n,) + shape).astype(np.uint16)
+    f[rng.random(f.shape) < 0.5] = 0  # sparse, as photon counts are
+    q = rng.integers(0, 4, size=shape).astype(np.int32)
+    for b in range(1, 4):
+        q.flat[b] = b
+    c, g2 = _run(f, q, extra_options={"tile": tile})
+    _check_exact(c, g2, og.dense_g2(f, q))
+
+
+def test_dense_u16_speckle_with_hot_pixels():
+    """Speckle counts with a hot-pixel tail (mean 40), 20 linear-width annuli."""
+    _announce("dense u16 speckle hot pixels")
+    from dynamix_amd import synth
+    shape, n = (48, 52), 600
+    mm = synth.mean_map(shape, 0.5, hot_fraction=0.01, hot_mean=40.0, seed=3)
+    f = synth.speckle_frames(shape, n, 0.5, 50.0, 11, np.uint16, mean_map_=mm)
+    q = synth.annular_qmask(shape, 20, widths="linear")
+    c, g2 = _run(f, q)
+    _check_exact(c, g2, og.dense_g2(f, q))
+
+
 def test_dense_i8_signed_values():
     _announce("dense i8 signed")
     rng = np.random.default_rng(2)
