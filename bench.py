@@ -17,6 +17,11 @@ Printed as one JSON line.  Besides the metric it reports
                 loop, restated in oracle/) on one bin of the same stack, all host cores;
   plain_fp32    this library's one-bin-per-launch FP32 comparator on the same stack;
   fft           the FFT path (FP32 Stockham power spectra) on the same stack;
+  uint16        the dense path on 2162 x 2068 x 5,000 uint16 frames (1 % hot pixels),
+                50 linear-width bins: byte-split digit planes, useful and executed ops;
+  serving       the continuous-batching loop: 200 seeded jobs arriving at 20/s (shapes
+                256^2, 514x1030, 1030x1065; 1k-20k frames; 10-100 bins; 70% dense
+                uint8 / 30% events): jobs/s, p50/p99 latency;
   events        the event path on 1030 x 1065 x 100,000 frames at 1e-3 counts/pixel/frame
                 (generated as events on the GPU): pixel-major input, and frame-sorted
                 (pixel, frame, count) input including its conversion.
@@ -173,6 +178,73 @@ def bench_events(seed):
             "coo_equals_csr": same, "input_flags": flags.value, "generate_s": gen_s}
 
 
+def bench_uint16(seed, reps=2):
+    """Dense path on the uint16 workload (byte-split digit planes)."""
+    import numpy as np
+    import torch
+    from dynamix_amd import _lib, synth
+    from dynamix_amd.correlator.common import Plan
+    wl = synth.WORKLOADS["uint16"]
+    shape, n = wl["shape"], wl["nframes"]
+    q = synth.workload_qmask("uint16")
+    frames = synth.workload_frames_device("uint16", seed)
+    plan = Plan(shape, n, q, np.uint16)
+    nb = plan.n_bins
+    num = torch.empty((nb, n), dtype=torch.int64, device="cuda")
+    sums = torch.empty_like(num)
+    den = torch.empty((nb, n), dtype=torch.float64, device="cuda")
+    g2 = torch.empty((nb, n), dtype=torch.float32, device="cuda")
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+    def step():
+        _lib.check(plan.lib.dxg2_correlate_dense(plan.handle, _lib.ptr(frames), _lib.ptr(num),
+                                                 _lib.ptr(sums), _lib.ptr(den), _lib.ptr(g2), st))
+    step()
+    torch.cuda.synchronize()
+    plan.set_profiling(True)
+    ms = _timed(step, reps, torch)
+    gather_ms, gram_ms, fin_ms, _, _ = plan.stage_times()
+    npix = int(plan.pixel_counts[:nb].sum())
+    useful = 2.0 * npix * n * (n + 1) / 2.0
+    executed = 4.0 * useful  # HH, LL and the cross product over twice the k-range
+    assert bool((den > 0).any().item()) and torch.isfinite(g2[den > 0]).all().item()
+    out = {"workload": "uint16: %dx%d x %d frames uint16, mean %g, %g%% hot pixels at %g, "
+                       "%d linear-width bins, %d binned px"
+                       % (shape[0], shape[1], n, wl["mean"], 100 * wl["hot_fraction"],
+                          wl["hot_mean"], nb, npix),
+           "ms": ms, "value": npix * n / (ms * 1e-3), "unit": "pixel*frames/s",
+           "stage_ms": {"gather": gather_ms, "gram": gram_ms, "finalise": fin_ms},
+           "gram_useful_TOPs": useful / (gram_ms * 1e-3) / 1e12,
+           "gram_executed_TOPs": executed / (gram_ms * 1e-3) / 1e12,
+           "gram_executed_frac_of_peak": executed / (gram_ms * 1e-3) / 1e12 / INT8_PEAK_TOPS}
+    plan.close()
+    return out
+
+
+def bench_serving(seed, n_jobs=200, rate=20.0):
+    """Serving loop: seeded Poisson arrivals of mixed dense / event jobs."""
+    import torch
+    from dynamix_amd import serving
+    t0 = time.perf_counter()
+    pool = serving.InputPool(serving.SERVE_SHAPES, 20000, seed)
+    pool_s = time.perf_counter() - t0
+    specs = serving.make_jobs(n_jobs, rate, seed + 1)
+    srv = serving.Server(pool)
+    try:
+        st = srv.run(specs)
+    finally:
+        srv.close()
+    recs = st.pop("records")
+    dense_ops = sum(r.ops for r in recs if r.spec.layout == "dense")
+    st.update({"arrival_rate_per_s": rate, "pool_GB": pool.nbytes() / 1e9, "pool_s": pool_s,
+               "streams": len(srv.streams), "dense_gram_TOPs": dense_ops / st["wall_s"] / 1e12,
+               "jobs": "%d jobs: shapes %s, 1k-20k frames, 10-100 bins, 70%% dense uint8 / 30%% "
+                       "events" % (n_jobs, list(serving.SERVE_SHAPES))})
+    del pool
+    torch.cuda.empty_cache()
+    return st
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -183,6 +255,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-fft", action="store_true")
     ap.add_argument("--no-events", action="store_true")
+    ap.add_argument("--no-uint16", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--seed", type=int, default=2026)
     ap.add_argument("--pmc-copy-gib", type=float, default=0.0,
                     help="after the run, copy this many GiB device to device (a kernel of "
@@ -204,8 +278,7 @@ def main():
     shape, n = wl["shape"], wl["nframes"]
     q = synth.workload_qmask(args.workload)
     log("generating %s frames %s x %d on the GPU" % (args.workload, shape, n))
-    frames = synth.speckle_frames_device(shape, n, wl["mean"], wl["tau_c"], args.seed,
-                                         np.dtype(wl["dtype"]))
+    frames = synth.workload_frames_device(args.workload, args.seed)
     torch.cuda.synchronize()
     plan = Plan(shape, n, q, np.dtype(wl["dtype"]))
     nb = plan.n_bins
@@ -290,12 +363,24 @@ def main():
     if not args.no_cpu:
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(frames, q)
-    if not args.no_events:
+    if not (args.no_events and args.no_uint16 and args.no_serving):
         del frames, num, sums, den, g2
         plan.close()
         torch.cuda.empty_cache()
+    if not args.no_uint16 and args.workload != "uint16":
+        log("uint16 path")
+        out["uint16"] = bench_uint16(args.seed)
+        torch.cuda.empty_cache()
+    if not args.no_events:
         log("event path")
         out["events"] = bench_events(args.seed)
+        torch.cuda.empty_cache()
+    if not args.no_serving:
+        log("serving loop")
+        out["serving"] = bench_serving(args.seed)
+        s = out["serving"]
+        log("serving: %.1f jobs/s, p50 %.0f ms, p99 %.0f ms" %
+            (s["jobs_per_s"], s["latency_p50_ms"], s["latency_p99_ms"]))
     if args.pmc_copy_gib > 0:
         nbytes = int(args.pmc_copy_gib * (1 << 30))
         src = torch.ones(nbytes, dtype=torch.uint8, device="cuda")

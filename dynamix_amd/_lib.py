@@ -43,13 +43,21 @@ class PlanDesc(ctypes.Structure):
                 ("n_scale_factors", c_int), ("flags", ctypes.c_uint32)]
 
 
+ALLOC_FN = ctypes.CFUNCTYPE(c_void_p, c_size_t, c_void_p, c_void_p)
+FREE_FN = ctypes.CFUNCTYPE(None, c_void_p, c_void_p, c_void_p)
+
 _SIGNATURES = {
+    "dxg2_set_allocator": (c_int, [ALLOC_FN, FREE_FN, c_void_p]),
     "dxg2_plan_create": (c_int, [ctypes.POINTER(PlanDesc), ctypes.POINTER(c_void_p)]),
     "dxg2_plan_destroy": (c_int, [c_void_p]),
+    "dxg2_plan_destroy_async": (c_int, [c_void_p, c_void_p]),
     "dxg2_plan_query": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "dxg2_plan_footprint": (c_int, [c_void_p, c_int, ctypes.POINTER(c_size_t)]),
     "dxg2_correlate_dense": (c_int, [c_void_p] * 7),
     "dxg2_correlate_dense_plain": (c_int, [c_void_p] * 4),
+    "dxg2_batch_create": (c_int, [ctypes.POINTER(c_void_p)]),
+    "dxg2_batch_destroy": (c_int, [c_void_p]),
+    "dxg2_batch_correlate_dense": (c_int, [c_void_p, c_int] + [c_void_p] * 7),
     "dxg2_correlate_fft": (c_int, [c_void_p] * 7),
     "dxg2_correlate_events_csr": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_i64,
                                           c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
@@ -63,6 +71,9 @@ _SIGNATURES = {
     "dxg2_synth_events": (c_int, [c_void_p, c_int, c_int, c_int, c_double, c_u64, c_int,
                                   c_void_p, c_void_p, c_void_p, c_i64, ctypes.POINTER(c_i64),
                                   c_void_p]),
+    "dxg2_events_window": (c_int, [c_void_p, c_void_p, c_void_p, c_i64, c_i64, c_int, c_int,
+                                   c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_i64,
+                                   c_void_p]),
     "dxg2_events_to_coo": (c_int, [c_void_p, c_void_p, c_void_p, c_i64, c_int, c_int, c_int,
                                    c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     "dxg2_last_error": (ctypes.c_char_p, []),

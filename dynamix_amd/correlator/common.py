@@ -141,8 +141,16 @@ class Plan(object):
         return list(t)
 
     def close(self):
+        """Release the plan (waits for the device)."""
         if getattr(self, "handle", None):
             self.lib.dxg2_plan_destroy(self.handle)
+            self.handle = None
+
+    def close_async(self, stream):
+        """Release the plan's device memory in stream order on `stream` (a torch stream
+        on which every call on the plan was issued or has completed)."""
+        if getattr(self, "handle", None):
+            self.lib.dxg2_plan_destroy_async(self.handle, ctypes.c_void_p(stream.cuda_stream))
             self.handle = None
 
     def __del__(self):

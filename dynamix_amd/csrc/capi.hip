@@ -15,10 +15,16 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
 #include "dxg2_internal.h"
+
+#include <algorithm>
+#include <memory>
+#include <mutex>
 
 using namespace dxg2;
 
@@ -42,6 +48,60 @@ int set_error(int code, const char* fmt, ...) {
         if (e_ != hipSuccess)                                                              \
             return set_error(DXG2_EHIP, "%s failed: %s", #call, hipGetErrorString(e_));     \
     } while (0)
+
+// Device memory of plans and batches comes from an allocator: by default hipMalloc /
+// hipFree (which waits for every kernel in flight, on every stream), or the caller's
+// (dxg2_set_allocator), e.g. a caching allocator whose frees never touch the device.
+// Every plan and batch keeps the allocator it was created with.
+struct DevAllocator {
+    dxg2_alloc_fn alloc = nullptr;
+    dxg2_free_fn free = nullptr;
+    void* ctx = nullptr;
+};
+
+std::mutex g_alloc_mu;
+DevAllocator g_alloc;
+
+DevAllocator current_allocator() {
+    std::lock_guard<std::mutex> lock(g_alloc_mu);
+    return g_alloc;
+}
+
+hipError_t dev_alloc(const DevAllocator& a, void** ptr, size_t bytes, hipStream_t s) {
+    bytes = std::max<size_t>(bytes, 1);
+    if (!a.alloc) return hipMalloc(ptr, bytes);
+    *ptr = a.alloc(bytes, (void*)s, a.ctx);
+    return *ptr ? hipSuccess : hipErrorOutOfMemory;
+}
+
+void dev_free(const DevAllocator& a, void* ptr, hipStream_t s) {
+    if (!ptr) return;
+    if (a.free)
+        a.free(ptr, (void*)s, a.ctx);
+    else
+        (void)hipFree(ptr);
+}
+
+// Plan set-up (workspace allocation and the upload of index lists and work lists) runs
+// on a library-owned non-blocking stream per device, through a pinned staging buffer:
+// building a plan never touches the null stream, so it never waits for work queued on
+// the caller's streams.  Set-ups are serialised by a mutex and wait only for their own
+// copies.
+struct SetupContext {
+    std::mutex mu;
+    hipStream_t stream = nullptr;
+    char* staging = nullptr;  // pinned
+    size_t cap = 0;
+};
+
+SetupContext& setup_context(int dev) {
+    static std::mutex mu;
+    static std::vector<std::unique_ptr<SetupContext>> ctx;
+    std::lock_guard<std::mutex> lock(mu);
+    if ((int)ctx.size() <= dev) ctx.resize(dev + 1);
+    if (!ctx[dev]) ctx[dev].reset(new SetupContext());
+    return *ctx[dev];
+}
 
 // Order in which the hardware hands workgroups to the 8 XCDs of an MI355X: workgroup
 // b runs on XCD b % 8.  remap_for_xcds() permutes a list so that XCD x receives the
@@ -83,6 +143,10 @@ struct dxg2_plan {
     std::vector<GramWork> gram_work;    // plain kernel: every item of uint8/int8 plans;
                                         // HH and LL items of uint16 plans
     std::vector<GramWork> gram_xwork;   // cross kernel: [H|L][L|H]^T items of uint16 plans
+    std::vector<GramWork> gram_raster, gram_xraster;  // the same before the XCD permutation
+    std::vector<float2> twiddles;                     // FFT twiddle table (upload source)
+    DevAllocator alloc;                               // allocator of every device buffer
+    int gram_group = 0;
     size_t slab_bytes = 0;
 
     // device (owned)
@@ -178,9 +242,12 @@ void build_gram_work(dxg2_plan* p) {
             triangle(b, 0, 2 * nkb, 2 * nb + b, 1);
         }
     }
-    GramWork pad{-1, 0, 0, 0, 0, 0, 0, 0};
+    GramWork pad{-1, 0, 0, 0, 0, 0, 0, 0};  // job 0
     p->gram_work = group > 0 ? remap_for_xcds(raster, group, pad) : raster;
     p->gram_xwork = group > 0 ? remap_for_xcds(xraster, group, pad) : xraster;
+    p->gram_raster = std::move(raster);
+    p->gram_xraster = std::move(xraster);
+    p->gram_group = group;
 }
 
 // Next unused event quad of the profiling pool (created on first use).
@@ -208,7 +275,7 @@ int fft_length(int n) {
     return L;
 }
 
-int ensure_fft_buffers(dxg2_plan* p) {
+int ensure_fft_buffers(dxg2_plan* p, hipStream_t s) {
     if (p->d_power) return DXG2_OK;
     const int L = fft_length(p->N);
     if (L > FFT_MAX_L)
@@ -217,24 +284,25 @@ int ensure_fft_buffers(dxg2_plan* p) {
     p->fft_L = L;
     for (int b = 0; b < p->n_bins; ++b)
         for (int kb = 0; kb < p->bins[b].nkb; ++kb) p->kb_work.push_back(KbWork{b, kb});
-    std::vector<float2> tw(L);
+    p->twiddles.resize(L);
     for (int m = 0; m < L; ++m) {
         const double a = -2.0 * M_PI * (double)m / (double)L;
-        tw[m] = make_float2((float)std::cos(a), (float)std::sin(a));
+        p->twiddles[m] = make_float2((float)std::cos(a), (float)std::sin(a));
     }
-    if (hipMalloc(&p->d_xs, std::max<size_t>(p->slab_bytes, 1)) != hipSuccess ||
-        hipMalloc(&p->d_tw, (size_t)L * sizeof(float2)) != hipSuccess ||
-        hipMalloc(&p->d_power, (size_t)std::max(p->n_bins, 1) * (L + 1) * sizeof(double)) !=
+    if (dev_alloc(p->alloc, (void**)&p->d_xs, p->slab_bytes, s) != hipSuccess ||
+        dev_alloc(p->alloc, (void**)&p->d_tw, (size_t)L * sizeof(float2), s) != hipSuccess ||
+        dev_alloc(p->alloc, (void**)&p->d_power, (size_t)std::max(p->n_bins, 1) * (L + 1) * sizeof(double), s) !=
             hipSuccess ||
-        hipMalloc(&p->d_fnum, (size_t)std::max(p->n_bins, 1) * p->N * sizeof(double)) !=
+        dev_alloc(p->alloc, (void**)&p->d_fnum, (size_t)std::max(p->n_bins, 1) * p->N * sizeof(double), s) !=
             hipSuccess ||
         (!p->kb_work.empty() &&
-         hipMalloc(&p->d_kbwork, p->kb_work.size() * sizeof(KbWork)) != hipSuccess))
+         dev_alloc(p->alloc, (void**)&p->d_kbwork, p->kb_work.size() * sizeof(KbWork), s) != hipSuccess))
         return set_error(DXG2_ENOMEM, "cannot allocate FFT workspace");
-    DXG2_HIP(hipMemcpy(p->d_tw, tw.data(), (size_t)L * sizeof(float2), hipMemcpyHostToDevice));
+    DXG2_HIP(hipMemcpyAsync(p->d_tw, p->twiddles.data(), (size_t)L * sizeof(float2),
+                            hipMemcpyHostToDevice, s));
     if (!p->kb_work.empty())
-        DXG2_HIP(hipMemcpy(p->d_kbwork, p->kb_work.data(), p->kb_work.size() * sizeof(KbWork),
-                           hipMemcpyHostToDevice));
+        DXG2_HIP(hipMemcpyAsync(p->d_kbwork, p->kb_work.data(),
+                                p->kb_work.size() * sizeof(KbWork), hipMemcpyHostToDevice, s));
     return DXG2_OK;
 }
 
@@ -245,30 +313,31 @@ void build_event_work(dxg2_plan* p) {
                 EventWork{b, k0, std::min<int>(p->bins[b].npix, k0 + EVENT_ITEM_PIXELS), 0});
 }
 
-int ensure_event_buffers(dxg2_plan* p, int64_t nnz_coo) {
+int ensure_event_buffers(dxg2_plan* p, int64_t nnz_coo, hipStream_t s) {
     if (!p->d_err) {
-        DXG2_HIP(hipMalloc(&p->d_err, sizeof(int)));
+        DXG2_HIP(dev_alloc(p->alloc, (void**)&p->d_err, sizeof(int), s));
         if (!p->event_work.empty()) {
-            DXG2_HIP(hipMalloc(&p->d_ework, p->event_work.size() * sizeof(EventWork)));
-            DXG2_HIP(hipMemcpy(p->d_ework, p->event_work.data(),
-                               p->event_work.size() * sizeof(EventWork), hipMemcpyHostToDevice));
+            DXG2_HIP(dev_alloc(p->alloc, (void**)&p->d_ework, p->event_work.size() * sizeof(EventWork), s));
+            DXG2_HIP(hipMemcpyAsync(p->d_ework, p->event_work.data(),
+                                    p->event_work.size() * sizeof(EventWork),
+                                    hipMemcpyHostToDevice, s));
         }
     }
     if (nnz_coo >= 0) {
         const int64_t npix = (int64_t)p->H * p->W;
         if (!p->d_cnt) {
-            DXG2_HIP(hipMalloc(&p->d_cnt, npix * 4));
-            DXG2_HIP(hipMalloc(&p->d_off, (npix + 1) * 4));
-            DXG2_HIP(hipMalloc(&p->d_scan, ((npix + 1023) / 1024 + 1) * 4));
+            DXG2_HIP(dev_alloc(p->alloc, (void**)&p->d_cnt, npix * 4, s));
+            DXG2_HIP(dev_alloc(p->alloc, (void**)&p->d_off, (npix + 1) * 4, s));
+            DXG2_HIP(dev_alloc(p->alloc, (void**)&p->d_scan, ((npix + 1023) / 1024 + 1) * 4, s));
         }
         if (nnz_coo > p->ev_cap) {
-            if (p->d_etimes) (void)hipFree(p->d_etimes);
-            if (p->d_edata) (void)hipFree(p->d_edata);
+            dev_free(p->alloc, p->d_etimes, s);
+            dev_free(p->alloc, p->d_edata, s);
             p->d_etimes = nullptr;
             p->d_edata = nullptr;
             const int64_t cap = std::max<int64_t>(nnz_coo, 1);
-            if (hipMalloc(&p->d_etimes, cap * 4) != hipSuccess ||
-                hipMalloc(&p->d_edata, cap * 2) != hipSuccess)
+            if (dev_alloc(p->alloc, (void**)&p->d_etimes, cap * 4, s) != hipSuccess ||
+                dev_alloc(p->alloc, &p->d_edata, cap * 2, s) != hipSuccess)
                 return set_error(DXG2_ENOMEM, "cannot allocate %lld events", (long long)cap);
             p->ev_cap = cap;
         }
@@ -276,23 +345,69 @@ int ensure_event_buffers(dxg2_plan* p, int64_t nnz_coo) {
     return DXG2_OK;
 }
 
-int ensure_dense_buffers(dxg2_plan* p) {
+// Slab of the dense path, allocated on the first dense call and cleared on its stream:
+// rows past the last frame stay zero for the life of the plan.
+int ensure_dense_buffers(dxg2_plan* p, hipStream_t s) {
     if (!p->d_slab && p->slab_bytes) {
-        if (hipMalloc(&p->d_slab, p->slab_bytes) != hipSuccess) {
+        if (dev_alloc(p->alloc, (void**)&p->d_slab, p->slab_bytes, s) != hipSuccess) {
             p->d_slab = nullptr;
             return set_error(DXG2_ENOMEM, "cannot allocate %zu bytes of slab", p->slab_bytes);
         }
-        // rows past the last frame stay zero for the life of the plan
-        DXG2_HIP(hipMemset(p->d_slab, 0, p->slab_bytes));
+        DXG2_HIP(hipMemsetAsync(p->d_slab, 0, p->slab_bytes, s));
     }
     return DXG2_OK;
 }
 
+// Clear the accumulators of a dense call and gather the frames into the slab.  Returns
+// the frame-sum rows the Gram's diagonal tiles add into: the caller's sums (uint8/int8)
+// or the plan's digit-plane sums S_H, S_L (uint16).
+int64_t* dense_gather(dxg2_plan* p, const void* d_frames, int64_t* d_sums, hipStream_t s) {
+    int64_t* sums = d_sums ? d_sums : p->d_sums;
+    const bool wide = p->dtype == DXG2_U16;
+    int64_t* gsums = wide ? p->d_psums : sums;
+    const int64_t nbn = (int64_t)p->n_bins * p->N;
+    launch_fill_i64(gsums, (wide ? 2 : 1) * nbn, s);
+    launch_fill_i64(p->d_ugram, (wide ? 3 : 1) * nbn, s);
+    launch_gather_i8((const uint8_t*)d_frames, p->dtype, (int64_t)p->H * p->W, p->N, p->nrows,
+                     p->d_pix, p->d_bins, p->d_gwork, (int)p->gather_work.size(), p->d_slab, s);
+    return gsums;
+}
+
+// Bias corrections, exact denominators and g2 after the Gram.
+void dense_finish(dxg2_plan* p, int64_t* gsums, int64_t* d_num, int64_t* d_sums, double* d_den,
+                  float* d_g2, hipStream_t s) {
+    int64_t* sums = d_sums ? d_sums : p->d_sums;
+    const bool wide = p->dtype == DXG2_U16;
+    launch_finalize(gsums, sums, p->d_ugram, p->d_bins, p->n_bins, p->N,
+                    wide ? 2 : (p->dtype == DXG2_U8 ? 1 : 0), p->d_prefix, p->d_smax, d_num,
+                    d_den, d_g2, s);
+}
+
 }  // namespace
+
+struct dxg2_batch {
+    hipEvent_t staged = nullptr;  // the last batch's descriptor copies have run
+    GramJob* h_jobs = nullptr;    // pinned staging
+    GramWork* h_work = nullptr;
+    GramJob* d_jobs = nullptr;
+    GramWork* d_work = nullptr;
+    size_t jobs_cap = 0, work_cap = 0;
+    std::vector<void*> retired;   // outgrown pinned buffers, freed with the batch (a
+                                  // pinned free can wait for the whole device)
+    DevAllocator alloc;
+};
 
 extern "C" {
 
 const char* dxg2_last_error(void) { return g_last_error.c_str(); }
+
+int dxg2_set_allocator(dxg2_alloc_fn alloc, dxg2_free_fn free_fn, void* ctx) {
+    if ((alloc == nullptr) != (free_fn == nullptr))
+        return set_error(DXG2_EINVAL, "give both allocator functions or neither");
+    std::lock_guard<std::mutex> lock(g_alloc_mu);
+    g_alloc = DevAllocator{alloc, free_fn, ctx};
+    return DXG2_OK;
+}
 
 const char* dxg2_version(void) { return "dxg2 0.1.0 gfx950"; }
 
@@ -308,6 +423,7 @@ int dxg2_plan_create(const dxg2_plan_desc* desc, dxg2_plan** out) {
     if (hw > (int64_t)INT32_MAX) return set_error(DXG2_EINVAL, "frame too large");
 
     dxg2_plan* p = new dxg2_plan();
+    p->alloc = current_allocator();
     p->H = desc->height;
     p->W = desc->width;
     p->N = desc->nframes;
@@ -370,11 +486,43 @@ int dxg2_plan_create(const dxg2_plan_desc* desc, dxg2_plan** out) {
         delete p;
         return set_error(DXG2_EHIP, "hipGetDevice: %s", hipGetErrorString(e));
     }
+    // Workspaces come from the stream-ordered pool on the set-up stream; the uploads go
+    // through the pinned staging buffer, and the plan waits for them before it is
+    // returned.
+    SetupContext& sc = setup_context(p->device);
+    std::lock_guard<std::mutex> setup_lock(sc.mu);
+    if (!sc.stream && hipStreamCreateWithFlags(&sc.stream, hipStreamNonBlocking) != hipSuccess) {
+        delete p;
+        return set_error(DXG2_EHIP, "cannot create the plan set-up stream");
+    }
+    hipStream_t s0 = sc.stream;
+    const size_t staged = p->pix_idx.size() * 4 + p->bins.size() * sizeof(BinDev) +
+                          (p->gather_work.size() + 1) * sizeof(GatherWork) +
+                          (p->gram_work.size() + p->gram_xwork.size() + 2) * sizeof(GramWork) +
+                          5 * 256;
+    if (sc.cap < staged) {
+        if (sc.staging) {
+            (void)hipStreamSynchronize(s0);
+            (void)hipHostFree(sc.staging);
+        }
+        sc.staging = nullptr;
+        sc.cap = 0;
+        const size_t cap = std::max<size_t>(staged, (size_t)1 << 24);
+        if (hipHostMalloc((void**)&sc.staging, cap, 0) != hipSuccess) {
+            delete p;
+            return set_error(DXG2_ENOMEM, "cannot allocate %zu bytes of pinned staging", cap);
+        }
+        sc.cap = cap;
+    }
+    size_t staged_at = 0;
     auto upload = [&](void** dst, const void* src, size_t bytes) -> int {
         if (bytes == 0) return DXG2_OK;
-        if (hipMalloc(dst, bytes) != hipSuccess)
+        if (dev_alloc(p->alloc, dst, bytes, s0) != hipSuccess)
             return set_error(DXG2_ENOMEM, "cannot allocate %zu bytes", bytes);
-        if (hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice) != hipSuccess)
+        char* stage = sc.staging + staged_at;
+        staged_at += (bytes + 255) / 256 * 256;
+        memcpy(stage, src, bytes);
+        if (hipMemcpyAsync(*dst, stage, bytes, hipMemcpyHostToDevice, s0) != hipSuccess)
             return set_error(DXG2_EHIP, "upload of %zu bytes failed", bytes);
         return DXG2_OK;
     };
@@ -393,17 +541,20 @@ int dxg2_plan_create(const dxg2_plan_desc* desc, dxg2_plan** out) {
     const size_t nb_n = (size_t)std::max(p->n_bins, 1) * p->N * sizeof(int64_t);
     const int planes = slab_planes(p->dtype);
     // numerator rows: one per bin, or HH, LL and cross rows for digit planes
-    if (!rc && hipMalloc(&p->d_ugram, (planes == 2 ? 3 : 1) * nb_n) != hipSuccess)
+    if (!rc && dev_alloc(p->alloc, (void**)&p->d_ugram, (planes == 2 ? 3 : 1) * nb_n, s0) != hipSuccess)
         rc = set_error(DXG2_ENOMEM, "cannot allocate numerator workspace");
-    if (!rc && hipMalloc(&p->d_sums, nb_n) != hipSuccess)
+    if (!rc && dev_alloc(p->alloc, (void**)&p->d_sums, nb_n, s0) != hipSuccess)
         rc = set_error(DXG2_ENOMEM, "cannot allocate sums workspace");
-    if (!rc && planes == 2 && hipMalloc(&p->d_psums, 2 * nb_n) != hipSuccess)
+    if (!rc && planes == 2 && dev_alloc(p->alloc, (void**)&p->d_psums, 2 * nb_n, s0) != hipSuccess)
         rc = set_error(DXG2_ENOMEM, "cannot allocate digit-plane sums workspace");
-    if (!rc && (hipMalloc(&p->d_prefix, (size_t)planes * std::max(p->n_bins, 1) *
-                                            (p->N + 1) * sizeof(int64_t)) != hipSuccess ||
-                hipMalloc(&p->d_smax, (size_t)std::max(p->n_bins, 1) * sizeof(int64_t)) !=
-                    hipSuccess))
+    if (!rc && (dev_alloc(p->alloc, (void**)&p->d_prefix, (size_t)planes * std::max(p->n_bins, 1) *
+                                                     (p->N + 1) * sizeof(int64_t), s0) !=
+                    hipSuccess ||
+                dev_alloc(p->alloc, (void**)&p->d_smax, (size_t)std::max(p->n_bins, 1) * sizeof(int64_t),
+                           s0) != hipSuccess))
         rc = set_error(DXG2_ENOMEM, "cannot allocate denominator workspace");
+    if (!rc && hipStreamSynchronize(s0) != hipSuccess)
+        rc = set_error(DXG2_EHIP, "plan upload failed");
     if (rc) {
         dxg2_plan_destroy(p);
         return rc;
@@ -412,20 +563,30 @@ int dxg2_plan_create(const dxg2_plan_desc* desc, dxg2_plan** out) {
     return DXG2_OK;
 }
 
-int dxg2_plan_destroy(dxg2_plan* p) {
-    if (!p) return DXG2_OK;
+static void plan_release(dxg2_plan* p, hipStream_t s) {
     void* bufs[] = {p->d_pix,   p->d_bins, p->d_gwork, p->d_mwork, p->d_xwork, p->d_slab,
                     p->d_ugram, p->d_sums, p->d_psums, p->d_prefix, p->d_smax, p->d_pa,
                     p->d_pg, p->d_pm,
                     p->d_ework, p->d_err,  p->d_cnt,   p->d_scan,  p->d_off,
                     p->d_etimes, p->d_edata, p->d_kbwork, p->d_xs, p->d_tw,
                     p->d_power, p->d_fnum};
-    for (void* b : bufs)
-        if (b) (void)hipFree(b);
+    for (void* b : bufs) dev_free(p->alloc, b, s);
     for (auto& q : p->ev_pool)
         for (auto e : q)
             if (e) (void)hipEventDestroy(e);
     delete p;
+}
+
+int dxg2_plan_destroy(dxg2_plan* p) {
+    if (!p) return DXG2_OK;
+    (void)hipDeviceSynchronize();  // no call on the plan may still be running
+    plan_release(p, nullptr);
+    return DXG2_OK;
+}
+
+int dxg2_plan_destroy_async(dxg2_plan* p, void* stream) {
+    if (!p) return DXG2_OK;
+    plan_release(p, (hipStream_t)stream);
     return DXG2_OK;
 }
 
@@ -510,22 +671,14 @@ int dxg2_correlate_dense(dxg2_plan* p, const void* d_frames, int64_t* d_num, int
     if (!p) return set_error(DXG2_EINVAL, "null plan");
     if (!d_frames || !d_g2) return set_error(DXG2_EINVAL, "frames and g2 must be non-null");
     if (p->n_bins == 0) return DXG2_OK;
-    int rc = ensure_dense_buffers(p);
-    if (rc) return rc;
     hipStream_t s = (hipStream_t)stream;
-    int64_t* sums = d_sums ? d_sums : p->d_sums;
-    const bool wide = p->dtype == DXG2_U16;
-    // the Gram's diagonal tiles write S (uint8/int8) or S_H, S_L (uint16) here
-    int64_t* gsums = wide ? p->d_psums : sums;
-    const int64_t nbn = (int64_t)p->n_bins * p->N;
+    int rc = ensure_dense_buffers(p, s);
+    if (rc) return rc;
     (void)hipGetLastError();
     hipEvent_t* ev = nullptr;
     if (p->profiling && (rc = next_events(p, &ev))) return rc;
     if (ev) DXG2_HIP(hipEventRecord(ev[0], s));
-    launch_fill_i64(gsums, (wide ? 2 : 1) * nbn, s);
-    launch_fill_i64(p->d_ugram, (wide ? 3 : 1) * nbn, s);
-    launch_gather_i8((const uint8_t*)d_frames, p->dtype, (int64_t)p->H * p->W, p->N, p->nrows,
-                     p->d_pix, p->d_bins, p->d_gwork, (int)p->gather_work.size(), p->d_slab, s);
+    int64_t* gsums = dense_gather(p, d_frames, d_sums, s);
     if (ev) DXG2_HIP(hipEventRecord(ev[1], s));
     const int biased = p->dtype == DXG2_I8 ? 0 : 1;
     launch_gram_i8(p->gram_tile, false, p->d_slab, p->d_bins, p->d_mwork,
@@ -533,10 +686,144 @@ int dxg2_correlate_dense(dxg2_plan* p, const void* d_frames, int64_t* d_num, int
     launch_gram_i8(p->gram_tile, true, p->d_slab, p->d_bins, p->d_xwork,
                    (int)p->gram_xwork.size(), p->N, p->nrows, biased, p->d_ugram, gsums, s);
     if (ev) DXG2_HIP(hipEventRecord(ev[2], s));
-    launch_finalize(gsums, sums, p->d_ugram, p->d_bins, p->n_bins, p->N,
-                    wide ? 2 : (p->dtype == DXG2_U8 ? 1 : 0), p->d_prefix, p->d_smax, d_num,
-                    d_den, d_g2, s);
+    dense_finish(p, gsums, d_num, d_sums, d_den, d_g2, s);
     if (ev) DXG2_HIP(hipEventRecord(ev[3], s));
+    DXG2_HIP(hipGetLastError());
+    return DXG2_OK;
+}
+
+int dxg2_batch_create(dxg2_batch** out) {
+    if (!out) return set_error(DXG2_EINVAL, "null argument");
+    dxg2_batch* b = new dxg2_batch();
+    b->alloc = current_allocator();
+    if (hipEventCreateWithFlags(&b->staged, hipEventDisableTiming) != hipSuccess) {
+        delete b;
+        return set_error(DXG2_EHIP, "hipEventCreate failed");
+    }
+    *out = b;
+    return DXG2_OK;
+}
+
+int dxg2_batch_destroy(dxg2_batch* b) {
+    if (!b) return DXG2_OK;
+    (void)hipEventSynchronize(b->staged);
+    (void)hipEventDestroy(b->staged);
+    (void)hipDeviceSynchronize();
+    dev_free(b->alloc, b->d_jobs, nullptr);
+    dev_free(b->alloc, b->d_work, nullptr);
+    b->retired.push_back(b->h_jobs);
+    b->retired.push_back(b->h_work);
+    for (void* h : b->retired)
+        if (h) (void)hipHostFree(h);
+    delete b;
+    return DXG2_OK;
+}
+
+int dxg2_batch_correlate_dense(dxg2_batch* b, int32_t n, dxg2_plan* const* plans,
+                               const void* const* d_frames, int64_t* const* d_num,
+                               int64_t* const* d_sums, double* const* d_den,
+                               float* const* d_g2, void* stream) {
+    if (!b || n < 0 || (n > 0 && (!plans || !d_frames || !d_g2)))
+        return set_error(DXG2_EINVAL, "null argument");
+    if (n == 0) return DXG2_OK;
+    const int tile = plans[0] ? plans[0]->gram_tile : 0;
+    for (int i = 0; i < n; ++i) {
+        if (!plans[i] || !d_frames[i] || !d_g2[i])
+            return set_error(DXG2_EINVAL, "job %d: null plan, frames or g2", i);
+        if (plans[i]->gram_tile != tile)
+            return set_error(DXG2_EINVAL, "job %d: Gram tile %d differs from %d", i,
+                             plans[i]->gram_tile, tile);
+        for (int k = 0; k < i; ++k)
+            if (plans[k] == plans[i])
+                return set_error(DXG2_EINVAL, "plan of job %d appears twice in the batch", i);
+        int rc = ensure_dense_buffers(plans[i], (hipStream_t)stream);
+        if (rc) return rc;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    (void)hipGetLastError();
+    // Work list: every job's items, largest jobs first, one XCD permutation over the lot.
+    std::vector<int> order(n);
+    for (int i = 0; i < n; ++i) order[i] = i;
+    auto ops = [&](int i) {
+        double w = 0;
+        for (const auto& it : plans[i]->gram_raster) w += it.kb1 - it.kb0;
+        for (const auto& it : plans[i]->gram_xraster) w += it.kb1 - it.kb0;
+        return w;
+    };
+    std::vector<double> wk(n);
+    for (int i = 0; i < n; ++i) wk[i] = ops(i);
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return wk[x] > wk[y]; });
+    std::vector<GramWork> plain, cross;
+    std::vector<GramJob> jobs(n);
+    for (int i : order) {
+        dxg2_plan* p = plans[i];
+        for (GramWork w : p->gram_raster) {
+            w.job = i;
+            plain.push_back(w);
+        }
+        for (GramWork w : p->gram_xraster) {
+            w.job = i;
+            cross.push_back(w);
+        }
+    }
+    const int group = plans[0]->gram_group;
+    GramWork pad{-1, 0, 0, 0, 0, 0, 0, 0};
+    if (group > 0) {
+        plain = remap_for_xcds(plain, group, pad);
+        cross = remap_for_xcds(cross, group, pad);
+    }
+    // Per-job gathers first: they also clear the outputs the Gram accumulates into.
+    for (int i = 0; i < n; ++i) {
+        dxg2_plan* p = plans[i];
+        if (p->n_bins == 0) continue;
+        int64_t* gsums = dense_gather(p, d_frames[i], d_sums ? d_sums[i] : nullptr, s);
+        jobs[i] = GramJob{p->d_slab, p->d_bins, p->d_ugram, gsums, p->N, p->nrows,
+                          p->dtype == DXG2_I8 ? 0 : 1, 0};
+    }
+    // Stage descriptors and work lists through pinned memory (reused once the previous
+    // batch's copies have run).
+    const size_t nw = plain.size() + cross.size();
+    DXG2_HIP(hipEventSynchronize(b->staged));
+    if (b->jobs_cap < (size_t)n) {  // grow geometrically; the outgrown buffers are kept
+        const size_t cap = std::max<size_t>(std::max<size_t>(2 * b->jobs_cap, n), 16);
+        b->retired.push_back(b->h_jobs);
+        dev_free(b->alloc, b->d_jobs, s);
+        b->h_jobs = nullptr;
+        b->d_jobs = nullptr;
+        b->jobs_cap = 0;
+        if (hipHostMalloc((void**)&b->h_jobs, cap * sizeof(GramJob), 0) != hipSuccess ||
+            dev_alloc(b->alloc, (void**)&b->d_jobs, cap * sizeof(GramJob), s) != hipSuccess)
+            return set_error(DXG2_ENOMEM, "cannot allocate batch job table");
+        b->jobs_cap = cap;
+    }
+    if (b->work_cap < nw) {
+        const size_t cap = std::max<size_t>(std::max<size_t>(2 * b->work_cap, nw), 1 << 16);
+        b->retired.push_back(b->h_work);
+        dev_free(b->alloc, b->d_work, s);
+        b->h_work = nullptr;
+        b->d_work = nullptr;
+        b->work_cap = 0;
+        if (hipHostMalloc((void**)&b->h_work, cap * sizeof(GramWork), 0) != hipSuccess ||
+            dev_alloc(b->alloc, (void**)&b->d_work, cap * sizeof(GramWork), s) != hipSuccess)
+            return set_error(DXG2_ENOMEM, "cannot allocate batch work list");
+        b->work_cap = cap;
+    }
+    std::copy(jobs.begin(), jobs.end(), b->h_jobs);
+    std::copy(plain.begin(), plain.end(), b->h_work);
+    std::copy(cross.begin(), cross.end(), b->h_work + plain.size());
+    DXG2_HIP(hipMemcpyAsync(b->d_jobs, b->h_jobs, n * sizeof(GramJob), hipMemcpyHostToDevice, s));
+    if (nw)
+        DXG2_HIP(hipMemcpyAsync(b->d_work, b->h_work, nw * sizeof(GramWork),
+                                hipMemcpyHostToDevice, s));
+    DXG2_HIP(hipEventRecord(b->staged, s));
+    launch_gram_i8_jobs(tile, false, b->d_jobs, b->d_work, (int)plain.size(), s);
+    launch_gram_i8_jobs(tile, true, b->d_jobs, b->d_work + plain.size(), (int)cross.size(), s);
+    for (int i = 0; i < n; ++i) {
+        dxg2_plan* p = plans[i];
+        if (p->n_bins == 0) continue;
+        dense_finish(p, jobs[i].sums, d_num ? d_num[i] : nullptr, d_sums ? d_sums[i] : nullptr,
+                     d_den ? d_den[i] : nullptr, d_g2[i], s);
+    }
     DXG2_HIP(hipGetLastError());
     return DXG2_OK;
 }
@@ -550,9 +837,9 @@ int dxg2_correlate_dense_plain(dxg2_plan* p, const void* d_frames, float* d_g2, 
     const size_t lda = (size_t)((mx + 7) / 8 * 8);
     if (!p->d_pa && lda > 0) {
         p->pa_elems = (size_t)p->N * lda;
-        if (hipMalloc(&p->d_pa, p->pa_elems * 4) != hipSuccess ||
-            hipMalloc(&p->d_pg, (size_t)p->N * p->N * 4) != hipSuccess ||
-            hipMalloc(&p->d_pm, (size_t)p->N * 4) != hipSuccess)
+        if (dev_alloc(p->alloc, (void**)&p->d_pa, p->pa_elems * 4, s) != hipSuccess ||
+            dev_alloc(p->alloc, (void**)&p->d_pg, (size_t)p->N * p->N * 4, s) != hipSuccess ||
+            dev_alloc(p->alloc, (void**)&p->d_pm, (size_t)p->N * 4, s) != hipSuccess)
             return set_error(DXG2_ENOMEM, "cannot allocate plain-path workspace");
     }
     (void)hipGetLastError();
@@ -604,7 +891,7 @@ int dxg2_correlate_events_csr(dxg2_plan* p, const int32_t* d_times, const void* 
     if (nnz < 0 || nnz > (int64_t)UINT32_MAX)
         return set_error(DXG2_EINVAL, "event count %lld out of range", (long long)nnz);
     if (p->n_bins == 0) return DXG2_OK;
-    int rc = ensure_event_buffers(p, -1);
+    int rc = ensure_event_buffers(p, -1, (hipStream_t)stream);
     if (rc) return rc;
     hipStream_t s = (hipStream_t)stream;
     (void)hipGetLastError();
@@ -626,7 +913,7 @@ int dxg2_correlate_events_coo(dxg2_plan* p, const uint32_t* d_pixel, const uint3
     if (p->dtype == DXG2_I8)
         return set_error(DXG2_EUNSUPPORTED, "COO event counts are unsigned (uint8 / uint16)");
     if (p->n_bins == 0) return DXG2_OK;
-    int rc = ensure_event_buffers(p, nnz);
+    int rc = ensure_event_buffers(p, nnz, (hipStream_t)stream);
     if (rc) return rc;
     hipStream_t s = (hipStream_t)stream;
     (void)hipGetLastError();
@@ -647,7 +934,7 @@ int dxg2_correlate_fft(dxg2_plan* p, const void* d_frames, double* d_num, int64_
     if (p->dtype != DXG2_U8)
         return set_error(DXG2_EUNSUPPORTED, "FFT path takes uint8 frames");
     if (p->n_bins == 0) return DXG2_OK;
-    int rc = ensure_fft_buffers(p);
+    int rc = ensure_fft_buffers(p, (hipStream_t)stream);
     if (rc) return rc;
     hipStream_t s = (hipStream_t)stream;
     int64_t* sums = d_sums ? d_sums : p->d_sums;
@@ -714,8 +1001,8 @@ int dxg2_synth_events(const float* d_mean_map, int32_t height, int32_t width, in
     hipStream_t s = (hipStream_t)stream;
     const float rho = tau_c > 0 ? (float)std::exp(-1.0 / tau_c) : 0.f;
     uint32_t *count = nullptr, *tot = nullptr;
-    DXG2_HIP(hipMalloc(&count, hw * 4));
-    DXG2_HIP(hipMalloc(&tot, ((hw + 1023) / 1024 + 1) * 4));
+    DXG2_HIP(dev_alloc(DevAllocator{}, (void**)&count, hw * 4, s));
+    DXG2_HIP(dev_alloc(DevAllocator{}, (void**)&tot, ((hw + 1023) / 1024 + 1) * 4, s));
     (void)hipGetLastError();
     launch_synth_events(0, dtype, nframes, hw, d_mean_map, rho, seed, count, nullptr, nullptr,
                         nullptr, s);
@@ -723,8 +1010,8 @@ int dxg2_synth_events(const float* d_mean_map, int32_t height, int32_t width, in
     uint32_t total = 0;
     hipError_t e = hipMemcpyAsync(&total, d_offsets + hw, 4, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
-    (void)hipFree(count);
-    (void)hipFree(tot);
+    dev_free(DevAllocator{}, count, s);
+    dev_free(DevAllocator{}, tot, s);
     if (e != hipSuccess) return set_error(DXG2_EHIP, "event count failed: %s", hipGetErrorString(e));
     *nnz = total;
     if (d_times) {
@@ -738,6 +1025,27 @@ int dxg2_synth_events(const float* d_mean_map, int32_t height, int32_t width, in
     return DXG2_OK;
 }
 
+int dxg2_events_window(const uint32_t* d_offsets, const int32_t* d_times, const void* d_data,
+                       int64_t nnz, int64_t npix, int32_t dtype, int32_t t0, int32_t t1,
+                       uint32_t* d_scratch, uint32_t* d_out_offsets, int32_t* d_out_times,
+                       void* d_out_data, int64_t capacity, void* stream) {
+    if (!d_offsets || !d_times || !d_data || !d_scratch || !d_out_offsets || !d_out_times ||
+        !d_out_data)
+        return set_error(DXG2_EINVAL, "null argument");
+    if (npix <= 0 || nnz < 0 || t0 < 0 || t1 < t0)
+        return set_error(DXG2_EINVAL, "bad window [%d, %d) or sizes", t0, t1);
+    if (dtype != DXG2_U8 && dtype != DXG2_U16)
+        return set_error(DXG2_EINVAL, "event counts are uint8 or uint16");
+    if (capacity < nnz)
+        return set_error(DXG2_ECAPACITY, "capacity %lld < %lld input events (the window is "
+                         "at most the input)", (long long)capacity, (long long)nnz);
+    (void)hipGetLastError();
+    launch_events_window(dtype, d_offsets, d_times, d_data, npix, t0, t1, d_scratch,
+                         d_out_offsets, d_out_times, d_out_data, (hipStream_t)stream);
+    DXG2_HIP(hipGetLastError());
+    return DXG2_OK;
+}
+
 int dxg2_events_to_coo(const int32_t* d_times, const void* d_data, const uint32_t* d_offsets,
                        int64_t nnz, int32_t height, int32_t width, int32_t nframes, int32_t dtype,
                        uint32_t* d_pixel, uint32_t* d_frame, void* d_count, void* stream) {
@@ -747,9 +1055,9 @@ int dxg2_events_to_coo(const int32_t* d_times, const void* d_data, const uint32_
     hipStream_t s = (hipStream_t)stream;
     const int64_t hw = (int64_t)height * width;
     uint32_t *fcount = nullptr, *fcursor = nullptr, *tot = nullptr;
-    DXG2_HIP(hipMalloc(&fcount, (size_t)nframes * 4));
-    DXG2_HIP(hipMalloc(&fcursor, (size_t)(nframes + 1) * 4));
-    DXG2_HIP(hipMalloc(&tot, (size_t)((nframes + 1023) / 1024 + 1) * 4));
+    DXG2_HIP(dev_alloc(DevAllocator{}, (void**)&fcount, (size_t)nframes * 4, s));
+    DXG2_HIP(dev_alloc(DevAllocator{}, (void**)&fcursor, (size_t)(nframes + 1) * 4, s));
+    DXG2_HIP(dev_alloc(DevAllocator{}, (void**)&tot, (size_t)((nframes + 1023) / 1024 + 1) * 4, s));
     (void)hipGetLastError();
     launch_fill_u32(fcount, nframes, s);
     launch_frame_count(d_times, nnz, nframes, fcount, s);
@@ -757,9 +1065,9 @@ int dxg2_events_to_coo(const int32_t* d_times, const void* d_data, const uint32_
     launch_to_coo(dtype, d_times, d_data, d_offsets, hw, nframes, fcursor, d_pixel, d_frame,
                   d_count, s);
     hipError_t e = hipStreamSynchronize(s);
-    (void)hipFree(fcount);
-    (void)hipFree(fcursor);
-    (void)hipFree(tot);
+    dev_free(DevAllocator{}, fcount, s);
+    dev_free(DevAllocator{}, fcursor, s);
+    dev_free(DevAllocator{}, tot, s);
     if (e != hipSuccess) return set_error(DXG2_EHIP, "to_coo failed: %s", hipGetErrorString(e));
     DXG2_HIP(hipGetLastError());
     return DXG2_OK;

@@ -44,6 +44,17 @@ struct GramWork {        // one output tile (i, j >= i) over a k-block range
     int32_t kb1;         // exclusive
     int32_t out;         // row of the numerator (and frame-sum) outputs it adds into
     int32_t cross;       // 1: B operand is the other digit plane ([H|L] x [L|H]^T)
+    int32_t job;         // grouped launches over several plans: index into GramJob[]
+};
+
+struct GramJob {         // one plan's buffers in a grouped Gram launch
+    const int8_t* slab;
+    const BinDev* bins;
+    int64_t* ugram;
+    int64_t* sums;
+    int32_t nframes;
+    int32_t nrows;
+    int32_t biased;
     int32_t pad;
 };
 
@@ -73,6 +84,9 @@ void launch_gather_i8(const uint8_t* frames, int dtype, int64_t hw, int nframes,
 void launch_gram_i8(int tile, bool cross, const int8_t* slab, const BinDev* bins,
                     const GramWork* work, int n_work, int nframes, int nrows, int biased,
                     int64_t* ugram, int64_t* sums, hipStream_t s);
+// Grouped launch: work items of several plans, each naming its job descriptor.
+void launch_gram_i8_jobs(int tile, bool cross, const GramJob* jobs, const GramWork* work,
+                         int n_work, hipStream_t s);
 // mode 0: numerators as accumulated; 1: uint8 bias correction; 2: uint16 digit planes
 // (ugram holds HH, LL, cross rows; plane_sums the per-plane S_H, S_L rows, and `sums`
 // receives S = 256 S_H + S_L).
@@ -104,6 +118,10 @@ int launch_fft_power(int L, const uint8_t* xs, const BinDev* bins, const KbWork*
 void launch_fft_inverse(const double* power, int L, int n_bins, int nframes, double* num,
                         hipStream_t s);
 void launch_excl_scan(const uint32_t* in, int64_t n, uint32_t* tot, uint32_t* out, hipStream_t s);
+void launch_events_window(int dtype, const uint32_t* offsets, const int32_t* times,
+                          const void* data, int64_t npix, int t0, int t1, uint32_t* scratch,
+                          uint32_t* out_offsets, int32_t* out_times, void* out_data,
+                          hipStream_t s);
 void launch_copy_u32(const uint32_t* src, uint32_t* dst, int64_t n, hipStream_t s);
 void launch_fill_u32(uint32_t* p, int64_t n, hipStream_t s);
 void launch_synth_events(int pass, int dtype, int nframes, int64_t hw, const float* mean_map,

@@ -261,6 +261,64 @@ __global__ void segment_sort_kernel(const uint32_t* __restrict__ offsets, uint32
 
 static unsigned grid_for(int64_t n, int64_t cap = 65536);
 
+// First index in times[b, e) (ascending) whose frame is >= t.
+__device__ __forceinline__ uint32_t lower_frame(const int32_t* __restrict__ times, uint32_t b,
+                                                uint32_t e, int t) {
+    while (b < e) {
+        const uint32_t m = b + (e - b) / 2;
+        if (times[m] < t)
+            b = m + 1;
+        else
+            e = m;
+    }
+    return b;
+}
+
+// Frame window [t0, t1) of a pixel-major event list: per-pixel counts, then (after a scan
+// of the counts into new offsets) the copy with frames re-based to t - t0.
+__global__ void window_count_kernel(const uint32_t* __restrict__ offsets,
+                                    const int32_t* __restrict__ times, int64_t npix, int t0,
+                                    int t1, uint32_t* __restrict__ count) {
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < npix;
+         p += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t b = offsets[p], e = offsets[p + 1];
+        count[p] = lower_frame(times, b, e, t1) - lower_frame(times, b, e, t0);
+    }
+}
+
+template <typename T>
+__global__ void window_copy_kernel(const uint32_t* __restrict__ offsets,
+                                   const int32_t* __restrict__ times, const T* __restrict__ data,
+                                   int64_t npix, int t0, const uint32_t* __restrict__ out_off,
+                                   int32_t* __restrict__ out_times, T* __restrict__ out_data) {
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < npix;
+         p += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t lo = lower_frame(times, offsets[p], offsets[p + 1], t0);
+        const uint32_t o = out_off[p], n = out_off[p + 1] - o;
+        for (uint32_t i = 0; i < n; ++i) {
+            out_times[o + i] = times[lo + i] - t0;
+            out_data[o + i] = data[lo + i];
+        }
+    }
+}
+
+void launch_events_window(int dtype, const uint32_t* offsets, const int32_t* times,
+                          const void* data, int64_t npix, int t0, int t1, uint32_t* scratch,
+                          uint32_t* out_offsets, int32_t* out_times, void* out_data,
+                          hipStream_t s) {
+    if (npix <= 0) return;
+    window_count_kernel<<<grid_for(npix), 256, 0, s>>>(offsets, times, npix, t0, t1, out_offsets);
+    launch_excl_scan(out_offsets, npix, scratch, out_offsets, s);  // in place
+    if (dtype == DXG2_U16)
+        window_copy_kernel<uint16_t><<<grid_for(npix), 256, 0, s>>>(
+            offsets, times, (const uint16_t*)data, npix, t0, out_offsets, out_times,
+            (uint16_t*)out_data);
+    else
+        window_copy_kernel<uint8_t><<<grid_for(npix), 256, 0, s>>>(
+            offsets, times, (const uint8_t*)data, npix, t0, out_offsets, out_times,
+            (uint8_t*)out_data);
+}
+
 void launch_excl_scan(const uint32_t* in, int64_t n, uint32_t* tot, uint32_t* out, hipStream_t s) {
     if (n <= 0) return;
     const int nb = (int)((n + SCAN_B - 1) / SCAN_B);

@@ -267,6 +267,58 @@ __global__ __launch_bounds__(256) void gram_i8_kernel(const int8_t* __restrict__
     }
 }
 
+// Grouped launch over several plans (continuous batching of dense jobs): the same tile
+// code, with the slab, bins, frame counts and outputs of each item's job.
+template <int WM, int WN, bool CROSS>
+__global__ __launch_bounds__(256) void gram_i8_jobs_kernel(const GramJob* __restrict__ jobs,
+                                                           const GramWork* __restrict__ work) {
+    using S = GramShape<WM, WN>;
+    static_assert(S::TM == S::TN, "square tiles");
+    __shared__ __attribute__((aligned(1024))) int8_t a0[S::PANEL], a1[S::PANEL], a2[S::PANEL],
+        a3[S::PANEL];
+    __shared__ __attribute__((aligned(1024))) int8_t b0[S::PANEL], b1[S::PANEL], b2[S::PANEL],
+        b3[S::PANEL];
+    __shared__ unsigned int sDiag[2 * S::NDIAG];
+
+    const GramWork w = work[blockIdx.x];
+    if (w.bin < 0) return;  // padding item
+    const GramJob j = jobs[w.job];
+    const BinDev bd = j.bins[w.bin];
+    const GramLds<WM, WN> lds{{a0, a1, a2, a3}, {b0, b1, b2, b3}, sDiag};
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6;
+    int64_t* out = j.ugram + (int64_t)w.out * j.nframes;
+    int64_t* srow = j.sums + (int64_t)w.out * j.nframes;
+    if (!CROSS && w.ti == w.tj) {
+        GramTile<WM, WN, true, false> t{j.slab + bd.slab_off, w.kb0, w.kb1, w.ti * S::TM,
+                                        w.tj * S::TN, j.nframes, j.nrows, tid, tid & 63,
+                                        wave >> 1, wave & 1, 0, lds};
+        t.run(j.biased, out, srow);
+    } else {
+        GramTile<WM, WN, false, CROSS> t{j.slab + bd.slab_off, w.kb0, w.kb1, w.ti * S::TM,
+                                         w.tj * S::TN, j.nframes, j.nrows, tid, tid & 63,
+                                         wave >> 1, wave & 1, bd.nkb, lds};
+        t.run(j.biased, out, srow);
+    }
+}
+
+void launch_gram_i8_jobs(int tile, bool cross, const GramJob* jobs, const GramWork* work,
+                         int n_work, hipStream_t s) {
+    if (n_work <= 0) return;
+    const dim3 grid((unsigned)n_work), block(256);
+    if (tile == 128) {
+        if (cross)
+            gram_i8_jobs_kernel<2, 2, true><<<grid, block, 0, s>>>(jobs, work);
+        else
+            gram_i8_jobs_kernel<2, 2, false><<<grid, block, 0, s>>>(jobs, work);
+    } else {
+        if (cross)
+            gram_i8_jobs_kernel<4, 4, true><<<grid, block, 0, s>>>(jobs, work);
+        else
+            gram_i8_jobs_kernel<4, 4, false><<<grid, block, 0, s>>>(jobs, work);
+    }
+}
+
 int gram_tile_size(int variant) { return variant == 1 ? 128 : 256; }
 
 void launch_gram_i8(int tile, bool cross, const int8_t* slab, const BinDev* bins,

@@ -115,6 +115,15 @@ def speckle_frames_device(shape, nframes, mean, tau_c, seed, dtype=np.uint8, mea
     return frames
 
 
+def workload_frames_device(name, seed):
+    """GPU frames of a named workload, with its hot-pixel population if it has one."""
+    c = WORKLOADS[name]
+    mm = mean_map(c["shape"], c["mean"], c.get("hot_fraction", 0.0), c.get("hot_mean", 0.0),
+                  seed)
+    return speckle_frames_device(c["shape"], c["nframes"], c["mean"], c["tau_c"], seed,
+                                 np.dtype(c["dtype"]), mean_map_=mm)
+
+
 def speckle_events_device(shape, nframes, mean, tau_c, seed, dtype=np.uint8, mean_map_=None):
     """GPU events of the same walk as speckle_frames_device, in the pixel-major layout:
     returns torch tensors (times int32, data, offsets uint32)."""

@@ -46,6 +46,19 @@ extern "C" {
 
 typedef struct dxg2_plan dxg2_plan;
 
+/* Device memory of plans and batches.  alloc returns at least `bytes` of device memory
+ * for use on `stream` (a hipStream_t), or NULL; free releases a pointer alloc returned.
+ * The library frees a plan's buffers only when no call on the plan can still be running
+ * (dxg2_plan_destroy waits for the device; with dxg2_plan_destroy_async the caller
+ * guarantees it).  dxg2_set_allocator(NULL, NULL, NULL) restores the built-in hipMalloc /
+ * hipFree (hipFree waits for every kernel in flight).  Plans and batches keep the
+ * allocator that was current when they were created.  A caching allocator whose frees
+ * never touch the device lets a serving loop create and release plans while other jobs
+ * run.  No reference counterpart. */
+typedef void* (*dxg2_alloc_fn)(size_t bytes, void* stream, void* ctx);
+typedef void (*dxg2_free_fn)(void* ptr, void* stream, void* ctx);
+int dxg2_set_allocator(dxg2_alloc_fn alloc, dxg2_free_fn free_fn, void* ctx);
+
 typedef struct dxg2_plan_desc {
     int32_t height;               /* frame rows (H) */
     int32_t width;                /* frame columns (W) */
@@ -63,7 +76,14 @@ typedef struct dxg2_plan_desc {
  * (correlator/dense.py:120-126), DenseCorrelator.__init__ (dense.py:149-213),
  * FFTCorrelator.__init__ (dense.py:277-300), EventCorrelator.__init__ (event.py:15-101). */
 int dxg2_plan_create(const dxg2_plan_desc* desc, dxg2_plan** out);
+/* Waits for the device, then releases the plan. */
 int dxg2_plan_destroy(dxg2_plan* plan);
+/* Releases the plan without waiting for the device: its buffers go to the plan's
+ * allocator's free with `stream`.  The caller guarantees that no call on the plan is
+ * still running.  With the built-in allocator this waits like dxg2_plan_destroy (hipFree);
+ * with a caching allocator it returns at once.  For serving loops.  No reference
+ * counterpart. */
+int dxg2_plan_destroy_async(dxg2_plan* plan, void* stream);
 
 /* Host-only queries (no device work).  pixel_counts / scale may be NULL.
  * Mirrors the n_bins / bins / scale_factors attributes (common.py:36-72). */
@@ -84,6 +104,22 @@ int dxg2_plan_footprint(const dxg2_plan* plan, int32_t path, size_t* bytes);
  * DenseCorrelator.correlate (dense.py:223-272 with densecorrelator.cl:6-176). */
 int dxg2_correlate_dense(dxg2_plan* plan, const void* d_frames, int64_t* d_num,
                          int64_t* d_sums, double* d_den, float* d_g2, void* stream);
+
+/* Grouped dense launch over several plans (continuous batching of dense jobs): each
+ * plan's gather, then ONE Gram launch over the work items of every plan (plus one for the
+ * cross-plane items of uint16 plans), then each plan's finalise, all on `stream`.  Plans
+ * must share the Gram tile size and appear once each.  Array arguments hold one entry per
+ * job (d_num, d_sums, d_den may be NULL arrays).  Results equal those of
+ * dxg2_correlate_dense on each plan alone.  The batch object owns the device job table
+ * and work list (grown as needed) and is used by one host thread at a time.
+ * No reference counterpart: the serving loop's grouped submission. */
+typedef struct dxg2_batch dxg2_batch;
+int dxg2_batch_create(dxg2_batch** out);
+int dxg2_batch_destroy(dxg2_batch* batch);
+int dxg2_batch_correlate_dense(dxg2_batch* batch, int32_t n, dxg2_plan* const* plans,
+                               const void* const* d_frames, int64_t* const* d_num,
+                               int64_t* const* d_sums, double* const* d_den,
+                               float* const* d_g2, void* stream);
 
 /* Plain comparator: one bin per launch, FP32 gather, FP32 full-square Gram, FP32
  * diagonal sums and FP32 mean outer-product denominator.
@@ -156,6 +192,17 @@ int dxg2_synth_events(const float* d_mean_map, int32_t height, int32_t width, in
                       double tau_c, uint64_t seed, int32_t dtype, uint32_t* d_offsets,
                       int32_t* d_times, void* d_data, int64_t capacity, int64_t* nnz,
                       void* stream);
+
+/* Frame window of a pixel-major event list: the events with frames in [t0, t1), frames
+ * re-based to t - t0, into d_out_* (offsets H*W+1 = npix+1 entries; capacity >= nnz, the
+ * window never holds more events than the input).  d_scratch: (npix + 1023) / 1024 + 1
+ * uint32 of device workspace.  Needs the input's frames ascending within each pixel.
+ * Stream-ordered; no host synchronisation.  No reference counterpart (feeds jobs that
+ * cover part of a recorded series). */
+int dxg2_events_window(const uint32_t* d_offsets, const int32_t* d_times, const void* d_data,
+                       int64_t nnz, int64_t npix, int32_t dtype, int32_t t0, int32_t t1,
+                       uint32_t* d_scratch, uint32_t* d_out_offsets, int32_t* d_out_times,
+                       void* d_out_data, int64_t capacity, void* stream);
 
 /* Pixel-major events -> frame-sorted COO (pixel, frame, count), pixel order within a
  * frame unspecified.  Waits for the stream.  Test-data utility; no reference counterpart. */
