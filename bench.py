@@ -125,6 +125,40 @@ def bench_fft(plan, frames, nb, n, pf):
             "fft_length": 2 * L, "spectra_gflops": flop / (t[1] * 1e-3) / 1e9}
 
 
+def bench_std(plan, frames, nb, n, pf, base_ms, reps=2):
+    """Dense path with calc_std on the same stack: dev(tau) from per-tile ratio passes."""
+    import torch
+    from dynamix_amd import _lib
+    num = torch.empty((nb, n), dtype=torch.int64, device="cuda")
+    sums = torch.empty_like(num)
+    den = torch.empty((nb, n), dtype=torch.float64, device="cuda")
+    g2 = torch.empty((nb, n), dtype=torch.float32, device="cuda")
+    dev = torch.empty_like(g2)
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+    def step():
+        _lib.check(plan.lib.dxg2_correlate_dense_std(plan.handle, _lib.ptr(frames), _lib.ptr(num),
+                                                     _lib.ptr(sums), _lib.ptr(den), _lib.ptr(g2),
+                                                     _lib.ptr(dev), st))
+    step()
+    torch.cuda.synchronize()
+    plan.set_profiling(True)
+    ms = _timed(step, reps, torch)
+    t = plan.stage_times()
+    plan.set_profiling(False)
+    size = ctypes.c_size_t()
+    _lib.check(plan.lib.dxg2_plan_footprint(plan.handle, 4, ctypes.byref(size)))
+    finite = torch.isfinite(dev).float().mean().item()
+    out = {"ms": ms, "value": pf / (ms * 1e-3), "unit": "pixel*frames/s",
+           "overhead_vs_res_only": ms / base_ms,
+           "stage_ms": {"gather_and_sums": t[0], "gram_store_and_ratio_passes": t[1],
+                        "finalise_and_dev": t[2]},
+           "workspace_bytes": size.value, "finite_fraction": finite}
+    del num, sums, den, g2, dev
+    torch.cuda.empty_cache()
+    return out
+
+
 def bench_events(seed):
     """Event path on the events workload: pixel-major and frame-sorted COO input."""
     import numpy as np
@@ -256,6 +290,7 @@ def main():
     ap.add_argument("--no-fft", action="store_true")
     ap.add_argument("--no-events", action="store_true")
     ap.add_argument("--no-uint16", action="store_true")
+    ap.add_argument("--no-std", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--seed", type=int, default=2026)
     ap.add_argument("--pmc-copy-gib", type=float, default=0.0,
@@ -360,6 +395,9 @@ def main():
     if not args.no_fft:
         log("fft path")
         out["fft"] = bench_fft(plan, frames, nb, n, pf)
+    if not args.no_std:
+        log("calc_std path")
+        out["calc_std"] = bench_std(plan, frames, nb, n, pf, ms)
     if not args.no_cpu:
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(frames, q)

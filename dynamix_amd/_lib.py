@@ -54,6 +54,7 @@ _SIGNATURES = {
     "dxg2_plan_query": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "dxg2_plan_footprint": (c_int, [c_void_p, c_int, ctypes.POINTER(c_size_t)]),
     "dxg2_correlate_dense": (c_int, [c_void_p] * 7),
+    "dxg2_correlate_dense_std": (c_int, [c_void_p] * 8),
     "dxg2_correlate_dense_plain": (c_int, [c_void_p] * 4),
     "dxg2_batch_create": (c_int, [ctypes.POINTER(c_void_p)]),
     "dxg2_batch_destroy": (c_int, [c_void_p]),

@@ -48,7 +48,7 @@ class _DenseHIP(BaseCorrelator):
             self._plans[dtype] = p
         return p
 
-    def _run_dense(self, frames):
+    def _run_dense(self, frames, calc_std=False):
         torch = _torch()
         d_frames, dtype = to_device_frames(frames, self.shape, self.nframes, torch)
         if dtype not in (np.dtype(np.uint8), np.dtype(np.int8), np.dtype(np.uint16)):
@@ -60,15 +60,21 @@ class _DenseHIP(BaseCorrelator):
         sums = torch.empty((nb, n), dtype=torch.int64, device="cuda")
         den = torch.empty((nb, n), dtype=torch.float64, device="cuda")
         g2 = torch.empty((nb, n), dtype=torch.float32, device="cuda")
-        if nb:
+        dev = torch.empty((nb, n), dtype=torch.float32, device="cuda") if calc_std else None
+        if nb and calc_std:
+            _lib.check(plan.lib.dxg2_correlate_dense_std(
+                plan.handle, _lib.ptr(d_frames), _lib.ptr(num), _lib.ptr(sums), _lib.ptr(den),
+                _lib.ptr(g2), _lib.ptr(dev), current_stream(torch)), "dxg2_correlate_dense_std")
+        elif nb:
             _lib.check(plan.lib.dxg2_correlate_dense(plan.handle, _lib.ptr(d_frames), _lib.ptr(num),
                                                      _lib.ptr(sums), _lib.ptr(den), _lib.ptr(g2),
                                                      current_stream(torch)),
                        "dxg2_correlate_dense")
         self.last_numerators, self.last_sums, self.last_denominators, self.last_g2 = num, sums, den, g2
-        if self.extra_options.get("return_device", False):
-            return g2
-        return g2.cpu().numpy()
+        if not self.extra_options.get("return_device", False):
+            g2 = g2.cpu().numpy()
+            dev = dev.cpu().numpy() if calc_std else None
+        return CorrelationResult(g2, dev) if calc_std else g2
 
 
 class MatMulCorrelator(_DenseHIP):
@@ -81,9 +87,11 @@ class MatMulCorrelator(_DenseHIP):
         self._init_dense()
 
     def correlate(self, frames, calc_std=False):
-        if calc_std:
-            raise NotImplementedError("calc_std is not implemented on the HIP path yet")
-        return self._run_dense(frames)
+        """g2 (n_bins, nframes) float32; with calc_std a CorrelationResult(res, dev) whose
+        dev(tau) is the population std over t of the ratios n_b G(t, t+tau) / (S(t)
+        S(t+tau)) divided by sqrt(nframes - tau) (dense.py:58-59,65-66), computed in
+        FP64 from the exact integer Gram."""
+        return self._run_dense(frames, calc_std=bool(calc_std))
 
 
 class DenseCorrelator(_DenseHIP):

@@ -166,6 +166,25 @@ struct dxg2_plan {
     float* d_pm = nullptr;
     size_t pa_elems = 0;
 
+    // calc_std (built on the first dxg2_correlate_dense_std call): per chunk of tiles,
+    // the plain and cross work items (each with its scratch slot) and the tiles.
+    struct StdChunk {
+        int tile0, tile1;  // [tile0, tile1) of std_tiles
+        int m0, m1;        // of std_mwork
+        int x0, x1;        // of std_xwork
+    };
+    bool std_built = false;
+    std::vector<GramWork> std_mwork, std_xwork;
+    std::vector<StdTile> std_tiles;
+    std::vector<StdChunk> std_chunks;
+    size_t std_slots = 0;         // scratch slots of the largest chunk
+    size_t std_parts = 0;         // StdPart entries: n_bins * T(T+1)/2 tiles * (2T - 1)
+    GramWork* d_std_mwork = nullptr;
+    GramWork* d_std_xwork = nullptr;
+    StdTile* d_std_tiles = nullptr;
+    int* d_std_scratch = nullptr;
+    StdPart* d_std_part = nullptr;
+
     // event path
     std::vector<EventWork> event_work;
     EventWork* d_ework = nullptr;
@@ -227,7 +246,7 @@ void build_gram_work(dxg2_plan* p) {
             for (int i0 = 0; i0 < nT; i0 += band)
                 for (int j = i0; j < nT; ++j)
                     for (int i = i0; i < std::min(nT, i0 + band) && i <= j; ++i)
-                        dst.push_back(GramWork{b, i, j, k0, k1, out, cross, 0});
+                        dst.push_back(GramWork{b, i, j, k0, k1, out, cross, 0, 0, 0});
         }
     };
     const int nb = p->n_bins;
@@ -242,12 +261,130 @@ void build_gram_work(dxg2_plan* p) {
             triangle(b, 0, 2 * nkb, 2 * nb + b, 1);
         }
     }
-    GramWork pad{-1, 0, 0, 0, 0, 0, 0, 0};  // job 0
+    GramWork pad{-1, 0, 0, 0, 0, 0, 0, 0, 0, 0};  // job 0
     p->gram_work = group > 0 ? remap_for_xcds(raster, group, pad) : raster;
     p->gram_xwork = group > 0 ? remap_for_xcds(xraster, group, pad) : xraster;
     p->gram_raster = std::move(raster);
     p->gram_xraster = std::move(xraster);
     p->gram_group = group;
+}
+
+// calc_std work: every upper-triangle tile of every bin, its work items (k-ranges of at
+// most MAX_KBLOCKS_PER_ITEM; for uint16 the HH, LL and cross items) numbered by scratch
+// slot, cut into chunks whose slots fit `cap_bytes` of int32 tiles.  Tiles follow the
+// band order of the plain launch; each chunk's items are dealt to the XCDs.
+void build_std_work(dxg2_plan* p, size_t cap_bytes) {
+    const int T = p->gram_tile;
+    const int nT = (p->N + T - 1) / T;
+    const int64_t ntri = (int64_t)nT * (nT + 1) / 2;
+    const int band = std::max(1, env_int("DXG2_GRAM_BAND", 8));
+    const int group = std::max(0, env_int("DXG2_GRAM_GROUP", 64));
+    const int kchunk = std::min(MAX_KBLOCKS_PER_ITEM,
+                                std::max(1, env_int("DXG2_GRAM_KCHUNK", MAX_KBLOCKS_PER_ITEM)));
+    const bool wide = slab_planes(p->dtype) == 2;
+    const size_t tile_bytes = (size_t)T * T * 4;
+    int max_items = 0;
+    for (const auto& bd : p->bins) {
+        const int n1 = (bd.nkb + kchunk - 1) / kchunk;
+        max_items = std::max(max_items, wide ? 2 * n1 + (2 * bd.nkb + kchunk - 1) / kchunk : n1);
+    }
+    const size_t cap_slots = std::max<size_t>((size_t)std::max(max_items, 1), cap_bytes / tile_bytes);
+    std::vector<GramWork> cm, cx;  // current chunk's items, in tile order
+    size_t slots = 0;
+    GramWork pad{-1, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    auto close_chunk = [&]() {  // once per chunk, before the next one opens and at the end
+        if (p->std_chunks.empty()) return;
+        auto& ch = p->std_chunks.back();
+        ch.tile1 = (int)p->std_tiles.size();
+        auto m = group > 0 ? remap_for_xcds(cm, group, pad) : cm;
+        auto x = group > 0 ? remap_for_xcds(cx, group, pad) : cx;
+        ch.m0 = (int)p->std_mwork.size();
+        p->std_mwork.insert(p->std_mwork.end(), m.begin(), m.end());
+        ch.m1 = (int)p->std_mwork.size();
+        ch.x0 = (int)p->std_xwork.size();
+        p->std_xwork.insert(p->std_xwork.end(), x.begin(), x.end());
+        ch.x1 = (int)p->std_xwork.size();
+        p->std_slots = std::max(p->std_slots, slots);
+        cm.clear();
+        cx.clear();
+        slots = 0;
+    };
+    const int nb = p->n_bins;
+    for (int b = 0; b < nb; ++b) {
+        const int nkb = p->bins[b].nkb;
+        if (nkb == 0) continue;
+        const int n_hh = (nkb + kchunk - 1) / kchunk;
+        const int n_x = wide ? (2 * nkb + kchunk - 1) / kchunk : 0;
+        const int n_items = n_hh * (wide ? 2 : 1) + n_x;
+        for (int i0 = 0; i0 < nT; i0 += band)
+            for (int j = i0; j < nT; ++j)
+                for (int i = i0; i < std::min(nT, i0 + band) && i <= j; ++i) {
+                    if (p->std_chunks.empty() || slots + n_items > cap_slots) {
+                        close_chunk();
+                        p->std_chunks.push_back({(int)p->std_tiles.size(), -1, 0, 0, 0, 0});
+                    }
+                    const int idx = (int)(b * ntri + (int64_t)i * nT - (int64_t)i * (i - 1) / 2 + (j - i));
+                    p->std_tiles.push_back(StdTile{b, i, j, (int)slots, n_hh, wide ? n_hh : 0, n_x, idx});
+                    auto add = [&](int k0, int k1, int out, int cross) {
+                        (cross ? cx : cm).push_back(
+                            GramWork{b, i, j, k0, k1, out, cross, 0, (int)slots++, 0});
+                    };
+                    for (int k0 = 0; k0 < nkb; k0 += kchunk) add(k0, std::min(nkb, k0 + kchunk), b, 0);
+                    if (wide) {
+                        for (int k0 = nkb; k0 < 2 * nkb; k0 += kchunk)
+                            add(k0, std::min(2 * nkb, k0 + kchunk), nb + b, 0);
+                        for (int k0 = 0; k0 < 2 * nkb; k0 += kchunk)
+                            add(k0, std::min(2 * nkb, k0 + kchunk), 2 * nb + b, 1);
+                    }
+                }
+    }
+    close_chunk();
+    p->std_parts = (size_t)std::max(nb, 1) * ntri * (2 * T - 1);
+}
+
+size_t std_scratch_cap() {
+    return (size_t)std::max(1, env_int("DXG2_STD_SCRATCH_MB", 4096)) << 20;
+}
+
+int ensure_std_buffers(dxg2_plan* p, hipStream_t s) {
+    if (p->std_built) return DXG2_OK;
+    build_std_work(p, std_scratch_cap());
+    const size_t T = p->gram_tile;
+    auto upload = [&](void** dst, const void* src, size_t bytes) -> int {
+        if (bytes == 0) return DXG2_OK;
+        if (dev_alloc(p->alloc, dst, bytes, s) != hipSuccess)
+            return set_error(DXG2_ENOMEM, "cannot allocate %zu bytes of calc_std work", bytes);
+        DXG2_HIP(hipMemcpyAsync(*dst, src, bytes, hipMemcpyHostToDevice, s));
+        return DXG2_OK;
+    };
+    int rc = upload((void**)&p->d_std_mwork, p->std_mwork.data(), p->std_mwork.size() * sizeof(GramWork));
+    if (!rc) rc = upload((void**)&p->d_std_xwork, p->std_xwork.data(), p->std_xwork.size() * sizeof(GramWork));
+    if (!rc) rc = upload((void**)&p->d_std_tiles, p->std_tiles.data(), p->std_tiles.size() * sizeof(StdTile));
+    if (!rc && dev_alloc(p->alloc, (void**)&p->d_std_scratch, std::max<size_t>(p->std_slots, 1) * T * T * 4, s) != hipSuccess)
+        rc = set_error(DXG2_ENOMEM, "cannot allocate %zu calc_std tile slots", p->std_slots);
+    if (!rc && dev_alloc(p->alloc, (void**)&p->d_std_part, p->std_parts * sizeof(StdPart), s) != hipSuccess)
+        rc = set_error(DXG2_ENOMEM, "cannot allocate calc_std diagonal segments");
+    if (rc) return rc;
+    // the host lists are uploaded from pageable memory: wait before anything may change them
+    DXG2_HIP(hipStreamSynchronize(s));
+    p->std_built = true;
+    return DXG2_OK;
+}
+
+size_t std_footprint(const dxg2_plan* p) {
+    const size_t T = p->gram_tile;
+    const size_t nT = (p->N + T - 1) / T;
+    const size_t tiles = (size_t)std::max(p->n_bins, 1) * nT * (nT + 1) / 2;
+    if (p->std_built)
+        return (p->std_mwork.size() + p->std_xwork.size()) * sizeof(GramWork) +
+               p->std_tiles.size() * sizeof(StdTile) + std::max<size_t>(p->std_slots, 1) * T * T * 4 +
+               p->std_parts * sizeof(StdPart);
+    // before the first call: every tile's items bounded by the scratch cap
+    const int planes = slab_planes(p->dtype);
+    const size_t items = tiles * (planes == 2 ? 4 : 1);
+    return items * (sizeof(GramWork) + 4) + tiles * sizeof(StdTile) +
+           std::min(items * T * T * 4, std_scratch_cap() + (size_t)T * T * 4 * 8) +
+           tiles * (2 * T - 1) * sizeof(StdPart);
 }
 
 // Next unused event quad of the profiling pool (created on first use).
@@ -569,7 +706,8 @@ static void plan_release(dxg2_plan* p, hipStream_t s) {
                     p->d_pg, p->d_pm,
                     p->d_ework, p->d_err,  p->d_cnt,   p->d_scan,  p->d_off,
                     p->d_etimes, p->d_edata, p->d_kbwork, p->d_xs, p->d_tw,
-                    p->d_power, p->d_fnum};
+                    p->d_power, p->d_fnum, p->d_std_mwork, p->d_std_xwork,
+                    p->d_std_tiles, p->d_std_scratch, p->d_std_part};
     for (void* b : bufs) dev_free(p->alloc, b, s);
     for (auto& q : p->ev_pool)
         for (auto e : q)
@@ -631,6 +769,9 @@ int dxg2_plan_footprint(const dxg2_plan* p, int32_t path, size_t* bytes) {
                      (size_t)p->ev_cap * 6;
             return DXG2_OK;
         }
+        case 4:
+            *bytes = std_footprint(p);
+            return DXG2_OK;
         default:
             return set_error(DXG2_EINVAL, "unknown path %d", path);
     }
@@ -687,6 +828,45 @@ int dxg2_correlate_dense(dxg2_plan* p, const void* d_frames, int64_t* d_num, int
                    (int)p->gram_xwork.size(), p->N, p->nrows, biased, p->d_ugram, gsums, s);
     if (ev) DXG2_HIP(hipEventRecord(ev[2], s));
     dense_finish(p, gsums, d_num, d_sums, d_den, d_g2, s);
+    if (ev) DXG2_HIP(hipEventRecord(ev[3], s));
+    DXG2_HIP(hipGetLastError());
+    return DXG2_OK;
+}
+
+int dxg2_correlate_dense_std(dxg2_plan* p, const void* d_frames, int64_t* d_num,
+                             int64_t* d_sums, double* d_den, float* d_g2, float* d_dev,
+                             void* stream) {
+    if (!p) return set_error(DXG2_EINVAL, "null plan");
+    if (!d_frames || !d_g2 || !d_dev)
+        return set_error(DXG2_EINVAL, "frames, g2 and dev must be non-null");
+    if (p->n_bins == 0) return DXG2_OK;
+    hipStream_t s = (hipStream_t)stream;
+    int rc = ensure_dense_buffers(p, s);
+    if (!rc) rc = ensure_std_buffers(p, s);
+    if (rc) return rc;
+    (void)hipGetLastError();
+    hipEvent_t* ev = nullptr;
+    if (p->profiling && (rc = next_events(p, &ev))) return rc;
+    if (ev) DXG2_HIP(hipEventRecord(ev[0], s));
+    int64_t* gsums = dense_gather(p, d_frames, d_sums, s);
+    launch_slab_sums(p->d_slab, p->dtype, p->d_bins, p->d_gwork, (int)p->gather_work.size(),
+                     p->n_bins, p->N, p->nrows, gsums, s);
+    if (ev) DXG2_HIP(hipEventRecord(ev[1], s));
+    const int biased = p->dtype == DXG2_I8 ? 0 : 1;
+    const int mode = p->dtype == DXG2_U16 ? 2 : (p->dtype == DXG2_U8 ? 1 : 0);
+    for (const auto& ch : p->std_chunks) {
+        launch_gram_i8_store(p->gram_tile, false, p->d_slab, p->d_bins, p->d_std_mwork + ch.m0,
+                             ch.m1 - ch.m0, p->N, p->nrows, biased, p->d_ugram,
+                             p->d_std_scratch, s);
+        launch_gram_i8_store(p->gram_tile, true, p->d_slab, p->d_bins, p->d_std_xwork + ch.x0,
+                             ch.x1 - ch.x0, p->N, p->nrows, biased, p->d_ugram,
+                             p->d_std_scratch, s);
+        launch_std_tiles(p->gram_tile, p->d_std_tiles + ch.tile0, ch.tile1 - ch.tile0, p->d_bins,
+                         p->d_std_scratch, gsums, mode, p->n_bins, p->N, p->d_std_part, s);
+    }
+    if (ev) DXG2_HIP(hipEventRecord(ev[2], s));
+    dense_finish(p, gsums, d_num, d_sums, d_den, d_g2, s);
+    launch_dev_finalize(p->d_std_part, p->d_bins, p->n_bins, p->N, p->gram_tile, d_dev, s);
     if (ev) DXG2_HIP(hipEventRecord(ev[3], s));
     DXG2_HIP(hipGetLastError());
     return DXG2_OK;
@@ -767,7 +947,7 @@ int dxg2_batch_correlate_dense(dxg2_batch* b, int32_t n, dxg2_plan* const* plans
         }
     }
     const int group = plans[0]->gram_group;
-    GramWork pad{-1, 0, 0, 0, 0, 0, 0, 0};
+    GramWork pad{-1, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     if (group > 0) {
         plain = remap_for_xcds(plain, group, pad);
         cross = remap_for_xcds(cross, group, pad);

@@ -45,6 +45,8 @@ struct GramWork {        // one output tile (i, j >= i) over a k-block range
     int32_t out;         // row of the numerator (and frame-sum) outputs it adds into
     int32_t cross;       // 1: B operand is the other digit plane ([H|L] x [L|H]^T)
     int32_t job;         // grouped launches over several plans: index into GramJob[]
+    int32_t slot;        // calc_std launches: tile scratch slot of this item
+    int32_t pad;
 };
 
 struct GramJob {         // one plan's buffers in a grouped Gram launch
@@ -56,6 +58,23 @@ struct GramJob {         // one plan's buffers in a grouped Gram launch
     int32_t nrows;
     int32_t biased;
     int32_t pad;
+};
+
+struct StdTile {         // calc_std: one upper-triangle tile of one bin
+    int32_t bin;
+    int32_t ti;
+    int32_t tj;
+    int32_t slot0;       // first scratch slot of its work items (HH, then LL, then cross)
+    int32_t n_hh;        // items of each kind (uint8/int8: all items are n_hh)
+    int32_t n_ll;
+    int32_t n_x;
+    int32_t idx;         // bin * T(T+1)/2 + row-major upper-triangle index of (ti, tj)
+};
+
+struct StdPart {         // one diagonal segment of one tile: count, mean, sum of squared
+    double n;            // deviations of its ratios
+    double mean;
+    double m2;
 };
 
 // Digit planes of a bin's slab: uint16 frames are split into a high-byte plane H
@@ -134,6 +153,20 @@ void launch_to_coo(int dtype, const int32_t* times, const void* data, const uint
                    void* cnt, hipStream_t s);
 void launch_synth(void* frames, int dtype, int nframes, int64_t hw, const float* mean_map,
                   float rho, uint64_t seed, hipStream_t s);
+
+// calc_std launches: the Gram that also stores every item's int32 tile in its slot,
+// frame sums from the slab, per-tile diagonal segments of the ratios, and dev.
+void launch_gram_i8_store(int tile, bool cross, const int8_t* slab, const BinDev* bins,
+                          const GramWork* work, int n_work, int nframes, int nrows, int biased,
+                          int64_t* ugram, int* tiles, hipStream_t s);
+void launch_slab_sums(const int8_t* slab, int dtype, const BinDev* bins, const GatherWork* work,
+                      int n_work, int n_bins, int nframes, int nrows, int64_t* sums,
+                      hipStream_t s);
+void launch_std_tiles(int tile, const StdTile* tiles, int n_tiles, const BinDev* bins,
+                      const int* scratch, const int64_t* sums, int mode, int n_bins, int nframes,
+                      StdPart* part, hipStream_t s);
+void launch_dev_finalize(const StdPart* part, const BinDev* bins, int n_bins, int nframes,
+                         int tile, float* dev, hipStream_t s);
 
 int gram_tile_size(int variant);
 

@@ -96,7 +96,7 @@ struct GramLds {
     unsigned int* diag;
 };
 
-template <int WM, int WN, bool DIAG, bool CROSS>
+template <int WM, int WN, bool DIAG, bool CROSS, bool STORE = false>
 struct GramTile {
     using S = GramShape<WM, WN>;
     static constexpr int L = DIAG ? S::LOADS : 2 * S::LOADS;  // DMA per thread per stage
@@ -170,8 +170,11 @@ struct GramTile {
         }
     }
 
+    // STORE: also write the int32 tile to `tile` (row-major T x T) and leave the frame
+    // sums to a separate pass (`sums` unused).
     __device__ __forceinline__ void run(int biased, int64_t* __restrict__ out,
-                                        int64_t* __restrict__ sums) {
+                                        int64_t* __restrict__ sums,
+                                        int* __restrict__ tile = nullptr) {
         unsigned int* sDiag = lds.diag;
         for (int i = tid; i < 2 * S::NDIAG; i += 256) sDiag[i] = 0u;
 #pragma unroll
@@ -192,7 +195,7 @@ struct GramTile {
             if (kb + 3 < kb1) stage<3>(kb + 3);
         }
 
-        if (DIAG && tid < S::TM) {
+        if (DIAG && !STORE && tid < S::TM) {
             const int t = row0 + tid;
             const int64_t v = (int64_t)rowsum + (biased ? (int64_t)128 * (kb1 - kb0) * KB : 0);
             if (t < nframes && v != 0)
@@ -220,6 +223,21 @@ struct GramTile {
                 }
                 __builtin_amdgcn_sched_barrier(0);
             }
+        if (STORE) {
+            // Lanes 0-31 and 32-63 of one store write two 128-byte row segments.
+#pragma unroll
+            for (int mi = 0; mi < WM; ++mi)
+#pragma unroll
+                for (int ni = 0; ni < WN; ++ni) {
+                    if (live(wr, wc, mi, ni)) {
+                        const int c = wc * WN * 32 + ni * 32 + (lane & 31);
+                        const int rbase = wr * WM * 32 + mi * 32 + 4 * (lane >> 5);
+#pragma unroll
+                        for (int v = 0; v < 16; ++v)
+                            tile[(rbase + 8 * (v >> 2) + (v & 3)) * S::TN + c] = acc[mi][ni][v];
+                    }
+                }
+        }
         __syncthreads();
         for (int i = tid; i < S::NDIAG; i += 256) {
             const long long v =
@@ -264,6 +282,44 @@ __global__ __launch_bounds__(256) void gram_i8_kernel(const int8_t* __restrict__
                                          w.tj * S::TN, nframes, nrows, tid, tid & 63,
                                          wave >> 1, wave & 1, bd.nkb, lds};
         t.run(biased, out, srow);
+    }
+}
+
+// calc_std variant: the same tiles (work items carry a scratch slot), which also write
+// their int32 tile to tiles + slot * T * T for the per-element ratio pass.  Frame sums
+// come from a separate pass over the slab, so that a chunk of tiles can be reduced as
+// soon as its Gram is done.
+template <int WM, int WN, bool CROSS>
+__global__ __launch_bounds__(256) void gram_i8_store_kernel(
+    const int8_t* __restrict__ slab, const BinDev* __restrict__ bins,
+    const GramWork* __restrict__ work, int nframes, int nrows, int biased,
+    int64_t* __restrict__ ugram, int* __restrict__ tiles) {
+    using S = GramShape<WM, WN>;
+    static_assert(S::TM == S::TN, "square tiles");
+    __shared__ __attribute__((aligned(1024))) int8_t a0[S::PANEL], a1[S::PANEL], a2[S::PANEL],
+        a3[S::PANEL];
+    __shared__ __attribute__((aligned(1024))) int8_t b0[S::PANEL], b1[S::PANEL], b2[S::PANEL],
+        b3[S::PANEL];
+    __shared__ unsigned int sDiag[2 * S::NDIAG];
+
+    const GramWork w = work[blockIdx.x];
+    if (w.bin < 0) return;  // padding item
+    const BinDev bd = bins[w.bin];
+    const GramLds<WM, WN> lds{{a0, a1, a2, a3}, {b0, b1, b2, b3}, sDiag};
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6;
+    int64_t* out = ugram + (int64_t)w.out * nframes;
+    int* tile = tiles + (int64_t)w.slot * S::TM * S::TN;
+    if (!CROSS && w.ti == w.tj) {
+        GramTile<WM, WN, true, false, true> t{slab + bd.slab_off, w.kb0, w.kb1, w.ti * S::TM,
+                                              w.tj * S::TN, nframes, nrows, tid, tid & 63,
+                                              wave >> 1, wave & 1, 0, lds};
+        t.run(biased, out, nullptr, tile);
+    } else {
+        GramTile<WM, WN, false, CROSS, true> t{slab + bd.slab_off, w.kb0, w.kb1, w.ti * S::TM,
+                                               w.tj * S::TN, nframes, nrows, tid, tid & 63,
+                                               wave >> 1, wave & 1, bd.nkb, lds};
+        t.run(biased, out, nullptr, tile);
     }
 }
 
@@ -320,6 +376,28 @@ void launch_gram_i8_jobs(int tile, bool cross, const GramJob* jobs, const GramWo
 }
 
 int gram_tile_size(int variant) { return variant == 1 ? 128 : 256; }
+
+void launch_gram_i8_store(int tile, bool cross, const int8_t* slab, const BinDev* bins,
+                          const GramWork* work, int n_work, int nframes, int nrows, int biased,
+                          int64_t* ugram, int* tiles, hipStream_t s) {
+    if (n_work <= 0) return;
+    const dim3 grid((unsigned)n_work), block(256);
+    if (tile == 128) {
+        if (cross)
+            gram_i8_store_kernel<2, 2, true><<<grid, block, 0, s>>>(slab, bins, work, nframes,
+                                                                    nrows, biased, ugram, tiles);
+        else
+            gram_i8_store_kernel<2, 2, false><<<grid, block, 0, s>>>(slab, bins, work, nframes,
+                                                                     nrows, biased, ugram, tiles);
+    } else {
+        if (cross)
+            gram_i8_store_kernel<4, 4, true><<<grid, block, 0, s>>>(slab, bins, work, nframes,
+                                                                    nrows, biased, ugram, tiles);
+        else
+            gram_i8_store_kernel<4, 4, false><<<grid, block, 0, s>>>(slab, bins, work, nframes,
+                                                                     nrows, biased, ugram, tiles);
+    }
+}
 
 void launch_gram_i8(int tile, bool cross, const int8_t* slab, const BinDev* bins,
                     const GramWork* work, int n_work, int nframes, int nrows, int biased,

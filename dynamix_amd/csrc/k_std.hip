@@ -1,0 +1,268 @@
+// calc_std: the spread of the per-frame-pair ratios along each diagonal.
+//
+// Reference (correlator/dense.py:58-59,65-66): for one bin with n_b pixels,
+//   dev(tau) = std_t( (G(t, t+tau) / n_b) / (m_t m_{t+tau}) ) / sqrt(N - tau),
+// the population std over t = 0 .. N-1-tau, with m_t = S(t) / n_b the frame mean.  The
+// ratio is r_t = n_b G(t, t+tau) / (S(t) S(t+tau)).  A zero frame sum on a diagonal
+// makes numpy's ratio inf or nan there, and the std nan.
+//
+// The ratio is not linear in G, so it needs every element of the Gram, not only the
+// diagonal sums.  The calc_std Gram launch writes each work item's int32 tile into a
+// scratch slot (work items of one tile: k-ranges, and for uint16 the HH, LL and cross
+// products), and one workgroup per tile here
+//   * forms the exact int64 G(t, u) = sum_w w U_w + c(t) + c(u) - Kc (bias corrections
+//     of the i8 digit planes, below) and r in FP64, a band of rows at a time in LDS;
+//   * walks each of the tile's 2T-1 diagonals and keeps (count, mean, M2) of its
+//     ratios, shifted by the segment's first ratio so that M2 does not cancel;
+// and the finaliser combines the segments of each diagonal (Chan, Golub and LeVeque's
+// pairwise update) in a fixed order, so dev is reproducible bit for bit.
+//
+// Bias corrections (bytes stored as v - 128, K = 64 nkb slots per plane, padded slots
+// hold v = 0):
+//   uint8:  G = U + 128 (S_t + S_u) - 2^14 K
+//   uint16: HH = U_HH + 128 (H_t + H_u) - 2^14 K,  LL likewise with L,
+//           X  = U_X + 128 (E_t + E_u) - 2^15 K with E = H + L,
+//           G  = 65536 HH + 256 X + LL
+//              = 65536 U_HH + 256 U_X + U_LL + c_t + c_u - 2^14 K 66049,
+//           c_t = 128 (65792 H_t + 257 L_t).
+#include "dxg2_internal.h"
+
+namespace dxg2 {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+// Per-frame bin sums straight from the slab (the calc_std launch needs them before its
+// first tile is reduced).  Block = 256 frames of one gather work item (<= 16 k-blocks).
+template <bool WIDE, bool BIAS>
+__global__ __launch_bounds__(256) void slab_sums_kernel(const int8_t* __restrict__ slab,
+                                                        const BinDev* __restrict__ bins,
+                                                        const GatherWork* __restrict__ work,
+                                                        int n_bins, int nframes, int nrows,
+                                                        int64_t* __restrict__ sums) {
+    const GatherWork w = work[blockIdx.x];
+    const int t = blockIdx.y * 256 + threadIdx.x;
+    if (t >= nframes) return;
+    const BinDev bd = bins[w.bin];
+    const int64_t plane = (int64_t)bd.nkb * nrows * KB;
+    int acc = 0, acc_l = 0;  // |sum| <= 16 * 64 * 128
+    for (int kb = w.kb0; kb < w.kb0 + w.nkb; ++kb) {
+        const v4i* row = reinterpret_cast<const v4i*>(slab + bd.slab_off +
+                                                      ((int64_t)kb * nrows + t) * KB);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const v4i x = row[q];
+            acc = __builtin_amdgcn_sdot4(x.x, 0x01010101, acc, false);
+            acc = __builtin_amdgcn_sdot4(x.y, 0x01010101, acc, false);
+            acc = __builtin_amdgcn_sdot4(x.z, 0x01010101, acc, false);
+            acc = __builtin_amdgcn_sdot4(x.w, 0x01010101, acc, false);
+        }
+        if (WIDE) {
+            const v4i* lrow = reinterpret_cast<const v4i*>(reinterpret_cast<const int8_t*>(row) +
+                                                           plane);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const v4i x = lrow[q];
+                acc_l = __builtin_amdgcn_sdot4(x.x, 0x01010101, acc_l, false);
+                acc_l = __builtin_amdgcn_sdot4(x.y, 0x01010101, acc_l, false);
+                acc_l = __builtin_amdgcn_sdot4(x.z, 0x01010101, acc_l, false);
+                acc_l = __builtin_amdgcn_sdot4(x.w, 0x01010101, acc_l, false);
+            }
+        }
+    }
+    const int64_t bias = BIAS ? (int64_t)128 * KB * w.nkb : 0;
+    const int64_t h = acc + bias;
+    if (h != 0)
+        atomicAdd(reinterpret_cast<unsigned long long*>(sums + (int64_t)w.bin * nframes + t),
+                  (unsigned long long)h);
+    if (WIDE) {
+        const int64_t l = acc_l + bias;
+        if (l != 0)
+            atomicAdd(reinterpret_cast<unsigned long long*>(
+                          sums + ((int64_t)n_bins + w.bin) * nframes + t),
+                      (unsigned long long)l);
+    }
+}
+
+void launch_slab_sums(const int8_t* slab, int dtype, const BinDev* bins, const GatherWork* work,
+                      int n_work, int n_bins, int nframes, int nrows, int64_t* sums,
+                      hipStream_t s) {
+    if (n_work <= 0 || nframes <= 0) return;
+    const dim3 grid((unsigned)n_work, (unsigned)((nframes + 255) / 256));
+    if (dtype == DXG2_U16)
+        slab_sums_kernel<true, true><<<grid, dim3(256), 0, s>>>(slab, bins, work, n_bins, nframes,
+                                                               nrows, sums);
+    else if (dtype == DXG2_U8)
+        slab_sums_kernel<false, true><<<grid, dim3(256), 0, s>>>(slab, bins, work, n_bins,
+                                                                nframes, nrows, sums);
+    else
+        slab_sums_kernel<false, false><<<grid, dim3(256), 0, s>>>(slab, bins, work, n_bins,
+                                                                 nframes, nrows, sums);
+}
+
+// Frame sum S(t) and bias term c(t) of frame t of bin b.  mode 0: int8 (no bias),
+// 1: uint8, 2: uint16 digit planes (sums holds S_H rows, then S_L rows).
+__device__ __forceinline__ void frame_terms(const int64_t* __restrict__ sums, int mode,
+                                            int n_bins, int nframes, int b, int t, int64_t& S,
+                                            int64_t& c) {
+    if (mode == 2) {
+        const int64_t h = sums[(int64_t)b * nframes + t];
+        const int64_t l = sums[((int64_t)n_bins + b) * nframes + t];
+        S = 256 * h + l;
+        c = 128 * (65792 * h + 257 * l);
+    } else {
+        S = sums[(int64_t)b * nframes + t];
+        c = mode == 1 ? 128 * S : 0;
+    }
+}
+
+template <int T>
+__global__ __launch_bounds__(256) void std_tile_kernel(const StdTile* __restrict__ tiles,
+                                                       const BinDev* __restrict__ bins,
+                                                       const int* __restrict__ scratch,
+                                                       const int64_t* __restrict__ sums,
+                                                       int mode, int n_bins, int nframes,
+                                                       StdPart* __restrict__ part) {
+    constexpr int BR = 8192 / T;        // rows per LDS band (64 KiB of doubles)
+    constexpr int RPP = 256 / T;        // rows per load pass
+    constexpr int NDIAG = 2 * T - 1;
+    constexpr int SPT = (NDIAG + 255) / 256;  // diagonals per thread
+    __shared__ double x[BR][T];         // ratio, NaN where the element is not counted
+
+    const StdTile tl = tiles[blockIdx.x];
+    const BinDev bd = bins[tl.bin];
+    const int tid = threadIdx.x;
+    const int c = tid % T;
+    const int rsub = tid / T;
+    const int t0 = tl.ti * T;
+    const int u = tl.tj * T + c;
+    const bool diag = tl.ti == tl.tj;
+    const int64_t K = (int64_t)KB * bd.nkb;
+    const int64_t Kc = mode == 0 ? 0 : (mode == 1 ? 16384 * K : 16384 * K * 66049);
+    const int w_hh = mode == 2 ? 65536 : 1;
+    const double npix = (double)bd.npix;
+    int64_t S_u = 0, c_u = 0;
+    if (u < nframes) frame_terms(sums, mode, n_bins, nframes, tl.bin, u, S_u, c_u);
+    const int64_t TT = (int64_t)T * T;
+    const int* base = scratch + (int64_t)tl.slot0 * TT;
+
+    int cnt[SPT];
+    double k0[SPT], s1[SPT], s2[SPT];
+#pragma unroll
+    for (int j = 0; j < SPT; ++j) {
+        cnt[j] = 0;
+        k0[j] = s1[j] = s2[j] = 0.0;
+    }
+
+    for (int r0 = 0; r0 < T && t0 + r0 < nframes; r0 += BR) {
+        __syncthreads();
+        for (int rr = rsub; rr < BR; rr += RPP) {
+            const int r = r0 + rr;
+            const int t = t0 + r;
+            double val = __builtin_nan("");
+            if (t < nframes && u < nframes && (!diag || c >= r)) {
+                int64_t S_t, c_t;
+                frame_terms(sums, mode, n_bins, nframes, tl.bin, t, S_t, c_t);
+                if (S_t != 0 && S_u != 0) {
+                    const int* e = base + (int64_t)r * T + c;
+                    int64_t g = c_t + c_u - Kc;
+                    int k = 0;
+                    for (; k < tl.n_hh; ++k) g += (int64_t)w_hh * e[k * TT];
+                    for (; k < tl.n_hh + tl.n_ll; ++k) g += e[k * TT];
+                    for (; k < tl.n_hh + tl.n_ll + tl.n_x; ++k) g += (int64_t)256 * e[k * TT];
+                    val = npix * (double)g / ((double)S_t * (double)S_u);
+                }
+            }
+            x[rr][c] = val;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < SPT; ++j) {
+            const int s = tid + 256 * j;
+            if (s < NDIAG) {
+                const int d = s - (T - 1);
+                const int lo = max(0, -d - r0);
+                const int hi = min(BR, T - d - r0);
+                for (int rr = lo; rr < hi; ++rr) {
+                    const double v = x[rr][r0 + rr + d];
+                    if (v == v) {
+                        if (cnt[j] == 0) k0[j] = v;
+                        const double dv = v - k0[j];
+                        s1[j] += dv;
+                        s2[j] = fma(dv, dv, s2[j]);
+                        ++cnt[j];
+                    }
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < SPT; ++j) {
+        const int s = tid + 256 * j;
+        if (s < NDIAG) {
+            StdPart p{0.0, 0.0, 0.0};
+            if (cnt[j] > 0) {
+                const double n = (double)cnt[j];
+                p.n = n;
+                p.mean = k0[j] + s1[j] / n;
+                p.m2 = fmax(s2[j] - s1[j] * s1[j] / n, 0.0);
+            }
+            part[(int64_t)tl.idx * NDIAG + s] = p;
+        }
+    }
+}
+
+void launch_std_tiles(int tile, const StdTile* tiles, int n_tiles, const BinDev* bins,
+                      const int* scratch, const int64_t* sums, int mode, int n_bins, int nframes,
+                      StdPart* part, hipStream_t s) {
+    if (n_tiles <= 0) return;
+    if (tile == 128)
+        std_tile_kernel<128><<<dim3((unsigned)n_tiles), dim3(256), 0, s>>>(
+            tiles, bins, scratch, sums, mode, n_bins, nframes, part);
+    else
+        std_tile_kernel<256><<<dim3((unsigned)n_tiles), dim3(256), 0, s>>>(
+            tiles, bins, scratch, sums, mode, n_bins, nframes, part);
+}
+
+// dev(tau) of every bin: the segments of diagonal tau lie in tiles (ti, ti + D) for
+// D = tau / T (slot tau - D T + T - 1) and D + 1 (slot tau - D T - 1, if >= 0).
+__global__ __launch_bounds__(256) void dev_finalize_kernel(const StdPart* __restrict__ part,
+                                                           const BinDev* __restrict__ bins,
+                                                           int nframes, int T, float* __restrict__ dev) {
+    const int b = blockIdx.x;
+    const int nT = (nframes + T - 1) / T;
+    const int64_t ntri = (int64_t)nT * (nT + 1) / 2;
+    const int ndiag = 2 * T - 1;
+    const bool empty = bins[b].npix == 0;
+    for (int tau = threadIdx.x; tau < nframes; tau += blockDim.x) {
+        double n = 0.0, mean = 0.0, m2 = 0.0;
+        const int D0 = tau / T;
+        for (int D = D0; D <= D0 + 1 && D < nT && !empty; ++D) {
+            const int s = tau - D * T + T - 1;
+            if (s < 0) continue;
+            for (int ti = 0; ti + D < nT; ++ti) {
+                const int64_t idx = (int64_t)b * ntri + (int64_t)ti * nT -
+                                    (int64_t)ti * (ti - 1) / 2 + D;
+                const StdPart p = part[idx * ndiag + s];
+                if (p.n == 0.0) continue;
+                const double nn = n + p.n;
+                const double delta = p.mean - mean;
+                mean += delta * (p.n / nn);
+                m2 += p.m2 + delta * delta * (n * p.n / nn);
+                n = nn;
+            }
+        }
+        float out = __builtin_nanf("");
+        if (!empty && n == (double)(nframes - tau))
+            out = (float)(sqrt(m2 / n) / sqrt(n));
+        dev[(int64_t)b * nframes + tau] = out;
+    }
+}
+
+void launch_dev_finalize(const StdPart* part, const BinDev* bins, int n_bins, int nframes,
+                         int tile, float* dev, hipStream_t s) {
+    if (n_bins <= 0 || nframes <= 0) return;
+    dev_finalize_kernel<<<dim3((unsigned)n_bins), dim3(256), 0, s>>>(part, bins, nframes, tile,
+                                                                     dev);
+}
+
+}  // namespace dxg2

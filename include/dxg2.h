@@ -90,8 +90,9 @@ int dxg2_plan_destroy_async(dxg2_plan* plan, void* stream);
 int dxg2_plan_query(const dxg2_plan* plan, int32_t* n_bins, int64_t* pixel_counts,
                     double* scale);
 
-/* Device bytes the plan holds for a path (0 dense, 1 plain, 2 fft, 3 events), for
- * admission control by memory footprint.  No reference counterpart. */
+/* Device bytes the plan holds for a path (0 dense, 1 plain, 2 fft, 3 events, 4 the
+ * extra workspaces of dense calc_std), for admission control by memory footprint.
+ * No reference counterpart. */
 int dxg2_plan_footprint(const dxg2_plan* plan, int32_t path, size_t* bytes);
 
 /* Dense batched path: qmask gather into per-bin i8 slabs (+ S_b(t)), one grouped
@@ -104,6 +105,20 @@ int dxg2_plan_footprint(const dxg2_plan* plan, int32_t path, size_t* bytes);
  * DenseCorrelator.correlate (dense.py:223-272 with densecorrelator.cl:6-176). */
 int dxg2_correlate_dense(dxg2_plan* plan, const void* d_frames, int64_t* d_num,
                          int64_t* d_sums, double* d_den, float* d_g2, void* stream);
+
+/* Dense path with calc_std: everything dxg2_correlate_dense computes, plus d_dev
+ * (n_bins, N) float32, dev(tau) = population std over t of the ratios
+ * r_t = n_b G(t, t+tau) / (S(t) S(t+tau)) divided by sqrt(N - tau), NaN where a frame sum
+ * on the diagonal is zero or the bin is empty.  Each Gram work item also writes its
+ * int32 tile to a scratch buffer (at most DXG2_STD_SCRATCH_MB MiB, default 4096; the
+ * launch is split into chunks of tiles that fit), and a per-tile pass forms r in FP64
+ * and keeps per-diagonal (count, mean, M2) segments, combined in a fixed order: dev is
+ * reproducible bit for bit.  d_num, d_sums, d_den, d_g2 are as for dxg2_correlate_dense.
+ * Replaces MatMulCorrelator.correlate(frames, calc_std=True) (dense.py:128-142; the
+ * std at dense.py:58-59,65-66). */
+int dxg2_correlate_dense_std(dxg2_plan* plan, const void* d_frames, int64_t* d_num,
+                             int64_t* d_sums, double* d_den, float* d_g2, float* d_dev,
+                             void* stream);
 
 /* Grouped dense launch over several plans (continuous batching of dense jobs): each
  * plan's gather, then ONE Gram launch over the work items of every plan (plus one for the

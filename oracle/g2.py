@@ -22,13 +22,18 @@ Parity pinning of each function (tests/test_oracle.py):
                                  (within its float32 rounding), single-pixel bin == 1,
                                  constant frames closed form.
   matmul_correlator_f32       -- bit-exact against golden vectors written by running
-                                 the reference py_dense_correlator (dense.py:32-70).
+                                 the reference py_dense_correlator (dense.py:32-70),
+                                 res and (calc_std) dev.
+  dense_dev_fp64              -- the golden dev vectors within the reference's float32
+                                 rounding; brute-force loops on a tiny stack.
   fft_correlator              -- against `numerators`/`denominators` within FP32
                                  tolerance; FFT length against the reference's
                                  get_next_power golden vectors (utils.py:50-60).
 """
 from bisect import bisect_left
 from itertools import product
+
+import math
 
 import numpy as np
 
@@ -197,13 +202,15 @@ def brute_force(frames, qmask):
 
 
 # --------------------------------------------------- reference float32 algorithm
-def py_dense_correlator_f32(frames, mask):
+def py_dense_correlator_f32(frames, mask, calc_std=False):
     """The reference numpy correlator for one bin, in its own float32 arithmetic.
 
     Follows dense.py:32-70 step for step (gather cast to float32 :49, float32 mean :50,
-    sgemm Gram :54, outer product of means :55, per-lag diagonal sums :61-64), so that
-    its float32 rounding matches the reference; it is the CPU baseline the GPU is timed
-    beside.  Returns float64 res of length N (the reference stores it into float32).
+    sgemm Gram :54, outer product of means :55, per-lag diagonal sums :61-64, and with
+    calc_std the std of the per-t ratios along each diagonal :65-66), so that its
+    float32 rounding matches the reference; it is the CPU baseline the GPU is timed
+    beside.  Returns float64 res of length N (the reference stores it into float32), or
+    (res, dev) with calc_std.
     """
     rows, cols = np.nonzero(mask > 0)
     a = np.asarray(frames[:, rows, cols], dtype=np.float32)
@@ -212,20 +219,54 @@ def py_dense_correlator_f32(frames, mask):
     gram = a @ a.T
     mm = m.T @ m
     res = np.zeros(nt)
-    for tau in range(nt):
-        dn = np.diagonal(gram, offset=tau) / npx
-        dd = np.diagonal(mm, offset=tau)
-        res[tau] = np.sum(dn) / np.sum(dd)
-    return res
+    dev = np.zeros(nt)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        for tau in range(nt):
+            dn = np.diagonal(gram, offset=tau) / npx
+            dd = np.diagonal(mm, offset=tau)
+            res[tau] = np.sum(dn) / np.sum(dd)
+            if calc_std:
+                # a Python float divisor keeps the quotient in float32 (math.sqrt at :66)
+                dev[tau] = np.std(dn / dd) / math.sqrt(len(dd))
+    return (res, dev) if calc_std else res
 
 
-def matmul_correlator_f32(frames, qmask=None):
-    """MatMulCorrelator.correlate (dense.py:128-142): one bin at a time, float32 out."""
+def matmul_correlator_f32(frames, qmask=None, calc_std=False):
+    """MatMulCorrelator.correlate (dense.py:128-142): one bin at a time, float32 out;
+    with calc_std also the float32 dev array."""
     q, bins = qmask_bins(frames.shape[1:], qmask)
     out = np.zeros((len(bins), frames.shape[0]), dtype=np.float32)
+    dev = np.zeros_like(out)
     for i, b in enumerate(bins):
-        out[i] = py_dense_correlator_f32(frames, q == b)
-    return out
+        r = py_dense_correlator_f32(frames, q == b, calc_std)
+        if calc_std:
+            out[i], dev[i] = r
+        else:
+            out[i] = r
+    return (out, dev) if calc_std else out
+
+
+def dense_dev_fp64(frames, qmask=None):
+    """dev(tau) of the reference's calc_std (dense.py:58-59,65-66) from exact integers:
+    per bin and lag, the population std over t of r_t = n_b G(t, t+tau) / (S(t) S(t+tau))
+    divided by sqrt(N - tau), in float64.  A zero frame sum on a diagonal gives inf/NaN
+    ratios, hence NaN, as numpy does."""
+    q, bins = qmask_bins(frames.shape[1:], qmask)
+    n = frames.shape[0]
+    dev = np.zeros((len(bins), n))
+    with np.errstate(divide="ignore", invalid="ignore"):
+        for i, b in enumerate(bins):
+            a = gather_bin(frames, q, b).astype(np.float64)
+            npx = a.shape[1]
+            if npx == 0:
+                dev[i] = np.nan
+                continue
+            g = a @ a.T               # exact: integer entries well below 2^53 in tests
+            s = a.sum(axis=1)
+            for tau in range(n):
+                r = npx * np.diagonal(g, offset=tau) / (s[:n - tau] * s[tau:])
+                dev[i, tau] = np.std(r) / np.sqrt(n - tau)
+    return dev
 
 
 # ------------------------------------------------------------------------ FFT path

@@ -23,6 +23,54 @@ def test_matmul_f32_bit_exact_vs_reference(golden, name):
 
 
 @pytest.mark.parametrize("name", DENSE_CASES)
+def test_matmul_f32_calc_std_bit_exact_vs_reference(golden, name):
+    frames, q, res, dev = _case(golden, name)
+    mine, mdev = og.matmul_correlator_f32(frames, q, calc_std=True)
+    np.testing.assert_array_equal(mine, res)
+    np.testing.assert_array_equal(mdev, dev)
+
+
+@pytest.mark.parametrize("name", DENSE_CASES)
+def test_exact_dev_matches_reference_within_f32(golden, name):
+    """The FP64 dev from exact integers and the reference's float32 dev agree to the
+    float32 rounding of the ratios they are the spread of."""
+    frames, q, _, dev = _case(golden, name)
+    d = og.dense_dev_fp64(frames, q)
+    fin = np.isfinite(dev)
+    assert np.array_equal(fin, np.isfinite(d))
+    # the std of ratios r_t ~ 1 loses the float32 rounding of each r_t in absolute terms
+    assert np.abs(d[fin] - dev[fin]).max() < 1e-5 * max(1.0, np.abs(dev[fin]).max())
+
+
+def test_dev_brute_force_and_closed_forms():
+    f, q = _tiny(21, n=9, hi=7)
+    d = og.dense_dev_fp64(f, q)
+    _, bins = og.qmask_bins(q.shape, q)
+    for i, b in enumerate(bins):
+        pix = [(y, x) for y in range(q.shape[0]) for x in range(q.shape[1]) if q[y, x] == b]
+        n = f.shape[0]
+        for tau in range(n):
+            r = []
+            for t in range(n - tau):
+                g = sum(int(f[t, y, x]) * int(f[t + tau, y, x]) for y, x in pix)
+                s1 = sum(int(f[t, y, x]) for y, x in pix)
+                s2 = sum(int(f[t + tau, y, x]) for y, x in pix)
+                r.append(len(pix) * g / (s1 * s2) if s1 * s2 else float("nan"))
+            if any(v != v for v in r):
+                assert np.isnan(d[i, tau])
+                continue
+            mu = sum(r) / len(r)
+            var = sum((v - mu) ** 2 for v in r) / len(r)
+            assert abs(d[i, tau] - (var ** 0.5) / len(r) ** 0.5) < 1e-12
+    # last lag: one ratio, zero spread; frames identical up to a factor: r_t constant
+    assert np.all(d[np.isfinite(d[:, -1]), -1] == 0.0)
+    c = np.random.default_rng(2).integers(1, 9, size=(4, 5)).astype(np.uint8)
+    fc = np.stack([c * k for k in (1, 2, 3, 1, 2, 3)]).astype(np.uint8)
+    qq = np.ones((4, 5), np.int32)
+    assert np.abs(og.dense_dev_fp64(fc, qq)).max() < 1e-15
+
+
+@pytest.mark.parametrize("name", DENSE_CASES)
 def test_exact_g2_matches_reference_within_f32(golden, name):
     """The exact FP64 g2 and the reference's float32 g2 agree to float32 rounding."""
     frames, q, res, _ = _case(golden, name)
