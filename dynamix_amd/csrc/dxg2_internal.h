@@ -12,14 +12,23 @@
 namespace dxg2 {
 
 // Slab geometry: every bin's gathered pixels are stored as i8, grouped in k-blocks of
-// KB pixels; a k-block holds all frames: slab[kb][t][KB] with t < nrows, where nrows is
-// nframes rounded up to SLAB_ROW_ALIGN and rows t >= nframes are zero.  A (rows x KB)
-// panel of one k-block is therefore one contiguous run of bytes.
+// KB = 64 pixels; a k-block holds all nrows frames (nframes rounded up to
+// SLAB_ROW_ALIGN; rows t >= nframes are zero).  Inside a k-block the bytes are laid out
+// for the MFMA operand loads: frames form groups of 32, and each (group g, half ks) is
+// one 1 KiB run in which lane l = 32 h + r holds the 16 bytes 16 (2 ks + h) .. +16 of
+// frame 32 g + r (slab_offset()).  A k-block is nrows * 64 contiguous bytes.
 constexpr int KB = 64;
 constexpr int SLAB_ROW_ALIGN = 256;
 // Largest number of k-blocks one Gram work item accumulates in int32 before it
 // flushes: |a*b| <= 2^14 for i8 operands, 2^14 * 2047 * 64 < 2^31.
 constexpr int MAX_KBLOCKS_PER_ITEM = 2047;
+
+// Byte offset of (k-block kb, frame t, byte kk < 64 of the k-block) in a bin's slab.
+__host__ __device__ inline int64_t slab_offset(int kb, int t, int kk, int nrows) {
+    const int q = kk >> 4;
+    return (((int64_t)kb * (nrows >> 5) + (t >> 5)) * 2 + (q >> 1)) * 1024 +
+           ((q & 1) * 32 + (t & 31)) * 16 + (kk & 15);
+}
 
 struct BinDev {
     int64_t slab_off;   // byte offset of the bin's slab

@@ -8,8 +8,8 @@
 //     consecutive list entries -- runs of neighbouring pixels, few cache lines;
 //   * after four frames each lane holds 4 frames x 1 pixel; a 4x4 byte transpose
 //     inside every lane quad (DPP quad broadcast + v_perm) gives lane 4q+i the dword
-//     of pixels 4q..4q+3 for frame t+i, so one store instruction writes the 256
-//     contiguous slab bytes of rows t..t+3.
+//     of pixels 4q..4q+3 for frame t+i, so one store instruction writes rows t..t+3
+//     of the k-block: four 64-byte runs of the fragment-contiguous slab layout.
 // The per-frame bin sums are formed later by the Gram kernel's diagonal tiles.
 #include "dxg2_internal.h"
 
@@ -88,7 +88,7 @@ __global__ __launch_bounds__(256) void gather_i8_kernel(
         const int p = pix_idx[bd.idx_off + (int64_t)kb * KB + lane];
         const int pc = p >= 0 ? p : 0;
         const uint32_t keep = p >= 0 ? 0xFFFFu : 0u;
-        int8_t* dst = slab + bd.slab_off + (int64_t)kb * nrows * KB + q4;
+        int8_t* dst = slab + bd.slab_off;
         for (int t = t0; t < tend; t += 4) {
             uint32_t xl = 0, xh = 0;
 #pragma unroll
@@ -100,14 +100,15 @@ __global__ __launch_bounds__(256) void gather_i8_kernel(
             }
             uint32_t lo = quad_transpose(xl, sel_lo, sel_hi);
             if (BIAS) lo ^= 0x80808080u;
+            const int64_t o = slab_offset(kb, t + qi, q4, nrows);
             if (WIDE) {
                 const uint32_t hi = quad_transpose(xh, sel_lo, sel_hi) ^ 0x80808080u;
                 if (t + qi < nframes) {
-                    *reinterpret_cast<uint32_t*>(dst + (int64_t)(t + qi) * KB) = hi;
-                    *reinterpret_cast<uint32_t*>(dst + plane + (int64_t)(t + qi) * KB) = lo;
+                    *reinterpret_cast<uint32_t*>(dst + o) = hi;
+                    *reinterpret_cast<uint32_t*>(dst + plane + o) = lo;
                 }
             } else if (t + qi < nframes) {
-                *reinterpret_cast<uint32_t*>(dst + (int64_t)(t + qi) * KB) = lo;
+                *reinterpret_cast<uint32_t*>(dst + o) = lo;
             }
         }
     }

@@ -4,36 +4,39 @@
 // itself followed by a host loop over the N diagonals (correlator/dense.py:54,61-64),
 // or the OpenCL loop of densecorrelator.cl:6-57.  Here one launch covers every bin:
 // a work item is one T x T output tile (rows ti, columns tj >= ti) of one bin's Gram
-// G = A A^T over a range of k-blocks.  Both operands are frame panels of the same i8
-// slab; v_mfma_i32_32x32x32_i8 accumulates exact int32 dot products (the work list
-// caps a range at 2047 k-blocks so |sum| < 2^31), and the epilogue folds the tile
-// into its 2T-1 diagonals tau = (tj-ti)*T + c - r in LDS before one 64-bit global
-// atomic per diagonal.  Nothing of size N x N is ever stored.
+// G = A A^T over a range of k-blocks.  v_mfma_i32_32x32x32_i8 accumulates exact int32
+// dot products (the work list caps a range at 2047 k-blocks so |sum| < 2^31), and the
+// epilogue folds the tile into its 2T-1 diagonals tau = (tj-ti)*T + c - r in LDS
+// before one 64-bit global atomic per diagonal.  Nothing of size N x N is stored
+// (except by the calc_std variant, which also writes its tile to a scratch slot).
 //
-// Data movement: frame panels go from HBM straight into LDS with
-// global_load_lds_dwordx4 (no register staging) through a ring of 4 one-k-block
-// stages, so three stages are in flight while the MFMAs consume the fourth.  Slab rows
-// are padded with zero rows to a multiple of 256 frames, so every load is in bounds
-// and rows past the last frame add nothing.  In LDS a panel row is 64 bytes = four
-// 16-byte quads, stored at quad (q ^ ((row >> 2) & 3)): the MFMA fragment reads (16
-// consecutive rows, one quad) and the DMA writes (contiguous 1 KiB) are both free of
-// bank conflicts.
+// Data movement: the slab is stored fragment-contiguous (slab_offset()): the 32 frames
+// x 32 bytes an MFMA operand takes from one k-block are one 1 KiB run in lane order.
+// Each wave (2 x 2 waves per tile, WM x WN blocks of 32 x 32 each) loads its operand
+// fragments straight from global memory into VGPRs with one global_load_dwordx4 per
+// fragment -- no LDS staging, no barriers in the loop.  Two register slots form a
+// software pipeline: the loads of k-block kb + 2 are issued right after the MFMAs of
+// kb have consumed the slot, so a full k-block of MFMAs (WM*WN*2 instructions) covers
+// every load's latency.  Two compiler behaviours must be kept out of the loop:
+//   * the machine scheduler sinks loads towards their first use (it would undo the
+//     pipeline), so sched_barrier fences pin the order MFMA group / load group;
+//   * the waitcnt pass takes the minimum over paths, so a conditionally issued load
+//     makes it wait for every load in flight: every path issues the same loads (the
+//     k-block index is clamped to the item's last k-block near the end).
+// Both operands of a wave are re-read by the wave beside it; those reads hit L1/L2.
 //
-// Diagonal tiles (ti == tj) read each frame row of their panel exactly once per
-// k-range, so they also produce the per-frame bin sums S_b(t) (densecorrelator.cl:
-// 65-150) from the staged bytes with v_dot4 -- no extra pass over the data.
+// Diagonal tiles (ti == tj): the wave below the diagonal has nothing to compute and
+// skips the loop; the other waves compute whole 32 x 32 blocks and the epilogue drops
+// lags below the diagonal.  The two waves on the diagonal also sum the bytes of their
+// A fragments (v_dot4), which gives the per-frame bin sums S_b(t) (densecorrelator.cl:
+// 65-150) with no extra pass over the data.
 //
 // uint16 frames arrive as two biased digit planes H, L (k-blocks [0, nkb) and
 // [nkb, 2 nkb) of the bin).  Work items then come in three kinds, each adding into its
 // own output row: H H^T over [0, nkb), L L^T over [nkb, 2 nkb) (both in the plain
 // kernel), and the cross product [H|L] [L|H]^T = H L^T + L H^T over [0, 2 nkb) in the
 // CROSS kernel, whose B operand reads k-block kb +- nkb.  The cross Gram is symmetric,
-// so the same upper-triangle tiles cover it; its diagonal tiles run as full tiles
-// (A != B) and the epilogue drops their lags below the diagonal.
-//
-// The loop is kept free of stores and of extra control flow: the compiler's tracking of
-// which LDS stage each DMA writes is easily lost (it then waits for every DMA in flight
-// before an LDS read), so the cross-plane variant is a separate kernel.
+// so the same upper-triangle tiles cover it; its diagonal tiles run as full tiles.
 //
 // LDS diagonal sums stay in 32-bit words: every int32 value v is split as
 // v = hi * 2^16 + lo with lo in [0, 2^16) and hi in [-2^15, 2^15); a diagonal of a
@@ -45,129 +48,86 @@ namespace dxg2 {
 
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v16i __attribute__((ext_vector_type(16)));
-typedef __attribute__((address_space(3))) void lds_void;
+typedef __attribute__((address_space(1))) const v4i global_v4i;
+
+// 16-byte load through a global-address-space pointer: a generic pointer (one read from
+// a job descriptor) would give flat loads, which the waitcnt pass also counts as LDS
+// traffic and drains before every MFMA group.
+__device__ __forceinline__ v4i gload16(const int8_t* p) {
+    return *reinterpret_cast<global_v4i*>(reinterpret_cast<uintptr_t>(p));
+}
+
+// 64-bit atomic add through a global-address-space pointer (see gload16).
+__device__ __forceinline__ void gatomic_add(int64_t* p, int64_t v) {
+    __hip_atomic_fetch_add(reinterpret_cast<__attribute__((address_space(1))) unsigned long long*>(
+                               reinterpret_cast<uintptr_t>(p)),
+                           (unsigned long long)v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// A bin descriptor read through a global-address-space pointer (two 16-byte loads).
+static_assert(sizeof(BinDev) == 32, "BinDev is loaded as two 16-byte words");
+__device__ __forceinline__ BinDev gload_bin(const BinDev* p) {
+    union {
+        v4i v[2];
+        BinDev b;
+    } u;
+    u.v[0] = gload16(reinterpret_cast<const int8_t*>(p));
+    u.v[1] = gload16(reinterpret_cast<const int8_t*>(p) + 16);
+    return u.b;
+}
 
 template <int WM, int WN>
 struct GramShape {
-    static constexpr int TM = 2 * WM * 32;          // 2 x 2 waves of WM x WN MFMA blocks
+    static constexpr int TM = 2 * WM * 32;  // 2 x 2 waves of WM x WN MFMA blocks
     static constexpr int TN = 2 * WN * 32;
-    static constexpr int PANEL = TM * KB;           // bytes of one staged panel (1 k-block)
     static constexpr int NDIAG = TM + TN - 1;
-    static constexpr int LOADS = TM / 64;           // 1 KiB DMA instructions per wave per panel
-};
-
-// Physical LDS offset of (row, logical 16-byte quad q) in a staged panel.
-__device__ __forceinline__ int swz(int row, int q) { return row * KB + ((q ^ (row >> 2)) & 3) * 16; }
-
-// Issue the DMA of one panel (rows row0 .. row0+TM of k-block kb) into LDS `dst`.
-// Instruction j of wave w fills LDS rows (w*LOADS + j)*16 .. +16; lane i writes the
-// 16 bytes at dst + 1 KiB*(w*LOADS+j) + 16 i, i.e. row +i/4, physical quad i%4, so it
-// fetches the logical quad (i%4) ^ ((row >> 2) & 3) of that row.
-template <int WM, int WN>
-__device__ __forceinline__ void dma_panel(const int8_t* __restrict__ slab_bin, int kb, int row0,
-                                          int nrows, int8_t* dst, int tid) {
-    using S = GramShape<WM, WN>;
-    const int w = tid >> 6;
-    const int i = tid & 63;
-#pragma unroll
-    for (int j = 0; j < S::LOADS; ++j) {
-        const int g = w * S::LOADS + j;
-        const int row = g * 16 + (i >> 2);
-        const int q = ((i & 3) ^ (row >> 2)) & 3;
-        const int8_t* src = slab_bin + ((int64_t)kb * nrows + row0 + row) * KB + q * 16;
-        __builtin_amdgcn_global_load_lds((const void*)src, (lds_void*)(dst + g * 1024), 16, 0, 0);
-    }
-}
-
-// s_waitcnt vmcnt(N), other counters left at their maximum.
-template <int N>
-__device__ __forceinline__ void wait_vm() {
-    __builtin_amdgcn_s_waitcnt((N & 0xF) | ((N >> 4) << 14) | (0x7 << 4) | (0xF << 8));
-}
-
-// LDS of one workgroup: a ring of 4 stages for each operand, as separate arrays so
-// every access names its stage statically (the compiler then waits only for the DMA
-// that wrote the stage being read).
-template <int WM, int WN>
-struct GramLds {
-    using S = GramShape<WM, WN>;
-    int8_t* a[4];
-    int8_t* b[4];
-    unsigned int* diag;
 };
 
 template <int WM, int WN, bool DIAG, bool CROSS, bool STORE = false>
 struct GramTile {
     using S = GramShape<WM, WN>;
-    static constexpr int L = DIAG ? S::LOADS : 2 * S::LOADS;  // DMA per thread per stage
     const int8_t* __restrict__ slab_bin;
     int kb0, kb1, row0, col0, nframes, nrows;
     int tid, lane, wr, wc;
     int nkbp;  // CROSS: k-blocks per digit plane (B reads kb + nkbp or kb - nkbp)
-    GramLds<WM, WN> lds;
+    unsigned int* sDiag;
+
     v16i acc[WM][WN];
-    int rowsum;
+    v4i fa[2][2][WM];  // [slot][ks][block]
+    v4i fb[2][2][WN];
+    int rs[WM];        // DIAG: partial frame sums of this lane's A rows
 
-    __device__ __forceinline__ static bool live(int wr, int wc, int mi, int ni) {
-        return !DIAG || (wc * WN * 32 + ni * 32 + 31 >= wr * WM * 32 + mi * 32);
-    }
-
-    template <int SLOT>
-    __device__ __forceinline__ void issue(int kb) {
-        dma_panel<WM, WN>(slab_bin, kb, row0, nrows, lds.a[SLOT], tid);
-        if (!DIAG)
-            dma_panel<WM, WN>(slab_bin, CROSS ? (kb < nkbp ? kb + nkbp : kb - nkbp) : kb, col0,
-                              nrows, lds.b[SLOT], tid);
-    }
-
-    // Consume stage kb from ring slot SLOT; refill slot (SLOT + 3) % 4 with kb + 3.
-    template <int SLOT>
-    __device__ __forceinline__ void stage(int kb) {
-        const int later = kb1 - 1 - kb;  // stages issued after kb that may still be in flight
-        if (later >= 2)
-            wait_vm<2 * L>();
-        else if (later == 1)
-            wait_vm<L>();
-        else
-            wait_vm<0>();
-        // The barrier orders (a) every wave's completed DMA of this stage before any
-        // read of it and (b) every read of the previous stage before its slot is refilled.
-        __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
-        __builtin_amdgcn_s_barrier();
-        if (kb + 3 < kb1) issue<(SLOT + 3) & 3>(kb + 3);
-        const int8_t* a_s = lds.a[SLOT];
-        const int8_t* b_s = DIAG ? lds.a[SLOT] : lds.b[SLOT];
-        if (DIAG && tid < S::TM) {
+    __device__ __forceinline__ void load(int s, int kb, const int8_t* pa, const int8_t* pb,
+                                         int64_t kstride) {
+        const int8_t* a = pa + kb * kstride;
+        const int kbb = CROSS ? (kb < nkbp ? kb + nkbp : kb - nkbp) : kb;
+        const int8_t* b = pb + kbb * kstride;
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const v4i x = *reinterpret_cast<const v4i*>(a_s + swz(tid, q));
-                rowsum = __builtin_amdgcn_sdot4(x.x, 0x01010101, rowsum, false);
-                rowsum = __builtin_amdgcn_sdot4(x.y, 0x01010101, rowsum, false);
-                rowsum = __builtin_amdgcn_sdot4(x.z, 0x01010101, rowsum, false);
-                rowsum = __builtin_amdgcn_sdot4(x.w, 0x01010101, rowsum, false);
+        for (int ks = 0; ks < 2; ++ks) {
+#pragma unroll
+            for (int m = 0; m < WM; ++m) fa[s][ks][m] = gload16(a + m * 2048 + ks * 1024);
+#pragma unroll
+            for (int n = 0; n < WN; ++n) fb[s][ks][n] = gload16(b + n * 2048 + ks * 1024);
+        }
+    }
+
+    __device__ __forceinline__ void mma(int s) {
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks)
+#pragma unroll
+            for (int m = 0; m < WM; ++m) {
+#pragma unroll
+                for (int n = 0; n < WN; ++n)
+                    acc[m][n] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[s][ks][m], fb[s][ks][n],
+                                                                      acc[m][n], 0, 0, 0);
+                if (DIAG) {
+                    const v4i x = fa[s][ks][m];
+                    rs[m] = __builtin_amdgcn_sdot4(x.x, 0x01010101, rs[m], false);
+                    rs[m] = __builtin_amdgcn_sdot4(x.y, 0x01010101, rs[m], false);
+                    rs[m] = __builtin_amdgcn_sdot4(x.z, 0x01010101, rs[m], false);
+                    rs[m] = __builtin_amdgcn_sdot4(x.w, 0x01010101, rs[m], false);
+                }
             }
-        }
-        const int frag_row = lane & 31;
-        const int frag_q = lane >> 5;
-#pragma unroll
-        for (int ks = 0; ks < KB / 32; ++ks) {
-            v4i a[WM], b[WN];
-#pragma unroll
-            for (int mi = 0; mi < WM; ++mi)
-                a[mi] = *reinterpret_cast<const v4i*>(
-                    a_s + swz(wr * WM * 32 + mi * 32 + frag_row, ks * 2 + frag_q));
-#pragma unroll
-            for (int ni = 0; ni < WN; ++ni)
-                b[ni] = *reinterpret_cast<const v4i*>(
-                    b_s + swz(wc * WN * 32 + ni * 32 + frag_row, ks * 2 + frag_q));
-#pragma unroll
-            for (int mi = 0; mi < WM; ++mi)
-#pragma unroll
-                for (int ni = 0; ni < WN; ++ni)
-                    if (live(wr, wc, mi, ni))
-                        acc[mi][ni] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[mi], b[ni],
-                                                                            acc[mi][ni], 0, 0, 0);
-        }
     }
 
     // STORE: also write the int32 tile to `tile` (row-major T x T) and leave the frame
@@ -175,79 +135,117 @@ struct GramTile {
     __device__ __forceinline__ void run(int biased, int64_t* __restrict__ out,
                                         int64_t* __restrict__ sums,
                                         int* __restrict__ tile = nullptr) {
-        unsigned int* sDiag = lds.diag;
         for (int i = tid; i < 2 * S::NDIAG; i += 256) sDiag[i] = 0u;
 #pragma unroll
-        for (int mi = 0; mi < WM; ++mi)
+        for (int m = 0; m < WM; ++m) {
+            rs[m] = 0;
 #pragma unroll
-            for (int ni = 0; ni < WN; ++ni)
+            for (int n = 0; n < WN; ++n)
 #pragma unroll
-                for (int v = 0; v < 16; ++v) acc[mi][ni][v] = 0;
-        rowsum = 0;
-
-        if (kb0 < kb1) issue<0>(kb0);
-        if (kb0 + 1 < kb1) issue<1>(kb0 + 1);
-        if (kb0 + 2 < kb1) issue<2>(kb0 + 2);
-        for (int kb = kb0; kb < kb1; kb += 4) {
-            stage<0>(kb);
-            if (kb + 1 < kb1) stage<1>(kb + 1);
-            if (kb + 2 < kb1) stage<2>(kb + 2);
-            if (kb + 3 < kb1) stage<3>(kb + 3);
+                for (int v = 0; v < 16; ++v) acc[m][n][v] = 0;
         }
 
-        if (DIAG && !STORE && tid < S::TM) {
-            const int t = row0 + tid;
-            const int64_t v = (int64_t)rowsum + (biased ? (int64_t)128 * (kb1 - kb0) * KB : 0);
-            if (t < nframes && v != 0)
-                atomicAdd(reinterpret_cast<unsigned long long*>(sums + t), (unsigned long long)v);
+        // Fragment (k-block kb, 32-frame group g, half ks) starts at
+        // slab_bin + (kb * ngroups + g) * 2048 + ks * 1024; lane l reads 16 bytes at +16 l.
+        const int64_t kstride = (int64_t)(nrows >> 5) * 2048;
+        const int8_t* pa = slab_bin + (int64_t)((row0 >> 5) + wr * WM) * 2048 + lane * 16;
+        const int8_t* pb = slab_bin + (int64_t)((col0 >> 5) + wc * WN) * 2048 + lane * 16;
+        const bool idle = DIAG && wr > wc;  // every block of this wave is below the diagonal
+        if (!idle && kb0 < kb1) {
+            const int last = kb1 - 1;
+            load(0, kb0, pa, pb, kstride);
+            load(1, min(kb0 + 1, last), pa, pb, kstride);
+            int kb = kb0;
+            for (; kb + 2 <= kb1; kb += 2) {
+                mma(0);
+                __builtin_amdgcn_sched_barrier(0);
+                load(0, min(kb + 2, last), pa, pb, kstride);
+                __builtin_amdgcn_sched_barrier(0);
+                mma(1);
+                __builtin_amdgcn_sched_barrier(0);
+                load(1, min(kb + 3, last), pa, pb, kstride);
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            if (kb < kb1) mma(0);  // odd count: slot 0 holds the last k-block
+        }
+
+        if (DIAG && !STORE && wr == wc) {
+            // lanes l and l + 32 hold the two 32-byte halves of frame row (l & 31)
+            const int64_t bias = biased ? (int64_t)128 * (kb1 - kb0) * KB : 0;
+#pragma unroll
+            for (int m = 0; m < WM; ++m) {
+                const int v = rs[m] + __shfl_xor(rs[m], 32);
+                const int t = row0 + wr * WM * 32 + m * 32 + lane;
+                const int64_t s64 = (int64_t)v + bias;
+                if (lane < 32 && t < nframes && s64 != 0) gatomic_add(sums + t, s64);
+            }
         }
 
         // Epilogue.  32x32 i32 accumulator layout: lane holds column (lane & 31) and
         // rows 8*(v/4) + 4*(lane/32) + (v%4), v = 0..15.  Diagonal slot c - r + TM - 1.
         __syncthreads();
+        if (!idle) {
 #pragma unroll
-        for (int mi = 0; mi < WM; ++mi)
+            for (int m = 0; m < WM; ++m)
 #pragma unroll
-            for (int ni = 0; ni < WN; ++ni) {
-                if (live(wr, wc, mi, ni)) {
-                    const int cbase = wc * WN * 32 + ni * 32 + (lane & 31);
-                    const int rbase = wr * WM * 32 + mi * 32 + 4 * (lane >> 5);
+                for (int n = 0; n < WN; ++n) {
+                    const int cbase = wc * WN * 32 + n * 32 + (lane & 31);
+                    const int rbase = wr * WM * 32 + m * 32 + 4 * (lane >> 5);
 #pragma unroll
                     for (int v = 0; v < 16; ++v) {
                         const int r = rbase + 8 * (v >> 2) + (v & 3);
-                        const int val = acc[mi][ni][v];
+                        const int val = acc[m][n][v];
                         const int slot = cbase - r + S::TM - 1;
                         atomicAdd(&sDiag[slot], (unsigned)(val & 0xFFFF));
                         atomicAdd(&sDiag[S::NDIAG + slot], (unsigned)(val >> 16));
                     }
+                    __builtin_amdgcn_sched_barrier(0);
                 }
-                __builtin_amdgcn_sched_barrier(0);
-            }
-        if (STORE) {
-            // Lanes 0-31 and 32-63 of one store write two 128-byte row segments.
+            if (STORE) {
+                // Lanes 0-31 and 32-63 of one store write two 128-byte row segments.
 #pragma unroll
-            for (int mi = 0; mi < WM; ++mi)
+                for (int m = 0; m < WM; ++m)
 #pragma unroll
-                for (int ni = 0; ni < WN; ++ni) {
-                    if (live(wr, wc, mi, ni)) {
-                        const int c = wc * WN * 32 + ni * 32 + (lane & 31);
-                        const int rbase = wr * WM * 32 + mi * 32 + 4 * (lane >> 5);
+                    for (int n = 0; n < WN; ++n) {
+                        const int c = wc * WN * 32 + n * 32 + (lane & 31);
+                        const int rbase = wr * WM * 32 + m * 32 + 4 * (lane >> 5);
 #pragma unroll
                         for (int v = 0; v < 16; ++v)
-                            tile[(rbase + 8 * (v >> 2) + (v & 3)) * S::TN + c] = acc[mi][ni][v];
+                            tile[(rbase + 8 * (v >> 2) + (v & 3)) * S::TN + c] = acc[m][n][v];
                     }
-                }
+            }
         }
         __syncthreads();
         for (int i = tid; i < S::NDIAG; i += 256) {
             const long long v =
                 (long long)(int)sDiag[S::NDIAG + i] * 65536 + (long long)sDiag[i];
             const int tau = col0 - row0 + i - (S::TM - 1);
-            if (v != 0 && tau >= 0 && tau < nframes)
-                atomicAdd(reinterpret_cast<unsigned long long*>(out + tau), (unsigned long long)v);
+            if (v != 0 && tau >= 0 && tau < nframes) gatomic_add(out + tau, v);
         }
     }
 };
+
+// One work item: the tile of `w` in a bin whose slab starts at slab_bin.
+template <int WM, int WN, bool CROSS, bool STORE>
+__device__ __forceinline__ void gram_item(const GramWork& w, const int8_t* slab_bin, int nkb,
+                                          int nframes, int nrows, int biased,
+                                          int64_t* __restrict__ out, int64_t* __restrict__ sums,
+                                          int* __restrict__ tile, unsigned int* sDiag) {
+    using S = GramShape<WM, WN>;
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6;
+    if (!CROSS && w.ti == w.tj) {
+        GramTile<WM, WN, true, false, STORE> t{slab_bin, w.kb0, w.kb1, w.ti * S::TM,
+                                               w.tj * S::TN, nframes, nrows, tid, tid & 63,
+                                               wave >> 1, wave & 1, 0, sDiag};
+        t.run(biased, out, sums, tile);
+    } else {
+        GramTile<WM, WN, false, CROSS, STORE> t{slab_bin, w.kb0, w.kb1, w.ti * S::TM,
+                                                w.tj * S::TN, nframes, nrows, tid, tid & 63,
+                                                wave >> 1, wave & 1, nkb, sDiag};
+        t.run(biased, out, sums, tile);
+    }
+}
 
 template <int WM, int WN, bool CROSS>
 __global__ __launch_bounds__(256) void gram_i8_kernel(const int8_t* __restrict__ slab,
@@ -258,31 +256,13 @@ __global__ __launch_bounds__(256) void gram_i8_kernel(const int8_t* __restrict__
                                                       int64_t* __restrict__ sums) {
     using S = GramShape<WM, WN>;
     static_assert(S::TM == S::TN, "square tiles");
-    __shared__ __attribute__((aligned(1024))) int8_t a0[S::PANEL], a1[S::PANEL], a2[S::PANEL],
-        a3[S::PANEL];
-    __shared__ __attribute__((aligned(1024))) int8_t b0[S::PANEL], b1[S::PANEL], b2[S::PANEL],
-        b3[S::PANEL];
     __shared__ unsigned int sDiag[2 * S::NDIAG];
-
     const GramWork w = work[blockIdx.x];
     if (w.bin < 0) return;  // padding item
     const BinDev bd = bins[w.bin];
-    const GramLds<WM, WN> lds{{a0, a1, a2, a3}, {b0, b1, b2, b3}, sDiag};
-    const int tid = threadIdx.x;
-    const int wave = tid >> 6;
-    int64_t* out = ugram + (int64_t)w.out * nframes;
-    int64_t* srow = sums + (int64_t)w.out * nframes;
-    if (!CROSS && w.ti == w.tj) {
-        GramTile<WM, WN, true, false> t{slab + bd.slab_off, w.kb0, w.kb1, w.ti * S::TM,
-                                        w.tj * S::TN, nframes, nrows, tid, tid & 63,
-                                        wave >> 1, wave & 1, 0, lds};
-        t.run(biased, out, srow);
-    } else {
-        GramTile<WM, WN, false, CROSS> t{slab + bd.slab_off, w.kb0, w.kb1, w.ti * S::TM,
-                                         w.tj * S::TN, nframes, nrows, tid, tid & 63,
-                                         wave >> 1, wave & 1, bd.nkb, lds};
-        t.run(biased, out, srow);
-    }
+    gram_item<WM, WN, CROSS, false>(w, slab + bd.slab_off, bd.nkb, nframes, nrows, biased,
+                                    ugram + (int64_t)w.out * nframes,
+                                    sums + (int64_t)w.out * nframes, nullptr, sDiag);
 }
 
 // calc_std variant: the same tiles (work items carry a scratch slot), which also write
@@ -296,31 +276,13 @@ __global__ __launch_bounds__(256) void gram_i8_store_kernel(
     int64_t* __restrict__ ugram, int* __restrict__ tiles) {
     using S = GramShape<WM, WN>;
     static_assert(S::TM == S::TN, "square tiles");
-    __shared__ __attribute__((aligned(1024))) int8_t a0[S::PANEL], a1[S::PANEL], a2[S::PANEL],
-        a3[S::PANEL];
-    __shared__ __attribute__((aligned(1024))) int8_t b0[S::PANEL], b1[S::PANEL], b2[S::PANEL],
-        b3[S::PANEL];
     __shared__ unsigned int sDiag[2 * S::NDIAG];
-
     const GramWork w = work[blockIdx.x];
     if (w.bin < 0) return;  // padding item
     const BinDev bd = bins[w.bin];
-    const GramLds<WM, WN> lds{{a0, a1, a2, a3}, {b0, b1, b2, b3}, sDiag};
-    const int tid = threadIdx.x;
-    const int wave = tid >> 6;
-    int64_t* out = ugram + (int64_t)w.out * nframes;
-    int* tile = tiles + (int64_t)w.slot * S::TM * S::TN;
-    if (!CROSS && w.ti == w.tj) {
-        GramTile<WM, WN, true, false, true> t{slab + bd.slab_off, w.kb0, w.kb1, w.ti * S::TM,
-                                              w.tj * S::TN, nframes, nrows, tid, tid & 63,
-                                              wave >> 1, wave & 1, 0, lds};
-        t.run(biased, out, nullptr, tile);
-    } else {
-        GramTile<WM, WN, false, CROSS, true> t{slab + bd.slab_off, w.kb0, w.kb1, w.ti * S::TM,
-                                               w.tj * S::TN, nframes, nrows, tid, tid & 63,
-                                               wave >> 1, wave & 1, bd.nkb, lds};
-        t.run(biased, out, nullptr, tile);
-    }
+    gram_item<WM, WN, CROSS, true>(w, slab + bd.slab_off, bd.nkb, nframes, nrows, biased,
+                                   ugram + (int64_t)w.out * nframes, nullptr,
+                                   tiles + (int64_t)w.slot * S::TM * S::TN, sDiag);
 }
 
 // Grouped launch over several plans (continuous batching of dense jobs): the same tile
@@ -330,32 +292,14 @@ __global__ __launch_bounds__(256) void gram_i8_jobs_kernel(const GramJob* __rest
                                                            const GramWork* __restrict__ work) {
     using S = GramShape<WM, WN>;
     static_assert(S::TM == S::TN, "square tiles");
-    __shared__ __attribute__((aligned(1024))) int8_t a0[S::PANEL], a1[S::PANEL], a2[S::PANEL],
-        a3[S::PANEL];
-    __shared__ __attribute__((aligned(1024))) int8_t b0[S::PANEL], b1[S::PANEL], b2[S::PANEL],
-        b3[S::PANEL];
     __shared__ unsigned int sDiag[2 * S::NDIAG];
-
     const GramWork w = work[blockIdx.x];
     if (w.bin < 0) return;  // padding item
     const GramJob j = jobs[w.job];
-    const BinDev bd = j.bins[w.bin];
-    const GramLds<WM, WN> lds{{a0, a1, a2, a3}, {b0, b1, b2, b3}, sDiag};
-    const int tid = threadIdx.x;
-    const int wave = tid >> 6;
-    int64_t* out = j.ugram + (int64_t)w.out * j.nframes;
-    int64_t* srow = j.sums + (int64_t)w.out * j.nframes;
-    if (!CROSS && w.ti == w.tj) {
-        GramTile<WM, WN, true, false> t{j.slab + bd.slab_off, w.kb0, w.kb1, w.ti * S::TM,
-                                        w.tj * S::TN, j.nframes, j.nrows, tid, tid & 63,
-                                        wave >> 1, wave & 1, 0, lds};
-        t.run(j.biased, out, srow);
-    } else {
-        GramTile<WM, WN, false, CROSS> t{j.slab + bd.slab_off, w.kb0, w.kb1, w.ti * S::TM,
-                                         w.tj * S::TN, j.nframes, j.nrows, tid, tid & 63,
-                                         wave >> 1, wave & 1, bd.nkb, lds};
-        t.run(j.biased, out, srow);
-    }
+    const BinDev bd = gload_bin(j.bins + w.bin);
+    gram_item<WM, WN, CROSS, false>(w, j.slab + bd.slab_off, bd.nkb, j.nframes, j.nrows,
+                                    j.biased, j.ugram + (int64_t)w.out * j.nframes,
+                                    j.sums + (int64_t)w.out * j.nframes, nullptr, sDiag);
 }
 
 void launch_gram_i8_jobs(int tile, bool cross, const GramJob* jobs, const GramWork* work,

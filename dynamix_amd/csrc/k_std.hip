@@ -46,22 +46,20 @@ __global__ __launch_bounds__(256) void slab_sums_kernel(const int8_t* __restrict
     const int64_t plane = (int64_t)bd.nkb * nrows * KB;
     int acc = 0, acc_l = 0;  // |sum| <= 16 * 64 * 128
     for (int kb = w.kb0; kb < w.kb0 + w.nkb; ++kb) {
-        const v4i* row = reinterpret_cast<const v4i*>(slab + bd.slab_off +
-                                                      ((int64_t)kb * nrows + t) * KB);
+        const int8_t* row = slab + bd.slab_off;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const v4i x = row[q];
+            const v4i x = *reinterpret_cast<const v4i*>(row + slab_offset(kb, t, 16 * q, nrows));
             acc = __builtin_amdgcn_sdot4(x.x, 0x01010101, acc, false);
             acc = __builtin_amdgcn_sdot4(x.y, 0x01010101, acc, false);
             acc = __builtin_amdgcn_sdot4(x.z, 0x01010101, acc, false);
             acc = __builtin_amdgcn_sdot4(x.w, 0x01010101, acc, false);
         }
         if (WIDE) {
-            const v4i* lrow = reinterpret_cast<const v4i*>(reinterpret_cast<const int8_t*>(row) +
-                                                           plane);
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                const v4i x = lrow[q];
+                const v4i x = *reinterpret_cast<const v4i*>(row + plane +
+                                                            slab_offset(kb, t, 16 * q, nrows));
                 acc_l = __builtin_amdgcn_sdot4(x.x, 0x01010101, acc_l, false);
                 acc_l = __builtin_amdgcn_sdot4(x.y, 0x01010101, acc_l, false);
                 acc_l = __builtin_amdgcn_sdot4(x.z, 0x01010101, acc_l, false);
