@@ -128,6 +128,24 @@ std::vector<T> remap_for_xcds(const std::vector<T>& in, int group, const T& pad)
     return out;
 }
 
+// The same chunks as remap_for_xcds(), as 8 unpadded per-XCD queues laid end to end:
+// queue x (chunks x, x+8, ...) is out[off[x] .. off[x+1]).
+template <typename T>
+std::vector<T> split_for_xcds(const std::vector<T>& in, int group, int* off) {
+    std::vector<std::vector<T>> per(N_XCD);
+    const size_t g = std::max(group, 1);
+    for (size_t c = 0; c * g < in.size(); ++c)
+        for (size_t i = c * g; i < std::min(in.size(), (c + 1) * g); ++i)
+            per[c % N_XCD].push_back(in[i]);
+    std::vector<T> out;
+    for (int x = 0; x < N_XCD; ++x) {
+        off[x] = (int)out.size();
+        out.insert(out.end(), per[x].begin(), per[x].end());
+    }
+    off[N_XCD] = (int)out.size();
+    return out;
+}
+
 }  // namespace
 
 struct dxg2_plan {
@@ -155,6 +173,15 @@ struct dxg2_plan {
     GatherWork* d_gwork = nullptr;
     GramWork* d_mwork = nullptr;
     GramWork* d_xwork = nullptr;
+    // persistent launch: per-XCD queues of the plain and cross items, their offsets
+    // (plain [0, 9), cross [9, 18)) and heads (plain [0, 8), cross [8, 16))
+    bool use_queue = true;
+    std::vector<GramWork> qm_work, qx_work;
+    int q_off[18] = {0};
+    GramWork* d_qmwork = nullptr;
+    GramWork* d_qxwork = nullptr;
+    int* d_qoff = nullptr;
+    int* d_heads = nullptr;
     int8_t* d_slab = nullptr;
     int64_t* d_ugram = nullptr;
     int64_t* d_sums = nullptr;
@@ -227,8 +254,8 @@ int env_int(const char* name, int dflt) {
 void build_gram_work(dxg2_plan* p) {
     const int T = p->gram_tile;
     const int nT = (p->N + T - 1) / T;
-    const int band = std::max(1, env_int("DXG2_GRAM_BAND", 8));
-    const int group = std::max(0, env_int("DXG2_GRAM_GROUP", 64));
+    const int band = std::max(1, env_int("DXG2_GRAM_BAND", 4));
+    const int group = std::max(0, env_int("DXG2_GRAM_GROUP", 32));
     const int kchunk = std::min(MAX_KBLOCKS_PER_ITEM,
                                 std::max(1, env_int("DXG2_GRAM_KCHUNK", MAX_KBLOCKS_PER_ITEM)));
     std::vector<int> order(p->n_bins);
@@ -264,6 +291,9 @@ void build_gram_work(dxg2_plan* p) {
     GramWork pad{-1, 0, 0, 0, 0, 0, 0, 0, 0, 0};  // job 0
     p->gram_work = group > 0 ? remap_for_xcds(raster, group, pad) : raster;
     p->gram_xwork = group > 0 ? remap_for_xcds(xraster, group, pad) : xraster;
+    p->use_queue = env_int("DXG2_GRAM_QUEUE", 1) != 0;
+    p->qm_work = split_for_xcds(raster, group > 0 ? group : 1, p->q_off);
+    p->qx_work = split_for_xcds(xraster, group > 0 ? group : 1, p->q_off + 9);
     p->gram_raster = std::move(raster);
     p->gram_xraster = std::move(xraster);
     p->gram_group = group;
@@ -277,8 +307,8 @@ void build_std_work(dxg2_plan* p, size_t cap_bytes) {
     const int T = p->gram_tile;
     const int nT = (p->N + T - 1) / T;
     const int64_t ntri = (int64_t)nT * (nT + 1) / 2;
-    const int band = std::max(1, env_int("DXG2_GRAM_BAND", 8));
-    const int group = std::max(0, env_int("DXG2_GRAM_GROUP", 64));
+    const int band = std::max(1, env_int("DXG2_GRAM_BAND", 4));
+    const int group = std::max(0, env_int("DXG2_GRAM_GROUP", 32));
     const int kchunk = std::min(MAX_KBLOCKS_PER_ITEM,
                                 std::max(1, env_int("DXG2_GRAM_KCHUNK", MAX_KBLOCKS_PER_ITEM)));
     const bool wide = slab_planes(p->dtype) == 2;
@@ -510,6 +540,27 @@ int64_t* dense_gather(dxg2_plan* p, const void* d_frames, int64_t* d_sums, hipSt
     return gsums;
 }
 
+// The Gram of every bin: the persistent launch over the per-XCD work queues (heads
+// cleared by a fill kernel, so the sequence can be captured in a graph), or one
+// workgroup per tile.
+void dense_gram(dxg2_plan* p, int64_t* gsums, hipStream_t s) {
+    const int biased = p->dtype == DXG2_I8 ? 0 : 1;
+    if (p->use_queue) {
+        launch_fill_i32(p->d_heads, 16, s);
+        launch_gram_i8_queue(p->gram_tile, false, p->d_slab, p->d_bins, p->d_qmwork,
+                             (int)p->qm_work.size(), p->d_qoff, p->d_heads, p->N, p->nrows,
+                             biased, p->d_ugram, gsums, s);
+        launch_gram_i8_queue(p->gram_tile, true, p->d_slab, p->d_bins, p->d_qxwork,
+                             (int)p->qx_work.size(), p->d_qoff + 9, p->d_heads + 8, p->N,
+                             p->nrows, biased, p->d_ugram, gsums, s);
+    } else {
+        launch_gram_i8(p->gram_tile, false, p->d_slab, p->d_bins, p->d_mwork,
+                       (int)p->gram_work.size(), p->N, p->nrows, biased, p->d_ugram, gsums, s);
+        launch_gram_i8(p->gram_tile, true, p->d_slab, p->d_bins, p->d_xwork,
+                       (int)p->gram_xwork.size(), p->N, p->nrows, biased, p->d_ugram, gsums, s);
+    }
+}
+
 // Bias corrections, exact denominators and g2 after the Gram.
 void dense_finish(dxg2_plan* p, int64_t* gsums, int64_t* d_num, int64_t* d_sums, double* d_den,
                   float* d_g2, hipStream_t s) {
@@ -636,7 +687,8 @@ int dxg2_plan_create(const dxg2_plan_desc* desc, dxg2_plan** out) {
     const size_t staged = p->pix_idx.size() * 4 + p->bins.size() * sizeof(BinDev) +
                           (p->gather_work.size() + 1) * sizeof(GatherWork) +
                           (p->gram_work.size() + p->gram_xwork.size() + 2) * sizeof(GramWork) +
-                          5 * 256;
+                          (p->qm_work.size() + p->qx_work.size() + 2) * sizeof(GramWork) +
+                          sizeof(p->q_off) + 9 * 256;
     if (sc.cap < staged) {
         if (sc.staging) {
             (void)hipStreamSynchronize(s0);
@@ -675,6 +727,13 @@ int dxg2_plan_create(const dxg2_plan_desc* desc, dxg2_plan** out) {
     if (!rc)
         rc = upload((void**)&p->d_xwork, p->gram_xwork.data(),
                     p->gram_xwork.size() * sizeof(GramWork));
+    if (!rc)
+        rc = upload((void**)&p->d_qmwork, p->qm_work.data(), p->qm_work.size() * sizeof(GramWork));
+    if (!rc)
+        rc = upload((void**)&p->d_qxwork, p->qx_work.data(), p->qx_work.size() * sizeof(GramWork));
+    if (!rc) rc = upload((void**)&p->d_qoff, p->q_off, sizeof(p->q_off));
+    if (!rc && dev_alloc(p->alloc, (void**)&p->d_heads, 16 * sizeof(int), s0) != hipSuccess)
+        rc = set_error(DXG2_ENOMEM, "cannot allocate work-queue heads");
     const size_t nb_n = (size_t)std::max(p->n_bins, 1) * p->N * sizeof(int64_t);
     const int planes = slab_planes(p->dtype);
     // numerator rows: one per bin, or HH, LL and cross rows for digit planes
@@ -707,6 +766,7 @@ static void plan_release(dxg2_plan* p, hipStream_t s) {
                     p->d_ework, p->d_err,  p->d_cnt,   p->d_scan,  p->d_off,
                     p->d_etimes, p->d_edata, p->d_kbwork, p->d_xs, p->d_tw,
                     p->d_power, p->d_fnum, p->d_std_mwork, p->d_std_xwork,
+                    p->d_qmwork, p->d_qxwork, p->d_qoff, p->d_heads,
                     p->d_std_tiles, p->d_std_scratch, p->d_std_part};
     for (void* b : bufs) dev_free(p->alloc, b, s);
     for (auto& q : p->ev_pool)
@@ -748,7 +808,8 @@ int dxg2_plan_footprint(const dxg2_plan* p, int32_t path, size_t* bytes) {
     switch (path) {
         case 0:
             *bytes = common + p->slab_bytes + p->gather_work.size() * sizeof(GatherWork) +
-                     (p->gram_work.size() + p->gram_xwork.size()) * sizeof(GramWork);
+                     (p->gram_work.size() + p->gram_xwork.size() + p->qm_work.size() +
+                      p->qx_work.size()) * sizeof(GramWork) + sizeof(p->q_off) + 16 * sizeof(int);
             return DXG2_OK;
         case 1: {
             int64_t mx = 0;
@@ -822,10 +883,7 @@ int dxg2_correlate_dense(dxg2_plan* p, const void* d_frames, int64_t* d_num, int
     int64_t* gsums = dense_gather(p, d_frames, d_sums, s);
     if (ev) DXG2_HIP(hipEventRecord(ev[1], s));
     const int biased = p->dtype == DXG2_I8 ? 0 : 1;
-    launch_gram_i8(p->gram_tile, false, p->d_slab, p->d_bins, p->d_mwork,
-                   (int)p->gram_work.size(), p->N, p->nrows, biased, p->d_ugram, gsums, s);
-    launch_gram_i8(p->gram_tile, true, p->d_slab, p->d_bins, p->d_xwork,
-                   (int)p->gram_xwork.size(), p->N, p->nrows, biased, p->d_ugram, gsums, s);
+    dense_gram(p, gsums, s);
     if (ev) DXG2_HIP(hipEventRecord(ev[2], s));
     dense_finish(p, gsums, d_num, d_sums, d_den, d_g2, s);
     if (ev) DXG2_HIP(hipEventRecord(ev[3], s));

@@ -112,6 +112,12 @@ void launch_gather_i8(const uint8_t* frames, int dtype, int64_t hw, int nframes,
 void launch_gram_i8(int tile, bool cross, const int8_t* slab, const BinDev* bins,
                     const GramWork* work, int n_work, int nframes, int nrows, int biased,
                     int64_t* ugram, int64_t* sums, hipStream_t s);
+// Persistent launch over per-XCD work queues: queue x is items [qoff[x], qoff[x+1]) of
+// `work`; heads (8 ints) must be zero.
+void launch_gram_i8_queue(int tile, bool cross, const int8_t* slab, const BinDev* bins,
+                          const GramWork* work, int n_work, const int* qoff, int* heads,
+                          int nframes, int nrows, int biased, int64_t* ugram, int64_t* sums,
+                          hipStream_t s);
 // Grouped launch: work items of several plans, each naming its job descriptor.
 void launch_gram_i8_jobs(int tile, bool cross, const GramJob* jobs, const GramWork* work,
                          int n_work, hipStream_t s);

@@ -42,6 +42,8 @@
 // v = hi * 2^16 + lo with lo in [0, 2^16) and hi in [-2^15, 2^15); a diagonal of a
 // T <= 256 tile collects at most 256 values, so sum(lo) < 2^24 and |sum(hi)| < 2^23
 // are exact, and the 64-bit total hi_sum * 2^16 + lo_sum is formed once per slot.
+#include <algorithm>
+
 #include "dxg2_internal.h"
 
 namespace dxg2 {
@@ -155,18 +157,28 @@ struct GramTile {
             const int last = kb1 - 1;
             load(0, kb0, pa, pb, kstride);
             load(1, min(kb0 + 1, last), pa, pb, kstride);
-            int kb = kb0;
-            for (; kb + 2 <= kb1; kb += 2) {
+            for (int kb = kb0; kb < kb1; kb += 2) {
                 mma(0);
                 __builtin_amdgcn_sched_barrier(0);
                 load(0, min(kb + 2, last), pa, pb, kstride);
                 __builtin_amdgcn_sched_barrier(0);
+                // An odd count ends with slot 1 holding a repeat of the last k-block:
+                // its A fragments are zeroed so those MFMAs add nothing (no tail code
+                // after the loop, whose live fragments would spill).
+                const bool repeat = kb + 1 > last;
+#pragma unroll
+                for (int ks = 0; ks < 2; ++ks)
+#pragma unroll
+                    for (int m = 0; m < WM; ++m) {
+                        const v4i x = fa[1][ks][m];
+                        fa[1][ks][m] = v4i{repeat ? 0 : x.x, repeat ? 0 : x.y, repeat ? 0 : x.z,
+                                           repeat ? 0 : x.w};
+                    }
                 mma(1);
                 __builtin_amdgcn_sched_barrier(0);
                 load(1, min(kb + 3, last), pa, pb, kstride);
                 __builtin_amdgcn_sched_barrier(0);
             }
-            if (kb < kb1) mma(0);  // odd count: slot 0 holds the last k-block
         }
 
         if (DIAG && !STORE && wr == wc) {
@@ -185,19 +197,22 @@ struct GramTile {
         // rows 8*(v/4) + 4*(lane/32) + (v%4), v = 0..15.  Diagonal slot c - r + TM - 1.
         __syncthreads();
         if (!idle) {
+            // slot = lane part + (n - m) 32 - 8 (v / 4) - v % 4; the second part is a
+            // compile-time constant >= MINOFF, so every atomic is one base VGPR plus an
+            // immediate offset (no per-element address registers).
+            constexpr int MINOFF = -(WM - 1) * 32 - 15;
+            unsigned int* sd = sDiag + (wc * WN * 32 + (lane & 31)) -
+                               (wr * WM * 32 + 4 * (lane >> 5)) + S::TM - 1 + MINOFF;
 #pragma unroll
             for (int m = 0; m < WM; ++m)
 #pragma unroll
                 for (int n = 0; n < WN; ++n) {
-                    const int cbase = wc * WN * 32 + n * 32 + (lane & 31);
-                    const int rbase = wr * WM * 32 + m * 32 + 4 * (lane >> 5);
 #pragma unroll
                     for (int v = 0; v < 16; ++v) {
-                        const int r = rbase + 8 * (v >> 2) + (v & 3);
+                        const int k = (n - m) * 32 - 8 * (v >> 2) - (v & 3) - MINOFF;
                         const int val = acc[m][n][v];
-                        const int slot = cbase - r + S::TM - 1;
-                        atomicAdd(&sDiag[slot], (unsigned)(val & 0xFFFF));
-                        atomicAdd(&sDiag[S::NDIAG + slot], (unsigned)(val >> 16));
+                        atomicAdd(sd + k, (unsigned)(val & 0xFFFF));
+                        atomicAdd(sd + S::NDIAG + k, (unsigned)(val >> 16));
                     }
                     __builtin_amdgcn_sched_barrier(0);
                 }
@@ -263,6 +278,97 @@ __global__ __launch_bounds__(256) void gram_i8_kernel(const int8_t* __restrict__
     gram_item<WM, WN, CROSS, false>(w, slab + bd.slab_off, bd.nkb, nframes, nrows, biased,
                                     ugram + (int64_t)w.out * nframes,
                                     sums + (int64_t)w.out * nframes, nullptr, sDiag);
+}
+
+// XCD (accelerator complex die) this wave runs on: HW_REG_XCC_ID bits [3:0].
+__device__ __forceinline__ int xcc_id() {
+    return (int)(__builtin_amdgcn_s_getreg((20) | (0 << 6) | (15 << 11)) & 0xF);
+}
+
+// Next work item for a workgroup on XCD `home`: the head of its own XCD's queue, then
+// (once that is drained) the other XCDs' queues in turn.  Queue x holds items
+// [qoff[x], qoff[x+1]) of the work list; heads[x] counts the items taken from it.
+__device__ __forceinline__ int next_item(int home, const int* __restrict__ qoff, int* heads) {
+    for (int k = 0; k < 8; ++k) {
+        const int x = (home + k) & 7;
+        const int n = qoff[x + 1] - qoff[x];
+        if (__hip_atomic_load(heads + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= n)
+            continue;
+        const int i = atomicAdd(heads + x, 1);
+        if (i < n) return qoff[x] + i;
+    }
+    return -1;
+}
+
+// Persistent grouped launch: one workgroup per resident slot, each taking tiles from
+// the device work queues until all are empty.  Tiles of one queue share frame panels
+// in their XCD's L2; stealing evens out the XCDs' finishing times.  heads[] must be
+// zero at launch.
+template <int WM, int WN, bool CROSS>
+__global__ __launch_bounds__(256) void gram_i8_queue_kernel(
+    const int8_t* __restrict__ slab, const BinDev* __restrict__ bins,
+    const GramWork* __restrict__ work, const int* __restrict__ qoff, int* heads, int nframes,
+    int nrows, int biased, int64_t* __restrict__ ugram, int64_t* __restrict__ sums) {
+    using S = GramShape<WM, WN>;
+    static_assert(S::TM == S::TN, "square tiles");
+    __shared__ unsigned int sDiag[2 * S::NDIAG];
+    __shared__ int s_item;
+    const int home = xcc_id();
+    for (;;) {
+        if (threadIdx.x == 0) s_item = next_item(home, qoff, heads);
+        __syncthreads();
+        const int it = s_item;
+        if (it < 0) break;
+        const GramWork w = work[it];
+        const BinDev bd = bins[w.bin];
+        gram_item<WM, WN, CROSS, false>(w, slab + bd.slab_off, bd.nkb, nframes, nrows, biased,
+                                        ugram + (int64_t)w.out * nframes,
+                                        sums + (int64_t)w.out * nframes, nullptr, sDiag);
+        __syncthreads();  // s_item and sDiag are reused by the next item
+    }
+}
+
+template <int WM, int WN, bool CROSS>
+int queue_grid() {
+    static int grid[64] = {0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    if (!grid[dev]) {
+        int cus = 0, per = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+            cus = 256;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &per, reinterpret_cast<const void*>(gram_i8_queue_kernel<WM, WN, CROSS>), 256, 0) !=
+                hipSuccess ||
+            per < 1)
+            per = 1;
+        grid[dev] = cus * per;
+    }
+    return grid[dev];
+}
+
+void launch_gram_i8_queue(int tile, bool cross, const int8_t* slab, const BinDev* bins,
+                          const GramWork* work, int n_work, const int* qoff, int* heads,
+                          int nframes, int nrows, int biased, int64_t* ugram, int64_t* sums,
+                          hipStream_t s) {
+    if (n_work <= 0) return;
+    const dim3 block(256);
+#define DXG2_QUEUE_LAUNCH(WM, CR)                                                             \
+    gram_i8_queue_kernel<WM, WM, CR><<<dim3((unsigned)std::min(n_work, queue_grid<WM, WM, CR>())), \
+                                       block, 0, s>>>(slab, bins, work, qoff, heads, nframes,    \
+                                                      nrows, biased, ugram, sums)
+    if (tile == 128) {
+        if (cross)
+            DXG2_QUEUE_LAUNCH(2, true);
+        else
+            DXG2_QUEUE_LAUNCH(2, false);
+    } else {
+        if (cross)
+            DXG2_QUEUE_LAUNCH(4, true);
+        else
+            DXG2_QUEUE_LAUNCH(4, false);
+    }
+#undef DXG2_QUEUE_LAUNCH
 }
 
 // calc_std variant: the same tiles (work items carry a scratch slot), which also write
