@@ -43,6 +43,10 @@ class PlanDesc(ctypes.Structure):
                 ("n_scale_factors", c_int), ("flags", ctypes.c_uint32)]
 
 
+class Tuning(ctypes.Structure):
+    _fields_ = [("tile", c_int), ("band", c_int), ("group", c_int), ("queue", c_int)]
+
+
 ALLOC_FN = ctypes.CFUNCTYPE(c_void_p, c_size_t, c_void_p, c_void_p)
 FREE_FN = ctypes.CFUNCTYPE(None, c_void_p, c_void_p, c_void_p)
 
@@ -55,6 +59,9 @@ _SIGNATURES = {
     "dxg2_plan_footprint": (c_int, [c_void_p, c_int, ctypes.POINTER(c_size_t)]),
     "dxg2_correlate_dense": (c_int, [c_void_p] * 7),
     "dxg2_correlate_dense_std": (c_int, [c_void_p] * 8),
+    "dxg2_correlate_dense_graph": (c_int, [c_void_p] * 7),
+    "dxg2_plan_get_tuning": (c_int, [c_void_p, ctypes.POINTER(Tuning)]),
+    "dxg2_plan_set_tuning": (c_int, [c_void_p, ctypes.POINTER(Tuning)]),
     "dxg2_correlate_dense_plain": (c_int, [c_void_p] * 4),
     "dxg2_batch_create": (c_int, [ctypes.POINTER(c_void_p)]),
     "dxg2_batch_destroy": (c_int, [c_void_p]),

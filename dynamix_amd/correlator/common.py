@@ -126,6 +126,18 @@ class Plan(object):
         _lib.check(self.lib.dxg2_plan_query(self.handle, None, self.pixel_counts.ctypes.data,
                                             self.scale.ctypes.data))
 
+    def get_tuning(self):
+        """The plan's Gram launch tuning: dict(tile, band, group, queue)."""
+        t = _lib.Tuning()
+        _lib.check(self.lib.dxg2_plan_get_tuning(self.handle, ctypes.byref(t)))
+        return {"tile": t.tile, "band": t.band, "group": t.group, "queue": t.queue}
+
+    def set_tuning(self, tile, band, group, queue):
+        """Rebuild the Gram work lists for a tuning (waits for the device)."""
+        t = _lib.Tuning(int(tile), int(band), int(group), int(queue))
+        _lib.check(self.lib.dxg2_plan_set_tuning(self.handle, ctypes.byref(t)),
+                   "dxg2_plan_set_tuning")
+
     def footprint(self, path=0):
         out = ctypes.c_size_t()
         _lib.check(self.lib.dxg2_plan_footprint(self.handle, path, ctypes.byref(out)))

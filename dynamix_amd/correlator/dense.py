@@ -12,6 +12,15 @@ FFTCorrelator     -- dense.py:275-369 (FFTWCorrelator) and cuda.py:129-219
                      time axis (FFTWCorrelator is the same class under the reference's
                      name).
 
+extra_options understood by the dense classes:
+  "tile": 128           -- 128-frame Gram tiles (default 256);
+  "return_device": True -- return g2 as a GPU tensor;
+  "autotune": True      -- tune the Gram launch for this workload (dynamix_amd.autotune):
+                           a cached result is applied when the plan is built, otherwise
+                           the first correlate() times the candidates on its frames;
+  "graph": True         -- replay the dense chain as one HIP graph per call, over
+                           device buffers that persist across calls.
+
 Results: float32 (n_bins, nframes), tau = 0 in column 0, NaN where the denominator is
 zero (an empty bin, or frames with no counts).  g2 = n_b * Num / Den with exact integer
 Num and Den and one FP64 division, i.e. within float32 rounding of the FP64 value.
@@ -33,6 +42,8 @@ class _DenseHIP(BaseCorrelator):
 
     def _init_dense(self):
         self._plans = {}
+        self._tuned = set()
+        self._graph_bufs = {}
         self.last_numerators = None
         self.last_sums = None
         self.last_denominators = None
@@ -46,7 +57,27 @@ class _DenseHIP(BaseCorrelator):
             flags = _lib.DXG2_FLAG_TILE128 if self.extra_options.get("tile") == 128 else 0
             p = Plan(self.shape, self.nframes, q, dtype, scale=None, flags=flags)
             self._plans[dtype] = p
+            if self.extra_options.get("autotune", False):
+                from .. import autotune
+                if autotune.apply_cached(p, self.nframes):
+                    self._tuned.add(dtype)
         return p
+
+    def _outputs(self, plan, dtype, torch, calc_std):
+        """Output tensors of one call: fresh ones, or (graph mode) the plan's persistent
+        set, whose addresses the captured graph holds."""
+        nb, n = max(self.n_bins, 0), self.nframes
+        graph = self.extra_options.get("graph", False) and not calc_std
+        if graph and dtype in self._graph_bufs:
+            return self._graph_bufs[dtype]
+        out = (torch.empty((nb, n), dtype=torch.int64, device="cuda"),
+               torch.empty((nb, n), dtype=torch.int64, device="cuda"),
+               torch.empty((nb, n), dtype=torch.float64, device="cuda"),
+               torch.empty((nb, n), dtype=torch.float32, device="cuda"),
+               torch.empty((nb, n), dtype=torch.float32, device="cuda") if calc_std else None)
+        if graph:
+            self._graph_bufs[dtype] = out
+        return out
 
     def _run_dense(self, frames, calc_std=False):
         torch = _torch()
@@ -55,13 +86,28 @@ class _DenseHIP(BaseCorrelator):
             raise NotImplementedError("the dense i8 path takes uint8, int8 or uint16 frames, "
                                       "got %s" % dtype)
         plan = self._plan(dtype)
-        nb, n = max(self.n_bins, 0), self.nframes
-        num = torch.empty((nb, n), dtype=torch.int64, device="cuda")
-        sums = torch.empty((nb, n), dtype=torch.int64, device="cuda")
-        den = torch.empty((nb, n), dtype=torch.float64, device="cuda")
-        g2 = torch.empty((nb, n), dtype=torch.float32, device="cuda")
-        dev = torch.empty((nb, n), dtype=torch.float32, device="cuda") if calc_std else None
-        if nb and calc_std:
+        nb = max(self.n_bins, 0)
+        if nb and self.extra_options.get("autotune", False) and dtype not in self._tuned:
+            from .. import autotune
+            autotune.autotune(plan, d_frames, self.nframes)
+            self._tuned.add(dtype)
+        graph = self.extra_options.get("graph", False) and not calc_std
+        if graph:
+            # the graph holds the frames' address too: keep a persistent input buffer
+            key = ("frames", dtype)
+            buf = self._graph_bufs.get(key)
+            if buf is None or buf.shape != d_frames.shape:
+                buf = torch.empty_like(d_frames)
+                self._graph_bufs[key] = buf
+            if buf.data_ptr() != d_frames.data_ptr():
+                buf.copy_(d_frames)
+            d_frames = buf
+        num, sums, den, g2, dev = self._outputs(plan, dtype, torch, calc_std)
+        if nb and graph:
+            _lib.check(plan.lib.dxg2_correlate_dense_graph(
+                plan.handle, _lib.ptr(d_frames), _lib.ptr(num), _lib.ptr(sums), _lib.ptr(den),
+                _lib.ptr(g2), current_stream(torch)), "dxg2_correlate_dense_graph")
+        elif nb and calc_std:
             _lib.check(plan.lib.dxg2_correlate_dense_std(
                 plan.handle, _lib.ptr(d_frames), _lib.ptr(num), _lib.ptr(sums), _lib.ptr(den),
                 _lib.ptr(g2), _lib.ptr(dev), current_stream(torch)), "dxg2_correlate_dense_std")
@@ -71,6 +117,8 @@ class _DenseHIP(BaseCorrelator):
                                                      current_stream(torch)),
                        "dxg2_correlate_dense")
         self.last_numerators, self.last_sums, self.last_denominators, self.last_g2 = num, sums, den, g2
+        if graph and self.extra_options.get("return_device", False):
+            g2 = g2.clone()  # the persistent buffer is rewritten by the next call
         if not self.extra_options.get("return_device", False):
             g2 = g2.cpu().numpy()
             dev = dev.cpu().numpy() if calc_std else None

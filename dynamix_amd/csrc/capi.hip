@@ -153,6 +153,13 @@ struct dxg2_plan {
     int nrows = 0;  // slab rows per k-block: N rounded up to SLAB_ROW_ALIGN
     int device = 0;
     int gram_tile = 256;
+    int tune_band = 4;    // tile rows per raster band of the Gram work list
+    int tune_group = 32;  // consecutive items per XCD chunk (0: no XCD permutation)
+    int tune_kchunk = MAX_KBLOCKS_PER_ITEM;
+    // graph replay of the dense chain (dxg2_correlate_dense_graph)
+    hipStream_t cap_stream = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    std::array<const void*, 5> gkey{};
     std::vector<int64_t> npix;
     std::vector<double> scale;
     std::vector<int32_t> pix_idx;
@@ -254,10 +261,9 @@ int env_int(const char* name, int dflt) {
 void build_gram_work(dxg2_plan* p) {
     const int T = p->gram_tile;
     const int nT = (p->N + T - 1) / T;
-    const int band = std::max(1, env_int("DXG2_GRAM_BAND", 4));
-    const int group = std::max(0, env_int("DXG2_GRAM_GROUP", 32));
-    const int kchunk = std::min(MAX_KBLOCKS_PER_ITEM,
-                                std::max(1, env_int("DXG2_GRAM_KCHUNK", MAX_KBLOCKS_PER_ITEM)));
+    const int band = p->tune_band;
+    const int group = p->tune_group;
+    const int kchunk = p->tune_kchunk;
     std::vector<int> order(p->n_bins);
     for (int b = 0; b < p->n_bins; ++b) order[b] = b;
     // Largest bins first so their long tiles start early (LPT).
@@ -291,7 +297,6 @@ void build_gram_work(dxg2_plan* p) {
     GramWork pad{-1, 0, 0, 0, 0, 0, 0, 0, 0, 0};  // job 0
     p->gram_work = group > 0 ? remap_for_xcds(raster, group, pad) : raster;
     p->gram_xwork = group > 0 ? remap_for_xcds(xraster, group, pad) : xraster;
-    p->use_queue = env_int("DXG2_GRAM_QUEUE", 1) != 0;
     p->qm_work = split_for_xcds(raster, group > 0 ? group : 1, p->q_off);
     p->qx_work = split_for_xcds(xraster, group > 0 ? group : 1, p->q_off + 9);
     p->gram_raster = std::move(raster);
@@ -307,10 +312,9 @@ void build_std_work(dxg2_plan* p, size_t cap_bytes) {
     const int T = p->gram_tile;
     const int nT = (p->N + T - 1) / T;
     const int64_t ntri = (int64_t)nT * (nT + 1) / 2;
-    const int band = std::max(1, env_int("DXG2_GRAM_BAND", 4));
-    const int group = std::max(0, env_int("DXG2_GRAM_GROUP", 32));
-    const int kchunk = std::min(MAX_KBLOCKS_PER_ITEM,
-                                std::max(1, env_int("DXG2_GRAM_KCHUNK", MAX_KBLOCKS_PER_ITEM)));
+    const int band = p->tune_band;
+    const int group = p->tune_group;
+    const int kchunk = p->tune_kchunk;
     const bool wide = slab_planes(p->dtype) == 2;
     const size_t tile_bytes = (size_t)T * T * 4;
     int max_items = 0;
@@ -618,6 +622,13 @@ int dxg2_plan_create(const dxg2_plan_desc* desc, dxg2_plan** out) {
     p->nrows = (desc->nframes + SLAB_ROW_ALIGN - 1) / SLAB_ROW_ALIGN * SLAB_ROW_ALIGN;
     p->dtype = desc->dtype;
     p->gram_tile = (desc->flags & DXG2_FLAG_TILE128) ? 128 : 256;
+    // Tuning defaults (environment overrides: DXG2_GRAM_BAND, _GROUP, _KCHUNK, _QUEUE);
+    // dxg2_plan_set_tuning replaces them.
+    p->tune_band = std::max(1, env_int("DXG2_GRAM_BAND", 4));
+    p->tune_group = std::max(0, env_int("DXG2_GRAM_GROUP", 32));
+    p->tune_kchunk = std::min(MAX_KBLOCKS_PER_ITEM,
+                              std::max(1, env_int("DXG2_GRAM_KCHUNK", MAX_KBLOCKS_PER_ITEM)));
+    p->use_queue = env_int("DXG2_GRAM_QUEUE", 1) != 0;
 
     // qmask -> bins (common.py:36-46)
     std::vector<std::vector<int32_t>> lists;
@@ -759,7 +770,33 @@ int dxg2_plan_create(const dxg2_plan_desc* desc, dxg2_plan** out) {
     return DXG2_OK;
 }
 
+// Release the captured dense graph (its work-list pointers are about to change or go).
+static void drop_graph(dxg2_plan* p) {
+    if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
+    p->gexec = nullptr;
+    p->gkey = {};
+}
+
+// Free the calc_std work lists and workspaces; the next calc_std call rebuilds them.
+static void drop_std(dxg2_plan* p, hipStream_t s) {
+    void* bufs[] = {p->d_std_mwork, p->d_std_xwork, p->d_std_tiles, p->d_std_scratch,
+                    p->d_std_part};
+    for (void* b : bufs) dev_free(p->alloc, b, s);
+    p->d_std_mwork = p->d_std_xwork = nullptr;
+    p->d_std_tiles = nullptr;
+    p->d_std_scratch = nullptr;
+    p->d_std_part = nullptr;
+    p->std_mwork.clear();
+    p->std_xwork.clear();
+    p->std_tiles.clear();
+    p->std_chunks.clear();
+    p->std_slots = p->std_parts = 0;
+    p->std_built = false;
+}
+
 static void plan_release(dxg2_plan* p, hipStream_t s) {
+    drop_graph(p);
+    if (p->cap_stream) (void)hipStreamDestroy(p->cap_stream);
     void* bufs[] = {p->d_pix,   p->d_bins, p->d_gwork, p->d_mwork, p->d_xwork, p->d_slab,
                     p->d_ugram, p->d_sums, p->d_psums, p->d_prefix, p->d_smax, p->d_pa,
                     p->d_pg, p->d_pm,
@@ -888,6 +925,90 @@ int dxg2_correlate_dense(dxg2_plan* p, const void* d_frames, int64_t* d_num, int
     dense_finish(p, gsums, d_num, d_sums, d_den, d_g2, s);
     if (ev) DXG2_HIP(hipEventRecord(ev[3], s));
     DXG2_HIP(hipGetLastError());
+    return DXG2_OK;
+}
+
+int dxg2_plan_get_tuning(const dxg2_plan* p, dxg2_tuning* t) {
+    if (!p || !t) return set_error(DXG2_EINVAL, "null argument");
+    t->tile = p->gram_tile;
+    t->band = p->tune_band;
+    t->group = p->tune_group;
+    t->queue = p->use_queue ? 1 : 0;
+    return DXG2_OK;
+}
+
+int dxg2_plan_set_tuning(dxg2_plan* p, const dxg2_tuning* t) {
+    if (!p || !t) return set_error(DXG2_EINVAL, "null argument");
+    if ((t->tile != 128 && t->tile != 256) || t->band < 1 || t->group < 0 ||
+        (t->queue != 0 && t->queue != 1))
+        return set_error(DXG2_EINVAL, "bad tuning (tile %d, band %d, group %d, queue %d)",
+                         t->tile, t->band, t->group, t->queue);
+    DXG2_HIP(hipDeviceSynchronize());  // calls on the plan may still read its work lists
+    drop_graph(p);
+    drop_std(p, nullptr);
+    p->gram_tile = t->tile;
+    p->tune_band = t->band;
+    p->tune_group = t->group;
+    p->use_queue = t->queue != 0;
+    p->gram_work.clear();
+    p->gram_xwork.clear();
+    build_gram_work(p);
+    struct {
+        GramWork** dst;
+        const std::vector<GramWork>* src;
+    } lists[] = {{&p->d_mwork, &p->gram_work},
+                 {&p->d_xwork, &p->gram_xwork},
+                 {&p->d_qmwork, &p->qm_work},
+                 {&p->d_qxwork, &p->qx_work}};
+    for (auto& l : lists) {
+        dev_free(p->alloc, *l.dst, nullptr);
+        *l.dst = nullptr;
+        if (l.src->empty()) continue;
+        const size_t bytes = l.src->size() * sizeof(GramWork);
+        if (dev_alloc(p->alloc, (void**)l.dst, bytes, nullptr) != hipSuccess)
+            return set_error(DXG2_ENOMEM, "cannot allocate %zu bytes of work list", bytes);
+        DXG2_HIP(hipMemcpy(*l.dst, l.src->data(), bytes, hipMemcpyHostToDevice));
+    }
+    DXG2_HIP(hipMemcpy(p->d_qoff, p->q_off, sizeof(p->q_off), hipMemcpyHostToDevice));
+    return DXG2_OK;
+}
+
+int dxg2_correlate_dense_graph(dxg2_plan* p, const void* d_frames, int64_t* d_num,
+                               int64_t* d_sums, double* d_den, float* d_g2, void* stream) {
+    if (!p) return set_error(DXG2_EINVAL, "null plan");
+    if (!d_frames || !d_g2) return set_error(DXG2_EINVAL, "frames and g2 must be non-null");
+    if (p->n_bins == 0) return DXG2_OK;
+    hipStream_t s = (hipStream_t)stream;
+    int rc = ensure_dense_buffers(p, s);  // the one-time slab clear runs eagerly, on s
+    if (rc) return rc;
+    const std::array<const void*, 5> key{d_frames, d_num, d_sums, d_den, d_g2};
+    if (!p->gexec || key != p->gkey) {
+        drop_graph(p);
+        if (!p->cap_stream)
+            DXG2_HIP(hipStreamCreateWithFlags(&p->cap_stream, hipStreamNonBlocking));
+        gram_queue_prepare(p->gram_tile);  // device queries outside the capture
+        (void)hipGetLastError();
+        DXG2_HIP(hipStreamBeginCapture(p->cap_stream, hipStreamCaptureModeThreadLocal));
+        int64_t* gsums = dense_gather(p, d_frames, d_sums, p->cap_stream);
+        dense_gram(p, gsums, p->cap_stream);
+        dense_finish(p, gsums, d_num, d_sums, d_den, d_g2, p->cap_stream);
+        hipGraph_t g = nullptr;
+        const hipError_t e1 = hipStreamEndCapture(p->cap_stream, &g);
+        const hipError_t e0 = hipGetLastError();
+        if (e0 != hipSuccess || e1 != hipSuccess || !g) {
+            if (g) (void)hipGraphDestroy(g);
+            return set_error(DXG2_EHIP, "capture of the dense chain failed: %s",
+                             hipGetErrorString(e1 != hipSuccess ? e1 : e0));
+        }
+        const hipError_t e2 = hipGraphInstantiate(&p->gexec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e2 != hipSuccess) {
+            p->gexec = nullptr;
+            return set_error(DXG2_EHIP, "graph instantiation failed: %s", hipGetErrorString(e2));
+        }
+        p->gkey = key;
+    }
+    DXG2_HIP(hipGraphLaunch(p->gexec, s));
     return DXG2_OK;
 }
 

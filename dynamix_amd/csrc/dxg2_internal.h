@@ -118,6 +118,9 @@ void launch_gram_i8_queue(int tile, bool cross, const int8_t* slab, const BinDev
                           const GramWork* work, int n_work, const int* qoff, int* heads,
                           int nframes, int nrows, int biased, int64_t* ugram, int64_t* sums,
                           hipStream_t s);
+// Device queries of the persistent launch's grid size, done once per device (outside
+// any stream capture).
+void gram_queue_prepare(int tile);
 // Grouped launch: work items of several plans, each naming its job descriptor.
 void launch_gram_i8_jobs(int tile, bool cross, const GramJob* jobs, const GramWork* work,
                          int n_work, hipStream_t s);

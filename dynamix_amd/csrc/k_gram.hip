@@ -347,6 +347,16 @@ int queue_grid() {
     return grid[dev];
 }
 
+void gram_queue_prepare(int tile) {
+    if (tile == 128) {
+        (void)queue_grid<2, 2, false>();
+        (void)queue_grid<2, 2, true>();
+    } else {
+        (void)queue_grid<4, 4, false>();
+        (void)queue_grid<4, 4, true>();
+    }
+}
+
 void launch_gram_i8_queue(int tile, bool cross, const int8_t* slab, const BinDev* bins,
                           const GramWork* work, int n_work, const int* qoff, int* heads,
                           int nframes, int nrows, int biased, int64_t* ugram, int64_t* sums,

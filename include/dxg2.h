@@ -120,6 +120,31 @@ int dxg2_correlate_dense_std(dxg2_plan* plan, const void* d_frames, int64_t* d_n
                              int64_t* d_sums, double* d_den, float* d_g2, float* d_dev,
                              void* stream);
 
+/* The dense chain of dxg2_correlate_dense (gather, persistent Gram, finalise) captured
+ * once into a HIP graph and replayed with one hipGraphLaunch on `stream`.  The graph is
+ * kept in the plan for the given buffer pointers: a call with other pointers captures
+ * again; dxg2_plan_set_tuning drops it.  Results are identical to dxg2_correlate_dense.
+ * The capture contains kernels only (every clear is a fill kernel).
+ * Replaces the per-call launch sequence of DenseCorrelator.correlate (dense.py:223-247);
+ * the graph itself has no reference counterpart. */
+int dxg2_correlate_dense_graph(dxg2_plan* plan, const void* d_frames, int64_t* d_num,
+                               int64_t* d_sums, double* d_den, float* d_g2, void* stream);
+
+/* Gram launch tuning of a plan: tile size (128 or 256 frames), raster band (tile rows),
+ * XCD chunk (consecutive items per XCD, 0 = none) and launch form (1 = persistent
+ * workgroups over per-XCD work queues, 0 = one workgroup per tile).  set_tuning waits
+ * for the device, rebuilds the work lists and drops the plan's graph and calc_std
+ * workspaces.  Used by the autotuner (dynamix_amd.autotune).  No reference counterpart:
+ * the reference takes block_size for its OpenCL kernels (dense.py:153,163). */
+typedef struct dxg2_tuning {
+    int32_t tile;
+    int32_t band;
+    int32_t group;
+    int32_t queue;
+} dxg2_tuning;
+int dxg2_plan_get_tuning(const dxg2_plan* plan, dxg2_tuning* out);
+int dxg2_plan_set_tuning(dxg2_plan* plan, const dxg2_tuning* tuning);
+
 /* Grouped dense launch over several plans (continuous batching of dense jobs): each
  * plan's gather, then ONE Gram launch over the work items of every plan (plus one for the
  * cross-plane items of uint16 plans), then each plan's finalise, all on `stream`.  Plans

@@ -75,3 +75,19 @@ def test_serving_job_stream_is_seeded_and_in_range():
     assert arr[0] == 0.0 and np.all(np.diff(arr) >= 0)
     rate = (len(a) - 1) / arr[-1]
     assert 15.0 < rate < 25.0
+
+
+def test_autotune_key_and_cache_roundtrip(tmp_path):
+    from dynamix_amd import autotune
+    k1 = autotune.tuning_key("uint8", 10000, [26000, 24000, 100, 0, 64, 65])
+    assert k1 == "uint8|N10000|kb0:1,1:2,9:2"
+    # bin order and empty bins do not change the key; sizes in another bucket do
+    assert autotune.tuning_key("uint8", 10000, [65, 64, 0, 24000, 100, 26000]) == k1
+    assert autotune.tuning_key("uint8", 10000, [26000, 24000, 100, 0, 64, 129]) != k1
+    assert autotune.tuning_key("uint16", 10000, [26000, 24000, 100, 64, 65]) != k1
+    p = str(tmp_path / "sub" / "t.json")
+    assert autotune.load_cache(p) == {}
+    autotune.save_entry(k1, {"tile": 256, "band": 4, "group": 32, "queue": 1}, p)
+    autotune.save_entry("other", {"tile": 128, "band": 2, "group": 16, "queue": 0}, p)
+    c = autotune.load_cache(p)
+    assert c[k1]["band"] == 4 and c["other"]["tile"] == 128

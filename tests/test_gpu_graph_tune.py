@@ -1,0 +1,146 @@
+"""HIP graph replay of the dense chain and Gram launch tuning on the GPU: every replay
+and every tuning gives the exact numerators of the eager default path.  Marked gpu."""
+import ctypes
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import g2 as og
+
+pytestmark = pytest.mark.gpu
+
+
+def _announce(msg):
+    print("[case] " + msg, flush=True)
+    sys.stdout.flush()
+
+
+def _stack(seed, n=300, shape=(20, 24), mean=0.6):
+    from dynamix_amd.synth import speckle_frames
+    return speckle_frames(shape, n, mean, 25.0, seed, np.uint8)
+
+
+def _qmask(seed, shape=(20, 24), nbins=4):
+    rng = np.random.default_rng(seed)
+    q = rng.integers(0, nbins + 1, size=shape).astype(np.int32)
+    for b in range(1, nbins + 1):
+        q.flat[b] = b
+    return q
+
+
+def test_graph_replays_equal_eager():
+    """Several replays of one captured graph, with eager work and synchronisations in
+    between, each against the eager result of the same frames."""
+    import torch
+    from dynamix_amd.correlator.dense import MatMulCorrelator
+    _announce("graph replays")
+    q = _qmask(1)
+    eager = MatMulCorrelator((20, 24), 300, q)
+    graph = MatMulCorrelator((20, 24), 300, q, extra_options={"graph": True})
+    for rep, seed in enumerate([3, 4, 3, 5, 6]):
+        f = _stack(seed)
+        g_e = eager.correlate(f)
+        n_e = eager.last_numerators.clone()
+        g_g = graph.correlate(torch.from_numpy(f).cuda())
+        torch.cuda.synchronize()
+        assert torch.equal(n_e, graph.last_numerators), "replay %d" % rep
+        np.testing.assert_array_equal(g_e.view(np.uint32), g_g.view(np.uint32))
+        # unrelated eager work between replays
+        x = torch.randn(1 << 16, device="cuda")
+        (x * 2).sum().item()
+    ref = og.dense_g2(_stack(6), q)
+    np.testing.assert_array_equal(graph.last_numerators.cpu().numpy(), ref["num"])
+
+
+def test_graph_recaptures_for_new_buffers_and_uint16():
+    import torch
+    from dynamix_amd import _lib
+    from dynamix_amd.correlator.common import Plan
+    _announce("graph recapture / uint16")
+    rng = np.random.default_rng(9)
+    f = rng.integers(0, 3000, size=(260, 16, 18)).astype(np.uint16)
+    q = _qmask(9, (16, 18), 3)
+    plan = Plan((16, 18), 260, q, np.uint16)
+    ref = og.dense_g2(f, q)
+    d = torch.from_numpy(f.astype(np.int32)).to(torch.int16).cuda()
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    for k in range(3):  # new output buffers each time: a capture per call
+        num = torch.empty((plan.n_bins, 260), dtype=torch.int64, device="cuda")
+        sums = torch.empty_like(num)
+        den = torch.empty((plan.n_bins, 260), dtype=torch.float64, device="cuda")
+        g2 = torch.empty((plan.n_bins, 260), dtype=torch.float32, device="cuda")
+        for _ in range(2):  # and a replay of it
+            _lib.check(plan.lib.dxg2_correlate_dense_graph(
+                plan.handle, _lib.ptr(d), _lib.ptr(num), _lib.ptr(sums), _lib.ptr(den),
+                _lib.ptr(g2), st))
+            torch.cuda.synchronize()
+            np.testing.assert_array_equal(num.cpu().numpy(), ref["num"])
+            np.testing.assert_array_equal(sums.cpu().numpy(), ref["sums"])
+    plan.close()
+
+
+@pytest.mark.parametrize("tile,band,group,queue", [
+    (256, 4, 32, 1), (256, 4, 32, 0), (128, 4, 32, 1), (128, 1, 0, 0), (256, 3, 8, 1),
+    (128, 16, 64, 1)])
+def test_every_tuning_gives_exact_numerators(tile, band, group, queue):
+    import torch
+    from dynamix_amd import _lib
+    from dynamix_amd.correlator.common import Plan
+    _announce("tuning %s" % ((tile, band, group, queue),))
+    f = _stack(12, n=700, shape=(30, 32), mean=0.4)
+    q = _qmask(12, (30, 32), 5)
+    ref = og.dense_g2(f, q)
+    plan = Plan((30, 32), 700, q, np.uint8)
+    plan.set_tuning(tile, band, group, queue)
+    assert plan.get_tuning() == {"tile": tile, "band": band, "group": group, "queue": queue}
+    d = torch.from_numpy(f).cuda()
+    nb = plan.n_bins
+    num = torch.empty((nb, 700), dtype=torch.int64, device="cuda")
+    sums = torch.empty_like(num)
+    den = torch.empty((nb, 700), dtype=torch.float64, device="cuda")
+    g2 = torch.empty((nb, 700), dtype=torch.float32, device="cuda")
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    for _ in range(2):
+        _lib.check(plan.lib.dxg2_correlate_dense(plan.handle, _lib.ptr(d), _lib.ptr(num),
+                                                 _lib.ptr(sums), _lib.ptr(den), _lib.ptr(g2), st))
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(num.cpu().numpy(), ref["num"])
+        np.testing.assert_array_equal(sums.cpu().numpy(), ref["sums"])
+    plan.close()
+
+
+def test_bad_tuning_is_rejected():
+    from dynamix_amd.correlator.common import Plan
+    plan = Plan((8, 8), 40, None, np.uint8)
+    for bad in [(64, 4, 32, 1), (256, 0, 32, 1), (256, 4, -1, 1), (256, 4, 32, 2)]:
+        with pytest.raises(ValueError):
+            plan.set_tuning(*bad)
+    plan.close()
+
+
+def test_autotune_records_and_reapplies(tmp_path, monkeypatch):
+    from dynamix_amd import autotune
+    from dynamix_amd.correlator.dense import MatMulCorrelator
+    _announce("autotune")
+    monkeypatch.setenv("DXG2_TUNING_CACHE", str(tmp_path / "tune.json"))
+    f = _stack(21, n=520, shape=(24, 30), mean=0.5)
+    q = _qmask(21, (24, 30), 3)
+    c = MatMulCorrelator((24, 30), 520, q, extra_options={"autotune": True})
+    g = c.correlate(f)
+    ref = og.dense_g2(f, q)
+    np.testing.assert_array_equal(c.last_numerators.cpu().numpy(), ref["num"])
+    cache = autotune.load_cache()
+    assert len(cache) == 1
+    (key, entry), = cache.items()
+    assert entry["gram_ms"] <= entry["default_ms"]
+    assert len(entry["trials"]) >= 10
+    plan = c._plans[np.dtype(np.uint8)]
+    assert key == autotune.plan_key(plan, 520)
+    assert plan.get_tuning() == {k: entry[k] for k in ("tile", "band", "group", "queue")}
+    c2 = MatMulCorrelator((24, 30), 520, q, extra_options={"autotune": True})
+    g2 = c2.correlate(f)
+    p2 = c2._plans[np.dtype(np.uint8)]
+    assert p2.get_tuning() == plan.get_tuning()
+    np.testing.assert_array_equal(g.view(np.uint32), g2.view(np.uint32))
