@@ -12,7 +12,10 @@ Printed as one JSON line.  Besides the metric it reports
   roofline      the Gram kernel's achieved int8 ops/s (algorithmic ops: the upper
                 triangle with the diagonal, 2 * sum_b n_b * N (N+1)/2, per launch)
                 over its average duration from HIP events recorded around each
-                launch inside the timed region, against the 5.0 POP/s dense int8 peak;
+                launch inside the timed region, against the 5.0 POP/s dense int8 peak
+                (the persistent launch over per-XCD work queues, with the fill kernel
+                that clears the queue heads);
+  calc_std      the dense path with calc_std=True (dev(tau)) on the same stack;
   cpu_baseline  the reference's numpy MatMul correlator (float32 sgemm + diagonal
                 loop, restated in oracle/) on one bin of the same stack, all host cores;
   plain_fp32    this library's one-bin-per-launch FP32 comparator on the same stack;
@@ -212,7 +215,7 @@ def bench_events(seed):
             "coo_equals_csr": same, "input_flags": flags.value, "generate_s": gen_s}
 
 
-def bench_uint16(seed, reps=2):
+def bench_uint16(seed, reps=2, tune=True):
     """Dense path on the uint16 workload (byte-split digit planes)."""
     import numpy as np
     import torch
@@ -233,6 +236,10 @@ def bench_uint16(seed, reps=2):
     def step():
         _lib.check(plan.lib.dxg2_correlate_dense(plan.handle, _lib.ptr(frames), _lib.ptr(num),
                                                  _lib.ptr(sums), _lib.ptr(den), _lib.ptr(g2), st))
+    tuning = None
+    if tune:
+        from dynamix_amd import autotune
+        tuning = autotune.autotune(plan, frames, n, reps=1, record=False)
     step()
     torch.cuda.synchronize()
     plan.set_profiling(True)
@@ -250,7 +257,9 @@ def bench_uint16(seed, reps=2):
            "stage_ms": {"gather": gather_ms, "gram": gram_ms, "finalise": fin_ms},
            "gram_useful_TOPs": useful / (gram_ms * 1e-3) / 1e12,
            "gram_executed_TOPs": executed / (gram_ms * 1e-3) / 1e12,
-           "gram_executed_frac_of_peak": executed / (gram_ms * 1e-3) / 1e12 / INT8_PEAK_TOPS}
+           "gram_executed_frac_of_peak": executed / (gram_ms * 1e-3) / 1e12 / INT8_PEAK_TOPS,
+           "gram_tuning": plan.get_tuning(),
+           "default_gram_ms": None if tuning is None else tuning["default_ms"]}
     plan.close()
     return out
 
@@ -291,6 +300,8 @@ def main():
     ap.add_argument("--no-events", action="store_true")
     ap.add_argument("--no-uint16", action="store_true")
     ap.add_argument("--no-std", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default Gram tuning instead of timing the candidates")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--seed", type=int, default=2026)
     ap.add_argument("--pmc-copy-gib", type=float, default=0.0,
@@ -327,6 +338,15 @@ def main():
     def step():
         _lib.check(lib.dxg2_correlate_dense(plan.handle, _lib.ptr(frames), _lib.ptr(num),
                                             _lib.ptr(sums), _lib.ptr(den), _lib.ptr(g2), stream))
+
+    tuning = None
+    if not args.no_autotune:
+        from dynamix_amd import autotune
+        log("autotune")
+        tuning = autotune.autotune(plan, frames, n, record=False)
+        log("tuning %s: gram %.3f ms (default %.3f ms)" % (
+            {k: tuning[k] for k in ("tile", "band", "group", "queue")}, tuning["gram_ms"],
+            tuning["default_ms"]))
 
     log("warmup %d" % args.warmup)
     for _ in range(args.warmup):
@@ -369,11 +389,16 @@ def main():
         "config": {"workload": "%s: %dx%d x %d frames %s, %d annular bins, %d binned px"
                                % (args.workload, shape[0], shape[1], n, wl["dtype"], nb, npix),
                    "path": "dense batched i8-MFMA Gram (gather + gram + finalise)",
-                   "stage_ms": {"gather": gather_ms, "gram": gram_ms, "finalise": fin_ms}},
+                   "stage_ms": {"gather": gather_ms, "gram": gram_ms, "finalise": fin_ms},
+                   "gram_tuning": plan.get_tuning(),
+                   "autotune": None if tuning is None else {
+                       "gram_ms": tuning["gram_ms"], "default_gram_ms": tuning["default_ms"],
+                       "candidates": len(tuning["trials"])}},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": INT8_PEAK_TOPS,
                      "unit": "TOP/s", "frac": achieved / INT8_PEAK_TOPS,
                      "traffic": read_traffic("gram_i8_%s_traffic.txt" % args.workload),
-                     "kernel": "gram_i8_kernel", "ops_per_launch": ops},
+                     "kernel": ("gram_i8_queue_kernel" if plan.get_tuning()["queue"]
+                                else "gram_i8_kernel"), "ops_per_launch": ops},
     }
 
     if not args.no_plain:
@@ -407,7 +432,7 @@ def main():
         torch.cuda.empty_cache()
     if not args.no_uint16 and args.workload != "uint16":
         log("uint16 path")
-        out["uint16"] = bench_uint16(args.seed)
+        out["uint16"] = bench_uint16(args.seed, tune=not args.no_autotune)
         torch.cuda.empty_cache()
     if not args.no_events:
         log("event path")

@@ -21,6 +21,7 @@ DXG2_EUNSUPPORTED = 4
 DXG2_ECAPACITY = 5
 
 DXG2_FLAG_TILE128 = 1
+DXG2_FLAG_NO_QUEUE = 2
 
 DXG2_U8 = 1
 DXG2_I8 = 2

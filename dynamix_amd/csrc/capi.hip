@@ -628,7 +628,7 @@ int dxg2_plan_create(const dxg2_plan_desc* desc, dxg2_plan** out) {
     p->tune_group = std::max(0, env_int("DXG2_GRAM_GROUP", 32));
     p->tune_kchunk = std::min(MAX_KBLOCKS_PER_ITEM,
                               std::max(1, env_int("DXG2_GRAM_KCHUNK", MAX_KBLOCKS_PER_ITEM)));
-    p->use_queue = env_int("DXG2_GRAM_QUEUE", 1) != 0;
+    p->use_queue = env_int("DXG2_GRAM_QUEUE", 1) != 0 && !(desc->flags & DXG2_FLAG_NO_QUEUE);
 
     // qmask -> bins (common.py:36-46)
     std::vector<std::vector<int32_t>> lists;

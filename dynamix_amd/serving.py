@@ -188,7 +188,7 @@ class Server(object):
 
     def __init__(self, pool, hbm_budget=None, reserve_bytes=8 << 30, max_batch=8, n_streams=2,
                  max_resident=32, keep_results=False, batch_ops=1e14, prep_threads=4,
-                 depth=2):
+                 depth=2, persistent_gram=True):
         import torch
         self.torch = torch
         self.pool = pool
@@ -200,6 +200,12 @@ class Server(object):
         self.budget = int(hbm_budget) if hbm_budget else \
             max(0, int(free) + int(idle) - int(reserve_bytes))
         self.max_batch = max_batch
+        # A persistent Gram launch holds every CU until its queues drain, so kernels of
+        # jobs on the other streams wait behind it; persistent_gram=False gives single
+        # dense jobs one workgroup per tile instead, which the hardware interleaves with
+        # other streams' work.  Measured on the 200-job mix: persistent p50 65 ms / p99
+        # 310 ms, per-tile 72 ms / 431 ms.
+        self.plan_flags = 0 if persistent_gram else _lib.DXG2_FLAG_NO_QUEUE
         self.batch_ops = batch_ops  # Gram work (int8 ops) one grouped launch may collect
         self.max_resident = max_resident
         self.keep_results = keep_results
@@ -239,7 +245,7 @@ class Server(object):
         t0 = time.perf_counter()
         job = Job(spec)
         job.qmask = job_qmask(spec)
-        job.plan = Plan(spec.shape, spec.nframes, job.qmask, np.uint8)
+        job.plan = Plan(spec.shape, spec.nframes, job.qmask, np.uint8, flags=self.plan_flags)
         nb, n = job.plan.n_bins, spec.nframes
         job.npix = int(job.plan.pixel_counts[:nb].sum())
         job.ops = 2.0 * job.npix * n * (n + 1) / 2.0

@@ -43,6 +43,9 @@ extern "C" {
 /* plan flags */
 #define DXG2_FLAG_NONE 0u
 #define DXG2_FLAG_TILE128 1u   /* Gram tiles of 128 frames (default 256) */
+#define DXG2_FLAG_NO_QUEUE 2u  /* one workgroup per Gram tile instead of the persistent
+                                  launch: leaves CUs free between tiles for kernels of
+                                  other streams (serving) */
 
 typedef struct dxg2_plan dxg2_plan;
 
