@@ -85,6 +85,15 @@ _SIGNATURES = {
                                    c_void_p]),
     "dxg2_events_to_coo": (c_int, [c_void_p, c_void_p, c_void_p, c_i64, c_int, c_int, c_int,
                                    c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "dxg2_compress_scratch_bytes": (c_int, [c_int, c_int, c_int, ctypes.POINTER(c_size_t)]),
+    "dxg2_compress_count": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_void_p,
+                                    c_void_p]),
+    "dxg2_compress_fill": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_void_p,
+                                   c_void_p, c_void_p, c_void_p]),
+    "dxg2_compress_push": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_int, c_void_p,
+                                   c_void_p, c_void_p, c_void_p, c_void_p]),
+    "dxg2_compress_compact": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_int,
+                                      c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "dxg2_last_error": (ctypes.c_char_p, []),
     "dxg2_version": (ctypes.c_char_p, []),
 }

@@ -3,7 +3,8 @@
 EventCorrelator   -- event.py:7-204.  Same constructor and correlate(vol_times,
                      vol_data, offsets) on the pixel-major event layout; also
                      correlate_coo(pixel, frame, count) for frame-sorted event lists.
-FramesCompressor  -- event.py:207-321.  Host-side producer of that layout.
+FramesCompressor  -- event.py:207-321.  Producer of that layout, on the GPU
+                     (k_compress.hip): whole stack or frame by frame.
 
 Differences from the reference, all deliberate:
   * numerators are recomputed from zero on every call (the reference zeroes res_int
@@ -144,50 +145,134 @@ class EventCorrelator(BaseCorrelator):
         return self._finish(torch, outs)
 
 
-class FramesCompressor(object):
-    """Dense frames -> (data, times, offsets) event layout (event.py:207-321).
+def _np_dtype_of_tensor(t):
+    """numpy dtype of a frames tensor (uint16 frames travel as int16 or uint16)."""
+    name = str(t.dtype).replace("torch.", "")
+    if name in ("int16", "uint16"):
+        return np.dtype(np.uint16)
+    return _dtype_of(name)
 
-    For every pixel in raster order, its non-zero samples sorted by frame are
-    concatenated; offsets[p] .. offsets[p+1] index pixel p's events.
+
+class FramesCompressor(object):
+    """Dense frames -> (data, times, offsets) event layout (event.py:207-321), on the GPU.
+
+    For every pixel in raster order, its samples with value > 0 (signed for int8), in
+    frame order, are concatenated; offsets[p] .. offsets[p+1] index pixel p's events.
+    numpy inputs give numpy outputs, like the reference; GPU tensors give GPU tensors
+    that dxg2_correlate_events_csr takes directly.
+
+    Differences from the reference, both deliberate: values are kept in the frames'
+    dtype (the reference stores them in `dtype`, int8 by default, where uint8 counts
+    above 127 turn negative and are then dropped from the data but not from the
+    offsets); a pixel with more than max_nnz events raises IndexError from
+    get_compacted_events, not from the process_frame call that overflowed.
     """
 
     def __init__(self, shape, nframes, max_nnz, dtype=np.int8):
-        self.shape = shape
-        self.nframes = nframes
+        self.shape = tuple(int(x) for x in shape)
+        self.nframes = int(nframes)
         self.dtype = dtype
-        self.max_nnz = max_nnz
+        self.max_nnz = int(max_nnz)
         self.npix = int(np.prod(self.shape))
-        self.events_counter = np.zeros(self.npix, dtype=np.int32)
-        self.events = np.zeros((self.npix, self.max_nnz), dtype=self.dtype)
-        self.times = np.zeros((self.npix, self.max_nnz), dtype=np.int32)
         self.frames_counter = 0
+        self._state = None  # device slots, allocated by the first frame (its dtype)
 
     @staticmethod
     def compress_all_stack(frames):
         """Whole stack at once (event.py:267-284)."""
-        assert frames.ndim == 3
-        n = frames.shape[0]
-        per_pixel = np.moveaxis(frames, 0, -1).reshape(-1, n)
-        pix, t = np.nonzero(per_pixel > 0)
-        offsets = np.zeros(per_pixel.shape[0] + 1, dtype=np.uint32)
-        offsets[1:] = np.cumsum(np.bincount(pix, minlength=per_pixel.shape[0]))
-        return per_pixel[pix, t], t.astype(np.int32), offsets
+        torch = _torch()
+        lib = _lib.load()
+        on_device = not isinstance(frames, np.ndarray)
+        if on_device:
+            assert frames.dim() == 3
+            d = frames.contiguous()
+            dt = _np_dtype_of_tensor(d)
+        else:
+            assert frames.ndim == 3
+            dt = np.dtype(frames.dtype)
+            d = torch.from_numpy(np.ascontiguousarray(frames).view(
+                np.int16 if dt == np.uint16 else dt)).cuda()
+        dt = _dtype_of(dt)
+        n, h, w = (int(x) for x in d.shape)
+        nbytes = ctypes.c_size_t()
+        _lib.check(lib.dxg2_compress_scratch_bytes(n, h, w, ctypes.byref(nbytes)))
+        scratch = torch.empty(max(nbytes.value // 4, 1), dtype=torch.int32, device="cuda")
+        offsets = torch.empty(h * w + 1, dtype=torch.int32, device="cuda")
+        st = current_stream(torch)
+        code = _lib.DTYPE_CODES[dt]
+        _lib.check(lib.dxg2_compress_count(_lib.ptr(d), code, n, h, w, _lib.ptr(scratch),
+                                           _lib.ptr(offsets), st), "dxg2_compress_count")
+        nnz = int(offsets[h * w].item()) & 0xFFFFFFFF
+        times = torch.empty(max(nnz, 1), dtype=torch.int32, device="cuda")
+        data = torch.empty(max(nnz, 1), dtype=d.dtype, device="cuda")
+        _lib.check(lib.dxg2_compress_fill(_lib.ptr(d), code, n, h, w, _lib.ptr(scratch),
+                                          _lib.ptr(offsets), _lib.ptr(times), _lib.ptr(data), st),
+                   "dxg2_compress_fill")
+        data, times = data[:nnz], times[:nnz]
+        if on_device:
+            return data, times, offsets
+        out = data.cpu().numpy()
+        return (out.view(np.uint16) if dt == np.uint16 else out, times.cpu().numpy(),
+                offsets.cpu().numpy().view(np.uint32))
+
+    def _slots(self, torch, tdtype, code):
+        if self._state is None:
+            n = self.npix * self.max_nnz
+            self._state = {
+                "code": code,
+                "counter": torch.zeros(self.npix, dtype=torch.int32, device="cuda"),
+                "events": torch.zeros(max(n, 1), dtype=tdtype, device="cuda"),
+                "times": torch.zeros(max(n, 1), dtype=torch.int32, device="cuda"),
+                "err": torch.zeros(1, dtype=torch.int32, device="cuda"),
+            }
+        elif self._state["code"] != code:
+            raise ValueError("frames of one compressor must share their dtype")
+        return self._state
 
     def process_frame(self, frame):
         """Append one frame's events (event.py:287-297)."""
-        f = frame.ravel()
-        m = f > 0
-        self.events[m, self.events_counter[m]] = f[m]
-        self.times[m, self.events_counter[m]] = self.frames_counter
-        self.events_counter[m] += 1
+        torch = _torch()
+        lib = _lib.load()
+        if isinstance(frame, np.ndarray):
+            dt = _dtype_of(frame.dtype)
+            d = torch.from_numpy(np.ascontiguousarray(frame).view(
+                np.int16 if dt == np.uint16 else dt)).cuda()
+        else:
+            d = frame.contiguous()
+            dt = _np_dtype_of_tensor(d)
+        assert d.numel() == self.npix, "frame size differs from the compressor's shape"
+        st = self._slots(torch, d.dtype, _lib.DTYPE_CODES[dt])
+        _lib.check(lib.dxg2_compress_push(
+            _lib.ptr(d), st["code"], self.shape[0], self.shape[1], self.frames_counter,
+            self.max_nnz, _lib.ptr(st["counter"]), _lib.ptr(st["events"]), _lib.ptr(st["times"]),
+            _lib.ptr(st["err"]), current_stream(torch)), "dxg2_compress_push")
         self.frames_counter += 1
 
-    def get_compacted_events(self, wait_for_all_frames=True):
-        """(data, times, offsets) of the frames processed so far (event.py:300-321)."""
+    def get_compacted_events(self, wait_for_all_frames=True, device=False):
+        """(data, times, offsets) of the frames processed so far (event.py:300-321);
+        GPU tensors with device=True."""
         if self.frames_counter < self.nframes - 1 and wait_for_all_frames:
             raise RuntimeError("Not all frames were compressed yet (%d/%d)"
                                % (self.frames_counter, self.nframes))
-        offsets = np.zeros(self.npix + 1, dtype=np.uint32)
-        offsets[1:] = np.cumsum(self.events_counter)
-        m = self.events.ravel() > 0
-        return self.events.ravel()[m], self.times.ravel()[m], offsets
+        torch = _torch()
+        lib = _lib.load()
+        if self._state is None:
+            self._slots(torch, torch.int8, _lib.DXG2_I8)
+        st = self._state
+        if int(st["err"].item()):
+            raise IndexError("a pixel has more than max_nnz=%d events" % self.max_nnz)
+        scratch = torch.empty((self.npix + 1023) // 1024 + 2, dtype=torch.int32, device="cuda")
+        offsets = torch.empty(self.npix + 1, dtype=torch.int32, device="cuda")
+        total = int(st["counter"].sum().item())
+        data = torch.empty(max(total, 1), dtype=st["events"].dtype, device="cuda")
+        times = torch.empty(max(total, 1), dtype=torch.int32, device="cuda")
+        _lib.check(lib.dxg2_compress_compact(
+            _lib.ptr(st["counter"]), _lib.ptr(st["events"]), _lib.ptr(st["times"]), st["code"],
+            self.shape[0], self.shape[1], self.max_nnz, _lib.ptr(scratch), _lib.ptr(offsets),
+            _lib.ptr(data), _lib.ptr(times), current_stream(torch)), "dxg2_compress_compact")
+        data, times = data[:total], times[:total]
+        if device:
+            return data, times, offsets
+        out = data.cpu().numpy()
+        return (out.view(np.uint16) if st["code"] == _lib.DXG2_U16 else out,
+                times.cpu().numpy(), offsets.cpu().numpy().view(np.uint32))

@@ -1405,6 +1405,82 @@ int dxg2_events_window(const uint32_t* d_offsets, const int32_t* d_times, const 
     return DXG2_OK;
 }
 
+int dxg2_compress_scratch_bytes(int32_t nframes, int32_t height, int32_t width, size_t* bytes) {
+    if (!bytes || nframes <= 0 || height <= 0 || width <= 0)
+        return set_error(DXG2_EINVAL, "bad compressor geometry");
+    *bytes = compress_scratch_bytes(nframes, (int64_t)height * width);
+    return DXG2_OK;
+}
+
+static int compress_args(const void* frames, int32_t dtype, int32_t nframes, int32_t height,
+                         int32_t width) {
+    if (!frames) return set_error(DXG2_EINVAL, "null frames");
+    if (nframes <= 0 || height <= 0 || width <= 0)
+        return set_error(DXG2_EINVAL, "bad frame stack (%d, %d, %d)", nframes, height, width);
+    if (dtype != DXG2_U8 && dtype != DXG2_I8 && dtype != DXG2_U16)
+        return set_error(DXG2_EINVAL, "unsupported dtype code %d", dtype);
+    return DXG2_OK;
+}
+
+int dxg2_compress_count(const void* d_frames, int32_t dtype, int32_t nframes, int32_t height,
+                        int32_t width, uint32_t* d_scratch, uint32_t* d_offsets, void* stream) {
+    int rc = compress_args(d_frames, dtype, nframes, height, width);
+    if (rc) return rc;
+    if (!d_scratch || !d_offsets) return set_error(DXG2_EINVAL, "null scratch or offsets");
+    (void)hipGetLastError();
+    launch_compress_count(dtype, d_frames, nframes, (int64_t)height * width, d_scratch,
+                          d_offsets, (hipStream_t)stream);
+    DXG2_HIP(hipGetLastError());
+    return DXG2_OK;
+}
+
+int dxg2_compress_fill(const void* d_frames, int32_t dtype, int32_t nframes, int32_t height,
+                       int32_t width, const uint32_t* d_scratch, const uint32_t* d_offsets,
+                       int32_t* d_times, void* d_data, void* stream) {
+    int rc = compress_args(d_frames, dtype, nframes, height, width);
+    if (rc) return rc;
+    if (!d_scratch || !d_offsets || !d_times || !d_data)
+        return set_error(DXG2_EINVAL, "null scratch, offsets or outputs");
+    (void)hipGetLastError();
+    launch_compress_fill(dtype, d_frames, nframes, (int64_t)height * width, d_scratch, d_offsets,
+                         d_times, d_data, (hipStream_t)stream);
+    DXG2_HIP(hipGetLastError());
+    return DXG2_OK;
+}
+
+int dxg2_compress_push(const void* d_frame, int32_t dtype, int32_t height, int32_t width,
+                       int32_t t, int32_t max_nnz, uint32_t* d_counter, void* d_events,
+                       int32_t* d_times, int32_t* d_err, void* stream) {
+    int rc = compress_args(d_frame, dtype, 1, height, width);
+    if (rc) return rc;
+    if (!d_counter || !d_events || !d_times || !d_err || max_nnz <= 0 || t < 0)
+        return set_error(DXG2_EINVAL, "bad compressor state");
+    (void)hipGetLastError();
+    launch_compress_push(dtype, d_frame, (int64_t)height * width, t, max_nnz, d_counter,
+                         d_events, d_times, d_err, (hipStream_t)stream);
+    DXG2_HIP(hipGetLastError());
+    return DXG2_OK;
+}
+
+int dxg2_compress_compact(const uint32_t* d_counter, const void* d_events,
+                          const int32_t* d_times, int32_t dtype, int32_t height, int32_t width,
+                          int32_t max_nnz, uint32_t* d_scratch, uint32_t* d_offsets,
+                          void* d_out_data, int32_t* d_out_times, void* stream) {
+    if (!d_counter || !d_events || !d_times || !d_scratch || !d_offsets || !d_out_data ||
+        !d_out_times)
+        return set_error(DXG2_EINVAL, "null argument");
+    if (height <= 0 || width <= 0 || max_nnz <= 0)
+        return set_error(DXG2_EINVAL, "bad compressor geometry");
+    if (dtype != DXG2_U8 && dtype != DXG2_I8 && dtype != DXG2_U16)
+        return set_error(DXG2_EINVAL, "unsupported dtype code %d", dtype);
+    (void)hipGetLastError();
+    launch_compress_compact(dtype, d_counter, d_events, d_times, (int64_t)height * width,
+                            max_nnz, d_scratch, d_offsets, d_out_data, d_out_times,
+                            (hipStream_t)stream);
+    DXG2_HIP(hipGetLastError());
+    return DXG2_OK;
+}
+
 int dxg2_events_to_coo(const int32_t* d_times, const void* d_data, const uint32_t* d_offsets,
                        int64_t nnz, int32_t height, int32_t width, int32_t nframes, int32_t dtype,
                        uint32_t* d_pixel, uint32_t* d_frame, void* d_count, void* stream) {

@@ -186,6 +186,24 @@ void launch_std_tiles(int tile, const StdTile* tiles, int n_tiles, const BinDev*
 void launch_dev_finalize(const StdPart* part, const BinDev* bins, int n_bins, int nframes,
                          int tile, float* dev, hipStream_t s);
 
+// Frames -> per-pixel event lists (FramesCompressor layout).  scratch holds
+// compress_scratch_bytes(); the count pass leaves offsets (hw + 1 entries) and the
+// per-chunk bases in it for the fill pass.
+size_t compress_scratch_bytes(int nframes, int64_t hw);
+void launch_compress_count(int dtype, const void* frames, int nframes, int64_t hw,
+                           uint32_t* scratch, uint32_t* offsets, hipStream_t s);
+void launch_compress_fill(int dtype, const void* frames, int nframes, int64_t hw,
+                          const uint32_t* scratch, const uint32_t* offsets, int32_t* times,
+                          void* data, hipStream_t s);
+void launch_compress_push(int dtype, const void* frame, int64_t hw, int t, int max_nnz,
+                          uint32_t* counter, void* events, int32_t* times, int* err,
+                          hipStream_t s);
+void launch_compress_compact(int dtype, const uint32_t* counter, const void* events,
+                             const int32_t* times, int64_t hw, int max_nnz, uint32_t* scratch,
+                             uint32_t* offsets, void* out_data, int32_t* out_times,
+                             hipStream_t s);
+void launch_clamp_u32(uint32_t* p, int64_t n, uint32_t hi, hipStream_t s);
+
 int gram_tile_size(int variant);
 
 }  // namespace dxg2

@@ -247,6 +247,36 @@ int dxg2_events_window(const uint32_t* d_offsets, const int32_t* d_times, const 
                        uint32_t* d_scratch, uint32_t* d_out_offsets, int32_t* d_out_times,
                        void* d_out_data, int64_t capacity, void* stream);
 
+/* Dense frames -> per-pixel event lists on the GPU (FramesCompressor, event.py:207-321):
+ * for every pixel p in raster order, its samples with value > 0 (signed for int8) in
+ * frame order, data[offsets[p] .. offsets[p+1]) and times[...] (int32).  This is the
+ * input layout of dxg2_correlate_events_csr.
+ *   count: offsets (H*W + 1 uint32, offsets[H*W] = number of events); the caller reads
+ *          the total to size times and data (the frame dtype);
+ *   fill:  writes times and data; it needs the same scratch, still holding what count
+ *          left in it (dxg2_compress_scratch_bytes: per-chunk counts of the frame axis).
+ * Replaces FramesCompressor.compress_all_stack (event.py:267-284). */
+int dxg2_compress_scratch_bytes(int32_t nframes, int32_t height, int32_t width,
+                                size_t* bytes);
+int dxg2_compress_count(const void* d_frames, int32_t dtype, int32_t nframes, int32_t height,
+                        int32_t width, uint32_t* d_scratch, uint32_t* d_offsets, void* stream);
+int dxg2_compress_fill(const void* d_frames, int32_t dtype, int32_t nframes, int32_t height,
+                       int32_t width, const uint32_t* d_scratch, const uint32_t* d_offsets,
+                       int32_t* d_times, void* d_data, void* stream);
+/* Streaming compressor: frame t appends its events to per-pixel slots (pixel p owns
+ * entries p*max_nnz .. (p+1)*max_nnz of events/times; counter: H*W uint32, zeroed by the
+ * caller).  A pixel with more than max_nnz events sets *d_err (int32, zeroed by the
+ * caller).  compact: the 1-D layout above from the slots; scratch holds
+ * (H*W + 1023)/1024 + 2 uint32.
+ * Replace FramesCompressor.process_frame / get_compacted_events (event.py:287-321). */
+int dxg2_compress_push(const void* d_frame, int32_t dtype, int32_t height, int32_t width,
+                       int32_t t, int32_t max_nnz, uint32_t* d_counter, void* d_events,
+                       int32_t* d_times, int32_t* d_err, void* stream);
+int dxg2_compress_compact(const uint32_t* d_counter, const void* d_events,
+                          const int32_t* d_times, int32_t dtype, int32_t height, int32_t width,
+                          int32_t max_nnz, uint32_t* d_scratch, uint32_t* d_offsets,
+                          void* d_out_data, int32_t* d_out_times, void* stream);
+
 /* Pixel-major events -> frame-sorted COO (pixel, frame, count), pixel order within a
  * frame unspecified.  Waits for the stream.  Test-data utility; no reference counterpart. */
 int dxg2_events_to_coo(const int32_t* d_times, const void* d_data, const uint32_t* d_offsets,
