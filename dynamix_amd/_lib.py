@@ -61,6 +61,7 @@ _SIGNATURES = {
     "dxg2_correlate_dense": (c_int, [c_void_p] * 7),
     "dxg2_correlate_dense_std": (c_int, [c_void_p] * 8),
     "dxg2_correlate_dense_graph": (c_int, [c_void_p] * 7),
+    "dxg2_correlate_dense_ttcf": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
     "dxg2_plan_get_tuning": (c_int, [c_void_p, ctypes.POINTER(Tuning)]),
     "dxg2_plan_set_tuning": (c_int, [c_void_p, ctypes.POINTER(Tuning)]),
     "dxg2_correlate_dense_plain": (c_int, [c_void_p] * 4),

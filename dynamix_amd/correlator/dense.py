@@ -125,6 +125,23 @@ class _DenseHIP(BaseCorrelator):
         return CorrelationResult(g2, dev) if calc_std else g2
 
 
+    def two_time(self, frames, q):
+        """Two-time correlation matrix of bin q (1-based): float32 (nframes, nframes),
+        the `trc` the reference's event correlators return for ttcf_par = q
+        (event_y.py:137-157), from the exact integer Gram in FP64."""
+        torch = _torch()
+        d_frames, dtype = to_device_frames(frames, self.shape, self.nframes, torch)
+        plan = self._plan(dtype)
+        n = self.nframes
+        trc = torch.empty((n, n), dtype=torch.float32, device="cuda")
+        _lib.check(plan.lib.dxg2_correlate_dense_ttcf(plan.handle, _lib.ptr(d_frames), int(q),
+                                                      _lib.ptr(trc), current_stream(torch)),
+                   "dxg2_correlate_dense_ttcf")
+        if self.extra_options.get("return_device", False):
+            return trc
+        return trc.cpu().numpy()
+
+
 class MatMulCorrelator(_DenseHIP):
     """Drop-in for MatMulCorrelator (dense.py:118-142).  scale_factor is accepted and,
     as in the reference, not used: g2 is normalised by the pixel count of each bin."""

@@ -146,6 +146,26 @@ std::vector<T> split_for_xcds(const std::vector<T>& in, int group, int* off) {
     return out;
 }
 
+// Work of the launches that store whole Gram tiles (calc_std, two-time): every
+// upper-triangle tile of the bins concerned, its work items (k-ranges; for uint16 the
+// HH, LL and cross items) numbered by scratch slot, cut into chunks whose slots fit the
+// scratch.
+struct TileWork {
+    struct Chunk {
+        int tile0, tile1;  // [tile0, tile1) of tiles
+        int m0, m1;        // of mwork
+        int x0, x1;        // of xwork
+    };
+    std::vector<GramWork> mwork, xwork;
+    std::vector<StdTile> tiles;
+    std::vector<Chunk> chunks;
+    size_t slots = 0;  // scratch slots of the largest chunk
+    GramWork* d_mwork = nullptr;
+    GramWork* d_xwork = nullptr;
+    StdTile* d_tiles = nullptr;
+    int* d_scratch = nullptr;
+};
+
 }  // namespace
 
 struct dxg2_plan {
@@ -200,24 +220,14 @@ struct dxg2_plan {
     float* d_pm = nullptr;
     size_t pa_elems = 0;
 
-    // calc_std (built on the first dxg2_correlate_dense_std call): per chunk of tiles,
-    // the plain and cross work items (each with its scratch slot) and the tiles.
-    struct StdChunk {
-        int tile0, tile1;  // [tile0, tile1) of std_tiles
-        int m0, m1;        // of std_mwork
-        int x0, x1;        // of std_xwork
-    };
+    // calc_std (built on the first dxg2_correlate_dense_std call) and the two-time output
+    // (for one bin, rebuilt when the bin changes): per-tile work with scratch slots
     bool std_built = false;
-    std::vector<GramWork> std_mwork, std_xwork;
-    std::vector<StdTile> std_tiles;
-    std::vector<StdChunk> std_chunks;
-    size_t std_slots = 0;         // scratch slots of the largest chunk
+    TileWork stdw;
     size_t std_parts = 0;         // StdPart entries: n_bins * T(T+1)/2 tiles * (2T - 1)
-    GramWork* d_std_mwork = nullptr;
-    GramWork* d_std_xwork = nullptr;
-    StdTile* d_std_tiles = nullptr;
-    int* d_std_scratch = nullptr;
     StdPart* d_std_part = nullptr;
+    int ttcf_bin = -1;
+    TileWork ttw;
 
     // event path
     std::vector<EventWork> event_work;
@@ -304,11 +314,9 @@ void build_gram_work(dxg2_plan* p) {
     p->gram_group = group;
 }
 
-// calc_std work: every upper-triangle tile of every bin, its work items (k-ranges of at
-// most MAX_KBLOCKS_PER_ITEM; for uint16 the HH, LL and cross items) numbered by scratch
-// slot, cut into chunks whose slots fit `cap_bytes` of int32 tiles.  Tiles follow the
-// band order of the plain launch; each chunk's items are dealt to the XCDs.
-void build_std_work(dxg2_plan* p, size_t cap_bytes) {
+// Tile work (see TileWork) for every bin, or for bin `only` (0-based) alone.  Tiles
+// follow the band order of the plain launch; each chunk's items are dealt to the XCDs.
+void build_tile_work(const dxg2_plan* p, int only, size_t cap_bytes, TileWork& tw) {
     const int T = p->gram_tile;
     const int nT = (p->N + T - 1) / T;
     const int64_t ntri = (int64_t)nT * (nT + 1) / 2;
@@ -318,33 +326,36 @@ void build_std_work(dxg2_plan* p, size_t cap_bytes) {
     const bool wide = slab_planes(p->dtype) == 2;
     const size_t tile_bytes = (size_t)T * T * 4;
     int max_items = 0;
-    for (const auto& bd : p->bins) {
-        const int n1 = (bd.nkb + kchunk - 1) / kchunk;
-        max_items = std::max(max_items, wide ? 2 * n1 + (2 * bd.nkb + kchunk - 1) / kchunk : n1);
+    for (int b = 0; b < p->n_bins; ++b) {
+        if (only >= 0 && b != only) continue;
+        const int nkb = p->bins[b].nkb;
+        const int n1 = (nkb + kchunk - 1) / kchunk;
+        max_items = std::max(max_items, wide ? 2 * n1 + (2 * nkb + kchunk - 1) / kchunk : n1);
     }
     const size_t cap_slots = std::max<size_t>((size_t)std::max(max_items, 1), cap_bytes / tile_bytes);
     std::vector<GramWork> cm, cx;  // current chunk's items, in tile order
     size_t slots = 0;
     GramWork pad{-1, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     auto close_chunk = [&]() {  // once per chunk, before the next one opens and at the end
-        if (p->std_chunks.empty()) return;
-        auto& ch = p->std_chunks.back();
-        ch.tile1 = (int)p->std_tiles.size();
+        if (tw.chunks.empty()) return;
+        auto& ch = tw.chunks.back();
+        ch.tile1 = (int)tw.tiles.size();
         auto m = group > 0 ? remap_for_xcds(cm, group, pad) : cm;
         auto x = group > 0 ? remap_for_xcds(cx, group, pad) : cx;
-        ch.m0 = (int)p->std_mwork.size();
-        p->std_mwork.insert(p->std_mwork.end(), m.begin(), m.end());
-        ch.m1 = (int)p->std_mwork.size();
-        ch.x0 = (int)p->std_xwork.size();
-        p->std_xwork.insert(p->std_xwork.end(), x.begin(), x.end());
-        ch.x1 = (int)p->std_xwork.size();
-        p->std_slots = std::max(p->std_slots, slots);
+        ch.m0 = (int)tw.mwork.size();
+        tw.mwork.insert(tw.mwork.end(), m.begin(), m.end());
+        ch.m1 = (int)tw.mwork.size();
+        ch.x0 = (int)tw.xwork.size();
+        tw.xwork.insert(tw.xwork.end(), x.begin(), x.end());
+        ch.x1 = (int)tw.xwork.size();
+        tw.slots = std::max(tw.slots, slots);
         cm.clear();
         cx.clear();
         slots = 0;
     };
     const int nb = p->n_bins;
     for (int b = 0; b < nb; ++b) {
+        if (only >= 0 && b != only) continue;
         const int nkb = p->bins[b].nkb;
         if (nkb == 0) continue;
         const int n_hh = (nkb + kchunk - 1) / kchunk;
@@ -353,12 +364,12 @@ void build_std_work(dxg2_plan* p, size_t cap_bytes) {
         for (int i0 = 0; i0 < nT; i0 += band)
             for (int j = i0; j < nT; ++j)
                 for (int i = i0; i < std::min(nT, i0 + band) && i <= j; ++i) {
-                    if (p->std_chunks.empty() || slots + n_items > cap_slots) {
+                    if (tw.chunks.empty() || slots + n_items > cap_slots) {
                         close_chunk();
-                        p->std_chunks.push_back({(int)p->std_tiles.size(), -1, 0, 0, 0, 0});
+                        tw.chunks.push_back({(int)tw.tiles.size(), -1, 0, 0, 0, 0});
                     }
                     const int idx = (int)(b * ntri + (int64_t)i * nT - (int64_t)i * (i - 1) / 2 + (j - i));
-                    p->std_tiles.push_back(StdTile{b, i, j, (int)slots, n_hh, wide ? n_hh : 0, n_x, idx});
+                    tw.tiles.push_back(StdTile{b, i, j, (int)slots, n_hh, wide ? n_hh : 0, n_x, idx});
                     auto add = [&](int k0, int k1, int out, int cross) {
                         (cross ? cx : cm).push_back(
                             GramWork{b, i, j, k0, k1, out, cross, 0, (int)slots++, 0});
@@ -373,7 +384,33 @@ void build_std_work(dxg2_plan* p, size_t cap_bytes) {
                 }
     }
     close_chunk();
-    p->std_parts = (size_t)std::max(nb, 1) * ntri * (2 * T - 1);
+}
+
+// Free a TileWork's device copies and scratch and clear its lists.
+void drop_tile_work(const DevAllocator& a, TileWork& tw, hipStream_t s) {
+    void* bufs[] = {tw.d_mwork, tw.d_xwork, tw.d_tiles, tw.d_scratch};
+    for (void* b : bufs) dev_free(a, b, s);
+    tw = TileWork();
+}
+
+// Upload a TileWork's lists (pageable host memory: the copies complete before return)
+// and allocate its scratch slots.
+int upload_tile_work(const DevAllocator& a, TileWork& tw, hipStream_t s, int T) {
+    auto upload = [&](void** dst, const void* src, size_t bytes) -> int {
+        if (bytes == 0) return DXG2_OK;
+        if (dev_alloc(a, dst, bytes, s) != hipSuccess)
+            return set_error(DXG2_ENOMEM, "cannot allocate %zu bytes of tile work", bytes);
+        DXG2_HIP(hipMemcpyAsync(*dst, src, bytes, hipMemcpyHostToDevice, s));
+        return DXG2_OK;
+    };
+    int rc = upload((void**)&tw.d_mwork, tw.mwork.data(), tw.mwork.size() * sizeof(GramWork));
+    if (!rc) rc = upload((void**)&tw.d_xwork, tw.xwork.data(), tw.xwork.size() * sizeof(GramWork));
+    if (!rc) rc = upload((void**)&tw.d_tiles, tw.tiles.data(), tw.tiles.size() * sizeof(StdTile));
+    if (!rc && dev_alloc(a, (void**)&tw.d_scratch, std::max<size_t>(tw.slots, 1) * T * T * 4, s) !=
+                   hipSuccess)
+        rc = set_error(DXG2_ENOMEM, "cannot allocate %zu Gram tile slots", tw.slots);
+    if (!rc) DXG2_HIP(hipStreamSynchronize(s));
+    return rc;
 }
 
 size_t std_scratch_cap() {
@@ -382,26 +419,27 @@ size_t std_scratch_cap() {
 
 int ensure_std_buffers(dxg2_plan* p, hipStream_t s) {
     if (p->std_built) return DXG2_OK;
-    build_std_work(p, std_scratch_cap());
-    const size_t T = p->gram_tile;
-    auto upload = [&](void** dst, const void* src, size_t bytes) -> int {
-        if (bytes == 0) return DXG2_OK;
-        if (dev_alloc(p->alloc, dst, bytes, s) != hipSuccess)
-            return set_error(DXG2_ENOMEM, "cannot allocate %zu bytes of calc_std work", bytes);
-        DXG2_HIP(hipMemcpyAsync(*dst, src, bytes, hipMemcpyHostToDevice, s));
-        return DXG2_OK;
-    };
-    int rc = upload((void**)&p->d_std_mwork, p->std_mwork.data(), p->std_mwork.size() * sizeof(GramWork));
-    if (!rc) rc = upload((void**)&p->d_std_xwork, p->std_xwork.data(), p->std_xwork.size() * sizeof(GramWork));
-    if (!rc) rc = upload((void**)&p->d_std_tiles, p->std_tiles.data(), p->std_tiles.size() * sizeof(StdTile));
-    if (!rc && dev_alloc(p->alloc, (void**)&p->d_std_scratch, std::max<size_t>(p->std_slots, 1) * T * T * 4, s) != hipSuccess)
-        rc = set_error(DXG2_ENOMEM, "cannot allocate %zu calc_std tile slots", p->std_slots);
+    build_tile_work(p, -1, std_scratch_cap(), p->stdw);
+    const int T = p->gram_tile;
+    const int nT = (p->N + T - 1) / T;
+    p->std_parts = (size_t)std::max(p->n_bins, 1) * ((size_t)nT * (nT + 1) / 2) * (2 * T - 1);
+    int rc = upload_tile_work(p->alloc, p->stdw, s, T);
     if (!rc && dev_alloc(p->alloc, (void**)&p->d_std_part, p->std_parts * sizeof(StdPart), s) != hipSuccess)
         rc = set_error(DXG2_ENOMEM, "cannot allocate calc_std diagonal segments");
     if (rc) return rc;
-    // the host lists are uploaded from pageable memory: wait before anything may change them
-    DXG2_HIP(hipStreamSynchronize(s));
     p->std_built = true;
+    return DXG2_OK;
+}
+
+// Tile work of the two-time output for bin b (0-based), rebuilt when b changes.
+int ensure_ttcf_buffers(dxg2_plan* p, int b, hipStream_t s) {
+    if (p->ttcf_bin == b) return DXG2_OK;
+    drop_tile_work(p->alloc, p->ttw, s);
+    p->ttcf_bin = -1;
+    build_tile_work(p, b, std_scratch_cap(), p->ttw);
+    int rc = upload_tile_work(p->alloc, p->ttw, s, p->gram_tile);
+    if (rc) return rc;
+    p->ttcf_bin = b;
     return DXG2_OK;
 }
 
@@ -410,8 +448,8 @@ size_t std_footprint(const dxg2_plan* p) {
     const size_t nT = (p->N + T - 1) / T;
     const size_t tiles = (size_t)std::max(p->n_bins, 1) * nT * (nT + 1) / 2;
     if (p->std_built)
-        return (p->std_mwork.size() + p->std_xwork.size()) * sizeof(GramWork) +
-               p->std_tiles.size() * sizeof(StdTile) + std::max<size_t>(p->std_slots, 1) * T * T * 4 +
+        return (p->stdw.mwork.size() + p->stdw.xwork.size()) * sizeof(GramWork) +
+               p->stdw.tiles.size() * sizeof(StdTile) + std::max<size_t>(p->stdw.slots, 1) * T * T * 4 +
                p->std_parts * sizeof(StdPart);
     // before the first call: every tile's items bounded by the scratch cap
     const int planes = slab_planes(p->dtype);
@@ -777,21 +815,15 @@ static void drop_graph(dxg2_plan* p) {
     p->gkey = {};
 }
 
-// Free the calc_std work lists and workspaces; the next calc_std call rebuilds them.
+// Free the calc_std and two-time work lists and workspaces; the next call rebuilds them.
 static void drop_std(dxg2_plan* p, hipStream_t s) {
-    void* bufs[] = {p->d_std_mwork, p->d_std_xwork, p->d_std_tiles, p->d_std_scratch,
-                    p->d_std_part};
-    for (void* b : bufs) dev_free(p->alloc, b, s);
-    p->d_std_mwork = p->d_std_xwork = nullptr;
-    p->d_std_tiles = nullptr;
-    p->d_std_scratch = nullptr;
+    drop_tile_work(p->alloc, p->stdw, s);
+    drop_tile_work(p->alloc, p->ttw, s);
+    dev_free(p->alloc, p->d_std_part, s);
     p->d_std_part = nullptr;
-    p->std_mwork.clear();
-    p->std_xwork.clear();
-    p->std_tiles.clear();
-    p->std_chunks.clear();
-    p->std_slots = p->std_parts = 0;
+    p->std_parts = 0;
     p->std_built = false;
+    p->ttcf_bin = -1;
 }
 
 static void plan_release(dxg2_plan* p, hipStream_t s) {
@@ -802,9 +834,9 @@ static void plan_release(dxg2_plan* p, hipStream_t s) {
                     p->d_pg, p->d_pm,
                     p->d_ework, p->d_err,  p->d_cnt,   p->d_scan,  p->d_off,
                     p->d_etimes, p->d_edata, p->d_kbwork, p->d_xs, p->d_tw,
-                    p->d_power, p->d_fnum, p->d_std_mwork, p->d_std_xwork,
-                    p->d_qmwork, p->d_qxwork, p->d_qoff, p->d_heads,
-                    p->d_std_tiles, p->d_std_scratch, p->d_std_part};
+                    p->d_power, p->d_fnum, p->d_qmwork, p->d_qxwork, p->d_qoff,
+                    p->d_heads};
+    drop_std(p, s);
     for (void* b : bufs) dev_free(p->alloc, b, s);
     for (auto& q : p->ev_pool)
         for (auto e : q)
@@ -1012,6 +1044,43 @@ int dxg2_correlate_dense_graph(dxg2_plan* p, const void* d_frames, int64_t* d_nu
     return DXG2_OK;
 }
 
+int dxg2_correlate_dense_ttcf(dxg2_plan* p, const void* d_frames, int32_t q, float* d_trc,
+                              void* stream) {
+    if (!p) return set_error(DXG2_EINVAL, "null plan");
+    if (!d_frames || !d_trc) return set_error(DXG2_EINVAL, "frames and trc must be non-null");
+    if (q < 1 || q > p->n_bins)
+        return set_error(DXG2_EINVAL, "bin %d is not in 1..%d", q, p->n_bins);
+    hipStream_t s = (hipStream_t)stream;
+    const int b = q - 1;
+    const int64_t N = p->N;
+    (void)hipGetLastError();
+    if (p->bins[b].npix == 0) {  // 0 / 0 everywhere
+        launch_fill_f32(d_trc, N * N, __builtin_nanf(""), s);
+        DXG2_HIP(hipGetLastError());
+        return DXG2_OK;
+    }
+    int rc = ensure_dense_buffers(p, s);
+    if (!rc) rc = ensure_ttcf_buffers(p, b, s);
+    if (rc) return rc;
+    int64_t* gsums = dense_gather(p, d_frames, nullptr, s);
+    launch_slab_sums(p->d_slab, p->dtype, p->d_bins, p->d_gwork, (int)p->gather_work.size(),
+                     p->n_bins, p->N, p->nrows, gsums, s);
+    const int biased = p->dtype == DXG2_I8 ? 0 : 1;
+    const int mode = p->dtype == DXG2_U16 ? 2 : (p->dtype == DXG2_U8 ? 1 : 0);
+    const TileWork& tw = p->ttw;
+    for (const auto& ch : tw.chunks) {
+        launch_gram_i8_store(p->gram_tile, false, p->d_slab, p->d_bins, tw.d_mwork + ch.m0,
+                             ch.m1 - ch.m0, p->N, p->nrows, biased, p->d_ugram, tw.d_scratch, s);
+        launch_gram_i8_store(p->gram_tile, true, p->d_slab, p->d_bins, tw.d_xwork + ch.x0,
+                             ch.x1 - ch.x0, p->N, p->nrows, biased, p->d_ugram, tw.d_scratch, s);
+        launch_ttcf_tiles(p->gram_tile, tw.d_tiles + ch.tile0, ch.tile1 - ch.tile0, p->d_bins,
+                          tw.d_scratch, gsums, mode, p->n_bins, p->N, d_trc, s);
+    }
+    launch_ttcf_diag(d_trc, p->N, gsums, mode, p->n_bins, b, s);
+    DXG2_HIP(hipGetLastError());
+    return DXG2_OK;
+}
+
 int dxg2_correlate_dense_std(dxg2_plan* p, const void* d_frames, int64_t* d_num,
                              int64_t* d_sums, double* d_den, float* d_g2, float* d_dev,
                              void* stream) {
@@ -1033,15 +1102,14 @@ int dxg2_correlate_dense_std(dxg2_plan* p, const void* d_frames, int64_t* d_num,
     if (ev) DXG2_HIP(hipEventRecord(ev[1], s));
     const int biased = p->dtype == DXG2_I8 ? 0 : 1;
     const int mode = p->dtype == DXG2_U16 ? 2 : (p->dtype == DXG2_U8 ? 1 : 0);
-    for (const auto& ch : p->std_chunks) {
-        launch_gram_i8_store(p->gram_tile, false, p->d_slab, p->d_bins, p->d_std_mwork + ch.m0,
-                             ch.m1 - ch.m0, p->N, p->nrows, biased, p->d_ugram,
-                             p->d_std_scratch, s);
-        launch_gram_i8_store(p->gram_tile, true, p->d_slab, p->d_bins, p->d_std_xwork + ch.x0,
-                             ch.x1 - ch.x0, p->N, p->nrows, biased, p->d_ugram,
-                             p->d_std_scratch, s);
-        launch_std_tiles(p->gram_tile, p->d_std_tiles + ch.tile0, ch.tile1 - ch.tile0, p->d_bins,
-                         p->d_std_scratch, gsums, mode, p->n_bins, p->N, p->d_std_part, s);
+    const TileWork& tw = p->stdw;
+    for (const auto& ch : tw.chunks) {
+        launch_gram_i8_store(p->gram_tile, false, p->d_slab, p->d_bins, tw.d_mwork + ch.m0,
+                             ch.m1 - ch.m0, p->N, p->nrows, biased, p->d_ugram, tw.d_scratch, s);
+        launch_gram_i8_store(p->gram_tile, true, p->d_slab, p->d_bins, tw.d_xwork + ch.x0,
+                             ch.x1 - ch.x0, p->N, p->nrows, biased, p->d_ugram, tw.d_scratch, s);
+        launch_std_tiles(p->gram_tile, tw.d_tiles + ch.tile0, ch.tile1 - ch.tile0, p->d_bins,
+                         tw.d_scratch, gsums, mode, p->n_bins, p->N, p->d_std_part, s);
     }
     if (ev) DXG2_HIP(hipEventRecord(ev[2], s));
     dense_finish(p, gsums, d_num, d_sums, d_den, d_g2, s);

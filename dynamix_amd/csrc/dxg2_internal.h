@@ -186,6 +186,13 @@ void launch_std_tiles(int tile, const StdTile* tiles, int n_tiles, const BinDev*
 void launch_dev_finalize(const StdPart* part, const BinDev* bins, int n_bins, int nframes,
                          int tile, float* dev, hipStream_t s);
 
+// Two-time correlation matrix of one bin from stored Gram tiles, then its diagonal.
+void launch_ttcf_tiles(int tile, const StdTile* tiles, int n_tiles, const BinDev* bins,
+                       const int* scratch, const int64_t* sums, int mode, int n_bins,
+                       int nframes, float* trc, hipStream_t s);
+void launch_ttcf_diag(float* trc, int nframes, const int64_t* sums, int mode, int n_bins, int b,
+                      hipStream_t s);
+void launch_fill_f32(float* p, int64_t n, float v, hipStream_t s);
 // Frames -> per-pixel event lists (FramesCompressor layout).  scratch holds
 // compress_scratch_bytes(); the count pass leaves offsets (hw + 1 entries) and the
 // per-chunk bases in it for the fill pass.

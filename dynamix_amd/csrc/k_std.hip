@@ -221,6 +221,142 @@ void launch_std_tiles(int tile, const StdTile* tiles, int n_tiles, const BinDev*
             tiles, bins, scratch, sums, mode, n_bins, nframes, part);
 }
 
+// Two-time correlation of one bin (event_y.py:137-157): trc(t, u) = n_b G(t, u) /
+// (S(t) S(u)) for u > t, mirrored to (u, t); NaN where S(t) S(u) = 0.  One workgroup per
+// tile, a band of BR rows at a time: the row-major upper entries are written straight
+// from the lanes (consecutive u), the mirrored ones from an LDS transpose, 32 lanes per
+// output row (128-byte runs).
+template <int T>
+__global__ __launch_bounds__(256) void ttcf_tile_kernel(const StdTile* __restrict__ tiles,
+                                                        const BinDev* __restrict__ bins,
+                                                        const int* __restrict__ scratch,
+                                                        const int64_t* __restrict__ sums,
+                                                        int mode, int n_bins, int nframes,
+                                                        float* __restrict__ trc) {
+    constexpr int BR = 32;
+    constexpr int RPP = 256 / T;
+    __shared__ float x[BR][T + 1];
+
+    const StdTile tl = tiles[blockIdx.x];
+    const BinDev bd = bins[tl.bin];
+    const int tid = threadIdx.x;
+    const int c = tid % T;
+    const int rsub = tid / T;
+    const int t0 = tl.ti * T;
+    const int u0 = tl.tj * T;
+    const int u = u0 + c;
+    const int64_t K = (int64_t)KB * bd.nkb;
+    const int64_t Kc = mode == 0 ? 0 : (mode == 1 ? 16384 * K : 16384 * K * 66049);
+    const int w_hh = mode == 2 ? 65536 : 1;
+    const double npix = (double)bd.npix;
+    const int64_t N = nframes;
+    int64_t S_u = 0, c_u = 0;
+    if (u < nframes) frame_terms(sums, mode, n_bins, nframes, tl.bin, u, S_u, c_u);
+    const int64_t TT = (int64_t)T * T;
+    const int* base = scratch + (int64_t)tl.slot0 * TT;
+
+    for (int r0 = 0; r0 < T && t0 + r0 < nframes; r0 += BR) {
+        __syncthreads();
+        for (int rr = rsub; rr < BR; rr += RPP) {
+            const int r = r0 + rr;
+            const int t = t0 + r;
+            float val = 0.0f;
+            if (t < nframes && u < nframes && u > t) {
+                int64_t S_t, c_t;
+                frame_terms(sums, mode, n_bins, nframes, tl.bin, t, S_t, c_t);
+                if (S_t != 0 && S_u != 0) {
+                    const int* e = base + (int64_t)r * T + c;
+                    int64_t g = c_t + c_u - Kc;
+                    int k = 0;
+                    for (; k < tl.n_hh; ++k) g += (int64_t)w_hh * e[k * TT];
+                    for (; k < tl.n_hh + tl.n_ll; ++k) g += e[k * TT];
+                    for (; k < tl.n_hh + tl.n_ll + tl.n_x; ++k) g += (int64_t)256 * e[k * TT];
+                    val = (float)(npix * (double)g / ((double)S_t * (double)S_u));
+                } else {
+                    val = __builtin_nanf("");
+                }
+                trc[(int64_t)t * N + u] = val;
+            }
+            x[rr][c] = val;
+        }
+        __syncthreads();
+        for (int k = tid; k < T * BR; k += 256) {
+            const int cu = k / BR, rr = k % BR;
+            const int uu = u0 + cu, t = t0 + r0 + rr;
+            if (t < nframes && uu < nframes && uu > t) trc[(int64_t)uu * N + t] = x[rr][cu];
+        }
+    }
+}
+
+// The diagonal of the two-time matrix: entries 0 .. N-2 take the mean of the positive
+// entries of the first off-diagonal (event_y.py:153-156); entry N-1 keeps 2 * 0 /
+// S(N-1)^2, i.e. 0, or NaN for an empty last frame.  One workgroup.
+__global__ __launch_bounds__(1024) void ttcf_diag_kernel(float* __restrict__ trc, int nframes,
+                                                          const int64_t* __restrict__ sums,
+                                                          int mode, int n_bins, int b) {
+    __shared__ double ssum[1024];
+    __shared__ int scnt[1024];
+    const int tid = threadIdx.x;
+    const int64_t N = nframes;
+    double s = 0.0;
+    int n = 0;
+    for (int j = tid; j < nframes - 1; j += 1024) {
+        const float v = trc[(int64_t)j * N + j + 1];
+        if (v > 0.0f) {
+            s += (double)v;
+            ++n;
+        }
+    }
+    ssum[tid] = s;
+    scnt[tid] = n;
+    __syncthreads();
+    for (int off = 512; off > 0; off >>= 1) {
+        if (tid < off) {
+            ssum[tid] += ssum[tid + off];
+            scnt[tid] += scnt[tid + off];
+        }
+        __syncthreads();
+    }
+    const float fill = scnt[0] > 0 ? (float)(ssum[0] / scnt[0]) : __builtin_nanf("");
+    for (int j = tid; j < nframes - 1; j += 1024) trc[(int64_t)j * N + j] = fill;
+    if (tid == 0) {
+        int64_t S, cb;
+        frame_terms(sums, mode, n_bins, nframes, b, nframes - 1, S, cb);
+        trc[(N - 1) * N + N - 1] = S != 0 ? 0.0f : __builtin_nanf("");
+    }
+}
+
+__global__ void fill_f32_kernel(float* __restrict__ p, int64_t n, float v) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+void launch_ttcf_tiles(int tile, const StdTile* tiles, int n_tiles, const BinDev* bins,
+                       const int* scratch, const int64_t* sums, int mode, int n_bins,
+                       int nframes, float* trc, hipStream_t s) {
+    if (n_tiles <= 0) return;
+    if (tile == 128)
+        ttcf_tile_kernel<128><<<dim3((unsigned)n_tiles), dim3(256), 0, s>>>(
+            tiles, bins, scratch, sums, mode, n_bins, nframes, trc);
+    else
+        ttcf_tile_kernel<256><<<dim3((unsigned)n_tiles), dim3(256), 0, s>>>(
+            tiles, bins, scratch, sums, mode, n_bins, nframes, trc);
+}
+
+void launch_ttcf_diag(float* trc, int nframes, const int64_t* sums, int mode, int n_bins, int b,
+                      hipStream_t s) {
+    if (nframes <= 0) return;
+    ttcf_diag_kernel<<<dim3(1), dim3(1024), 0, s>>>(trc, nframes, sums, mode, n_bins, b);
+}
+
+void launch_fill_f32(float* p, int64_t n, float v, hipStream_t s) {
+    if (n <= 0) return;
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    fill_f32_kernel<<<dim3((unsigned)blocks), dim3(256), 0, s>>>(p, n, v);
+}
+
 // dev(tau) of every bin: the segments of diagonal tau lie in tiles (ti, ti + D) for
 // D = tau / T (slot tau - D T + T - 1) and D + 1 (slot tau - D T - 1, if >= 0).
 __global__ __launch_bounds__(256) void dev_finalize_kernel(const StdPart* __restrict__ part,

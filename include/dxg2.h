@@ -123,6 +123,17 @@ int dxg2_correlate_dense_std(dxg2_plan* plan, const void* d_frames, int64_t* d_n
                              int64_t* d_sums, double* d_den, float* d_g2, float* d_dev,
                              void* stream);
 
+/* Two-time correlation matrix of bin q (1-based), d_trc: (N, N) float32 row-major,
+ * trc(t, u) = n_b G(t, u) / (S(t) S(u)) for t != u, where G counts only pairs of samples
+ * from two different frames; the diagonal 0 .. N-2 holds the mean of the positive
+ * entries of the first off-diagonal and entry (N-1, N-1) is 0 (NaN where a frame sum is
+ * zero: 0/0, as in numpy).  Computed in FP64 from the exact integer Gram tiles of that
+ * bin (the tiles are stored in a scratch of at most DXG2_STD_SCRATCH_MB MiB).
+ * Replaces the ttcf_par output `trc` of the event correlators (correlator/event_y.py:
+ * 137-157, nbecorrts_q). */
+int dxg2_correlate_dense_ttcf(dxg2_plan* plan, const void* d_frames, int32_t q, float* d_trc,
+                              void* stream);
+
 /* The dense chain of dxg2_correlate_dense (gather, persistent Gram, finalise) captured
  * once into a HIP graph and replayed with one hipGraphLaunch on `stream`.  The graph is
  * kept in the plan for the given buffer pointers: a call with other pointers captures

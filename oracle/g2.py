@@ -26,6 +26,10 @@ Parity pinning of each function (tests/test_oracle.py):
                                  res and (calc_std) dev.
   dense_dev_fp64              -- the golden dev vectors within the reference's float32
                                  rounding; brute-force loops on a tiny stack.
+  two_time_fp64               -- a loop restatement of the reference's event pair loop
+                                 and TTCF normalisation (event_y.py:20-41,137-157) on
+                                 tiny stacks; closed form for time-constant frames;
+                                 symmetry; diagonal rule.
   fft_correlator              -- against `numerators`/`denominators` within FP32
                                  tolerance; FFT length against the reference's
                                  get_next_power golden vectors (utils.py:50-60).
@@ -270,6 +274,69 @@ def dense_dev_fp64(frames, qmask=None):
 
 
 # ------------------------------------------------------------------------ FFT path
+def two_time_fp64(frames, qmask, q):
+    """Two-time correlation matrix of bin q (1-based), as the reference builds `trc` for
+    ttcf_par = q (event_y.py:137-157), in FP64 from exact integers:
+
+      cc(t, u)  = sum_p I(t, p) I(u, p) for u > t (pairs of events within a pixel, the
+                  event loop of event_y.py:20-41; nothing on or below the diagonal);
+      trc(t, u) = n_b cc(t, u) / (S(t) S(u))       (cor / norm, event_y.py:139-140,149)
+    then trc += trc^T (event_y.py:150-152: the lower triangle is empty), and the
+    diagonal entries 0 .. N-2 become the mean of the positive entries of the first
+    off-diagonal (event_y.py:153-156); trc[N-1, N-1] keeps 2 * 0 / S(N-1)^2.  A zero
+    S(t) S(u) gives NaN (0/0), as numpy's float division does.
+    """
+    qq, bins = qmask_bins(frames.shape[1:], qmask)
+    a = gather_bin(frames, qq, q).astype(np.float64)
+    n = frames.shape[0]
+    npx = a.shape[1]
+    g = a @ a.T                      # exact: integer entries well below 2^53 in tests
+    s = a.sum(axis=1)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        trc = npx * np.triu(g, 1) / np.outer(s, s)
+        trc = trc + trc.T
+    if n > 1:
+        d1 = np.diagonal(trc, 1)
+        pos = d1[d1 > 0]
+        fill = pos.mean() if pos.size else np.nan
+        idx = np.arange(n - 1)
+        trc[idx, idx] = fill
+    return trc
+
+
+def two_time_loops(frames, qmask, q):
+    """The same matrix by the reference's own loop structure (event_y.py:20-41,137-157):
+    per pixel, every pair of nonzero samples (t < u) adds I(t) I(u); then the float
+    normalisation, symmetrisation and diagonal rule element by element.  Small inputs."""
+    qq, _ = qmask_bins(frames.shape[1:], qmask)
+    n = frames.shape[0]
+    pix = [p for p in range(qq.size) if qq.flat[p] == q]
+    cc = [[0] * n for _ in range(n)]
+    mint = [0] * n
+    for p in pix:
+        ev = [(t, int(frames.reshape(n, -1)[t, p])) for t in range(n)
+              if frames.reshape(n, -1)[t, p] != 0]
+        for j, (t0, v0) in enumerate(ev):
+            mint[t0] += v0
+            for t1, v1 in ev[j + 1:]:
+                cc[t0][t1] += v0 * v1
+    trc = [[0.0] * n for _ in range(n)]
+    for t in range(n):
+        for u in range(n):
+            den = mint[t] * mint[u]
+            cor = len(pix) * cc[t][u] if u > t else 0
+            trc[t][u] = cor / den if den else float("nan")
+    for t in range(n):
+        for u in range(t + 1):
+            trc[t][u] = trc[t][u] + trc[u][t] if u < t else 2 * trc[t][t]
+            trc[u][t] = trc[t][u]
+    pos = [trc[j][j + 1] for j in range(n - 1) if trc[j][j + 1] > 0]
+    fill = sum(pos) / len(pos) if pos else float("nan")
+    for j in range(n - 1):
+        trc[j][j] = fill
+    return np.array(trc)
+
+
 def fft_powers():
     """All 2^a 3^b 5^c 7^d with a in [2,15], b<=9, c<=6, d<=5 (utils.py:29-47)."""
     vals = set()

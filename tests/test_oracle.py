@@ -70,6 +70,46 @@ def test_dev_brute_force_and_closed_forms():
     assert np.abs(og.dense_dev_fp64(fc, qq)).max() < 1e-15
 
 
+@pytest.mark.parametrize("seed", range(3))
+def test_two_time_vs_reference_loops(seed):
+    rng = np.random.default_rng(seed)
+    f = (rng.random((9, 4, 5)) < 0.5) * rng.integers(1, 6, size=(9, 4, 5))
+    f = f.astype(np.uint8)
+    q = rng.integers(0, 3, size=(4, 5)).astype(np.int32)
+    q.flat[0], q.flat[1] = 1, 2
+    for b in (1, 2):
+        a = og.two_time_fp64(f, q, b)
+        r = og.two_time_loops(f, q, b)
+        assert np.array_equal(np.isnan(a), np.isnan(r))
+        m = ~np.isnan(r)
+        np.testing.assert_allclose(a[m], r[m], rtol=1e-13)
+
+
+def test_two_time_closed_forms():
+    rng = np.random.default_rng(4)
+    c = rng.integers(1, 9, size=(5, 6)).astype(np.uint8)
+    f = np.broadcast_to(c, (12, 5, 6)).copy()
+    q = np.ones((5, 6), np.int32)
+    trc = og.two_time_fp64(f, q, 1)
+    cb = c.astype(np.float64).ravel()
+    v = cb.size * (cb ** 2).sum() / cb.sum() ** 2
+    off = ~np.eye(12, dtype=bool)
+    np.testing.assert_allclose(trc[off], v, rtol=1e-14)
+    np.testing.assert_allclose(np.diagonal(trc)[:-1], v, rtol=1e-14)
+    assert trc[-1, -1] == 0.0
+    # a last frame with no counts: its row, column and corner are NaN
+    f0 = f.copy()
+    f0[-1] = 0
+    t0 = og.two_time_fp64(f0, q, 1)
+    assert np.isnan(t0[-1]).all() and np.isnan(t0[:, -1]).all()
+    assert np.isfinite(t0[:-1, :-1]).all()
+    g = rng.integers(0, 4, size=(15, 5, 6)).astype(np.uint8)
+    t2 = og.two_time_fp64(g, q, 1)
+    assert np.array_equal(np.isnan(t2), np.isnan(t2.T))
+    m = ~np.isnan(t2)
+    np.testing.assert_array_equal(t2[m], t2.T[m])
+
+
 @pytest.mark.parametrize("name", DENSE_CASES)
 def test_exact_g2_matches_reference_within_f32(golden, name):
     """The exact FP64 g2 and the reference's float32 g2 agree to float32 rounding."""
