@@ -16,6 +16,8 @@ Printed as one JSON line.  Besides the metric it reports
                 (the persistent launch over per-XCD work queues, with the fill kernel
                 that clears the queue heads);
   calc_std      the dense path with calc_std=True (dev(tau)) on the same stack;
+  compress      the on-device frames -> events compaction of the same stack;
+  ttcf          the two-time matrix of one bin of the same stack;
   cpu_baseline  the reference's numpy MatMul correlator (float32 sgemm + diagonal
                 loop, restated in oracle/) on one bin of the same stack, all host cores;
   plain_fp32    this library's one-bin-per-launch FP32 comparator on the same stack;
@@ -162,6 +164,47 @@ def bench_std(plan, frames, nb, n, pf, base_ms, reps=2):
     return out
 
 
+def bench_compress(frames, reps=2):
+    """On-device frames -> events compaction (FramesCompressor) of the headline stack."""
+    import torch
+    from dynamix_amd.correlator.event import FramesCompressor
+    FramesCompressor.compress_all_stack(frames)  # warm
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        d, t, o = FramesCompressor.compress_all_stack(frames)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / reps
+    nnz = int(d.shape[0])
+    read = 2.0 * frames.numel() * frames.element_size()      # count and fill passes
+    written = nnz * (4 + frames.element_size()) + (o.numel()) * 4
+    return {"ms": ms, "events": nnz, "bytes_read": read, "bytes_written": written,
+            "GBps": (read + written) / (ms * 1e-3) / 1e9,
+            "frac_of_hbm": (read + written) / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "note": "includes the device-to-host read of the event count between the passes"}
+
+
+def bench_ttcf(plan, frames, n, q=10, reps=2):
+    """Two-time matrix (N x N float32) of one bin of the headline stack."""
+    import torch
+    from dynamix_amd import _lib
+    trc = torch.empty((n, n), dtype=torch.float32, device="cuda")
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+    def step():
+        _lib.check(plan.lib.dxg2_correlate_dense_ttcf(plan.handle, _lib.ptr(frames), q,
+                                                      _lib.ptr(trc), st))
+    step()
+    torch.cuda.synchronize()
+    ms = _timed(step, reps, torch)
+    finite = torch.isfinite(trc).float().mean().item()
+    out = {"ms": ms, "bin": q, "pixels": int(plan.pixel_counts[q - 1]),
+           "matrix_bytes": n * n * 4, "finite_fraction": finite}
+    del trc
+    torch.cuda.empty_cache()
+    return out
+
+
 def bench_events(seed):
     """Event path on the events workload: pixel-major and frame-sorted COO input."""
     import numpy as np
@@ -300,6 +343,8 @@ def main():
     ap.add_argument("--no-events", action="store_true")
     ap.add_argument("--no-uint16", action="store_true")
     ap.add_argument("--no-std", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the compaction and two-time sections")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default Gram tuning instead of timing the candidates")
     ap.add_argument("--no-serving", action="store_true")
@@ -423,6 +468,11 @@ def main():
     if not args.no_std:
         log("calc_std path")
         out["calc_std"] = bench_std(plan, frames, nb, n, pf, ms)
+    if not args.no_extras:
+        log("compaction path")
+        out["compress"] = bench_compress(frames)
+        log("two-time path")
+        out["ttcf"] = bench_ttcf(plan, frames, n)
     if not args.no_cpu:
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(frames, q)

@@ -185,6 +185,15 @@ struct dxg2_plan {
     std::vector<int32_t> pix_idx;
     std::vector<BinDev> bins;
     std::vector<GatherWork> gather_work;
+    // band gather (uint8/int8): bands of detector rows, their (bin, k-block) list, the
+    // k-blocks left to gather_i8_kernel, and the LDS bytes staged per frame
+    std::vector<BandWork> gband;
+    std::vector<int2> gband_kb;
+    std::vector<GatherWork> gfall;
+    int gband_srow = 0;
+    BandWork* d_gband = nullptr;
+    int2* d_gband_kb = nullptr;
+    GatherWork* d_gfall = nullptr;
     std::vector<GramWork> gram_work;    // plain kernel: every item of uint8/int8 plans;
                                         // HH and LL items of uint16 plans
     std::vector<GramWork> gram_xwork;   // cross kernel: [H|L][L|H]^T items of uint16 plans
@@ -261,6 +270,57 @@ namespace {
 int env_int(const char* name, int dflt) {
     const char* v = getenv(name);
     return (v && *v) ? atoi(v) : dflt;
+}
+
+// Band gather lists (uint8/int8 frames): rows per band RB such that 4 frames x (RB + 2)
+// rows fit in 64 KiB of LDS; a k-block goes to the band of its first row if its last
+// row is within the band's 2 rows of overlap, else to the fallback list.
+void build_band_work(dxg2_plan* p, const std::vector<std::vector<int32_t>>& lists) {
+    p->gband.clear();
+    p->gband_kb.clear();
+    p->gfall.clear();
+    p->gband_srow = 0;
+    const int W = p->W, H = p->H;
+    const int RB = 16352 / W - 2;
+    if (p->dtype == DXG2_U16 || RB < 1 || env_int("DXG2_GATHER_BAND", 1) == 0) return;
+    const int nbands = (H + RB - 1) / RB;
+    std::vector<std::vector<int2>> per(nbands);
+    for (int b = 0; b < p->n_bins; ++b) {
+        const auto& L = lists[b];
+        const int n = (int)L.size();
+        int fall0 = -1;
+        auto flush = [&](int kb_end) {
+            for (int k = fall0; k < kb_end; k += 16)
+                p->gfall.push_back(GatherWork{b, k, std::min(16, kb_end - k), 0});
+            fall0 = -1;
+        };
+        const int nkb = (n + KB - 1) / KB;
+        for (int kb = 0; kb < nkb; ++kb) {
+            const int ymin = L[kb * KB] / W;
+            const int ymax = L[std::min(n, (kb + 1) * KB) - 1] / W;
+            const int band = ymin / RB;
+            if (ymax - band * RB <= RB + 1) {
+                if (fall0 >= 0) flush(kb);
+                per[band].push_back(make_int2(b, kb));
+            } else if (fall0 < 0) {
+                fall0 = kb;
+            }
+        }
+        if (fall0 >= 0) flush(nkb);
+    }
+    // one work item per band and run of at most 4 x 32 k-blocks (the band kernel keeps
+    // 32 k-blocks per wave in registers)
+    const int kmax = 4 * 32;
+    for (int band = 0; band < nbands; ++band) {
+        const int y0 = band * RB;
+        for (size_t k = 0; k < per[band].size(); k += kmax) {
+            const int k0 = (int)p->gband_kb.size();
+            const size_t k1 = std::min(per[band].size(), k + kmax);
+            p->gband_kb.insert(p->gband_kb.end(), per[band].begin() + k, per[band].begin() + k1);
+            p->gband.push_back(BandWork{y0, std::min(RB + 2, H - y0), k0, (int)p->gband_kb.size()});
+        }
+    }
+    p->gband_srow = ((RB + 2) * W + 16 + 15) / 16 * 16;
 }
 
 // Tile order of the grouped Gram launch.  Within a bin (largest bins first) the
@@ -577,8 +637,17 @@ int64_t* dense_gather(dxg2_plan* p, const void* d_frames, int64_t* d_sums, hipSt
     const int64_t nbn = (int64_t)p->n_bins * p->N;
     launch_fill_i64(gsums, (wide ? 2 : 1) * nbn, s);
     launch_fill_i64(p->d_ugram, (wide ? 3 : 1) * nbn, s);
-    launch_gather_i8((const uint8_t*)d_frames, p->dtype, (int64_t)p->H * p->W, p->N, p->nrows,
-                     p->d_pix, p->d_bins, p->d_gwork, (int)p->gather_work.size(), p->d_slab, s);
+    const int64_t hw = (int64_t)p->H * p->W;
+    if (!p->gband.empty() && reinterpret_cast<uintptr_t>(d_frames) % 16 == 0) {
+        launch_gather_band((const uint8_t*)d_frames, p->dtype, hw, p->W, p->N, p->nrows,
+                           p->gband_srow, p->d_pix, p->d_bins, p->d_gband, (int)p->gband.size(),
+                           p->d_gband_kb, p->d_slab, s);
+        launch_gather_i8((const uint8_t*)d_frames, p->dtype, hw, p->N, p->nrows, p->d_pix,
+                         p->d_bins, p->d_gfall, (int)p->gfall.size(), p->d_slab, s);
+    } else {
+        launch_gather_i8((const uint8_t*)d_frames, p->dtype, hw, p->N, p->nrows, p->d_pix,
+                         p->d_bins, p->d_gwork, (int)p->gather_work.size(), p->d_slab, s);
+    }
     return gsums;
 }
 
@@ -715,6 +784,7 @@ int dxg2_plan_create(const dxg2_plan_desc* desc, dxg2_plan** out) {
         slab_off += (int64_t)slab_planes(p->dtype) * nkb * KB * p->nrows;
     }
     p->slab_bytes = (size_t)slab_off;
+    build_band_work(p, lists);
     build_gram_work(p);
     build_event_work(p);
 
@@ -734,7 +804,9 @@ int dxg2_plan_create(const dxg2_plan_desc* desc, dxg2_plan** out) {
     }
     hipStream_t s0 = sc.stream;
     const size_t staged = p->pix_idx.size() * 4 + p->bins.size() * sizeof(BinDev) +
-                          (p->gather_work.size() + 1) * sizeof(GatherWork) +
+                          (p->gather_work.size() + p->gfall.size() + 2) * sizeof(GatherWork) +
+                          (p->gband.size() + 1) * sizeof(BandWork) +
+                          (p->gband_kb.size() + 1) * sizeof(int2) +
                           (p->gram_work.size() + p->gram_xwork.size() + 2) * sizeof(GramWork) +
                           (p->qm_work.size() + p->qx_work.size() + 2) * sizeof(GramWork) +
                           sizeof(p->q_off) + 9 * 256;
@@ -770,6 +842,10 @@ int dxg2_plan_create(const dxg2_plan_desc* desc, dxg2_plan** out) {
     if (!rc)
         rc = upload((void**)&p->d_gwork, p->gather_work.data(),
                     p->gather_work.size() * sizeof(GatherWork));
+    if (!rc) rc = upload((void**)&p->d_gband, p->gband.data(), p->gband.size() * sizeof(BandWork));
+    if (!rc)
+        rc = upload((void**)&p->d_gband_kb, p->gband_kb.data(), p->gband_kb.size() * sizeof(int2));
+    if (!rc) rc = upload((void**)&p->d_gfall, p->gfall.data(), p->gfall.size() * sizeof(GatherWork));
     if (!rc)
         rc = upload((void**)&p->d_mwork, p->gram_work.data(),
                     p->gram_work.size() * sizeof(GramWork));
@@ -835,7 +911,7 @@ static void plan_release(dxg2_plan* p, hipStream_t s) {
                     p->d_ework, p->d_err,  p->d_cnt,   p->d_scan,  p->d_off,
                     p->d_etimes, p->d_edata, p->d_kbwork, p->d_xs, p->d_tw,
                     p->d_power, p->d_fnum, p->d_qmwork, p->d_qxwork, p->d_qoff,
-                    p->d_heads};
+                    p->d_heads, p->d_gband, p->d_gband_kb, p->d_gfall};
     drop_std(p, s);
     for (void* b : bufs) dev_free(p->alloc, b, s);
     for (auto& q : p->ev_pool)

@@ -45,6 +45,13 @@ struct GatherWork {      // one workgroup of the gather kernel (x dimension)
     int32_t pad;
 };
 
+struct BandWork {        // one band of detector rows of the band gather (x dimension)
+    int32_t y0;          // first staged row
+    int32_t rows;        // rows staged (the band plus its overlap, clipped at H)
+    int32_t k0;          // k-blocks [k0, k1) of the (bin, k-block) list
+    int32_t k1;
+};
+
 struct GramWork {        // one output tile (i, j >= i) over a k-block range
     int32_t bin;
     int32_t ti;          // row tile
@@ -108,6 +115,12 @@ void launch_fill_i64(int64_t* p, int64_t n, hipStream_t s);
 void launch_gather_i8(const uint8_t* frames, int dtype, int64_t hw, int nframes, int nrows,
                       const int32_t* pix_idx, const BinDev* bins, const GatherWork* work,
                       int n_work, int8_t* slab, hipStream_t s);
+// Band gather of uint8/int8 frames: stage `rows` detector rows of 4 frames in LDS
+// (srow bytes per frame), write every k-block of the band from there.
+void launch_gather_band(const uint8_t* frames, int dtype, int64_t hw, int width, int nframes,
+                        int nrows, int srow, const int32_t* pix_idx, const BinDev* bins,
+                        const BandWork* bands, int n_bands, const int2* band_kb, int8_t* slab,
+                        hipStream_t s);
 // cross: the work items are cross-plane ([H|L] [L|H]^T) items of uint16 frames.
 void launch_gram_i8(int tile, bool cross, const int8_t* slab, const BinDev* bins,
                     const GramWork* work, int n_work, int nframes, int nrows, int biased,

@@ -11,9 +11,14 @@
 //     of pixels 4q..4q+3 for frame t+i, so one store instruction writes rows t..t+3
 //     of the k-block: four 64-byte runs of the fragment-contiguous slab layout.
 // The per-frame bin sums are formed later by the Gram kernel's diagonal tiles.
+//
+// For uint8/int8 frames the band gather below does most k-blocks: it reads the frames
+// with 16-byte loads through LDS instead of one byte per lane and load.
 #include "dxg2_internal.h"
 
 namespace dxg2 {
+
+typedef __attribute__((address_space(3))) void lds_void;
 
 __global__ void fill_i64_kernel(int64_t* __restrict__ p, int64_t n) {
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -112,6 +117,126 @@ __global__ __launch_bounds__(256) void gather_i8_kernel(
             }
         }
     }
+}
+
+// Band gather (uint8 / int8 frames).  A workgroup takes one band of detector rows and
+// up to BAND_KMAX k-blocks whose pixels lie in those rows, for BAND_FRAMES frames.  Each
+// wave loads its k-blocks' pixel quads (relative to the band) and slab addresses once,
+// into registers; then, 4 frames at a time, the workgroup stages the band's rows of the
+// 4 frames in LDS -- one contiguous byte range per frame, read with 16-byte loads from
+// its 16-byte aligned start -- and lane 4 q + i assembles list entries 4q .. 4q+3 of
+// frame t + i from LDS and stores them as one dword (4 x 64-byte runs per store, as in
+// gather_i8_kernel).  A slab address splits into a k-block part and a frame part
+// (slab_offset(kb, t, kk) = slab_offset(kb, 0, kk) + (t/32) 2048 + (t%32) 16).  The
+// work list gives each k-block to the band of its first row, with 2 rows of overlap
+// staged below the band, so k-blocks spanning up to 3 rows are covered; the others go
+// to gather_i8_kernel.
+constexpr int BAND_KMAX = 32;       // k-blocks per wave
+constexpr int BAND_FRAMES = 64;     // frames per workgroup
+
+template <bool BIAS>
+__global__ __launch_bounds__(256) void gather_band_kernel(
+    const uint8_t* __restrict__ frames, int64_t hw, int width, int nframes, int nrows, int srow,
+    const int32_t* __restrict__ pix_idx, const BinDev* __restrict__ bins,
+    const BandWork* __restrict__ bands, const int2* __restrict__ band_kb,
+    int8_t* __restrict__ slab) {
+    extern __shared__ __attribute__((aligned(16))) uint8_t stage[];  // 4 x srow bytes
+    __shared__ int shift[4];
+    const BandWork bw = bands[blockIdx.x];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int qq = lane >> 2;  // quad of list entries 4qq .. 4qq+3
+    const int i = lane & 3;    // frame t + i
+    const int64_t band0 = (int64_t)bw.y0 * width;
+    const int nk = (bw.k1 - bw.k0 - wave + 3) / 4;  // this wave's k-blocks: k0 + wave + 4j
+
+    int4 pq[BAND_KMAX];
+    int64_t sbase[BAND_KMAX];
+#pragma unroll
+    for (int j = 0; j < BAND_KMAX; ++j) {
+        if (j < nk) {
+            const int2 bk = band_kb[bw.k0 + wave + 4 * j];
+            const BinDev bd = bins[bk.x];
+            const int4 p = *reinterpret_cast<const int4*>(pix_idx + bd.idx_off +
+                                                          (int64_t)bk.y * KB + 4 * qq);
+            pq[j] = make_int4(p.x >= 0 ? (int)(p.x - band0) : -1, p.y >= 0 ? (int)(p.y - band0) : -1,
+                              p.z >= 0 ? (int)(p.z - band0) : -1, p.w >= 0 ? (int)(p.w - band0) : -1);
+            sbase[j] = bd.slab_off + slab_offset(bk.y, 0, 4 * qq, nrows);
+        }
+    }
+
+    const int64_t total = (int64_t)nframes * hw;
+    const int64_t span = (int64_t)bw.rows * width;
+    const int tb = blockIdx.y * BAND_FRAMES;
+    const int te = min(nframes, tb + BAND_FRAMES);
+    for (int t0 = tb; t0 < te; t0 += 4) {
+        __syncthreads();  // the previous frames' stage has been read
+        // DMA the 16-byte chunks of each frame's rows straight into LDS: the chunks of a
+        // 64-lane group land contiguously (lane l at base + 16 l), as the stage is laid
+        // out.  A chunk that would run past the end of the frame buffer is left to the
+        // byte loads below.
+        int64_t tail_a = -1;
+        int tail_f = 0, tail_c = 0;
+        for (int f = 0; f < 4 && t0 + f < nframes; ++f) {
+            const int64_t a0 = (int64_t)(t0 + f) * hw + band0;
+            const int64_t A = a0 & ~(int64_t)15;
+            if (threadIdx.x == 0) shift[f] = (int)(a0 - A);
+            const int nchunk = (int)((a0 + span - A + 15) >> 4);
+            for (int g = wave; g * 64 < nchunk; g += 4) {
+                const int c = g * 64 + lane;
+                const int64_t a = A + 16 * (int64_t)c;
+                if (c < nchunk && a + 16 <= total)
+                    __builtin_amdgcn_global_load_lds(
+                        (const void*)(frames + a),
+                        (lds_void*)(stage + f * srow + 16 * g * 64), 16, 0, 0);
+                else if (c < nchunk) {
+                    tail_a = a;
+                    tail_f = f;
+                    tail_c = c;
+                }
+            }
+        }
+        __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): every DMA of this thread landed
+        if (tail_a >= 0) {  // bytes inside the buffer only
+            uint8_t* d = stage + tail_f * srow + 16 * tail_c;
+            for (int e = 0; e < 16; ++e) d[e] = tail_a + e < total ? frames[tail_a + e] : 0;
+        }
+        __syncthreads();
+        const int t = t0 + i;
+        if (t < nframes) {
+            const uint8_t* st = stage + i * srow + shift[i];
+            const int64_t fo = (int64_t)(t >> 5) * 2048 + (t & 31) * 16;
+#pragma unroll
+            for (int j = 0; j < BAND_KMAX; ++j) {
+                if (j < nk) {
+                    const int4 p = pq[j];
+                    uint32_t v = (p.x >= 0 ? (uint32_t)st[p.x] : 0u) |
+                                 ((p.y >= 0 ? (uint32_t)st[p.y] : 0u) << 8) |
+                                 ((p.z >= 0 ? (uint32_t)st[p.z] : 0u) << 16) |
+                                 ((p.w >= 0 ? (uint32_t)st[p.w] : 0u) << 24);
+                    if (BIAS) v ^= 0x80808080u;
+                    *reinterpret_cast<uint32_t*>(slab + sbase[j] + fo) = v;
+                }
+            }
+        }
+    }
+}
+
+void launch_gather_band(const uint8_t* frames, int dtype, int64_t hw, int width, int nframes,
+                        int nrows, int srow, const int32_t* pix_idx, const BinDev* bins,
+                        const BandWork* bands, int n_bands, const int2* band_kb, int8_t* slab,
+                        hipStream_t s) {
+    if (n_bands <= 0 || nframes <= 0) return;
+    const dim3 grid((unsigned)n_bands, (unsigned)((nframes + BAND_FRAMES - 1) / BAND_FRAMES));
+    const size_t lds = (size_t)4 * srow;
+    if (dtype == DXG2_U8)
+        gather_band_kernel<true><<<grid, dim3(256), lds, s>>>(frames, hw, width, nframes, nrows,
+                                                              srow, pix_idx, bins, bands, band_kb,
+                                                              slab);
+    else
+        gather_band_kernel<false><<<grid, dim3(256), lds, s>>>(frames, hw, width, nframes, nrows,
+                                                               srow, pix_idx, bins, bands,
+                                                               band_kb, slab);
 }
 
 void launch_gather_i8(const uint8_t* frames, int dtype, int64_t hw, int nframes, int nrows,
