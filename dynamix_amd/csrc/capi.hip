@@ -281,8 +281,14 @@ void build_band_work(dxg2_plan* p, const std::vector<std::vector<int32_t>>& list
     p->gfall.clear();
     p->gband_srow = 0;
     const int W = p->W, H = p->H;
-    const int RB = 16352 / W - 2;
-    if (p->dtype == DXG2_U16 || RB < 1 || env_int("DXG2_GATHER_BAND", 1) == 0) return;
+    // bands of RB rows, staged with one row of overlap below: RB + 1 rows of a frame fit
+    // BAND_ROW_BYTES (with the 16-byte alignment slack)
+    const int RB = (BAND_ROW_BYTES - 32) / W - 1;
+    // the band kernel keeps 16-bit offsets into the staged rows and slab addresses / 4
+    // in 32 bits
+    if (p->dtype == DXG2_U16 || RB < 1 || p->slab_bytes >= ((size_t)1 << 34) ||
+        env_int("DXG2_GATHER_BAND", 1) == 0)
+        return;
     const int nbands = (H + RB - 1) / RB;
     std::vector<std::vector<int2>> per(nbands);
     for (int b = 0; b < p->n_bins; ++b) {
@@ -299,7 +305,9 @@ void build_band_work(dxg2_plan* p, const std::vector<std::vector<int32_t>>& list
             const int ymin = L[kb * KB] / W;
             const int ymax = L[std::min(n, (kb + 1) * KB) - 1] / W;
             const int band = ymin / RB;
-            if (ymax - band * RB <= RB + 1) {
+            // a partial last k-block (entries past the end of the list) goes to the
+            // general gather: the band kernel reads every entry of its k-blocks
+            if ((kb + 1) * KB <= n && ymax - band * RB <= RB) {
                 if (fall0 >= 0) flush(kb);
                 per[band].push_back(make_int2(b, kb));
             } else if (fall0 < 0) {
@@ -308,19 +316,30 @@ void build_band_work(dxg2_plan* p, const std::vector<std::vector<int32_t>>& list
         }
         if (fall0 >= 0) flush(nkb);
     }
-    // one work item per band and run of at most 4 x 32 k-blocks (the band kernel keeps
-    // 32 k-blocks per wave in registers)
-    const int kmax = 4 * 32;
+    // one work item per band and run of at most BAND_ITEM_KMAX k-blocks (the band kernel
+    // keeps BAND_KMAX k-blocks per wave in registers)
+    const int kmax = BAND_ITEM_KMAX;
     for (int band = 0; band < nbands; ++band) {
         const int y0 = band * RB;
         for (size_t k = 0; k < per[band].size(); k += kmax) {
             const int k0 = (int)p->gband_kb.size();
             const size_t k1 = std::min(per[band].size(), k + kmax);
             p->gband_kb.insert(p->gband_kb.end(), per[band].begin() + k, per[band].begin() + k1);
-            p->gband.push_back(BandWork{y0, std::min(RB + 2, H - y0), k0, (int)p->gband_kb.size()});
+            p->gband.push_back(BandWork{y0, std::min(RB + 1, H - y0), k0, (int)p->gband_kb.size()});
         }
     }
-    p->gband_srow = ((RB + 2) * W + 16 + 15) / 16 * 16;
+    p->gband_srow = ((RB + 1) * W + 16 + 15) / 16 * 16;
+    // the band kernel stages BAND_STAGE frames of rows in LDS
+    int dev = 0, lds_max = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) !=
+            hipSuccess ||
+        (size_t)BAND_STAGE * p->gband_srow + 64 > (size_t)lds_max) {
+        p->gband.clear();
+        p->gband_kb.clear();
+        p->gfall.clear();
+        p->gband_srow = 0;
+    }
 }
 
 // Tile order of the grouped Gram launch.  Within a bin (largest bins first) the

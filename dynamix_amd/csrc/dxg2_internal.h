@@ -45,6 +45,14 @@ struct GatherWork {      // one workgroup of the gather kernel (x dimension)
     int32_t pad;
 };
 
+// Band gather: a workgroup of 4 waves stages BAND_STAGE frames of a band's rows in LDS
+// (BAND_STAGE x srow bytes, srow <= BAND_ROW_BYTES so that two workgroups share a CU);
+// each wave keeps BAND_KMAX k-blocks in registers, a work item at most BAND_ITEM_KMAX.
+constexpr int BAND_STAGE = 8;
+constexpr int BAND_ROW_BYTES = 9728;
+constexpr int BAND_KMAX = 36;
+constexpr int BAND_ITEM_KMAX = 4 * BAND_KMAX;
+
 struct BandWork {        // one band of detector rows of the band gather (x dimension)
     int32_t y0;          // first staged row
     int32_t rows;        // rows staged (the band plus its overlap, clipped at H)

@@ -119,76 +119,87 @@ __global__ __launch_bounds__(256) void gather_i8_kernel(
     }
 }
 
-// Band gather (uint8 / int8 frames).  A workgroup takes one band of detector rows and
-// up to BAND_KMAX k-blocks whose pixels lie in those rows, for BAND_FRAMES frames.  Each
-// wave loads its k-blocks' pixel quads (relative to the band) and slab addresses once,
-// into registers; then, 4 frames at a time, the workgroup stages the band's rows of the
-// 4 frames in LDS -- one contiguous byte range per frame, read with 16-byte loads from
-// its 16-byte aligned start -- and lane 4 q + i assembles list entries 4q .. 4q+3 of
-// frame t + i from LDS and stores them as one dword (4 x 64-byte runs per store, as in
-// gather_i8_kernel).  A slab address splits into a k-block part and a frame part
-// (slab_offset(kb, t, kk) = slab_offset(kb, 0, kk) + (t/32) 2048 + (t%32) 16).  The
-// work list gives each k-block to the band of its first row, with 2 rows of overlap
-// staged below the band, so k-blocks spanning up to 3 rows are covered; the others go
-// to gather_i8_kernel.
-constexpr int BAND_KMAX = 32;       // k-blocks per wave
-constexpr int BAND_FRAMES = 64;     // frames per workgroup
+// Band gather (uint8 / int8 frames).  A workgroup takes one band of detector rows and up
+// to BAND_ITEM_KMAX full k-blocks whose pixels lie in those rows, for BAND_FRAMES frames.
+// Each wave loads its k-blocks' pixel quads (16-bit offsets into the band) and slab
+// addresses (in 4-byte units) once, into registers.  Then, BAND_STAGE frames at a time,
+// the workgroup copies the band's rows of each frame -- one contiguous byte range, from
+// its 16-byte aligned start -- into LDS with 16-byte global->LDS DMA, and lane
+// (h, w, i) = (lane / 32, lane / 8 % 4, lane % 8) assembles list entries 16 (2 p + h) +
+// 4 w .. +3 of frame t0 + i in pass p = 0, 1 and stores them as one dword.  In the slab
+// a quarter (16 entries) of frames t .. t+7 (t % 8 = 0) is one 128-byte line, so each
+// store writes two whole lines.  A slab address splits into a k-block part and a frame
+// part (slab_offset(kb, t, kk) = slab_offset(kb, 0, kk) + (t/32) 2048 + (t%32) 16).  The
+// work list gives each k-block to the band of its first row, with one row of overlap
+// staged below the band, so k-blocks spanning two rows are covered; the others, and
+// every bin's partial last k-block, go to gather_i8_kernel.  Two workgroups share a CU:
+// one copies while the other assembles.
+constexpr int BAND_FRAMES = 128;    // frames per workgroup (a multiple of BAND_STAGE)
+constexpr int BAND_GROUP = 4;       // k-blocks whose LDS reads are issued together
+static_assert(BAND_FRAMES % BAND_STAGE == 0 && BAND_STAGE == 8, "band stage");
 
 template <bool BIAS>
-__global__ __launch_bounds__(256) void gather_band_kernel(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void gather_band_kernel(
     const uint8_t* __restrict__ frames, int64_t hw, int width, int nframes, int nrows, int srow,
     const int32_t* __restrict__ pix_idx, const BinDev* __restrict__ bins,
     const BandWork* __restrict__ bands, const int2* __restrict__ band_kb,
     int8_t* __restrict__ slab) {
-    extern __shared__ __attribute__((aligned(16))) uint8_t stage[];  // 4 x srow bytes
-    __shared__ int shift[4];
+    extern __shared__ __attribute__((aligned(16))) uint8_t stage[];  // BAND_STAGE x srow
+    __shared__ int shift[BAND_STAGE];
     const BandWork bw = bands[blockIdx.x];
     const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    const int qq = lane >> 2;  // quad of list entries 4qq .. 4qq+3
-    const int i = lane & 3;    // frame t + i
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int i = lane & 7;          // frame t0 + i
+    const int w = (lane >> 3) & 3;   // dword w of a quarter
+    const int h = lane >> 5;         // quarter 2 p + h in pass p
     const int64_t band0 = (int64_t)bw.y0 * width;
+    const int64_t total = (int64_t)nframes * hw;
+    const int64_t span = (int64_t)bw.rows * width;
     const int nk = (bw.k1 - bw.k0 - wave + 3) / 4;  // this wave's k-blocks: k0 + wave + 4j
 
-    int4 pq[BAND_KMAX];
-    int64_t sbase[BAND_KMAX];
+    uint2 pq0[BAND_KMAX], pq1[BAND_KMAX];  // 4 x 16-bit offsets into the staged rows, pass 0/1
+    uint32_t sbase[BAND_KMAX];             // slab address / 4
+    auto pack = [&](int4 p) {
+        return make_uint2((uint32_t)(p.x - band0) | (uint32_t)(p.y - band0) << 16,
+                          (uint32_t)(p.z - band0) | (uint32_t)(p.w - band0) << 16);
+    };
+    if (nk > 0) {
+        // slots past nk repeat the wave's last k-block: a group then needs no per-slot
+        // branch, and the repeats store the same bytes to the same addresses
 #pragma unroll
-    for (int j = 0; j < BAND_KMAX; ++j) {
-        if (j < nk) {
-            const int2 bk = band_kb[bw.k0 + wave + 4 * j];
-            const BinDev bd = bins[bk.x];
-            const int4 p = *reinterpret_cast<const int4*>(pix_idx + bd.idx_off +
-                                                          (int64_t)bk.y * KB + 4 * qq);
-            pq[j] = make_int4(p.x >= 0 ? (int)(p.x - band0) : -1, p.y >= 0 ? (int)(p.y - band0) : -1,
-                              p.z >= 0 ? (int)(p.z - band0) : -1, p.w >= 0 ? (int)(p.w - band0) : -1);
-            sbase[j] = bd.slab_off + slab_offset(bk.y, 0, 4 * qq, nrows);
+        for (int j = 0; j < BAND_KMAX; ++j) {
+            if (j < ((nk + BAND_GROUP - 1) / BAND_GROUP) * BAND_GROUP) {
+                const int2 bk = band_kb[bw.k0 + wave + 4 * min(j, nk - 1)];
+                const BinDev bd = bins[bk.x];
+                const int32_t* pl = pix_idx + bd.idx_off + (int64_t)bk.y * KB + 16 * h + 4 * w;
+                pq0[j] = pack(*reinterpret_cast<const int4*>(pl));
+                pq1[j] = pack(*reinterpret_cast<const int4*>(pl + 32));
+                sbase[j] = (uint32_t)((bd.slab_off + slab_offset(bk.y, 0, 16 * h + 4 * w, nrows)) >> 2);
+            }
         }
     }
 
-    const int64_t total = (int64_t)nframes * hw;
-    const int64_t span = (int64_t)bw.rows * width;
     const int tb = blockIdx.y * BAND_FRAMES;
     const int te = min(nframes, tb + BAND_FRAMES);
-    for (int t0 = tb; t0 < te; t0 += 4) {
+    for (int t0 = tb; t0 < te; t0 += BAND_STAGE) {
         __syncthreads();  // the previous frames' stage has been read
-        // DMA the 16-byte chunks of each frame's rows straight into LDS: the chunks of a
-        // 64-lane group land contiguously (lane l at base + 16 l), as the stage is laid
-        // out.  A chunk that would run past the end of the frame buffer is left to the
-        // byte loads below.
+        // the chunks of a 64-lane group land contiguously (lane l at base + 16 l), as the
+        // stage is laid out; a chunk running past the end of the frame buffer (the last
+        // frame's last chunk at most) is left to the byte loads below
         int64_t tail_a = -1;
         int tail_f = 0, tail_c = 0;
-        for (int f = 0; f < 4 && t0 + f < nframes; ++f) {
+        for (int f = 0; f < BAND_STAGE && t0 + f < nframes; ++f) {
             const int64_t a0 = (int64_t)(t0 + f) * hw + band0;
             const int64_t A = a0 & ~(int64_t)15;
             if (threadIdx.x == 0) shift[f] = (int)(a0 - A);
             const int nchunk = (int)((a0 + span - A + 15) >> 4);
+            uint8_t* dst = stage + f * srow;
             for (int g = wave; g * 64 < nchunk; g += 4) {
                 const int c = g * 64 + lane;
                 const int64_t a = A + 16 * (int64_t)c;
                 if (c < nchunk && a + 16 <= total)
-                    __builtin_amdgcn_global_load_lds(
-                        (const void*)(frames + a),
-                        (lds_void*)(stage + f * srow + 16 * g * 64), 16, 0, 0);
+                    __builtin_amdgcn_global_load_lds((const void*)(frames + a),
+                                                     (lds_void*)(dst + 16 * g * 64), 16, 0, 0);
                 else if (c < nchunk) {
                     tail_a = a;
                     tail_f = f;
@@ -196,26 +207,44 @@ __global__ __launch_bounds__(256) void gather_band_kernel(
                 }
             }
         }
-        __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): every DMA of this thread landed
+        __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this thread's copies landed
         if (tail_a >= 0) {  // bytes inside the buffer only
             uint8_t* d = stage + tail_f * srow + 16 * tail_c;
             for (int e = 0; e < 16; ++e) d[e] = tail_a + e < total ? frames[tail_a + e] : 0;
         }
         __syncthreads();
         const int t = t0 + i;
-        if (t < nframes) {
+        if (nk > 0 && t < nframes) {
             const uint8_t* st = stage + i * srow + shift[i];
-            const int64_t fo = (int64_t)(t >> 5) * 2048 + (t & 31) * 16;
+            int8_t* out = slab + (int64_t)(t >> 5) * 2048 + (t & 31) * 16;
+            auto bytes = [&](uint2 p) -> uint32_t {
+                uint32_t v = (uint32_t)st[p.x & 0xFFFFu] | (uint32_t)st[p.x >> 16] << 8 |
+                             (uint32_t)st[p.y & 0xFFFFu] << 16 | (uint32_t)st[p.y >> 16] << 24;
+                return BIAS ? v ^ 0x80808080u : v;
+            };
+            // Groups of BAND_GROUP k-blocks: the empty asm marks a group's packed
+            // registers as rewritten in every stage, so their unpacked and widened forms
+            // are not hoisted out of the frame loop (that would triple the registers),
+            // while the group's LDS reads can all be in flight together.
+            static_assert(BAND_KMAX % BAND_GROUP == 0 && BAND_GROUP == 4, "k-block groups");
 #pragma unroll
-            for (int j = 0; j < BAND_KMAX; ++j) {
-                if (j < nk) {
-                    const int4 p = pq[j];
-                    uint32_t v = (p.x >= 0 ? (uint32_t)st[p.x] : 0u) |
-                                 ((p.y >= 0 ? (uint32_t)st[p.y] : 0u) << 8) |
-                                 ((p.z >= 0 ? (uint32_t)st[p.z] : 0u) << 16) |
-                                 ((p.w >= 0 ? (uint32_t)st[p.w] : 0u) << 24);
-                    if (BIAS) v ^= 0x80808080u;
-                    *reinterpret_cast<uint32_t*>(slab + sbase[j] + fo) = v;
+            for (int g = 0; g < BAND_KMAX; g += BAND_GROUP) {
+                if (g < nk) {
+                    asm volatile(""
+                                 : "+v"(pq0[g].x), "+v"(pq0[g].y), "+v"(pq1[g].x), "+v"(pq1[g].y),
+                                   "+v"(sbase[g]), "+v"(pq0[g + 1].x), "+v"(pq0[g + 1].y),
+                                   "+v"(pq1[g + 1].x), "+v"(pq1[g + 1].y), "+v"(sbase[g + 1]));
+                    asm volatile(""
+                                 : "+v"(pq0[g + 2].x), "+v"(pq0[g + 2].y), "+v"(pq1[g + 2].x),
+                                   "+v"(pq1[g + 2].y), "+v"(sbase[g + 2]), "+v"(pq0[g + 3].x),
+                                   "+v"(pq0[g + 3].y), "+v"(pq1[g + 3].x), "+v"(pq1[g + 3].y),
+                                   "+v"(sbase[g + 3]));
+#pragma unroll
+                    for (int j = g; j < g + BAND_GROUP; ++j) {
+                        int8_t* o = out + ((int64_t)sbase[j] << 2);
+                        *reinterpret_cast<uint32_t*>(o) = bytes(pq0[j]);
+                        *reinterpret_cast<uint32_t*>(o + 1024) = bytes(pq1[j]);
+                    }
                 }
             }
         }
@@ -228,15 +257,13 @@ void launch_gather_band(const uint8_t* frames, int dtype, int64_t hw, int width,
                         hipStream_t s) {
     if (n_bands <= 0 || nframes <= 0) return;
     const dim3 grid((unsigned)n_bands, (unsigned)((nframes + BAND_FRAMES - 1) / BAND_FRAMES));
-    const size_t lds = (size_t)4 * srow;
+    const size_t lds = (size_t)BAND_STAGE * srow;
     if (dtype == DXG2_U8)
-        gather_band_kernel<true><<<grid, dim3(256), lds, s>>>(frames, hw, width, nframes, nrows,
-                                                              srow, pix_idx, bins, bands, band_kb,
-                                                              slab);
+        gather_band_kernel<true><<<grid, dim3(256), lds, s>>>(
+            frames, hw, width, nframes, nrows, srow, pix_idx, bins, bands, band_kb, slab);
     else
-        gather_band_kernel<false><<<grid, dim3(256), lds, s>>>(frames, hw, width, nframes, nrows,
-                                                               srow, pix_idx, bins, bands,
-                                                               band_kb, slab);
+        gather_band_kernel<false><<<grid, dim3(256), lds, s>>>(
+            frames, hw, width, nframes, nrows, srow, pix_idx, bins, bands, band_kb, slab);
 }
 
 void launch_gather_i8(const uint8_t* frames, int dtype, int64_t hw, int nframes, int nrows,
