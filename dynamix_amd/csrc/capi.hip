@@ -660,9 +660,7 @@ int64_t* dense_gather(dxg2_plan* p, const void* d_frames, int64_t* d_sums, hipSt
     if (!p->gband.empty() && reinterpret_cast<uintptr_t>(d_frames) % 16 == 0) {
         launch_gather_band((const uint8_t*)d_frames, p->dtype, hw, p->W, p->N, p->nrows,
                            p->gband_srow, p->d_pix, p->d_bins, p->d_gband, (int)p->gband.size(),
-                           p->d_gband_kb, p->d_slab, s);
-        launch_gather_i8((const uint8_t*)d_frames, p->dtype, hw, p->N, p->nrows, p->d_pix,
-                         p->d_bins, p->d_gfall, (int)p->gfall.size(), p->d_slab, s);
+                           p->d_gband_kb, p->d_gfall, (int)p->gfall.size(), p->d_slab, s);
     } else {
         launch_gather_i8((const uint8_t*)d_frames, p->dtype, hw, p->N, p->nrows, p->d_pix,
                          p->d_bins, p->d_gwork, (int)p->gather_work.size(), p->d_slab, s);

@@ -127,8 +127,8 @@ void launch_gather_i8(const uint8_t* frames, int dtype, int64_t hw, int nframes,
 // (srow bytes per frame), write every k-block of the band from there.
 void launch_gather_band(const uint8_t* frames, int dtype, int64_t hw, int width, int nframes,
                         int nrows, int srow, const int32_t* pix_idx, const BinDev* bins,
-                        const BandWork* bands, int n_bands, const int2* band_kb, int8_t* slab,
-                        hipStream_t s);
+                        const BandWork* bands, int n_bands, const int2* band_kb,
+                        const GatherWork* fall, int n_fall, int8_t* slab, hipStream_t s);
 // cross: the work items are cross-plane ([H|L] [L|H]^T) items of uint16 frames.
 void launch_gram_i8(int tile, bool cross, const int8_t* slab, const BinDev* bins,
                     const GramWork* work, int n_work, int nframes, int nrows, int biased,

@@ -71,12 +71,12 @@ __device__ __forceinline__ uint32_t quad_transpose(uint32_t x, uint32_t sel_lo, 
 // T = uint16_t writes the high bytes to k-block kb and the low bytes to k-block
 // kb + nkb of the bin (two digit planes).
 template <typename T, bool BIAS>
-__global__ __launch_bounds__(256) void gather_i8_kernel(
-    const T* __restrict__ frames, int64_t hw, int nframes, int nrows,
-    const int32_t* __restrict__ pix_idx, const BinDev* __restrict__ bins,
-    const GatherWork* __restrict__ work, int8_t* __restrict__ slab) {
+__device__ __forceinline__ void gather_general(const T* __restrict__ frames, int64_t hw,
+                                               int nframes, int nrows,
+                                               const int32_t* __restrict__ pix_idx,
+                                               const BinDev* __restrict__ bins, const GatherWork w,
+                                               int t0, int tend, int8_t* __restrict__ slab) {
     constexpr bool WIDE = sizeof(T) == 2;
-    const GatherWork w = work[blockIdx.x];
     const BinDev bd = bins[w.bin];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
@@ -85,8 +85,6 @@ __global__ __launch_bounds__(256) void gather_i8_kernel(
     // byte selectors for v_perm: lo = (y0[qi], y1[qi], 0, 0), hi = (0, 0, y2[qi], y3[qi])
     const uint32_t sel_lo = (uint32_t)qi | ((uint32_t)(4 + qi) << 8) | 0x0c0c0000u;
     const uint32_t sel_hi = 0x00000c0cu | ((uint32_t)qi << 16) | ((uint32_t)(4 + qi) << 24);
-    const int t0 = blockIdx.y * GATHER_TF;
-    const int tend = min(nframes, t0 + GATHER_TF);
     const int64_t plane = (int64_t)bd.nkb * nrows * KB;  // bytes between digit planes
 
     for (int kb = w.kb0 + wave; kb < w.kb0 + w.nkb; kb += 4) {
@@ -119,6 +117,16 @@ __global__ __launch_bounds__(256) void gather_i8_kernel(
     }
 }
 
+template <typename T, bool BIAS>
+__global__ __launch_bounds__(256) void gather_i8_kernel(
+    const T* __restrict__ frames, int64_t hw, int nframes, int nrows,
+    const int32_t* __restrict__ pix_idx, const BinDev* __restrict__ bins,
+    const GatherWork* __restrict__ work, int8_t* __restrict__ slab) {
+    const int t0 = blockIdx.y * GATHER_TF;
+    gather_general<T, BIAS>(frames, hw, nframes, nrows, pix_idx, bins, work[blockIdx.x], t0,
+                            min(nframes, t0 + GATHER_TF), slab);
+}
+
 // Band gather (uint8 / int8 frames).  A workgroup takes one band of detector rows and up
 // to BAND_ITEM_KMAX full k-blocks whose pixels lie in those rows, for BAND_FRAMES frames.
 // Each wave loads its k-blocks' pixel quads (16-bit offsets into the band) and slab
@@ -142,10 +150,17 @@ template <bool BIAS>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void gather_band_kernel(
     const uint8_t* __restrict__ frames, int64_t hw, int width, int nframes, int nrows, int srow,
     const int32_t* __restrict__ pix_idx, const BinDev* __restrict__ bins,
-    const BandWork* __restrict__ bands, const int2* __restrict__ band_kb,
-    int8_t* __restrict__ slab) {
+    const BandWork* __restrict__ bands, int n_bands, const int2* __restrict__ band_kb,
+    const GatherWork* __restrict__ fall, int8_t* __restrict__ slab) {
     extern __shared__ __attribute__((aligned(16))) uint8_t stage[];  // BAND_STAGE x srow
     __shared__ int shift[BAND_STAGE];
+    if ((int)blockIdx.x >= n_bands) {  // a k-block run the bands do not cover
+        const int t0 = blockIdx.y * BAND_FRAMES;
+        gather_general<uint8_t, BIAS>(frames, hw, nframes, nrows, pix_idx, bins,
+                                      fall[blockIdx.x - n_bands], t0,
+                                      min(nframes, t0 + BAND_FRAMES), slab);
+        return;
+    }
     const BandWork bw = bands[blockIdx.x];
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -253,17 +268,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 
 void launch_gather_band(const uint8_t* frames, int dtype, int64_t hw, int width, int nframes,
                         int nrows, int srow, const int32_t* pix_idx, const BinDev* bins,
-                        const BandWork* bands, int n_bands, const int2* band_kb, int8_t* slab,
-                        hipStream_t s) {
-    if (n_bands <= 0 || nframes <= 0) return;
-    const dim3 grid((unsigned)n_bands, (unsigned)((nframes + BAND_FRAMES - 1) / BAND_FRAMES));
+                        const BandWork* bands, int n_bands, const int2* band_kb,
+                        const GatherWork* fall, int n_fall, int8_t* slab, hipStream_t s) {
+    if (n_bands + n_fall <= 0 || nframes <= 0) return;
+    const dim3 grid((unsigned)(n_bands + n_fall),
+                    (unsigned)((nframes + BAND_FRAMES - 1) / BAND_FRAMES));
     const size_t lds = (size_t)BAND_STAGE * srow;
     if (dtype == DXG2_U8)
-        gather_band_kernel<true><<<grid, dim3(256), lds, s>>>(
-            frames, hw, width, nframes, nrows, srow, pix_idx, bins, bands, band_kb, slab);
+        gather_band_kernel<true><<<grid, dim3(256), lds, s>>>(frames, hw, width, nframes, nrows,
+                                                              srow, pix_idx, bins, bands, n_bands,
+                                                              band_kb, fall, slab);
     else
-        gather_band_kernel<false><<<grid, dim3(256), lds, s>>>(
-            frames, hw, width, nframes, nrows, srow, pix_idx, bins, bands, band_kb, slab);
+        gather_band_kernel<false><<<grid, dim3(256), lds, s>>>(frames, hw, width, nframes, nrows,
+                                                               srow, pix_idx, bins, bands, n_bands,
+                                                               band_kb, fall, slab);
 }
 
 void launch_gather_i8(const uint8_t* frames, int dtype, int64_t hw, int nframes, int nrows,
