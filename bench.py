@@ -20,6 +20,9 @@ Printed as one JSON line.  Besides the metric it reports
   ttcf          the two-time matrix of one bin of the same stack;
   cpu_baseline  the reference's numpy MatMul correlator (float32 sgemm + diagonal
                 loop, restated in oracle/) on one bin of the same stack, all host cores;
+  cpu_baseline_fft  the reference's FFT correlator (float32 real FFTs of zero-padded
+                pixel series, restated in oracle/ with scipy.fft) on pixel chunks of the
+                same stack, all host cores;
   plain_fp32    this library's one-bin-per-launch FP32 comparator on the same stack;
   fft           the FFT path (FP32 Stockham power spectra) on the same stack;
   uint16        the dense path on 2162 x 2068 x 5,000 uint16 frames (1 % hot pixels),
@@ -89,6 +92,51 @@ def cpu_baseline(frames_dev, qmask, target_s=15.0):
                       "oracle/g2.py py_dense_correlator_f32) on whole bins %s x %d frames "
                       "of the same stack: %.1f s, OpenBLAS threads = %d"
                       % (", ".join(done), n, spent, cores)}
+
+
+def gather_roofline(qmask, shape, n, gather_ms):
+    """HBM roofline of the gather stage: every frame byte read once, and one byte per
+    binned pixel (rounded up to whole 64-pixel k-blocks) per frame written to the slab."""
+    import numpy as np
+    counts = np.bincount(qmask.ravel())[1:]
+    read = float(n) * shape[0] * shape[1]
+    write = float(n) * int((64 * np.ceil(counts / 64.0)).sum())
+    gbps = (read + write) / (gather_ms * 1e-3) / 1e9
+    return {"ms": gather_ms, "bytes_read": read, "bytes_written": write, "GBps": gbps,
+            "peak_GBps": HBM_PEAK_GBS, "frac_of_hbm": gbps / HBM_PEAK_GBS,
+            "kernel": "gather_band_kernel"}
+
+
+def cpu_baseline_fft(frames_dev, qmask, target_s=8.0, chunk=4096):
+    """Reference-equivalent FFT correlator (oracle fft_correlator, scipy.fft with every
+    host core) on chunks of `chunk` pixels of the bins, taken in order of closeness to
+    the mean bin size, until target_s is spent.  Each chunk is transformed as one bin:
+    the FFT work per pixel is the same as in the whole bin."""
+    import numpy as np
+    import torch
+    from oracle import g2 as og
+    n = frames_dev.shape[0]
+    counts = np.bincount(qmask.ravel())[1:]
+    order = np.argsort(np.abs(counts - counts.mean())) + 1
+    cores = len(os.sched_getaffinity(0))
+    done, pf, spent = [], 0, 0.0
+    for b in order:
+        pix = np.flatnonzero(qmask.ravel() == b)[:chunk]
+        idx = torch.from_numpy(pix).cuda()
+        a = frames_dev.view(n, -1).index_select(1, idx).cpu().numpy()   # (N, chunk) uint8
+        t0 = time.perf_counter()
+        og.fft_correlator(a.reshape(n, 1, -1), np.ones((1, a.shape[1]), dtype=np.int32),
+                          workers=cores)
+        spent += time.perf_counter() - t0
+        done.append("%d" % b)
+        pf += a.shape[1] * n
+        if spent >= target_s:
+            break
+    return {"value": pf / spent, "unit": "pixel*frames/s", "cores": cores, "kind": "port",
+            "sample": "FFT correlator (scipy.fft float32 rfft/irfft at Nf = %d, oracle/g2.py "
+                      "fft_correlator) on the first %d pixels of bins %s x %d frames of the "
+                      "same stack: %.1f s, scipy workers = %d"
+                      % (og.next_fft_length(2 * n), chunk, ", ".join(done), n, spent, cores)}
 
 
 def _timed(fn, reps, torch):
@@ -439,6 +487,7 @@ def main():
                    "autotune": None if tuning is None else {
                        "gram_ms": tuning["gram_ms"], "default_gram_ms": tuning["default_ms"],
                        "candidates": len(tuning["trials"])}},
+        "gather": gather_roofline(q, shape, n, gather_ms),
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": INT8_PEAK_TOPS,
                      "unit": "TOP/s", "frac": achieved / INT8_PEAK_TOPS,
                      "traffic": read_traffic("gram_i8_%s_traffic.txt" % args.workload),
@@ -476,6 +525,12 @@ def main():
     if not args.no_cpu:
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(frames, q)
+        if not args.no_fft:
+            log("cpu baseline (FFT)")
+            out["cpu_baseline_fft"] = cpu_baseline_fft(frames, q)
+            if "fft" in out:
+                out["fft"]["speedup_vs_cpu_fft"] = (out["fft"]["value"] /
+                                                    out["cpu_baseline_fft"]["value"])
     if not (args.no_events and args.no_uint16 and args.no_serving):
         del frames, num, sums, den, g2
         plan.close()

@@ -94,6 +94,9 @@ class Plan(object):
     """Owns one dxg2 plan (device work lists and workspaces)."""
 
     def __init__(self, shape, nframes, qmask, dtype, scale=None, flags=0):
+        # torch first: the plan allocates and uploads through HIP, and a HIP runtime
+        # started by the library before torch is imported leaves torch without a device
+        _torch()
         self.lib = _lib.load()
         self.dtype = np.dtype(dtype)
         if self.dtype not in _lib.DTYPE_CODES:

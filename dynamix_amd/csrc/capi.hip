@@ -247,7 +247,8 @@ struct dxg2_plan {
     uint32_t* d_off = nullptr;     // CSR offsets built from COO input
     int32_t* d_etimes = nullptr;   // CSR frames built from COO input
     void* d_edata = nullptr;       // CSR counts built from COO input
-    int64_t ev_cap = 0;            // capacity of d_etimes / d_edata in events
+    int64_t ev_cap = 0;            // capacity of d_etimes / d_edata / the sort buffers in events
+    CooSortBufs coo_sort{nullptr, nullptr, nullptr, nullptr, 0};  // COO -> CSR sort workspace
 
     // FFT path
     std::vector<KbWork> kb_work;   // every k-block of every bin
@@ -619,14 +620,31 @@ int ensure_event_buffers(dxg2_plan* p, int64_t nnz_coo, hipStream_t s) {
             DXG2_HIP(dev_alloc(p->alloc, (void**)&p->d_scan, ((npix + 1023) / 1024 + 1) * 4, s));
         }
         if (nnz_coo > p->ev_cap) {
+            CooSortBufs& cs = p->coo_sort;
             dev_free(p->alloc, p->d_etimes, s);
             dev_free(p->alloc, p->d_edata, s);
+            dev_free(p->alloc, cs.keys, s);
+            dev_free(p->alloc, cs.vals0, s);
+            dev_free(p->alloc, cs.vals1, s);
+            dev_free(p->alloc, cs.tmp, s);
             p->d_etimes = nullptr;
             p->d_edata = nullptr;
+            cs = CooSortBufs{nullptr, nullptr, nullptr, nullptr, 0};
+            p->ev_cap = 0;
             const int64_t cap = std::max<int64_t>(nnz_coo, 1);
+            const int64_t vb = coo_narrow_values(p->dtype, p->N) ? 4 : 8;
+            const size_t tmp = coo_sort_bytes(p->dtype, p->N, cap, (uint32_t)npix);
+            if (tmp == 0)
+                return set_error(DXG2_EHIP, "cannot size the sort of %lld events",
+                                 (long long)cap);
             if (dev_alloc(p->alloc, (void**)&p->d_etimes, cap * 4, s) != hipSuccess ||
-                dev_alloc(p->alloc, &p->d_edata, cap * 2, s) != hipSuccess)
+                dev_alloc(p->alloc, &p->d_edata, cap * 2, s) != hipSuccess ||
+                dev_alloc(p->alloc, (void**)&cs.keys, cap * 4, s) != hipSuccess ||
+                dev_alloc(p->alloc, &cs.vals0, cap * vb, s) != hipSuccess ||
+                dev_alloc(p->alloc, &cs.vals1, cap * vb, s) != hipSuccess ||
+                dev_alloc(p->alloc, &cs.tmp, tmp, s) != hipSuccess)
                 return set_error(DXG2_ENOMEM, "cannot allocate %lld events", (long long)cap);
+            cs.tmp_bytes = tmp;
             p->ev_cap = cap;
         }
     }
@@ -928,7 +946,8 @@ static void plan_release(dxg2_plan* p, hipStream_t s) {
                     p->d_ework, p->d_err,  p->d_cnt,   p->d_scan,  p->d_off,
                     p->d_etimes, p->d_edata, p->d_kbwork, p->d_xs, p->d_tw,
                     p->d_power, p->d_fnum, p->d_qmwork, p->d_qxwork, p->d_qoff,
-                    p->d_heads, p->d_gband, p->d_gband_kb, p->d_gfall};
+                    p->d_heads, p->d_gband, p->d_gband_kb, p->d_gfall,
+                    p->coo_sort.keys, p->coo_sort.vals0, p->coo_sort.vals1, p->coo_sort.tmp};
     drop_std(p, s);
     for (void* b : bufs) dev_free(p->alloc, b, s);
     for (auto& q : p->ev_pool)
@@ -989,7 +1008,8 @@ int dxg2_plan_footprint(const dxg2_plan* p, int32_t path, size_t* bytes) {
             const int64_t npix = (int64_t)p->H * p->W;
             *bytes = common + p->event_work.size() * sizeof(EventWork) + sizeof(int) +
                      (size_t)(2 * npix + 1 + (npix + 1023) / 1024 + 1) * 4 +
-                     (size_t)p->ev_cap * 6;
+                     (size_t)p->ev_cap * (6 + 4 + 2 * (coo_narrow_values(p->dtype, p->N) ? 4 : 8)) +
+                     p->coo_sort.tmp_bytes;
             return DXG2_OK;
         }
         case 4:
@@ -1441,8 +1461,10 @@ int dxg2_correlate_events_coo(dxg2_plan* p, const uint32_t* d_pixel, const uint3
     launch_fill_i64(d_sums ? d_sums : p->d_sums, nbn, s);
     launch_fill_i64(p->d_ugram, nbn, s);
     launch_fill_i32(p->d_err, 1, s);
-    launch_coo_to_csr(p->dtype, d_pixel, d_frame, d_count, nnz, (uint32_t)((int64_t)p->H * p->W),
-                      p->d_cnt, p->d_scan, p->d_off, p->d_etimes, p->d_edata, p->d_err, s);
+    if (launch_coo_to_csr(p->dtype, d_pixel, d_frame, d_count, nnz,
+                          (uint32_t)((int64_t)p->H * p->W), p->N, p->d_cnt, p->d_scan, p->d_off,
+                          p->coo_sort, p->d_etimes, p->d_edata, p->d_err, s) != hipSuccess)
+        return set_error(DXG2_EHIP, "sort of %lld COO events failed", (long long)nnz);
     return events_from_csr(p, p->d_etimes, p->d_edata, p->d_off, nnz, d_num, d_sums, d_den,
                            d_g2, s);
 }

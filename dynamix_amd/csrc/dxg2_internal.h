@@ -158,9 +158,22 @@ void launch_event_pairs(int dtype, const int32_t* times, const void* data,
                         const uint32_t* offsets, const int32_t* pix_idx, const BinDev* bins,
                         const EventWork* work, int n_work, int nframes, int64_t nnz,
                         int64_t* num, int64_t* sums, int* err, hipStream_t s);
-void launch_coo_to_csr(int dtype, const uint32_t* pix, const uint32_t* frame, const void* cnt,
-                       int64_t nnz, uint32_t npix, uint32_t* count, uint32_t* scan_tot,
-                       uint32_t* offsets, int32_t* times, void* data, int* err, hipStream_t s);
+// Workspace of the COO -> CSR sort: sorted keys, packed (frame, count) words before and
+// after the sort (4 bytes each when coo_narrow_values, else 8), and the sort's scratch.
+struct CooSortBufs {
+    uint32_t* keys;
+    void* vals0;
+    void* vals1;
+    void* tmp;
+    size_t tmp_bytes;
+};
+bool coo_narrow_values(int dtype, int nframes);
+size_t coo_sort_bytes(int dtype, int nframes, int64_t cap, uint32_t npix);
+hipError_t launch_coo_to_csr(int dtype, const uint32_t* pix, const uint32_t* frame,
+                             const void* cnt, int64_t nnz, uint32_t npix, int nframes,
+                             uint32_t* count, uint32_t* scan_tot, uint32_t* offsets,
+                             const CooSortBufs& sort, int32_t* times, void* data, int* err,
+                             hipStream_t s);
 void launch_fill_i32(int* p, int64_t n, hipStream_t s);
 void launch_finalize_f64(const int64_t* sums, const double* fnum, const BinDev* bins, int n_bins,
                          int nframes, int64_t* prefix, int64_t* smax, double* num, double* den,

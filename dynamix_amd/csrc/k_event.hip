@@ -17,8 +17,12 @@
 //    pixel, as compress_all_stack produces, event.py:267-284); a violation sets a
 //    flag the host turns into an error instead of reading out of bounds.
 // Frame-sorted COO input (pixel, frame, count) is first turned into this pixel-major
-// layout on the device: per-pixel counts, an exclusive scan, a scatter through atomic
-// cursors, and a per-pixel insertion sort of the (nearly ordered) segments.
+// layout on the device: per-pixel counts and an exclusive scan give the offsets; the
+// (frame, count) pairs, packed into one word, are ordered by pixel with a stable LSD
+// radix sort (rocPRIM) over the bits a pixel index needs, so each pixel's events keep
+// the input's frame order; the packed words are then split into times and counts.
+#include <rocprim/device/device_radix_sort.hpp>
+
 #include "dxg2_internal.h"
 
 namespace dxg2 {
@@ -222,41 +226,36 @@ __global__ void copy_u32_kernel(const uint32_t* __restrict__ src, uint32_t* __re
         dst[i] = src[i];
 }
 
-template <typename T>
-__global__ void coo_scatter_kernel(const uint32_t* __restrict__ pix, const uint32_t* __restrict__ frame,
-                                   const T* __restrict__ cnt, int64_t nnz, uint32_t npix,
-                                   uint32_t* __restrict__ cursor, int32_t* __restrict__ times,
-                                   T* __restrict__ data) {
+// (frame, count) of event e packed as frame << CB | count; a frame past the series sets
+// flag bit 0 (the packing keeps CB bits of count and the rest for the frame).
+template <typename V, typename T, int CB>
+__global__ void coo_pack_kernel(const uint32_t* __restrict__ frame, const T* __restrict__ cnt,
+                                int64_t nnz, int nframes, V* __restrict__ out,
+                                int* __restrict__ err) {
     for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < nnz;
          e += (int64_t)gridDim.x * blockDim.x) {
-        const uint32_t p = pix[e];
-        if (p >= npix) continue;
-        const uint32_t pos = atomicAdd(&cursor[p], 1u);
-        times[pos] = (int32_t)frame[e];
-        data[pos] = cnt[e];
+        const uint32_t f = frame[e];
+        if (f >= (uint32_t)nframes) atomicOr(err, 1);
+        out[e] = ((V)f << CB) | (V)cnt[e];
     }
 }
 
-// Per-pixel insertion sort of (time, value) by time; segments arrive nearly sorted.
-template <typename T>
-__global__ void segment_sort_kernel(const uint32_t* __restrict__ offsets, uint32_t npix,
-                                    int32_t* __restrict__ times, T* __restrict__ data) {
-    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < npix;
-         p += (int64_t)gridDim.x * blockDim.x) {
-        const uint32_t b = offsets[p], e = offsets[p + 1];
-        for (uint32_t i = b + 1; i < e; ++i) {
-            const int32_t t = times[i];
-            const T d = data[i];
-            uint32_t j = i;
-            while (j > b && times[j - 1] > t) {
-                times[j] = times[j - 1];
-                data[j] = data[j - 1];
-                --j;
-            }
-            times[j] = t;
-            data[j] = d;
-        }
+template <typename V, typename T, int CB>
+__global__ void coo_unpack_kernel(const V* __restrict__ in, int64_t nnz,
+                                  int32_t* __restrict__ times, T* __restrict__ data) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < nnz;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const V v = in[e];
+        times[e] = (int32_t)(uint32_t)(v >> CB);
+        data[e] = (T)(v & (((V)1 << CB) - 1));
     }
+}
+
+// Key bits of pixel indices below npix.
+static int pixel_key_bits(uint32_t npix) {
+    int b = 1;
+    while (b < 32 && ((uint64_t)1 << b) < npix) ++b;
+    return b;
 }
 
 static unsigned grid_for(int64_t n, int64_t cap = 65536);
@@ -342,30 +341,60 @@ static unsigned grid_for(int64_t n, int64_t cap) {
     return (unsigned)g;
 }
 
-void launch_coo_to_csr(int dtype, const uint32_t* pix, const uint32_t* frame, const void* cnt,
-                       int64_t nnz, uint32_t npix, uint32_t* count, uint32_t* scan_tot,
-                       uint32_t* offsets, int32_t* times, void* data, int* err, hipStream_t s) {
+bool coo_narrow_values(int dtype, int nframes) {
+    return dtype != DXG2_U16 && nframes <= (1 << 24);
+}
+
+size_t coo_sort_bytes(int dtype, int nframes, int64_t cap, uint32_t npix) {
+    size_t bytes = 0;
+    const int bits = pixel_key_bits(npix);
+    hipError_t rc;
+    if (coo_narrow_values(dtype, nframes))
+        rc = rocprim::radix_sort_pairs(nullptr, bytes, (const uint32_t*)nullptr, (uint32_t*)nullptr,
+                                       (const uint32_t*)nullptr, (uint32_t*)nullptr, (size_t)cap,
+                                       0, bits, (hipStream_t)0);
+    else
+        rc = rocprim::radix_sort_pairs(nullptr, bytes, (const uint32_t*)nullptr, (uint32_t*)nullptr,
+                                       (const uint64_t*)nullptr, (uint64_t*)nullptr, (size_t)cap,
+                                       0, bits, (hipStream_t)0);
+    return rc == hipSuccess ? bytes : 0;
+}
+
+template <typename V, typename T, int CB>
+static hipError_t coo_sort_split(const uint32_t* pix, const uint32_t* frame, const T* cnt,
+                                 int64_t nnz, uint32_t npix, int nframes, const CooSortBufs& b,
+                                 int32_t* times, T* data, int* err, hipStream_t s) {
+    V* v0 = reinterpret_cast<V*>(b.vals0);
+    V* v1 = reinterpret_cast<V*>(b.vals1);
+    coo_pack_kernel<V, T, CB><<<grid_for(nnz), 256, 0, s>>>(frame, cnt, nnz, nframes, v0, err);
+    size_t bytes = b.tmp_bytes;
+    const hipError_t rc = rocprim::radix_sort_pairs(b.tmp, bytes, pix, b.keys, v0, v1, (size_t)nnz,
+                                                    0, pixel_key_bits(npix), s);
+    if (rc != hipSuccess) return rc;
+    coo_unpack_kernel<V, T, CB><<<grid_for(nnz), 256, 0, s>>>(v1, nnz, times, data);
+    return hipSuccess;
+}
+
+hipError_t launch_coo_to_csr(int dtype, const uint32_t* pix, const uint32_t* frame,
+                             const void* cnt, int64_t nnz, uint32_t npix, int nframes,
+                             uint32_t* count, uint32_t* scan_tot, uint32_t* offsets,
+                             const CooSortBufs& sort, int32_t* times, void* data, int* err,
+                             hipStream_t s) {
     fill_u32_kernel<<<grid_for(npix), 256, 0, s>>>(count, npix, 0u);
     if (nnz > 0) coo_count_kernel<<<grid_for(nnz), 256, 0, s>>>(pix, nnz, npix, count, err);
     const int nb = (int)((npix + SCAN_B - 1) / SCAN_B);
     scan_block_totals<<<nb, 256, 0, s>>>(count, npix, scan_tot);
     scan_totals<<<1, 256, 0, s>>>(scan_tot, nb);
     scan_apply<<<nb, 256, 0, s>>>(count, npix, scan_tot, offsets);
-    // cursors start at the offsets (reuse the count buffer)
-    copy_u32_kernel<<<grid_for(npix), 256, 0, s>>>(offsets, count, npix);
-    if (nnz > 0) {
-        if (dtype == DXG2_U16) {
-            coo_scatter_kernel<uint16_t><<<grid_for(nnz), 256, 0, s>>>(
-                pix, frame, (const uint16_t*)cnt, nnz, npix, count, times, (uint16_t*)data);
-            segment_sort_kernel<uint16_t><<<grid_for(npix), 256, 0, s>>>(offsets, npix, times,
-                                                                         (uint16_t*)data);
-        } else {
-            coo_scatter_kernel<uint8_t><<<grid_for(nnz), 256, 0, s>>>(
-                pix, frame, (const uint8_t*)cnt, nnz, npix, count, times, (uint8_t*)data);
-            segment_sort_kernel<uint8_t><<<grid_for(npix), 256, 0, s>>>(offsets, npix, times,
-                                                                        (uint8_t*)data);
-        }
-    }
+    if (nnz <= 0) return hipSuccess;
+    if (dtype == DXG2_U16)
+        return coo_sort_split<uint64_t, uint16_t, 16>(pix, frame, (const uint16_t*)cnt, nnz, npix,
+                                                      nframes, sort, times, (uint16_t*)data, err, s);
+    if (coo_narrow_values(dtype, nframes))
+        return coo_sort_split<uint32_t, uint8_t, 8>(pix, frame, (const uint8_t*)cnt, nnz, npix,
+                                                    nframes, sort, times, (uint8_t*)data, err, s);
+    return coo_sort_split<uint64_t, uint8_t, 16>(pix, frame, (const uint8_t*)cnt, nnz, npix,
+                                                 nframes, sort, times, (uint8_t*)data, err, s);
 }
 
 }  // namespace dxg2

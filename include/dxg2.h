@@ -207,7 +207,10 @@ int dxg2_correlate_events_csr(dxg2_plan* plan, const int32_t* d_times, const voi
 
 /* Event path from frame-sorted COO: d_pixel uint32 raster pixel index, d_frame uint32,
  * d_count uint8 (or uint16 for a DXG2_U16 plan), nnz events.  Converted to the CSR
- * layout above on the device (count, scan, scatter, per-pixel sort), then as above.
+ * layout above on the device (per-pixel counts and scan; a stable radix sort by pixel,
+ * which keeps each pixel's events in input order), then as above.  Events of one pixel
+ * must therefore come in increasing frame order; a list that breaks this is reported by
+ * dxg2_plan_event_status (flag bit 0).
  * The layout FramesCompressor.compress_all_stack produces (correlator/event.py:267-284)
  * is the CSR form; this COO form has no reference counterpart. */
 int dxg2_correlate_events_coo(dxg2_plan* plan, const uint32_t* d_pixel, const uint32_t* d_frame,

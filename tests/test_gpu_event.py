@@ -69,6 +69,44 @@ def test_events_coo_matches_csr():
     _check(c, g2, ev.event_g2(times, data, offsets, q, 300))
 
 
+def test_events_coo_uint16():
+    """uint16 counts take the 64-bit packed (frame, count) words through the sort."""
+    _announce("events coo uint16")
+    from dynamix_amd.correlator.event import EventCorrelator
+    rng = np.random.default_rng(11)
+    f = (rng.random((200, 9, 11)) < 0.15) * rng.integers(1, 3000, size=(200, 9, 11))
+    f = f.astype(np.uint16)
+    q = rng.integers(0, 5, size=(9, 11)).astype(np.int32)
+    data, times, offsets = ev.compress_all_stack(f)
+    pix, fr, cnt = ev.csr_to_coo(times, data, offsets)
+    c = EventCorrelator((9, 11), 200, 10, qmask=q, dtype=np.uint16)
+    g2 = c.correlate_coo(pix, fr, cnt.astype(np.uint16))
+    _check(c, g2, ev.event_g2(times, data, offsets, q, 200))
+    dense = og.dense_g2(f, q)
+    np.testing.assert_array_equal(c.last_numerators.cpu().numpy(), dense["num"])
+
+
+def test_events_coo_input_errors():
+    """A pixel whose events are not in increasing frame order, or a frame past the
+    series, is reported as invalid input (the sort keeps each pixel's input order)."""
+    _announce("events coo errors")
+    from dynamix_amd.correlator.event import EventCorrelator
+    f = _sparse_stack(5, 100, (6, 7), 0.3)
+    q = np.ones((6, 7), dtype=np.int32)
+    data, times, offsets = ev.compress_all_stack(f)
+    pix, fr, cnt = ev.csr_to_coo(times, data, offsets)
+    c = EventCorrelator((6, 7), 100, 10, qmask=q, dtype=np.uint8)
+    with pytest.raises(ValueError, match="strictly increasing"):
+        c.correlate_coo(pix[::-1].copy(), fr[::-1].copy(), cnt[::-1].astype(np.uint8))
+    bad = fr.copy()
+    bad[len(bad) // 2] = 100
+    with pytest.raises(ValueError, match="out of"):
+        c.correlate_coo(pix, bad, cnt.astype(np.uint8))
+    # the plan recovers: a valid list afterwards gives the reference result
+    g2 = c.correlate_coo(pix, fr, cnt.astype(np.uint8))
+    _check(c, g2, ev.event_g2(times, data, offsets, q, 100))
+
+
 def test_events_scale_factor_and_int8_and_goldens(golden):
     _announce("events int8 + scale")
     from dynamix_amd.correlator.event import EventCorrelator
