@@ -224,6 +224,7 @@ struct dxg2_plan {
     int64_t* d_psums = nullptr;   // digit planes: S_H rows then S_L rows (n_bins each)
     int64_t* d_prefix = nullptr;  // per bin and plane: prefix sums of S(t), N + 1 values
     int64_t* d_smax = nullptr;    // per bin: max |S(t)|
+    double* d_dpre = nullptr;     // per bin and lag: FP64 Den(tau) before the finalise
     float* d_pa = nullptr;
     float* d_pg = nullptr;
     float* d_pm = nullptr;
@@ -713,8 +714,8 @@ void dense_finish(dxg2_plan* p, int64_t* gsums, int64_t* d_num, int64_t* d_sums,
     int64_t* sums = d_sums ? d_sums : p->d_sums;
     const bool wide = p->dtype == DXG2_U16;
     launch_finalize(gsums, sums, p->d_ugram, p->d_bins, p->n_bins, p->N,
-                    wide ? 2 : (p->dtype == DXG2_U8 ? 1 : 0), p->d_prefix, p->d_smax, d_num,
-                    d_den, d_g2, s);
+                    wide ? 2 : (p->dtype == DXG2_U8 ? 1 : 0), p->d_prefix, p->d_smax, p->d_dpre,
+                    d_num, d_den, d_g2, s);
 }
 
 }  // namespace
@@ -907,7 +908,9 @@ int dxg2_plan_create(const dxg2_plan_desc* desc, dxg2_plan** out) {
                                                      (p->N + 1) * sizeof(int64_t), s0) !=
                     hipSuccess ||
                 dev_alloc(p->alloc, (void**)&p->d_smax, (size_t)std::max(p->n_bins, 1) * sizeof(int64_t),
-                           s0) != hipSuccess))
+                           s0) != hipSuccess ||
+                dev_alloc(p->alloc, (void**)&p->d_dpre,
+                          (size_t)std::max(p->n_bins, 1) * p->N * sizeof(double), s0) != hipSuccess))
         rc = set_error(DXG2_ENOMEM, "cannot allocate denominator workspace");
     if (!rc && hipStreamSynchronize(s0) != hipSuccess)
         rc = set_error(DXG2_EHIP, "plan upload failed");
@@ -941,7 +944,7 @@ static void plan_release(dxg2_plan* p, hipStream_t s) {
     drop_graph(p);
     if (p->cap_stream) (void)hipStreamDestroy(p->cap_stream);
     void* bufs[] = {p->d_pix,   p->d_bins, p->d_gwork, p->d_mwork, p->d_xwork, p->d_slab,
-                    p->d_ugram, p->d_sums, p->d_psums, p->d_prefix, p->d_smax, p->d_pa,
+                    p->d_ugram, p->d_sums, p->d_psums, p->d_prefix, p->d_smax, p->d_dpre, p->d_pa,
                     p->d_pg, p->d_pm,
                     p->d_ework, p->d_err,  p->d_cnt,   p->d_scan,  p->d_off,
                     p->d_etimes, p->d_edata, p->d_kbwork, p->d_xs, p->d_tw,
@@ -984,7 +987,7 @@ int dxg2_plan_footprint(const dxg2_plan* p, int32_t path, size_t* bytes) {
     const size_t nb_n = (size_t)std::max(p->n_bins, 1) * p->N * sizeof(int64_t);
     const int planes = slab_planes(p->dtype);
     size_t common = p->pix_idx.size() * 4 + p->bins.size() * sizeof(BinDev) +
-                    (planes == 2 ? 6 : 2) * nb_n +
+                    (planes == 2 ? 7 : 3) * nb_n +
                     (size_t)std::max(p->n_bins, 1) * (planes * (p->N + 1) + 1) * sizeof(int64_t);
     switch (path) {
         case 0:
@@ -1417,7 +1420,7 @@ static int events_from_csr(dxg2_plan* p, const int32_t* times, const void* data,
     launch_event_pairs(p->dtype, times, data, offsets, p->d_pix, p->d_bins, p->d_ework,
                        (int)p->event_work.size(), p->N, nnz, p->d_ugram, sums, p->d_err, s);
     launch_finalize(sums, sums, p->d_ugram, p->d_bins, p->n_bins, p->N, 0, p->d_prefix,
-                    p->d_smax, d_num, d_den, d_g2, s);
+                    p->d_smax, p->d_dpre, d_num, d_den, d_g2, s);
     DXG2_HIP(hipGetLastError());
     return DXG2_OK;
 }
@@ -1495,7 +1498,7 @@ int dxg2_correlate_fft(dxg2_plan* p, const void* d_frames, double* d_num, int64_
     launch_fft_inverse(p->d_power, p->fft_L, p->n_bins, p->N, p->d_fnum, s);
     if (ev) DXG2_HIP(hipEventRecord(ev[2], s));
     launch_finalize_f64(sums, p->d_fnum, p->d_bins, p->n_bins, p->N, p->d_prefix, p->d_smax,
-                        d_num, d_den, d_g2, s);
+                        p->d_dpre, d_num, d_den, d_g2, s);
     if (ev) DXG2_HIP(hipEventRecord(ev[3], s));
     DXG2_HIP(hipGetLastError());
     return DXG2_OK;

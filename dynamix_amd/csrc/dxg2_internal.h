@@ -150,7 +150,8 @@ void launch_gram_i8_jobs(int tile, bool cross, const GramJob* jobs, const GramWo
 // receives S = 256 S_H + S_L).
 void launch_finalize(const int64_t* plane_sums, int64_t* sums, const int64_t* ugram,
                      const BinDev* bins, int n_bins, int nframes, int mode, int64_t* prefix,
-                     int64_t* smax, int64_t* num, double* den, float* g2, hipStream_t s);
+                     int64_t* smax, double* dpre, int64_t* num, double* den, float* g2,
+                     hipStream_t s);
 void launch_plain_bin(const uint8_t* frames, int dtype, int64_t hw, int nframes,
                       const int32_t* pix, int npix, float* a, float* gram, float* means,
                       float* g2_row, hipStream_t s);
@@ -176,8 +177,8 @@ hipError_t launch_coo_to_csr(int dtype, const uint32_t* pix, const uint32_t* fra
                              hipStream_t s);
 void launch_fill_i32(int* p, int64_t n, hipStream_t s);
 void launch_finalize_f64(const int64_t* sums, const double* fnum, const BinDev* bins, int n_bins,
-                         int nframes, int64_t* prefix, int64_t* smax, double* num, double* den,
-                         float* g2, hipStream_t s);
+                         int nframes, int64_t* prefix, int64_t* smax, double* dpre, double* num,
+                         double* den, float* g2, hipStream_t s);
 void launch_gather_pm(const uint8_t* frames, int dtype, int64_t hw, int nframes, int nrows,
                       const int32_t* pix_idx, const BinDev* bins, const KbWork* work, int n_work,
                       uint8_t* xs, hipStream_t s);

@@ -12,8 +12,9 @@
 //
 // Exactness: a per-bin prep pass forms the prefix sums P(t) = sum_{u<t} S(u) and
 // M = max_t |S(t)|.  When N M^2 < 2^52 every product and every partial sum of Den is an
-// integer below 2^53, so FP64 fused multiply-adds are exact; otherwise the lag loop
-// accumulates in 128-bit integers.  Both give the same Den.
+// integer below 2^53, so FP64 fused multiply-adds are exact (den_fp64_kernel, register
+// lag windows); otherwise the lag loop accumulates in 128-bit integers.  Both give the
+// same Den.
 //
 // Bias correction (uint8 frames are stored as s = v - 128, padded pixels as -128):
 //   G(t,t') = U(t,t') + 128 (S(t) + S(t')) - 2^14 K_pad
@@ -146,6 +147,77 @@ __device__ __forceinline__ void den_lags(const int64_t* __restrict__ S, int nfra
     }
 }
 
+// Den(tau) = sum_t S(t) S(t+tau) in FP64 for the bins where that is exact (N M^2 < 2^52,
+// the others are left to finalize_kernel's 128-bit path).  Block (x, bin, z) covers the
+// lags tau0 = DEN_SPAN x + [0, DEN_SPAN) over `cpb` passes of DEN_CH frames from
+// t = DEN_CH cpb z; thread `tid` owns the DEN_W consecutive lags tau0 + DEN_W tid + j and
+// keeps the window S(t + lag .. lag + DEN_W - 1) in registers: each step over t reads one
+// new S value (plus the broadcast S(t)) for DEN_W FMAs, the window registers rotating with
+// the unrolled step.  The staged lag-side values sit at index i + i / 16 in LDS, so the
+// lanes' reads, DEN_W apart, fall on distinct banks.  Terms past the end of the series
+// read the zero padding, so one bound serves all lags of a thread.  The blocks' partial
+// sums go to den (zeroed first) by FP64 atomic adds: every product and partial sum is an
+// integer below 2^53, so neither the FMA order nor the order of the adds changes Den.
+constexpr int DEN_W = 16;                       // consecutive lags per thread
+constexpr int DEN_SPAN = FIN_THREADS * DEN_W;   // lags per block
+constexpr int DEN_CH = 1024;                    // frames staged per pass
+constexpr int DEN_SB = DEN_CH + DEN_SPAN + DEN_W;  // lag-side values staged per pass
+constexpr int DEN_BLOCKS = 2048;                // blocks aimed at per launch
+
+__device__ __forceinline__ int den_pad(int i) { return i + (i >> 4); }
+
+__global__ __launch_bounds__(FIN_THREADS) void den_fp64_kernel(const int64_t* __restrict__ sums,
+                                                               int nframes, int cpb,
+                                                               const int64_t* __restrict__ smax,
+                                                               double* __restrict__ den) {
+    __shared__ double sa[DEN_CH];
+    __shared__ double sb[DEN_SB + DEN_SB / 16 + 1];
+    const int b = blockIdx.y;
+    const int tau0 = blockIdx.x * DEN_SPAN;
+    const int tid = threadIdx.x;
+    const double m = (double)smax[b];
+    if (!((double)nframes * m * m < 4503599627370496.0)) return;  // 128-bit path
+    const int tb = blockIdx.z * cpb * DEN_CH;
+    const int te = min(nframes - tau0, tb + cpb * DEN_CH);
+    if (tb >= te) return;
+    const int64_t* S = sums + (int64_t)b * nframes;
+    const int lag = tau0 + DEN_W * tid;  // smallest lag of this thread
+    double acc[DEN_W];
+#pragma unroll
+    for (int j = 0; j < DEN_W; ++j) acc[j] = 0.0;
+    for (int t0 = tb; t0 < te; t0 += DEN_CH) {
+        __syncthreads();
+        for (int i = tid; i < DEN_CH; i += FIN_THREADS)
+            sa[i] = t0 + i < nframes ? (double)S[t0 + i] : 0.0;
+        for (int i = tid; i < DEN_SB; i += FIN_THREADS) {
+            const int64_t t = (int64_t)t0 + tau0 + i;
+            sb[den_pad(i)] = t < nframes ? (double)S[t] : 0.0;
+        }
+        __syncthreads();
+        int lim = nframes - lag - t0;
+        if (lim > DEN_CH) lim = DEN_CH;
+        const int base = DEN_W * tid;
+        double y[DEN_W];
+#pragma unroll
+        for (int j = 0; j < DEN_W - 1; ++j) y[j] = sb[den_pad(base + j)];
+        y[DEN_W - 1] = 0.0;
+        for (int k0 = 0; k0 < lim; k0 += DEN_W) {
+#pragma unroll
+            for (int st = 0; st < DEN_W; ++st) {
+                y[(st + DEN_W - 1) % DEN_W] = sb[den_pad(k0 + st + base + DEN_W - 1)];
+                const double x = sa[k0 + st];
+#pragma unroll
+                for (int j = 0; j < DEN_W; ++j)
+                    acc[j] = __builtin_fma(x, y[(st + j) % DEN_W], acc[j]);
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < DEN_W; ++j)
+        if (lag + j < nframes && acc[j] != 0.0)
+            atomicAdd(den + (int64_t)b * nframes + lag + j, acc[j]);
+}
+
 // sum_{t < N - tau} X(t) + sum_{t >= tau} X(t) from the exclusive prefix sums P of X.
 __device__ __forceinline__ int64_t edge_sums(const int64_t* P, int nframes, int tau) {
     return P[nframes - tau] + P[nframes] - P[tau];
@@ -160,7 +232,8 @@ __global__ __launch_bounds__(FIN_THREADS) void finalize_kernel(
     const int64_t* __restrict__ sums, const NumT* __restrict__ ugram,
     const BinDev* __restrict__ bins, int nframes, int mode,
     const int64_t* __restrict__ prefix, const int64_t* __restrict__ smax,
-    NumT* __restrict__ num, double* __restrict__ den, float* __restrict__ g2) {
+    const double* __restrict__ dpre, NumT* __restrict__ num, double* __restrict__ den,
+    float* __restrict__ g2) {
     __shared__ __attribute__((aligned(16))) unsigned char stage[(2 * FIN_CH + FIN_SPAN) * 8];
     const int b = blockIdx.y;
     const int tau0 = blockIdx.x * FIN_SPAN;
@@ -170,12 +243,12 @@ __global__ __launch_bounds__(FIN_THREADS) void finalize_kernel(
 
     double d[FIN_LAGS];
     const double m = (double)smax[b];
-    if ((double)nframes * m * m < 4503599627370496.0) {  // 2^52: FP64 sums stay exact
-        double acc[FIN_LAGS] = {};
-        den_lags<double, double>(S, nframes, tau0, reinterpret_cast<double*>(stage),
-                                 reinterpret_cast<double*>(stage) + FIN_CH, acc);
+    if ((double)nframes * m * m < 4503599627370496.0) {  // 2^52: den_fp64_kernel's Den
 #pragma unroll
-        for (int j = 0; j < FIN_LAGS; ++j) d[j] = acc[j];
+        for (int j = 0; j < FIN_LAGS; ++j) {
+            const int tau = tau0 + tid + FIN_THREADS * j;
+            d[j] = tau < nframes ? dpre[(int64_t)b * nframes + tau] : 0.0;
+        }
     } else {
         __int128 acc[FIN_LAGS] = {};
         den_lags<int64_t, __int128>(S, nframes, tau0, reinterpret_cast<int64_t*>(stage),
@@ -217,9 +290,24 @@ __global__ __launch_bounds__(FIN_THREADS) void finalize_kernel(
     }
 }
 
+static void launch_den_fp64(const int64_t* sums, int n_bins, int nframes, const int64_t* smax,
+                            double* dpre, hipStream_t s) {
+    launch_fill_i64(reinterpret_cast<int64_t*>(dpre), (int64_t)n_bins * nframes, s);  // +0.0
+    const int64_t lag_blocks = (nframes + DEN_SPAN - 1) / DEN_SPAN;
+    const int64_t chunks = (nframes + DEN_CH - 1) / DEN_CH;
+    // frame passes per block: about DEN_BLOCKS blocks in all (half of them, above the
+    // last lag of their chunk, return at once)
+    int64_t cpb = (lag_blocks * chunks * n_bins + DEN_BLOCKS - 1) / DEN_BLOCKS;
+    if (cpb < 1) cpb = 1;
+    const int64_t zb = (chunks + cpb - 1) / cpb;
+    den_fp64_kernel<<<dim3((unsigned)lag_blocks, (unsigned)n_bins, (unsigned)zb),
+                      dim3(FIN_THREADS), 0, s>>>(sums, nframes, (int)cpb, smax, dpre);
+}
+
 void launch_finalize(const int64_t* plane_sums, int64_t* sums, const int64_t* ugram,
                      const BinDev* bins, int n_bins, int nframes, int mode, int64_t* prefix,
-                     int64_t* smax, int64_t* num, double* den, float* g2, hipStream_t s) {
+                     int64_t* smax, double* dpre, int64_t* num, double* den, float* g2,
+                     hipStream_t s) {
     if (n_bins <= 0 || nframes <= 0) return;
     if (mode == 2)
         sums_prep_kernel<true><<<dim3((unsigned)n_bins), dim3(PREP_THREADS), 0, s>>>(
@@ -227,20 +315,22 @@ void launch_finalize(const int64_t* plane_sums, int64_t* sums, const int64_t* ug
     else
         sums_prep_kernel<false><<<dim3((unsigned)n_bins), dim3(PREP_THREADS), 0, s>>>(
             sums, sums, nframes, prefix, smax);
+    launch_den_fp64(sums, n_bins, nframes, smax, dpre, s);
     dim3 grid((unsigned)((nframes + FIN_SPAN - 1) / FIN_SPAN), (unsigned)n_bins);
-    finalize_kernel<int64_t><<<grid, dim3(FIN_THREADS), 0, s>>>(sums, ugram, bins, nframes, mode,
-                                                                prefix, smax, num, den, g2);
+    finalize_kernel<int64_t><<<grid, dim3(FIN_THREADS), 0, s>>>(
+        sums, ugram, bins, nframes, mode, prefix, smax, dpre, num, den, g2);
 }
 
 void launch_finalize_f64(const int64_t* sums, const double* fnum, const BinDev* bins, int n_bins,
-                         int nframes, int64_t* prefix, int64_t* smax, double* num, double* den,
-                         float* g2, hipStream_t s) {
+                         int nframes, int64_t* prefix, int64_t* smax, double* dpre, double* num,
+                         double* den, float* g2, hipStream_t s) {
     if (n_bins <= 0 || nframes <= 0) return;
     sums_prep_kernel<false><<<dim3((unsigned)n_bins), dim3(PREP_THREADS), 0, s>>>(
         sums, const_cast<int64_t*>(sums), nframes, prefix, smax);
+    launch_den_fp64(sums, n_bins, nframes, smax, dpre, s);
     dim3 grid((unsigned)((nframes + FIN_SPAN - 1) / FIN_SPAN), (unsigned)n_bins);
-    finalize_kernel<double><<<grid, dim3(FIN_THREADS), 0, s>>>(sums, fnum, bins, nframes, 0,
-                                                               prefix, smax, num, den, g2);
+    finalize_kernel<double><<<grid, dim3(FIN_THREADS), 0, s>>>(
+        sums, fnum, bins, nframes, 0, prefix, smax, dpre, num, den, g2);
 }
 
 }  // namespace dxg2
