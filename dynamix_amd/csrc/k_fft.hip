@@ -79,18 +79,52 @@ __global__ __launch_bounds__(256) void gather_pm_kernel(const uint8_t* __restric
     }
 }
 
-// S_b(t) = sum over the bin's pixels of xs[k][t]; grid (nrows / 256, n_bins).
+// S_b(t) = sum over the bin's pixels of xs[k][t].
+// grid (frame tiles of 256, bins, pixel chunks of PM_CH).  Thread (r, g) = (tid / 16,
+// tid % 16) reads 16 frames (16 bytes) of pixel rows chunk + r, r + 16, ... and adds
+// them into 16-bit lanes (even and odd bytes of each word apart: at most PM_CH / 16 rows
+// of 255, below 2^16); the 16 row groups are reduced in LDS and each frame's chunk total
+// goes to sums (zeroed first) with one 64-bit atomic add.
+constexpr int PM_CH = 1024;
+
 __global__ __launch_bounds__(256) void pm_sums_kernel(const uint8_t* __restrict__ xs,
                                                       const BinDev* __restrict__ bins, int nframes,
                                                       int nrows, int64_t* __restrict__ sums) {
+    __shared__ uint32_t part[16][256 + 1];
     const int b = blockIdx.y;
     const BinDev bd = bins[b];
-    const int t = blockIdx.x * 256 + threadIdx.x;
-    if (t >= nframes) return;
-    const uint8_t* base = xs + bd.slab_off + t;
-    int64_t s = 0;
-    for (int k = 0; k < bd.npix; ++k) s += base[(int64_t)k * nrows];
-    sums[(int64_t)b * nframes + t] = s;
+    const int k0 = blockIdx.z * PM_CH;
+    if (k0 >= bd.npix) return;
+    const int k1 = min(bd.npix, k0 + PM_CH);
+    const int t0 = blockIdx.x * 256;
+    const int r = threadIdx.x >> 4, g = threadIdx.x & 15;
+    uint32_t ev[4] = {0u, 0u, 0u, 0u}, od[4] = {0u, 0u, 0u, 0u};
+    const uint8_t* base = xs + bd.slab_off + t0 + 16 * g;
+    for (int k = k0 + r; k < k1; k += 16) {
+        const uint4 v = *reinterpret_cast<const uint4*>(base + (int64_t)k * nrows);
+        const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            ev[q] += w[q] & 0x00FF00FFu;
+            od[q] += (w[q] >> 8) & 0x00FF00FFu;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {  // frame 16 g + 4 q + {0, 1, 2, 3}
+        part[r][16 * g + 4 * q + 0] = ev[q] & 0xFFFFu;
+        part[r][16 * g + 4 * q + 1] = od[q] & 0xFFFFu;
+        part[r][16 * g + 4 * q + 2] = ev[q] >> 16;
+        part[r][16 * g + 4 * q + 3] = od[q] >> 16;
+    }
+    __syncthreads();
+    const int t = t0 + threadIdx.x;
+    if (t < nframes) {
+        uint32_t tot = 0;
+#pragma unroll
+        for (int i = 0; i < 16; ++i) tot += part[i][threadIdx.x];
+        if (tot) atomicAdd(reinterpret_cast<unsigned long long*>(sums + (int64_t)b * nframes + t),
+                           (unsigned long long)tot);
+    }
 }
 
 // --------------------------------------------------------------------- forward FFT
@@ -260,11 +294,13 @@ void launch_gather_pm(const uint8_t* frames, int dtype, int64_t hw, int nframes,
         frames, dtype, hw, nframes, nrows, pix_idx, bins, work, xs);
 }
 
-void launch_pm_sums(const uint8_t* xs, const BinDev* bins, int n_bins, int nframes, int nrows,
-                    int64_t* sums, hipStream_t s) {
-    if (n_bins <= 0) return;
-    pm_sums_kernel<<<dim3((unsigned)((nframes + 255) / 256), (unsigned)n_bins), 256, 0, s>>>(
-        xs, bins, nframes, nrows, sums);
+void launch_pm_sums(const uint8_t* xs, const BinDev* bins, int n_bins, int max_npix, int nframes,
+                    int nrows, int64_t* sums, hipStream_t s) {
+    if (n_bins <= 0 || max_npix <= 0) return;
+    launch_fill_i64(sums, (int64_t)n_bins * nframes, s);
+    pm_sums_kernel<<<dim3((unsigned)((nframes + 255) / 256), (unsigned)n_bins,
+                          (unsigned)((max_npix + PM_CH - 1) / PM_CH)),
+                     256, 0, s>>>(xs, bins, nframes, nrows, sums);
 }
 
 int launch_fft_power(int L, const uint8_t* xs, const BinDev* bins, const KbWork* work,
