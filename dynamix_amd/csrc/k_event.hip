@@ -27,8 +27,9 @@
 
 namespace dxg2 {
 
-constexpr int EW = 16384;       // lags per LDS window (128 KiB of 64-bit counters)
-constexpr int ECAP = 512;       // events of one pixel staged in LDS per wave
+constexpr int EW = 8192;        // lags per LDS window (64 KiB of 64-bit counters: two
+                                // workgroups per CU with the staging below)
+constexpr int ECAP = 256;       // events of one pixel staged in LDS per wave
 
 template <typename T>
 __device__ __forceinline__ long long as_count(T v) { return (long long)v; }
