@@ -259,6 +259,165 @@ __global__ __launch_bounds__(FftCfg<L>::THREADS) void fft_power_kernel(
     }
 }
 
+// L = 16384 (the largest series): radix-16 Stockham passes with the 16-point DFT in
+// registers, 16384 = 16 * 16 * 16 * 4, so four passes and eight barriers per series
+// instead of seven radix-4 passes.  Thread j of 1024 takes butterfly j of each radix-16
+// pass (16 complex values, 32 VGPRs) and four butterflies of the final radix-4 pass.
+// A pass with stride ns reads z[j + r L/R], multiplies by exp(-2 pi i r k / (R ns)),
+// k = j mod ns, and writes output q to z[(j - k) R + k + q ns].  The first pass writes
+// 16 apart, so LDS index i is stored at i + i/16: consecutive lanes then land on
+// distinct banks.  DFT-16 = 4 x 4: DFT-4 over r1 for each r0, twiddles W16^(r0 q0),
+// DFT-4 over r0 for each q0, output q0 + 4 q1.
+constexpr int F16_L = 16384;
+constexpr int F16_T = 1024;
+
+__device__ __forceinline__ int fpad(int i) { return i + (i >> 4); }
+
+__device__ __forceinline__ void dft4(c32& x0, c32& x1, c32& x2, c32& x3) {
+    const c32 s02 = cadd(x0, x2), d02 = csub(x0, x2);
+    const c32 s13 = cadd(x1, x3), d13 = cmuli_neg(csub(x1, x3));
+    x0 = cadd(s02, s13);
+    x1 = cadd(d02, d13);
+    x2 = csub(s02, s13);
+    x3 = csub(d02, d13);
+}
+
+// exp(-2 pi i m / 16) for m = r0 q0 in {0, 1, 2, 3, 4, 6, 9}
+__device__ __forceinline__ c32 w16(int m) {
+    constexpr float C1 = 0.92387953251128674f, S1 = 0.38268343236508978f;
+    constexpr float H = 0.70710678118654752f;
+    switch (m) {
+        case 1: return c32{C1, -S1};
+        case 2: return c32{H, -H};
+        case 3: return c32{S1, -C1};
+        case 4: return c32{0.f, -1.f};
+        case 6: return c32{-H, -H};
+        case 9: return c32{-C1, S1};
+        default: return c32{1.f, 0.f};
+    }
+}
+
+// in place, transposed: afterwards a[4 q0 + q1] = sum_r a[r] exp(-2 pi i r q / 16) for
+// q = q0 + 4 q1 (the caller stores slot 4 q0 + q1 as output q0 + 4 q1)
+__device__ __forceinline__ void dft16(c32 (&a)[16]) {
+#pragma unroll
+    for (int r0 = 0; r0 < 4; ++r0) dft4(a[r0], a[4 + r0], a[8 + r0], a[12 + r0]);
+    // now a[4 q0 + r0] holds b_{r0}[q0]
+#pragma unroll
+    for (int r0 = 1; r0 < 4; ++r0)
+#pragma unroll
+        for (int q0 = 1; q0 < 4; ++q0) a[4 * q0 + r0] = cmul(a[4 * q0 + r0], w16(r0 * q0));
+#pragma unroll
+    for (int q0 = 0; q0 < 4; ++q0) dft4(a[4 * q0], a[4 * q0 + 1], a[4 * q0 + 2], a[4 * q0 + 3]);
+}
+
+// one radix-16 pass of stride NS (1, 16 or 256) for butterfly j = threadIdx.x
+template <int NS>
+__device__ __forceinline__ void pass16(c32* z, const float2* __restrict__ tw) {
+    const int j = threadIdx.x;
+    const int k = j & (NS - 1);
+    c32 a[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        c32 v = z[fpad(j + r * (F16_L / 16))];
+        if (NS > 1 && r) {
+            const float2 t = tw[(r * k * (F16_L / (16 * NS))) & (F16_L - 1)];
+            v = cmul(v, c32{t.x, t.y});
+        }
+        a[r] = v;
+    }
+    dft16(a);
+    __syncthreads();  // every read of this pass is done
+    const int o = (j - k) * 16 + k;
+#pragma unroll
+    for (int q0 = 0; q0 < 4; ++q0)
+#pragma unroll
+        for (int q1 = 0; q1 < 4; ++q1) z[fpad(o + (q0 + 4 * q1) * NS)] = a[4 * q0 + q1];
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(F16_T) void fft16_power_kernel(
+    const uint8_t* __restrict__ xs, const BinDev* __restrict__ bins,
+    const KbWork* __restrict__ work, int nframes, int nrows, const float2* __restrict__ tw,
+    double* __restrict__ power) {
+    __shared__ c32 z[F16_L + F16_L / 16];
+    constexpr int L = F16_L;
+    constexpr int NK = (L + 1 + F16_T - 1) / F16_T;  // power bins per thread
+    const KbWork w = work[blockIdx.x];
+    const BinDev bd = bins[w.bin];
+    const int tid = threadIdx.x;
+    double acc[NK];
+#pragma unroll
+    for (int m = 0; m < NK; ++m) acc[m] = 0.0;
+
+    const int kend = min(KB, bd.npix - w.kb * KB);
+    for (int jp = 0; jp < kend; ++jp) {
+        const uint8_t* x = xs + bd.slab_off + ((int64_t)w.kb * KB + jp) * nrows;
+        // z[n] = x[2n] + i x[2n+1] (zero past the last frame)
+#pragma unroll
+        for (int m = 0; m < L / F16_T; ++m) {
+            const int n = tid + m * F16_T;
+            const int t = 2 * n;
+            float re = 0.f, im = 0.f;
+            if (t + 1 < nrows) {
+                const uint16_t two = *reinterpret_cast<const uint16_t*>(x + t);
+                re = t < nframes ? (float)(two & 0xFF) : 0.f;
+                im = t + 1 < nframes ? (float)(two >> 8) : 0.f;
+            }
+            z[fpad(n)] = c32{re, im};
+        }
+        __syncthreads();
+        pass16<1>(z, tw);
+        pass16<16>(z, tw);
+        pass16<256>(z, tw);
+        // radix-4 pass of stride 4096: butterfly jj reads and writes z[jj + 4096 r] (jj <
+        // 4096), so each thread works in place on its own four points
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            const int jj = tid + m * F16_T;
+            c32 v[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                c32 a = z[fpad(jj + r * (L / 4))];
+                if (r) {
+                    const float2 t = tw[(r * jj) & (L - 1)];
+                    a = cmul(a, c32{t.x, t.y});
+                }
+                v[r] = a;
+            }
+            dft4(v[0], v[1], v[2], v[3]);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) z[fpad(jj + r * (L / 4))] = v[r];
+        }
+        __syncthreads();
+        // unpack X[k] = (Z[k] + conj Z[L-k]) / 2 - i W^k (Z[k] - conj Z[L-k]) / 2,
+        // W = exp(-pi i / L); accumulate |X[k]|^2
+#pragma unroll
+        for (int m = 0; m < NK; ++m) {
+            const int k = tid + m * F16_T;
+            if (k <= L) {
+                const c32 a = z[fpad(k & (L - 1))];
+                const c32 b = z[fpad((L - k) & (L - 1))];
+                const c32 s2 = c32{a.x + b.x, a.y - b.y};
+                const c32 d = c32{a.x - b.x, a.y + b.y};
+                float sn, cs;
+                sincospif((float)k / (float)L, &sn, &cs);
+                const c32 wd = cmul(c32{cs, -sn}, d);
+                const float xr = 0.5f * (s2.x + wd.y);
+                const float xi = 0.5f * (s2.y - wd.x);
+                acc[m] += (double)xr * xr + (double)xi * xi;
+            }
+        }
+        __syncthreads();
+    }
+    double* prow = power + (int64_t)w.bin * (L + 1);
+#pragma unroll
+    for (int m = 0; m < NK; ++m) {
+        const int k = tid + m * F16_T;
+        if (k <= L && acc[m] != 0.0) atomicAdd(prow + k, acc[m]);
+    }
+}
+
 // Num(tau) from the power spectrum, FP64; grid (ceil(N / 256), n_bins).  Every
 // thread reads the same ps[k] in step, so the spectrum is read as uniform loads.
 __global__ __launch_bounds__(256) void fft_inverse_kernel(const double* __restrict__ power,
@@ -317,7 +476,10 @@ int launch_fft_power(int L, const uint8_t* xs, const BinDev* bins, const KbWork*
         DXG2_FFT_CASE(2048)
         DXG2_FFT_CASE(4096)
         DXG2_FFT_CASE(8192)
-        DXG2_FFT_CASE(16384)
+        case F16_L:
+            fft16_power_kernel<<<dim3((unsigned)n_work), F16_T, 0, s>>>(xs, bins, work, nframes,
+                                                                       nrows, tw, power);
+            return 0;
         default:
             return -1;
     }
