@@ -184,6 +184,48 @@ def test_dense_reference_goldens(golden):
         _check_exact(c, g2, og.dense_g2(f, q))
 
 
+@pytest.mark.parametrize("dtype", [np.uint8, np.int8])
+def test_dense_band_gather_and_uncovered_kblocks(dtype):
+    """A 1030-wide detector makes 8-row bands; 40 interleaved bins give k-blocks that
+    span one, two or more rows, so both the band workgroups and the general-gather
+    workgroups of the launch run; 37 frames end inside an 8-frame stage."""
+    _announce("dense band gather %s" % np.dtype(dtype).name)
+    rng = np.random.default_rng(37)
+    shape, n = (70, 1030), 37
+    f = _speckle(37, n, shape, 0.8)
+    if dtype == np.int8:
+        f = (f.astype(np.int16) - 2).clip(-128, 127).astype(np.int8)
+    q = rng.integers(0, 41, size=shape).astype(np.int32)
+    for b in range(1, 41):
+        q.flat[b] = b
+    c, g2 = _run(f, q)
+    _check_exact(c, g2, og.dense_g2(f, q))
+
+
+def test_dense_unaligned_frames_and_wide_detector():
+    """Frames whose device address is not 16-byte aligned take the general gather; a
+    detector too wide for an 8-frame band stage has no band work.  Both match the
+    oracle."""
+    import torch
+    _announce("dense unaligned / wide")
+    from dynamix_amd.correlator.dense import MatMulCorrelator
+    shape, n = (30, 1030), 21
+    f = _speckle(21, n, shape, 0.5)
+    q = np.random.default_rng(21).integers(0, 7, size=shape).astype(np.int32)
+    buf = torch.zeros(f.size + 1, dtype=torch.uint8, device="cuda")
+    buf[1:] = torch.from_numpy(f.ravel()).cuda()
+    view = buf[1:].view(n, *shape)
+    assert view.data_ptr() % 16 != 0
+    c = MatMulCorrelator(shape, n, q)
+    g2 = c.correlate(view)
+    _check_exact(c, g2, og.dense_g2(f, q))
+    wide, nw = (3, 6000), 19
+    fw = _speckle(19, nw, wide, 0.5)
+    qw = np.random.default_rng(19).integers(0, 4, size=wide).astype(np.int32)
+    c, g2 = _run(fw, qw)
+    _check_exact(c, g2, og.dense_g2(fw, qw))
+
+
 def test_dense_device_input_and_repeat_calls():
     """A GPU tensor is borrowed; repeated calls give identical results (sums and
     numerators are re-zeroed each call, unlike the event reference's res_int)."""
