@@ -125,6 +125,7 @@ __global__ __launch_bounds__(256) void std_tile_kernel(const StdTile* __restrict
     constexpr int NDIAG = 2 * T - 1;
     constexpr int SPT = (NDIAG + 255) / 256;  // diagonals per thread
     __shared__ double x[BR][T];         // ratio, NaN where the element is not counted
+    __shared__ int64_t rowS[BR], rowC[BR];  // S(t), c(t) of the band's rows
 
     const StdTile tl = tiles[blockIdx.x];
     const BinDev bd = bins[tl.bin];
@@ -142,6 +143,7 @@ __global__ __launch_bounds__(256) void std_tile_kernel(const StdTile* __restrict
     if (u < nframes) frame_terms(sums, mode, n_bins, nframes, tl.bin, u, S_u, c_u);
     const int64_t TT = (int64_t)T * T;
     const int* base = scratch + (int64_t)tl.slot0 * TT;
+    const int nk = tl.n_hh + tl.n_ll + tl.n_x;
 
     int cnt[SPT];
     double k0[SPT], s1[SPT], s2[SPT];
@@ -153,35 +155,55 @@ __global__ __launch_bounds__(256) void std_tile_kernel(const StdTile* __restrict
 
     for (int r0 = 0; r0 < T && t0 + r0 < nframes; r0 += BR) {
         __syncthreads();
+        if (tid < BR) {
+            int64_t S_t = 0, c_t = 0;
+            if (t0 + r0 + tid < nframes)
+                frame_terms(sums, mode, n_bins, nframes, tl.bin, t0 + r0 + tid, S_t, c_t);
+            rowS[tid] = S_t;
+            rowC[tid] = c_t;
+        }
+        __syncthreads();
+        // the band's rows in batches of 8, so that their tile loads are in flight together
+#pragma unroll 8
         for (int rr = rsub; rr < BR; rr += RPP) {
             const int r = r0 + rr;
             const int t = t0 + r;
             double val = __builtin_nan("");
-            if (t < nframes && u < nframes && (!diag || c >= r)) {
-                int64_t S_t, c_t;
-                frame_terms(sums, mode, n_bins, nframes, tl.bin, t, S_t, c_t);
-                if (S_t != 0 && S_u != 0) {
-                    const int* e = base + (int64_t)r * T + c;
-                    int64_t g = c_t + c_u - Kc;
+            const int64_t S_t = rowS[rr];
+            if (t < nframes && u < nframes && (!diag || c >= r) && S_t != 0 && S_u != 0) {
+                const int* e = base + (int64_t)r * T + c;
+                int64_t g = rowC[rr] + c_u - Kc;
+                if (nk == 1) {
+                    g += e[0];  // uint8 / int8 with one k-chunk: the common case
+                } else {
                     int k = 0;
                     for (; k < tl.n_hh; ++k) g += (int64_t)w_hh * e[k * TT];
                     for (; k < tl.n_hh + tl.n_ll; ++k) g += e[k * TT];
-                    for (; k < tl.n_hh + tl.n_ll + tl.n_x; ++k) g += (int64_t)256 * e[k * TT];
-                    val = npix * (double)g / ((double)S_t * (double)S_u);
+                    for (; k < nk; ++k) g += (int64_t)256 * e[k * TT];
                 }
+                val = npix * (double)g / ((double)S_t * (double)S_u);
             }
             x[rr][c] = val;
         }
         __syncthreads();
+        // Each thread's diagonals walk the band's rows in order (the statistics depend on
+        // that order, which is fixed); the SPT diagonals of a thread advance together.
+        int lo[SPT], hi[SPT], dd[SPT];
+        int rmin = BR, rmax = 0;
 #pragma unroll
         for (int j = 0; j < SPT; ++j) {
             const int s = tid + 256 * j;
-            if (s < NDIAG) {
-                const int d = s - (T - 1);
-                const int lo = max(0, -d - r0);
-                const int hi = min(BR, T - d - r0);
-                for (int rr = lo; rr < hi; ++rr) {
-                    const double v = x[rr][r0 + rr + d];
+            dd[j] = s - (T - 1);
+            lo[j] = s < NDIAG ? max(0, -dd[j] - r0) : BR;
+            hi[j] = s < NDIAG ? min(BR, T - dd[j] - r0) : 0;
+            rmin = min(rmin, lo[j]);
+            rmax = max(rmax, hi[j]);
+        }
+        for (int rr = rmin; rr < rmax; ++rr) {
+#pragma unroll
+            for (int j = 0; j < SPT; ++j) {
+                if (rr >= lo[j] && rr < hi[j]) {
+                    const double v = x[rr][r0 + rr + dd[j]];
                     if (v == v) {
                         if (cnt[j] == 0) k0[j] = v;
                         const double dv = v - k0[j];
