@@ -152,11 +152,10 @@ struct GramTile {
         const int64_t kstride = (int64_t)(nrows >> 5) * 2048;
         const int8_t* pa = slab_bin + (int64_t)((row0 >> 5) + wr * WM) * 2048 + lane * 16;
         const int8_t* pb = slab_bin + (int64_t)((col0 >> 5) + wc * WN) * 2048 + lane * 16;
-        // idle: every block of this wave is below the diagonal, or (outside the calc_std
-        // store) lies wholly in the padding frames past the last one
-        const bool idle = (DIAG && wr > wc) ||
-                          (!STORE && (row0 + wr * WM * 32 >= nframes ||
-                                      col0 + wc * WN * 32 >= nframes));
+        // idle: every block of this wave is below the diagonal, or lies wholly in the
+        // padding frames past the last one (a STORE tile's readers skip both parts)
+        const bool idle = (DIAG && wr > wc) || row0 + wr * WM * 32 >= nframes ||
+                          col0 + wc * WN * 32 >= nframes;
         if (!idle && kb0 < kb1) {
             const int last = kb1 - 1;
             load(0, kb0, pa, pb, kstride);
