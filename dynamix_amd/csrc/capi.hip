@@ -221,7 +221,10 @@ struct dxg2_plan {
     int8_t* d_slab = nullptr;
     int64_t* d_ugram = nullptr;
     int64_t* d_sums = nullptr;
-    int64_t* d_psums = nullptr;   // digit planes: S_H rows then S_L rows (n_bins each)
+    int64_t* d_psums = nullptr;   // digit planes: S_H, S_L, then S_Lw rows (n_bins each)
+    uint8_t* d_wflag = nullptr;   // uint16: per detector pixel, a count > 255 in some frame
+    int32_t* d_pixw = nullptr;    // uint16: the pixel lists reordered wide-first
+    int* d_nwkb = nullptr;        // uint16: per bin, k-blocks holding the wide pixels
     int64_t* d_prefix = nullptr;  // per bin and plane: prefix sums of S(t), N + 1 values
     int64_t* d_smax = nullptr;    // per bin: max |S(t)|
     double* d_dpre = nullptr;     // per bin and lag: FP64 Den(tau) before the finalise
@@ -382,7 +385,10 @@ void build_gram_work(dxg2_plan* p) {
         } else {  // digit planes: H H^T, L L^T, [H|L][L|H]^T
             triangle(b, 0, nkb, b, 0);
             triangle(b, nkb, 2 * nkb, nb + b, 0);
-            triangle(b, 0, 2 * nkb, 2 * nb + b, 1);
+            // the cross items of each half apart, so that an item clipped to the wide
+            // k-blocks keeps one k-range
+            triangle(b, 0, nkb, 2 * nb + b, 1);
+            triangle(b, nkb, 2 * nkb, 2 * nb + b, 1);
         }
     }
     GramWork pad{-1, 0, 0, 0, 0, 0, 0, 0, 0, 0};  // job 0
@@ -668,7 +674,11 @@ int ensure_dense_buffers(dxg2_plan* p, hipStream_t s) {
 // Clear the accumulators of a dense call and gather the frames into the slab.  Returns
 // the frame-sum rows the Gram's diagonal tiles add into: the caller's sums (uint8/int8)
 // or the plan's digit-plane sums S_H, S_L (uint16).
-int64_t* dense_gather(dxg2_plan* p, const void* d_frames, int64_t* d_sums, hipStream_t s) {
+// wide_first (uint16 plans; the plain dense calls): the bins' pixels go into the slab
+// wide-first (k_wide.hip), for a Gram clipped to the wide k-blocks (dense_gram and
+// dense_finish with the same flag).
+int64_t* dense_gather(dxg2_plan* p, const void* d_frames, int64_t* d_sums, hipStream_t s,
+                      bool wide_first = false) {
     int64_t* sums = d_sums ? d_sums : p->d_sums;
     const bool wide = p->dtype == DXG2_U16;
     int64_t* gsums = wide ? p->d_psums : sums;
@@ -676,46 +686,58 @@ int64_t* dense_gather(dxg2_plan* p, const void* d_frames, int64_t* d_sums, hipSt
     launch_fill_i64(gsums, (wide ? 2 : 1) * nbn, s);
     launch_fill_i64(p->d_ugram, (wide ? 3 : 1) * nbn, s);
     const int64_t hw = (int64_t)p->H * p->W;
+    const int32_t* pix = p->d_pix;
+    if (wide && wide_first) {
+        launch_wide_first((const uint16_t*)d_frames, hw, p->N, p->d_pix, p->d_bins, p->n_bins,
+                          p->d_wflag, p->d_pixw, p->d_nwkb, s);
+        pix = p->d_pixw;
+    }
     if (!p->gband.empty() && reinterpret_cast<uintptr_t>(d_frames) % 16 == 0) {
         launch_gather_band((const uint8_t*)d_frames, p->dtype, hw, p->W, p->N, p->nrows,
-                           p->gband_srow, p->d_pix, p->d_bins, p->d_gband, (int)p->gband.size(),
+                           p->gband_srow, pix, p->d_bins, p->d_gband, (int)p->gband.size(),
                            p->d_gband_kb, p->d_gfall, (int)p->gfall.size(), p->d_slab, s);
     } else {
-        launch_gather_i8((const uint8_t*)d_frames, p->dtype, hw, p->N, p->nrows, p->d_pix,
+        launch_gather_i8((const uint8_t*)d_frames, p->dtype, hw, p->N, p->nrows, pix,
                          p->d_bins, p->d_gwork, (int)p->gather_work.size(), p->d_slab, s);
     }
+    if (wide && wide_first)
+        launch_wide_low_sums(p->d_slab, p->d_bins, p->d_nwkb, p->n_bins, p->N, p->nrows,
+                             gsums + 2 * nbn, s);
     return gsums;
 }
 
 // The Gram of every bin: the persistent launch over the per-XCD work queues (heads
 // cleared by a fill kernel, so the sequence can be captured in a graph), or one
 // workgroup per tile.
-void dense_gram(dxg2_plan* p, int64_t* gsums, hipStream_t s) {
+void dense_gram(dxg2_plan* p, int64_t* gsums, hipStream_t s, bool wide_first = false) {
     const int biased = p->dtype == DXG2_I8 ? 0 : 1;
+    const int* nwkb = wide_first && p->dtype == DXG2_U16 ? p->d_nwkb : nullptr;
     if (p->use_queue) {
         launch_fill_i32(p->d_heads, 16, s);
         launch_gram_i8_queue(p->gram_tile, false, p->d_slab, p->d_bins, p->d_qmwork,
                              (int)p->qm_work.size(), p->d_qoff, p->d_heads, p->N, p->nrows,
-                             biased, p->d_ugram, gsums, s);
+                             biased, nwkb, p->d_ugram, gsums, s);
         launch_gram_i8_queue(p->gram_tile, true, p->d_slab, p->d_bins, p->d_qxwork,
                              (int)p->qx_work.size(), p->d_qoff + 9, p->d_heads + 8, p->N,
-                             p->nrows, biased, p->d_ugram, gsums, s);
+                             p->nrows, biased, nwkb, p->d_ugram, gsums, s);
     } else {
         launch_gram_i8(p->gram_tile, false, p->d_slab, p->d_bins, p->d_mwork,
-                       (int)p->gram_work.size(), p->N, p->nrows, biased, p->d_ugram, gsums, s);
+                       (int)p->gram_work.size(), p->N, p->nrows, biased, nwkb, p->d_ugram, gsums,
+                       s);
         launch_gram_i8(p->gram_tile, true, p->d_slab, p->d_bins, p->d_xwork,
-                       (int)p->gram_xwork.size(), p->N, p->nrows, biased, p->d_ugram, gsums, s);
+                       (int)p->gram_xwork.size(), p->N, p->nrows, biased, nwkb, p->d_ugram, gsums,
+                       s);
     }
 }
 
 // Bias corrections, exact denominators and g2 after the Gram.
 void dense_finish(dxg2_plan* p, int64_t* gsums, int64_t* d_num, int64_t* d_sums, double* d_den,
-                  float* d_g2, hipStream_t s) {
+                  float* d_g2, hipStream_t s, bool wide_first = false) {
     int64_t* sums = d_sums ? d_sums : p->d_sums;
     const bool wide = p->dtype == DXG2_U16;
-    launch_finalize(gsums, sums, p->d_ugram, p->d_bins, p->n_bins, p->N,
-                    wide ? 2 : (p->dtype == DXG2_U8 ? 1 : 0), p->d_prefix, p->d_smax, p->d_dpre,
-                    d_num, d_den, d_g2, s);
+    const int mode = wide ? (wide_first ? 3 : 2) : (p->dtype == DXG2_U8 ? 1 : 0);
+    launch_finalize(gsums, sums, p->d_ugram, p->d_bins, p->n_bins, p->N, mode, p->d_prefix,
+                    p->d_smax, p->d_dpre, wide ? p->d_nwkb : nullptr, d_num, d_den, d_g2, s);
 }
 
 }  // namespace
@@ -845,7 +867,9 @@ int dxg2_plan_create(const dxg2_plan_desc* desc, dxg2_plan** out) {
                           (p->gband_kb.size() + 1) * sizeof(int2) +
                           (p->gram_work.size() + p->gram_xwork.size() + 2) * sizeof(GramWork) +
                           (p->qm_work.size() + p->qx_work.size() + 2) * sizeof(GramWork) +
-                          sizeof(p->q_off) + 9 * 256;
+                          sizeof(p->q_off) +
+                          (slab_planes(p->dtype) == 2 ? p->pix_idx.size() * 4 : 0) +
+                          16 * 256;  // each of the (at most 16) uploads starts 256-aligned
     if (sc.cap < staged) {
         if (sc.staging) {
             (void)hipStreamSynchronize(s0);
@@ -902,9 +926,16 @@ int dxg2_plan_create(const dxg2_plan_desc* desc, dxg2_plan** out) {
         rc = set_error(DXG2_ENOMEM, "cannot allocate numerator workspace");
     if (!rc && dev_alloc(p->alloc, (void**)&p->d_sums, nb_n, s0) != hipSuccess)
         rc = set_error(DXG2_ENOMEM, "cannot allocate sums workspace");
-    if (!rc && planes == 2 && dev_alloc(p->alloc, (void**)&p->d_psums, 2 * nb_n, s0) != hipSuccess)
+    if (!rc && planes == 2 && dev_alloc(p->alloc, (void**)&p->d_psums, 3 * nb_n, s0) != hipSuccess)
         rc = set_error(DXG2_ENOMEM, "cannot allocate digit-plane sums workspace");
-    if (!rc && (dev_alloc(p->alloc, (void**)&p->d_prefix, (size_t)planes * std::max(p->n_bins, 1) *
+    if (!rc && planes == 2 &&
+        (dev_alloc(p->alloc, (void**)&p->d_wflag, (size_t)p->H * p->W + 16, s0) != hipSuccess ||
+         dev_alloc(p->alloc, (void**)&p->d_nwkb, (size_t)std::max(p->n_bins, 1) * sizeof(int),
+                   s0) != hipSuccess))
+        rc = set_error(DXG2_ENOMEM, "cannot allocate wide-pixel workspace");
+    if (!rc && planes == 2) rc = upload((void**)&p->d_pixw, p->pix_idx.data(), p->pix_idx.size() * 4);
+    if (!rc && (dev_alloc(p->alloc, (void**)&p->d_prefix, (size_t)(planes == 2 ? 3 : 1) *
+                                                     std::max(p->n_bins, 1) *
                                                      (p->N + 1) * sizeof(int64_t), s0) !=
                     hipSuccess ||
                 dev_alloc(p->alloc, (void**)&p->d_smax, (size_t)std::max(p->n_bins, 1) * sizeof(int64_t),
@@ -950,7 +981,8 @@ static void plan_release(dxg2_plan* p, hipStream_t s) {
                     p->d_etimes, p->d_edata, p->d_kbwork, p->d_xs, p->d_tw,
                     p->d_power, p->d_fnum, p->d_qmwork, p->d_qxwork, p->d_qoff,
                     p->d_heads, p->d_gband, p->d_gband_kb, p->d_gfall,
-                    p->coo_sort.keys, p->coo_sort.vals0, p->coo_sort.vals1, p->coo_sort.tmp};
+                    p->coo_sort.keys, p->coo_sort.vals0, p->coo_sort.vals1, p->coo_sort.tmp,
+                    p->d_wflag, p->d_pixw, p->d_nwkb};
     drop_std(p, s);
     for (void* b : bufs) dev_free(p->alloc, b, s);
     for (auto& q : p->ev_pool)
@@ -987,8 +1019,12 @@ int dxg2_plan_footprint(const dxg2_plan* p, int32_t path, size_t* bytes) {
     const size_t nb_n = (size_t)std::max(p->n_bins, 1) * p->N * sizeof(int64_t);
     const int planes = slab_planes(p->dtype);
     size_t common = p->pix_idx.size() * 4 + p->bins.size() * sizeof(BinDev) +
-                    (planes == 2 ? 7 : 3) * nb_n +
-                    (size_t)std::max(p->n_bins, 1) * (planes * (p->N + 1) + 1) * sizeof(int64_t);
+                    (planes == 2 ? 8 : 3) * nb_n +
+                    (size_t)std::max(p->n_bins, 1) * ((planes == 2 ? 3 : 1) * (p->N + 1) + 1) *
+                        sizeof(int64_t) +
+                    (planes == 2 ? p->pix_idx.size() * 4 + (size_t)p->H * p->W +
+                                       (size_t)std::max(p->n_bins, 1) * 4
+                                 : 0);
     switch (path) {
         case 0:
             *bytes = common + p->slab_bytes + p->gather_work.size() * sizeof(GatherWork) +
@@ -1065,12 +1101,11 @@ int dxg2_correlate_dense(dxg2_plan* p, const void* d_frames, int64_t* d_num, int
     hipEvent_t* ev = nullptr;
     if (p->profiling && (rc = next_events(p, &ev))) return rc;
     if (ev) DXG2_HIP(hipEventRecord(ev[0], s));
-    int64_t* gsums = dense_gather(p, d_frames, d_sums, s);
+    int64_t* gsums = dense_gather(p, d_frames, d_sums, s, true);
     if (ev) DXG2_HIP(hipEventRecord(ev[1], s));
-    const int biased = p->dtype == DXG2_I8 ? 0 : 1;
-    dense_gram(p, gsums, s);
+    dense_gram(p, gsums, s, true);
     if (ev) DXG2_HIP(hipEventRecord(ev[2], s));
-    dense_finish(p, gsums, d_num, d_sums, d_den, d_g2, s);
+    dense_finish(p, gsums, d_num, d_sums, d_den, d_g2, s, true);
     if (ev) DXG2_HIP(hipEventRecord(ev[3], s));
     DXG2_HIP(hipGetLastError());
     return DXG2_OK;
@@ -1137,9 +1172,9 @@ int dxg2_correlate_dense_graph(dxg2_plan* p, const void* d_frames, int64_t* d_nu
         gram_queue_prepare(p->gram_tile);  // device queries outside the capture
         (void)hipGetLastError();
         DXG2_HIP(hipStreamBeginCapture(p->cap_stream, hipStreamCaptureModeThreadLocal));
-        int64_t* gsums = dense_gather(p, d_frames, d_sums, p->cap_stream);
-        dense_gram(p, gsums, p->cap_stream);
-        dense_finish(p, gsums, d_num, d_sums, d_den, d_g2, p->cap_stream);
+        int64_t* gsums = dense_gather(p, d_frames, d_sums, p->cap_stream, true);
+        dense_gram(p, gsums, p->cap_stream, true);
+        dense_finish(p, gsums, d_num, d_sums, d_den, d_g2, p->cap_stream, true);
         hipGraph_t g = nullptr;
         const hipError_t e1 = hipStreamEndCapture(p->cap_stream, &g);
         const hipError_t e0 = hipGetLastError();
@@ -1420,7 +1455,7 @@ static int events_from_csr(dxg2_plan* p, const int32_t* times, const void* data,
     launch_event_pairs(p->dtype, times, data, offsets, p->d_pix, p->d_bins, p->d_ework,
                        (int)p->event_work.size(), p->N, nnz, p->d_ugram, sums, p->d_err, s);
     launch_finalize(sums, sums, p->d_ugram, p->d_bins, p->n_bins, p->N, 0, p->d_prefix,
-                    p->d_smax, p->d_dpre, d_num, d_den, d_g2, s);
+                    p->d_smax, p->d_dpre, nullptr, d_num, d_den, d_g2, s);
     DXG2_HIP(hipGetLastError());
     return DXG2_OK;
 }

@@ -130,15 +130,25 @@ void launch_gather_band(const uint8_t* frames, int dtype, int64_t hw, int width,
                         const BandWork* bands, int n_bands, const int2* band_kb,
                         const GatherWork* fall, int n_fall, int8_t* slab, hipStream_t s);
 // cross: the work items are cross-plane ([H|L] [L|H]^T) items of uint16 frames.
+// nwkb (uint16 slabs filled wide-first, else null): per bin, the k-blocks the HH and
+// cross items are clipped to.
 void launch_gram_i8(int tile, bool cross, const int8_t* slab, const BinDev* bins,
                     const GramWork* work, int n_work, int nframes, int nrows, int biased,
-                    int64_t* ugram, int64_t* sums, hipStream_t s);
+                    const int* nwkb, int64_t* ugram, int64_t* sums, hipStream_t s);
 // Persistent launch over per-XCD work queues: queue x is items [qoff[x], qoff[x+1]) of
 // `work`; heads (8 ints) must be zero.
 void launch_gram_i8_queue(int tile, bool cross, const int8_t* slab, const BinDev* bins,
                           const GramWork* work, int n_work, const int* qoff, int* heads,
-                          int nframes, int nrows, int biased, int64_t* ugram, int64_t* sums,
-                          hipStream_t s);
+                          int nframes, int nrows, int biased, const int* nwkb, int64_t* ugram,
+                          int64_t* sums, hipStream_t s);
+// uint16 wide pixels first (k_wide.hip): flags of the pixels with a count > 255 in any
+// frame, each bin's pixel list reordered wide-first into pix_w, and nwkb per bin.
+void launch_wide_first(const uint16_t* frames, int64_t hw, int nframes, const int32_t* pix,
+                       const BinDev* bins, int n_bins, uint8_t* flag, int32_t* pix_w, int* nwkb,
+                       hipStream_t s);
+// Low-byte frame sums over each bin's first nwkb k-blocks of the low plane.
+void launch_wide_low_sums(const int8_t* slab, const BinDev* bins, const int* nwkb, int n_bins,
+                          int nframes, int nrows, int64_t* lsums, hipStream_t s);
 // Device queries of the persistent launch's grid size, done once per device (outside
 // any stream capture).
 void gram_queue_prepare(int tile);
@@ -150,8 +160,8 @@ void launch_gram_i8_jobs(int tile, bool cross, const GramJob* jobs, const GramWo
 // receives S = 256 S_H + S_L).
 void launch_finalize(const int64_t* plane_sums, int64_t* sums, const int64_t* ugram,
                      const BinDev* bins, int n_bins, int nframes, int mode, int64_t* prefix,
-                     int64_t* smax, double* dpre, int64_t* num, double* den, float* g2,
-                     hipStream_t s);
+                     int64_t* smax, double* dpre, const int* nwkb, int64_t* num, double* den,
+                     float* g2, hipStream_t s);
 void launch_plain_bin(const uint8_t* frames, int dtype, int64_t hw, int nframes,
                       const int32_t* pix, int npix, float* a, float* gram, float* means,
                       float* g2_row, hipStream_t s);

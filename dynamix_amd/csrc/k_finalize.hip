@@ -21,7 +21,8 @@
 // summed over t = 0..N-1-tau:
 //   Num(tau) = U(tau) + 128 (P(N-tau) + P(N) - P(tau)) - 2^14 K_pad (N-tau).
 // uint16 frames v = 256 h + l are stored as two biased digit planes; with the plane
-// sums S_H, S_L (prefix sums P_H, P_L) the same identity gives
+// sums S_H, S_L (prefix sums P_H, P_L) the same identity gives (mode 2; mode 3, below,
+// when HH and X are clipped to the wide k-blocks)
 //   HH = U_HH + 128 E_H - 2^14 K_pad (N-tau),  LL = U_LL + 128 E_L - 2^14 K_pad (N-tau),
 //   X  = U_X + 128 (E_H + E_L) - 2^15 K_pad (N-tau),   E = P(N-tau) + P(N) - P(tau),
 // and Num = 65536 HH + 256 X + LL (formed in 128 bits), S = 256 S_H + S_L.
@@ -63,23 +64,27 @@ __device__ __forceinline__ int64_t block_scan(int64_t v, int64_t* wave_tot, int6
     return before + x - v;
 }
 
-// One block per bin.  S(t) are the rows of plane_sums; WIDE (uint16 digit planes): rows
-// b (S_H) and n_bins + b (S_L), combined into sums[b][t] = 256 S_H + S_L.  Writes the
-// exclusive prefix sums of each plane, prefix[plane * n_bins + b][t] for t = 0..N, and
-// smax[b] = max |S|.
-template <bool WIDE>
+// One block per bin.  S(t) are the rows of plane_sums; PLANES >= 2 (uint16 digit
+// planes): rows b (S_H) and n_bins + b (S_L), combined into sums[b][t] = 256 S_H + S_L;
+// PLANES = 3 adds rows 2 n_bins + b (S_Lw, the low bytes over the wide k-blocks, which
+// only the bias correction uses).  Writes the exclusive prefix sums of each plane,
+// prefix[plane * n_bins + b][t] for t = 0..N, and smax[b] = max |S|.
+template <int PLANES>
 __global__ __launch_bounds__(PREP_THREADS) void sums_prep_kernel(
     const int64_t* __restrict__ plane_sums, int64_t* __restrict__ sums, int nframes,
     int64_t* __restrict__ prefix, int64_t* __restrict__ smax) {
     __shared__ int64_t wave_tot[PREP_THREADS / 64];
     __shared__ int64_t wave_max[PREP_THREADS / 64];
+    constexpr bool WIDE = PLANES >= 2;
     const int b = blockIdx.x, nb = gridDim.x;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int64_t* S0 = plane_sums + (int64_t)b * nframes;
     const int64_t* S1 = plane_sums + (int64_t)(nb + b) * nframes;
+    const int64_t* S2 = plane_sums + (int64_t)(2 * nb + b) * nframes;
     int64_t* P0 = prefix + (int64_t)b * (nframes + 1);
     int64_t* P1 = prefix + (int64_t)(nb + b) * (nframes + 1);
-    int64_t c0 = 0, c1 = 0, mx = 0;
+    int64_t* P2 = prefix + (int64_t)(2 * nb + b) * (nframes + 1);
+    int64_t c0 = 0, c1 = 0, c2 = 0, mx = 0;
     for (int t0 = 0; t0 < nframes; t0 += PREP_THREADS) {
         const int t = t0 + tid;
         const int64_t v0 = t < nframes ? S0[t] : 0;
@@ -95,11 +100,17 @@ __global__ __launch_bounds__(PREP_THREADS) void sums_prep_kernel(
                 sums[(int64_t)b * nframes + t] = v;
             }
         }
+        if (PLANES == 3) {
+            const int64_t v2 = t < nframes ? S2[t] : 0;
+            const int64_t e2 = block_scan(v2, wave_tot, c2);
+            if (t < nframes) P2[t] = e2;
+        }
         mx = max(mx, v < 0 ? -v : v);
     }
     if (tid == 0) {
         P0[nframes] = c0;
         if (WIDE) P1[nframes] = c1;
+        if (PLANES == 3) P2[nframes] = c2;
     }
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) mx = max(mx, __shfl_xor(mx, d, 64));
@@ -226,14 +237,19 @@ __device__ __forceinline__ int64_t edge_sums(const int64_t* P, int nframes, int 
 // grid (ceil(N / FIN_SPAN), n_bins), FIN_THREADS threads.  NumT is int64_t (exact
 // numerators of the Gram and event paths) or double (FFT path).  mode 0: numerators as
 // accumulated; 1: uint8 bias correction; 2: uint16 digit planes (ugram rows b, n_bins + b,
-// 2 n_bins + b hold the biased HH, LL and cross sums).
+// 2 n_bins + b hold the biased HH, LL and cross sums); 3: as 2, with the HH and cross
+// sums taken over the first nwkb[b] k-blocks only (pixels ordered wide-first, so the high
+// bytes past them are zero):
+//   HH = U_HH + 128 E_H - 2^14 K_w (N-tau),   X = U_X + 128 (E_H + E_Lw) - 2^15 K_w (N-tau)
+// with K_w = 64 nwkb[b] and E_Lw from the low-byte sums over those k-blocks (prefix
+// plane 2).
 template <typename NumT>
 __global__ __launch_bounds__(FIN_THREADS) void finalize_kernel(
     const int64_t* __restrict__ sums, const NumT* __restrict__ ugram,
     const BinDev* __restrict__ bins, int nframes, int mode,
     const int64_t* __restrict__ prefix, const int64_t* __restrict__ smax,
-    const double* __restrict__ dpre, NumT* __restrict__ num, double* __restrict__ den,
-    float* __restrict__ g2) {
+    const double* __restrict__ dpre, const int* __restrict__ nwkb, NumT* __restrict__ num,
+    double* __restrict__ den, float* __restrict__ g2) {
     __shared__ __attribute__((aligned(16))) unsigned char stage[(2 * FIN_CH + FIN_SPAN) * 8];
     const int b = blockIdx.y;
     const int tau0 = blockIdx.x * FIN_SPAN;
@@ -268,14 +284,20 @@ __global__ __launch_bounds__(FIN_THREADS) void finalize_kernel(
         const int64_t pairs = kpad * (nframes - tau);  // padded pixel pairs on the diagonal
         double nd;
         NumT n;
-        if (mode == 2) {
+        if (mode >= 2) {
             // biased digit products: (h-128)(h'-128) etc.; padding entries are -128 too
             const int64_t* P1 = prefix + (int64_t)(nb + b) * (nframes + 1);
             const int64_t eh = edge_sums(P, nframes, tau), el = edge_sums(P1, nframes, tau);
+            int64_t elx = el, pairs_h = pairs;
+            if (mode == 3) {
+                const int64_t* P2 = prefix + (int64_t)(2 * nb + b) * (nframes + 1);
+                elx = edge_sums(P2, nframes, tau);
+                pairs_h = (int64_t)nwkb[b] * KB * (nframes - tau);
+            }
             const int64_t stride = (int64_t)nb * nframes;
-            const int64_t hh = (int64_t)ugram[o] + 128 * eh - 16384 * pairs;
+            const int64_t hh = (int64_t)ugram[o] + 128 * eh - 16384 * pairs_h;
             const int64_t ll = (int64_t)ugram[o + stride] + 128 * el - 16384 * pairs;
-            const int64_t x = (int64_t)ugram[o + 2 * stride] + 128 * (eh + el) - 32768 * pairs;
+            const int64_t x = (int64_t)ugram[o + 2 * stride] + 128 * (eh + elx) - 32768 * pairs_h;
             const __int128 v = (__int128)hh * 65536 + (__int128)x * 256 + ll;
             n = (NumT)(int64_t)v;
             nd = i128_to_double(v);
@@ -306,31 +328,34 @@ static void launch_den_fp64(const int64_t* sums, int n_bins, int nframes, const 
 
 void launch_finalize(const int64_t* plane_sums, int64_t* sums, const int64_t* ugram,
                      const BinDev* bins, int n_bins, int nframes, int mode, int64_t* prefix,
-                     int64_t* smax, double* dpre, int64_t* num, double* den, float* g2,
-                     hipStream_t s) {
+                     int64_t* smax, double* dpre, const int* nwkb, int64_t* num, double* den,
+                     float* g2, hipStream_t s) {
     if (n_bins <= 0 || nframes <= 0) return;
-    if (mode == 2)
-        sums_prep_kernel<true><<<dim3((unsigned)n_bins), dim3(PREP_THREADS), 0, s>>>(
+    if (mode == 3)
+        sums_prep_kernel<3><<<dim3((unsigned)n_bins), dim3(PREP_THREADS), 0, s>>>(
+            plane_sums, sums, nframes, prefix, smax);
+    else if (mode == 2)
+        sums_prep_kernel<2><<<dim3((unsigned)n_bins), dim3(PREP_THREADS), 0, s>>>(
             plane_sums, sums, nframes, prefix, smax);
     else
-        sums_prep_kernel<false><<<dim3((unsigned)n_bins), dim3(PREP_THREADS), 0, s>>>(
+        sums_prep_kernel<1><<<dim3((unsigned)n_bins), dim3(PREP_THREADS), 0, s>>>(
             sums, sums, nframes, prefix, smax);
     launch_den_fp64(sums, n_bins, nframes, smax, dpre, s);
     dim3 grid((unsigned)((nframes + FIN_SPAN - 1) / FIN_SPAN), (unsigned)n_bins);
     finalize_kernel<int64_t><<<grid, dim3(FIN_THREADS), 0, s>>>(
-        sums, ugram, bins, nframes, mode, prefix, smax, dpre, num, den, g2);
+        sums, ugram, bins, nframes, mode, prefix, smax, dpre, nwkb, num, den, g2);
 }
 
 void launch_finalize_f64(const int64_t* sums, const double* fnum, const BinDev* bins, int n_bins,
                          int nframes, int64_t* prefix, int64_t* smax, double* dpre, double* num,
                          double* den, float* g2, hipStream_t s) {
     if (n_bins <= 0 || nframes <= 0) return;
-    sums_prep_kernel<false><<<dim3((unsigned)n_bins), dim3(PREP_THREADS), 0, s>>>(
+    sums_prep_kernel<1><<<dim3((unsigned)n_bins), dim3(PREP_THREADS), 0, s>>>(
         sums, const_cast<int64_t*>(sums), nframes, prefix, smax);
     launch_den_fp64(sums, n_bins, nframes, smax, dpre, s);
     dim3 grid((unsigned)((nframes + FIN_SPAN - 1) / FIN_SPAN), (unsigned)n_bins);
     finalize_kernel<double><<<grid, dim3(FIN_THREADS), 0, s>>>(
-        sums, fnum, bins, nframes, 0, prefix, smax, dpre, num, den, g2);
+        sums, fnum, bins, nframes, 0, prefix, smax, dpre, nullptr, num, den, g2);
 }
 
 }  // namespace dxg2

@@ -243,22 +243,34 @@ struct GramTile {
     }
 };
 
-// One work item: the tile of `w` in a bin whose slab starts at slab_bin.
+// k-block end of item `w` once clipped to the bin's wide k-blocks (uint16 plans with the
+// pixels reordered wide-first; nwkb null: no clipping).  HH items (plain, k-blocks below
+// nkb) and cross items stop after the high plane's nonzero k-blocks; LL items are whole.
+template <bool CROSS>
+__device__ __forceinline__ int clipped_kb1(const GramWork& w, int nkb, const int* nwkb) {
+    if (!nwkb) return w.kb1;
+    const int nw = nwkb[w.bin];
+    if (w.kb0 < nkb) return min(w.kb1, nw);
+    return CROSS ? min(w.kb1, nkb + nw) : w.kb1;
+}
+
+// One work item: the tile of `w` in a bin whose slab starts at slab_bin, over k-blocks
+// [w.kb0, kb1).
 template <int WM, int WN, bool CROSS, bool STORE>
-__device__ __forceinline__ void gram_item(const GramWork& w, const int8_t* slab_bin, int nkb,
-                                          int nframes, int nrows, int biased,
+__device__ __forceinline__ void gram_item(const GramWork& w, int kb1, const int8_t* slab_bin,
+                                          int nkb, int nframes, int nrows, int biased,
                                           int64_t* __restrict__ out, int64_t* __restrict__ sums,
                                           int* __restrict__ tile, unsigned int* sDiag) {
     using S = GramShape<WM, WN>;
     const int tid = threadIdx.x;
     const int wave = tid >> 6;
     if (!CROSS && w.ti == w.tj) {
-        GramTile<WM, WN, true, false, STORE> t{slab_bin, w.kb0, w.kb1, w.ti * S::TM,
+        GramTile<WM, WN, true, false, STORE> t{slab_bin, w.kb0, kb1, w.ti * S::TM,
                                                w.tj * S::TN, nframes, nrows, tid, tid & 63,
                                                wave >> 1, wave & 1, 0, sDiag};
         t.run(biased, out, sums, tile);
     } else {
-        GramTile<WM, WN, false, CROSS, STORE> t{slab_bin, w.kb0, w.kb1, w.ti * S::TM,
+        GramTile<WM, WN, false, CROSS, STORE> t{slab_bin, w.kb0, kb1, w.ti * S::TM,
                                                 w.tj * S::TN, nframes, nrows, tid, tid & 63,
                                                 wave >> 1, wave & 1, nkb, sDiag};
         t.run(biased, out, sums, tile);
@@ -270,6 +282,7 @@ __global__ __launch_bounds__(256) void gram_i8_kernel(const int8_t* __restrict__
                                                       const BinDev* __restrict__ bins,
                                                       const GramWork* __restrict__ work,
                                                       int nframes, int nrows, int biased,
+                                                      const int* __restrict__ nwkb,
                                                       int64_t* __restrict__ ugram,
                                                       int64_t* __restrict__ sums) {
     using S = GramShape<WM, WN>;
@@ -278,7 +291,9 @@ __global__ __launch_bounds__(256) void gram_i8_kernel(const int8_t* __restrict__
     const GramWork w = work[blockIdx.x];
     if (w.bin < 0) return;  // padding item
     const BinDev bd = bins[w.bin];
-    gram_item<WM, WN, CROSS, false>(w, slab + bd.slab_off, bd.nkb, nframes, nrows, biased,
+    const int kb1 = clipped_kb1<CROSS>(w, bd.nkb, nwkb);
+    if (kb1 <= w.kb0) return;  // nothing of this item is left
+    gram_item<WM, WN, CROSS, false>(w, kb1, slab + bd.slab_off, bd.nkb, nframes, nrows, biased,
                                     ugram + (int64_t)w.out * nframes,
                                     sums + (int64_t)w.out * nframes, nullptr, sDiag);
 }
@@ -311,7 +326,8 @@ template <int WM, int WN, bool CROSS>
 __global__ __launch_bounds__(256) void gram_i8_queue_kernel(
     const int8_t* __restrict__ slab, const BinDev* __restrict__ bins,
     const GramWork* __restrict__ work, const int* __restrict__ qoff, int* heads, int nframes,
-    int nrows, int biased, int64_t* __restrict__ ugram, int64_t* __restrict__ sums) {
+    int nrows, int biased, const int* __restrict__ nwkb, int64_t* __restrict__ ugram,
+    int64_t* __restrict__ sums) {
     using S = GramShape<WM, WN>;
     static_assert(S::TM == S::TN, "square tiles");
     __shared__ unsigned int sDiag[2 * S::NDIAG];
@@ -324,9 +340,11 @@ __global__ __launch_bounds__(256) void gram_i8_queue_kernel(
         if (it < 0) break;
         const GramWork w = work[it];
         const BinDev bd = bins[w.bin];
-        gram_item<WM, WN, CROSS, false>(w, slab + bd.slab_off, bd.nkb, nframes, nrows, biased,
-                                        ugram + (int64_t)w.out * nframes,
-                                        sums + (int64_t)w.out * nframes, nullptr, sDiag);
+        const int kb1 = clipped_kb1<CROSS>(w, bd.nkb, nwkb);
+        if (kb1 > w.kb0)  // the same for every thread of the workgroup
+            gram_item<WM, WN, CROSS, false>(w, kb1, slab + bd.slab_off, bd.nkb, nframes, nrows,
+                                            biased, ugram + (int64_t)w.out * nframes,
+                                            sums + (int64_t)w.out * nframes, nullptr, sDiag);
         __syncthreads();  // s_item and sDiag are reused by the next item
     }
 }
@@ -362,14 +380,14 @@ void gram_queue_prepare(int tile) {
 
 void launch_gram_i8_queue(int tile, bool cross, const int8_t* slab, const BinDev* bins,
                           const GramWork* work, int n_work, const int* qoff, int* heads,
-                          int nframes, int nrows, int biased, int64_t* ugram, int64_t* sums,
-                          hipStream_t s) {
+                          int nframes, int nrows, int biased, const int* nwkb, int64_t* ugram,
+                          int64_t* sums, hipStream_t s) {
     if (n_work <= 0) return;
     const dim3 block(256);
 #define DXG2_QUEUE_LAUNCH(WM, CR)                                                             \
     gram_i8_queue_kernel<WM, WM, CR><<<dim3((unsigned)std::min(n_work, queue_grid<WM, WM, CR>())), \
                                        block, 0, s>>>(slab, bins, work, qoff, heads, nframes,    \
-                                                      nrows, biased, ugram, sums)
+                                                      nrows, biased, nwkb, ugram, sums)
     if (tile == 128) {
         if (cross)
             DXG2_QUEUE_LAUNCH(2, true);
@@ -399,7 +417,7 @@ __global__ __launch_bounds__(256) void gram_i8_store_kernel(
     const GramWork w = work[blockIdx.x];
     if (w.bin < 0) return;  // padding item
     const BinDev bd = bins[w.bin];
-    gram_item<WM, WN, CROSS, true>(w, slab + bd.slab_off, bd.nkb, nframes, nrows, biased,
+    gram_item<WM, WN, CROSS, true>(w, w.kb1, slab + bd.slab_off, bd.nkb, nframes, nrows, biased,
                                    ugram + (int64_t)w.out * nframes, nullptr,
                                    tiles + (int64_t)w.slot * S::TM * S::TN, sDiag);
 }
@@ -416,7 +434,7 @@ __global__ __launch_bounds__(256) void gram_i8_jobs_kernel(const GramJob* __rest
     if (w.bin < 0) return;  // padding item
     const GramJob j = jobs[w.job];
     const BinDev bd = gload_bin(j.bins + w.bin);
-    gram_item<WM, WN, CROSS, false>(w, j.slab + bd.slab_off, bd.nkb, j.nframes, j.nrows,
+    gram_item<WM, WN, CROSS, false>(w, w.kb1, j.slab + bd.slab_off, bd.nkb, j.nframes, j.nrows,
                                     j.biased, j.ugram + (int64_t)w.out * j.nframes,
                                     j.sums + (int64_t)w.out * j.nframes, nullptr, sDiag);
 }
@@ -464,23 +482,23 @@ void launch_gram_i8_store(int tile, bool cross, const int8_t* slab, const BinDev
 
 void launch_gram_i8(int tile, bool cross, const int8_t* slab, const BinDev* bins,
                     const GramWork* work, int n_work, int nframes, int nrows, int biased,
-                    int64_t* ugram, int64_t* sums, hipStream_t s) {
+                    const int* nwkb, int64_t* ugram, int64_t* sums, hipStream_t s) {
     if (n_work <= 0) return;
     const dim3 grid((unsigned)n_work), block(256);
     if (tile == 128) {
         if (cross)
             gram_i8_kernel<2, 2, true><<<grid, block, 0, s>>>(slab, bins, work, nframes, nrows,
-                                                              biased, ugram, sums);
+                                                              biased, nwkb, ugram, sums);
         else
             gram_i8_kernel<2, 2, false><<<grid, block, 0, s>>>(slab, bins, work, nframes, nrows,
-                                                               biased, ugram, sums);
+                                                               biased, nwkb, ugram, sums);
     } else {
         if (cross)
             gram_i8_kernel<4, 4, true><<<grid, block, 0, s>>>(slab, bins, work, nframes, nrows,
-                                                              biased, ugram, sums);
+                                                              biased, nwkb, ugram, sums);
         else
             gram_i8_kernel<4, 4, false><<<grid, block, 0, s>>>(slab, bins, work, nframes, nrows,
-                                                               biased, ugram, sums);
+                                                               biased, nwkb, ugram, sums);
     }
 }
 
