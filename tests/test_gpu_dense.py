@@ -111,6 +111,34 @@ def test_dense_u16_byte_split_vs_oracle(n, shape, hi, tile):
     _check_exact(c, g2, og.dense_g2(f, q))
 
 
+@pytest.mark.parametrize("tile", [256, 128])
+@pytest.mark.parametrize("n_wide", [0, 1, 37, 300])
+def test_dense_u16_wide_pixels_first(n_wide, tile):
+    """uint16 counts below 256 except on n_wide scattered pixels: the dense call orders
+    each bin's wide pixels first and clips the high-byte Gram items to their k-blocks
+    (none at all for n_wide = 0, one k-block for 1, part of a k-block for 37, several for
+    300); numerators, sums and denominators stay exact, and calc_std (unclipped) agrees."""
+    _announce("dense u16 wide-first n_wide=%d tile=%d" % (n_wide, tile))
+    rng = np.random.default_rng(100 + n_wide)
+    shape, n = (40, 60), 290
+    f = rng.integers(0, 256, size=(n,) + shape).astype(np.uint16)
+    f[rng.random(f.shape) < 0.6] = 0
+    wide = rng.choice(shape[0] * shape[1], size=n_wide, replace=False)
+    for p in wide:
+        t = rng.integers(0, n, size=3)
+        f[t, p // shape[1], p % shape[1]] = rng.integers(256, 65536, size=3)
+    q = rng.integers(0, 6, size=shape).astype(np.int32)
+    for b in range(1, 6):
+        q.flat[b] = b
+    c, g2 = _run(f, q, extra_options={"tile": tile})
+    ref = og.dense_g2(f, q)
+    _check_exact(c, g2, ref)
+    from dynamix_amd.correlator.dense import MatMulCorrelator
+    cs = MatMulCorrelator(shape, n, q, extra_options={"tile": tile})
+    cs.correlate(f, calc_std=True)
+    np.testing.assert_array_equal(cs.last_numerators.cpu().numpy(), ref["num"])
+
+
 def test_dense_u16_speckle_with_hot_pixels():
     """Speckle counts with a hot-pixel tail (mean 40), 20 linear-width annuli."""
     _announce("dense u16 speckle hot pixels")
