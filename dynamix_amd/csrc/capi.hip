@@ -698,7 +698,8 @@ int64_t* dense_gather(dxg2_plan* p, const void* d_frames, int64_t* d_sums, hipSt
                            p->d_gband_kb, p->d_gfall, (int)p->gfall.size(), p->d_slab, s);
     } else {
         launch_gather_i8((const uint8_t*)d_frames, p->dtype, hw, p->N, p->nrows, pix,
-                         p->d_bins, p->d_gwork, (int)p->gather_work.size(), p->d_slab, s);
+                         p->d_bins, p->d_gwork, (int)p->gather_work.size(), p->d_slab, s,
+                         wide && wide_first ? p->d_nwkb : nullptr);
     }
     if (wide && wide_first)
         launch_wide_low_sums(p->d_slab, p->d_bins, p->d_nwkb, p->n_bins, p->N, p->nrows,

@@ -122,7 +122,7 @@ struct KbWork {          // one k-block (64 pixels) of one bin
 void launch_fill_i64(int64_t* p, int64_t n, hipStream_t s);
 void launch_gather_i8(const uint8_t* frames, int dtype, int64_t hw, int nframes, int nrows,
                       const int32_t* pix_idx, const BinDev* bins, const GatherWork* work,
-                      int n_work, int8_t* slab, hipStream_t s);
+                      int n_work, int8_t* slab, hipStream_t s, const int* nwkb = nullptr);
 // Band gather of uint8/int8 frames: stage `rows` detector rows of 4 frames in LDS
 // (srow bytes per frame), write every k-block of the band from there.
 void launch_gather_band(const uint8_t* frames, int dtype, int64_t hw, int width, int nframes,

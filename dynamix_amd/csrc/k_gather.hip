@@ -69,15 +69,19 @@ __device__ __forceinline__ uint32_t quad_transpose(uint32_t x, uint32_t sel_lo, 
 // BIAS (uint8 and uint16 input): bytes are stored as v ^ 0x80 = v - 128 (i8 MFMA
 // operands); padded list entries (index -1) read as 0 and become -128 the same way.
 // T = uint16_t writes the high bytes to k-block kb and the low bytes to k-block
-// kb + nkb of the bin (two digit planes).
+// kb + nkb of the bin (two digit planes); with nwkb (pixel lists ordered wide-first) the
+// high bytes of k-blocks past nwkb[bin], all zero and never read by the clipped Gram, are
+// not written.
 template <typename T, bool BIAS>
 __device__ __forceinline__ void gather_general(const T* __restrict__ frames, int64_t hw,
                                                int nframes, int nrows,
                                                const int32_t* __restrict__ pix_idx,
                                                const BinDev* __restrict__ bins, const GatherWork w,
-                                               int t0, int tend, int8_t* __restrict__ slab) {
+                                               int t0, int tend, int8_t* __restrict__ slab,
+                                               const int* __restrict__ nwkb = nullptr) {
     constexpr bool WIDE = sizeof(T) == 2;
     const BinDev bd = bins[w.bin];
+    const int hi_end = WIDE && nwkb ? nwkb[w.bin] : bd.nkb;  // k-blocks whose high bytes go out
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int qi = lane & 3;        // frame offset this lane stores after the transpose
@@ -107,7 +111,7 @@ __device__ __forceinline__ void gather_general(const T* __restrict__ frames, int
             if (WIDE) {
                 const uint32_t hi = quad_transpose(xh, sel_lo, sel_hi) ^ 0x80808080u;
                 if (t + qi < nframes) {
-                    *reinterpret_cast<uint32_t*>(dst + o) = hi;
+                    if (kb < hi_end) *reinterpret_cast<uint32_t*>(dst + o) = hi;
                     *reinterpret_cast<uint32_t*>(dst + plane + o) = lo;
                 }
             } else if (t + qi < nframes) {
@@ -121,10 +125,11 @@ template <typename T, bool BIAS>
 __global__ __launch_bounds__(256) void gather_i8_kernel(
     const T* __restrict__ frames, int64_t hw, int nframes, int nrows,
     const int32_t* __restrict__ pix_idx, const BinDev* __restrict__ bins,
-    const GatherWork* __restrict__ work, int8_t* __restrict__ slab) {
+    const GatherWork* __restrict__ work, int8_t* __restrict__ slab,
+    const int* __restrict__ nwkb) {
     const int t0 = blockIdx.y * GATHER_TF;
     gather_general<T, BIAS>(frames, hw, nframes, nrows, pix_idx, bins, work[blockIdx.x], t0,
-                            min(nframes, t0 + GATHER_TF), slab);
+                            min(nframes, t0 + GATHER_TF), slab, nwkb);
 }
 
 // Band gather (uint8 / int8 frames).  A workgroup takes one band of detector rows and up
@@ -286,19 +291,21 @@ void launch_gather_band(const uint8_t* frames, int dtype, int64_t hw, int width,
 
 void launch_gather_i8(const uint8_t* frames, int dtype, int64_t hw, int nframes, int nrows,
                       const int32_t* pix_idx, const BinDev* bins, const GatherWork* work,
-                      int n_work, int8_t* slab, hipStream_t s) {
+                      int n_work, int8_t* slab, hipStream_t s, const int* nwkb) {
     if (n_work <= 0 || nframes <= 0) return;
     dim3 grid((unsigned)n_work, (unsigned)((nframes + GATHER_TF - 1) / GATHER_TF));
     if (dtype == DXG2_U16)
         gather_i8_kernel<uint16_t, true><<<grid, dim3(256), 0, s>>>(
             reinterpret_cast<const uint16_t*>(frames), hw, nframes, nrows, pix_idx, bins, work,
-            slab);
+            slab, nwkb);
     else if (dtype == DXG2_U8)
         gather_i8_kernel<uint8_t, true><<<grid, dim3(256), 0, s>>>(frames, hw, nframes, nrows,
-                                                                  pix_idx, bins, work, slab);
+                                                                  pix_idx, bins, work, slab,
+                                                                  nullptr);
     else
         gather_i8_kernel<uint8_t, false><<<grid, dim3(256), 0, s>>>(frames, hw, nframes, nrows,
-                                                                   pix_idx, bins, work, slab);
+                                                                   pix_idx, bins, work, slab,
+                                                                   nullptr);
 }
 
 }  // namespace dxg2
