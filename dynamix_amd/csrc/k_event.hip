@@ -97,12 +97,37 @@ __global__ __launch_bounds__(256) void event_pairs_kernel(
                 const int mid = (lo + hi) >> 1;
                 if (tt[mid] < target) lo = mid + 1; else hi = mid;
             }
-            for (int j = lo; j < n; ++j) {
-                const int tau = tt[j] - ti;
-                if (tau >= w1) break;
+            // partners j in [lo, end): lag below the window's end (t_j < t_i + w1)
+            int end = n;
+            {
+                int a = lo, b2 = n;
+                const int lim = ti + w1;
+                while (a < b2) {
+                    const int mid = (a + b2) >> 1;
+                    if (tt[mid] < lim) a = mid + 1; else b2 = mid;
+                }
+                end = a;
+            }
+            // four partners per step, so that their reads and atomics overlap
+            int j = lo;
+            for (; j + 3 < end; j += 4) {
+                int tj[4];
+                long long dj[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    tj[u] = tt[j + u];
+                    dj[u] = staged ? st_d[wave][j + u] : as_count<T>(data[beg + j + u]);
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const long long prod = di * dj[u];
+                    if (prod != 0) atomicAdd(&hist[tj[u] - ti - w0], (unsigned long long)prod);
+                }
+            }
+            for (; j < end; ++j) {
                 const long long dj = staged ? st_d[wave][j] : as_count<T>(data[beg + j]);
                 const long long prod = di * dj;
-                if (prod != 0) atomicAdd(&hist[tau - w0], (unsigned long long)prod);
+                if (prod != 0) atomicAdd(&hist[tt[j] - ti - w0], (unsigned long long)prod);
             }
         }
         __builtin_amdgcn_wave_barrier();
