@@ -298,11 +298,15 @@ def bench_events(seed):
     same = bool((num == num_csr).all().item())
     hw = shape[0] * shape[1]
     conv_bytes = 9 * nnz + 2 * 5 * nnz + 4 * (hw + 1)
+    # CSR call's algorithmic HBM traffic: each event's time and value read once, the
+    # offsets and the binned pixel list once, the four (bin, tau) output planes written once.
+    csr_bytes = 5 * nnz + 4 * (hw + 1) + 4 * int(binned.sum()) + nb * n * (8 + 8 + 8 + 4)
     plan.close()
     return {"workload": "events: %dx%d x %d frames, mean %g, %d bins, %d events, %d pairs"
                         % (shape[0], shape[1], n, wl["mean"], nb, nnz, pairs),
             "csr_ms": ms_csr, "coo_ms": ms_coo, "pairs_per_s": pairs / (ms_csr * 1e-3),
             "events_per_s": nnz / (ms_csr * 1e-3), "pixel_frames_per_s": int(binned.sum()) * n / (ms_csr * 1e-3),
+            "csr_hbm_GBps": csr_bytes / (ms_csr * 1e-3) / 1e9,
             "coo_to_csr_GBps": conv_bytes / ((ms_coo - ms_csr) * 1e-3) / 1e9 if ms_coo > ms_csr else None,
             "coo_equals_csr": same, "input_flags": flags.value, "generate_s": gen_s}
 
