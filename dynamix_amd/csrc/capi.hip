@@ -71,7 +71,13 @@ hipError_t dev_alloc(const DevAllocator& a, void** ptr, size_t bytes, hipStream_
     bytes = std::max<size_t>(bytes, 1);
     if (!a.alloc) return hipMalloc(ptr, bytes);
     *ptr = a.alloc(bytes, (void*)s, a.ctx);
-    return *ptr ? hipSuccess : hipErrorOutOfMemory;
+    if (!*ptr) return hipErrorOutOfMemory;
+    if (reinterpret_cast<uintptr_t>(*ptr) % 256 != 0) {  // the kernels' vector accesses
+        if (a.free) a.free(*ptr, (void*)s, a.ctx);
+        *ptr = nullptr;
+        return hipErrorInvalidValue;
+    }
+    return hipSuccess;
 }
 
 void dev_free(const DevAllocator& a, void* ptr, hipStream_t s) {

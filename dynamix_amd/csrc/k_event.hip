@@ -7,7 +7,8 @@
 //
 // Here:
 //  * work items are (bin, range of the bin's pixel list) x (lag window of EW lags);
-//    a wave takes one pixel at a time, stages its events in LDS, and each lane takes
+//    a wave takes one pixel at a time (the next pixel's events load into registers
+//    meanwhile), stages its events in LDS, and each lane takes
 //    events i = lane, lane+64, ...; a binary search finds the first partner j > i
 //    whose lag t_j - t_i reaches the window, and the pairs up to the window's end go
 //    to a 64-bit LDS histogram (exact for any count type), flushed once per work
@@ -30,9 +31,17 @@ namespace dxg2 {
 constexpr int EW = 8192;        // lags per LDS window (64 KiB of 64-bit counters: two
                                 // workgroups per CU with the staging below)
 constexpr int ECAP = 256;       // events of one pixel staged in LDS per wave
+constexpr int EPF = ECAP / 64;  // event rounds of the next pixel prefetched in registers
 
 template <typename T>
 __device__ __forceinline__ long long as_count(T v) { return (long long)v; }
+
+// Per-pixel bookkeeping of one wave: pixel index and event range, valid or not.
+struct EvPixel {
+    int p;
+    uint32_t beg, end;
+    int n;  // events to correlate: 0 for an empty, unbinned or invalid range
+};
 
 template <typename T>
 __global__ __launch_bounds__(256) void event_pairs_kernel(
@@ -42,48 +51,103 @@ __global__ __launch_bounds__(256) void event_pairs_kernel(
     int64_t nnz, int64_t* __restrict__ num, int64_t* __restrict__ sums, int* __restrict__ err) {
     __shared__ unsigned long long hist[EW];
     __shared__ int st_t[4][ECAP];
-    __shared__ long long st_d[4][ECAP];
+    __shared__ int st_d[4][ECAP];  // uint8, int8 and uint16 counts fit 32 bits
     const EventWork w = work[blockIdx.x];
     const BinDev bd = bins[w.bin];
     const int w0 = blockIdx.y * EW;
     if (w0 >= nframes) return;
     const int w1 = min(nframes, w0 + EW);
     const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     for (int k = threadIdx.x; k < EW; k += 256) hist[k] = 0ull;
     __syncthreads();
     int64_t* srow = sums + (int64_t)w.bin * nframes;
 
-    for (int pi = w.k0 + wave; pi < w.k1; pi += 4) {
-        const int p = pix_idx[bd.idx_off + pi];
-        if (p < 0) continue;
-        const uint32_t beg = offsets[p];
-        const uint32_t end = offsets[p + 1];
-        if (end <= beg || end > (uint64_t)nnz) {
-            if ((end < beg || end > (uint64_t)nnz) && lane == 0) atomicOr(err, 2);
-            continue;
+    // The wave takes pixels w.k0 + wave + 4r, r < nr.  Their list entries and offsets are
+    // loaded 64 at a time (lane r % 64 holds pixel r), and while the pairs of pixel r are
+    // counted, the first ECAP events of pixel r + 1 are already loading into registers,
+    // so the chain list entry -> offsets -> events is not waited for once per pixel.
+    const int npx = w.k1 - w.k0 - wave;
+    const int nr = npx > 0 ? (npx + 3) / 4 : 0;
+    int mp = -1;
+    uint32_t mb = 0u, me = 0u;
+    auto load_meta = [&](int r0) {
+        const int r = r0 + lane;
+        mp = -1;
+        mb = me = 0u;
+        if (r < nr) {
+            mp = pix_idx[bd.idx_off + w.k0 + wave + 4 * r];
+            if (mp >= 0) {
+                mb = offsets[mp];
+                me = offsets[mp + 1];
+            }
         }
-        const int n = (int)(end - beg);
+    };
+    auto meta = [&](int r) {
+        EvPixel x;
+        x.p = __builtin_amdgcn_readlane(mp, r & 63);
+        x.beg = (uint32_t)__builtin_amdgcn_readlane((int)mb, r & 63);
+        x.end = (uint32_t)__builtin_amdgcn_readlane((int)me, r & 63);
+        const bool bad = x.p >= 0 && (x.end < x.beg || x.end > (uint64_t)nnz);
+        x.n = (x.p < 0 || bad || x.end <= x.beg) ? 0 : (int)(x.end - x.beg);
+        if (bad && lane == 0) atomicOr(err, 2);
+        return x;
+    };
+    int ct[EPF];
+    int cd[EPF];
+    auto prefetch = [&](const EvPixel& x) {
+#pragma unroll
+        for (int q = 0; q < EPF; ++q) {
+            const int k = lane + 64 * q;
+            if (k < x.n && x.n <= ECAP) {
+                ct[q] = times[x.beg + k];
+                cd[q] = (int)data[x.beg + k];
+            }
+        }
+    };
+
+    EvPixel cur = {-1, 0u, 0u, 0};
+    if (nr > 0) {
+        load_meta(0);
+        cur = meta(0);
+        prefetch(cur);
+    }
+    for (int r = 0; r < nr; ++r) {
+        const int n = cur.n;
+        const uint32_t beg = cur.beg;
         const bool staged = n <= ECAP;
+        if (n > 0 && staged) {
+#pragma unroll
+            for (int q = 0; q < EPF; ++q) {
+                const int k = lane + 64 * q;
+                if (k < n) {
+                    st_t[wave][k] = ct[q];
+                    st_d[wave][k] = cd[q];
+                }
+            }
+        }
+        // the next pixel's range, and its events into the registers just staged
+        if (r + 1 < nr) {
+            if (((r + 1) & 63) == 0) load_meta(r + 1);
+            cur = meta(r + 1);
+            prefetch(cur);
+        }
+        if (n == 0) continue;
+        __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): staged values visible to the wave
+        __builtin_amdgcn_wave_barrier();
         const int* tt = staged ? st_t[wave] : times + beg;
-        // stage + validate
+        // validate: frames in [0, N) and strictly increasing
         int bad = 0;
         for (int k = lane; k < n; k += 64) {
-            const int t = times[beg + k];
+            const int t = tt[k];
             if (t < 0 || t >= nframes) bad = 1;
-            if (k + 1 < n && times[beg + k + 1] <= t) bad = 1;
-            if (staged) {
-                st_t[wave][k] = t;
-                st_d[wave][k] = as_count<T>(data[beg + k]);
-            }
+            if (k + 1 < n && tt[k + 1] <= t) bad = 1;
         }
         if (bad) atomicOr(err, 1);
         if (__any(bad)) continue;
-        __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): staged values visible to the wave
-        __builtin_amdgcn_wave_barrier();
         for (int i = lane; i < n; i += 64) {
             const int ti = tt[i];
-            const long long di = staged ? st_d[wave][i] : as_count<T>(data[beg + i]);
+            const long long di = staged ? (long long)st_d[wave][i] : as_count<T>(data[beg + i]);
             if (w0 == 0) {
                 atomicAdd(&hist[0], (unsigned long long)(di * di));
                 if (di != 0)
@@ -116,7 +180,7 @@ __global__ __launch_bounds__(256) void event_pairs_kernel(
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     tj[u] = tt[j + u];
-                    dj[u] = staged ? st_d[wave][j + u] : as_count<T>(data[beg + j + u]);
+                    dj[u] = staged ? (long long)st_d[wave][j + u] : as_count<T>(data[beg + j + u]);
                 }
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
@@ -125,7 +189,7 @@ __global__ __launch_bounds__(256) void event_pairs_kernel(
                 }
             }
             for (; j < end; ++j) {
-                const long long dj = staged ? st_d[wave][j] : as_count<T>(data[beg + j]);
+                const long long dj = staged ? (long long)st_d[wave][j] : as_count<T>(data[beg + j]);
                 const long long prod = di * dj;
                 if (prod != 0) atomicAdd(&hist[tt[j] - ti - w0], (unsigned long long)prod);
             }

@@ -17,41 +17,46 @@
 
 namespace dxg2 {
 
-// One thread per 8 pixels (16 bytes of a frame row) when the stack is 16-byte aligned and
-// H*W is a multiple of 8, else per pixel; flag[p] = 1 if any frame has a count > 255.
+// Grid (pixel groups, frame chunks).  A thread ORs the counts of 8 pixels (16 bytes of a
+// frame row) when the stack is 16-byte aligned and H*W is a multiple of 8, else of one
+// pixel, over frames [y*fch, (y+1)*fch), and sets the flag byte of each pixel with a high
+// byte by a vector atomicOr on the flag words (pixel 4j+k is byte k of word j), which
+// start at zero.  Splitting the frame axis keeps many more loads in flight than one
+// thread per pixel group over all frames; a flag word is written only when a chunk holds
+// a count > 255, which is rare.
 template <bool VEC>
 __global__ __launch_bounds__(256) void wide_flag_kernel(const uint16_t* __restrict__ frames,
-                                                        int64_t hw, int nframes,
-                                                        uint8_t* __restrict__ flag) {
+                                                        int64_t hw, int nframes, int fch,
+                                                        uint32_t* __restrict__ flagw) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int t0 = blockIdx.y * fch;
+    const int t1 = min(nframes, t0 + fch);
     if (VEC) {
         if (8 * i >= hw) return;
         const uint16_t* src = frames + 8 * i;
         uint32_t acc[4] = {0u, 0u, 0u, 0u};
 #pragma unroll 8
-        for (int t = 0; t < nframes; ++t) {
+        for (int t = t0; t < t1; ++t) {
             const uint4 v = *reinterpret_cast<const uint4*>(src + (int64_t)t * hw);
             acc[0] |= v.x;
             acc[1] |= v.y;
             acc[2] |= v.z;
             acc[3] |= v.w;
         }
-        uint8_t f[8];
+        uint32_t f[2] = {0u, 0u};
 #pragma unroll
         for (int q = 0; q < 4; ++q) {  // pixel 2q in the low half of word q, 2q+1 high
-            f[2 * q] = (acc[q] & 0xFF00u) ? 1 : 0;
-            f[2 * q + 1] = (acc[q] & 0xFF000000u) ? 1 : 0;
+            const int sh = 8 * (2 * q & 3);
+            f[q >> 1] |= ((acc[q] & 0xFF00u) ? 1u : 0u) << sh;
+            f[q >> 1] |= ((acc[q] & 0xFF000000u) ? 1u : 0u) << (sh + 8);
         }
-        *reinterpret_cast<uint2*>(flag + 8 * i) =
-            make_uint2((uint32_t)f[0] | (uint32_t)f[1] << 8 | (uint32_t)f[2] << 16 |
-                           (uint32_t)f[3] << 24,
-                       (uint32_t)f[4] | (uint32_t)f[5] << 8 | (uint32_t)f[6] << 16 |
-                           (uint32_t)f[7] << 24);
+        if (f[0]) atomicOr(flagw + 2 * i, f[0]);
+        if (f[1]) atomicOr(flagw + 2 * i + 1, f[1]);
     } else {
         if (i >= hw) return;
         uint32_t acc = 0u;
-        for (int t = 0; t < nframes; ++t) acc |= frames[(int64_t)t * hw + i];
-        flag[i] = (acc & 0xFF00u) ? 1 : 0;
+        for (int t = t0; t < t1; ++t) acc |= frames[(int64_t)t * hw + i];
+        if (acc & 0xFF00u) atomicOr(flagw + (i >> 2), 1u << (8 * (int)(i & 3)));
     }
 }
 
@@ -138,15 +143,20 @@ __global__ __launch_bounds__(256) void wide_low_sums_kernel(const int8_t* __rest
 void launch_wide_first(const uint16_t* frames, int64_t hw, int nframes, const int32_t* pix,
                        const BinDev* bins, int n_bins, uint8_t* flag, int32_t* pix_w, int* nwkb,
                        hipStream_t s) {
-    if (n_bins <= 0) return;
-    if (reinterpret_cast<uintptr_t>(frames) % 16 == 0 && hw % 8 == 0) {
-        const int64_t n8 = hw / 8;
-        wide_flag_kernel<true><<<dim3((unsigned)((n8 + 255) / 256)), 256, 0, s>>>(frames, hw,
-                                                                                 nframes, flag);
-    } else {
-        wide_flag_kernel<false><<<dim3((unsigned)((hw + 255) / 256)), 256, 0, s>>>(frames, hw,
-                                                                                 nframes, flag);
-    }
+    if (n_bins <= 0 || nframes <= 0) return;
+    uint32_t* flagw = reinterpret_cast<uint32_t*>(flag);  // 256-byte aligned, >= hw + 3 bytes
+    launch_fill_u32(flagw, (hw + 3) / 4, s);
+    const bool vec = reinterpret_cast<uintptr_t>(frames) % 16 == 0 && hw % 8 == 0;
+    const int64_t bx = ((vec ? hw / 8 : hw) + 255) / 256;
+    // about 16k workgroups in all, each thread over at least 128 frames
+    const int64_t want = std::max<int64_t>(1, 16384 / std::max<int64_t>(bx, 1));
+    const int ny = (int)std::max<int64_t>(1, std::min<int64_t>(want, nframes / 128));
+    const int fch = (nframes + ny - 1) / ny;
+    const dim3 grid((unsigned)bx, (unsigned)((nframes + fch - 1) / fch));
+    if (vec)
+        wide_flag_kernel<true><<<grid, 256, 0, s>>>(frames, hw, nframes, fch, flagw);
+    else
+        wide_flag_kernel<false><<<grid, 256, 0, s>>>(frames, hw, nframes, fch, flagw);
     wide_partition_kernel<<<dim3((unsigned)n_bins), 1024, 0, s>>>(pix, bins, flag, pix_w, nwkb);
 }
 

@@ -50,7 +50,9 @@ extern "C" {
 typedef struct dxg2_plan dxg2_plan;
 
 /* Device memory of plans and batches.  alloc returns at least `bytes` of device memory
- * for use on `stream` (a hipStream_t), or NULL; free releases a pointer alloc returned.
+ * for use on `stream` (a hipStream_t), aligned to 256 bytes, or NULL (a pointer that is
+ * not so aligned is freed and counts as a failed allocation); free releases a pointer
+ * alloc returned.
  * The library frees a plan's buffers only when no call on the plan can still be running
  * (dxg2_plan_destroy waits for the device; with dxg2_plan_destroy_async the caller
  * guarantees it).  dxg2_set_allocator(NULL, NULL, NULL) restores the built-in hipMalloc /

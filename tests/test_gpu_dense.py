@@ -139,6 +139,37 @@ def test_dense_u16_wide_pixels_first(n_wide, tile):
     np.testing.assert_array_equal(cs.last_numerators.cpu().numpy(), ref["num"])
 
 
+@pytest.mark.parametrize("unaligned", [False, True])
+def test_dense_u16_wide_flags_odd_width(unaligned):
+    """uint16 wide-pixel flags when H*W is not a multiple of 8 or the frames' device
+    address is not 16-byte aligned (per-pixel flag pass), over several frame chunks:
+    counts above 255 in the first and last frames and on both sides of a chunk edge."""
+    import torch
+    _announce("dense u16 wide flags odd width unaligned=%s" % unaligned)
+    rng = np.random.default_rng(7)
+    shape, n = (37, 41), 300
+    f = rng.integers(0, 200, size=(n,) + shape).astype(np.uint16)
+    f[rng.random(f.shape) < 0.6] = 0
+    hw = shape[0] * shape[1]
+    wide = rng.choice(hw, size=12, replace=False)
+    for k, p in enumerate(wide):
+        t = (0, n - 1, 149, 150, 151)[k % 5]
+        f[t, p // shape[1], p % shape[1]] = 256 + 1000 * k
+    q = rng.integers(0, 5, size=shape).astype(np.int32)
+    ref = og.dense_g2(f, q)
+    from dynamix_amd.correlator.dense import MatMulCorrelator
+    c = MatMulCorrelator(shape, n, q)
+    if unaligned:
+        buf = torch.zeros(f.size + 1, dtype=torch.int16, device="cuda")
+        buf[1:] = torch.from_numpy(f.ravel().view(np.int16)).cuda()
+        view = buf[1:].view(torch.uint16).view(n, *shape)
+        assert view.data_ptr() % 16 != 0
+        g2 = c.correlate(view)
+    else:
+        g2 = c.correlate(f)
+    _check_exact(c, g2, ref)
+
+
 def test_dense_u16_speckle_with_hot_pixels():
     """Speckle counts with a hot-pixel tail (mean 40), 20 linear-width annuli."""
     _announce("dense u16 speckle hot pixels")

@@ -39,6 +39,7 @@ def _check(c, g2, ref):
     (50, (6, 7), 0.3, 3),
     (400, (20, 25), 0.05, 5),
     (17000, (4, 5), 0.02, 2),      # several lag windows; pixels past the LDS staging cap
+    (300, (60, 80), 0.05, 2),      # bins over 1024 pixels: several work items per bin
 ])
 def test_events_csr_vs_oracle_and_dense(n, shape, mean, nbins):
     _announce("events csr n=%d shape=%s" % (n, shape))
