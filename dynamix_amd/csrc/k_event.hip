@@ -6,12 +6,12 @@
 // res[bin][t_i - t_j] += d_i d_j, plus sums[bin][t_i] += d_i.
 //
 // Here:
-//  * work items are (bin, range of the bin's pixel list) x (lag window of EW lags);
+//  * work items are (bin, range of the bin's pixel list) x (lag window);
 //    a wave takes one pixel at a time (the next pixel's events load into registers
 //    meanwhile), stages its events in LDS, and each lane takes
-//    events i = lane, lane+64, ...; a binary search finds the first partner j > i
-//    whose lag t_j - t_i reaches the window, and the pairs up to the window's end go
-//    to a 64-bit LDS histogram (exact for any count type), flushed once per work
+//    events i = lane + 128 r and i + 64 together; a binary search finds the first
+//    partner j > i whose lag t_j - t_i reaches the window, and the pairs up to the
+//    window's end go to an LDS lag histogram (EvHist below), flushed once per work
 //    item with one global 64-bit atomic per non-zero lag;
 //  * S_b(t) and the self pairs (lag 0, d_i^2) are added in the first window;
 //  * the input is validated on the device (frames in [0, N), strictly increasing per
@@ -28,13 +28,42 @@
 
 namespace dxg2 {
 
-constexpr int EW = 8192;        // lags per LDS window (64 KiB of 64-bit counters: two
-                                // workgroups per CU with the staging below)
 constexpr int ECAP = 256;       // events of one pixel staged in LDS per wave
 constexpr int EPF = ECAP / 64;  // event rounds of the next pixel prefetched in registers
 
+// LDS lag histogram of one work item: 32 KiB, so that four workgroups (16 waves, enough
+// to cover the dependent LDS reads of the searches) share a CU with their staging.
+//  * 8-bit counts: 8192 lags of 32-bit counters.  A pixel adds at most sum_k d_k^2 to
+//    any one lag (Cauchy-Schwarz: at a fixed lag each event pairs with at most one
+//    partner), so each wave adds to the histogram only while its pixels' sum of d^2
+//    stays below `budget`, and sends the pairs of a pixel beyond it straight to the
+//    global 64-bit numerators.  Four waves then bound every counter below 2^32 (uint8)
+//    or its magnitude below 2^31 (int8, two's complement), so it is exact.
+//  * uint16 counts: 4096 lags of 64-bit counters, exact for any data.
+template <typename T> struct EvHist {
+    using type = uint32_t;
+    static constexpr int ew = 8192;
+    static constexpr long long budget = std::is_signed<T>::value ? (1ll << 29) : (1ll << 30);
+    static __device__ __forceinline__ unsigned long long widen(uint32_t v) {
+        return std::is_signed<T>::value ? (unsigned long long)(long long)(int32_t)v
+                                        : (unsigned long long)v;
+    }
+};
+template <> struct EvHist<uint16_t> {
+    using type = unsigned long long;
+    static constexpr int ew = 4096;
+    static constexpr long long budget = 0;  // no bound needed
+    static __device__ __forceinline__ unsigned long long widen(unsigned long long v) { return v; }
+};
+
 template <typename T>
 __device__ __forceinline__ long long as_count(T v) { return (long long)v; }
+
+__device__ __forceinline__ long long wave_sum64(long long v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
 
 // Per-pixel bookkeeping of one wave: pixel index and event range, valid or not.
 struct EvPixel {
@@ -49,7 +78,10 @@ __global__ __launch_bounds__(256) void event_pairs_kernel(
     const uint32_t* __restrict__ offsets, const int32_t* __restrict__ pix_idx,
     const BinDev* __restrict__ bins, const EventWork* __restrict__ work, int nframes,
     int64_t nnz, int64_t* __restrict__ num, int64_t* __restrict__ sums, int* __restrict__ err) {
-    __shared__ unsigned long long hist[EW];
+    using H = typename EvHist<T>::type;
+    constexpr int EW = EvHist<T>::ew;
+    constexpr bool BOUNDED = sizeof(H) == 4;
+    __shared__ H hist[EW];
     __shared__ int st_t[4][ECAP];
     __shared__ int st_d[4][ECAP];  // uint8, int8 and uint16 counts fit 32 bits
     const EventWork w = work[blockIdx.x];
@@ -59,9 +91,11 @@ __global__ __launch_bounds__(256) void event_pairs_kernel(
     const int w1 = min(nframes, w0 + EW);
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    for (int k = threadIdx.x; k < EW; k += 256) hist[k] = 0ull;
+    for (int k = threadIdx.x; k < EW; k += 256) hist[k] = 0;
     __syncthreads();
     int64_t* srow = sums + (int64_t)w.bin * nframes;
+    unsigned long long* nwin = reinterpret_cast<unsigned long long*>(num + (int64_t)w.bin * nframes + w0);
+    long long used = 0;  // this wave's sum of d^2 added to the histogram (BOUNDED)
 
     // The wave takes pixels w.k0 + wave + 4r, r < nr.  Their list entries and offsets are
     // loaded 64 at a time (lane r % 64 holds pixel r), and while the pairs of pixel r are
@@ -116,6 +150,7 @@ __global__ __launch_bounds__(256) void event_pairs_kernel(
         const int n = cur.n;
         const uint32_t beg = cur.beg;
         const bool staged = n <= ECAP;
+        long long sq = 0;  // this lane's share of the pixel's sum of d^2 (BOUNDED)
         if (n > 0 && staged) {
 #pragma unroll
             for (int q = 0; q < EPF; ++q) {
@@ -123,6 +158,7 @@ __global__ __launch_bounds__(256) void event_pairs_kernel(
                 if (k < n) {
                     st_t[wave][k] = ct[q];
                     st_d[wave][k] = cd[q];
+                    if (BOUNDED) sq += (long long)cd[q] * cd[q];
                 }
             }
         }
@@ -145,62 +181,93 @@ __global__ __launch_bounds__(256) void event_pairs_kernel(
         }
         if (bad) atomicOr(err, 1);
         if (__any(bad)) continue;
-        for (int i = lane; i < n; i += 64) {
-            const int ti = tt[i];
-            const long long di = staged ? (long long)st_d[wave][i] : as_count<T>(data[beg + i]);
-            if (w0 == 0) {
-                atomicAdd(&hist[0], (unsigned long long)(di * di));
-                if (di != 0)
-                    atomicAdd(reinterpret_cast<unsigned long long*>(srow + ti),
-                              (unsigned long long)di);
-            }
-            // first j > i with t_j - t_i >= max(w0, 1)
-            const int target = ti + (w0 > 0 ? w0 : 1);
-            int lo = i + 1, hi = n;
-            while (lo < hi) {
-                const int mid = (lo + hi) >> 1;
-                if (tt[mid] < target) lo = mid + 1; else hi = mid;
-            }
-            // partners j in [lo, end): lag below the window's end (t_j < t_i + w1)
-            int end = n;
-            {
-                int a = lo, b2 = n;
-                const int lim = ti + w1;
-                while (a < b2) {
-                    const int mid = (a + b2) >> 1;
-                    if (tt[mid] < lim) a = mid + 1; else b2 = mid;
+        bool to_lds = true;
+        if constexpr (BOUNDED) {
+            if (!staged)
+                for (int k = lane; k < n; k += 64) {
+                    const long long d = as_count<T>(data[beg + k]);
+                    sq += d * d;
                 }
-                end = a;
-            }
-            // four partners per step, so that their reads and atomics overlap
-            int j = lo;
-            for (; j + 3 < end; j += 4) {
+            sq = wave_sum64(sq);
+            to_lds = used + sq < EvHist<T>::budget;
+            if (to_lds) used += sq;
+        }
+        auto add = [&](int lag, long long prod) {  // lag in [0, w1 - w0)
+            if (to_lds)
+                atomicAdd(&hist[lag], (H)prod);
+            else
+                atomicAdd(nwin + lag, (unsigned long long)prod);
+        };
+        // pairs (i, j), j > i, whose lag t_j - t_i falls in [max(w0, 1), w1): partners
+        // from the first j with t_j >= t_i + max(w0, 1) while t_j < t_i + w1, four per
+        // step so that their reads and atomics overlap.
+        auto partners = [&](int ti, long long di, int j) {
+            const int lim = ti + w1;
+            for (; j < n; j += 4) {
                 int tj[4];
                 long long dj[4];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
-                    tj[u] = tt[j + u];
-                    dj[u] = staged ? (long long)st_d[wave][j + u] : as_count<T>(data[beg + j + u]);
+                    const bool in = j + u < n;
+                    tj[u] = in ? tt[j + u] : lim;
+                    dj[u] = !in ? 0ll
+                                : staged ? (long long)st_d[wave][j + u]
+                                         : as_count<T>(data[beg + j + u]);
                 }
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     const long long prod = di * dj[u];
-                    if (prod != 0) atomicAdd(&hist[tj[u] - ti - w0], (unsigned long long)prod);
+                    if (tj[u] < lim && prod != 0) add(tj[u] - ti - w0, prod);
+                }
+                if (tj[3] >= lim) break;  // times increase: the rest lie past the window
+            }
+        };
+        // Each lane takes events i and i + 64 together, so that their two searches for
+        // the first partner overlap.
+        for (int i0 = lane; i0 < n; i0 += 128) {
+            const int i1 = i0 + 64;
+            const bool has1 = i1 < n;
+            const int ti0 = tt[i0];
+            const int ti1 = has1 ? tt[i1] : 0;
+            const long long d0 = staged ? (long long)st_d[wave][i0] : as_count<T>(data[beg + i0]);
+            const long long d1 = !has1 ? 0ll
+                                       : staged ? (long long)st_d[wave][i1]
+                                                : as_count<T>(data[beg + i1]);
+            if (w0 == 0) {
+                add(0, d0 * d0 + d1 * d1);
+                if (d0 != 0)
+                    atomicAdd(reinterpret_cast<unsigned long long*>(srow + ti0),
+                              (unsigned long long)d0);
+                if (d1 != 0)
+                    atomicAdd(reinterpret_cast<unsigned long long*>(srow + ti1),
+                              (unsigned long long)d1);
+            }
+            // first partner: i + 1 in the first window (times strictly increase), else
+            // the first j > i with t_j - t_i >= w0
+            int lo0 = i0 + 1, lo1 = i1 + 1;
+            if (w0 > 0) {
+                int hi0 = n, hi1 = has1 ? n : lo1;
+                const int g0 = ti0 + w0, g1 = ti1 + w0;
+                while (lo0 < hi0 || lo1 < hi1) {
+                    if (lo0 < hi0) {
+                        const int mid = (lo0 + hi0) >> 1;
+                        if (tt[mid] < g0) lo0 = mid + 1; else hi0 = mid;
+                    }
+                    if (lo1 < hi1) {
+                        const int mid = (lo1 + hi1) >> 1;
+                        if (tt[mid] < g1) lo1 = mid + 1; else hi1 = mid;
+                    }
                 }
             }
-            for (; j < end; ++j) {
-                const long long dj = staged ? (long long)st_d[wave][j] : as_count<T>(data[beg + j]);
-                const long long prod = di * dj;
-                if (prod != 0) atomicAdd(&hist[tt[j] - ti - w0], (unsigned long long)prod);
-            }
+            partners(ti0, d0, lo0);
+            if (has1) partners(ti1, d1, lo1);
         }
         __builtin_amdgcn_wave_barrier();
     }
     __syncthreads();
-    int64_t* nrow = num + (int64_t)w.bin * nframes;
     for (int k = threadIdx.x; k < w1 - w0; k += 256) {
-        const unsigned long long v = hist[k];
-        if (v != 0ull) atomicAdd(reinterpret_cast<unsigned long long*>(nrow + w0 + k), v);
+        const H v = hist[k];
+        if (v != 0) atomicAdd(nwin + k, EvHist<T>::widen(v));
     }
 }
 
@@ -209,17 +276,17 @@ void launch_event_pairs(int dtype, const int32_t* times, const void* data,
                         const EventWork* work, int n_work, int nframes, int64_t nnz,
                         int64_t* num, int64_t* sums, int* err, hipStream_t s) {
     if (n_work <= 0 || nframes <= 0) return;
-    dim3 grid((unsigned)n_work, (unsigned)((nframes + EW - 1) / EW));
+    auto grid = [&](int ew) { return dim3((unsigned)n_work, (unsigned)((nframes + ew - 1) / ew)); };
     if (dtype == DXG2_U16)
-        event_pairs_kernel<uint16_t><<<grid, 256, 0, s>>>(times, (const uint16_t*)data, offsets,
+        event_pairs_kernel<uint16_t><<<grid(EvHist<uint16_t>::ew), 256, 0, s>>>(times, (const uint16_t*)data, offsets,
                                                           pix_idx, bins, work, nframes, nnz,
                                                           num, sums, err);
     else if (dtype == DXG2_I8)
-        event_pairs_kernel<int8_t><<<grid, 256, 0, s>>>(times, (const int8_t*)data, offsets,
+        event_pairs_kernel<int8_t><<<grid(EvHist<int8_t>::ew), 256, 0, s>>>(times, (const int8_t*)data, offsets,
                                                         pix_idx, bins, work, nframes, nnz, num,
                                                         sums, err);
     else
-        event_pairs_kernel<uint8_t><<<grid, 256, 0, s>>>(times, (const uint8_t*)data, offsets,
+        event_pairs_kernel<uint8_t><<<grid(EvHist<uint8_t>::ew), 256, 0, s>>>(times, (const uint8_t*)data, offsets,
                                                          pix_idx, bins, work, nframes, nnz,
                                                          num, sums, err);
 }

@@ -123,6 +123,44 @@ def test_events_scale_factor_and_int8_and_goldens(golden):
         _check(c, g2, ref)
 
 
+@pytest.mark.parametrize("case", ["u8_staged", "u8_long", "i8_staged"])
+def test_events_saturated_pixels_stay_exact(case):
+    """Saturated counts on every pixel: 8-bit counts use 32-bit LDS lag counters while a
+    wave's sum of d^2 stays within its bound, and send the pairs of later pixels straight
+    to the 64-bit numerators.  Pixels with 256 staged events (uint8 from a dense stack,
+    against the dense oracle; int8 event lists of both signs, which the compressor
+    would not produce, against the event oracle) and pixels with 4000 unstaged events
+    (uint8) pass that bound within one wave."""
+    _announce("events saturated %s" % case)
+    from dynamix_amd.correlator.event import EventCorrelator
+    rng = np.random.default_rng(31)
+    if case == "i8_staged":
+        shape, n = (32, 32), 300
+        npx = shape[0] * shape[1]
+        times = np.concatenate([np.sort(rng.choice(n, size=256, replace=False))
+                                for _ in range(npx)]).astype(np.int32)
+        data = rng.choice(np.array([-128, 127], dtype=np.int8), size=times.size)
+        offsets = (256 * np.arange(npx + 1)).astype(np.uint32)
+        q = np.ones(shape, np.int32)
+        c = EventCorrelator(shape, n, 256, qmask=q, dtype=np.int8)
+        g2 = c.correlate(times, data, offsets)
+        _check(c, g2, ev.event_g2(times, data, offsets, q, n))
+        return
+    if case == "u8_long":
+        shape, n = (4, 5), 4000
+        f = np.full((n,) + shape, 255, dtype=np.uint8)
+    else:
+        shape, n = (32, 32), 300
+        f = np.zeros((n,) + shape, dtype=np.uint8)
+        for p in range(shape[0] * shape[1]):
+            f[rng.choice(n, size=256, replace=False), p // shape[1], p % shape[1]] = 255
+    q = np.ones(shape, np.int32)
+    data, times, offsets = ev.compress_all_stack(f)
+    c = EventCorrelator(shape, n, int(np.diff(offsets).max()), qmask=q, dtype=np.uint8)
+    g2 = c.correlate(times, data, offsets)
+    _check(c, g2, og.dense_g2(f, q))
+
+
 def test_events_repeat_calls_do_not_accumulate():
     _announce("events repeat")
     from dynamix_amd.correlator.event import EventCorrelator
