@@ -91,31 +91,46 @@ __device__ __forceinline__ void gather_general(const T* __restrict__ frames, int
     const uint32_t sel_hi = 0x00000c0cu | ((uint32_t)qi << 16) | ((uint32_t)(4 + qi) << 24);
     const int64_t plane = (int64_t)bd.nkb * nrows * KB;  // bytes between digit planes
 
-    for (int kb = w.kb0 + wave; kb < w.kb0 + w.nkb; kb += 4) {
-        const int p = pix_idx[bd.idx_off + (int64_t)kb * KB + lane];
-        const int pc = p >= 0 ? p : 0;
-        const uint32_t keep = p >= 0 ? 0xFFFFu : 0u;
-        int8_t* dst = slab + bd.slab_off;
+    // Wave w takes the 4 consecutive k-blocks kb0 + 4 w .. + 3 of each 16 and walks them
+    // frame by frame together, so that at each frame the workgroup reads up to 16
+    // consecutive k-blocks' pixels (up to 2 KiB of contiguous frame row for uint16).
+    const int kend = w.kb0 + w.nkb;
+    int8_t* dst = slab + bd.slab_off;
+    for (int kbase = w.kb0 + 4 * wave; kbase < kend; kbase += 16) {
+        const int nk = min(4, kend - kbase);
+        int pc[4];
+        uint32_t keep[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int p = j < nk ? pix_idx[bd.idx_off + (int64_t)(kbase + j) * KB + lane] : -1;
+            pc[j] = p >= 0 ? p : 0;
+            keep[j] = p >= 0 ? 0xFFFFu : 0u;
+        }
         for (int t = t0; t < tend; t += 4) {
-            uint32_t xl = 0, xh = 0;
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                const int tj = t + j < nframes ? t + j : nframes - 1;
-                const uint32_t v = (uint32_t)frames[(int64_t)tj * hw + pc] & keep;
-                xl |= (v & 0xFFu) << (8 * j);
-                if (WIDE) xh |= (v >> 8) << (8 * j);
-            }
-            uint32_t lo = quad_transpose(xl, sel_lo, sel_hi);
-            if (BIAS) lo ^= 0x80808080u;
-            const int64_t o = slab_offset(kb, t + qi, q4, nrows);
-            if (WIDE) {
-                const uint32_t hi = quad_transpose(xh, sel_lo, sel_hi) ^ 0x80808080u;
-                if (t + qi < nframes) {
-                    if (kb < hi_end) *reinterpret_cast<uint32_t*>(dst + o) = hi;
-                    *reinterpret_cast<uint32_t*>(dst + plane + o) = lo;
+                if (j >= nk) break;
+                const int kb = kbase + j;
+                uint32_t xl = 0, xh = 0;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int ti = t + i < nframes ? t + i : nframes - 1;
+                    const uint32_t v = (uint32_t)frames[(int64_t)ti * hw + pc[j]] & keep[j];
+                    xl |= (v & 0xFFu) << (8 * i);
+                    if (WIDE) xh |= (v >> 8) << (8 * i);
                 }
-            } else if (t + qi < nframes) {
-                *reinterpret_cast<uint32_t*>(dst + o) = lo;
+                uint32_t lo = quad_transpose(xl, sel_lo, sel_hi);
+                if (BIAS) lo ^= 0x80808080u;
+                const int64_t o = slab_offset(kb, t + qi, q4, nrows);
+                if (WIDE) {
+                    const uint32_t hi = quad_transpose(xh, sel_lo, sel_hi) ^ 0x80808080u;
+                    if (t + qi < nframes) {
+                        if (kb < hi_end) *reinterpret_cast<uint32_t*>(dst + o) = hi;
+                        *reinterpret_cast<uint32_t*>(dst + plane + o) = lo;
+                    }
+                } else if (t + qi < nframes) {
+                    *reinterpret_cast<uint32_t*>(dst + o) = lo;
+                }
             }
         }
     }
