@@ -2,14 +2,15 @@
 // two digit planes).
 //
 // The reference gathers one bin at a time on the host (correlator/dense.py:48-49,
-// dense.py:317-319).  Here one launch covers every bin.  A wave owns one k-block (64
-// consecutive entries of a bin's raster-ordered pixel list) and a 64-frame tile:
+// dense.py:317-319).  Here one launch covers every bin.  A wave owns four k-blocks (64
+// consecutive entries each of a bin's raster-ordered pixel list) and a frame tile:
 //   * each load instruction reads entry `lane` of one frame, so the 64 lanes read 64
 //     consecutive list entries -- runs of neighbouring pixels, few cache lines;
-//   * after four frames each lane holds 4 frames x 1 pixel; a 4x4 byte transpose
-//     inside every lane quad (DPP quad broadcast + v_perm) gives lane 4q+i the dword
-//     of pixels 4q..4q+3 for frame t+i, so one store instruction writes rows t..t+3
-//     of the k-block: four 64-byte runs of the fragment-contiguous slab layout.
+//   * after eight frames each lane holds 8 frames x 1 pixel per k-block; a 4x4 byte
+//     transpose inside every lane quad (DPP quad broadcast + v_perm) gives lane 4q+i
+//     the dword of pixels 4q..4q+3 for frame t+i (and t+4+i), so one store instruction
+//     writes rows t..t+3 of the k-block: four 64-byte runs of the fragment-contiguous
+//     slab layout.
 // The per-frame bin sums are formed later by the Gram kernel's diagonal tiles.
 //
 // For uint8/int8 frames the band gather below does most k-blocks: it reads the frames
@@ -65,7 +66,7 @@ __device__ __forceinline__ uint32_t quad_transpose(uint32_t x, uint32_t sel_lo, 
 }
 
 // Work layout: blockIdx.x -> GatherWork (bin, first k-block, k-block count <= 16),
-// blockIdx.y -> 64-frame tile.  Wave w handles k-blocks kb0 + w, kb0 + w + 4, ...
+// blockIdx.y -> 64-frame tile.  Wave w handles k-blocks kb0 + 4 w .. kb0 + 4 w + 3.
 // BIAS (uint8 and uint16 input): bytes are stored as v ^ 0x80 = v - 128 (i8 MFMA
 // operands); padded list entries (index -1) read as 0 and become -128 the same way.
 // T = uint16_t writes the high bytes to k-block kb and the low bytes to k-block
@@ -80,6 +81,7 @@ __device__ __forceinline__ void gather_general(const T* __restrict__ frames, int
                                                int t0, int tend, int8_t* __restrict__ slab,
                                                const int* __restrict__ nwkb = nullptr) {
     constexpr bool WIDE = sizeof(T) == 2;
+    constexpr int FG = 8;           // frames per step
     const BinDev bd = bins[w.bin];
     const int hi_end = WIDE && nwkb ? nwkb[w.bin] : bd.nkb;  // k-blocks whose high bytes go out
     const int lane = threadIdx.x & 63;
@@ -92,44 +94,65 @@ __device__ __forceinline__ void gather_general(const T* __restrict__ frames, int
     const int64_t plane = (int64_t)bd.nkb * nrows * KB;  // bytes between digit planes
 
     // Wave w takes the 4 consecutive k-blocks kb0 + 4 w .. + 3 of each 16 and walks them
-    // frame by frame together, so that at each frame the workgroup reads up to 16
+    // FG frames at a time together, so that at each frame the workgroup reads up to 16
     // consecutive k-blocks' pixels (up to 2 KiB of contiguous frame row for uint16).
+    // Slots past the item's last k-block repeat that k-block: every step then issues
+    // all 4 FG = 32 loads before its first wait (one basic block, so the waits count
+    // down through them), and a repeat stores the same bytes to the same addresses.
+    // Whole-line stores (an exchange across the wave halves so that each store writes
+    // two 128-byte slab lines) measured slower here than these 64-byte runs.
     const int kend = w.kb0 + w.nkb;
     int8_t* dst = slab + bd.slab_off;
     for (int kbase = w.kb0 + 4 * wave; kbase < kend; kbase += 16) {
         const int nk = min(4, kend - kbase);
-        int pc[4];
+        int kbj[4];
+        const T* src[4];
         uint32_t keep[4];
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            const int p = j < nk ? pix_idx[bd.idx_off + (int64_t)(kbase + j) * KB + lane] : -1;
-            pc[j] = p >= 0 ? p : 0;
+            kbj[j] = kbase + min(j, nk - 1);
+            const int p = pix_idx[bd.idx_off + (int64_t)kbj[j] * KB + lane];
+            src[j] = frames + (p >= 0 ? p : 0);
             keep[j] = p >= 0 ? 0xFFFFu : 0u;
         }
-        for (int t = t0; t < tend; t += 4) {
+        for (int t = t0; t < tend; t += FG) {
+            uint32_t v[4][FG];
+#pragma unroll
+            for (int i = 0; i < FG; ++i) {
+                const int64_t ro = (int64_t)(t + i < nframes ? t + i : nframes - 1) * hw;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) v[j][i] = (uint32_t)src[j][ro];
+            }
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                if (j >= nk) break;
-                const int kb = kbase + j;
-                uint32_t xl = 0, xh = 0;
+                // dwords of frames t + qi (a) and t + 4 + qi (b), low and high digits;
+                // the transposes run with every lane active (DPP reads the quad's other
+                // lanes, including those past the last frame)
+                uint32_t d[2][2];
 #pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    const int ti = t + i < nframes ? t + i : nframes - 1;
-                    const uint32_t v = (uint32_t)frames[(int64_t)ti * hw + pc[j]] & keep[j];
-                    xl |= (v & 0xFFu) << (8 * i);
-                    if (WIDE) xh |= (v >> 8) << (8 * i);
-                }
-                uint32_t lo = quad_transpose(xl, sel_lo, sel_hi);
-                if (BIAS) lo ^= 0x80808080u;
-                const int64_t o = slab_offset(kb, t + qi, q4, nrows);
-                if (WIDE) {
-                    const uint32_t hi = quad_transpose(xh, sel_lo, sel_hi) ^ 0x80808080u;
-                    if (t + qi < nframes) {
-                        if (kb < hi_end) *reinterpret_cast<uint32_t*>(dst + o) = hi;
-                        *reinterpret_cast<uint32_t*>(dst + plane + o) = lo;
+                for (int g = 0; g < 2; ++g) {
+                    uint32_t xl = 0, xh = 0;
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        const uint32_t x = v[j][4 * g + i] & keep[j];
+                        xl |= (x & 0xFFu) << (8 * i);
+                        if (WIDE) xh |= (x >> 8) << (8 * i);
                     }
-                } else if (t + qi < nframes) {
-                    *reinterpret_cast<uint32_t*>(dst + o) = lo;
+                    d[g][0] = quad_transpose(xl, sel_lo, sel_hi) ^ (BIAS ? 0x80808080u : 0u);
+                    d[g][1] = WIDE ? quad_transpose(xh, sel_lo, sel_hi) ^ 0x80808080u : 0u;
+                }
+#pragma unroll
+                for (int g = 0; g < 2; ++g) {
+                    const int tq = t + 4 * g + qi;
+                    const int64_t o = slab_offset(kbj[j], tq, q4, nrows);
+                    if (tq < nframes) {
+                        if (WIDE) {
+                            if (kbj[j] < hi_end) *reinterpret_cast<uint32_t*>(dst + o) = d[g][1];
+                            *reinterpret_cast<uint32_t*>(dst + plane + o) = d[g][0];
+                        } else {
+                            *reinterpret_cast<uint32_t*>(dst + o) = d[g][0];
+                        }
+                    }
                 }
             }
         }
