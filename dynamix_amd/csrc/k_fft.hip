@@ -311,10 +311,9 @@ __device__ __forceinline__ void dft16(c32 (&a)[16]) {
     for (int q0 = 0; q0 < 4; ++q0) dft4(a[4 * q0], a[4 * q0 + 1], a[4 * q0 + 2], a[4 * q0 + 3]);
 }
 
-// one radix-16 pass of stride NS (1, 16 or 256) for butterfly j = threadIdx.x
+// one radix-16 pass of stride NS (1, 16 or 256) for butterfly j (the thread index)
 template <int NS>
-__device__ __forceinline__ void pass16(c32* z, const float2* __restrict__ tw) {
-    const int j = threadIdx.x;
+__device__ __forceinline__ void pass16(c32* z, const float2* __restrict__ tw, int j) {
     const int k = j & (NS - 1);
     c32 a[16];
 #pragma unroll
@@ -353,10 +352,16 @@ __global__ __launch_bounds__(F16_T) void fft16_power_kernel(
     const int kend = min(KB, bd.npix - w.kb * KB);
     for (int jp = 0; jp < kend; ++jp) {
         const uint8_t* x = xs + bd.slab_off + ((int64_t)w.kb * KB + jp) * nrows;
+        // A thread's addresses, twiddles and unpack factors are the same for every
+        // series; an opaque copy of the thread index per series keeps the compiler from
+        // hoisting them all out of this loop into registers (which spilled to scratch):
+        // they are recomputed, and the twiddles re-read from L1/L2, instead.
+        int tj = tid;
+        asm volatile("" : "+v"(tj));
         // z[n] = x[2n] + i x[2n+1] (zero past the last frame)
 #pragma unroll
         for (int m = 0; m < L / F16_T; ++m) {
-            const int n = tid + m * F16_T;
+            const int n = tj + m * F16_T;
             const int t = 2 * n;
             float re = 0.f, im = 0.f;
             if (t + 1 < nrows) {
@@ -367,14 +372,14 @@ __global__ __launch_bounds__(F16_T) void fft16_power_kernel(
             z[fpad(n)] = c32{re, im};
         }
         __syncthreads();
-        pass16<1>(z, tw);
-        pass16<16>(z, tw);
-        pass16<256>(z, tw);
+        pass16<1>(z, tw, tj);
+        pass16<16>(z, tw, tj);
+        pass16<256>(z, tw, tj);
         // radix-4 pass of stride 4096: butterfly jj reads and writes z[jj + 4096 r] (jj <
         // 4096), so each thread works in place on its own four points
 #pragma unroll
         for (int m = 0; m < 4; ++m) {
-            const int jj = tid + m * F16_T;
+            const int jj = tj + m * F16_T;
             c32 v[4];
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
@@ -394,7 +399,7 @@ __global__ __launch_bounds__(F16_T) void fft16_power_kernel(
         // W = exp(-pi i / L); accumulate |X[k]|^2
 #pragma unroll
         for (int m = 0; m < NK; ++m) {
-            const int k = tid + m * F16_T;
+            const int k = tj + m * F16_T;
             if (k <= L) {
                 const c32 a = z[fpad(k & (L - 1))];
                 const c32 b = z[fpad((L - k) & (L - 1))];
