@@ -264,7 +264,7 @@ struct dxg2_plan {
     std::vector<KbWork> kb_work;   // every k-block of every bin
     KbWork* d_kbwork = nullptr;
     uint8_t* d_xs = nullptr;       // pixel-major bytes, same size as the slab
-    float2* d_tw = nullptr;        // exp(-2 pi i m / L)
+    float2* d_tw = nullptr;        // exp(-2 pi i m / L), m < L; then exp(-pi i k / L), k <= L
     double* d_power = nullptr;     // (n_bins, L + 1)
     double* d_fnum = nullptr;      // (n_bins, N)
     int fft_L = 0;
@@ -586,13 +586,19 @@ int ensure_fft_buffers(dxg2_plan* p, hipStream_t s) {
     p->fft_L = L;
     for (int b = 0; b < p->n_bins; ++b)
         for (int kb = 0; kb < p->bins[b].nkb; ++kb) p->kb_work.push_back(KbWork{b, kb});
-    p->twiddles.resize(L);
+    // [0, L): exp(-2 pi i m / L), the transform's twiddles; [L, 2L]: exp(-pi i k / L),
+    // k = 0..L, the factor that unpacks the half-length complex transform
+    p->twiddles.resize(2 * (size_t)L + 1);
     for (int m = 0; m < L; ++m) {
         const double a = -2.0 * M_PI * (double)m / (double)L;
         p->twiddles[m] = make_float2((float)std::cos(a), (float)std::sin(a));
     }
+    for (int k = 0; k <= L; ++k) {
+        const double a = -M_PI * (double)k / (double)L;
+        p->twiddles[L + k] = make_float2((float)std::cos(a), (float)std::sin(a));
+    }
     if (dev_alloc(p->alloc, (void**)&p->d_xs, p->slab_bytes, s) != hipSuccess ||
-        dev_alloc(p->alloc, (void**)&p->d_tw, (size_t)L * sizeof(float2), s) != hipSuccess ||
+        dev_alloc(p->alloc, (void**)&p->d_tw, p->twiddles.size() * sizeof(float2), s) != hipSuccess ||
         dev_alloc(p->alloc, (void**)&p->d_power, (size_t)std::max(p->n_bins, 1) * (L + 1) * sizeof(double), s) !=
             hipSuccess ||
         dev_alloc(p->alloc, (void**)&p->d_fnum, (size_t)std::max(p->n_bins, 1) * p->N * sizeof(double), s) !=
@@ -600,7 +606,7 @@ int ensure_fft_buffers(dxg2_plan* p, hipStream_t s) {
         (!p->kb_work.empty() &&
          dev_alloc(p->alloc, (void**)&p->d_kbwork, p->kb_work.size() * sizeof(KbWork), s) != hipSuccess))
         return set_error(DXG2_ENOMEM, "cannot allocate FFT workspace");
-    DXG2_HIP(hipMemcpyAsync(p->d_tw, p->twiddles.data(), (size_t)L * sizeof(float2),
+    DXG2_HIP(hipMemcpyAsync(p->d_tw, p->twiddles.data(), p->twiddles.size() * sizeof(float2),
                             hipMemcpyHostToDevice, s));
     if (!p->kb_work.empty())
         DXG2_HIP(hipMemcpyAsync(p->d_kbwork, p->kb_work.data(),
@@ -1046,7 +1052,7 @@ int dxg2_plan_footprint(const dxg2_plan* p, int32_t path, size_t* bytes) {
         }
         case 2: {
             const int L = fft_length(p->N);
-            *bytes = common + p->slab_bytes + (size_t)L * sizeof(float2) +
+            *bytes = common + p->slab_bytes + (2 * (size_t)L + 1) * sizeof(float2) +
                      (size_t)std::max(p->n_bins, 1) * ((L + 1) + p->N) * sizeof(double);
             return DXG2_OK;
         }

@@ -140,8 +140,8 @@ __device__ __forceinline__ c32 csub(c32 a, c32 b) { return c32{a.x - b.x, a.y - 
 __device__ __forceinline__ c32 cmuli_neg(c32 a) { return c32{a.y, -a.x}; }
 
 // grid: one workgroup per (bin, k-block) of 64 pixels.  LDS holds L complex points.
-// tw[m] = exp(-2 pi i m / L), m < L (host FP64, rounded); the unpacking factor
-// exp(-pi i k / L) comes from sincospif.
+// tw[m] = exp(-2 pi i m / L), m < L, and tw[L + k] = exp(-pi i k / L), k <= L, the
+// unpacking factor (host FP64, rounded to FP32).
 template <int L>
 __global__ __launch_bounds__(FftCfg<L>::THREADS) void fft_power_kernel(
     const uint8_t* __restrict__ xs, const BinDev* __restrict__ bins,
@@ -241,9 +241,8 @@ __global__ __launch_bounds__(FftCfg<L>::THREADS) void fft_power_kernel(
                 const c32 b = z[(L - k) & (L - 1)];
                 const c32 s = c32{a.x + b.x, a.y - b.y};   // Z[k] + conj Z[L-k]
                 const c32 d = c32{a.x - b.x, a.y + b.y};   // Z[k] - conj Z[L-k]
-                float sn, cs;
-                sincospif((float)k / (float)L, &sn, &cs);   // W^k = cos - i sin (pi k / L)
-                const c32 wd = cmul(c32{cs, -sn}, d);
+                const float2 u = tw[L + k];                 // W^k = exp(-pi i k / L)
+                const c32 wd = cmul(c32{u.x, u.y}, d);
                 const float xr = 0.5f * (s.x + wd.y);      // s/2 - i wd/2
                 const float xi = 0.5f * (s.y - wd.x);
                 acc[m] += (double)xr * xr + (double)xi * xi;
@@ -405,9 +404,8 @@ __global__ __launch_bounds__(F16_T) void fft16_power_kernel(
                 const c32 b = z[fpad((L - k) & (L - 1))];
                 const c32 s2 = c32{a.x + b.x, a.y - b.y};
                 const c32 d = c32{a.x - b.x, a.y + b.y};
-                float sn, cs;
-                sincospif((float)k / (float)L, &sn, &cs);
-                const c32 wd = cmul(c32{cs, -sn}, d);
+                const float2 u = tw[L + k];  // W^k = exp(-pi i k / L)
+                const c32 wd = cmul(c32{u.x, u.y}, d);
                 const float xr = 0.5f * (s2.x + wd.y);
                 const float xi = 0.5f * (s2.y - wd.x);
                 acc[m] += (double)xr * xr + (double)xi * xi;
