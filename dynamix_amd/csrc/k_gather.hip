@@ -81,7 +81,7 @@ __device__ __forceinline__ void gather_general(const T* __restrict__ frames, int
                                                int t0, int tend, int8_t* __restrict__ slab,
                                                const int* __restrict__ nwkb = nullptr) {
     constexpr bool WIDE = sizeof(T) == 2;
-    constexpr int FG = 8;           // frames per step
+    constexpr int FG = 8;  // frames per step (a multiple of 4 dividing 64; 16 measured slower)
     const BinDev bd = bins[w.bin];
     const int hi_end = WIDE && nwkb ? nwkb[w.bin] : bd.nkb;  // k-blocks whose high bytes go out
     const int lane = threadIdx.x & 63;
@@ -125,12 +125,12 @@ __device__ __forceinline__ void gather_general(const T* __restrict__ frames, int
             }
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                // dwords of frames t + qi (a) and t + 4 + qi (b), low and high digits;
+                // dwords of frames t + 4 g + qi, low and high digits;
                 // the transposes run with every lane active (DPP reads the quad's other
                 // lanes, including those past the last frame)
-                uint32_t d[2][2];
+                uint32_t d[FG / 4][2];
 #pragma unroll
-                for (int g = 0; g < 2; ++g) {
+                for (int g = 0; g < FG / 4; ++g) {
                     uint32_t xl = 0, xh = 0;
 #pragma unroll
                     for (int i = 0; i < 4; ++i) {
@@ -142,7 +142,7 @@ __device__ __forceinline__ void gather_general(const T* __restrict__ frames, int
                     d[g][1] = WIDE ? quad_transpose(xh, sel_lo, sel_hi) ^ 0x80808080u : 0u;
                 }
 #pragma unroll
-                for (int g = 0; g < 2; ++g) {
+                for (int g = 0; g < FG / 4; ++g) {
                     const int tq = t + 4 * g + qi;
                     const int64_t o = slab_offset(kbj[j], tq, q4, nrows);
                     if (tq < nframes) {
