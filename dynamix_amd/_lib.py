@@ -84,6 +84,8 @@ _SIGNATURES = {
     "dxg2_events_window": (c_int, [c_void_p, c_void_p, c_void_p, c_i64, c_i64, c_int, c_int,
                                    c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_i64,
                                    c_void_p]),
+    "dxg2_events_to_columns": (c_int, [c_void_p, c_void_p, c_void_p, c_i64, c_int, c_int,
+                                       c_void_p, c_i64, c_void_p, c_void_p, c_void_p]),
     "dxg2_events_to_coo": (c_int, [c_void_p, c_void_p, c_void_p, c_i64, c_int, c_int, c_int,
                                    c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     "dxg2_compress_scratch_bytes": (c_int, [c_int, c_int, c_int, ctypes.POINTER(c_size_t)]),

@@ -44,6 +44,7 @@ class _DenseHIP(BaseCorrelator):
         self._plans = {}
         self._tuned = set()
         self._graph_bufs = {}
+        self._last_plan = None
         self.last_numerators = None
         self.last_sums = None
         self.last_denominators = None
@@ -82,10 +83,8 @@ class _DenseHIP(BaseCorrelator):
     def _run_dense(self, frames, calc_std=False):
         torch = _torch()
         d_frames, dtype = to_device_frames(frames, self.shape, self.nframes, torch)
-        if dtype not in (np.dtype(np.uint8), np.dtype(np.int8), np.dtype(np.uint16)):
-            raise NotImplementedError("the dense i8 path takes uint8, int8 or uint16 frames, "
-                                      "got %s" % dtype)
         plan = self._plan(dtype)
+        self._last_plan = plan
         nb = max(self.n_bins, 0)
         if nb and self.extra_options.get("autotune", False) and dtype not in self._tuned:
             from .. import autotune
@@ -127,8 +126,9 @@ class _DenseHIP(BaseCorrelator):
 
     def two_time(self, frames, q):
         """Two-time correlation matrix of bin q (1-based): float32 (nframes, nframes),
-        the `trc` the reference's event correlators return for ttcf_par = q
-        (event_y.py:137-157), from the exact integer Gram in FP64."""
+        the `trc` the reference's event correlator ecorrts_q returns for ttcf_par = q
+        (event_y.py:169-230 with WXPCS/fecorrt.f:1-26: n_b every pixel of the bin, the
+        pair counts symmetrised, the diagonal rule), from the exact integer Gram in FP64."""
         torch = _torch()
         d_frames, dtype = to_device_frames(frames, self.shape, self.nframes, torch)
         plan = self._plan(dtype)
@@ -162,8 +162,11 @@ class MatMulCorrelator(_DenseHIP):
 class DenseCorrelator(_DenseHIP):
     """Drop-in for the OpenCL DenseCorrelator (dense.py:145-247).
 
-    `dtype` is the frames dtype; correlate() asserts the frames match it, as the
-    reference's _set_data does (common.py:178-180).  ctx/devicetype/platformid/
+    `dtype` is the frames dtype (default "f", float32, as in the reference);
+    correlate() asserts numpy frames match it, as the reference's _set_data does
+    (common.py:178-180).  Frames of any numeric dtype run on the exact integer path in
+    the narrowest of uint8 / int8 / uint16 that holds their values (narrow_counts:
+    integer counts in [0, 65535] or [-128, 127], else ValueError).  ctx/devicetype/platformid/
     deviceid/block_size/memory are OpenCL device selection and are ignored; `profile`
     turns on per-stage HIP event timing (see `stage_times`).
     """
@@ -175,12 +178,8 @@ class DenseCorrelator(_DenseHIP):
         super()._set_parameters(shape, nframes, qmask, None, extra_options)
         self._set_weights(weights)
         self.dtype = np.dtype(dtype)
-        if self.dtype not in (np.dtype(np.uint8), np.dtype(np.int8)):
-            raise NotImplementedError("DenseCorrelator on the HIP path takes uint8 or int8 frames")
         self.profile = bool(profile)
         self._init_dense()
-        if self.profile:
-            self._plan(self.dtype).set_profiling(True)
 
     def correlate(self, frames):
         dt = frames.dtype if isinstance(frames, np.ndarray) else None
@@ -188,10 +187,20 @@ class DenseCorrelator(_DenseHIP):
             assert np.dtype(dt) == self.dtype, "frames dtype %s != %s" % (dt, self.dtype)
         return self._run_dense(frames)
 
+    def _plan(self, dtype):
+        fresh = np.dtype(dtype) not in self._plans
+        p = super()._plan(dtype)
+        if fresh and self.profile:
+            p.set_profiling(True)
+        return p
+
     def stage_times(self):
         """Average [gather, gram, finalise, total] ms and the number of calls averaged,
-        over the calls since the last read (profile=True)."""
-        return self._plan(self.dtype).stage_times()
+        over the calls since the last read (profile=True), of the plan the last call used
+        (frames of dtype "f" run on the narrowest exact integer type of their values)."""
+        if self._last_plan is None:
+            return [0.0] * 5
+        return self._last_plan.stage_times()
 
 
 class PlainMatMulCorrelator(_DenseHIP):
@@ -247,7 +256,7 @@ class FFTCorrelator(_DenseHIP):
         self._init_dense()
         self.precompute_fft_plans = precompute_fft_plans
         self.Nf = fft_length(self.nframes)
-        self.n_pix = {int(b): int(np.sum(self.qmask == b)) for b in self.bins}
+        self.n_pix = {int(b): int(n) for b, n in zip(self.bins, self.pixel_counts)}
 
     def correlate(self, frames):
         torch = _torch()

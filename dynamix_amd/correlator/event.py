@@ -20,7 +20,7 @@ import ctypes
 import numpy as np
 
 from .. import _lib
-from .common import BaseCorrelator, Plan, _torch, current_stream
+from .common import EXACT_DTYPES, BaseCorrelator, Plan, _torch, current_stream, narrow_counts
 
 _FLAG_TEXT = {1: "frames out of [0, nframes) or not strictly increasing within a pixel",
               2: "offsets decreasing or past the number of events",
@@ -50,17 +50,29 @@ class EventCorrelator(BaseCorrelator):
         super().__init__()
         super()._set_parameters(shape, nframes, qmask, scale_factor, extra_options)
         self._set_weights(weights)
-        self.dtype = _dtype_of(dtype)
+        self.dtype = np.dtype(dtype)
         self._set_events_size(max_events_count, total_events_count)
         self.allow_reallocate = allow_reallocate
-        q = self.qmask if self._qmask_given else None
-        scale = np.array([self.scale_factors[b] for b in self.bins], dtype=np.float64) \
-            if self.n_bins > 0 else None
-        self._plan = Plan(self.shape, self.nframes, q, self.dtype, scale=scale)
+        self._plans = {}
+        self._ttcf = {}
+        if self.dtype in EXACT_DTYPES:
+            self._plan_for(self.dtype)   # built up front, as the reference allocates
         self.last_numerators = None
         self.last_sums = None
         self.last_denominators = None
         self.last_g2 = None
+
+    def _plan_for(self, dtype):
+        """The plan of one count type (counts of another dtype are narrowed first)."""
+        p = self._plans.get(dtype)
+        if p is None:
+            q = self.qmask if self._qmask_given else None
+            scale = np.array([self.scale_factors[b] for b in self.bins], dtype=np.float64) \
+                if self.n_bins > 0 else None
+            p = Plan(self.shape, self.nframes, q, dtype, scale=scale)
+            self._plans[dtype] = p
+        self._plan = p
+        return p
 
     def _set_events_size(self, max_events_count, total_events_count):  # event.py:61-63
         self.max_events_count = max_events_count
@@ -114,12 +126,14 @@ class EventCorrelator(BaseCorrelator):
             assert np.dtype(vol_data.dtype) == self.dtype, \
                 "vol_data dtype %s != %s" % (vol_data.dtype, self.dtype)
         t = self._dev(torch, vol_times, np.int32)
-        d = self._dev(torch, vol_data, self.dtype)
+        d, dt = narrow_counts(vol_data, torch, "event counts")
+        d = self._dev(torch, d, dt)
         o = self._dev(torch, offsets, np.uint32)
+        plan = self._plan_for(dt)
         outs = self._outputs(torch)
         if self.n_bins:
-            _lib.check(self._plan.lib.dxg2_correlate_events_csr(
-                self._plan.handle, _lib.ptr(t), _lib.ptr(d), _lib.ptr(o),
+            _lib.check(plan.lib.dxg2_correlate_events_csr(
+                plan.handle, _lib.ptr(t), _lib.ptr(d), _lib.ptr(o),
                 ctypes.c_int64(int(vol_data.shape[0])), *[_lib.ptr(x) for x in outs],
                 current_stream(torch)), "dxg2_correlate_events_csr")
         return self._finish(torch, outs)
@@ -135,14 +149,62 @@ class EventCorrelator(BaseCorrelator):
                 raise ValueError("Too many events and allow_reallocate was set to False")
         px = self._dev(torch, pixel, np.uint32)
         fr = self._dev(torch, frame, np.uint32)
-        ct = self._dev(torch, count, self.dtype)
+        ct, dt = narrow_counts(count, torch, "event counts")
+        if dt == np.dtype(np.int8):
+            raise ValueError("frame-sorted event counts are unsigned (uint8 or uint16)")
+        ct = self._dev(torch, ct, dt)
+        plan = self._plan_for(dt)
         outs = self._outputs(torch)
         if self.n_bins:
-            _lib.check(self._plan.lib.dxg2_correlate_events_coo(
-                self._plan.handle, _lib.ptr(px), _lib.ptr(fr), _lib.ptr(ct),
+            _lib.check(plan.lib.dxg2_correlate_events_coo(
+                plan.handle, _lib.ptr(px), _lib.ptr(fr), _lib.ptr(ct),
                 ctypes.c_int64(int(pixel.shape[0])), *[_lib.ptr(x) for x in outs],
                 current_stream(torch)), "dxg2_correlate_events_coo")
         return self._finish(torch, outs)
+
+    def two_time(self, vol_times, vol_data, offsets, q):
+        """Two-time correlation matrix of bin q (1-based) from pixel-major events: float32
+        (nframes, nframes), the `trc` the reference's event correlator ecorrts_q returns
+        for ttcf_par = q (event_y.py:169-230 with WXPCS/fecorrt.f:1-26).  The bin's events
+        become a dense (nframes, n_b) stack on the GPU (dxg2_events_to_columns), whose
+        two-time matrix comes from the exact integer Gram in FP64
+        (dxg2_correlate_dense_ttcf) with n_b = every pixel of the bin."""
+        from .dense import MatMulCorrelator
+        torch = _torch()
+        self._check_event_arrays(vol_times, vol_data, offsets)
+        if not 1 <= int(q) <= max(self.n_bins, 0):
+            raise ValueError("bin %d is not in 1..%d" % (int(q), max(self.n_bins, 0)))
+        n = self.nframes
+        cols = np.flatnonzero(self.qmask.ravel() == int(q)).astype(np.int32)
+        if cols.size == 0:
+            trc = torch.full((n, n), float("nan"), dtype=torch.float32, device="cuda")
+        else:
+            d, dt = narrow_counts(vol_data, torch, "event counts")
+            t = self._dev(torch, vol_times, np.int32)
+            d = self._dev(torch, d, dt)
+            o = self._dev(torch, offsets, np.uint32)
+            tdt = {np.dtype(np.uint8): torch.uint8, np.dtype(np.int8): torch.int8,
+                   np.dtype(np.uint16): torch.uint16}[dt]
+            dense = torch.empty((n, 1, cols.size), dtype=tdt, device="cuda")
+            dcols = torch.from_numpy(cols).to("cuda")
+            err = torch.zeros(1, dtype=torch.int32, device="cuda")
+            lib = _lib.load()
+            _lib.check(lib.dxg2_events_to_columns(
+                _lib.ptr(o), _lib.ptr(t), _lib.ptr(d), int(np.prod(self.shape)), n,
+                _lib.DTYPE_CODES[dt], _lib.ptr(dcols), int(cols.size), _lib.ptr(dense),
+                _lib.ptr(err), current_stream(torch)), "dxg2_events_to_columns")
+            if int(err.item()):
+                raise ValueError("invalid event input: frames out of [0, nframes)")
+            key = (int(q), dt)
+            sub = self._ttcf.get(key)
+            if sub is None:
+                sub = MatMulCorrelator((1, int(cols.size)), n, None,
+                                       extra_options={"return_device": True})
+                self._ttcf[key] = sub
+            trc = sub.two_time(dense, 1)
+        if self.extra_options.get("return_device", False):
+            return trc
+        return trc.cpu().numpy()
 
 
 def _np_dtype_of_tensor(t):

@@ -579,6 +579,7 @@ int fft_length(int n) {
 
 int ensure_fft_buffers(dxg2_plan* p, hipStream_t s) {
     if (p->d_power) return DXG2_OK;
+    p->kb_work.clear();
     const int L = fft_length(p->N);
     if (L > FFT_MAX_L)
         return set_error(DXG2_EUNSUPPORTED, "FFT path supports up to %d frames (got %d)",
@@ -597,15 +598,29 @@ int ensure_fft_buffers(dxg2_plan* p, hipStream_t s) {
         const double a = -M_PI * (double)k / (double)L;
         p->twiddles[L + k] = make_float2((float)std::cos(a), (float)std::sin(a));
     }
-    if (dev_alloc(p->alloc, (void**)&p->d_xs, p->slab_bytes, s) != hipSuccess ||
-        dev_alloc(p->alloc, (void**)&p->d_tw, p->twiddles.size() * sizeof(float2), s) != hipSuccess ||
-        dev_alloc(p->alloc, (void**)&p->d_power, (size_t)std::max(p->n_bins, 1) * (L + 1) * sizeof(double), s) !=
-            hipSuccess ||
-        dev_alloc(p->alloc, (void**)&p->d_fnum, (size_t)std::max(p->n_bins, 1) * p->N * sizeof(double), s) !=
+    // d_power is allocated last: it marks the workspace complete.  A failure releases
+    // what this call allocated, so a later call starts again from nothing.
+    uint8_t* xs = nullptr;
+    float2* tw = nullptr;
+    double *power = nullptr, *fnum = nullptr;
+    KbWork* kbw = nullptr;
+    if (dev_alloc(p->alloc, (void**)&xs, p->slab_bytes, s) != hipSuccess ||
+        dev_alloc(p->alloc, (void**)&tw, p->twiddles.size() * sizeof(float2), s) != hipSuccess ||
+        dev_alloc(p->alloc, (void**)&fnum, (size_t)std::max(p->n_bins, 1) * p->N * sizeof(double), s) !=
             hipSuccess ||
         (!p->kb_work.empty() &&
-         dev_alloc(p->alloc, (void**)&p->d_kbwork, p->kb_work.size() * sizeof(KbWork), s) != hipSuccess))
+         dev_alloc(p->alloc, (void**)&kbw, p->kb_work.size() * sizeof(KbWork), s) != hipSuccess) ||
+        dev_alloc(p->alloc, (void**)&power, (size_t)std::max(p->n_bins, 1) * (L + 1) * sizeof(double), s) !=
+            hipSuccess) {
+        for (void* b : {(void*)xs, (void*)tw, (void*)fnum, (void*)kbw, (void*)power})
+            dev_free(p->alloc, b, s);
         return set_error(DXG2_ENOMEM, "cannot allocate FFT workspace");
+    }
+    p->d_xs = xs;
+    p->d_tw = tw;
+    p->d_fnum = fnum;
+    p->d_kbwork = kbw;
+    p->d_power = power;
     DXG2_HIP(hipMemcpyAsync(p->d_tw, p->twiddles.data(), p->twiddles.size() * sizeof(float2),
                             hipMemcpyHostToDevice, s));
     if (!p->kb_work.empty())
@@ -623,20 +638,43 @@ void build_event_work(dxg2_plan* p) {
 
 int ensure_event_buffers(dxg2_plan* p, int64_t nnz_coo, hipStream_t s) {
     if (!p->d_err) {
-        DXG2_HIP(dev_alloc(p->alloc, (void**)&p->d_err, sizeof(int), s));
-        if (!p->event_work.empty()) {
-            DXG2_HIP(dev_alloc(p->alloc, (void**)&p->d_ework, p->event_work.size() * sizeof(EventWork), s));
-            DXG2_HIP(hipMemcpyAsync(p->d_ework, p->event_work.data(),
-                                    p->event_work.size() * sizeof(EventWork),
-                                    hipMemcpyHostToDevice, s));
+        // d_err is set last: it marks the pair below as complete
+        int* err = nullptr;
+        EventWork* ework = nullptr;
+        const size_t wbytes = p->event_work.size() * sizeof(EventWork);
+        if (dev_alloc(p->alloc, (void**)&err, sizeof(int), s) != hipSuccess ||
+            (wbytes && dev_alloc(p->alloc, (void**)&ework, wbytes, s) != hipSuccess)) {
+            dev_free(p->alloc, err, s);
+            dev_free(p->alloc, ework, s);
+            return set_error(DXG2_ENOMEM, "cannot allocate the event work list");
         }
+        if (wbytes) {
+            const hipError_t e = hipMemcpyAsync(ework, p->event_work.data(), wbytes,
+                                                hipMemcpyHostToDevice, s);
+            if (e != hipSuccess) {
+                dev_free(p->alloc, err, s);
+                dev_free(p->alloc, ework, s);
+                return set_error(DXG2_EHIP, "event work upload failed: %s", hipGetErrorString(e));
+            }
+        }
+        p->d_ework = ework;
+        p->d_err = err;
     }
     if (nnz_coo >= 0) {
         const int64_t npix = (int64_t)p->H * p->W;
         if (!p->d_cnt) {
-            DXG2_HIP(dev_alloc(p->alloc, (void**)&p->d_cnt, npix * 4, s));
-            DXG2_HIP(dev_alloc(p->alloc, (void**)&p->d_off, (npix + 1) * 4, s));
-            DXG2_HIP(dev_alloc(p->alloc, (void**)&p->d_scan, ((npix + 1023) / 1024 + 1) * 4, s));
+            // all three or none: a partial allocation is released so a later call retries
+            uint32_t *cnt = nullptr, *off = nullptr, *scan = nullptr;
+            if (dev_alloc(p->alloc, (void**)&cnt, npix * 4, s) != hipSuccess ||
+                dev_alloc(p->alloc, (void**)&off, (npix + 1) * 4, s) != hipSuccess ||
+                dev_alloc(p->alloc, (void**)&scan, ((npix + 1023) / 1024 + 1) * 4, s) !=
+                    hipSuccess) {
+                for (void* b : {(void*)cnt, (void*)off, (void*)scan}) dev_free(p->alloc, b, s);
+                return set_error(DXG2_ENOMEM, "cannot allocate the COO conversion workspace");
+            }
+            p->d_cnt = cnt;
+            p->d_off = off;
+            p->d_scan = scan;
         }
         if (nnz_coo > p->ev_cap) {
             CooSortBufs& cs = p->coo_sort;
@@ -1140,8 +1178,38 @@ int dxg2_plan_set_tuning(dxg2_plan* p, const dxg2_tuning* t) {
         return set_error(DXG2_EINVAL, "bad tuning (tile %d, band %d, group %d, queue %d)",
                          t->tile, t->band, t->group, t->queue);
     DXG2_HIP(hipDeviceSynchronize());  // calls on the plan may still read its work lists
-    drop_graph(p);
-    drop_std(p, nullptr);
+    // The new work lists are built on a copy of the plan's tuning state and uploaded into
+    // new buffers; the plan switches to them only when every upload succeeded, so a
+    // failure leaves it whole, with its previous tuning.
+    struct TuningState {
+        int tile, band, group;
+        bool queue;
+        std::vector<GramWork> mwork, xwork, raster, xraster, qm, qx;
+        int q_off[18];
+        int gram_group;
+    };
+    auto save = [&]() {
+        TuningState st{p->gram_tile, p->tune_band, p->tune_group, p->use_queue,
+                       p->gram_work, p->gram_xwork, p->gram_raster, p->gram_xraster,
+                       p->qm_work, p->qx_work, {}, p->gram_group};
+        memcpy(st.q_off, p->q_off, sizeof(p->q_off));
+        return st;
+    };
+    auto restore = [&](TuningState& st) {
+        p->gram_tile = st.tile;
+        p->tune_band = st.band;
+        p->tune_group = st.group;
+        p->use_queue = st.queue;
+        p->gram_work.swap(st.mwork);
+        p->gram_xwork.swap(st.xwork);
+        p->gram_raster.swap(st.raster);
+        p->gram_xraster.swap(st.xraster);
+        p->qm_work.swap(st.qm);
+        p->qx_work.swap(st.qx);
+        memcpy(p->q_off, st.q_off, sizeof(p->q_off));
+        p->gram_group = st.gram_group;
+    };
+    TuningState old = save();
     p->gram_tile = t->tile;
     p->tune_band = t->band;
     p->tune_group = t->group;
@@ -1149,23 +1217,43 @@ int dxg2_plan_set_tuning(dxg2_plan* p, const dxg2_tuning* t) {
     p->gram_work.clear();
     p->gram_xwork.clear();
     build_gram_work(p);
-    struct {
-        GramWork** dst;
-        const std::vector<GramWork>* src;
-    } lists[] = {{&p->d_mwork, &p->gram_work},
-                 {&p->d_xwork, &p->gram_xwork},
-                 {&p->d_qmwork, &p->qm_work},
-                 {&p->d_qxwork, &p->qx_work}};
-    for (auto& l : lists) {
-        dev_free(p->alloc, *l.dst, nullptr);
-        *l.dst = nullptr;
-        if (l.src->empty()) continue;
-        const size_t bytes = l.src->size() * sizeof(GramWork);
-        if (dev_alloc(p->alloc, (void**)l.dst, bytes, nullptr) != hipSuccess)
-            return set_error(DXG2_ENOMEM, "cannot allocate %zu bytes of work list", bytes);
-        DXG2_HIP(hipMemcpy(*l.dst, l.src->data(), bytes, hipMemcpyHostToDevice));
+    const std::vector<GramWork>* src[4] = {&p->gram_work, &p->gram_xwork, &p->qm_work,
+                                           &p->qx_work};
+    GramWork** dst[4] = {&p->d_mwork, &p->d_xwork, &p->d_qmwork, &p->d_qxwork};
+    GramWork* fresh[4] = {nullptr, nullptr, nullptr, nullptr};
+    int rc = DXG2_OK;
+    for (int i = 0; i < 4 && !rc; ++i) {
+        if (src[i]->empty()) continue;
+        const size_t bytes = src[i]->size() * sizeof(GramWork);
+        if (dev_alloc(p->alloc, (void**)&fresh[i], bytes, nullptr) != hipSuccess) {
+            fresh[i] = nullptr;
+            rc = set_error(DXG2_ENOMEM, "cannot allocate %zu bytes of work list; the plan "
+                           "keeps its previous tuning", bytes);
+        } else if (hipMemcpy(fresh[i], src[i]->data(), bytes, hipMemcpyHostToDevice) !=
+                   hipSuccess) {
+            rc = set_error(DXG2_EHIP, "work list upload failed; the plan keeps its previous "
+                           "tuning");
+        }
     }
-    DXG2_HIP(hipMemcpy(p->d_qoff, p->q_off, sizeof(p->q_off), hipMemcpyHostToDevice));
+    // the queue offsets live in a fixed 18-entry buffer: written only on success below
+    if (rc) {
+        for (GramWork* f : fresh) dev_free(p->alloc, f, nullptr);
+        restore(old);
+        return rc;
+    }
+    const hipError_t e = hipMemcpy(p->d_qoff, p->q_off, sizeof(p->q_off), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        for (GramWork* f : fresh) dev_free(p->alloc, f, nullptr);
+        restore(old);
+        (void)hipMemcpy(p->d_qoff, p->q_off, sizeof(p->q_off), hipMemcpyHostToDevice);
+        return set_error(DXG2_EHIP, "queue offsets upload failed: %s", hipGetErrorString(e));
+    }
+    drop_graph(p);
+    drop_std(p, nullptr);
+    for (int i = 0; i < 4; ++i) {
+        dev_free(p->alloc, *dst[i], nullptr);
+        *dst[i] = fresh[i];
+    }
     return DXG2_OK;
 }
 
@@ -1637,6 +1725,24 @@ int dxg2_events_window(const uint32_t* d_offsets, const int32_t* d_times, const 
     (void)hipGetLastError();
     launch_events_window(dtype, d_offsets, d_times, d_data, npix, t0, t1, d_scratch,
                          d_out_offsets, d_out_times, d_out_data, (hipStream_t)stream);
+    DXG2_HIP(hipGetLastError());
+    return DXG2_OK;
+}
+
+int dxg2_events_to_columns(const uint32_t* d_offsets, const int32_t* d_times,
+                           const void* d_data, int64_t npix, int32_t nframes, int32_t dtype,
+                           const int32_t* d_cols, int64_t n_cols, void* d_out, int32_t* d_err,
+                           void* stream) {
+    if (!d_offsets || !d_times || !d_data || !d_out || (n_cols > 0 && !d_cols))
+        return set_error(DXG2_EINVAL, "null argument");
+    if (npix <= 0 || nframes <= 0 || n_cols < 0)
+        return set_error(DXG2_EINVAL, "bad sizes (npix %lld, nframes %d, columns %lld)",
+                         (long long)npix, nframes, (long long)n_cols);
+    if (dtype != DXG2_U8 && dtype != DXG2_I8 && dtype != DXG2_U16)
+        return set_error(DXG2_EINVAL, "unsupported dtype code %d", dtype);
+    (void)hipGetLastError();
+    launch_events_to_columns(dtype, d_offsets, d_times, d_data, npix, d_cols, n_cols, nframes,
+                             d_out, d_err, (hipStream_t)stream);
     DXG2_HIP(hipGetLastError());
     return DXG2_OK;
 }

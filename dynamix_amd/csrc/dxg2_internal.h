@@ -204,6 +204,9 @@ void launch_events_window(int dtype, const uint32_t* offsets, const int32_t* tim
                           const void* data, int64_t npix, int t0, int t1, uint32_t* scratch,
                           uint32_t* out_offsets, int32_t* out_times, void* out_data,
                           hipStream_t s);
+void launch_events_to_columns(int dtype, const uint32_t* offsets, const int32_t* times,
+                              const void* data, int64_t npix, const int32_t* cols,
+                              int64_t n_cols, int nframes, void* out, int* err, hipStream_t s);
 void launch_copy_u32(const uint32_t* src, uint32_t* dst, int64_t n, hipStream_t s);
 void launch_fill_u32(uint32_t* p, int64_t n, hipStream_t s);
 void launch_synth_events(int pass, int dtype, int nframes, int64_t hw, const float* mean_map,

@@ -475,6 +475,65 @@ void launch_events_window(int dtype, const uint32_t* offsets, const int32_t* tim
             (uint8_t*)out_data);
 }
 
+// Events of listed pixels -> dense columns: out[t * n_cols + k] = the count of pixel
+// cols[k] at frame t (zero where it has no event).  One thread per listed pixel walks its
+// events; a frame outside [0, nframes) sets bit 0 of *err and is skipped, a listed pixel
+// outside [0, npix) sets bit 1.  The caller's buffer is cleared first by fill_bytes_kernel.
+__global__ void fill_bytes_kernel(uint32_t* __restrict__ p, int64_t words) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < words;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0u;
+}
+
+__global__ void fill_tail_kernel(uint8_t* __restrict__ p, int64_t from, int64_t to) {
+    const int64_t i = from + threadIdx.x;
+    if (i < to) p[i] = 0;
+}
+
+template <typename T>
+__global__ void events_to_columns_kernel(const uint32_t* __restrict__ offsets,
+                                         const int32_t* __restrict__ times,
+                                         const T* __restrict__ data, int64_t npix,
+                                         const int32_t* __restrict__ cols, int64_t n_cols,
+                                         int nframes, T* __restrict__ out, int* __restrict__ err) {
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n_cols;
+         k += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t p = cols[k];
+        if (p < 0 || p >= npix) {
+            if (err) atomicOr(err, 2);
+            continue;
+        }
+        const uint32_t e1 = offsets[p + 1];
+        for (uint32_t i = offsets[p]; i < e1; ++i) {
+            const int32_t t = times[i];
+            if (t < 0 || t >= nframes) {
+                if (err) atomicOr(err, 1);
+                continue;
+            }
+            out[(int64_t)t * n_cols + k] = data[i];
+        }
+    }
+}
+
+void launch_events_to_columns(int dtype, const uint32_t* offsets, const int32_t* times,
+                              const void* data, int64_t npix, const int32_t* cols,
+                              int64_t n_cols, int nframes, void* out, int* err, hipStream_t s) {
+    const int64_t bytes = n_cols * nframes * (dtype == DXG2_U16 ? 2 : 1);
+    const int64_t words = bytes / 4;
+    if (words > 0) fill_bytes_kernel<<<grid_for(words), 256, 0, s>>>((uint32_t*)out, words);
+    if (bytes > words * 4)
+        fill_tail_kernel<<<1, 64, 0, s>>>((uint8_t*)out, words * 4, bytes);
+    if (n_cols <= 0) return;
+    if (dtype == DXG2_U16)
+        events_to_columns_kernel<uint16_t><<<grid_for(n_cols), 256, 0, s>>>(
+            offsets, times, (const uint16_t*)data, npix, cols, n_cols, nframes, (uint16_t*)out,
+            err);
+    else
+        events_to_columns_kernel<uint8_t><<<grid_for(n_cols), 256, 0, s>>>(
+            offsets, times, (const uint8_t*)data, npix, cols, n_cols, nframes, (uint8_t*)out,
+            err);
+}
+
 void launch_excl_scan(const uint32_t* in, int64_t n, uint32_t* tot, uint32_t* out, hipStream_t s) {
     if (n <= 0) return;
     const int nb = (int)((n + SCAN_B - 1) / SCAN_B);

@@ -283,7 +283,9 @@ __global__ __launch_bounds__(256) void ttcf_tile_kernel(const StdTile* __restric
             const int r = r0 + rr;
             const int t = t0 + r;
             float val = 0.0f;
-            if (t < nframes && u < nframes && u > t) {
+            // the last diagonal entry keeps the pairs within frame N-1: G(t, t) - S(t)
+            const bool corner = t == nframes - 1 && u == t;
+            if (t < nframes && u < nframes && (u > t || corner)) {
                 int64_t S_t, c_t;
                 frame_terms(sums, mode, n_bins, nframes, tl.bin, t, S_t, c_t);
                 if (S_t != 0 && S_u != 0) {
@@ -293,6 +295,7 @@ __global__ __launch_bounds__(256) void ttcf_tile_kernel(const StdTile* __restric
                     for (; k < tl.n_hh; ++k) g += (int64_t)w_hh * e[k * TT];
                     for (; k < tl.n_hh + tl.n_ll; ++k) g += e[k * TT];
                     for (; k < tl.n_hh + tl.n_ll + tl.n_x; ++k) g += (int64_t)256 * e[k * TT];
+                    if (corner) g -= S_t;
                     val = (float)(npix * (double)g / ((double)S_t * (double)S_u));
                 } else {
                     val = __builtin_nanf("");
@@ -311,8 +314,8 @@ __global__ __launch_bounds__(256) void ttcf_tile_kernel(const StdTile* __restric
 }
 
 // The diagonal of the two-time matrix: entries 0 .. N-2 take the mean of the positive
-// entries of the first off-diagonal (event_y.py:153-156); entry N-1 keeps 2 * 0 /
-// S(N-1)^2, i.e. 0, or NaN for an empty last frame.  One workgroup.
+// entries of the first off-diagonal (event_y.py:224-227); entry (N-1, N-1), the pairs
+// within the last frame, is written by ttcf_tile_kernel.  One workgroup.
 __global__ __launch_bounds__(1024) void ttcf_diag_kernel(float* __restrict__ trc, int nframes,
                                                           const int64_t* __restrict__ sums,
                                                           int mode, int n_bins, int b) {
@@ -341,11 +344,6 @@ __global__ __launch_bounds__(1024) void ttcf_diag_kernel(float* __restrict__ trc
     }
     const float fill = scnt[0] > 0 ? (float)(ssum[0] / scnt[0]) : __builtin_nanf("");
     for (int j = tid; j < nframes - 1; j += 1024) trc[(int64_t)j * N + j] = fill;
-    if (tid == 0) {
-        int64_t S, cb;
-        frame_terms(sums, mode, n_bins, nframes, b, nframes - 1, S, cb);
-        trc[(N - 1) * N + N - 1] = S != 0 ? 0.0f : __builtin_nanf("");
-    }
 }
 
 __global__ void fill_f32_kernel(float* __restrict__ p, int64_t n, float v) {

@@ -126,13 +126,13 @@ int dxg2_correlate_dense_std(dxg2_plan* plan, const void* d_frames, int64_t* d_n
                              void* stream);
 
 /* Two-time correlation matrix of bin q (1-based), d_trc: (N, N) float32 row-major,
- * trc(t, u) = n_b G(t, u) / (S(t) S(u)) for t != u, where G counts only pairs of samples
- * from two different frames; the diagonal 0 .. N-2 holds the mean of the positive
- * entries of the first off-diagonal and entry (N-1, N-1) is 0 (NaN where a frame sum is
- * zero: 0/0, as in numpy).  Computed in FP64 from the exact integer Gram tiles of that
- * bin (the tiles are stored in a scratch of at most DXG2_STD_SCRATCH_MB MiB).
- * Replaces the ttcf_par output `trc` of the event correlators (correlator/event_y.py:
- * 137-157, nbecorrts_q). */
+ * trc(t, u) = n_b G(t, u) / (S(t) S(u)) for t != u, with n_b every pixel of the bin; the
+ * diagonal 0 .. N-2 holds the mean of the positive entries of the first off-diagonal and
+ * entry (N-1, N-1) is n_b (G(N-1, N-1) - S(N-1)) / S(N-1)^2, the photon pairs within the
+ * last frame (NaN wherever a frame sum is zero).  Computed in FP64 from the exact integer
+ * Gram tiles of that bin (the tiles are stored in a scratch of at most
+ * DXG2_STD_SCRATCH_MB MiB).  Replaces the ttcf_par output `trc` of ecorrts_q
+ * (correlator/event_y.py:169-230 with WXPCS/fecorrt.f:1-26). */
 int dxg2_correlate_dense_ttcf(dxg2_plan* plan, const void* d_frames, int32_t q, float* d_trc,
                               void* stream);
 
@@ -262,6 +262,20 @@ int dxg2_events_window(const uint32_t* d_offsets, const int32_t* d_times, const 
                        int64_t nnz, int64_t npix, int32_t dtype, int32_t t0, int32_t t1,
                        uint32_t* d_scratch, uint32_t* d_out_offsets, int32_t* d_out_times,
                        void* d_out_data, int64_t capacity, void* stream);
+
+/* Events of a list of pixels as dense columns: d_out (nframes, n_cols) of the count dtype,
+ * d_out[t * n_cols + k] = the count of pixel d_cols[k] at frame t, 0 where it has no event
+ * (the buffer is cleared first).  Input: the pixel-major layout of
+ * dxg2_correlate_events_csr (npix = H*W pixels).  *d_err (int32, may be NULL, zeroed by
+ * the caller) gets bit 0 for an event frame outside [0, nframes) (skipped) and bit 1 for
+ * a listed pixel outside [0, npix).  Feeds the two-time output of event input: the
+ * reference builds its two-time matrix from event lists (ecorrts_q / nbecorrts_q,
+ * correlator/event_y.py:104-230); here a bin's events become a dense (nframes, n_b)
+ * stack for the exact Gram path (dxg2_correlate_dense_ttcf). */
+int dxg2_events_to_columns(const uint32_t* d_offsets, const int32_t* d_times,
+                           const void* d_data, int64_t npix, int32_t nframes, int32_t dtype,
+                           const int32_t* d_cols, int64_t n_cols, void* d_out, int32_t* d_err,
+                           void* stream);
 
 /* Dense frames -> per-pixel event lists on the GPU (FramesCompressor, event.py:207-321):
  * for every pixel p in raster order, its samples with value > 0 (signed for int8) in

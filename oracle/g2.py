@@ -17,6 +17,8 @@ Parity pinning of each function (tests/test_oracle.py):
   frame_sums, numerators      -- brute-force triple loops on tiny inputs; invariants
                                  Num(0) = sum I^2, permutation invariance, constant
                                  frames; equality with the event-list restatement.
+  sums_numerators_fast        -- equality with frame_sums / numerators and the brute
+                                 force on the same stacks.
   denominators                -- brute force with Python integers; np.correlate.
   g2_fp64                     -- golden vectors of the reference py_dense_correlator
                                  (within its float32 rounding), single-pixel bin == 1,
@@ -26,10 +28,11 @@ Parity pinning of each function (tests/test_oracle.py):
                                  res and (calc_std) dev.
   dense_dev_fp64              -- the golden dev vectors within the reference's float32
                                  rounding; brute-force loops on a tiny stack.
-  two_time_fp64               -- a loop restatement of the reference's event pair loop
-                                 and TTCF normalisation (event_y.py:20-41,137-157) on
-                                 tiny stacks; closed form for time-constant frames;
-                                 symmetry; diagonal rule.
+  two_time_fp64               -- a loop restatement of the reference's photon-pair
+                                 count and TTCF normalisation (ecorrts_q,
+                                 event_y.py:169-230; fecorrt.f:1-26) on tiny stacks;
+                                 closed form for time-constant frames; symmetry;
+                                 diagonal rule; the last diagonal corner.
   fft_correlator              -- against `numerators`/`denominators` within FP32
                                  tolerance; FFT length against the reference's
                                  get_next_power golden vectors (utils.py:50-60).
@@ -134,6 +137,38 @@ def numerators(frames, qmask, bins):
     """Num_b(tau), exact int64, shape (n_bins, N)."""
     return np.stack([bin_numerator(gather_bin(frames, qmask, b)) for b in bins]) \
         if len(bins) else np.zeros((0, frames.shape[0]), np.int64)
+
+
+def sums_numerators_fast(frames, qmask, bins):
+    """(S_b(t), Num_b(tau)) as frame_sums / numerators, for detector-sized stacks.
+
+    The stack is transposed once to pixel-major rows, each bin's rows are taken by
+    index, and the bin's Gram A_b A_b^T is formed by one float64 BLAS product when every
+    partial sum is an integer below 2^53 (exact), otherwise in int64.  Same quantities,
+    one pass over the stack instead of one per bin.
+    """
+    n = frames.shape[0]
+    q = np.asarray(qmask).ravel()
+    xt = np.ascontiguousarray(frames.reshape(n, -1).T)      # (H*W, N)
+    order = np.argsort(q, kind="stable")
+    qs = q[order]
+    s = np.zeros((len(bins), n), dtype=np.int64)
+    num = np.zeros((len(bins), n), dtype=np.int64)
+    for i, b in enumerate(bins):
+        lo, hi = np.searchsorted(qs, b, "left"), np.searchsorted(qs, b, "right")
+        if hi == lo:
+            continue
+        at = xt[order[lo:hi]]                                  # (n_b, N)
+        s[i] = at.astype(np.int64).sum(axis=0)
+        amax = int(np.abs(at).max())
+        if amax * amax * (hi - lo) < 2 ** 53:
+            af = at.astype(np.float64)
+            g = (af.T @ af).astype(np.int64)
+        else:
+            ai = at.astype(np.int64)
+            g = ai.T @ ai
+        num[i] = _diag_sums(g)
+    return s, num
 
 
 def denominators(sums):
@@ -275,16 +310,20 @@ def dense_dev_fp64(frames, qmask=None):
 
 # ------------------------------------------------------------------------ FFT path
 def two_time_fp64(frames, qmask, q):
-    """Two-time correlation matrix of bin q (1-based), as the reference builds `trc` for
-    ttcf_par = q (event_y.py:137-157), in FP64 from exact integers:
+    """Two-time correlation matrix of bin q (1-based) as the reference's ecorrts_q builds
+    `trc` for ttcf_par = q (event_y.py:169-230 with WXPCS/fecorrt.f:1-26), in FP64 from
+    exact integers:
 
-      cc(t, u)  = sum_p I(t, p) I(u, p) for u > t (pairs of events within a pixel, the
-                  event loop of event_y.py:20-41; nothing on or below the diagonal);
-      trc(t, u) = n_b cc(t, u) / (S(t) S(u))       (cor / norm, event_y.py:139-140,149)
-    then trc += trc^T (event_y.py:150-152: the lower triangle is empty), and the
-    diagonal entries 0 .. N-2 become the mean of the positive entries of the first
-    off-diagonal (event_y.py:153-156); trc[N-1, N-1] keeps 2 * 0 / S(N-1)^2.  A zero
-    S(t) S(u) gives NaN (0/0), as numpy's float division does.
+      cor(t, u) = sum_p I(t, p) I(u, p) for t != u: fecorrt counts every pair of photons
+                  of one pixel in two frames and symmetrises (fecorrt.f:8-25);
+      cor(t, t) = sum_p I(t, p) (I(t, p) - 1): the pairs of photons within one frame,
+                  counted once each way by the symmetrisation;
+      trc(t, u) = n_b cor(t, u) / (S(t) S(u)), n_b every pixel of the bin
+                  (for_norm, event_y.py:189,208-209,223);
+    then the diagonal entries 0 .. N-2 become the mean of the positive entries of the
+    first off-diagonal (event_y.py:224-227); trc[N-1, N-1] keeps its value.  Where
+    S(t) S(u) = 0 the entry is NaN (numpy gives 0/0 = NaN for photon counts, whose
+    frame sum is 0 only with every count 0).
     """
     qq, bins = qmask_bins(frames.shape[1:], qmask)
     a = gather_bin(frames, qq, q).astype(np.float64)
@@ -292,9 +331,11 @@ def two_time_fp64(frames, qmask, q):
     npx = a.shape[1]
     g = a @ a.T                      # exact: integer entries well below 2^53 in tests
     s = a.sum(axis=1)
+    cor = g.copy()
+    cor[np.diag_indices(n)] -= s
+    ss = np.outer(s, s)
     with np.errstate(divide="ignore", invalid="ignore"):
-        trc = npx * np.triu(g, 1) / np.outer(s, s)
-        trc = trc + trc.T
+        trc = np.where(ss != 0, npx * cor / np.where(ss != 0, ss, 1.0), np.nan)
     if n > 1:
         d1 = np.diagonal(trc, 1)
         pos = d1[d1 > 0]
@@ -305,31 +346,40 @@ def two_time_fp64(frames, qmask, q):
 
 
 def two_time_loops(frames, qmask, q):
-    """The same matrix by the reference's own loop structure (event_y.py:20-41,137-157):
-    per pixel, every pair of nonzero samples (t < u) adds I(t) I(u); then the float
-    normalisation, symmetrisation and diagonal rule element by element.  Small inputs."""
+    """The same matrix by the reference's own steps, on small non-negative stacks:
+    every photon becomes one (pixel, frame) entry, sorted by pixel then frame (the
+    lexsort of event_y.py:176-178); fecorrt's loop counts each pair of entries of one
+    pixel into cor(t_later, t_earlier) and then symmetrises cor(j, i) = cor(i, j) =
+    cor(j, i) + cor(i, j) for j >= i (fecorrt.f:8-25); S(t) is the photon histogram
+    over frames (event_y.py:195); trc = n_b cor / (S S^T) element by element; the
+    diagonal rule last (event_y.py:224-227)."""
     qq, _ = qmask_bins(frames.shape[1:], qmask)
     n = frames.shape[0]
+    flat = frames.reshape(n, -1)
     pix = [p for p in range(qq.size) if qq.flat[p] == q]
-    cc = [[0] * n for _ in range(n)]
-    mint = [0] * n
+    entries = []                      # (pixel, frame) per photon
     for p in pix:
-        ev = [(t, int(frames.reshape(n, -1)[t, p])) for t in range(n)
-              if frames.reshape(n, -1)[t, p] != 0]
-        for j, (t0, v0) in enumerate(ev):
-            mint[t0] += v0
-            for t1, v1 in ev[j + 1:]:
-                cc[t0][t1] += v0 * v1
+        for t in range(n):
+            entries.extend([(p, t)] * int(flat[t, p]))
+    entries.sort()
+    cor = [[0] * n for _ in range(n)]
+    for i in range(len(entries)):
+        j = i + 1
+        while j < len(entries) and entries[j][0] == entries[i][0]:
+            cor[entries[j][1]][entries[i][1]] += 1
+            j += 1
+    for i in range(n):
+        for j in range(i, n):
+            cor[j][i] = cor[j][i] + cor[i][j]
+            cor[i][j] = cor[j][i]
+    hist = [0] * n
+    for _, t in entries:
+        hist[t] += 1
     trc = [[0.0] * n for _ in range(n)]
     for t in range(n):
         for u in range(n):
-            den = mint[t] * mint[u]
-            cor = len(pix) * cc[t][u] if u > t else 0
-            trc[t][u] = cor / den if den else float("nan")
-    for t in range(n):
-        for u in range(t + 1):
-            trc[t][u] = trc[t][u] + trc[u][t] if u < t else 2 * trc[t][t]
-            trc[u][t] = trc[t][u]
+            den = hist[t] * hist[u]
+            trc[t][u] = len(pix) * cor[t][u] / den if den else float("nan")
     pos = [trc[j][j + 1] for j in range(n - 1) if trc[j][j + 1] > 0]
     fill = sum(pos) / len(pos) if pos else float("nan")
     for j in range(n - 1):

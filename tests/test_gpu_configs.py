@@ -1,0 +1,399 @@
+"""GPU parity at the benchmark workloads' real geometry (marked gpu).
+
+Each test is named after the workload it runs:
+  cfg1  256 x 256 x 1000 uint8, 10 annuli: in full against the oracle, plus the
+        reference's own float32 MatMul algorithm within its 5e-3 test tolerance;
+  cfg2  514 x 1030 uint8, 20 annuli with six 8-row gap bands, mean 0.02, tau_c 200:
+        at N = 1024 against the oracle (exact), and at the full N = 10,000 through
+        size-independent properties computed independently with torch on the same
+        frames (per-bin Num(tau) at sampled lags including Num(0) = sum I^2, S_b(t) as
+        a segmented sum, the exact denominator of those sums, g2 from them), the plain
+        FP32 comparator and the FFT path within their tolerances;
+  cfg3  2162 x 2068 uint16 with 1 % hot pixels at mean 40, 50 linear-width annuli (14
+        bins over 131,008 pixels, so their Gram items split in k) at N = 300, autotuned,
+        against the oracle (exact);
+  cfg4  1030 x 1065, mean 1e-3 events, 30 annuli: frame-sorted COO and pixel-major CSR
+        at N = 10,000 against the event oracle (exact); at the full N = 100,000 the COO
+        and CSR results are identical and S_b(t), Num_b(0..3) and sampled Den_b(tau)
+        equal independent torch / numpy sums;
+  cfg5  the serving loop at the real frame shapes {256^2, 514 x 1030, 1030 x 1065} with
+        1k-2k frames and 10-100 bins: every job in the mix equals the job run alone, and
+        one dense and one event job equal the oracle.
+"""
+import ctypes
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import events as ev
+from oracle import g2 as og
+
+pytestmark = pytest.mark.gpu
+
+G2_RTOL = 1e-6        # g2 vs FP64: float32 rounding of one FP64 ratio of exact integers
+PLAIN_RTOL = 1e-4     # the plain FP32 comparator (FP32 sums of products and means)
+FFT_NUM_TOL = 2e-7    # |Num_fft - Num| <= FFT_NUM_TOL * Num(0) per bin (FP32 spectra)
+REF_F32_ATOL = 5e-3   # the reference's own tolerance for its float32 correlators
+                      # (dynamix/correlator/test/test_dense.py:86)
+
+
+def _announce(msg):
+    print("[case] " + msg, flush=True)
+    sys.stdout.flush()
+
+
+def assert_g2_close(g2, g, rtol):
+    """Same NaN pattern; exact zeros where the FP64 value is 0; relative error <= rtol."""
+    g2 = np.asarray(g2)
+    fin = np.isfinite(g)
+    assert np.array_equal(fin, np.isfinite(g2)), "NaN pattern differs"
+    zero = fin & (g == 0)
+    assert (g2[zero] == 0).all()
+    nz = fin & (g != 0)
+    rel = np.abs(g2[nz].astype(np.float64) - g[nz]) / np.abs(g[nz])
+    assert rel.max(initial=0.0) <= rtol, rel.max()
+
+
+def _check_exact(c, g2, sums, num, scale):
+    np.testing.assert_array_equal(c.last_sums.cpu().numpy(), sums)
+    np.testing.assert_array_equal(c.last_numerators.cpu().numpy(), num)
+    den = og.denominators(sums)
+    assert den.dtype != object
+    np.testing.assert_array_equal(c.last_denominators.cpu().numpy(), den.astype(np.float64))
+    ref = og.g2_fp64(num, den, scale)
+    assert_g2_close(g2, ref, G2_RTOL)
+    return ref
+
+
+def _frames(shape, n, name, seed):
+    from dynamix_amd import synth
+    w = synth.WORKLOADS[name]
+    mm = synth.mean_map(shape, w["mean"], w.get("hot_fraction", 0.0), w.get("hot_mean", 0.0),
+                        seed)
+    return synth.speckle_frames_device(shape, n, w["mean"], w["tau_c"], seed,
+                                       np.dtype(w["dtype"]), mean_map_=mm)
+
+
+# ------------------------------------------------------------------------- cfg1
+def test_cfg1_small_256x256x1000_u8_10bins_full():
+    """The CPU-reference workload in full: exact vs the oracle; the reference's float32
+    MatMul algorithm agrees within its own test tolerance; FFT and calc_std too."""
+    from dynamix_amd import synth
+    from dynamix_amd.correlator.dense import FFTCorrelator, MatMulCorrelator
+    _announce("cfg1 256x256x1000 u8 10 bins")
+    w = synth.WORKLOADS["small"]
+    frames = synth.workload_frames_device("small", seed=101)
+    q = synth.workload_qmask("small")
+    n = w["nframes"]
+    c = MatMulCorrelator(w["shape"], n, q)
+    res = c.correlate(frames, calc_std=True)
+    f = frames.cpu().numpy()
+    ref = og.dense_g2(f, q)
+    _check_exact(c, res.res, ref["sums"], ref["num"], ref["scale"])
+    # dev(tau) against the FP64 two-pass std of the exact ratios
+    dev = og.dense_dev_fp64(f, q)
+    fin = np.isfinite(dev)
+    assert np.array_equal(fin, np.isfinite(res.dev))
+    assert (np.abs(res.dev[fin] - dev[fin]) <= 1e-6 * np.abs(dev[fin]) + 1e-12).all()
+    # the reference's own float32 algorithm (dense.py:32-70) on the same stack
+    f32 = og.matmul_correlator_f32(f, q)
+    fin = np.isfinite(f32)
+    assert np.array_equal(fin, np.isfinite(res.res))
+    assert np.abs(res.res[fin] - f32[fin]).max() <= REF_F32_ATOL
+    # FFT path on the same stack
+    cf = FFTCorrelator(w["shape"], n, q)
+    cf.correlate(frames)
+    num_fft = cf.last_numerators.cpu().numpy()
+    assert (np.abs(num_fft - ref["num"]) <= FFT_NUM_TOL * ref["num"][:, :1]).all()
+
+
+# ------------------------------------------------------------------------- cfg2
+def test_cfg2_headline_geometry_n1024_vs_oracle():
+    """514 x 1030 detector, the headline's 20 annuli and gap bands, N = 1024: the band
+    gather over W = 1030 rows, exact against the event-pair oracle on the same frames;
+    the OpenCL-class drop-in gives the identical result."""
+    from dynamix_amd import synth
+    from dynamix_amd.correlator.dense import DenseCorrelator, MatMulCorrelator
+    _announce("cfg2 514x1030x1024 u8 20 bins")
+    shape, n = (514, 1030), 1024
+    q = synth.workload_qmask("headline")
+    frames = _frames(shape, n, "headline", seed=202)
+    c = MatMulCorrelator(shape, n, q)
+    g2 = c.correlate(frames)
+    f = frames.cpu().numpy()
+    d, t, o = ev.compress_all_stack(f)
+    _, bins = og.qmask_bins(shape, q)
+    s, num = ev.event_sums_numerators_fast(t, d, o, q, n, bins)
+    assert (s.sum(axis=1) > 0).all()
+    _check_exact(c, g2, s, num, og.scale_factors(q, bins))
+    cd = DenseCorrelator(shape, n, q, dtype=np.uint8)
+    g2d = cd.correlate(frames)
+    np.testing.assert_array_equal(g2d.view(np.uint32), g2.view(np.uint32))
+
+
+def _bin_index(q, torch):
+    """Binned pixels sorted by bin: (pixel indices, their bin numbers) on the GPU."""
+    qf = torch.from_numpy(np.ascontiguousarray(q.ravel())).to("cuda").long()
+    order = torch.argsort(qf, stable=True)
+    qs = qf[order]
+    keep = qs > 0
+    return order[keep], qs[keep]
+
+
+def _torch_num_tau(a, bidx, nb, tau, torch, chunk=256):
+    """Num_b(tau) = sum_t sum_{p in b} a[t, p] a[t + tau, p], int64, for gathered
+    frames a (N, P) and the bin number of each column."""
+    n = a.shape[0]
+    per_pix = torch.zeros(a.shape[1], dtype=torch.int64, device="cuda")
+    for t0 in range(0, n - tau, chunk):
+        t1 = min(n - tau, t0 + chunk)
+        per_pix += (a[t0:t1].to(torch.int32) * a[t0 + tau:t1 + tau].to(torch.int32)).sum(
+            dim=0, dtype=torch.int64)
+    out = torch.zeros(nb + 1, dtype=torch.int64, device="cuda")
+    out.index_add_(0, bidx, per_pix)
+    return out[1:]
+
+
+def _torch_sums(a, bidx, nb, torch, chunk=256):
+    n = a.shape[0]
+    out = torch.zeros((n, nb + 1), dtype=torch.int64, device="cuda")
+    for t0 in range(0, n, chunk):
+        t1 = min(n, t0 + chunk)
+        out[t0:t1].index_add_(1, bidx, a[t0:t1].to(torch.int64))
+    return out[:, 1:].T.contiguous()
+
+
+def test_cfg2_headline_full_n10000_properties():
+    """The headline stack itself (514 x 1030 x 10,000 uint8, 20 bins): numerators at
+    sampled lags (Num(0) = sum I^2 among them) and S_b(t) from independent torch
+    reductions, the exact denominator of those sums, g2 from them; the plain FP32
+    comparator and the FFT path on the same frames within their tolerances."""
+    import torch
+    from dynamix_amd import synth
+    from dynamix_amd.correlator.dense import (FFTCorrelator, MatMulCorrelator,
+                                              PlainMatMulCorrelator)
+    _announce("cfg2 full 514x1030x10000 u8 20 bins")
+    w = synth.WORKLOADS["headline"]
+    shape, n = w["shape"], w["nframes"]
+    q = synth.workload_qmask("headline")
+    frames = synth.workload_frames_device("headline", seed=203)
+    c = MatMulCorrelator(shape, n, q)
+    g2 = c.correlate(frames)
+    num = c.last_numerators.cpu().numpy()
+    sums = c.last_sums.cpu().numpy()
+    nb = q.max()
+    idx, bidx = _bin_index(q, torch)
+    a = frames.view(n, -1).index_select(1, idx)           # (N, 479,980) uint8
+    np.testing.assert_array_equal(_torch_sums(a, bidx, nb, torch).cpu().numpy(), sums)
+    for tau in (0, 1, 2, 37, 200, 1023, 4096, 9998, 9999):
+        ref = _torch_num_tau(a, bidx, nb, tau, torch).cpu().numpy()
+        np.testing.assert_array_equal(num[:, tau], ref, err_msg="tau=%d" % tau)
+    del a
+    _, bins = og.qmask_bins(shape, q)
+    g = _check_exact(c, g2, sums, num, og.scale_factors(q, bins))
+    # plain FP32 one-bin-per-launch comparator
+    cp = PlainMatMulCorrelator(shape, n, q)
+    gp = cp.correlate(frames)
+    fin = np.isfinite(g) & (g != 0)
+    assert np.array_equal(np.isfinite(g), np.isfinite(gp))
+    assert (np.abs(gp[fin] - g[fin]) <= PLAIN_RTOL * np.abs(g[fin])).all()
+    del cp
+    # FFT path
+    cf = FFTCorrelator(shape, n, q)
+    cf.correlate(frames)
+    num_fft = cf.last_numerators.cpu().numpy()
+    assert (np.abs(num_fft - num) <= FFT_NUM_TOL * num[:, :1]).all()
+
+
+# ------------------------------------------------------------------------- cfg3
+def test_cfg3_uint16_2162x2068_50bins_n300_autotuned_vs_oracle(monkeypatch, tmp_path):
+    """uint16 frames with the hot-pixel tail, 50 linear-width annuli (n_b up to
+    304,661: k-split Gram items) with wide pixels first, autotuned: exact."""
+    from dynamix_amd import synth
+    from dynamix_amd.correlator.dense import MatMulCorrelator
+    _announce("cfg3 2162x2068x300 u16 50 bins autotuned")
+    monkeypatch.setenv("DXG2_TUNING_CACHE", str(tmp_path / "gram_tuning.json"))
+    shape, n = (2162, 2068), 300
+    q = synth.workload_qmask("uint16")
+    counts = np.bincount(q.ravel())[1:]
+    assert (counts > 2047 * 64).sum() >= 10
+    frames = _frames(shape, n, "uint16", seed=303)
+    c = MatMulCorrelator(shape, n, q, extra_options={"autotune": True})
+    g2 = c.correlate(frames)
+    f = frames.cpu().numpy()
+    del frames
+    assert f.max() > 255, "no wide (> 255) pixel: the byte split's high plane is idle"
+    _, bins = og.qmask_bins(shape, q)
+    s, num = og.sums_numerators_fast(f, q, bins)
+    _check_exact(c, g2, s, num, og.scale_factors(q, bins))
+
+
+# ------------------------------------------------------------------------- cfg4
+def _events(shape, n, seed):
+    from dynamix_amd import synth
+    w = synth.WORKLOADS["events"]
+    t, d, o = synth.speckle_events_device(shape, n, w["mean"], w["tau_c"], seed, np.uint8)
+    pix, fr, cnt = synth.events_to_coo_device(t, d, o, shape, n)
+    return (t, d, o), (pix, fr, cnt)
+
+
+def test_cfg4_events_1030x1065_n10000_coo_csr_vs_oracle():
+    from dynamix_amd import synth
+    from dynamix_amd.correlator.event import EventCorrelator
+    _announce("cfg4 events 1030x1065x10000 30 bins")
+    shape, n = (1030, 1065), 10000
+    q = synth.workload_qmask("events")
+    (t, d, o), (pix, fr, cnt) = _events(shape, n, seed=404)
+    assert (fr[1:] >= fr[:-1]).all().item(), "COO input is not frame-sorted"
+    c = EventCorrelator(shape, n, max_events_count=1, qmask=q, dtype=np.uint8)
+    g2 = c.correlate_coo(pix, fr, cnt)
+    res_coo = [x.cpu().numpy() for x in (c.last_numerators, c.last_sums, c.last_denominators)]
+    tn, dn, on = t.cpu().numpy(), d.cpu().numpy(), o.cpu().numpy().view(np.uint32)
+    _, bins = og.qmask_bins(shape, q)
+    s, num = ev.event_sums_numerators_fast(tn, dn, on, q, n, bins)
+    assert num[:, 0].min() > 0
+    _check_exact(c, g2, s, num, og.scale_factors(q, bins))
+    g2c = c.correlate(t, d, o)
+    np.testing.assert_array_equal(g2c.view(np.uint32), g2.view(np.uint32))
+    for x, y in zip(res_coo, (c.last_numerators, c.last_sums, c.last_denominators)):
+        np.testing.assert_array_equal(x, y.cpu().numpy())
+
+
+def test_cfg4_events_full_n100000_properties():
+    """The full event workload (1030 x 1065 x 100,000 frames, ~1.1e8 events): COO and
+    CSR inputs give identical results; S_b(t), Num_b(0) = sum d^2 and Num_b(1..3)
+    (pairs of events at most three apart within a pixel) from torch reductions over the
+    event list; Den_b at sampled lags from the sums in exact integers; g2 from them."""
+    import torch
+    from dynamix_amd import synth
+    from dynamix_amd.correlator.event import EventCorrelator
+    _announce("cfg4 full events 1030x1065x100000 30 bins")
+    w = synth.WORKLOADS["events"]
+    shape, n = w["shape"], w["nframes"]
+    q = synth.workload_qmask("events")
+    (t, d, o), (pix, fr, cnt) = _events(shape, n, seed=405)
+    nnz = int(t.shape[0])
+    assert nnz > 5e7
+    c = EventCorrelator(shape, n, max_events_count=1, qmask=q, dtype=np.uint8)
+    g2 = c.correlate_coo(pix, fr, cnt)
+    num, sums, den = [x.cpu().numpy() for x in
+                      (c.last_numerators, c.last_sums, c.last_denominators)]
+    del pix, fr, cnt
+    g2c = c.correlate(t, d, o)
+    np.testing.assert_array_equal(g2c.view(np.uint32), g2.view(np.uint32))
+    np.testing.assert_array_equal(c.last_numerators.cpu().numpy(), num)
+    np.testing.assert_array_equal(c.last_sums.cpu().numpy(), sums)
+    nb = int(q.max())
+    qd = torch.from_numpy(q.ravel()).to("cuda").long()
+    counts = o[1:].long() - o[:-1].long()
+    ep = torch.repeat_interleave(torch.arange(q.size, device="cuda"), counts)
+    eb = qd[ep]                                     # bin of each event (0 = excluded)
+    dv = d.long()
+    tv = t.long()
+    s_ref = torch.zeros((nb + 1) * n, dtype=torch.int64, device="cuda")
+    s_ref.index_add_(0, eb * n + tv, dv)
+    np.testing.assert_array_equal(s_ref.view(nb + 1, n)[1:].cpu().numpy(), sums)
+    for tau in range(4):
+        acc = torch.zeros(nb + 1, dtype=torch.int64, device="cuda")
+        for k in range(0, tau + 1):
+            if k == 0:
+                if tau == 0:
+                    acc.index_add_(0, eb, dv * dv)
+                continue
+            same = (ep[k:] == ep[:-k]) & ((tv[k:] - tv[:-k]) == tau)
+            acc.index_add_(0, eb[:-k][same], (dv[:-k] * dv[k:])[same])
+        np.testing.assert_array_equal(num[:, tau], acc[1:].cpu().numpy(), err_msg="tau=%d" % tau)
+    for tau in (0, 1, 7, 200, 4321, 54321, n - 1):
+        ref = [int(np.dot(sums[i, :n - tau], sums[i, tau:])) for i in range(nb)]
+        np.testing.assert_array_equal(den[:, tau], np.array(ref, dtype=np.float64))
+    scale = og.scale_factors(q, np.arange(1, nb + 1))
+    with np.errstate(divide="ignore", invalid="ignore"):
+        ref_g2 = scale[:, None] * num.astype(np.float64) / den
+    assert_g2_close(g2, ref_g2, G2_RTOL)
+
+
+# ------------------------------------------------------------------------- cfg5
+SERVE_POOL_FRAMES = 2000
+
+
+def _window_events(pool, spec):
+    t, d, o = pool.events[spec.shape]
+    tn, dn = t.cpu().numpy(), d.cpu().numpy()
+    on = o.cpu().numpy().view(np.uint32).astype(np.int64)
+    hw = spec.shape[0] * spec.shape[1]
+    pix = np.repeat(np.arange(hw), np.diff(on))
+    keep = (tn >= spec.frame0) & (tn < spec.frame0 + spec.nframes)
+    off = np.zeros(hw + 1, np.uint32)
+    off[1:] = np.cumsum(np.bincount(pix[keep], minlength=hw))
+    return tn[keep] - spec.frame0, dn[keep], off
+
+
+def _alone(pool, spec):
+    """The job by itself through the single-plan entry points."""
+    import torch
+    from dynamix_amd import _lib
+    from dynamix_amd.correlator.common import Plan
+    from dynamix_amd.serving import job_qmask
+    q = job_qmask(spec)
+    plan = Plan(spec.shape, spec.nframes, q, np.uint8)
+    nb, n = plan.n_bins, spec.nframes
+    num = torch.empty((nb, n), dtype=torch.int64, device="cuda")
+    sums = torch.empty_like(num)
+    den = torch.empty((nb, n), dtype=torch.float64, device="cuda")
+    g2 = torch.empty((nb, n), dtype=torch.float32, device="cuda")
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    outs = [_lib.ptr(x) for x in (num, sums, den, g2)]
+    if spec.layout == "dense":
+        f = pool.dense[spec.shape][spec.frame0:spec.frame0 + n]
+        _lib.check(plan.lib.dxg2_correlate_dense(plan.handle, _lib.ptr(f), *outs, st))
+    else:
+        tw, dw, ow = _window_events(pool, spec)
+        tt = torch.from_numpy(tw.astype(np.int32)).cuda()
+        dd = torch.from_numpy(dw).cuda()
+        oo = torch.from_numpy(ow.view(np.int32)).cuda()
+        _lib.check(plan.lib.dxg2_correlate_events_csr(plan.handle, _lib.ptr(tt), _lib.ptr(dd),
+                                                      _lib.ptr(oo), int(tw.size), *outs, st))
+    torch.cuda.synchronize()
+    res = {"num": num.cpu().numpy(), "sums": sums.cpu().numpy(), "g2": g2.cpu().numpy()}
+    plan.close()
+    return res, q
+
+
+def test_cfg5_serving_real_shapes_mix_equals_alone():
+    from dynamix_amd.serving import SERVE_SHAPES, InputPool, Server, make_jobs
+    _announce("cfg5 serving real shapes")
+    pool = InputPool(SERVE_SHAPES, SERVE_POOL_FRAMES, seed=55, dense_mean=0.02, event_mean=1e-3,
+                     tau_c=200.0)
+    specs = make_jobs(10, rate=200.0, seed=505, nframes=(1000, SERVE_POOL_FRAMES),
+                      bins=(10, 100), dense_fraction=0.7, pool_frames=SERVE_POOL_FRAMES)
+    assert {s.layout for s in specs} == {"dense", "events"}
+    assert {tuple(s.shape) for s in specs} == set(SERVE_SHAPES)
+    srv = Server(pool, max_batch=4, keep_results=True)
+    try:
+        stats = srv.run(specs)
+    finally:
+        srv.close()
+    done = stats["records"]
+    assert stats["jobs"] == len(specs) and not stats.get("failed")
+    checked = set()
+    for j in sorted(done, key=lambda j: j.npix * j.spec.nframes):
+        alone, q = _alone(pool, j.spec)
+        np.testing.assert_array_equal(j.result["num"], alone["num"])
+        np.testing.assert_array_equal(j.result["sums"], alone["sums"])
+        np.testing.assert_array_equal(j.result["g2"].view(np.uint32), alone["g2"].view(np.uint32))
+        if j.spec.layout in checked:
+            continue
+        checked.add(j.spec.layout)      # the smallest job of each layout vs the oracle
+        _, bins = og.qmask_bins(q.shape, q)
+        if j.spec.layout == "dense":
+            f = pool.dense[j.spec.shape][j.spec.frame0:j.spec.frame0 + j.spec.nframes]
+            s, num = og.sums_numerators_fast(f.cpu().numpy(), q, bins)
+        else:
+            tw, dw, ow = _window_events(pool, j.spec)
+            s, num = ev.event_sums_numerators_fast(tw, dw, ow, q, j.spec.nframes, bins)
+        np.testing.assert_array_equal(alone["num"], num)
+        np.testing.assert_array_equal(alone["sums"], s)
+        ref = og.g2_fp64(num, og.denominators(s), og.scale_factors(q, bins))
+        assert_g2_close(alone["g2"], ref, G2_RTOL)
+    assert checked == {"dense", "events"}

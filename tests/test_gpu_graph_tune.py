@@ -144,3 +144,65 @@ def test_autotune_records_and_reapplies(tmp_path, monkeypatch):
     p2 = c2._plans[np.dtype(np.uint8)]
     assert p2.get_tuning() == plan.get_tuning()
     np.testing.assert_array_equal(g.view(np.uint32), g2.view(np.uint32))
+
+
+def test_set_tuning_allocation_failure_keeps_plan_whole():
+    """A set_tuning whose work-list allocation fails returns ENOMEM (MemoryError) and
+    leaves the plan on its previous tuning, still exact; the same tuning succeeds once
+    allocations work again."""
+    import torch
+    from dynamix_amd import _lib
+    from dynamix_amd.correlator.common import Plan
+    _announce("set_tuning allocation failure")
+    lib = _lib.load()
+    state = {"fail": False, "live": {}}
+
+    def alloc(nbytes, stream, ctx):
+        if state["fail"]:
+            return None
+        t = torch.empty(int(nbytes) + 256, dtype=torch.uint8, device="cuda")
+        addr = (t.data_ptr() + 255) // 256 * 256
+        state["live"][addr] = t
+        return addr
+
+    def free(ptr, stream, ctx):
+        state["live"].pop(int(ptr), None)
+
+    cb_a, cb_f = _lib.ALLOC_FN(alloc), _lib.FREE_FN(free)
+    _lib.check(lib.dxg2_set_allocator(cb_a, cb_f, None))
+    try:
+        q = _qmask(7)
+        f = _stack(8)
+        d = torch.from_numpy(f).cuda()
+        plan = Plan((20, 24), 300, q, np.uint8)
+        ref = og.dense_g2(f, q)
+
+        def run():
+            num = torch.empty((plan.n_bins, 300), dtype=torch.int64, device="cuda")
+            g2 = torch.empty((plan.n_bins, 300), dtype=torch.float32, device="cuda")
+            st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+            _lib.check(lib.dxg2_correlate_dense(plan.handle, _lib.ptr(d), _lib.ptr(num), None,
+                                                None, _lib.ptr(g2), st))
+            torch.cuda.synchronize()
+            return num.cpu().numpy(), g2.cpu().numpy()
+
+        num0, g20 = run()
+        np.testing.assert_array_equal(num0, ref["num"])
+        before = plan.get_tuning()
+        new = {"tile": 128, "band": 2, "group": 16, "queue": 1 - before["queue"]}
+        state["fail"] = True
+        with pytest.raises(MemoryError):
+            plan.set_tuning(**new)
+        state["fail"] = False
+        assert plan.get_tuning() == before
+        num1, g21 = run()
+        np.testing.assert_array_equal(num1, num0)
+        np.testing.assert_array_equal(g21.view(np.uint32), g20.view(np.uint32))
+        plan.set_tuning(**new)
+        assert plan.get_tuning() == new
+        num2, _ = run()
+        np.testing.assert_array_equal(num2, num0)
+        plan.close()
+    finally:
+        lib.dxg2_set_allocator(_lib.ALLOC_FN(), _lib.FREE_FN(), None)
+        torch.cuda.synchronize()

@@ -73,3 +73,32 @@ def test_two_time_bad_bin():
     c = MatMulCorrelator((4, 4), 10, None)
     with pytest.raises(ValueError):
         c.two_time(f, 2)
+
+
+@pytest.mark.parametrize("dtype", ["uint8", "uint16"])
+def test_two_time_from_events_equals_dense_and_oracle(dtype):
+    """EventCorrelator.two_time on the pixel-major events of a stack equals the dense
+    two-time matrix of the same stack bit for bit, and the oracle within float32
+    rounding; a bin with no pixels is all NaN."""
+    from dynamix_amd.correlator.dense import MatMulCorrelator
+    from dynamix_amd.correlator.event import EventCorrelator
+    from dynamix_amd.synth import speckle_frames
+    from oracle import events as ev
+    _announce("ttcf from events %s" % dtype)
+    shape, n = (14, 16), 300
+    f = speckle_frames(shape, n, 0.4, 20.0, 11, np.dtype(dtype))
+    if dtype == "uint16":
+        f[5, 3, 4] = 4000
+    q = _bins(np.random.default_rng(2), shape, 3)
+    q[q == 2] = 4      # bin 2 has no pixels
+    d, t, o = ev.compress_all_stack(f)
+    ce = EventCorrelator(shape, n, max_events_count=1, qmask=q, dtype=np.dtype(dtype))
+    cd = MatMulCorrelator(shape, n, q)
+    for b in (1, 3, 4):
+        te = ce.two_time(t, d, o, b)
+        td = cd.two_time(f, b)
+        np.testing.assert_array_equal(te.view(np.uint32), td.view(np.uint32))
+        _check(te, og.two_time_fp64(f, q, b))
+    assert np.isnan(ce.two_time(t, d, o, 2)).all()
+    with pytest.raises(ValueError):
+        ce.two_time(t, d, o, 5)

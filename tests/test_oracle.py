@@ -96,8 +96,11 @@ def test_two_time_closed_forms():
     off = ~np.eye(12, dtype=bool)
     np.testing.assert_allclose(trc[off], v, rtol=1e-14)
     np.testing.assert_allclose(np.diagonal(trc)[:-1], v, rtol=1e-14)
-    assert trc[-1, -1] == 0.0
-    # a last frame with no counts: its row, column and corner are NaN
+    # the corner keeps the photon pairs within the last frame: n_b sum c(c-1) / (sum c)^2
+    np.testing.assert_allclose(trc[-1, -1], cb.size * (cb * (cb - 1)).sum() / cb.sum() ** 2,
+                               rtol=1e-14)
+    # a last frame with no counts: its row, column and corner are NaN (0/0); fecorrt
+    # symmetrises the pair counts before the normalisation, so both triangles are NaN
     f0 = f.copy()
     f0[-1] = 0
     t0 = og.two_time_fp64(f0, q, 1)
@@ -286,3 +289,77 @@ def test_coo_csr_round_trip():
     np.testing.assert_array_equal(t2, t)
     np.testing.assert_array_equal(d2, d)
     np.testing.assert_array_equal(o2, o)
+
+
+# ------------------------------------------- fast restatements for detector sizes
+@pytest.mark.parametrize("seed", range(4))
+def test_fast_dense_oracle_equals_per_bin_and_brute_force(seed):
+    """sums_numerators_fast (one transpose, float64 or int64 Gram) equals the per-bin
+    restatement and the Python-integer brute force, int8 negatives included."""
+    f, q = _tiny(seed, n=9, shape=(4, 5), hi=256 if seed % 2 else 5, nbins=3)
+    if seed == 2:
+        f = (f.astype(np.int16) - 2).astype(np.int8)
+    _, bins = og.qmask_bins(f.shape[1:], q)
+    s, num = og.sums_numerators_fast(f, q, bins)
+    np.testing.assert_array_equal(s, og.frame_sums(f, q, bins))
+    np.testing.assert_array_equal(num, og.numerators(f, q, bins))
+    bnum, bs, _ = og.brute_force(f, q)
+    assert [[int(x) for x in r] for r in num] == [[int(x) for x in r] for r in bnum]
+    assert [[int(x) for x in r] for r in s] == [[int(x) for x in r] for r in bs]
+
+
+def test_fast_dense_oracle_int64_path():
+    """Values whose squares times the pixel count reach 2^53 take the int64 Gram."""
+    rng = np.random.default_rng(4)
+    f = rng.integers(60000, 65536, size=(6, 3, 3)).astype(np.uint16)
+    q = np.ones((3, 3), np.int32)
+    assert 65535 ** 2 * 9 < 2 ** 53   # float path here; force the int64 path below
+    _, bins = og.qmask_bins((3, 3), q)
+    s, num = og.sums_numerators_fast(f, q, bins)
+    np.testing.assert_array_equal(num, og.numerators(f, q, bins))
+    big = np.full((4, 1, 2100), 65535, dtype=np.uint16)   # 65535^2 * 2100 > 2^53
+    qb = np.ones((1, 2100), np.int32)
+    s2, num2 = og.sums_numerators_fast(big, qb, np.array([1]))
+    assert [int(x) for x in num2[0]] == [65535 ** 2 * 2100 * (4 - tau) for tau in range(4)]
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_fast_event_oracle_equals_loops_and_dense(seed):
+    """event_sums_numerators_fast equals the per-pixel restatement, the brute-force
+    pair loop and the dense oracle on the same stack (pixels with many events, empty
+    pixels, excluded pixels and a bin with no events)."""
+    rng = np.random.default_rng(100 + seed)
+    n = 40
+    f = (rng.random((n, 6, 7)) < (0.05 + 0.2 * seed)).astype(np.uint8) * \
+        rng.integers(1, 256 if seed % 2 else 4, size=(n, 6, 7)).astype(np.uint8)
+    q = rng.integers(0, 5, size=(6, 7)).astype(np.int32)
+    q[0, :] = 4
+    f[:, 0, :] = 0           # bin 4 holds pixels without events only
+    q.flat[8] = 1
+    q.flat[9] = 2
+    q.flat[10] = 3
+    d, t, o = ev.compress_all_stack(f)
+    _, bins = og.qmask_bins(q.shape, q)
+    s, num = ev.event_sums_numerators_fast(t, d, o, q, n, bins)
+    s0, num0 = ev.event_sums_numerators(t, d, o, q, n, bins)
+    np.testing.assert_array_equal(s, s0)
+    np.testing.assert_array_equal(num, num0)
+    bnum, bs = ev.event_brute_force(t, d, o, q, n)
+    assert [[int(x) for x in r] for r in num] == [[int(x) for x in r] for r in bnum]
+    assert [[int(x) for x in r] for r in s] == [[int(x) for x in r] for r in bs]
+    dn = og.dense_g2(f, q)
+    np.testing.assert_array_equal(num, dn["num"])
+    np.testing.assert_array_equal(s, dn["sums"])
+
+
+def test_fast_event_oracle_single_long_pixel():
+    """One pixel with an event in every frame: Num(tau) = sum_t d_t d_(t+tau)."""
+    n = 64
+    d = np.arange(1, n + 1, dtype=np.uint8)
+    t = np.arange(n, dtype=np.int32)
+    o = np.array([0, 0, n, n], dtype=np.uint32)
+    q = np.array([[0, 1, 1]], np.int32)
+    s, num = ev.event_sums_numerators_fast(t, d, o, q, n, np.array([1]))
+    dd = d.astype(np.int64)
+    np.testing.assert_array_equal(num[0], [int((dd[:n - k] * dd[k:]).sum()) for k in range(n)])
+    np.testing.assert_array_equal(s[0], dd)
