@@ -23,6 +23,8 @@ DXG2_ECAPACITY = 5
 DXG2_FLAG_TILE128 = 1
 DXG2_FLAG_NO_QUEUE = 2
 
+DXG2_STATIC_SCRATCH_WORDS = 65536   # static workspace of dxg2_events_window_noalloc
+
 DXG2_U8 = 1
 DXG2_I8 = 2
 DXG2_U16 = 3
@@ -84,6 +86,9 @@ _SIGNATURES = {
     "dxg2_events_window": (c_int, [c_void_p, c_void_p, c_void_p, c_i64, c_i64, c_int, c_int,
                                    c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_i64,
                                    c_void_p]),
+    "dxg2_events_window_noalloc": (c_int, [c_void_p, c_void_p, c_void_p, c_i64, c_i64, c_int,
+                                           c_int, c_int, c_void_p, c_void_p, c_void_p, c_i64,
+                                           c_void_p]),
     "dxg2_events_to_columns": (c_int, [c_void_p, c_void_p, c_void_p, c_i64, c_int, c_int,
                                        c_void_p, c_i64, c_void_p, c_void_p, c_void_p]),
     "dxg2_events_to_coo": (c_int, [c_void_p, c_void_p, c_void_p, c_i64, c_int, c_int, c_int,

@@ -1747,6 +1747,63 @@ int dxg2_events_to_columns(const uint32_t* d_offsets, const int32_t* d_times,
     return DXG2_OK;
 }
 
+// Allocation-free variant of dxg2_events_window: the scan's block totals live in a
+// file-scope device array instead of a caller workspace.  One 32-bit total per 1024
+// pixels, so the array bounds the pixel count at DXG2_STATIC_SCRATCH_WORDS * 1024.
+// Each GPU holds its own copy at its own address; host code reaches it only through the
+// address hipGetSymbolAddress returns for the current device, fetched once per device.
+// The scan overwrites every total it reads, so nothing needs clearing between calls;
+// calls that overlap on one GPU (different streams, or threads) race on the array.
+static __device__ __align__(256) uint32_t scratch[DXG2_STATIC_SCRATCH_WORDS];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+uint32_t* g_scratch_ptr[kMaxDevices] = {};
+
+int static_scratch(uint32_t** out) {
+    int dev = 0;
+    DXG2_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return set_error(DXG2_EUNSUPPORTED, "device %d", dev);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        DXG2_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[dev] = static_cast<uint32_t*>(ptr);
+    }
+    *out = g_scratch_ptr[dev];
+    return DXG2_OK;
+}
+}  // namespace
+
+int dxg2_events_window_noalloc(const uint32_t* d_offsets, const int32_t* d_times,
+                               const void* d_data, int64_t nnz, int64_t npix, int32_t dtype,
+                               int32_t t0, int32_t t1, uint32_t* d_out_offsets,
+                               int32_t* d_out_times, void* d_out_data, int64_t capacity,
+                               void* stream) {
+    if (!d_offsets || !d_times || !d_data || !d_out_offsets || !d_out_times || !d_out_data)
+        return set_error(DXG2_EINVAL, "null argument");
+    if (npix <= 0 || nnz < 0 || t0 < 0 || t1 < t0)
+        return set_error(DXG2_EINVAL, "bad window [%d, %d) or sizes", t0, t1);
+    if (npix > (int64_t)DXG2_STATIC_SCRATCH_WORDS * 1024)
+        return set_error(DXG2_EINVAL, "%lld pixels exceed the static scratch limit of %lld "
+                         "pixels; use dxg2_events_window with a workspace", (long long)npix,
+                         (long long)DXG2_STATIC_SCRATCH_WORDS * 1024);
+    if (dtype != DXG2_U8 && dtype != DXG2_U16)
+        return set_error(DXG2_EINVAL, "event counts are uint8 or uint16");
+    if (capacity < nnz)
+        return set_error(DXG2_ECAPACITY, "capacity %lld < %lld input events (the window is "
+                         "at most the input)", (long long)capacity, (long long)nnz);
+    uint32_t* ws = nullptr;
+    const int rc = static_scratch(&ws);
+    if (rc != DXG2_OK) return rc;
+    (void)hipGetLastError();
+    launch_events_window(dtype, d_offsets, d_times, d_data, npix, t0, t1, ws, d_out_offsets,
+                         d_out_times, d_out_data, (hipStream_t)stream);
+    DXG2_HIP(hipGetLastError());
+    return DXG2_OK;
+}
+
 int dxg2_compress_scratch_bytes(int32_t nframes, int32_t height, int32_t width, size_t* bytes) {
     if (!bytes || nframes <= 0 || height <= 0 || width <= 0)
         return set_error(DXG2_EINVAL, "bad compressor geometry");

@@ -277,6 +277,22 @@ int dxg2_events_to_columns(const uint32_t* d_offsets, const int32_t* d_times,
                            const int32_t* d_cols, int64_t n_cols, void* d_out, int32_t* d_err,
                            void* stream);
 
+/* dxg2_events_window without the workspace argument, for callers that cannot allocate
+ * device memory (code under HIP graph capture, serving loops that must not call
+ * hipMalloc).  No reference counterpart.  The scan's workspace is a static device array
+ * of DXG2_STATIC_SCRATCH_WORDS uint32 (256 KiB, held by each GPU the library runs on
+ * until the process ends), so npix is at most DXG2_STATIC_SCRATCH_WORDS * 1024
+ * (67,108,864 pixels); larger inputs return DXG2_EINVAL.  The call allocates nothing and
+ * launches kernels only.  Every call on one GPU shares that array: calls that overlap on
+ * one GPU (concurrent streams or host threads) race and must be serialised by the caller,
+ * e.g. by issuing them on one stream.  Results equal dxg2_events_window's. */
+#define DXG2_STATIC_SCRATCH_WORDS 65536
+int dxg2_events_window_noalloc(const uint32_t* d_offsets, const int32_t* d_times,
+                               const void* d_data, int64_t nnz, int64_t npix, int32_t dtype,
+                               int32_t t0, int32_t t1, uint32_t* d_out_offsets,
+                               int32_t* d_out_times, void* d_out_data, int64_t capacity,
+                               void* stream);
+
 /* Dense frames -> per-pixel event lists on the GPU (FramesCompressor, event.py:207-321):
  * for every pixel p in raster order, its samples with value > 0 (signed for int8) in
  * frame order, data[offsets[p] .. offsets[p+1]) and times[...] (int32).  This is the

@@ -91,3 +91,21 @@ def test_autotune_key_and_cache_roundtrip(tmp_path):
     autotune.save_entry("other", {"tile": 128, "band": 2, "group": 16, "queue": 0}, p)
     c = autotune.load_cache(p)
     assert c[k1]["band"] == 4 and c["other"]["tile"] == 128
+
+
+def test_noalloc_window_rejects_oversize_on_host():
+    """dxg2_events_window_noalloc refuses pixel counts beyond its static workspace before
+    any device call (the pointers are never touched)."""
+    import ctypes
+    lib = L.load()
+    fake = ctypes.c_void_p(256)
+    too_many = L.DXG2_STATIC_SCRATCH_WORDS * 1024 + 1
+    rc = lib.dxg2_events_window_noalloc(fake, fake, fake, 10, too_many, L.DXG2_U8, 0, 5, fake,
+                                        fake, fake, 10, None)
+    assert rc == L.DXG2_EINVAL
+    assert b"static scratch limit" in lib.dxg2_last_error()
+    rc = lib.dxg2_events_window_noalloc(None, fake, fake, 10, 100, L.DXG2_U8, 0, 5, fake, fake,
+                                        fake, 10, None)
+    assert rc == L.DXG2_EINVAL
+    assert L.DXG2_STATIC_SCRATCH_WORDS == int(
+        [ln for ln in open(L.HEADER) if "#define DXG2_STATIC_SCRATCH_WORDS" in ln][0].split()[2])
