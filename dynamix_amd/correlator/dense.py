@@ -228,19 +228,25 @@ class PlainMatMulCorrelator(_DenseHIP):
         return g2.cpu().numpy()
 
 
+# Half-lengths L (complex points) of the in-LDS FFT plans: 2^a 5^b, 1024 <= L <= 16384.
+FFT_HALF_LENGTHS = (1024, 1280, 2048, 2560, 4096, 5120, 8192, 10240, 16384)
+
+
 def fft_length(nframes):
-    """Nf = 2 L, L the smallest power of two >= max(nframes, 1024); Nf >= 2 N - 1 is all the
-    linear autocorrelation needs (the reference uses get_next_power(2N), utils.py:50-60)."""
-    L = 1024
-    while L < nframes:
-        L *= 2
-    return 2 * L
+    """Nf = 2 L for the smallest L of FFT_HALF_LENGTHS with L >= nframes: Nf >= 2 N - 1
+    is all the linear autocorrelation needs (the reference uses get_next_power(2N),
+    utils.py:50-60, another smooth length of the same kind).  0 when N > 16384."""
+    for L in FFT_HALF_LENGTHS:
+        if L >= nframes:
+            return 2 * L
+    return 0
 
 
 class FFTCorrelator(_DenseHIP):
     """Drop-in for FFTWCorrelator / CUFFTCorrelator (dense.py:275-369, cuda.py:129-219).
 
-    uint8 frames, nframes <= 16384.  `weights` and `precompute_fft_plans` are accepted;
+    Frames of any dtype the exact paths take (uint8, int8, uint16, or wider types
+    narrowed to them), nframes <= 16384.  `weights` and `precompute_fft_plans` are accepted;
     weights are ignored as in the reference FFT correlators (dense.py:277-285), and the
     FFT needs no plan.  scale_factor is ignored (the reference divides by the pixel count,
     dense.py:366).  Numerators carry FP32 FFT rounding: |Num - Num_exact| <= 2e-7 * Num(0)
@@ -261,8 +267,6 @@ class FFTCorrelator(_DenseHIP):
     def correlate(self, frames):
         torch = _torch()
         d_frames, dtype = to_device_frames(frames, self.shape, self.nframes, torch)
-        if dtype != np.dtype(np.uint8):
-            raise NotImplementedError("the FFT path takes uint8 frames, got %s" % dtype)
         plan = self._plan(dtype)
         nb, n = max(self.n_bins, 0), self.nframes
         num = torch.empty((nb, n), dtype=torch.float64, device="cuda")

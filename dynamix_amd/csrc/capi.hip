@@ -566,24 +566,18 @@ int next_events(dxg2_plan* p, hipEvent_t** out) {
 // Pixels of one bin's list handled by one event work item.
 constexpr int EVENT_ITEM_PIXELS = 1024;
 
-// FFT length: Nf = 2L with L the smallest power of two >= max(N, 1024) (any Nf >= 2N - 1
-// gives the linear autocorrelation); the LDS holds L complex points, so L <= 16384.
-constexpr int FFT_MIN_L = 1024;
-constexpr int FFT_MAX_L = 16384;
-
-int fft_length(int n) {
-    int L = FFT_MIN_L;
-    while (L < n) L *= 2;
-    return L;
-}
+// FFT length: Nf = 2L with L the smallest length of the mixed-radix set (2^a 5^b, at
+// least 1024) with L >= N; any Nf >= 2N - 1 gives the linear autocorrelation.  A series
+// of L complex points is transformed in LDS, so L <= 16384.
+int fft_length(int n) { return fft_lds_length(std::max(n, 1)); }
 
 int ensure_fft_buffers(dxg2_plan* p, hipStream_t s) {
     if (p->d_power) return DXG2_OK;
     p->kb_work.clear();
     const int L = fft_length(p->N);
-    if (L > FFT_MAX_L)
-        return set_error(DXG2_EUNSUPPORTED, "FFT path supports up to %d frames (got %d)",
-                         FFT_MAX_L, p->N);
+    if (L == 0)
+        return set_error(DXG2_EUNSUPPORTED, "FFT path supports up to 16384 frames (got %d)",
+                         p->N);
     p->fft_L = L;
     for (int b = 0; b < p->n_bins; ++b)
         for (int kb = 0; kb < p->bins[b].nkb; ++kb) p->kb_work.push_back(KbWork{b, kb});
@@ -1612,8 +1606,6 @@ int dxg2_correlate_fft(dxg2_plan* p, const void* d_frames, double* d_num, int64_
                        double* d_den, float* d_g2, void* stream) {
     if (!p) return set_error(DXG2_EINVAL, "null plan");
     if (!d_frames || !d_g2) return set_error(DXG2_EINVAL, "frames and g2 must be non-null");
-    if (p->dtype != DXG2_U8)
-        return set_error(DXG2_EUNSUPPORTED, "FFT path takes uint8 frames");
     if (p->n_bins == 0) return DXG2_OK;
     int rc = ensure_fft_buffers(p, (hipStream_t)stream);
     if (rc) return rc;
@@ -1628,10 +1620,12 @@ int dxg2_correlate_fft(dxg2_plan* p, const void* d_frames, double* d_num, int64_
                      p->d_pix, p->d_bins, p->d_kbwork, (int)p->kb_work.size(), p->d_xs, s);
     int64_t max_npix = 0;
     for (auto n : p->npix) max_npix = std::max(max_npix, n);
-    launch_pm_sums(p->d_xs, p->d_bins, p->n_bins, (int)max_npix, p->N, p->nrows, sums, s);
+    launch_pm_sums(p->d_xs, p->dtype, p->d_bins, p->n_bins, (int)max_npix, p->N, p->nrows, sums,
+                   s);
     if (ev) DXG2_HIP(hipEventRecord(ev[1], s));
-    if (launch_fft_power(p->fft_L, p->d_xs, p->d_bins, p->d_kbwork, (int)p->kb_work.size(), p->N,
-                         p->nrows, p->d_tw, p->d_power, s))
+    if (launch_fft_power(p->fft_L, p->dtype, p->d_xs, p->d_bins, p->d_kbwork,
+                         (int)p->kb_work.size(), p->nrows, p->d_tw, p->d_tw + p->fft_L,
+                         p->d_power, s))
         return set_error(DXG2_EUNSUPPORTED, "no FFT kernel for length %d", p->fft_L);
     launch_fft_inverse(p->d_power, p->fft_L, p->n_bins, p->N, p->d_fnum, s);
     if (ev) DXG2_HIP(hipEventRecord(ev[2], s));

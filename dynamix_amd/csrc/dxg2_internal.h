@@ -192,10 +192,11 @@ void launch_finalize_f64(const int64_t* sums, const double* fnum, const BinDev* 
 void launch_gather_pm(const uint8_t* frames, int dtype, int64_t hw, int nframes, int nrows,
                       const int32_t* pix_idx, const BinDev* bins, const KbWork* work, int n_work,
                       uint8_t* xs, hipStream_t s);
-void launch_pm_sums(const uint8_t* xs, const BinDev* bins, int n_bins, int max_npix, int nframes,
-                    int nrows, int64_t* sums, hipStream_t s);
-int launch_fft_power(int L, const uint8_t* xs, const BinDev* bins, const KbWork* work,
-                     int n_work, int nframes, int nrows, const float2* tw, double* power,
+void launch_pm_sums(const uint8_t* xs, int dtype, const BinDev* bins, int n_bins, int max_npix,
+                    int nframes, int nrows, int64_t* sums, hipStream_t s);
+int fft_lds_length(int n);
+int launch_fft_power(int L, int dtype, const uint8_t* xs, const BinDev* bins, const KbWork* work,
+                     int n_work, int nrows, const float2* tw, const float2* uw, double* power,
                      hipStream_t s);
 void launch_fft_inverse(const double* power, int L, int n_bins, int nframes, double* num,
                         hipStream_t s);

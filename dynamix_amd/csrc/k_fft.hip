@@ -9,16 +9,23 @@
 // Here, per bin:
 //   Num(tau) = IFFT_Nf( sum_p |X_p(f)|^2 )[tau],   X_p = FFT_Nf(x_p zero-padded),
 // (the product with the reversed series is the same autocorrelation; the inverse is
-// linear, so one inverse per bin replaces one per pixel).
-//  * gather_pm_kernel: frames -> pixel-major bytes xs[k][t] (k in the bin's raster
-//    list order, t < nrows, zero past the last frame), 64 pixels x 256 frames per
-//    workgroup through an LDS transpose.
-//  * pm_sums_kernel: exact S_b(t) from xs.
-//  * fft_power_kernel: one workgroup per 64 pixels of a bin; each pixel's series is
-//    packed as z[n] = x[2n] + i x[2n+1] (L = Nf/2 complex points), transformed by an
-//    FP32 Stockham FFT in LDS (Govindaraju et al., "High performance discrete Fourier
-//    transforms on graphics processors", SC 2008: radix-4 passes, one radix-2 pass when
-//    log2 L is odd), unpacked to X[k], k = 0..L, and |X[k]|^2 is accumulated in FP64.
+// linear, so one inverse per bin replaces one per pixel).  Nf = 2L with L >= N from a
+// set of mixed-radix lengths (2^a 5^b), so a linear autocorrelation needs no more.
+//  * gather_pm_kernel<E>: frames -> pixel-major series xs[k][t] of the frame element
+//    type E (uint8, int8 or uint16; k in the bin's raster list order, t < nrows, zero
+//    past the last frame), 64 pixels x 256 frames per workgroup through an LDS transpose.
+//  * pm_sums_kernel<E>: exact S_b(t) from xs.
+//  * fft_spectra_kernel<E, L, T, R...>: one workgroup per (bin, k-block of 64 series);
+//    each series is packed as z[n] = x[2n] + i x[2n+1] (L complex points) and
+//    transformed by an FP32 mixed-radix Stockham FFT (Govindaraju et al., "High
+//    performance discrete Fourier transforms on graphics processors", SC 2008) with
+//    register DFTs of radix 4, 5, 8 or 16, then unpacked to X[k], k = 0..L, and |X[k]|^2
+//    is accumulated in FP64 per thread.  The first pass reads its inputs straight from
+//    HBM into registers (the next series' are loaded while the current one is
+//    transformed); the twiddles of the middle passes come from small LDS tables and those
+//    of the last pass and the unpacking factors are held in registers, so the series
+//    loop issues no other global load.  The last pass works in place (each butterfly
+//    writes the points it read).
 //  * fft_inverse_kernel: Num(tau) = (P_0 + (-1)^tau P_L + 2 sum_{k=1}^{L-1} P_k
 //    cos(2 pi k tau / Nf)) / Nf in FP64, the cosine by an FP64 rotation recurrence.
 // Twiddle tables are computed on the host in FP64 and rounded to FP32.
@@ -26,108 +33,6 @@
 
 namespace dxg2 {
 
-// threads per FFT workgroup: L/4 butterflies per radix-4 pass, at most 1024 threads
-template <int L>
-struct FftCfg {
-    static constexpr int THREADS = L / 4 < 1024 ? L / 4 : 1024;
-};
-
-// --------------------------------------------------------------- pixel-major gather
-// grid (bin k-blocks, frame tiles of 256).  Lane = pixel j of the k-block; each wave
-// stages 64 frames, the LDS tile is [64 pixels][256 frames + 4].
-__global__ __launch_bounds__(256) void gather_pm_kernel(const uint8_t* __restrict__ frames,
-                                                        int dtype, int64_t hw, int nframes,
-                                                        int nrows, const int32_t* __restrict__ pix_idx,
-                                                        const BinDev* __restrict__ bins,
-                                                        const KbWork* __restrict__ work,
-                                                        uint8_t* __restrict__ xs) {
-    __shared__ uint8_t tile[64][256 + 4];
-    const KbWork w = work[blockIdx.x];
-    const BinDev bd = bins[w.bin];
-    const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    const int t0 = blockIdx.y * 256;
-    const int p = pix_idx[bd.idx_off + (int64_t)w.kb * KB + lane];
-    const int pc = p >= 0 ? p : 0;
-    for (int f = wave; f < 256; f += 4) {
-        const int t = t0 + f;
-        uint8_t v = 0;
-        if (t < nframes && p >= 0) {
-            if (dtype == DXG2_U16) {
-                const uint16_t x = reinterpret_cast<const uint16_t*>(frames)[(int64_t)t * hw + pc];
-                v = x > 255 ? 255 : (uint8_t)x;   // not reached: the FFT path takes 8-bit input
-            } else {
-                v = frames[(int64_t)t * hw + pc];
-            }
-        }
-        tile[lane][f] = v;
-    }
-    __syncthreads();
-    // write: pixel row j gets 256 contiguous bytes; 64 threads per 1 KiB? -> thread
-    // writes 16 bytes: pixel j = tid / 16, bytes (tid % 16) * 16 .. +16 of the tile row
-    for (int c = threadIdx.x; c < 64 * 16; c += 256) {
-        const int j = c >> 4;
-        const int o = (c & 15) * 16;
-        uint32_t wd[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-            wd[q] = (uint32_t)tile[j][o + 4 * q] | ((uint32_t)tile[j][o + 4 * q + 1] << 8) |
-                    ((uint32_t)tile[j][o + 4 * q + 2] << 16) |
-                    ((uint32_t)tile[j][o + 4 * q + 3] << 24);
-        uint8_t* dst = xs + bd.slab_off + ((int64_t)w.kb * KB + j) * nrows + t0 + o;
-        *reinterpret_cast<uint4*>(dst) = make_uint4(wd[0], wd[1], wd[2], wd[3]);
-    }
-}
-
-// S_b(t) = sum over the bin's pixels of xs[k][t].
-// grid (frame tiles of 256, bins, pixel chunks of PM_CH).  Thread (r, g) = (tid / 16,
-// tid % 16) reads 16 frames (16 bytes) of pixel rows chunk + r, r + 16, ... and adds
-// them into 16-bit lanes (even and odd bytes of each word apart: at most PM_CH / 16 rows
-// of 255, below 2^16); the 16 row groups are reduced in LDS and each frame's chunk total
-// goes to sums (zeroed first) with one 64-bit atomic add.
-constexpr int PM_CH = 1024;
-
-__global__ __launch_bounds__(256) void pm_sums_kernel(const uint8_t* __restrict__ xs,
-                                                      const BinDev* __restrict__ bins, int nframes,
-                                                      int nrows, int64_t* __restrict__ sums) {
-    __shared__ uint32_t part[16][256 + 1];
-    const int b = blockIdx.y;
-    const BinDev bd = bins[b];
-    const int k0 = blockIdx.z * PM_CH;
-    if (k0 >= bd.npix) return;
-    const int k1 = min(bd.npix, k0 + PM_CH);
-    const int t0 = blockIdx.x * 256;
-    const int r = threadIdx.x >> 4, g = threadIdx.x & 15;
-    uint32_t ev[4] = {0u, 0u, 0u, 0u}, od[4] = {0u, 0u, 0u, 0u};
-    const uint8_t* base = xs + bd.slab_off + t0 + 16 * g;
-    for (int k = k0 + r; k < k1; k += 16) {
-        const uint4 v = *reinterpret_cast<const uint4*>(base + (int64_t)k * nrows);
-        const uint32_t w[4] = {v.x, v.y, v.z, v.w};
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            ev[q] += w[q] & 0x00FF00FFu;
-            od[q] += (w[q] >> 8) & 0x00FF00FFu;
-        }
-    }
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {  // frame 16 g + 4 q + {0, 1, 2, 3}
-        part[r][16 * g + 4 * q + 0] = ev[q] & 0xFFFFu;
-        part[r][16 * g + 4 * q + 1] = od[q] & 0xFFFFu;
-        part[r][16 * g + 4 * q + 2] = ev[q] >> 16;
-        part[r][16 * g + 4 * q + 3] = od[q] >> 16;
-    }
-    __syncthreads();
-    const int t = t0 + threadIdx.x;
-    if (t < nframes) {
-        uint32_t tot = 0;
-#pragma unroll
-        for (int i = 0; i < 16; ++i) tot += part[i][threadIdx.x];
-        if (tot) atomicAdd(reinterpret_cast<unsigned long long*>(sums + (int64_t)b * nframes + t),
-                           (unsigned long long)tot);
-    }
-}
-
-// --------------------------------------------------------------------- forward FFT
 struct c32 {
     float x, y;
 };
@@ -136,152 +41,86 @@ __device__ __forceinline__ c32 cmul(c32 a, c32 b) {
 }
 __device__ __forceinline__ c32 cadd(c32 a, c32 b) { return c32{a.x + b.x, a.y + b.y}; }
 __device__ __forceinline__ c32 csub(c32 a, c32 b) { return c32{a.x - b.x, a.y - b.y}; }
-// -i * a
-__device__ __forceinline__ c32 cmuli_neg(c32 a) { return c32{a.y, -a.x}; }
+__device__ __forceinline__ c32 negi(c32 a) { return c32{a.y, -a.x}; }   // -i a
+__device__ __forceinline__ c32 cscale(c32 a, float s) { return c32{a.x * s, a.y * s}; }
+__device__ __forceinline__ c32 c_of(float2 t) { return c32{t.x, t.y}; }
 
-// grid: one workgroup per (bin, k-block) of 64 pixels.  LDS holds L complex points.
-// tw[m] = exp(-2 pi i m / L), m < L, and tw[L + k] = exp(-pi i k / L), k <= L, the
-// unpacking factor (host FP64, rounded to FP32).
-template <int L>
-__global__ __launch_bounds__(FftCfg<L>::THREADS) void fft_power_kernel(
-    const uint8_t* __restrict__ xs, const BinDev* __restrict__ bins,
-    const KbWork* __restrict__ work, int nframes, int nrows, const float2* __restrict__ tw,
-    double* __restrict__ power) {
-    __shared__ c32 z[L];
-    constexpr int FFT_THREADS = FftCfg<L>::THREADS;
-    constexpr int LOG2 = __builtin_ctz(L);
-    constexpr int PER = L / FFT_THREADS;          // complex points per thread (load)
-    constexpr int BF4 = L / 4 / FFT_THREADS;      // radix-4 butterflies per thread per pass
-    constexpr int NK = (L + 1 + FFT_THREADS - 1) / FFT_THREADS;  // power bins per thread
-    const KbWork w = work[blockIdx.x];
-    const BinDev bd = bins[w.bin];
-    const int tid = threadIdx.x;
-    double acc[NK];
-#pragma unroll
-    for (int m = 0; m < NK; ++m) acc[m] = 0.0;
+// ------------------------------------------------------------- register DFTs
+// Dft<R>::run(a): in place, natural order, a[k] = sum_n a[n] exp(-2 pi i n k / R).
+template <int R>
+struct Dft;
 
-    const int kend = min(KB, bd.npix - w.kb * KB);
-    for (int j = 0; j < kend; ++j) {
-        const uint8_t* x = xs + bd.slab_off + ((int64_t)w.kb * KB + j) * nrows;
-        // load z[n] = x[2n] + i x[2n+1] (zero past the last frame)
-#pragma unroll
-        for (int m = 0; m < PER; ++m) {
-            const int n = tid + m * FFT_THREADS;
-            const int t = 2 * n;
-            float re = 0.f, im = 0.f;
-            if (t + 1 < nrows) {
-                const uint16_t two = *reinterpret_cast<const uint16_t*>(x + t);
-                re = t < nframes ? (float)(two & 0xFF) : 0.f;
-                im = t + 1 < nframes ? (float)(two >> 8) : 0.f;
-            }
-            z[n] = c32{re, im};
-        }
-        __syncthreads();
-        int ns = 1;
-        if (LOG2 & 1) {  // one radix-2 pass, Ns = 1
-            c32 v[PER / 2][2];
-#pragma unroll
-            for (int m = 0; m < PER / 2; ++m) {
-                const int jj = tid + m * FFT_THREADS;
-                v[m][0] = z[jj];
-                v[m][1] = z[jj + L / 2];
-            }
-            __syncthreads();
-#pragma unroll
-            for (int m = 0; m < PER / 2; ++m) {
-                const int jj = tid + m * FFT_THREADS;
-                z[2 * jj] = cadd(v[m][0], v[m][1]);
-                z[2 * jj + 1] = csub(v[m][0], v[m][1]);
-            }
-            __syncthreads();
-            ns = 2;
-        }
-        // kept rolled: unrolled, the twiddle loads of every pass would be hoisted above
-        // the barriers and overflow the register file
-#pragma unroll 1
-        for (; ns < L; ns *= 4) {
-            c32 v[BF4][4];
-            const int tstep = L / (4 * ns);   // twiddle index step: exp(-2 pi i r k / (4 ns))
-#pragma unroll
-            for (int m = 0; m < BF4; ++m) {
-                const int jj = tid + m * FFT_THREADS;
-                const int k = jj % ns;
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    c32 a = z[jj + r * (L / 4)];
-                    if (r) {
-                        const float2 t = tw[(r * k * tstep) & (L - 1)];
-                        a = cmul(a, c32{t.x, t.y});
-                    }
-                    v[m][r] = a;
-                }
-            }
-            __syncthreads();
-#pragma unroll
-            for (int m = 0; m < BF4; ++m) {
-                const int jj = tid + m * FFT_THREADS;
-                const int k = jj % ns;
-                const c32 s02 = cadd(v[m][0], v[m][2]), d02 = csub(v[m][0], v[m][2]);
-                const c32 s13 = cadd(v[m][1], v[m][3]), d13 = cmuli_neg(csub(v[m][1], v[m][3]));
-                const int o = (jj / ns) * ns * 4 + k;
-                z[o] = cadd(s02, s13);
-                z[o + ns] = cadd(d02, d13);
-                z[o + 2 * ns] = csub(s02, s13);
-                z[o + 3 * ns] = csub(d02, d13);
-            }
-            __syncthreads();
-        }
-        // unpack X[k] = (Z[k] + conj Z[L-k]) / 2 - i W^k (Z[k] - conj Z[L-k]) / 2,
-        // W = exp(-pi i / L); accumulate |X[k]|^2
-#pragma unroll
-        for (int m = 0; m < NK; ++m) {
-            const int k = tid + m * FFT_THREADS;
-            if (k <= L) {
-                const c32 a = z[k & (L - 1)];
-                const c32 b = z[(L - k) & (L - 1)];
-                const c32 s = c32{a.x + b.x, a.y - b.y};   // Z[k] + conj Z[L-k]
-                const c32 d = c32{a.x - b.x, a.y + b.y};   // Z[k] - conj Z[L-k]
-                const float2 u = tw[L + k];                 // W^k = exp(-pi i k / L)
-                const c32 wd = cmul(c32{u.x, u.y}, d);
-                const float xr = 0.5f * (s.x + wd.y);      // s/2 - i wd/2
-                const float xi = 0.5f * (s.y - wd.x);
-                acc[m] += (double)xr * xr + (double)xi * xi;
-            }
-        }
-        __syncthreads();
+template <>
+struct Dft<2> {
+    __device__ __forceinline__ static void run(c32* a) {
+        const c32 s = cadd(a[0], a[1]), d = csub(a[0], a[1]);
+        a[0] = s;
+        a[1] = d;
     }
-    double* prow = power + (int64_t)w.bin * (L + 1);
-#pragma unroll
-    for (int m = 0; m < NK; ++m) {
-        const int k = tid + m * FFT_THREADS;
-        if (k <= L && acc[m] != 0.0) atomicAdd(prow + k, acc[m]);
+};
+
+template <>
+struct Dft<4> {
+    __device__ __forceinline__ static void run(c32* a) {
+        const c32 s02 = cadd(a[0], a[2]), d02 = csub(a[0], a[2]);
+        const c32 s13 = cadd(a[1], a[3]), d13 = negi(csub(a[1], a[3]));
+        a[0] = cadd(s02, s13);
+        a[1] = cadd(d02, d13);
+        a[2] = csub(s02, s13);
+        a[3] = csub(d02, d13);
+    }
+};
+
+template <>
+struct Dft<5> {
+    __device__ __forceinline__ static void run(c32* a) {
+        // cos(2 pi/5), cos(4 pi/5), sin(2 pi/5), sin(4 pi/5)
+        constexpr float C1 = 0.30901699437494745f, C2 = -0.80901699437494734f;
+        constexpr float S1 = 0.95105651629515353f, S2 = 0.58778525229247325f;
+        const c32 t1 = cadd(a[1], a[4]), t2 = cadd(a[2], a[3]);
+        const c32 t3 = csub(a[1], a[4]), t4 = csub(a[2], a[3]);
+        const c32 b1 = c32{a[0].x + C1 * t1.x + C2 * t2.x, a[0].y + C1 * t1.y + C2 * t2.y};
+        const c32 b2 = c32{a[0].x + C2 * t1.x + C1 * t2.x, a[0].y + C2 * t1.y + C1 * t2.y};
+        const c32 d1 = c32{S1 * t3.x + S2 * t4.x, S1 * t3.y + S2 * t4.y};
+        const c32 d2 = c32{S2 * t3.x - S1 * t4.x, S2 * t3.y - S1 * t4.y};
+        a[0] = cadd(a[0], cadd(t1, t2));
+        a[1] = cadd(b1, negi(d1));
+        a[4] = csub(b1, negi(d1));
+        a[2] = cadd(b2, negi(d2));
+        a[3] = csub(b2, negi(d2));
+    }
+};
+
+// exp(-2 pi i m / 8) for m < 4
+__device__ __forceinline__ c32 w8(int m) {
+    constexpr float H = 0.70710678118654752f;
+    switch (m) {
+        case 1: return c32{H, -H};
+        case 2: return c32{0.f, -1.f};
+        case 3: return c32{-H, -H};
+        default: return c32{1.f, 0.f};
     }
 }
 
-// L = 16384 (the largest series): radix-16 Stockham passes with the 16-point DFT in
-// registers, 16384 = 16 * 16 * 16 * 4, so four passes and eight barriers per series
-// instead of seven radix-4 passes.  Thread j of 1024 takes butterfly j of each radix-16
-// pass (16 complex values, 32 VGPRs) and four butterflies of the final radix-4 pass.
-// A pass with stride ns reads z[j + r L/R], multiplies by exp(-2 pi i r k / (R ns)),
-// k = j mod ns, and writes output q to z[(j - k) R + k + q ns].  The first pass writes
-// 16 apart, so LDS index i is stored at i + i/16: consecutive lanes then land on
-// distinct banks.  DFT-16 = 4 x 4: DFT-4 over r1 for each r0, twiddles W16^(r0 q0),
-// DFT-4 over r0 for each q0, output q0 + 4 q1.
-constexpr int F16_L = 16384;
-constexpr int F16_T = 1024;
+template <>
+struct Dft<8> {
+    // n = 2 m + p: E_p = DFT4 over m of a[2m + p]; a[k] = E0[k] + w8^k E1[k],
+    // a[k + 4] = E0[k] - w8^k E1[k]
+    __device__ __forceinline__ static void run(c32* a) {
+        c32 e0[4] = {a[0], a[2], a[4], a[6]};
+        c32 e1[4] = {a[1], a[3], a[5], a[7]};
+        Dft<4>::run(e0);
+        Dft<4>::run(e1);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const c32 t = k ? cmul(w8(k), e1[k]) : e1[k];
+            a[k] = cadd(e0[k], t);
+            a[k + 4] = csub(e0[k], t);
+        }
+    }
+};
 
-__device__ __forceinline__ int fpad(int i) { return i + (i >> 4); }
-
-__device__ __forceinline__ void dft4(c32& x0, c32& x1, c32& x2, c32& x3) {
-    const c32 s02 = cadd(x0, x2), d02 = csub(x0, x2);
-    const c32 s13 = cadd(x1, x3), d13 = cmuli_neg(csub(x1, x3));
-    x0 = cadd(s02, s13);
-    x1 = cadd(d02, d13);
-    x2 = csub(s02, s13);
-    x3 = csub(d02, d13);
-}
-
-// exp(-2 pi i m / 16) for m = r0 q0 in {0, 1, 2, 3, 4, 6, 9}
+// exp(-2 pi i m / 16) for m = p k in {0, 1, 2, 3, 4, 6, 9}
 __device__ __forceinline__ c32 w16(int m) {
     constexpr float C1 = 0.92387953251128674f, S1 = 0.38268343236508978f;
     constexpr float H = 0.70710678118654752f;
@@ -296,116 +135,309 @@ __device__ __forceinline__ c32 w16(int m) {
     }
 }
 
-// in place, transposed: afterwards a[4 q0 + q1] = sum_r a[r] exp(-2 pi i r q / 16) for
-// q = q0 + 4 q1 (the caller stores slot 4 q0 + q1 as output q0 + 4 q1)
-__device__ __forceinline__ void dft16(c32 (&a)[16]) {
+template <>
+struct Dft<16> {
+    // n = 4 m + p: E_p = DFT4 over m of a[4m + p]; a[k + 4q] = DFT4 over p of
+    // (w16^(p k) E_p[k]), output q
+    __device__ __forceinline__ static void run(c32* a) {
+        c32 e[4][4];
 #pragma unroll
-    for (int r0 = 0; r0 < 4; ++r0) dft4(a[r0], a[4 + r0], a[8 + r0], a[12 + r0]);
-    // now a[4 q0 + r0] holds b_{r0}[q0]
+        for (int p = 0; p < 4; ++p) {
 #pragma unroll
-    for (int r0 = 1; r0 < 4; ++r0)
+            for (int m = 0; m < 4; ++m) e[p][m] = a[4 * m + p];
+            Dft<4>::run(e[p]);
+        }
 #pragma unroll
-        for (int q0 = 1; q0 < 4; ++q0) a[4 * q0 + r0] = cmul(a[4 * q0 + r0], w16(r0 * q0));
+        for (int k = 0; k < 4; ++k) {
+            c32 v[4];
 #pragma unroll
-    for (int q0 = 0; q0 < 4; ++q0) dft4(a[4 * q0], a[4 * q0 + 1], a[4 * q0 + 2], a[4 * q0 + 3]);
+            for (int p = 0; p < 4; ++p) v[p] = (p && k) ? cmul(w16(p * k), e[p][k]) : e[p][k];
+            Dft<4>::run(v);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) a[k + 4 * q] = v[q];
+        }
+    }
+};
+
+// LDS index of point i: the low 4 bits are xor-ed with the next 4, so the stride-16
+// accesses of a pass hit 16 different 8-byte slots and the array needs no padding.
+__device__ __forceinline__ int sw(int i) { return i ^ ((i >> 4) & 15); }
+
+// ----------------------------------------------------------- series layout
+// Raw load of complex point i of a series: elements 2i and 2i + 1 (one 16-bit word for
+// 8-bit frames, one 32-bit word for 16-bit frames), zero past the row.
+template <typename E>
+struct Elem;
+template <>
+struct Elem<uint8_t> {
+    static constexpr int SIZE = 1;
+    __device__ __forceinline__ static uint32_t load(const uint8_t* row, int i, int nrows) {
+        return 2 * i < nrows ? (uint32_t) reinterpret_cast<const uint16_t*>(row)[i] : 0u;
+    }
+    __device__ __forceinline__ static c32 value(uint32_t r) {
+        return c32{(float)(r & 0xFFu), (float)(r >> 8)};
+    }
+};
+template <>
+struct Elem<int8_t> {
+    static constexpr int SIZE = 1;
+    __device__ __forceinline__ static uint32_t load(const uint8_t* row, int i, int nrows) {
+        return 2 * i < nrows ? (uint32_t) reinterpret_cast<const uint16_t*>(row)[i] : 0u;
+    }
+    __device__ __forceinline__ static c32 value(uint32_t r) {
+        return c32{(float)(int8_t)(r & 0xFFu), (float)(int8_t)((r >> 8) & 0xFFu)};
+    }
+};
+template <>
+struct Elem<uint16_t> {
+    static constexpr int SIZE = 2;
+    __device__ __forceinline__ static uint32_t load(const uint8_t* row, int i, int nrows) {
+        return 2 * i < nrows ? reinterpret_cast<const uint32_t*>(row)[i] : 0u;
+    }
+    __device__ __forceinline__ static c32 value(uint32_t r) {
+        return c32{(float)(r & 0xFFFFu), (float)(r >> 16)};
+    }
+};
+
+// --------------------------------------------------------------- pixel-major gather
+// grid (bin k-blocks, frame tiles of 256).  Lane = pixel j of the k-block; each wave
+// stages 64 frames, the LDS tile is [64 pixels][256 frames + pad].
+template <typename E>
+__global__ __launch_bounds__(256) void gather_pm_kernel(const uint8_t* __restrict__ frames,
+                                                        int64_t hw, int nframes, int nrows,
+                                                        const int32_t* __restrict__ pix_idx,
+                                                        const BinDev* __restrict__ bins,
+                                                        const KbWork* __restrict__ work,
+                                                        uint8_t* __restrict__ xs) {
+    constexpr int ES = sizeof(E);
+    __shared__ E tile[64][256 + 16 / ES];
+    const KbWork w = work[blockIdx.x];
+    const BinDev bd = bins[w.bin];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int t0 = blockIdx.y * 256;
+    const int p = pix_idx[bd.idx_off + (int64_t)w.kb * KB + lane];
+    const int pc = p >= 0 ? p : 0;
+    const E* fr = reinterpret_cast<const E*>(frames);
+    for (int f = wave; f < 256; f += 4) {
+        const int t = t0 + f;
+        tile[lane][f] = (t < nframes && p >= 0) ? fr[(int64_t)t * hw + pc] : (E)0;
+    }
+    __syncthreads();
+    // pixel row j gets 256 * ES contiguous bytes, written 16 bytes per thread
+    constexpr int CHUNKS = 16 * ES;            // 16-byte chunks per row
+    for (int c = threadIdx.x; c < 64 * CHUNKS; c += 256) {
+        const int j = c / CHUNKS;
+        const int o = (c % CHUNKS) * (16 / ES);    // first element of the chunk
+        uint4 v;
+        uint8_t* vb = reinterpret_cast<uint8_t*>(&v);
+#pragma unroll
+        for (int q = 0; q < 16 / ES; ++q) {
+            const E e = tile[j][o + q];
+            __builtin_memcpy(vb + q * ES, &e, ES);
+        }
+        uint8_t* dst = xs + bd.slab_off + (((int64_t)w.kb * KB + j) * nrows + t0 + o) * ES;
+        *reinterpret_cast<uint4*>(dst) = v;
+    }
 }
 
-// one radix-16 pass of stride NS (1, 16 or 256) for butterfly j (the thread index)
-template <int NS>
-__device__ __forceinline__ void pass16(c32* z, const float2* __restrict__ tw, int j) {
-    const int k = j & (NS - 1);
-    c32 a[16];
+// S_b(t) = sum over the bin's pixels of xs[k][t].
+// grid (frame tiles of 256, bins, pixel chunks of PM_CH).  Thread (r, g) = (tid / 16,
+// tid % 16) adds frames 16 g .. 16 g + 15 of pixel rows chunk + r, r + 16, ... in 32-bit
+// lanes (at most PM_CH / 16 rows of |value| <= 65535: below 2^31); the 16 row groups
+// are reduced in LDS and each frame's chunk total goes to sums (zeroed first) with one
+// 64-bit atomic add.
+constexpr int PM_CH = 1024;
+
+template <typename E>
+__global__ __launch_bounds__(256) void pm_sums_kernel(const uint8_t* __restrict__ xs,
+                                                      const BinDev* __restrict__ bins,
+                                                      int nframes, int nrows,
+                                                      int64_t* __restrict__ sums) {
+    __shared__ int32_t part[16][256 + 1];
+    const int b = blockIdx.y;
+    const BinDev bd = bins[b];
+    const int k0 = blockIdx.z * PM_CH;
+    if (k0 >= bd.npix) return;
+    const int k1 = min(bd.npix, k0 + PM_CH);
+    const int t0 = blockIdx.x * 256;
+    const int r = threadIdx.x >> 4, g = threadIdx.x & 15;
+    int32_t acc[16];
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
-        c32 v = z[fpad(j + r * (F16_L / 16))];
-        if (NS > 1 && r) {
-            const float2 t = tw[(r * k * (F16_L / (16 * NS))) & (F16_L - 1)];
-            v = cmul(v, c32{t.x, t.y});
-        }
-        a[r] = v;
+    for (int q = 0; q < 16; ++q) acc[q] = 0;
+    const E* base = reinterpret_cast<const E*>(xs + bd.slab_off) + t0 + 16 * g;
+    for (int k = k0 + r; k < k1; k += 16) {
+        const E* row = base + (int64_t)k * nrows;
+        E v[16];
+        __builtin_memcpy(v, row, sizeof(v));    // 16 or 32 aligned bytes
+#pragma unroll
+        for (int q = 0; q < 16; ++q) acc[q] += (int32_t)v[q];
     }
-    dft16(a);
-    __syncthreads();  // every read of this pass is done
-    const int o = (j - k) * 16 + k;
 #pragma unroll
-    for (int q0 = 0; q0 < 4; ++q0)
+    for (int q = 0; q < 16; ++q) part[r][16 * g + q] = acc[q];
+    __syncthreads();
+    const int t = t0 + threadIdx.x;
+    if (t < nframes) {
+        int64_t tot = 0;
 #pragma unroll
-        for (int q1 = 0; q1 < 4; ++q1) z[fpad(o + (q0 + 4 * q1) * NS)] = a[4 * q0 + q1];
+        for (int i = 0; i < 16; ++i) tot += part[i][threadIdx.x];
+        if (tot) atomicAdd(reinterpret_cast<unsigned long long*>(sums + (int64_t)b * nframes + t),
+                           (unsigned long long)tot);
+    }
+}
+
+// ------------------------------------------------------------------- spectra
+// A plan: L complex points, T threads, radices R1..R4 (R4 = 1 when there are three
+// passes).  The last pass's twiddles are held in registers, the middle passes' in LDS
+// tables (pass p's table holds exp(-2 pi i m / P_p), m < P_p = R1 ... R_p).
+template <int L_, int T_, int R1_, int R2_, int R3_, int R4_>
+struct FftPlan {
+    static constexpr int L = L_, T = T_, R1 = R1_, R2 = R2_, R3 = R3_, R4 = R4_;
+    static constexpr int NP = R4 > 1 ? 4 : 3;
+    static constexpr int RL = NP == 4 ? R4 : R3;        // radix of the last pass
+    static constexpr int NSL = L / RL;                   // its stride
+    static constexpr int P2 = R1 * R2;                   // table of pass 2
+    static constexpr int P3 = NP == 4 ? R1 * R2 * R3 : 1;  // table of pass 3 (4 passes)
+    static_assert(R1 * R2 * R3 * R4 == L, "radices must multiply to L");
+};
+
+template <typename Plan, int R, int NS, int PT>
+__device__ __forceinline__ void mid_pass(c32* z, const c32* tab, int tid) {
+    constexpr int L = Plan::L, T = Plan::T;
+    constexpr int NB = L / R;
+    constexpr int M = (NB + T - 1) / T;
+    c32 v[M][R];
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+        const int b = tid + m * T;
+        if (NB % T == 0 || b < NB) {
+            const int k = b % NS;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                c32 a = z[sw(b + r * NB)];
+                if (r) a = cmul(a, tab[r * k]);      // exp(-2 pi i r k / (R NS)), PT = R NS
+                v[m][r] = a;
+            }
+            Dft<R>::run(v[m]);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+        const int b = tid + m * T;
+        if (NB % T == 0 || b < NB) {
+            const int k = b % NS;
+            const int o = (b - k) * R + k;
+#pragma unroll
+            for (int q = 0; q < R; ++q) z[sw(o + q * NS)] = v[m][q];
+        }
+    }
     __syncthreads();
 }
 
-__global__ __launch_bounds__(F16_T) void fft16_power_kernel(
+template <typename E, typename Plan>
+__global__ __launch_bounds__(Plan::T) void fft_spectra_kernel(
     const uint8_t* __restrict__ xs, const BinDev* __restrict__ bins,
-    const KbWork* __restrict__ work, int nframes, int nrows, const float2* __restrict__ tw,
-    double* __restrict__ power) {
-    __shared__ c32 z[F16_L + F16_L / 16];
-    constexpr int L = F16_L;
-    constexpr int NK = (L + 1 + F16_T - 1) / F16_T;  // power bins per thread
+    const KbWork* __restrict__ work, int nrows, const float2* __restrict__ tw,
+    const float2* __restrict__ uw, double* __restrict__ power) {
+    constexpr int L = Plan::L, T = Plan::T;
+    constexpr int R1 = Plan::R1, R2 = Plan::R2, R3 = Plan::R3, RL = Plan::RL;
+    constexpr int NB1 = L / R1, M1 = (NB1 + T - 1) / T;       // first pass
+    constexpr int NBL = L / RL, ML = (NBL + T - 1) / T;       // last pass
+    constexpr int MU = (L + 1 + T - 1) / T;                   // unpacked points
+    __shared__ c32 z[L];
+    __shared__ c32 tab2[Plan::P2];
+    __shared__ c32 tab3[Plan::P3];
     const KbWork w = work[blockIdx.x];
     const BinDev bd = bins[w.bin];
     const int tid = threadIdx.x;
-    double acc[NK];
+
+    for (int m = tid; m < Plan::P2; m += T) tab2[m] = c_of(tw[m * (L / Plan::P2)]);
+    if (Plan::NP == 4)
+        for (int m = tid; m < Plan::P3; m += T) tab3[m] = c_of(tw[m * (L / Plan::P3)]);
+    double acc[MU];
 #pragma unroll
-    for (int m = 0; m < NK; ++m) acc[m] = 0.0;
+    for (int m = 0; m < MU; ++m) acc[m] = 0.0;
 
     const int kend = min(KB, bd.npix - w.kb * KB);
-    for (int jp = 0; jp < kend; ++jp) {
-        const uint8_t* x = xs + bd.slab_off + ((int64_t)w.kb * KB + jp) * nrows;
-        // A thread's addresses, twiddles and unpack factors are the same for every
-        // series; an opaque copy of the thread index per series keeps the compiler from
-        // hoisting them all out of this loop into registers (which spilled to scratch):
-        // they are recomputed, and the twiddles re-read from L1/L2, instead.
-        int tj = tid;
-        asm volatile("" : "+v"(tj));
-        // z[n] = x[2n] + i x[2n+1] (zero past the last frame)
+    const uint8_t* rows = xs + bd.slab_off + (int64_t)w.kb * KB * nrows * Elem<E>::SIZE;
+    const int64_t row_bytes = (int64_t)nrows * Elem<E>::SIZE;
+    uint32_t raw[M1][R1];
 #pragma unroll
-        for (int m = 0; m < L / F16_T; ++m) {
-            const int n = tj + m * F16_T;
-            const int t = 2 * n;
-            float re = 0.f, im = 0.f;
-            if (t + 1 < nrows) {
-                const uint16_t two = *reinterpret_cast<const uint16_t*>(x + t);
-                re = t < nframes ? (float)(two & 0xFF) : 0.f;
-                im = t + 1 < nframes ? (float)(two >> 8) : 0.f;
+    for (int m = 0; m < M1; ++m)
+#pragma unroll
+        for (int r = 0; r < R1; ++r) {
+            const int b = tid + m * T;
+            raw[m][r] = (NB1 % T == 0 || b < NB1) ? Elem<E>::load(rows, b + r * NB1, nrows) : 0u;
+        }
+    __syncthreads();   // tables
+
+    for (int j = 0; j < kend; ++j) {
+        // Every address, twiddle and factor below depends on the thread only; an opaque
+        // copy of the thread index per series keeps the compiler from holding them all
+        // in registers across the series loop (which spills): they are recomputed and
+        // the last pass's twiddles and the unpacking factors re-read from L1/L2.  By
+        // then the next series' loads issued after the first pass have landed.
+        int t = tid;
+        asm volatile("" : "+v"(t));
+        // first pass (stride 1, no twiddles) from the registers loaded during the
+        // previous series: z[b + r NB1] -> outputs q at b R1 + q
+#pragma unroll
+        for (int m = 0; m < M1; ++m) {
+            const int b = t + m * T;
+            if (NB1 % T == 0 || b < NB1) {
+                c32 v[R1];
+#pragma unroll
+                for (int r = 0; r < R1; ++r) v[r] = Elem<E>::value(raw[m][r]);
+                Dft<R1>::run(v);
+#pragma unroll
+                for (int q = 0; q < R1; ++q) z[sw(b * R1 + q)] = v[q];
             }
-            z[fpad(n)] = c32{re, im};
+        }
+        // the next series' inputs, in flight during this series' passes
+        if (j + 1 < kend) {
+            const uint8_t* nxt = rows + (int64_t)(j + 1) * row_bytes;
+#pragma unroll
+            for (int m = 0; m < M1; ++m)
+#pragma unroll
+                for (int r = 0; r < R1; ++r) {
+                    const int b = t + m * T;
+                    if (NB1 % T == 0 || b < NB1)
+                        raw[m][r] = Elem<E>::load(nxt, b + r * NB1, nrows);
+                }
         }
         __syncthreads();
-        pass16<1>(z, tw, tj);
-        pass16<16>(z, tw, tj);
-        pass16<256>(z, tw, tj);
-        // radix-4 pass of stride 4096: butterfly jj reads and writes z[jj + 4096 r] (jj <
-        // 4096), so each thread works in place on its own four points
+        mid_pass<Plan, R2, R1, Plan::P2>(z, tab2, t);
+        if (Plan::NP == 4) mid_pass<Plan, R3, R1 * R2, Plan::P3>(z, tab3, t);
+        // last pass, in place: butterfly b reads and writes z[b + r NSL]; its twiddles
+        // are exp(-2 pi i r b / L) (k = b < NSL)
 #pragma unroll
-        for (int m = 0; m < 4; ++m) {
-            const int jj = tj + m * F16_T;
-            c32 v[4];
+        for (int m = 0; m < ML; ++m) {
+            const int b = t + m * T;
+            if (NBL % T == 0 || b < NBL) {
+                c32 v[RL];
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                c32 a = z[fpad(jj + r * (L / 4))];
-                if (r) {
-                    const float2 t = tw[(r * jj) & (L - 1)];
-                    a = cmul(a, c32{t.x, t.y});
+                for (int r = 0; r < RL; ++r) {
+                    const c32 a = z[sw(b + r * NBL)];
+                    v[r] = r ? cmul(a, c_of(tw[r * b])) : a;
                 }
-                v[r] = a;
-            }
-            dft4(v[0], v[1], v[2], v[3]);
+                Dft<RL>::run(v);
 #pragma unroll
-            for (int r = 0; r < 4; ++r) z[fpad(jj + r * (L / 4))] = v[r];
+                for (int q = 0; q < RL; ++q) z[sw(b + q * NBL)] = v[q];
+            }
         }
         __syncthreads();
         // unpack X[k] = (Z[k] + conj Z[L-k]) / 2 - i W^k (Z[k] - conj Z[L-k]) / 2,
         // W = exp(-pi i / L); accumulate |X[k]|^2
 #pragma unroll
-        for (int m = 0; m < NK; ++m) {
-            const int k = tj + m * F16_T;
+        for (int m = 0; m < MU; ++m) {
+            const int k = t + m * T;
             if (k <= L) {
-                const c32 a = z[fpad(k & (L - 1))];
-                const c32 b = z[fpad((L - k) & (L - 1))];
-                const c32 s2 = c32{a.x + b.x, a.y - b.y};
-                const c32 d = c32{a.x - b.x, a.y + b.y};
-                const float2 u = tw[L + k];  // W^k = exp(-pi i k / L)
-                const c32 wd = cmul(c32{u.x, u.y}, d);
+                const c32 a = z[sw(k == L ? 0 : k)];
+                const c32 c = z[sw(k == 0 ? 0 : L - k)];
+                const c32 s2 = c32{a.x + c.x, a.y - c.y};
+                const c32 d = c32{a.x - c.x, a.y + c.y};
+                const c32 wd = cmul(c_of(uw[k]), d);
                 const float xr = 0.5f * (s2.x + wd.y);
                 const float xi = 0.5f * (s2.y - wd.x);
                 acc[m] += (double)xr * xr + (double)xi * xi;
@@ -415,8 +447,8 @@ __global__ __launch_bounds__(F16_T) void fft16_power_kernel(
     }
     double* prow = power + (int64_t)w.bin * (L + 1);
 #pragma unroll
-    for (int m = 0; m < NK; ++m) {
-        const int k = tid + m * F16_T;
+    for (int m = 0; m < MU; ++m) {
+        const int k = tid + m * T;
         if (k <= L && acc[m] != 0.0) atomicAdd(prow + k, acc[m]);
     }
 }
@@ -448,41 +480,96 @@ __global__ __launch_bounds__(256) void fft_inverse_kernel(const double* __restri
     num[(int64_t)b * nframes + tau] = (ps[0] + last + 2.0 * acc) / nf;
 }
 
+// ------------------------------------------------------------------ launchers
+template <typename E>
+static void gather_pm_typed(const uint8_t* frames, int64_t hw, int nframes, int nrows,
+                            const int32_t* pix_idx, const BinDev* bins, const KbWork* work,
+                            int n_work, uint8_t* xs, hipStream_t s) {
+    gather_pm_kernel<E><<<dim3((unsigned)n_work, (unsigned)(nrows / 256)), 256, 0, s>>>(
+        frames, hw, nframes, nrows, pix_idx, bins, work, xs);
+}
+
 void launch_gather_pm(const uint8_t* frames, int dtype, int64_t hw, int nframes, int nrows,
                       const int32_t* pix_idx, const BinDev* bins, const KbWork* work, int n_work,
                       uint8_t* xs, hipStream_t s) {
     if (n_work <= 0) return;
-    gather_pm_kernel<<<dim3((unsigned)n_work, (unsigned)(nrows / 256)), 256, 0, s>>>(
-        frames, dtype, hw, nframes, nrows, pix_idx, bins, work, xs);
+    if (dtype == DXG2_U16)
+        gather_pm_typed<uint16_t>(frames, hw, nframes, nrows, pix_idx, bins, work, n_work, xs, s);
+    else
+        gather_pm_typed<uint8_t>(frames, hw, nframes, nrows, pix_idx, bins, work, n_work, xs, s);
 }
 
-void launch_pm_sums(const uint8_t* xs, const BinDev* bins, int n_bins, int max_npix, int nframes,
-                    int nrows, int64_t* sums, hipStream_t s) {
+void launch_pm_sums(const uint8_t* xs, int dtype, const BinDev* bins, int n_bins, int max_npix,
+                    int nframes, int nrows, int64_t* sums, hipStream_t s) {
     if (n_bins <= 0 || max_npix <= 0) return;
     launch_fill_i64(sums, (int64_t)n_bins * nframes, s);
-    pm_sums_kernel<<<dim3((unsigned)((nframes + 255) / 256), (unsigned)n_bins,
-                          (unsigned)((max_npix + PM_CH - 1) / PM_CH)),
-                     256, 0, s>>>(xs, bins, nframes, nrows, sums);
+    const dim3 grid((unsigned)((nframes + 255) / 256), (unsigned)n_bins,
+                    (unsigned)((max_npix + PM_CH - 1) / PM_CH));
+    if (dtype == DXG2_U16)
+        pm_sums_kernel<uint16_t><<<grid, 256, 0, s>>>(xs, bins, nframes, nrows, sums);
+    else if (dtype == DXG2_I8)
+        pm_sums_kernel<int8_t><<<grid, 256, 0, s>>>(xs, bins, nframes, nrows, sums);
+    else
+        pm_sums_kernel<uint8_t><<<grid, 256, 0, s>>>(xs, bins, nframes, nrows, sums);
 }
 
-int launch_fft_power(int L, const uint8_t* xs, const BinDev* bins, const KbWork* work,
-                     int n_work, int nframes, int nrows, const float2* tw, double* power,
+// Lengths with a plan: L (complex points), threads, radices.
+using Plan1024 = FftPlan<1024, 64, 16, 16, 4, 1>;
+using Plan1280 = FftPlan<1280, 128, 16, 16, 5, 1>;
+using Plan2048 = FftPlan<2048, 128, 16, 16, 8, 1>;
+using Plan2560 = FftPlan<2560, 256, 16, 4, 8, 5>;
+using Plan4096 = FftPlan<4096, 256, 16, 16, 16, 1>;
+using Plan5120 = FftPlan<5120, 320, 16, 16, 5, 4>;
+using Plan8192 = FftPlan<8192, 512, 16, 16, 8, 4>;
+using Plan10240 = FftPlan<10240, 640, 16, 16, 5, 8>;
+using Plan16384 = FftPlan<16384, 1024, 16, 16, 4, 16>;
+
+static const int kFftLengths[] = {1024, 1280, 2048, 2560, 4096, 5120, 8192, 10240, 16384};
+
+int fft_lds_length(int n) {
+    for (int L : kFftLengths)
+        if (L >= n) return L;
+    return 0;
+}
+
+template <typename E, typename Plan>
+static void spectra_typed(const uint8_t* xs, const BinDev* bins, const KbWork* work, int n_work,
+                          int nrows, const float2* tw, const float2* uw, double* power,
+                          hipStream_t s) {
+    fft_spectra_kernel<E, Plan><<<dim3((unsigned)n_work), Plan::T, 0, s>>>(xs, bins, work, nrows,
+                                                                          tw, uw, power);
+}
+
+template <typename Plan>
+static void spectra_dtype(int dtype, const uint8_t* xs, const BinDev* bins, const KbWork* work,
+                          int n_work, int nrows, const float2* tw, const float2* uw,
+                          double* power, hipStream_t s) {
+    if (dtype == DXG2_U16)
+        spectra_typed<uint16_t, Plan>(xs, bins, work, n_work, nrows, tw, uw, power, s);
+    else if (dtype == DXG2_I8)
+        spectra_typed<int8_t, Plan>(xs, bins, work, n_work, nrows, tw, uw, power, s);
+    else
+        spectra_typed<uint8_t, Plan>(xs, bins, work, n_work, nrows, tw, uw, power, s);
+}
+
+int launch_fft_power(int L, int dtype, const uint8_t* xs, const BinDev* bins, const KbWork* work,
+                     int n_work, int nrows, const float2* tw, const float2* uw, double* power,
                      hipStream_t s) {
     if (n_work <= 0) return 0;
-#define DXG2_FFT_CASE(LL)                                                                     \
-    case LL:                                                                                  \
-        fft_power_kernel<LL><<<dim3((unsigned)n_work), FftCfg<LL>::THREADS, 0, s>>>(          \
-            xs, bins, work, nframes, nrows, tw, power);                                       \
+#define DXG2_FFT_CASE(LL)                                                                  \
+    case LL:                                                                               \
+        spectra_dtype<Plan##LL>(dtype, xs, bins, work, n_work, nrows, tw, uw, power, s);   \
         return 0;
     switch (L) {
         DXG2_FFT_CASE(1024)
+        DXG2_FFT_CASE(1280)
         DXG2_FFT_CASE(2048)
+        DXG2_FFT_CASE(2560)
         DXG2_FFT_CASE(4096)
+        DXG2_FFT_CASE(5120)
         DXG2_FFT_CASE(8192)
-        case F16_L:
-            fft16_power_kernel<<<dim3((unsigned)n_work), F16_T, 0, s>>>(xs, bins, work, nframes,
-                                                                       nrows, tw, power);
-            return 0;
+        DXG2_FFT_CASE(10240)
+        DXG2_FFT_CASE(16384)
         default:
             return -1;
     }

@@ -63,11 +63,11 @@ def test_serving_job_stream_is_seeded_and_in_range():
     from dynamix_amd.serving import make_jobs, SERVE_SHAPES
     a = make_jobs(300, rate=20.0, seed=9)
     b = make_jobs(300, rate=20.0, seed=9)
-    assert [(x.shape, x.nframes, x.n_bins, x.layout, x.tau_c, x.frame0, x.arrival) for x in a] == \
-           [(x.shape, x.nframes, x.n_bins, x.layout, x.tau_c, x.frame0, x.arrival) for x in b]
+    assert [(x.shape, x.nframes, x.n_bins, x.layout, x.tau_c, x.seed, x.arrival) for x in a] == \
+           [(x.shape, x.nframes, x.n_bins, x.layout, x.tau_c, x.seed, x.arrival) for x in b]
     assert all(x.shape in SERVE_SHAPES for x in a)
     assert all(1000 <= x.nframes <= 20000 and 10 <= x.n_bins <= 100 for x in a)
-    assert all(0 <= x.frame0 <= 20000 - x.nframes for x in a)
+    assert len({x.seed for x in a}) == len(a)
     assert all(10.0 <= x.tau_c <= 500.0 for x in a)
     dense = sum(x.layout == "dense" for x in a) / len(a)
     assert 0.6 < dense < 0.8

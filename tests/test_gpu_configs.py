@@ -314,86 +314,21 @@ def test_cfg4_events_full_n100000_properties():
 
 
 # ------------------------------------------------------------------------- cfg5
-SERVE_POOL_FRAMES = 2000
-
-
-def _window_events(pool, spec):
-    t, d, o = pool.events[spec.shape]
-    tn, dn = t.cpu().numpy(), d.cpu().numpy()
-    on = o.cpu().numpy().view(np.uint32).astype(np.int64)
-    hw = spec.shape[0] * spec.shape[1]
-    pix = np.repeat(np.arange(hw), np.diff(on))
-    keep = (tn >= spec.frame0) & (tn < spec.frame0 + spec.nframes)
-    off = np.zeros(hw + 1, np.uint32)
-    off[1:] = np.cumsum(np.bincount(pix[keep], minlength=hw))
-    return tn[keep] - spec.frame0, dn[keep], off
-
-
-def _alone(pool, spec):
-    """The job by itself through the single-plan entry points."""
-    import torch
-    from dynamix_amd import _lib
-    from dynamix_amd.correlator.common import Plan
-    from dynamix_amd.serving import job_qmask
-    q = job_qmask(spec)
-    plan = Plan(spec.shape, spec.nframes, q, np.uint8)
-    nb, n = plan.n_bins, spec.nframes
-    num = torch.empty((nb, n), dtype=torch.int64, device="cuda")
-    sums = torch.empty_like(num)
-    den = torch.empty((nb, n), dtype=torch.float64, device="cuda")
-    g2 = torch.empty((nb, n), dtype=torch.float32, device="cuda")
-    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
-    outs = [_lib.ptr(x) for x in (num, sums, den, g2)]
-    if spec.layout == "dense":
-        f = pool.dense[spec.shape][spec.frame0:spec.frame0 + n]
-        _lib.check(plan.lib.dxg2_correlate_dense(plan.handle, _lib.ptr(f), *outs, st))
-    else:
-        tw, dw, ow = _window_events(pool, spec)
-        tt = torch.from_numpy(tw.astype(np.int32)).cuda()
-        dd = torch.from_numpy(dw).cuda()
-        oo = torch.from_numpy(ow.view(np.int32)).cuda()
-        _lib.check(plan.lib.dxg2_correlate_events_csr(plan.handle, _lib.ptr(tt), _lib.ptr(dd),
-                                                      _lib.ptr(oo), int(tw.size), *outs, st))
-    torch.cuda.synchronize()
-    res = {"num": num.cpu().numpy(), "sums": sums.cpu().numpy(), "g2": g2.cpu().numpy()}
-    plan.close()
-    return res, q
-
-
 def test_cfg5_serving_real_shapes_mix_equals_alone():
-    from dynamix_amd.serving import SERVE_SHAPES, InputPool, Server, make_jobs
+    """The serving workload's shapes, frame counts, bin counts, layouts and per-job
+    correlation times (1k-2k frames so the oracle stays fast): every job in the mix
+    equals the job run alone; the smallest dense and event jobs equal the oracle."""
+    from test_gpu_serving import check_mix_equals_alone
+    from dynamix_amd.serving import SERVE_SHAPES, Server, make_jobs
     _announce("cfg5 serving real shapes")
-    pool = InputPool(SERVE_SHAPES, SERVE_POOL_FRAMES, seed=55, dense_mean=0.02, event_mean=1e-3,
-                     tau_c=200.0)
-    specs = make_jobs(10, rate=200.0, seed=505, nframes=(1000, SERVE_POOL_FRAMES),
-                      bins=(10, 100), dense_fraction=0.7, pool_frames=SERVE_POOL_FRAMES)
+    specs = make_jobs(10, rate=200.0, seed=505, nframes=(1000, 2000), bins=(10, 100),
+                      dense_fraction=0.7)
     assert {s.layout for s in specs} == {"dense", "events"}
     assert {tuple(s.shape) for s in specs} == set(SERVE_SHAPES)
-    srv = Server(pool, max_batch=4, keep_results=True)
+    srv = Server(max_batch=4, keep_results=True)
     try:
         stats = srv.run(specs)
     finally:
         srv.close()
-    done = stats["records"]
-    assert stats["jobs"] == len(specs) and not stats.get("failed")
-    checked = set()
-    for j in sorted(done, key=lambda j: j.npix * j.spec.nframes):
-        alone, q = _alone(pool, j.spec)
-        np.testing.assert_array_equal(j.result["num"], alone["num"])
-        np.testing.assert_array_equal(j.result["sums"], alone["sums"])
-        np.testing.assert_array_equal(j.result["g2"].view(np.uint32), alone["g2"].view(np.uint32))
-        if j.spec.layout in checked:
-            continue
-        checked.add(j.spec.layout)      # the smallest job of each layout vs the oracle
-        _, bins = og.qmask_bins(q.shape, q)
-        if j.spec.layout == "dense":
-            f = pool.dense[j.spec.shape][j.spec.frame0:j.spec.frame0 + j.spec.nframes]
-            s, num = og.sums_numerators_fast(f.cpu().numpy(), q, bins)
-        else:
-            tw, dw, ow = _window_events(pool, j.spec)
-            s, num = ev.event_sums_numerators_fast(tw, dw, ow, q, j.spec.nframes, bins)
-        np.testing.assert_array_equal(alone["num"], num)
-        np.testing.assert_array_equal(alone["sums"], s)
-        ref = og.g2_fp64(num, og.denominators(s), og.scale_factors(q, bins))
-        assert_g2_close(alone["g2"], ref, G2_RTOL)
-    assert checked == {"dense", "events"}
+    assert stats["jobs"] == len(specs) and stats["failed"] == 0, stats["failures"]
+    assert check_mix_equals_alone(stats["records"]) == {"dense", "events"}
