@@ -232,21 +232,29 @@ class PlainMatMulCorrelator(_DenseHIP):
 FFT_HALF_LENGTHS = (1024, 1280, 2048, 2560, 4096, 5120, 8192, 10240, 16384)
 
 
+# Long series (N > 16384): L = L1 L2, a row length L1 and a column count L2.
+FFT_ROW_LENGTHS = (8192, 5120, 4096, 2560, 2048, 1280, 1024)
+FFT_COL_LENGTHS = (2, 4, 5, 8, 16)
+
+
 def fft_length(nframes):
-    """Nf = 2 L for the smallest L of FFT_HALF_LENGTHS with L >= nframes: Nf >= 2 N - 1
+    """Nf = 2 L for the smallest L of FFT_HALF_LENGTHS with L >= nframes, or above 16384
+    the smallest product L1 L2 >= nframes of FFT_ROW_LENGTHS x FFT_COL_LENGTHS: Nf >= 2 N - 1
     is all the linear autocorrelation needs (the reference uses get_next_power(2N),
-    utils.py:50-60, another smooth length of the same kind).  0 when N > 16384."""
+    utils.py:50-60, another smooth length of the same kind).  0 when N > 131072."""
     for L in FFT_HALF_LENGTHS:
         if L >= nframes:
             return 2 * L
-    return 0
+    long = [a * b for a in FFT_ROW_LENGTHS for b in FFT_COL_LENGTHS if a * b >= nframes]
+    return 2 * min(long) if long else 0
 
 
 class FFTCorrelator(_DenseHIP):
     """Drop-in for FFTWCorrelator / CUFFTCorrelator (dense.py:275-369, cuda.py:129-219).
 
     Frames of any dtype the exact paths take (uint8, int8, uint16, or wider types
-    narrowed to them), nframes <= 16384.  `weights` and `precompute_fft_plans` are accepted;
+    narrowed to them), nframes <= 131072 (above 16384 the transform is a four-step FFT
+    through an HBM scratch of at most DXG2_FFT_SCRATCH_MB MiB).  `weights` and `precompute_fft_plans` are accepted;
     weights are ignored as in the reference FFT correlators (dense.py:277-285), and the
     FFT needs no plan.  scale_factor is ignored (the reference divides by the pixel count,
     dense.py:366).  Numerators carry FP32 FFT rounding: |Num - Num_exact| <= 2e-7 * Num(0)

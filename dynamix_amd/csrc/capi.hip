@@ -268,6 +268,9 @@ struct dxg2_plan {
     double* d_power = nullptr;     // (n_bins, L + 1)
     double* d_fnum = nullptr;      // (n_bins, N)
     int fft_L = 0;
+    int fft_L1 = 0, fft_L2 = 0;    // long series: L = L1 L2 through the scratch below
+    int fft_chunk = 0;             // k-blocks per four-step chunk
+    float2* d_fy = nullptr;        // four-step scratch: fft_chunk * 64 * L complex
 
     // Profiling: each correlate call records 4 events on its stream (no host sync);
     // dxg2_plan_stage_times() waits for the last one and averages every recorded call.
@@ -567,20 +570,40 @@ int next_events(dxg2_plan* p, hipEvent_t** out) {
 constexpr int EVENT_ITEM_PIXELS = 1024;
 
 // FFT length: Nf = 2L with L the smallest length of the mixed-radix set (2^a 5^b, at
-// least 1024) with L >= N; any Nf >= 2N - 1 gives the linear autocorrelation.  A series
-// of L complex points is transformed in LDS, so L <= 16384.
-int fft_length(int n) { return fft_lds_length(std::max(n, 1)); }
+// least 1024) with L >= N; any Nf >= 2N - 1 gives the linear autocorrelation.  Up to
+// 16384 a series is transformed in LDS; above, L = L1 L2 goes through the four-step
+// path (L <= 131072).  *L1 / *L2 are 0 for the in-LDS path.
+int fft_length(int n, int* L1 = nullptr, int* L2 = nullptr) {
+    int l1 = 0, l2 = 0;
+    int L = fft_lds_length(std::max(n, 1));
+    if (L == 0) L = fft_four_step_length(n, &l1, &l2);
+    if (L1) *L1 = l1;
+    if (L2) *L2 = l2;
+    return L;
+}
+
+// k-blocks per four-step chunk: the scratch holds 64 series of L complex points per
+// k-block, at most DXG2_FFT_SCRATCH_MB MiB (default 1024).
+int fft_chunk_kblocks(int L, int total_kb) {
+    const size_t cap = (size_t)std::max(1, env_int("DXG2_FFT_SCRATCH_MB", 1024)) << 20;
+    const size_t per = (size_t)KB * L * sizeof(float2);
+    return (int)std::max<size_t>(1, std::min<size_t>((size_t)std::max(total_kb, 1), cap / per));
+}
 
 int ensure_fft_buffers(dxg2_plan* p, hipStream_t s) {
     if (p->d_power) return DXG2_OK;
     p->kb_work.clear();
-    const int L = fft_length(p->N);
+    int L1 = 0, L2 = 0;
+    const int L = fft_length(p->N, &L1, &L2);
     if (L == 0)
-        return set_error(DXG2_EUNSUPPORTED, "FFT path supports up to 16384 frames (got %d)",
+        return set_error(DXG2_EUNSUPPORTED, "FFT path supports up to 131072 frames (got %d)",
                          p->N);
     p->fft_L = L;
+    p->fft_L1 = L1;
+    p->fft_L2 = L2;
     for (int b = 0; b < p->n_bins; ++b)
         for (int kb = 0; kb < p->bins[b].nkb; ++kb) p->kb_work.push_back(KbWork{b, kb});
+    p->fft_chunk = L1 ? fft_chunk_kblocks(L, (int)p->kb_work.size()) : 0;
     // [0, L): exp(-2 pi i m / L), the transform's twiddles; [L, 2L]: exp(-pi i k / L),
     // k = 0..L, the factor that unpacks the half-length complex transform
     p->twiddles.resize(2 * (size_t)L + 1);
@@ -598,7 +621,10 @@ int ensure_fft_buffers(dxg2_plan* p, hipStream_t s) {
     float2* tw = nullptr;
     double *power = nullptr, *fnum = nullptr;
     KbWork* kbw = nullptr;
-    if (dev_alloc(p->alloc, (void**)&xs, p->slab_bytes, s) != hipSuccess ||
+    float2* fy = nullptr;
+    if ((L1 && dev_alloc(p->alloc, (void**)&fy, (size_t)p->fft_chunk * KB * L * sizeof(float2),
+                         s) != hipSuccess) ||
+        dev_alloc(p->alloc, (void**)&xs, p->slab_bytes, s) != hipSuccess ||
         dev_alloc(p->alloc, (void**)&tw, p->twiddles.size() * sizeof(float2), s) != hipSuccess ||
         dev_alloc(p->alloc, (void**)&fnum, (size_t)std::max(p->n_bins, 1) * p->N * sizeof(double), s) !=
             hipSuccess ||
@@ -606,10 +632,11 @@ int ensure_fft_buffers(dxg2_plan* p, hipStream_t s) {
          dev_alloc(p->alloc, (void**)&kbw, p->kb_work.size() * sizeof(KbWork), s) != hipSuccess) ||
         dev_alloc(p->alloc, (void**)&power, (size_t)std::max(p->n_bins, 1) * (L + 1) * sizeof(double), s) !=
             hipSuccess) {
-        for (void* b : {(void*)xs, (void*)tw, (void*)fnum, (void*)kbw, (void*)power})
+        for (void* b : {(void*)xs, (void*)tw, (void*)fnum, (void*)kbw, (void*)power, (void*)fy})
             dev_free(p->alloc, b, s);
         return set_error(DXG2_ENOMEM, "cannot allocate FFT workspace");
     }
+    p->d_fy = fy;
     p->d_xs = xs;
     p->d_tw = tw;
     p->d_fnum = fnum;
@@ -1024,7 +1051,7 @@ static void plan_release(dxg2_plan* p, hipStream_t s) {
                     p->d_pg, p->d_pm,
                     p->d_ework, p->d_err,  p->d_cnt,   p->d_scan,  p->d_off,
                     p->d_etimes, p->d_edata, p->d_kbwork, p->d_xs, p->d_tw,
-                    p->d_power, p->d_fnum, p->d_qmwork, p->d_qxwork, p->d_qoff,
+                    p->d_power, p->d_fnum, p->d_fy, p->d_qmwork, p->d_qxwork, p->d_qoff,
                     p->d_heads, p->d_gband, p->d_gband_kb, p->d_gfall,
                     p->coo_sort.keys, p->coo_sort.vals0, p->coo_sort.vals1, p->coo_sort.tmp,
                     p->d_wflag, p->d_pixw, p->d_nwkb};
@@ -1083,8 +1110,13 @@ int dxg2_plan_footprint(const dxg2_plan* p, int32_t path, size_t* bytes) {
             return DXG2_OK;
         }
         case 2: {
-            const int L = fft_length(p->N);
-            *bytes = common + p->slab_bytes + (2 * (size_t)L + 1) * sizeof(float2) +
+            int L1 = 0, L2 = 0;
+            const int L = fft_length(p->N, &L1, &L2);
+            int64_t nkb = 0;
+            for (const auto& bd : p->bins) nkb += bd.nkb;
+            const size_t scratch =
+                L1 ? (size_t)fft_chunk_kblocks(L, (int)nkb) * KB * L * sizeof(float2) : 0;
+            *bytes = common + p->slab_bytes + (2 * (size_t)L + 1) * sizeof(float2) + scratch +
                      (size_t)std::max(p->n_bins, 1) * ((L + 1) + p->N) * sizeof(double);
             return DXG2_OK;
         }
@@ -1595,8 +1627,8 @@ int dxg2_correlate_events_coo(dxg2_plan* p, const uint32_t* d_pixel, const uint3
     launch_fill_i64(p->d_ugram, nbn, s);
     launch_fill_i32(p->d_err, 1, s);
     if (launch_coo_to_csr(p->dtype, d_pixel, d_frame, d_count, nnz,
-                          (uint32_t)((int64_t)p->H * p->W), p->N, p->d_cnt, p->d_scan, p->d_off,
-                          p->coo_sort, p->d_etimes, p->d_edata, p->d_err, s) != hipSuccess)
+                          (uint32_t)((int64_t)p->H * p->W), p->N, p->d_off, p->coo_sort,
+                          p->d_etimes, p->d_edata, p->d_err, s) != hipSuccess)
         return set_error(DXG2_EHIP, "sort of %lld COO events failed", (long long)nnz);
     return events_from_csr(p, p->d_etimes, p->d_edata, p->d_off, nnz, d_num, d_sums, d_den,
                            d_g2, s);
@@ -1623,10 +1655,20 @@ int dxg2_correlate_fft(dxg2_plan* p, const void* d_frames, double* d_num, int64_
     launch_pm_sums(p->d_xs, p->dtype, p->d_bins, p->n_bins, (int)max_npix, p->N, p->nrows, sums,
                    s);
     if (ev) DXG2_HIP(hipEventRecord(ev[1], s));
-    if (launch_fft_power(p->fft_L, p->dtype, p->d_xs, p->d_bins, p->d_kbwork,
-                         (int)p->kb_work.size(), p->nrows, p->d_tw, p->d_tw + p->fft_L,
-                         p->d_power, s))
+    if (p->fft_L1) {
+        const int total = (int)p->kb_work.size();
+        for (int kb0 = 0; kb0 < total; kb0 += p->fft_chunk)
+            if (launch_fft_four_step(p->fft_L1, p->fft_L2, p->dtype, p->d_xs, p->d_bins,
+                                     p->d_kbwork, kb0, std::min(p->fft_chunk, total - kb0),
+                                     p->nrows, p->d_tw, p->d_tw + p->fft_L, p->d_fy,
+                                     p->d_power, s))
+                return set_error(DXG2_EUNSUPPORTED, "no FFT kernel for length %d x %d",
+                                 p->fft_L1, p->fft_L2);
+    } else if (launch_fft_power(p->fft_L, p->dtype, p->d_xs, p->d_bins, p->d_kbwork,
+                                (int)p->kb_work.size(), p->nrows, p->d_tw, p->d_tw + p->fft_L,
+                                p->d_power, s)) {
         return set_error(DXG2_EUNSUPPORTED, "no FFT kernel for length %d", p->fft_L);
+    }
     launch_fft_inverse(p->d_power, p->fft_L, p->n_bins, p->N, p->d_fnum, s);
     if (ev) DXG2_HIP(hipEventRecord(ev[2], s));
     launch_finalize_f64(sums, p->d_fnum, p->d_bins, p->n_bins, p->N, p->d_prefix, p->d_smax,

@@ -182,9 +182,8 @@ bool coo_narrow_values(int dtype, int nframes);
 size_t coo_sort_bytes(int dtype, int nframes, int64_t cap, uint32_t npix);
 hipError_t launch_coo_to_csr(int dtype, const uint32_t* pix, const uint32_t* frame,
                              const void* cnt, int64_t nnz, uint32_t npix, int nframes,
-                             uint32_t* count, uint32_t* scan_tot, uint32_t* offsets,
-                             const CooSortBufs& sort, int32_t* times, void* data, int* err,
-                             hipStream_t s);
+                             uint32_t* offsets, const CooSortBufs& sort, int32_t* times,
+                             void* data, int* err, hipStream_t s);
 void launch_fill_i32(int* p, int64_t n, hipStream_t s);
 void launch_finalize_f64(const int64_t* sums, const double* fnum, const BinDev* bins, int n_bins,
                          int nframes, int64_t* prefix, int64_t* smax, double* dpre, double* num,
@@ -195,6 +194,10 @@ void launch_gather_pm(const uint8_t* frames, int dtype, int64_t hw, int nframes,
 void launch_pm_sums(const uint8_t* xs, int dtype, const BinDev* bins, int n_bins, int max_npix,
                     int nframes, int nrows, int64_t* sums, hipStream_t s);
 int fft_lds_length(int n);
+int fft_four_step_length(int n, int* L1, int* L2);
+int launch_fft_four_step(int L1, int L2, int dtype, const uint8_t* xs, const BinDev* bins,
+                         const KbWork* work, int kb0, int n_kb, int nrows, const float2* tw,
+                         const float2* uw, float2* y, double* power, hipStream_t s);
 int launch_fft_power(int L, int dtype, const uint8_t* xs, const BinDev* bins, const KbWork* work,
                      int n_work, int nrows, const float2* tw, const float2* uw, double* power,
                      hipStream_t s);

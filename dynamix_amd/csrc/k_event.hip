@@ -18,12 +18,11 @@
 //    pixel, as compress_all_stack produces, event.py:267-284); a violation sets a
 //    flag the host turns into an error instead of reading out of bounds.
 // Frame-sorted COO input (pixel, frame, count) is first turned into this pixel-major
-// layout on the device: per-pixel counts and an exclusive scan give the offsets; the
-// (frame, count) pairs, packed into one word, are ordered by pixel with a stable LSD
-// radix sort (rocPRIM) over the bits a pixel index needs, so each pixel's events keep
-// the input's frame order; the packed words are then split into times and counts.
-#include <rocprim/device/device_radix_sort.hpp>
-
+// layout on the device: the (frame, count) pairs, packed into one word, are ordered by
+// pixel with this file's stable LSD radix sort (8-bit digits over the bits a pixel
+// index needs), so each pixel's events keep the input's frame order; the offsets are
+// the lower bounds of every pixel in the sorted keys, and the packed words are split
+// into times and counts.
 #include "dxg2_internal.h"
 
 namespace dxg2 {
@@ -386,13 +385,14 @@ __global__ void copy_u32_kernel(const uint32_t* __restrict__ src, uint32_t* __re
 // (frame, count) of event e packed as frame << CB | count; a frame past the series sets
 // flag bit 0 (the packing keeps CB bits of count and the rest for the frame).
 template <typename V, typename T, int CB>
-__global__ void coo_pack_kernel(const uint32_t* __restrict__ frame, const T* __restrict__ cnt,
-                                int64_t nnz, int nframes, V* __restrict__ out,
-                                int* __restrict__ err) {
+__global__ void coo_pack_kernel(const uint32_t* __restrict__ pix, const uint32_t* __restrict__ frame,
+                                const T* __restrict__ cnt, int64_t nnz, uint32_t npix, int nframes,
+                                V* __restrict__ out, int* __restrict__ err) {
     for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < nnz;
          e += (int64_t)gridDim.x * blockDim.x) {
         const uint32_t f = frame[e];
         if (f >= (uint32_t)nframes) atomicOr(err, 1);
+        if (pix[e] >= npix) atomicOr(err, 4);
         out[e] = ((V)f << CB) | (V)cnt[e];
     }
 }
@@ -561,56 +561,223 @@ bool coo_narrow_values(int dtype, int nframes) {
     return dtype != DXG2_U16 && nframes <= (1 << 24);
 }
 
+// ------------------------------------------------------- stable LSD radix sort
+// Sorts (key, value) pairs by the key bits [0, 8 * passes), 8 bits per pass, stably.
+// Per pass over tiles of RS_TILE = 4096 pairs (4 waves x 16 rounds x 64 lanes, element
+// index = wave * 1024 + round * 64 + lane):
+//   radix_hist_kernel     digit histogram of each tile -> hist[digit * n_tiles + tile];
+//   exclusive scan of hist (digit-major): the output position of each tile's first
+//                         element of each digit;
+//   radix_scatter_kernel  each wave ranks its 16 rounds in order: the lanes holding one
+//                         digit find each other with 8 ballots (one per digit bit), the
+//                         rank is the wave's running count of that digit plus the lanes
+//                         below with it; the waves' counts give the cross-wave base; the
+//                         tile is then reordered by digit in LDS and written out in runs.
+constexpr int RS_TILE = 4096;
+constexpr int RS_ROUNDS = 16;   // per wave
+
+__global__ __launch_bounds__(256) void radix_hist_kernel(const uint32_t* __restrict__ keys,
+                                                         int64_t n, int shift,
+                                                         uint32_t* __restrict__ hist,
+                                                         int n_tiles) {
+    __shared__ uint32_t cnt[256];
+    cnt[threadIdx.x] = 0u;
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * RS_TILE;
+    for (int i = threadIdx.x; i < RS_TILE; i += 256) {
+        const int64_t e = base + i;
+        if (e < n) atomicAdd(&cnt[(keys[e] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    hist[(int64_t)threadIdx.x * n_tiles + blockIdx.x] = cnt[threadIdx.x];
+}
+
+template <typename V>
+__global__ __launch_bounds__(256) void radix_scatter_kernel(
+    const uint32_t* __restrict__ keys_in, const V* __restrict__ vals_in, int64_t n, int shift,
+    const uint32_t* __restrict__ pos, int n_tiles, uint32_t* __restrict__ keys_out,
+    V* __restrict__ vals_out) {
+    __shared__ uint32_t wave_cnt[4][256];     // per wave: count of each digit so far
+    __shared__ uint32_t local_off[256];       // the tile's exclusive digit offsets
+    __shared__ uint32_t tot;
+    __shared__ uint32_t s_keys[RS_TILE];
+    __shared__ V s_vals[RS_TILE];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    for (int w = 0; w < 4; ++w) wave_cnt[w][tid] = 0u;
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * RS_TILE + wave * (RS_TILE / 4);
+    const uint64_t below = (1ull << lane) - 1ull;
+    uint32_t key[RS_ROUNDS];
+    V val[RS_ROUNDS];
+    uint32_t rank[RS_ROUNDS];
+#pragma unroll
+    for (int r = 0; r < RS_ROUNDS; ++r) {
+        const int64_t e = base + r * 64 + lane;
+        const bool valid = e < n;
+        key[r] = valid ? keys_in[e] : 0u;
+        val[r] = valid ? vals_in[e] : (V)0;
+    }
+#pragma unroll
+    for (int r = 0; r < RS_ROUNDS; ++r) {
+        const int64_t e = base + r * 64 + lane;
+        const bool valid = e < n;
+        const uint32_t d = (key[r] >> shift) & 255u;
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const uint64_t m = __ballot((d >> b) & 1u);
+            peers &= ((d >> b) & 1u) ? m : ~m;
+        }
+        const uint32_t before = wave_cnt[wave][d];      // same value for every peer
+        rank[r] = before + (uint32_t)__popcll(peers & below);
+        if (valid && (peers & below) == 0ull)           // the lowest peer updates
+            wave_cnt[wave][d] = before + (uint32_t)__popcll(peers);
+        __builtin_amdgcn_wave_barrier();
+    }
+    __syncthreads();
+    // per digit (thread = digit): the waves' counts, their prefix, the tile's total
+    const uint32_t c0 = wave_cnt[0][tid], c1 = wave_cnt[1][tid], c2 = wave_cnt[2][tid],
+                   c3 = wave_cnt[3][tid];
+    const uint32_t total = c0 + c1 + c2 + c3;
+    __syncthreads();
+    wave_cnt[0][tid] = 0u;
+    wave_cnt[1][tid] = c0;
+    wave_cnt[2][tid] = c0 + c1;
+    wave_cnt[3][tid] = c0 + c1 + c2;
+    // exclusive scan of the 256 totals into local_off (Hillis-Steele in LDS)
+    local_off[tid] = total;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        const uint32_t x = tid >= off ? local_off[tid - off] : 0u;
+        __syncthreads();
+        local_off[tid] += x;
+        __syncthreads();
+    }
+    const uint32_t incl = local_off[tid];
+    __syncthreads();
+    local_off[tid] = incl - total;
+    if (tid == 255) tot = incl;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < RS_ROUNDS; ++r) {
+        const int64_t e = base + r * 64 + lane;
+        if (e < n) {
+            const uint32_t d = (key[r] >> shift) & 255u;
+            const uint32_t p = local_off[d] + wave_cnt[wave][d] + rank[r];
+            s_keys[p] = key[r];
+            s_vals[p] = val[r];
+        }
+    }
+    __syncthreads();
+    const uint32_t count = tot;
+    for (uint32_t i = tid; i < count; i += 256) {
+        const uint32_t k = s_keys[i];
+        const uint32_t d = (k >> shift) & 255u;
+        const int64_t o = (int64_t)pos[(int64_t)d * n_tiles + blockIdx.x] + (i - local_off[d]);
+        keys_out[o] = k;
+        vals_out[o] = s_vals[i];
+    }
+}
+
+static int radix_passes(uint32_t npix) { return (pixel_key_bits(npix) + 7) / 8; }
+
+static int64_t radix_tiles(int64_t n) { return (n + RS_TILE - 1) / RS_TILE; }
+
+// Workspace of the sort beyond the caller's keys / value buffers: a second key buffer,
+// the histogram and its scan totals.
 size_t coo_sort_bytes(int dtype, int nframes, int64_t cap, uint32_t npix) {
-    size_t bytes = 0;
-    const int bits = pixel_key_bits(npix);
-    hipError_t rc;
-    if (coo_narrow_values(dtype, nframes))
-        rc = rocprim::radix_sort_pairs(nullptr, bytes, (const uint32_t*)nullptr, (uint32_t*)nullptr,
-                                       (const uint32_t*)nullptr, (uint32_t*)nullptr, (size_t)cap,
-                                       0, bits, (hipStream_t)0);
-    else
-        rc = rocprim::radix_sort_pairs(nullptr, bytes, (const uint32_t*)nullptr, (uint32_t*)nullptr,
-                                       (const uint64_t*)nullptr, (uint64_t*)nullptr, (size_t)cap,
-                                       0, bits, (hipStream_t)0);
-    return rc == hipSuccess ? bytes : 0;
+    (void)dtype;
+    (void)nframes;
+    (void)npix;
+    const int64_t nt = radix_tiles(std::max<int64_t>(cap, 1));
+    const int64_t hist = 256 * nt;
+    // second key buffer (64-aligned), histogram + the scan's grand total, scan totals
+    return (size_t)((cap + 63) / 64 * 64) * 4 + (size_t)(hist + 1) * 4 +
+           (size_t)((hist + 1023) / 1024 + 1) * 4 + 256;
+}
+
+// Sorts (pix, v0) by pixel; returns the buffers holding the sorted keys and values.
+template <typename V>
+static void radix_sort_by_pixel(const uint32_t* pix, V* v0, V* v1, int64_t n, uint32_t npix,
+                                const CooSortBufs& b, const uint32_t** keys_sorted,
+                                const V** vals_sorted, hipStream_t s) {
+    const int64_t nt = radix_tiles(n);
+    uint32_t* keys2 = reinterpret_cast<uint32_t*>(b.tmp);
+    uint32_t* hist = keys2 + ((n + 63) / 64) * 64;
+    uint32_t* tot = hist + 256 * nt + 1;   // the in-place scan writes hist[256 nt]
+    const uint32_t* kin = pix;
+    const V* vin = v0;
+    uint32_t* kbuf[2] = {b.keys, keys2};
+    V* vbuf[2] = {v1, v0};
+    const int passes = radix_passes(npix);
+    for (int p = 0; p < passes; ++p) {
+        uint32_t* kout = kbuf[p & 1];
+        V* vout = vbuf[p & 1];
+        radix_hist_kernel<<<(unsigned)nt, 256, 0, s>>>(kin, n, 8 * p, hist, (int)nt);
+        launch_excl_scan(hist, 256 * nt, tot, hist, s);
+        radix_scatter_kernel<V><<<(unsigned)nt, 256, 0, s>>>(kin, vin, n, 8 * p, hist, (int)nt,
+                                                             kout, vout);
+        kin = kout;
+        vin = vout;
+    }
+    *keys_sorted = kin;
+    *vals_sorted = vin;
+}
+
+// offsets[p] = the first position of a key >= p in the sorted keys, p = 0 .. npix.
+__global__ void offsets_from_keys_kernel(const uint32_t* __restrict__ keys, int64_t n,
+                                         uint32_t npix, uint32_t* __restrict__ offsets) {
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p <= (int64_t)npix;
+         p += (int64_t)gridDim.x * blockDim.x) {
+        int64_t lo = 0, hi = n;
+        while (lo < hi) {
+            const int64_t m = lo + (hi - lo) / 2;
+            if (keys[m] < (uint32_t)p)
+                lo = m + 1;
+            else
+                hi = m;
+        }
+        offsets[p] = (uint32_t)lo;
+    }
 }
 
 template <typename V, typename T, int CB>
-static hipError_t coo_sort_split(const uint32_t* pix, const uint32_t* frame, const T* cnt,
-                                 int64_t nnz, uint32_t npix, int nframes, const CooSortBufs& b,
-                                 int32_t* times, T* data, int* err, hipStream_t s) {
+static void coo_sort_split(const uint32_t* pix, const uint32_t* frame, const T* cnt, int64_t nnz,
+                           uint32_t npix, int nframes, const CooSortBufs& b, uint32_t* offsets,
+                           int32_t* times, T* data, int* err, hipStream_t s) {
     V* v0 = reinterpret_cast<V*>(b.vals0);
     V* v1 = reinterpret_cast<V*>(b.vals1);
-    coo_pack_kernel<V, T, CB><<<grid_for(nnz), 256, 0, s>>>(frame, cnt, nnz, nframes, v0, err);
-    size_t bytes = b.tmp_bytes;
-    const hipError_t rc = rocprim::radix_sort_pairs(b.tmp, bytes, pix, b.keys, v0, v1, (size_t)nnz,
-                                                    0, pixel_key_bits(npix), s);
-    if (rc != hipSuccess) return rc;
-    coo_unpack_kernel<V, T, CB><<<grid_for(nnz), 256, 0, s>>>(v1, nnz, times, data);
-    return hipSuccess;
+    coo_pack_kernel<V, T, CB><<<grid_for(nnz), 256, 0, s>>>(pix, frame, cnt, nnz, npix, nframes,
+                                                           v0, err);
+    const uint32_t* keys = nullptr;
+    const V* vals = nullptr;
+    radix_sort_by_pixel<V>(pix, v0, v1, nnz, npix, b, &keys, &vals, s);
+    offsets_from_keys_kernel<<<grid_for((int64_t)npix + 1), 256, 0, s>>>(keys, nnz, npix,
+                                                                        offsets);
+    coo_unpack_kernel<V, T, CB><<<grid_for(nnz), 256, 0, s>>>(vals, nnz, times, data);
 }
 
 hipError_t launch_coo_to_csr(int dtype, const uint32_t* pix, const uint32_t* frame,
                              const void* cnt, int64_t nnz, uint32_t npix, int nframes,
-                             uint32_t* count, uint32_t* scan_tot, uint32_t* offsets,
-                             const CooSortBufs& sort, int32_t* times, void* data, int* err,
-                             hipStream_t s) {
-    fill_u32_kernel<<<grid_for(npix), 256, 0, s>>>(count, npix, 0u);
-    if (nnz > 0) coo_count_kernel<<<grid_for(nnz), 256, 0, s>>>(pix, nnz, npix, count, err);
-    const int nb = (int)((npix + SCAN_B - 1) / SCAN_B);
-    scan_block_totals<<<nb, 256, 0, s>>>(count, npix, scan_tot);
-    scan_totals<<<1, 256, 0, s>>>(scan_tot, nb);
-    scan_apply<<<nb, 256, 0, s>>>(count, npix, scan_tot, offsets);
-    if (nnz <= 0) return hipSuccess;
+                             uint32_t* offsets, const CooSortBufs& sort, int32_t* times,
+                             void* data, int* err, hipStream_t s) {
+    if (nnz <= 0) {
+        fill_u32_kernel<<<grid_for((int64_t)npix + 1), 256, 0, s>>>(offsets, (int64_t)npix + 1, 0u);
+        return hipGetLastError();
+    }
     if (dtype == DXG2_U16)
-        return coo_sort_split<uint64_t, uint16_t, 16>(pix, frame, (const uint16_t*)cnt, nnz, npix,
-                                                      nframes, sort, times, (uint16_t*)data, err, s);
-    if (coo_narrow_values(dtype, nframes))
-        return coo_sort_split<uint32_t, uint8_t, 8>(pix, frame, (const uint8_t*)cnt, nnz, npix,
-                                                    nframes, sort, times, (uint8_t*)data, err, s);
-    return coo_sort_split<uint64_t, uint8_t, 16>(pix, frame, (const uint8_t*)cnt, nnz, npix,
-                                                 nframes, sort, times, (uint8_t*)data, err, s);
+        coo_sort_split<uint64_t, uint16_t, 16>(pix, frame, (const uint16_t*)cnt, nnz, npix,
+                                               nframes, sort, offsets, times, (uint16_t*)data,
+                                               err, s);
+    else if (coo_narrow_values(dtype, nframes))
+        coo_sort_split<uint32_t, uint8_t, 8>(pix, frame, (const uint8_t*)cnt, nnz, npix, nframes,
+                                             sort, offsets, times, (uint8_t*)data, err, s);
+    else
+        coo_sort_split<uint64_t, uint8_t, 16>(pix, frame, (const uint8_t*)cnt, nnz, npix,
+                                              nframes, sort, offsets, times, (uint8_t*)data, err,
+                                              s);
+    return hipGetLastError();
 }
 
 }  // namespace dxg2

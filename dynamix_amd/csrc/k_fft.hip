@@ -453,6 +453,217 @@ __global__ __launch_bounds__(Plan::T) void fft_spectra_kernel(
     }
 }
 
+// ------------------------------------------------- long series: four-step through HBM
+// For N > 16384 the half-length is L = L1 L2 (L1 a plan length up to 8192, L2 in
+// {2, 4, 5, 8, 16}) and the transform is split (Bailey's four-step FFT):
+//   n = n1 + L1 n2, k = k2 + L2 k1,
+//   Y[k2][n1] = exp(-2 pi i n1 k2 / L) DFT_L2 over n2 of z[n1 + L1 n2]   (four_step_cols)
+//   Z[k2 + L2 k1] = DFT_L1 over n1 of Y[k2][n1]                          (four_step_rows)
+// through a scratch Y of L complex values per series, for a chunk of k-blocks at a time.
+// Z[k] pairs with Z[L - k] = Z[(L2 - k2) + L2 (L1 - 1 - k1)] for k2 > 0 (and with
+// Z[L2 ((L1 - k1) mod L1)] for k2 = 0), so a workgroup of four_step_rows transforms the
+// two rows k2 and L2 - k2 of a series and unpacks them together.
+
+// grid (series of the chunk, ceil(L1 / 256)); thread = n1.  Series s of the chunk is row
+// j = s % 64 of k-block work[kb0 + s / 64].
+template <typename E, int L2>
+__global__ __launch_bounds__(256) void four_step_cols_kernel(
+    const uint8_t* __restrict__ xs, const BinDev* __restrict__ bins,
+    const KbWork* __restrict__ work, int kb0, int nrows, int L1,
+    const float2* __restrict__ tw, float2* __restrict__ y) {
+    const int s = blockIdx.x;
+    const KbWork w = work[kb0 + s / KB];
+    const BinDev bd = bins[w.bin];
+    const int j = s % KB;
+    if (j >= bd.npix - w.kb * KB) return;     // padding rows of a bin's last k-block
+    const int n1 = blockIdx.y * 256 + threadIdx.x;
+    if (n1 >= L1) return;
+    const uint8_t* row = xs + bd.slab_off +
+                         ((int64_t)w.kb * KB + j) * nrows * Elem<E>::SIZE;
+    c32 v[L2];
+#pragma unroll
+    for (int n2 = 0; n2 < L2; ++n2) v[n2] = Elem<E>::value(Elem<E>::load(row, n1 + L1 * n2, nrows));
+    Dft<L2>::run(v);
+    float2* out = y + (int64_t)s * L1 * L2;
+#pragma unroll
+    for (int k2 = 0; k2 < L2; ++k2) {
+        const c32 r = k2 ? cmul(v[k2], c_of(tw[n1 * k2])) : v[k2];
+        out[(int64_t)k2 * L1 + n1] = make_float2(r.x, r.y);
+    }
+}
+
+template <int L2>
+struct RowPairs {   // (row A, row B) with B = L2 - A, or B = A for a row that pairs itself
+    static constexpr int N = L2 / 2 + 1;
+    __device__ __forceinline__ static void get(int i, int& a, int& b) {
+        a = i;
+        b = (i == 0 || 2 * i == L2) ? i : L2 - i;
+    }
+};
+
+// First pass of one row from registers (complex values loaded from Y).
+template <typename Plan>
+__device__ __forceinline__ void first_pass_c(c32* z, const c32 (&raw)[(Plan::L / Plan::R1 + Plan::T - 1) / Plan::T][Plan::R1], int t) {
+    constexpr int L = Plan::L, T = Plan::T, R1 = Plan::R1;
+    constexpr int NB1 = L / R1, M1 = (NB1 + T - 1) / T;
+#pragma unroll
+    for (int m = 0; m < M1; ++m) {
+        const int b = t + m * T;
+        if (NB1 % T == 0 || b < NB1) {
+            c32 v[R1];
+#pragma unroll
+            for (int r = 0; r < R1; ++r) v[r] = raw[m][r];
+            Dft<R1>::run(v);
+#pragma unroll
+            for (int q = 0; q < R1; ++q) z[sw(b * R1 + q)] = v[q];
+        }
+    }
+}
+
+template <typename Plan>
+__device__ __forceinline__ void load_row(c32 (&raw)[(Plan::L / Plan::R1 + Plan::T - 1) / Plan::T][Plan::R1],
+                                         const float2* __restrict__ src, int t) {
+    constexpr int L = Plan::L, T = Plan::T, R1 = Plan::R1;
+    constexpr int NB1 = L / R1, M1 = (NB1 + T - 1) / T;
+#pragma unroll
+    for (int m = 0; m < M1; ++m)
+#pragma unroll
+        for (int r = 0; r < R1; ++r) {
+            const int b = t + m * T;
+            if (NB1 % T == 0 || b < NB1) raw[m][r] = c_of(src[b + r * NB1]);
+        }
+}
+
+// Last pass of one row in place; twiddles exp(-2 pi i r b / L1) = tw[r b L2].
+template <typename Plan>
+__device__ __forceinline__ void last_pass_row(c32* z, const float2* __restrict__ tw, int l2,
+                                              int t) {
+    constexpr int L = Plan::L, T = Plan::T, RL = Plan::RL;
+    constexpr int NBL = L / RL, ML = (NBL + T - 1) / T;
+#pragma unroll
+    for (int m = 0; m < ML; ++m) {
+        const int b = t + m * T;
+        if (NBL % T == 0 || b < NBL) {
+            c32 v[RL];
+#pragma unroll
+            for (int r = 0; r < RL; ++r) {
+                const c32 a = z[sw(b + r * NBL)];
+                v[r] = r ? cmul(a, c_of(tw[r * b * l2])) : a;
+            }
+            Dft<RL>::run(v);
+#pragma unroll
+            for (int q = 0; q < RL; ++q) z[sw(b + q * NBL)] = v[q];
+        }
+    }
+}
+
+// grid (k-blocks of the chunk, row pairs); Plan is the row plan (L = L1 <= 8192).  The
+// power of bin b is accumulated at k = k2 + L2 k1 (and k = L by the row-0 workgroup).
+template <typename Plan, int L2>
+__global__ __launch_bounds__(Plan::T) void four_step_rows_kernel(
+    const float2* __restrict__ y, const BinDev* __restrict__ bins,
+    const KbWork* __restrict__ work, int kb0, const float2* __restrict__ tw,
+    const float2* __restrict__ uw, double* __restrict__ power) {
+    constexpr int L1 = Plan::L, T = Plan::T, R1 = Plan::R1, R2 = Plan::R2, R3 = Plan::R3;
+    constexpr int L = L1 * L2;
+    constexpr int M1 = (L1 / R1 + T - 1) / T;
+    constexpr int MU = (L1 + T - 1) / T;
+    __shared__ c32 za[L1];
+    __shared__ c32 zb[L1];
+    __shared__ c32 tab2[Plan::P2];
+    __shared__ c32 tab3[Plan::P3];
+    const int kbl = blockIdx.x;
+    const KbWork w = work[kb0 + kbl];
+    const BinDev bd = bins[w.bin];
+    int ra, rb;
+    RowPairs<L2>::get(blockIdx.y, ra, rb);
+    const bool self = ra == rb;
+    const int tid = threadIdx.x;
+    for (int m = tid; m < Plan::P2; m += T) tab2[m] = c_of(tw[m * (L1 / Plan::P2) * L2]);
+    if (Plan::NP == 4)
+        for (int m = tid; m < Plan::P3; m += T) tab3[m] = c_of(tw[m * (L1 / Plan::P3) * L2]);
+    double acc_a[MU], acc_b[MU];
+    double acc_last = 0.0;   // k = L (row 0 only)
+#pragma unroll
+    for (int m = 0; m < MU; ++m) acc_a[m] = acc_b[m] = 0.0;
+    const int kend = min(KB, bd.npix - w.kb * KB);
+    const float2* ybase = y + (int64_t)kbl * KB * L;
+    c32 raw_a[M1][R1], raw_b[M1][R1];
+    load_row<Plan>(raw_a, ybase + (int64_t)ra * L1, tid);
+    if (!self) load_row<Plan>(raw_b, ybase + (int64_t)rb * L1, tid);
+    __syncthreads();
+    for (int j = 0; j < kend; ++j) {
+        int t = tid;
+        asm volatile("" : "+v"(t));
+        first_pass_c<Plan>(za, raw_a, t);
+        if (!self) first_pass_c<Plan>(zb, raw_b, t);
+        if (j + 1 < kend) {
+            const float2* nxt = ybase + (int64_t)(j + 1) * L;
+            load_row<Plan>(raw_a, nxt + (int64_t)ra * L1, t);
+            if (!self) load_row<Plan>(raw_b, nxt + (int64_t)rb * L1, t);
+        }
+        __syncthreads();
+        mid_pass<Plan, R2, R1, Plan::P2>(za, tab2, t);
+        if (!self) mid_pass<Plan, R2, R1, Plan::P2>(zb, tab2, t);
+        if (Plan::NP == 4) {
+            mid_pass<Plan, R3, R1 * R2, Plan::P3>(za, tab3, t);
+            if (!self) mid_pass<Plan, R3, R1 * R2, Plan::P3>(zb, tab3, t);
+        }
+        last_pass_row<Plan>(za, tw, L2, t);
+        if (!self) last_pass_row<Plan>(zb, tw, L2, t);
+        __syncthreads();
+        // unpack: X[k] = (Z[k] + conj Z[L-k]) / 2 - i W^k (Z[k] - conj Z[L-k]) / 2
+        const c32* zo = self ? za : zb;   // the row holding the partners of row A
+#pragma unroll
+        for (int m = 0; m < MU; ++m) {
+            const int k1 = t + m * T;
+            if (k1 < L1) {
+                // row A point k1 and its partner
+                {
+                    const int ka = ra + L2 * k1;
+                    const int pk = ra == 0 ? (L1 - k1) % L1 : L1 - 1 - k1;
+                    const c32 a = za[sw(k1)];
+                    const c32 c = zo[sw(pk)];
+                    const c32 s2 = c32{a.x + c.x, a.y - c.y};
+                    const c32 d = c32{a.x - c.x, a.y + c.y};
+                    const c32 wd = cmul(c_of(uw[ka]), d);
+                    const float xr = 0.5f * (s2.x + wd.y), xi = 0.5f * (s2.y - wd.x);
+                    acc_a[m] += (double)xr * xr + (double)xi * xi;
+                }
+                if (!self) {   // row B point k1, partner in row A
+                    const int kb_ = rb + L2 * k1;
+                    const c32 a = zb[sw(k1)];
+                    const c32 c = za[sw(L1 - 1 - k1)];
+                    const c32 s2 = c32{a.x + c.x, a.y - c.y};
+                    const c32 d = c32{a.x - c.x, a.y + c.y};
+                    const c32 wd = cmul(c_of(uw[kb_]), d);
+                    const float xr = 0.5f * (s2.x + wd.y), xi = 0.5f * (s2.y - wd.x);
+                    acc_b[m] += (double)xr * xr + (double)xi * xi;
+                }
+            }
+        }
+        if (ra == 0 && t == 0) {   // k = L: Z[L] = Z[0]
+            const c32 a = za[sw(0)];
+            const c32 s2 = c32{a.x + a.x, 0.f};
+            const c32 d = c32{0.f, a.y + a.y};
+            const c32 wd = cmul(c_of(uw[L]), d);
+            const float xr = 0.5f * (s2.x + wd.y), xi = 0.5f * (s2.y - wd.x);
+            acc_last += (double)xr * xr + (double)xi * xi;
+        }
+        __syncthreads();
+    }
+    double* prow = power + (int64_t)w.bin * (L + 1);
+#pragma unroll
+    for (int m = 0; m < MU; ++m) {
+        const int k1 = tid + m * T;
+        if (k1 < L1) {
+            if (acc_a[m] != 0.0) atomicAdd(prow + ra + L2 * k1, acc_a[m]);
+            if (!self && acc_b[m] != 0.0) atomicAdd(prow + rb + L2 * k1, acc_b[m]);
+        }
+    }
+    if (ra == 0 && tid == 0 && acc_last != 0.0) atomicAdd(prow + L, acc_last);
+}
+
 // Num(tau) from the power spectrum, FP64; grid (ceil(N / 256), n_bins).  Every
 // thread reads the same ps[k] in step, so the spectrum is read as uniform loads.
 __global__ __launch_bounds__(256) void fft_inverse_kernel(const double* __restrict__ power,
@@ -574,6 +785,88 @@ int launch_fft_power(int L, int dtype, const uint8_t* xs, const BinDev* bins, co
             return -1;
     }
 #undef DXG2_FFT_CASE
+}
+
+// Long series: L = L1 L2 with L1 a row plan (<= 8192) and L2 in {2, 4, 5, 8, 16}; the
+// smallest such L >= n above 16384, the larger L1 on ties.  Returns 0 past 131072.
+static const int kRowLengths[] = {8192, 5120, 4096, 2560, 2048, 1280, 1024};
+static const int kColLengths[] = {2, 4, 5, 8, 16};
+
+int fft_four_step_length(int n, int* L1, int* L2) {
+    int best = 0;
+    for (int l1 : kRowLengths)
+        for (int l2 : kColLengths) {
+            const int L = l1 * l2;
+            if (L >= n && L > 16384 && (best == 0 || L < best)) {
+                best = L;
+                *L1 = l1;
+                *L2 = l2;
+            }
+        }
+    return best;
+}
+
+template <typename E, int L2>
+static void cols_typed(const uint8_t* xs, const BinDev* bins, const KbWork* work, int kb0,
+                       int n_kb, int nrows, int L1, const float2* tw, float2* y, hipStream_t s) {
+    four_step_cols_kernel<E, L2><<<dim3((unsigned)(n_kb * KB), (unsigned)((L1 + 255) / 256)),
+                                   256, 0, s>>>(xs, bins, work, kb0, nrows, L1, tw, y);
+}
+
+template <int L2>
+static void cols_dtype(int dtype, const uint8_t* xs, const BinDev* bins, const KbWork* work,
+                       int kb0, int n_kb, int nrows, int L1, const float2* tw, float2* y,
+                       hipStream_t s) {
+    if (dtype == DXG2_U16)
+        cols_typed<uint16_t, L2>(xs, bins, work, kb0, n_kb, nrows, L1, tw, y, s);
+    else if (dtype == DXG2_I8)
+        cols_typed<int8_t, L2>(xs, bins, work, kb0, n_kb, nrows, L1, tw, y, s);
+    else
+        cols_typed<uint8_t, L2>(xs, bins, work, kb0, n_kb, nrows, L1, tw, y, s);
+}
+
+template <typename Plan, int L2>
+static void rows_launch(const float2* y, const BinDev* bins, const KbWork* work, int kb0,
+                        int n_kb, const float2* tw, const float2* uw, double* power,
+                        hipStream_t s) {
+    four_step_rows_kernel<Plan, L2><<<dim3((unsigned)n_kb, (unsigned)RowPairs<L2>::N), Plan::T,
+                                      0, s>>>(y, bins, work, kb0, tw, uw, power);
+}
+
+template <int L2>
+static int rows_plan(int L1, const float2* y, const BinDev* bins, const KbWork* work, int kb0,
+                     int n_kb, const float2* tw, const float2* uw, double* power,
+                     hipStream_t s) {
+    switch (L1) {
+        case 1024: rows_launch<Plan1024, L2>(y, bins, work, kb0, n_kb, tw, uw, power, s); return 0;
+        case 1280: rows_launch<Plan1280, L2>(y, bins, work, kb0, n_kb, tw, uw, power, s); return 0;
+        case 2048: rows_launch<Plan2048, L2>(y, bins, work, kb0, n_kb, tw, uw, power, s); return 0;
+        case 2560: rows_launch<Plan2560, L2>(y, bins, work, kb0, n_kb, tw, uw, power, s); return 0;
+        case 4096: rows_launch<Plan4096, L2>(y, bins, work, kb0, n_kb, tw, uw, power, s); return 0;
+        case 5120: rows_launch<Plan5120, L2>(y, bins, work, kb0, n_kb, tw, uw, power, s); return 0;
+        case 8192: rows_launch<Plan8192, L2>(y, bins, work, kb0, n_kb, tw, uw, power, s); return 0;
+        default: return -1;
+    }
+}
+
+// One chunk of k-blocks [kb0, kb0 + n_kb) through the scratch y (n_kb * 64 * L complex).
+int launch_fft_four_step(int L1, int L2, int dtype, const uint8_t* xs, const BinDev* bins,
+                         const KbWork* work, int kb0, int n_kb, int nrows, const float2* tw,
+                         const float2* uw, float2* y, double* power, hipStream_t s) {
+    if (n_kb <= 0) return 0;
+    switch (L2) {
+        case 2: cols_dtype<2>(dtype, xs, bins, work, kb0, n_kb, nrows, L1, tw, y, s);
+                return rows_plan<2>(L1, y, bins, work, kb0, n_kb, tw, uw, power, s);
+        case 4: cols_dtype<4>(dtype, xs, bins, work, kb0, n_kb, nrows, L1, tw, y, s);
+                return rows_plan<4>(L1, y, bins, work, kb0, n_kb, tw, uw, power, s);
+        case 5: cols_dtype<5>(dtype, xs, bins, work, kb0, n_kb, nrows, L1, tw, y, s);
+                return rows_plan<5>(L1, y, bins, work, kb0, n_kb, tw, uw, power, s);
+        case 8: cols_dtype<8>(dtype, xs, bins, work, kb0, n_kb, nrows, L1, tw, y, s);
+                return rows_plan<8>(L1, y, bins, work, kb0, n_kb, tw, uw, power, s);
+        case 16: cols_dtype<16>(dtype, xs, bins, work, kb0, n_kb, nrows, L1, tw, y, s);
+                 return rows_plan<16>(L1, y, bins, work, kb0, n_kb, tw, uw, power, s);
+        default: return -1;
+    }
 }
 
 void launch_fft_inverse(const double* power, int L, int n_bins, int nframes, double* num,

@@ -5,6 +5,12 @@ to pin dynamix_amd/tools.py.
 radi_loops      tools.py:14-50   radial average on integer radii + interpolated model
 make_q_loops    tools.py:262-295 per-pixel q and annulus test
 cftomt_loops    tools.py:146-172 multitau resampling, one row at a time
+
+Parity pinning: radi, cftomt and format_result are also pinned directly by golden
+vectors written by running those reference functions (tests/golden/make_golden.py;
+tests/test_tools.py compares dynamix_amd/tools.py with them).  make_q needs its ini-file
+configuration and is pinned by make_q_loops and the closed forms of tests/test_tools.py
+only.
 """
 import math
 

@@ -9,6 +9,7 @@ definitions out of the reference source files (`ast`) and runs them with numpy:
   dynamix/correlator/common.py  BaseCorrelator               (lines 10-80)
   dynamix/correlator/event.py   FramesCompressor             (lines 207-321)
   dynamix/utils.py              generate_powers, get_next_power (lines 29-60)
+  dynamix/tools/tools.py        radi, cftomt, format_result  (lines 14-50, 146-172, 313-346)
 
 Output: tests/golden/reference_vectors.npz (data only: seeded inputs and the
 reference's outputs).  Nothing of the reference's source is stored.
@@ -149,6 +150,50 @@ def main():
                    19999, 20000, 20001, 65536, 99999, 200000, 123457], dtype=np.int64)
     out["nextpow_n"] = ns
     out["nextpow_val"] = np.array([int(get_next_power(int(v))) for v in ns], dtype=np.int64)
+
+    # ---- host tools: radial average, multitau resampling, result formatting
+    tl = load_defs(os.path.join(REF, "tools", "tools.py"), ["radi", "cftomt", "format_result"])
+    for name, shape, cx, cy in [("a", (23, 31), 12.3, 9.7), ("b", (40, 37), 5.6, 30.2)]:
+        saxs = rng.random(shape) * 100
+        mask = (rng.random(shape) < 0.05).astype(np.uint8)
+        mask[shape[0] // 2 - 1: shape[0] // 2 + 2, :] = 1
+        rad, q, new_saxs = tl["radi"](saxs, mask, cx, cy)
+        out["radi_%s_saxs" % name] = saxs
+        out["radi_%s_mask" % name] = mask
+        out["radi_%s_center" % name] = np.array([cx, cy])
+        out["radi_%s_rad" % name] = rad
+        out["radi_%s_q" % name] = q
+        out["radi_%s_model" % name] = new_saxs
+    for name, n, par, dt in [("f32_16", 1000, 16, np.float32), ("f64_8", 333, 8, np.float64),
+                             ("f32_32", 97, 32, np.float32)]:
+        d = np.zeros((n, 3), dt)
+        d[:, 0] = np.arange(1, n + 1)
+        d[:, 1] = 1 + 0.3 * np.exp(-np.arange(n) / 50.0) + rng.normal(0, 1e-3, n)
+        d[:, 2] = rng.random(n) * 1e-2
+        out["cftomt_%s_in" % name] = d
+        out["cftomt_%s_par" % name] = np.int64(par)
+        out["cftomt_%s_out" % name] = tl["cftomt"](d, par)
+    Result = namedtuple("Result", "res dev trc")
+    n, nq, shape = 300, 3, (12, 14)
+    qq = rng.integers(0, nq + 1, size=shape).astype(np.int32)
+    for b in range(1, nq + 1):
+        qq.flat[b] = b
+    flat = (1 + 0.1 * rng.random(shape)).astype(np.float32)
+    cdata = (5 + rng.random(shape)).astype(np.float32)
+    res = (1 + 0.2 * rng.random((nq, n))).astype(np.float32)
+    dev = (0.01 * rng.random((nq, n))).astype(np.float32)
+    trc = rng.random((n, n)).astype(np.float32)
+    per_q, table, trc_out = tl["format_result"](Result(res, dev, trc), qq, flat, cdata, 0.01, 2)
+    out["fmt_qmask"] = qq
+    out["fmt_flat"] = flat
+    out["fmt_cdata"] = cdata
+    out["fmt_res"] = res
+    out["fmt_dev"] = dev
+    out["fmt_trc"] = trc
+    out["fmt_table"] = table
+    for i, m in enumerate(per_q):
+        out["fmt_q%d" % (i + 1)] = m
+    out["fmt_trc_out"] = trc_out
 
     np.savez_compressed(OUT, **out)
     print("wrote", OUT, "with", len(out), "arrays")

@@ -25,10 +25,27 @@ def _qmask(n, shape, nbins):
     return q
 
 
-def _check(c, g2, f, q):
+def _sparse_ref(f, q):
+    """The oracle through the event pairs (long, sparse stacks): every non-zero sample,
+    negative int8 values included, as a pixel-major event list."""
+    from oracle import events as ev
+    n = f.shape[0]
+    per_pixel = np.moveaxis(f, 0, -1).reshape(-1, n)
+    pix, t = np.nonzero(per_pixel != 0)
+    d = per_pixel[pix, t]
+    o = np.zeros(per_pixel.shape[0] + 1, np.int64)
+    o[1:] = np.cumsum(np.bincount(pix, minlength=per_pixel.shape[0]))
+    _, bins = og.qmask_bins(q.shape, q)
+    s, num = ev.event_sums_numerators_fast(t, d, o, q, f.shape[0], bins)
+    den = og.denominators(s)
+    sc = og.scale_factors(q, bins)
+    return {"sums": s, "num": num, "den": den, "scale": sc, "g2": og.g2_fp64(num, den, sc)}
+
+
+def _check(c, g2, f, q, ref=None):
     """Exact sums and denominators; numerators within NUM_TOL * Num(0) per bin; g2 at
     every lag within the bound that numerator error gives through Den(tau)."""
-    ref = og.dense_g2(f, q)
+    ref = ref or og.dense_g2(f, q)
     np.testing.assert_array_equal(c.last_sums.cpu().numpy(), ref["sums"])
     den = np.asarray(ref["den"], np.float64)
     np.testing.assert_array_equal(c.last_denominators.cpu().numpy(), den)
@@ -82,3 +99,28 @@ def test_fft_int8_and_uint16(n):
                          mean_map_=mean_map(shape, 0.5, 0.1, 300.0, n))
     assert hot.max() > 255
     _check(c, c.correlate(hot), hot, q)
+
+
+@pytest.mark.parametrize("n,L,dtype", [(16385, 20480, "uint8"), (20000, 20480, "uint16"),
+                                       (20000, 20480, "int8"), (45000, 65536, "uint8")])
+def test_fft_long_series_four_step(n, L, dtype, monkeypatch):
+    """N > 16384: the four-step transform through the HBM scratch (several chunks of
+    k-blocks with a 1 MiB scratch cap), exact sums, numerators within tolerance."""
+    from dynamix_amd.correlator.dense import FFTCorrelator, fft_length
+    from dynamix_amd.synth import speckle_frames
+    _announce("fft four-step n=%d L=%d %s" % (n, L, dtype))
+    monkeypatch.setenv("DXG2_FFT_SCRATCH_MB", "1")   # one k-block per chunk
+    assert fft_length(n) == 2 * L
+    shape = (9, 16)
+    if dtype == "int8":
+        rng = np.random.default_rng(n)
+        f = (rng.random((n,) + shape) < 0.03) * rng.integers(-5, 6, size=(n,) + shape)
+        f = f.astype(np.int8)
+    else:
+        f = speckle_frames(shape, n, 0.04, 60.0, n, np.dtype(dtype))
+        if dtype == "uint16":
+            f[::97, 2, 3] = 40000
+    q = _qmask(n, shape, 3)
+    q[:, :8] = 1          # bin 1 spans several k-blocks
+    c = FFTCorrelator(shape, n, q)
+    _check(c, c.correlate(f), f, q, _sparse_ref(f, q))

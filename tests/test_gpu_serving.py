@@ -85,7 +85,7 @@ def test_mix_equals_alone():
     assert {s.layout for s in specs} == {"dense", "events"}
     srv = Server(max_batch=4, n_streams=2, depth=1, keep_results=True)
     try:
-        stats = srv.run(specs)
+        stats = srv.run(specs, realtime=False)   # all at once: dense jobs queue and group
     finally:
         srv.close()
     assert stats["jobs"] == len(specs) and stats["failed"] == 0, stats["failures"]

@@ -129,3 +129,43 @@ def test_format_result():
         cf[:, 2] = dev[i - 1] / corr
         np.testing.assert_array_equal(m, ot.cftomt_loops(cf))
         np.testing.assert_array_equal(table[:, 2 * i - 1:2 * i + 1], m[:, 1:])
+
+
+# ------------------------------------------------ golden vectors of the reference tools
+@pytest.mark.parametrize("name", ["a", "b"])
+def test_radial_average_matches_reference(golden, name):
+    """tools.radi (tools.py:14-50) run on the same inputs: radii, ring means, radius map
+    and the interpolated pattern."""
+    cx, cy = golden["radi_%s_center" % name]
+    rad, r, model = tools.radial_average(golden["radi_%s_saxs" % name],
+                                         golden["radi_%s_mask" % name], cx, cy)
+    ref = golden["radi_%s_rad" % name]
+    np.testing.assert_array_equal(rad[:, 0], ref[:, 0])
+    # ring means: the same sums in another order (bincount vs np.histogram weights)
+    np.testing.assert_allclose(rad[:, 1], ref[:, 1], rtol=1e-12, equal_nan=True)
+    np.testing.assert_array_equal(r, golden["radi_%s_q" % name])
+    np.testing.assert_array_equal(model, golden["radi_%s_model" % name])
+
+
+@pytest.mark.parametrize("name", ["f32_16", "f64_8", "f32_32"])
+def test_multitau_matches_reference(golden, name):
+    """tools.cftomt (tools.py:146-172) run on the same functions, bit for bit."""
+    d = golden["cftomt_%s_in" % name]
+    par = int(golden["cftomt_%s_par" % name])
+    out = tools.multitau(d, par)
+    ref = golden["cftomt_%s_out" % name]
+    assert out.shape == ref.shape and out.dtype == ref.dtype
+    np.testing.assert_array_equal(out, ref)
+
+
+def test_format_result_matches_reference(golden):
+    """tools.format_result (tools.py:313-346) on the same result: the per-q multitau
+    functions, the table for saving and the returned two-time matrix."""
+    r = Result(golden["fmt_res"], golden["fmt_dev"], golden["fmt_trc"])
+    per_q, table, trc = tools.format_result(r, golden["fmt_qmask"], golden["fmt_flat"],
+                                            golden["fmt_cdata"], 0.01, 2)
+    assert len(per_q) == 3
+    for i, m in enumerate(per_q):
+        np.testing.assert_array_equal(m, golden["fmt_q%d" % (i + 1)])
+    np.testing.assert_array_equal(table, golden["fmt_table"])
+    np.testing.assert_array_equal(trc, golden["fmt_trc_out"])
