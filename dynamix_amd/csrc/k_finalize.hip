@@ -174,8 +174,87 @@ constexpr int DEN_SPAN = FIN_THREADS * DEN_W;   // lags per block
 constexpr int DEN_CH = 1024;                    // frames staged per pass
 constexpr int DEN_SB = DEN_CH + DEN_SPAN + DEN_W;  // lag-side values staged per pass
 constexpr int DEN_BLOCKS = 2048;                // blocks aimed at per launch
+// Bins whose frame sums all satisfy |S| <= DEN_I16_MAX take den_i16_kernel: 16 products
+// of at most DEN_I16_MAX^2 stay below 2^31.
+constexpr int64_t DEN_I16_MAX = 11585;
 
 __device__ __forceinline__ int den_pad(int i) { return i + (i >> 4); }
+
+typedef short den_s2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ uint32_t pack_s16(int64_t a, int64_t b) {
+    return (uint32_t)(uint16_t)(int16_t)a | ((uint32_t)(uint16_t)(int16_t)b << 16);
+}
+
+// Den(tau) for the bins with |S| <= DEN_I16_MAX (and N M^2 < 2^52): the same blocks,
+// passes and register lag windows as den_fp64_kernel, on 16-bit integers two frames at a
+// time.  LDS holds pairs (S(t), S(t+1)) packed in one word, so one v_dot2_i32_i16 adds
+// S(t) S(t+lag) + S(t+1) S(t+1+lag) into a 32-bit accumulator; each thread's window
+// advances by two packed words per step, and the 32-bit sums (16 products of at most
+// DEN_I16_MAX^2 since the last flush) move to 64-bit accumulators every DEN_W frames.
+// Exact integer arithmetic throughout; the blocks' partial sums are added into den
+// (zeroed first) in FP64, exact below 2^53.
+__global__ __launch_bounds__(FIN_THREADS) void den_i16_kernel(const int64_t* __restrict__ sums,
+                                                              int nframes, int cpb,
+                                                              const int64_t* __restrict__ smax,
+                                                              double* __restrict__ den) {
+    __shared__ uint32_t sa[DEN_CH / 2];
+    __shared__ uint32_t sb[DEN_SB + DEN_SB / 16 + 1];
+    const int b = blockIdx.y;
+    const int tau0 = blockIdx.x * DEN_SPAN;
+    const int tid = threadIdx.x;
+    const double m = (double)smax[b];
+    if (!((double)nframes * m * m < 4503599627370496.0)) return;  // 128-bit path
+    if (smax[b] > DEN_I16_MAX) return;                            // den_fp64_kernel's bin
+    const int tb = blockIdx.z * cpb * DEN_CH;
+    const int te = min(nframes - tau0, tb + cpb * DEN_CH);
+    if (tb >= te) return;
+    const int64_t* S = sums + (int64_t)b * nframes;
+    const int lag = tau0 + DEN_W * tid;  // smallest lag of this thread
+    int64_t acc[DEN_W];
+#pragma unroll
+    for (int j = 0; j < DEN_W; ++j) acc[j] = 0;
+    auto at = [&](int64_t t) -> int64_t { return t < nframes ? S[t] : 0; };
+    for (int t0 = tb; t0 < te; t0 += DEN_CH) {
+        __syncthreads();
+        for (int i = tid; i < DEN_CH / 2; i += FIN_THREADS)
+            sa[i] = pack_s16(at((int64_t)t0 + 2 * i), at((int64_t)t0 + 2 * i + 1));
+        for (int i = tid; i < DEN_SB; i += FIN_THREADS) {
+            const int64_t t = (int64_t)t0 + tau0 + i;
+            sb[den_pad(i)] = pack_s16(at(t), at(t + 1));
+        }
+        __syncthreads();
+        int lim = nframes - lag - t0;
+        if (lim > DEN_CH) lim = DEN_CH;
+        const int base = DEN_W * tid;
+        uint32_t y[DEN_W];
+#pragma unroll
+        for (int j = 0; j < DEN_W - 2; ++j) y[j] = sb[den_pad(base + j)];
+        y[DEN_W - 2] = y[DEN_W - 1] = 0u;
+        for (int k0 = 0; k0 < lim; k0 += DEN_W) {
+            int part[DEN_W];
+#pragma unroll
+            for (int j = 0; j < DEN_W; ++j) part[j] = 0;
+#pragma unroll
+            for (int st = 0; st < DEN_W / 2; ++st) {
+                const int k = k0 + 2 * st;
+                y[(2 * st + DEN_W - 2) % DEN_W] = sb[den_pad(k + base + DEN_W - 2)];
+                y[(2 * st + DEN_W - 1) % DEN_W] = sb[den_pad(k + base + DEN_W - 1)];
+                const den_s2 x = __builtin_bit_cast(den_s2, sa[k >> 1]);
+#pragma unroll
+                for (int j = 0; j < DEN_W; ++j)
+                    part[j] = __builtin_amdgcn_sdot2(
+                        x, __builtin_bit_cast(den_s2, y[(2 * st + j) % DEN_W]), part[j], false);
+            }
+#pragma unroll
+            for (int j = 0; j < DEN_W; ++j) acc[j] += part[j];
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < DEN_W; ++j)
+        if (lag + j < nframes && acc[j] != 0)
+            atomicAdd(den + (int64_t)b * nframes + lag + j, (double)acc[j]);
+}
 
 __global__ __launch_bounds__(FIN_THREADS) void den_fp64_kernel(const int64_t* __restrict__ sums,
                                                                int nframes, int cpb,
@@ -188,6 +267,7 @@ __global__ __launch_bounds__(FIN_THREADS) void den_fp64_kernel(const int64_t* __
     const int tid = threadIdx.x;
     const double m = (double)smax[b];
     if (!((double)nframes * m * m < 4503599627370496.0)) return;  // 128-bit path
+    if (smax[b] <= DEN_I16_MAX) return;                           // den_i16_kernel's bin
     const int tb = blockIdx.z * cpb * DEN_CH;
     const int te = min(nframes - tau0, tb + cpb * DEN_CH);
     if (tb >= te) return;
@@ -324,6 +404,8 @@ static void launch_den_fp64(const int64_t* sums, int n_bins, int nframes, const 
     const int64_t zb = (chunks + cpb - 1) / cpb;
     den_fp64_kernel<<<dim3((unsigned)lag_blocks, (unsigned)n_bins, (unsigned)zb),
                       dim3(FIN_THREADS), 0, s>>>(sums, nframes, (int)cpb, smax, dpre);
+    den_i16_kernel<<<dim3((unsigned)lag_blocks, (unsigned)n_bins, (unsigned)zb),
+                     dim3(FIN_THREADS), 0, s>>>(sums, nframes, (int)cpb, smax, dpre);
 }
 
 void launch_finalize(const int64_t* plane_sums, int64_t* sums, const int64_t* ugram,
