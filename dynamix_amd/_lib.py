@@ -56,6 +56,8 @@ FREE_FN = ctypes.CFUNCTYPE(None, c_void_p, c_void_p, c_void_p)
 _SIGNATURES = {
     "dxg2_set_allocator": (c_int, [ALLOC_FN, FREE_FN, c_void_p]),
     "dxg2_plan_create": (c_int, [ctypes.POINTER(PlanDesc), ctypes.POINTER(c_void_p)]),
+    "dxg2_plan_dry_run": (c_int, [ctypes.POINTER(PlanDesc), ctypes.POINTER(ctypes.c_int64),
+                                  ctypes.c_int32]),
     "dxg2_plan_destroy": (c_int, [c_void_p]),
     "dxg2_plan_destroy_async": (c_int, [c_void_p, c_void_p]),
     "dxg2_plan_query": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),

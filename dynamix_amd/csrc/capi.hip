@@ -840,16 +840,30 @@ int dxg2_set_allocator(dxg2_alloc_fn alloc, dxg2_free_fn free_fn, void* ctx) {
 
 const char* dxg2_version(void) { return "dxg2 0.1.0 gfx950"; }
 
-int dxg2_plan_create(const dxg2_plan_desc* desc, dxg2_plan** out) {
-    if (!desc || !out) return set_error(DXG2_EINVAL, "null argument");
-    *out = nullptr;
-    if (desc->height <= 0 || desc->width <= 0)
-        return set_error(DXG2_EINVAL, "bad frame shape (%d, %d)", desc->height, desc->width);
-    if (desc->nframes <= 0) return set_error(DXG2_EINVAL, "nframes must be > 0");
-    if (desc->dtype != DXG2_U8 && desc->dtype != DXG2_I8 && desc->dtype != DXG2_U16)
-        return set_error(DXG2_EINVAL, "unsupported dtype code %d", desc->dtype);
+namespace {
+
+// Host side of plan construction (no device call): validates the description and builds
+// the bins, padded pixel lists, slab layout, gather, band, Gram and event work lists.
+// Returns the plan (or nullptr with the error recorded).
+dxg2_plan* build_plan_host(const dxg2_plan_desc* desc, int* rc) {
+    *rc = DXG2_OK;
+    if (desc->height <= 0 || desc->width <= 0) {
+        *rc = set_error(DXG2_EINVAL, "bad frame shape (%d, %d)", desc->height, desc->width);
+        return nullptr;
+    }
+    if (desc->nframes <= 0) {
+        *rc = set_error(DXG2_EINVAL, "nframes must be > 0");
+        return nullptr;
+    }
+    if (desc->dtype != DXG2_U8 && desc->dtype != DXG2_I8 && desc->dtype != DXG2_U16) {
+        *rc = set_error(DXG2_EINVAL, "unsupported dtype code %d", desc->dtype);
+        return nullptr;
+    }
     const int64_t hw = (int64_t)desc->height * desc->width;
-    if (hw > (int64_t)INT32_MAX) return set_error(DXG2_EINVAL, "frame too large");
+    if (hw > (int64_t)INT32_MAX) {
+        *rc = set_error(DXG2_EINVAL, "frame too large");
+        return nullptr;
+    }
 
     dxg2_plan* p = new dxg2_plan();
     p->alloc = current_allocator();
@@ -885,13 +899,11 @@ int dxg2_plan_create(const dxg2_plan_desc* desc, dxg2_plan** out) {
         }
     }
     // scale factors (common.py:56-72)
-    if (desc->scale_factors) {
-        if (desc->n_scale_factors != p->n_bins) {
-            const int nb = p->n_bins;
-            delete p;
-            return set_error(DXG2_EINVAL, "got %d scale factors for %d bins",
-                             desc->n_scale_factors, nb);
-        }
+    if (desc->scale_factors && desc->n_scale_factors != p->n_bins) {
+        *rc = set_error(DXG2_EINVAL, "got %d scale factors for %d bins", desc->n_scale_factors,
+                        p->n_bins);
+        delete p;
+        return nullptr;
     }
     int64_t idx_off = 0, slab_off = 0;
     for (int b = 0; b < p->n_bins; ++b) {
@@ -917,6 +929,43 @@ int dxg2_plan_create(const dxg2_plan_desc* desc, dxg2_plan** out) {
     build_band_work(p, lists);
     build_gram_work(p);
     build_event_work(p);
+    return p;
+}
+
+}  // namespace
+
+int dxg2_plan_dry_run(const dxg2_plan_desc* desc, int64_t* summary, int32_t n) {
+    if (!desc || (n > 0 && !summary)) return set_error(DXG2_EINVAL, "null argument");
+    int rc = DXG2_OK;
+    dxg2_plan* p = build_plan_host(desc, &rc);
+    if (!p) return rc;
+    TileWork tw;
+    build_tile_work(p, -1, std_scratch_cap(), tw);
+    int64_t items = 0;
+    for (const auto& c : tw.chunks) items += (c.m1 - c.m0) + (c.x1 - c.x0);
+    const int64_t v[] = {p->n_bins,
+                         (int64_t)p->pix_idx.size(),
+                         (int64_t)p->slab_bytes,
+                         (int64_t)p->gather_work.size(),
+                         (int64_t)p->gband.size(),
+                         (int64_t)p->gband_kb.size(),
+                         (int64_t)p->gfall.size(),
+                         (int64_t)(p->gram_work.size() + p->gram_xwork.size()),
+                         (int64_t)(p->qm_work.size() + p->qx_work.size()),
+                         (int64_t)p->event_work.size(),
+                         (int64_t)tw.tiles.size(),
+                         items};
+    for (int32_t i = 0; i < n && i < (int32_t)(sizeof(v) / sizeof(v[0])); ++i) summary[i] = v[i];
+    delete p;
+    return DXG2_OK;
+}
+
+int dxg2_plan_create(const dxg2_plan_desc* desc, dxg2_plan** out) {
+    if (!desc || !out) return set_error(DXG2_EINVAL, "null argument");
+    *out = nullptr;
+    int rc0 = DXG2_OK;
+    dxg2_plan* p = build_plan_host(desc, &rc0);
+    if (!p) return rc0;
 
     hipError_t e = hipGetDevice(&p->device);
     if (e != hipSuccess) {

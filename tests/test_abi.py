@@ -37,6 +37,26 @@ def test_error_mapping_on_bad_desc():
         L.check(rc)
 
 
+def test_plan_dry_run_headline_layout():
+    """The host half of plan creation on the headline geometry, with no device call:
+    20 bins, each padded to whole 64-pixel k-blocks, slab rows rounded up to 256."""
+    import ctypes
+    lib = L.load()
+    q = np.ascontiguousarray(synth.workload_qmask("headline"), dtype=np.int32)
+    d = L.PlanDesc()
+    d.height, d.width, d.nframes, d.dtype = q.shape[0], q.shape[1], 1000, L.DXG2_U8
+    d.qmask = q.ctypes.data
+    s = (ctypes.c_int64 * 12)()
+    L.check(lib.dxg2_plan_dry_run(ctypes.byref(d), s, 12))
+    counts = np.bincount(q.ravel()[q.ravel() > 0], minlength=21)[1:]
+    padded = int(((counts + 63) // 64 * 64).sum())
+    assert s[0] == 20 and s[1] == padded and s[2] == padded * 1024
+    assert min(s[3:12]) >= 0 and s[7] > 0
+    d.dtype = L.DXG2_U16
+    L.check(lib.dxg2_plan_dry_run(ctypes.byref(d), s, 12))
+    assert s[2] == 2 * padded * 1024
+
+
 def test_headline_qmask_geometry():
     q = synth.workload_qmask("headline")
     assert q.shape == (514, 1030)
