@@ -153,6 +153,17 @@ def _timed(fn, reps, torch):
     return (time.perf_counter() - t0) * 1e3 / reps
 
 
+def _stage_times(plan, fn, reps, torch):
+    """The library's per-stage HIP-event times of `reps` calls of fn (ms)."""
+    torch.cuda.synchronize()
+    plan.set_profiling(True)
+    for _ in range(reps):
+        fn()
+    t = plan.stage_times()
+    plan.set_profiling(False)
+    return t
+
+
 def bench_fft(plan, frames, nb, n, pf):
     """FFT path on the same stack; its stage times come from the library's HIP events."""
     import torch
@@ -286,6 +297,7 @@ def bench_events(seed):
         _lib.check(plan.lib.dxg2_correlate_events_csr(plan.handle, _lib.ptr(times), _lib.ptr(data),
                                                       _lib.ptr(offsets), nnz, *outs, st))
     ms_csr = _timed(csr, 2, torch)
+    stages_csr = _stage_times(plan, csr, 2, torch)
     num_csr = num.clone()
     flags = ctypes.c_int32()
     _lib.check(plan.lib.dxg2_plan_event_status(plan.handle, ctypes.byref(flags), st))
@@ -297,7 +309,13 @@ def bench_events(seed):
         _lib.check(plan.lib.dxg2_correlate_events_coo(plan.handle, _lib.ptr(pix), _lib.ptr(fr),
                                                       _lib.ptr(cnt), nnz, *outs, st))
     ms_coo = _timed(coo, 2, torch)
+    stages_coo = _stage_times(plan, coo, 2, torch)
     same = bool((num == num_csr).all().item())
+    # LDS atomic-add ceiling of this GPU (random counters, the pair kernel's LDS
+    # footprint): every pair costs at least one such add
+    lds_rate = ctypes.c_double()
+    _lib.check(plan.lib.dxg2_measure_lds_atomics(0, 4096, ctypes.byref(lds_rate), st))
+    pairs_ms = stages_csr[1]
     hw = shape[0] * shape[1]
     conv_bytes = 9 * nnz + 2 * 5 * nnz + 4 * (hw + 1)
     # CSR call's algorithmic HBM traffic: each event's time and value read once, the
@@ -310,7 +328,15 @@ def bench_events(seed):
             "events_per_s": nnz / (ms_csr * 1e-3), "pixel_frames_per_s": int(binned.sum()) * n / (ms_csr * 1e-3),
             "csr_hbm_GBps": csr_bytes / (ms_csr * 1e-3) / 1e9,
             "coo_to_csr_GBps": conv_bytes / ((ms_coo - ms_csr) * 1e-3) / 1e9 if ms_coo > ms_csr else None,
-            "coo_equals_csr": same, "input_flags": flags.value, "generate_s": gen_s}
+            "coo_equals_csr": same, "input_flags": flags.value, "generate_s": gen_s,
+            "stage_ms_csr": {"pairs_and_sums": stages_csr[1], "denominator_and_finalise": stages_csr[2]},
+            "stage_ms_coo": {"coo_to_csr": stages_coo[0], "pairs_and_sums": stages_coo[1],
+                             "denominator_and_finalise": stages_coo[2]},
+            "pairs_kernel": {"kernel": "event_pairs_kernel", "pairs_per_s": pairs / (pairs_ms * 1e-3),
+                             "lds_atomic_ceiling_per_s": lds_rate.value,
+                             "frac_of_lds_atomic_ceiling": pairs / (pairs_ms * 1e-3) / lds_rate.value,
+                             "note": "one LDS atomic add per pair at least; the ceiling is "
+                                     "dxg2_measure_lds_atomics (random counters) on this GPU"}}
 
 
 def bench_uint16(seed, reps=2, tune=True):
@@ -597,9 +623,11 @@ def main():
         src = torch.ones(nbytes, dtype=torch.uint8, device="cuda")
         dst = torch.empty_like(src)
         torch.cuda.synchronize()
-        dst.copy_(src)
+        # an elementwise kernel (a plain device copy may go to the DMA engines, which
+        # the counters do not see): reads and writes nbytes once each
+        torch.add(src, 1, out=dst)
         torch.cuda.synchronize()
-        log("copied %d bytes device to device" % nbytes)
+        log("copied %d bytes device to device (elementwise add kernel)" % nbytes)
         del src, dst
     print(json.dumps(out), flush=True)
 

@@ -56,6 +56,8 @@ FREE_FN = ctypes.CFUNCTYPE(None, c_void_p, c_void_p, c_void_p)
 _SIGNATURES = {
     "dxg2_set_allocator": (c_int, [ALLOC_FN, FREE_FN, c_void_p]),
     "dxg2_plan_create": (c_int, [ctypes.POINTER(PlanDesc), ctypes.POINTER(c_void_p)]),
+    "dxg2_measure_lds_atomics": (c_int, [ctypes.c_int32, ctypes.c_int32,
+                                         ctypes.POINTER(ctypes.c_double), c_void_p]),
     "dxg2_plan_dry_run": (c_int, [ctypes.POINTER(PlanDesc), ctypes.POINTER(ctypes.c_int64),
                                   ctypes.c_int32]),
     "dxg2_plan_destroy": (c_int, [c_void_p]),
@@ -72,6 +74,7 @@ _SIGNATURES = {
     "dxg2_batch_create": (c_int, [ctypes.POINTER(c_void_p)]),
     "dxg2_batch_destroy": (c_int, [c_void_p]),
     "dxg2_batch_correlate_dense": (c_int, [c_void_p, c_int] + [c_void_p] * 7),
+    "dxg2_batch_correlate_events": (c_int, [c_void_p, c_int] + [c_void_p] * 9),
     "dxg2_correlate_fft": (c_int, [c_void_p] * 7),
     "dxg2_correlate_events_csr": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_i64,
                                           c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),

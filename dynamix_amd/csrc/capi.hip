@@ -134,6 +134,25 @@ std::vector<T> remap_for_xcds(const std::vector<T>& in, int group, const T& pad)
     return out;
 }
 
+// Gather order: items sorted by the detector position (raster index) of their first
+// pixel, then dealt to the XCDs in chunks of `group`, so the k-blocks of all bins that
+// read the same stretch of detector rows run together on one XCD and share the frame
+// lines in its L2 instead of each XCD fetching them again.  The list is padded to a
+// multiple of 8 with `pad` (a no-op item): a 2-D grid then keeps item i on XCD i % 8
+// for every frame tile.
+template <typename T, typename Key>
+std::vector<T> raster_order_for_xcds(const std::vector<T>& in, Key key, int group, const T& pad) {
+    std::vector<std::pair<int64_t, size_t>> k(in.size());
+    for (size_t i = 0; i < in.size(); ++i) k[i] = {key(in[i]), i};
+    std::stable_sort(k.begin(), k.end(),
+                     [](const std::pair<int64_t, size_t>& a, const std::pair<int64_t, size_t>& b) {
+                         return a.first < b.first;
+                     });
+    std::vector<T> sorted(in.size());
+    for (size_t i = 0; i < in.size(); ++i) sorted[i] = in[k[i].second];
+    return remap_for_xcds(sorted, group, pad);
+}
+
 // The same chunks as remap_for_xcds(), as 8 unpadded per-XCD queues laid end to end:
 // queue x (chunks x, x+8, ...) is out[off[x] .. off[x+1]).
 template <typename T>
@@ -263,6 +282,8 @@ struct dxg2_plan {
     // FFT path
     std::vector<KbWork> kb_work;   // every k-block of every bin
     KbWork* d_kbwork = nullptr;
+    std::vector<KbWork> kb_gather; // the same in gather order (kb -1: padding)
+    KbWork* d_kbgather = nullptr;
     uint8_t* d_xs = nullptr;       // pixel-major bytes, same size as the slab
     float2* d_tw = nullptr;        // exp(-2 pi i m / L), m < L; then exp(-pi i k / L), k <= L
     double* d_power = nullptr;     // (n_bins, L + 1)
@@ -603,6 +624,12 @@ int ensure_fft_buffers(dxg2_plan* p, hipStream_t s) {
     p->fft_L2 = L2;
     for (int b = 0; b < p->n_bins; ++b)
         for (int kb = 0; kb < p->bins[b].nkb; ++kb) p->kb_work.push_back(KbWork{b, kb});
+    p->kb_gather = raster_order_for_xcds(
+        p->kb_work,
+        [&](const KbWork& w) {
+            return (int64_t)p->pix_idx[p->bins[w.bin].idx_off + (int64_t)w.kb * KB];
+        },
+        PM_XCD_GROUP, KbWork{0, -1});
     p->fft_chunk = L1 ? fft_chunk_kblocks(L, (int)p->kb_work.size()) : 0;
     // [0, L): exp(-2 pi i m / L), the transform's twiddles; [L, 2L]: exp(-pi i k / L),
     // k = 0..L, the factor that unpacks the half-length complex transform
@@ -621,6 +648,7 @@ int ensure_fft_buffers(dxg2_plan* p, hipStream_t s) {
     float2* tw = nullptr;
     double *power = nullptr, *fnum = nullptr;
     KbWork* kbw = nullptr;
+    KbWork* kbg = nullptr;
     float2* fy = nullptr;
     if ((L1 && dev_alloc(p->alloc, (void**)&fy, (size_t)p->fft_chunk * KB * L * sizeof(float2),
                          s) != hipSuccess) ||
@@ -629,10 +657,13 @@ int ensure_fft_buffers(dxg2_plan* p, hipStream_t s) {
         dev_alloc(p->alloc, (void**)&fnum, (size_t)std::max(p->n_bins, 1) * p->N * sizeof(double), s) !=
             hipSuccess ||
         (!p->kb_work.empty() &&
-         dev_alloc(p->alloc, (void**)&kbw, p->kb_work.size() * sizeof(KbWork), s) != hipSuccess) ||
+         (dev_alloc(p->alloc, (void**)&kbw, p->kb_work.size() * sizeof(KbWork), s) != hipSuccess ||
+          dev_alloc(p->alloc, (void**)&kbg, p->kb_gather.size() * sizeof(KbWork), s) !=
+              hipSuccess)) ||
         dev_alloc(p->alloc, (void**)&power, (size_t)std::max(p->n_bins, 1) * (L + 1) * sizeof(double), s) !=
             hipSuccess) {
-        for (void* b : {(void*)xs, (void*)tw, (void*)fnum, (void*)kbw, (void*)power, (void*)fy})
+        for (void* b : {(void*)xs, (void*)tw, (void*)fnum, (void*)kbw, (void*)kbg, (void*)power,
+                        (void*)fy})
             dev_free(p->alloc, b, s);
         return set_error(DXG2_ENOMEM, "cannot allocate FFT workspace");
     }
@@ -641,20 +672,35 @@ int ensure_fft_buffers(dxg2_plan* p, hipStream_t s) {
     p->d_tw = tw;
     p->d_fnum = fnum;
     p->d_kbwork = kbw;
+    p->d_kbgather = kbg;
     p->d_power = power;
     DXG2_HIP(hipMemcpyAsync(p->d_tw, p->twiddles.data(), p->twiddles.size() * sizeof(float2),
                             hipMemcpyHostToDevice, s));
-    if (!p->kb_work.empty())
+    if (!p->kb_work.empty()) {
         DXG2_HIP(hipMemcpyAsync(p->d_kbwork, p->kb_work.data(),
                                 p->kb_work.size() * sizeof(KbWork), hipMemcpyHostToDevice, s));
+        DXG2_HIP(hipMemcpyAsync(p->d_kbgather, p->kb_gather.data(),
+                                p->kb_gather.size() * sizeof(KbWork), hipMemcpyHostToDevice, s));
+    }
     return DXG2_OK;
 }
 
+// Larger items flush fewer lag histograms and repeat less per-item work; the size is
+// the largest power of two from EVENT_ITEM_PIXELS up to 8 times that for which the
+// items still give about 512 workgroups over the series' lag windows (two per CU).
+// DXG2_EVENT_ITEM_PIXELS fixes it instead.
 void build_event_work(dxg2_plan* p) {
+    int64_t total = 0;
+    for (const auto& n : p->npix) total += n;
+    const int windows = (p->N + event_lag_window(p->dtype) - 1) / event_lag_window(p->dtype);
+    const int64_t want_items = std::max<int64_t>(1, (512 + windows - 1) / windows);
+    int item = EVENT_ITEM_PIXELS;
+    while (item < 8 * EVENT_ITEM_PIXELS && total / (2 * item) >= want_items) item *= 2;
+    item = std::max(64, env_int("DXG2_EVENT_ITEM_PIXELS", item));
     for (int b = 0; b < p->n_bins; ++b)
-        for (int k0 = 0; k0 < p->bins[b].npix; k0 += EVENT_ITEM_PIXELS)
+        for (int k0 = 0; k0 < p->bins[b].npix; k0 += item)
             p->event_work.push_back(
-                EventWork{b, k0, std::min<int>(p->bins[b].npix, k0 + EVENT_ITEM_PIXELS), 0});
+                EventWork{b, k0, std::min<int>(p->bins[b].npix, k0 + item), 0});
 }
 
 int ensure_event_buffers(dxg2_plan* p, int64_t nnz_coo, hipStream_t s) {
@@ -821,6 +867,11 @@ struct dxg2_batch {
     GramJob* d_jobs = nullptr;
     GramWork* d_work = nullptr;
     size_t jobs_cap = 0, work_cap = 0;
+    EventJob* h_ejobs = nullptr;  // grouped event launches
+    EventWork* h_ework = nullptr;
+    EventJob* d_ejobs = nullptr;
+    EventWork* d_ework = nullptr;
+    size_t ejobs_cap = 0, ework_cap = 0;
     std::vector<void*> retired;   // outgrown pinned buffers, freed with the batch (a
                                   // pinned free can wait for the whole device)
     DevAllocator alloc;
@@ -926,6 +977,12 @@ dxg2_plan* build_plan_host(const dxg2_plan_desc* desc, int* rc) {
         slab_off += (int64_t)slab_planes(p->dtype) * nkb * KB * p->nrows;
     }
     p->slab_bytes = (size_t)slab_off;
+    p->gather_work = raster_order_for_xcds(
+        p->gather_work,
+        [&](const GatherWork& g) {
+            return (int64_t)p->pix_idx[p->bins[g.bin].idx_off + (int64_t)g.kb0 * KB];
+        },
+        GATHER_XCD_GROUP, GatherWork{0, 0, 0, 0});
     build_band_work(p, lists);
     build_gram_work(p);
     build_event_work(p);
@@ -1099,7 +1156,7 @@ static void plan_release(dxg2_plan* p, hipStream_t s) {
                     p->d_ugram, p->d_sums, p->d_psums, p->d_prefix, p->d_smax, p->d_dpre, p->d_pa,
                     p->d_pg, p->d_pm,
                     p->d_ework, p->d_err,  p->d_cnt,   p->d_scan,  p->d_off,
-                    p->d_etimes, p->d_edata, p->d_kbwork, p->d_xs, p->d_tw,
+                    p->d_etimes, p->d_edata, p->d_kbwork, p->d_kbgather, p->d_xs, p->d_tw,
                     p->d_power, p->d_fnum, p->d_fy, p->d_qmwork, p->d_qxwork, p->d_qoff,
                     p->d_heads, p->d_gband, p->d_gband_kb, p->d_gfall,
                     p->coo_sort.keys, p->coo_sort.vals0, p->coo_sort.vals1, p->coo_sort.tmp,
@@ -1166,7 +1223,8 @@ int dxg2_plan_footprint(const dxg2_plan* p, int32_t path, size_t* bytes) {
             const size_t scratch =
                 L1 ? (size_t)fft_chunk_kblocks(L, (int)nkb) * KB * L * sizeof(float2) : 0;
             *bytes = common + p->slab_bytes + (2 * (size_t)L + 1) * sizeof(float2) + scratch +
-                     (size_t)std::max(p->n_bins, 1) * ((L + 1) + p->N) * sizeof(double);
+                     (size_t)std::max(p->n_bins, 1) * ((L + 1) + p->N) * sizeof(double) +
+                     (2 * (size_t)nkb + N_XCD) * sizeof(KbWork);
             return DXG2_OK;
         }
         case 3: {
@@ -1465,8 +1523,12 @@ int dxg2_batch_destroy(dxg2_batch* b) {
     (void)hipDeviceSynchronize();
     dev_free(b->alloc, b->d_jobs, nullptr);
     dev_free(b->alloc, b->d_work, nullptr);
+    dev_free(b->alloc, b->d_ejobs, nullptr);
+    dev_free(b->alloc, b->d_ework, nullptr);
     b->retired.push_back(b->h_jobs);
     b->retired.push_back(b->h_work);
+    b->retired.push_back(b->h_ejobs);
+    b->retired.push_back(b->h_ework);
     for (void* h : b->retired)
         if (h) (void)hipHostFree(h);
     delete b;
@@ -1582,6 +1644,103 @@ int dxg2_batch_correlate_dense(dxg2_batch* b, int32_t n, dxg2_plan* const* plans
     return DXG2_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Pinned staging + device copy of a batch table, grown geometrically; outgrown pinned
+// buffers are kept until the batch is destroyed (a pinned free can wait for the device).
+template <typename T>
+int grow_table(dxg2_batch* b, T** h, T** d, size_t* cap, size_t need, size_t min_cap,
+               hipStream_t s) {
+    if (*cap >= need) return DXG2_OK;
+    const size_t c = std::max<size_t>(std::max<size_t>(2 * *cap, need), min_cap);
+    b->retired.push_back(*h);
+    dev_free(b->alloc, *d, s);
+    *h = nullptr;
+    *d = nullptr;
+    *cap = 0;
+    if (hipHostMalloc((void**)h, c * sizeof(T), 0) != hipSuccess ||
+        dev_alloc(b->alloc, (void**)d, c * sizeof(T), s) != hipSuccess)
+        return set_error(DXG2_ENOMEM, "cannot allocate batch table");
+    *cap = c;
+    return DXG2_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dxg2_batch_correlate_events(dxg2_batch* b, int32_t n, dxg2_plan* const* plans,
+                                const int32_t* const* d_times, const void* const* d_data,
+                                const uint32_t* const* d_offsets, const int64_t* nnz,
+                                int64_t* const* d_num, int64_t* const* d_sums,
+                                double* const* d_den, float* const* d_g2, void* stream) {
+    if (!b || n < 0 || (n > 0 && (!plans || !d_times || !d_data || !d_offsets || !nnz || !d_g2)))
+        return set_error(DXG2_EINVAL, "null argument");
+    if (n == 0) return DXG2_OK;
+    hipStream_t s = (hipStream_t)stream;
+    const int dtype = plans[0] ? plans[0]->dtype : 0;
+    for (int i = 0; i < n; ++i) {
+        dxg2_plan* p = plans[i];
+        if (!p || !d_offsets[i] || !d_g2[i] || (nnz[i] > 0 && (!d_times[i] || !d_data[i])))
+            return set_error(DXG2_EINVAL, "job %d: null plan, times, data, offsets or g2", i);
+        if (nnz[i] < 0 || nnz[i] > (int64_t)UINT32_MAX)
+            return set_error(DXG2_EINVAL, "job %d: event count %lld out of range", i,
+                             (long long)nnz[i]);
+        if (p->dtype != dtype)
+            return set_error(DXG2_EINVAL, "job %d: count type differs from job 0's", i);
+        for (int k = 0; k < i; ++k)
+            if (plans[k] == p)
+                return set_error(DXG2_EINVAL, "plan of job %d appears twice in the batch", i);
+        int rc = ensure_event_buffers(p, -1, s);
+        if (rc) return rc;
+    }
+    (void)hipGetLastError();
+    // one work list over every job's items, each tagged with its job; the launch covers
+    // the lag windows of the longest series
+    std::vector<EventJob> jobs(n);
+    std::vector<EventWork> work;
+    int max_frames = 0;
+    for (int i = 0; i < n; ++i) {
+        dxg2_plan* p = plans[i];
+        const int64_t nbn = (int64_t)p->n_bins * p->N;
+        int64_t* sums = d_sums && d_sums[i] ? d_sums[i] : p->d_sums;
+        launch_fill_i64(sums, nbn, s);
+        launch_fill_i64(p->d_ugram, nbn, s);
+        launch_fill_i32(p->d_err, 1, s);
+        jobs[i] = EventJob{d_times[i], d_data[i], d_offsets[i], p->d_pix, p->d_bins, nnz[i],
+                           p->d_ugram, sums, p->d_err, p->N, 0};
+        if (p->n_bins == 0) continue;
+        max_frames = std::max(max_frames, p->N);
+        for (EventWork w : p->event_work) {
+            w.pad = i;
+            work.push_back(w);
+        }
+    }
+    DXG2_HIP(hipEventSynchronize(b->staged));
+    int rc = grow_table(b, &b->h_ejobs, &b->d_ejobs, &b->ejobs_cap, (size_t)n, 16, s);
+    if (!rc) rc = grow_table(b, &b->h_ework, &b->d_ework, &b->ework_cap, work.size(), 4096, s);
+    if (rc) return rc;
+    std::copy(jobs.begin(), jobs.end(), b->h_ejobs);
+    std::copy(work.begin(), work.end(), b->h_ework);
+    DXG2_HIP(hipMemcpyAsync(b->d_ejobs, b->h_ejobs, n * sizeof(EventJob), hipMemcpyHostToDevice, s));
+    if (!work.empty())
+        DXG2_HIP(hipMemcpyAsync(b->d_ework, b->h_ework, work.size() * sizeof(EventWork),
+                                hipMemcpyHostToDevice, s));
+    DXG2_HIP(hipEventRecord(b->staged, s));
+    launch_event_pairs_jobs(dtype, b->d_ejobs, b->d_ework, (int)work.size(), max_frames, s);
+    for (int i = 0; i < n; ++i) {
+        dxg2_plan* p = plans[i];
+        if (p->n_bins == 0) continue;
+        launch_finalize(jobs[i].sums, jobs[i].sums, p->d_ugram, p->d_bins, p->n_bins, p->N, 0,
+                        p->d_prefix, p->d_smax, p->d_dpre, nullptr, d_num ? d_num[i] : nullptr,
+                        d_den ? d_den[i] : nullptr, d_g2[i], s);
+    }
+    DXG2_HIP(hipGetLastError());
+    return DXG2_OK;
+}
+
 int dxg2_correlate_dense_plain(dxg2_plan* p, const void* d_frames, float* d_g2, void* stream) {
     if (!p) return set_error(DXG2_EINVAL, "null plan");
     if (!d_frames || !d_g2) return set_error(DXG2_EINVAL, "frames and g2 must be non-null");
@@ -1624,14 +1783,18 @@ int dxg2_correlate_dense_plain(dxg2_plan* p, const void* d_frames, float* d_g2, 
 
 // Shared tail of both event entry points: pairs + sums from pixel-major CSR, then the
 // exact denominator and g2 (no bias: counts are used as they are).
+// With profiling on, `ev` holds the call's events: [0] and [1] are already recorded
+// around the input conversion (none for CSR input), [2] follows the pairs, [3] the rest.
 static int events_from_csr(dxg2_plan* p, const int32_t* times, const void* data,
                            const uint32_t* offsets, int64_t nnz, int64_t* d_num, int64_t* d_sums,
-                           double* d_den, float* d_g2, hipStream_t s) {
+                           double* d_den, float* d_g2, hipStream_t s, hipEvent_t* ev) {
     int64_t* sums = d_sums ? d_sums : p->d_sums;
     launch_event_pairs(p->dtype, times, data, offsets, p->d_pix, p->d_bins, p->d_ework,
                        (int)p->event_work.size(), p->N, nnz, p->d_ugram, sums, p->d_err, s);
+    if (ev) DXG2_HIP(hipEventRecord(ev[2], s));
     launch_finalize(sums, sums, p->d_ugram, p->d_bins, p->n_bins, p->N, 0, p->d_prefix,
                     p->d_smax, p->d_dpre, nullptr, d_num, d_den, d_g2, s);
+    if (ev) DXG2_HIP(hipEventRecord(ev[3], s));
     DXG2_HIP(hipGetLastError());
     return DXG2_OK;
 }
@@ -1653,7 +1816,14 @@ int dxg2_correlate_events_csr(dxg2_plan* p, const int32_t* d_times, const void* 
     launch_fill_i64(d_sums ? d_sums : p->d_sums, nbn, s);
     launch_fill_i64(p->d_ugram, nbn, s);
     launch_fill_i32(p->d_err, 1, s);
-    return events_from_csr(p, d_times, d_data, d_offsets, nnz, d_num, d_sums, d_den, d_g2, s);
+    hipEvent_t* ev = nullptr;
+    if (p->profiling && (rc = next_events(p, &ev))) return rc;
+    if (ev) {
+        DXG2_HIP(hipEventRecord(ev[0], s));
+        DXG2_HIP(hipEventRecord(ev[1], s));
+    }
+    return events_from_csr(p, d_times, d_data, d_offsets, nnz, d_num, d_sums, d_den, d_g2, s,
+                           ev);
 }
 
 int dxg2_correlate_events_coo(dxg2_plan* p, const uint32_t* d_pixel, const uint32_t* d_frame,
@@ -1675,12 +1845,16 @@ int dxg2_correlate_events_coo(dxg2_plan* p, const uint32_t* d_pixel, const uint3
     launch_fill_i64(d_sums ? d_sums : p->d_sums, nbn, s);
     launch_fill_i64(p->d_ugram, nbn, s);
     launch_fill_i32(p->d_err, 1, s);
+    hipEvent_t* ev = nullptr;
+    if (p->profiling && (rc = next_events(p, &ev))) return rc;
+    if (ev) DXG2_HIP(hipEventRecord(ev[0], s));
     if (launch_coo_to_csr(p->dtype, d_pixel, d_frame, d_count, nnz,
                           (uint32_t)((int64_t)p->H * p->W), p->N, p->d_off, p->coo_sort,
                           p->d_etimes, p->d_edata, p->d_err, s) != hipSuccess)
         return set_error(DXG2_EHIP, "sort of %lld COO events failed", (long long)nnz);
+    if (ev) DXG2_HIP(hipEventRecord(ev[1], s));
     return events_from_csr(p, p->d_etimes, p->d_edata, p->d_off, nnz, d_num, d_sums, d_den,
-                           d_g2, s);
+                           d_g2, s, ev);
 }
 
 int dxg2_correlate_fft(dxg2_plan* p, const void* d_frames, double* d_num, int64_t* d_sums,
@@ -1698,7 +1872,7 @@ int dxg2_correlate_fft(dxg2_plan* p, const void* d_frames, double* d_num, int64_
     if (ev) DXG2_HIP(hipEventRecord(ev[0], s));
     launch_fill_i64(reinterpret_cast<int64_t*>(p->d_power), (int64_t)p->n_bins * (p->fft_L + 1), s);
     launch_gather_pm((const uint8_t*)d_frames, p->dtype, (int64_t)p->H * p->W, p->N, p->nrows,
-                     p->d_pix, p->d_bins, p->d_kbwork, (int)p->kb_work.size(), p->d_xs, s);
+                     p->d_pix, p->d_bins, p->d_kbgather, (int)p->kb_gather.size(), p->d_xs, s);
     int64_t max_npix = 0;
     for (auto n : p->npix) max_npix = std::max(max_npix, n);
     launch_pm_sums(p->d_xs, p->dtype, p->d_bins, p->n_bins, (int)max_npix, p->N, p->nrows, sums,
@@ -1962,6 +2136,40 @@ int dxg2_compress_compact(const uint32_t* d_counter, const void* d_events,
                             max_nnz, d_scratch, d_offsets, d_out_data, d_out_times,
                             (hipStream_t)stream);
     DXG2_HIP(hipGetLastError());
+    return DXG2_OK;
+}
+
+int dxg2_measure_lds_atomics(int32_t mode, int32_t iters, double* adds_per_s, void* stream) {
+    if (!adds_per_s || (mode != 0 && mode != 1) || iters <= 0)
+        return set_error(DXG2_EINVAL, "bad argument");
+    hipStream_t s = (hipStream_t)stream;
+    int dev = 0, cus = 0;
+    DXG2_HIP(hipGetDevice(&dev));
+    DXG2_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const int blocks = std::max(cus, 1) * 4 * 8;  // eight rounds of four workgroups per CU
+    uint32_t* out = nullptr;
+    DXG2_HIP(hipMalloc((void**)&out, (size_t)blocks * sizeof(uint32_t)));
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    hipError_t e = hipEventCreate(&ev[0]);
+    if (e == hipSuccess) e = hipEventCreate(&ev[1]);
+    float ms = 0.f;
+    if (e == hipSuccess) {
+        launch_lds_atomic_rate(mode, blocks, iters, out, s);  // warm-up
+        e = hipEventRecord(ev[0], s);
+    }
+    if (e == hipSuccess) {
+        launch_lds_atomic_rate(mode, blocks, iters, out, s);
+        e = hipEventRecord(ev[1], s);
+    }
+    if (e == hipSuccess) e = hipEventSynchronize(ev[1]);
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, ev[0], ev[1]);
+    for (hipEvent_t x : ev)
+        if (x) (void)hipEventDestroy(x);
+    (void)hipFree(out);
+    if (e != hipSuccess)
+        return set_error(DXG2_EHIP, "LDS atomic measurement: %s", hipGetErrorString(e));
+    *adds_per_s = (double)blocks * 256.0 * iters / (std::max(ms, 1e-6f) * 1e-3);
     return DXG2_OK;
 }
 

@@ -49,6 +49,10 @@ struct GatherWork {      // one workgroup of the gather kernel (x dimension)
 // (BAND_STAGE x srow bytes, srow <= BAND_ROW_BYTES so that two workgroups share a CU);
 // each wave keeps BAND_KMAX k-blocks in registers, a work item at most BAND_ITEM_KMAX.
 constexpr int BAND_STAGE = 8;
+// Gather work in detector order, dealt to the XCDs in chunks of this many items:
+// dense gather items (up to 16 k-blocks each) and pixel-major gather k-blocks.
+constexpr int GATHER_XCD_GROUP = 16;
+constexpr int PM_XCD_GROUP = 128;
 constexpr int BAND_ROW_BYTES = 9728;
 constexpr int BAND_KMAX = 36;
 constexpr int BAND_ITEM_KMAX = 4 * BAND_KMAX;
@@ -110,6 +114,20 @@ struct EventWork {       // one workgroup of the event pair kernel (x dimension)
     int32_t bin;
     int32_t k0;          // range [k0, k1) of the bin's pixel list
     int32_t k1;
+    int32_t pad;         // job of a grouped launch (0 otherwise)
+};
+
+struct EventJob {        // one job of a grouped event launch
+    const int32_t* times;
+    const void* data;
+    const uint32_t* offsets;
+    const int32_t* pix;
+    const BinDev* bins;
+    int64_t nnz;
+    int64_t* num;        // the job's int64 numerator accumulator (n_bins, nframes)
+    int64_t* sums;
+    int* err;
+    int32_t nframes;
     int32_t pad;
 };
 
@@ -119,6 +137,7 @@ struct KbWork {          // one k-block (64 pixels) of one bin
 };
 
 // ------------------------------------------------------------------ launchers
+void launch_lds_atomic_rate(int mode, int blocks, int iters, uint32_t* out, hipStream_t s);
 void launch_fill_i64(int64_t* p, int64_t n, hipStream_t s);
 void launch_gather_i8(const uint8_t* frames, int dtype, int64_t hw, int nframes, int nrows,
                       const int32_t* pix_idx, const BinDev* bins, const GatherWork* work,
@@ -165,6 +184,10 @@ void launch_finalize(const int64_t* plane_sums, int64_t* sums, const int64_t* ug
 void launch_plain_bin(const uint8_t* frames, int dtype, int64_t hw, int nframes,
                       const int32_t* pix, int npix, float* a, float* gram, float* means,
                       float* g2_row, hipStream_t s);
+// Lags per workgroup of the event pair kernel (its lag window) for a count type.
+int event_lag_window(int dtype);
+void launch_event_pairs_jobs(int dtype, const EventJob* jobs, const EventWork* work, int n_work,
+                             int max_frames, hipStream_t s);
 void launch_event_pairs(int dtype, const int32_t* times, const void* data,
                         const uint32_t* offsets, const int32_t* pix_idx, const BinDev* bins,
                         const EventWork* work, int n_work, int nframes, int64_t nnz,

@@ -30,19 +30,24 @@ namespace dxg2 {
 constexpr int ECAP = 256;       // events of one pixel staged in LDS per wave
 constexpr int EPF = ECAP / 64;  // event rounds of the next pixel prefetched in registers
 
-// LDS lag histogram of one work item: 32 KiB, so that four workgroups (16 waves, enough
-// to cover the dependent LDS reads of the searches) share a CU with their staging.
-//  * 8-bit counts: 8192 lags of 32-bit counters.  A pixel adds at most sum_k d_k^2 to
+// LDS lag histogram of one work item, shared by the NW waves of its workgroup: 8 KiB of
+// counters per wave, so that with NW = 4 four workgroups (16 waves, enough to cover the
+// dependent LDS reads of the searches) share a CU with their staging, and with NW = 8
+// two workgroups of twice the lag window do.
+//  * 8-bit counts: 2048 NW lags of 32-bit counters.  A pixel adds at most sum_k d_k^2 to
 //    any one lag (Cauchy-Schwarz: at a fixed lag each event pairs with at most one
 //    partner), so each wave adds to the histogram only while its pixels' sum of d^2
-//    stays below `budget`, and sends the pairs of a pixel beyond it straight to the
-//    global 64-bit numerators.  Four waves then bound every counter below 2^32 (uint8)
-//    or its magnitude below 2^31 (int8, two's complement), so it is exact.
-//  * uint16 counts: 4096 lags of 64-bit counters, exact for any data.
+//    stays below budget(NW) = 2^32 / NW (uint8) or 2^31 / NW (int8), and sends the pairs
+//    of a pixel beyond it straight to the global 64-bit numerators.  The NW waves then
+//    bound every counter below 2^32 (uint8) or its magnitude below 2^31 (int8, two's
+//    complement), so it is exact.
+//  * uint16 counts: 1024 NW lags of 64-bit counters, exact for any data.
 template <typename T> struct EvHist {
     using type = uint32_t;
-    static constexpr int ew = 8192;
-    static constexpr long long budget = std::is_signed<T>::value ? (1ll << 29) : (1ll << 30);
+    static constexpr int lags_per_wave = 2048;
+    static constexpr long long budget(int nw) {
+        return (std::is_signed<T>::value ? (1ll << 31) : (1ll << 32)) / nw;
+    }
     static __device__ __forceinline__ unsigned long long widen(uint32_t v) {
         return std::is_signed<T>::value ? (unsigned long long)(long long)(int32_t)v
                                         : (unsigned long long)v;
@@ -50,8 +55,8 @@ template <typename T> struct EvHist {
 };
 template <> struct EvHist<uint16_t> {
     using type = unsigned long long;
-    static constexpr int ew = 4096;
-    static constexpr long long budget = 0;  // no bound needed
+    static constexpr int lags_per_wave = 1024;
+    static constexpr long long budget(int) { return 0; }  // no bound needed
     static __device__ __forceinline__ unsigned long long widen(unsigned long long v) { return v; }
 };
 
@@ -71,37 +76,39 @@ struct EvPixel {
     int n;  // events to correlate: 0 for an empty, unbinned or invalid range
 };
 
-template <typename T>
-__global__ __launch_bounds__(256) void event_pairs_kernel(
+// The pairs of one work item (a run of one bin's pixels) in lag window blockIdx.y, with
+// the workgroup's LDS: the lag histogram and each wave's staged events (uint8, int8 and
+// uint16 counts fit 32 bits).
+template <typename T, int NW>
+__device__ __forceinline__ void event_pairs_item(
     const int32_t* __restrict__ times, const T* __restrict__ data,
     const uint32_t* __restrict__ offsets, const int32_t* __restrict__ pix_idx,
-    const BinDev* __restrict__ bins, const EventWork* __restrict__ work, int nframes,
-    int64_t nnz, int64_t* __restrict__ num, int64_t* __restrict__ sums, int* __restrict__ err) {
+    const BinDev* __restrict__ bins, const EventWork w, int nframes, int64_t nnz,
+    int64_t* __restrict__ num, int64_t* __restrict__ sums, int* __restrict__ err,
+    typename EvHist<T>::type* __restrict__ hist, int (*__restrict__ st_t)[ECAP],
+    int (*__restrict__ st_d)[ECAP]) {
     using H = typename EvHist<T>::type;
-    constexpr int EW = EvHist<T>::ew;
+    constexpr int EW = EvHist<T>::lags_per_wave * NW;
+    constexpr int NT = 64 * NW;
     constexpr bool BOUNDED = sizeof(H) == 4;
-    __shared__ H hist[EW];
-    __shared__ int st_t[4][ECAP];
-    __shared__ int st_d[4][ECAP];  // uint8, int8 and uint16 counts fit 32 bits
-    const EventWork w = work[blockIdx.x];
     const BinDev bd = bins[w.bin];
     const int w0 = blockIdx.y * EW;
     if (w0 >= nframes) return;
     const int w1 = min(nframes, w0 + EW);
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    for (int k = threadIdx.x; k < EW; k += 256) hist[k] = 0;
+    for (int k = threadIdx.x; k < EW; k += NT) hist[k] = 0;
     __syncthreads();
     int64_t* srow = sums + (int64_t)w.bin * nframes;
     unsigned long long* nwin = reinterpret_cast<unsigned long long*>(num + (int64_t)w.bin * nframes + w0);
     long long used = 0;  // this wave's sum of d^2 added to the histogram (BOUNDED)
 
-    // The wave takes pixels w.k0 + wave + 4r, r < nr.  Their list entries and offsets are
+    // The wave takes pixels w.k0 + wave + NW r, r < nr.  Their list entries and offsets are
     // loaded 64 at a time (lane r % 64 holds pixel r), and while the pairs of pixel r are
     // counted, the first ECAP events of pixel r + 1 are already loading into registers,
     // so the chain list entry -> offsets -> events is not waited for once per pixel.
     const int npx = w.k1 - w.k0 - wave;
-    const int nr = npx > 0 ? (npx + 3) / 4 : 0;
+    const int nr = npx > 0 ? (npx + NW - 1) / NW : 0;
     int mp = -1;
     uint32_t mb = 0u, me = 0u;
     auto load_meta = [&](int r0) {
@@ -109,7 +116,7 @@ __global__ __launch_bounds__(256) void event_pairs_kernel(
         mp = -1;
         mb = me = 0u;
         if (r < nr) {
-            mp = pix_idx[bd.idx_off + w.k0 + wave + 4 * r];
+            mp = pix_idx[bd.idx_off + w.k0 + wave + NW * r];
             if (mp >= 0) {
                 mb = offsets[mp];
                 me = offsets[mp + 1];
@@ -146,10 +153,11 @@ __global__ __launch_bounds__(256) void event_pairs_kernel(
         prefetch(cur);
     }
     for (int r = 0; r < nr; ++r) {
-        const int n = cur.n;
-        const uint32_t beg = cur.beg;
+        // wave-uniform (the compiler keeps them, and the loops they bound, scalar)
+        const int n = __builtin_amdgcn_readfirstlane(cur.n);
+        const uint32_t beg = (uint32_t)__builtin_amdgcn_readfirstlane((int)cur.beg);
         const bool staged = n <= ECAP;
-        long long sq = 0;  // this lane's share of the pixel's sum of d^2 (BOUNDED)
+        long long sq = 0;  // this lane's share of the pixel's sum of d^2
         if (n > 0 && staged) {
 #pragma unroll
             for (int q = 0; q < EPF; ++q) {
@@ -157,7 +165,7 @@ __global__ __launch_bounds__(256) void event_pairs_kernel(
                 if (k < n) {
                     st_t[wave][k] = ct[q];
                     st_d[wave][k] = cd[q];
-                    if (BOUNDED) sq += (long long)cd[q] * cd[q];
+                    sq += (long long)cd[q] * cd[q];
                 }
             }
         }
@@ -170,104 +178,257 @@ __global__ __launch_bounds__(256) void event_pairs_kernel(
         if (n == 0) continue;
         __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): staged values visible to the wave
         __builtin_amdgcn_wave_barrier();
-        const int* tt = staged ? st_t[wave] : times + beg;
-        // validate: frames in [0, N) and strictly increasing
-        int bad = 0;
-        for (int k = lane; k < n; k += 64) {
-            const int t = tt[k];
-            if (t < 0 || t >= nframes) bad = 1;
-            if (k + 1 < n && tt[k + 1] <= t) bad = 1;
-        }
-        if (bad) atomicOr(err, 1);
-        if (__any(bad)) continue;
-        bool to_lds = true;
-        if constexpr (BOUNDED) {
-            if (!staged)
-                for (int k = lane; k < n; k += 64) {
-                    const long long d = as_count<T>(data[beg + k]);
-                    sq += d * d;
-                }
-            sq = wave_sum64(sq);
-            to_lds = used + sq < EvHist<T>::budget;
-            if (to_lds) used += sq;
-        }
-        auto add = [&](int lag, long long prod) {  // lag in [0, w1 - w0)
-            if (to_lds)
-                atomicAdd(&hist[lag], (H)prod);
-            else
-                atomicAdd(nwin + lag, (unsigned long long)prod);
-        };
-        // pairs (i, j), j > i, whose lag t_j - t_i falls in [max(w0, 1), w1): partners
-        // from the first j with t_j >= t_i + max(w0, 1) while t_j < t_i + w1, four per
-        // step so that their reads and atomics overlap.
-        auto partners = [&](int ti, long long di, int j) {
-            const int lim = ti + w1;
-            for (; j < n; j += 4) {
-                int tj[4];
-                long long dj[4];
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const bool in = j + u < n;
-                    tj[u] = in ? tt[j + u] : lim;
-                    dj[u] = !in ? 0ll
-                                : staged ? (long long)st_d[wave][j + u]
-                                         : as_count<T>(data[beg + j + u]);
-                }
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const long long prod = di * dj[u];
-                    if (tj[u] < lim && prod != 0) add(tj[u] - ti - w0, prod);
-                }
-                if (tj[3] >= lim) break;  // times increase: the rest lie past the window
-            }
-        };
-        // Each lane takes events i and i + 64 together, so that their two searches for
-        // the first partner overlap.
-        for (int i0 = lane; i0 < n; i0 += 128) {
-            const int i1 = i0 + 64;
-            const bool has1 = i1 < n;
-            const int ti0 = tt[i0];
-            const int ti1 = has1 ? tt[i1] : 0;
-            const long long d0 = staged ? (long long)st_d[wave][i0] : as_count<T>(data[beg + i0]);
-            const long long d1 = !has1 ? 0ll
-                                       : staged ? (long long)st_d[wave][i1]
-                                                : as_count<T>(data[beg + i1]);
+        // The pixel's pairs, with its events read from LDS (staged) or from HBM; the two
+        // are separate instantiations so that each reads through its own address space.
+        auto pixel = [&](auto staged_tag) {
+            constexpr bool S = decltype(staged_tag)::value;
+            auto time_of = [&](int k) -> int {
+                if constexpr (S) return st_t[wave][k];
+                else return times[beg + k];
+            };
+            // 8-bit counts multiply in 32 bits (|d_i d_j| <= 255^2), 16-bit in 64
+            using C = typename std::conditional<sizeof(T) == 1, int, long long>::type;
+            auto count_of = [&](int k) -> C {
+                if constexpr (S) return (C)st_d[wave][k];
+                else return (C)as_count<T>(data[beg + k]);
+            };
+            // Validation, in the first lag window (the one that also writes the frame
+            // sums): frames in [0, N) and strictly increasing.  An invalid pixel is
+            // skipped there and flagged, and the call reports the error; the other
+            // windows do not repeat the check, and their lag test below keeps every
+            // write inside the window whatever the input.
             if (w0 == 0) {
-                add(0, d0 * d0 + d1 * d1);
-                if (d0 != 0)
-                    atomicAdd(reinterpret_cast<unsigned long long*>(srow + ti0),
-                              (unsigned long long)d0);
-                if (d1 != 0)
-                    atomicAdd(reinterpret_cast<unsigned long long*>(srow + ti1),
-                              (unsigned long long)d1);
+                int bad = 0;
+                for (int k = lane; k < n; k += 64) {
+                    const int t = time_of(k);
+                    if (t < 0 || t >= nframes) bad = 1;
+                    if (k + 1 < n && time_of(k + 1) <= t) bad = 1;
+                }
+                if (bad) atomicOr(err, 1);
+                if (__any(bad)) return;
             }
-            // first partner: i + 1 in the first window (times strictly increase), else
-            // the first j > i with t_j - t_i >= w0
-            int lo0 = i0 + 1, lo1 = i1 + 1;
-            if (w0 > 0) {
-                int hi0 = n, hi1 = has1 ? n : lo1;
-                const int g0 = ti0 + w0, g1 = ti1 + w0;
-                while (lo0 < hi0 || lo1 < hi1) {
-                    if (lo0 < hi0) {
-                        const int mid = (lo0 + hi0) >> 1;
-                        if (tt[mid] < g0) lo0 = mid + 1; else hi0 = mid;
+            long long sq_lane = sq;
+            if constexpr (!S)
+                for (int k = lane; k < n; k += 64) {
+                    const long long d = count_of(k);
+                    sq_lane += d * d;
+                }
+            const long long sqw = wave_sum64(sq_lane);  // sum of d^2: the lag-0 term
+            bool to_lds = true;
+            if constexpr (BOUNDED) {
+                to_lds = used + sqw < EvHist<T>::budget(NW);
+                if (to_lds) used += sqw;
+            }
+            if (w0 == 0 && lane == 0 && sqw != 0) {  // lag 0
+                if (to_lds)
+                    atomicAdd(&hist[0], (H)sqw);
+                else
+                    atomicAdd(nwin, (unsigned long long)sqw);
+            }
+            const unsigned span = (unsigned)(w1 - w0);
+            // the rest in one of two instantiations: LDS counters, or (past the budget)
+            // the global 64-bit numerators, so the pair loop carries no branch on it
+            auto events = [&](auto lds_tag) {
+            constexpr bool LDS = decltype(lds_tag)::value;
+            auto add = [&](int lag, C prod) {  // lag in [0, w1 - w0)
+                if constexpr (LDS)
+                    atomicAdd(&hist[lag], (H)prod);
+                else
+                    atomicAdd(nwin + lag, (unsigned long long)(long long)prod);
+            };
+            // pairs (i, j), j > i, with lag t_j - t_i in [max(w0, 1), w1): partners from
+            // the first j with t_j >= t_i + max(w0, 1) while t_j < t_i + w1, four per
+            // step so that their reads and atomics overlap.  The lag test is one unsigned
+            // compare, which also rejects a negative lag.
+            auto partners = [&](int ti, C di, int j) {
+                const int base = ti + w0;
+                for (; j < n; j += 4) {
+                    int tj[4];
+                    C dj[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const bool in = j + u < n;
+                        const int jj = in ? j + u : n - 1;
+                        const int t = time_of(jj);
+                        const C d = count_of(jj);
+                        tj[u] = in ? t : base + (int)span;
+                        dj[u] = in ? d : (C)0;
                     }
-                    if (lo1 < hi1) {
-                        const int mid = (lo1 + hi1) >> 1;
-                        if (tt[mid] < g1) lo1 = mid + 1; else hi1 = mid;
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const unsigned lag = (unsigned)(tj[u] - base);
+                        const C prod = di * dj[u];
+                        if (lag < span && prod != 0) add((int)lag, prod);
+                    }
+                    if (tj[3] - base >= (int)span) break;  // times increase: the rest are later
+                }
+            };
+            // wave-uniform steps of the first-partner search: powers of two from the
+            // largest not above n
+            const int top = 1 << (31 - __builtin_clz(n));
+            // Each lane takes events i and i + 64 together, so that their two searches
+            // for the first partner overlap.
+            for (int i0 = lane; i0 < n; i0 += 128) {
+                const int i1 = i0 + 64;
+                const bool has1 = i1 < n;
+                const int ti0 = time_of(i0);
+                const int ti1 = has1 ? time_of(i1) : 0;
+                const C d0 = count_of(i0);
+                const C d1 = has1 ? count_of(i1) : (C)0;
+                if (w0 == 0) {
+                    if (d0 != 0)
+                        atomicAdd(reinterpret_cast<unsigned long long*>(srow + ti0),
+                                  (unsigned long long)(long long)d0);
+                    if (d1 != 0)
+                        atomicAdd(reinterpret_cast<unsigned long long*>(srow + ti1),
+                                  (unsigned long long)(long long)d1);
+                }
+                // first partner: i + 1 in the first window (times strictly increase),
+                // else one past the last p >= i with t_p < t_i + w0, by a descent over
+                // power-of-two steps (a monotone test; reads clamped inside the list)
+                int p0 = i0, p1 = has1 ? i1 : n - 1;
+                if (w0 > 0) {
+                    const int g0 = ti0 + w0, g1 = ti1 + w0;
+                    for (int step = top; step > 0; step >>= 1) {
+                        const int q0 = p0 + step, q1 = p1 + step;
+                        const int t0 = time_of(min(q0, n - 1));
+                        const int t1 = time_of(min(q1, n - 1));
+                        if (q0 < n && t0 < g0) p0 = q0;
+                        if (q1 < n && t1 < g1) p1 = q1;
                     }
                 }
+                partners(ti0, d0, p0 + 1);
+                if (has1) partners(ti1, d1, p1 + 1);
             }
-            partners(ti0, d0, lo0);
-            if (has1) partners(ti1, d1, lo1);
-        }
+            };
+            if (to_lds)
+                events(std::true_type{});
+            else
+                events(std::false_type{});
+        };
+        if (staged)
+            pixel(std::true_type{});
+        else
+            pixel(std::false_type{});
         __builtin_amdgcn_wave_barrier();
     }
     __syncthreads();
-    for (int k = threadIdx.x; k < w1 - w0; k += 256) {
+    for (int k = threadIdx.x; k < w1 - w0; k += NT) {
         const H v = hist[k];
         if (v != 0) atomicAdd(nwin + k, EvHist<T>::widen(v));
     }
+}
+
+template <typename T, int NW>
+__global__ __launch_bounds__(64 * NW) void event_pairs_kernel(
+    const int32_t* __restrict__ times, const T* __restrict__ data,
+    const uint32_t* __restrict__ offsets, const int32_t* __restrict__ pix_idx,
+    const BinDev* __restrict__ bins, const EventWork* __restrict__ work, int nframes,
+    int64_t nnz, int64_t* __restrict__ num, int64_t* __restrict__ sums, int* __restrict__ err) {
+    __shared__ typename EvHist<T>::type hist[EvHist<T>::lags_per_wave * NW];
+    __shared__ int st_t[NW][ECAP];
+    __shared__ int st_d[NW][ECAP];
+    event_pairs_item<T, NW>(times, data, offsets, pix_idx, bins, work[blockIdx.x], nframes, nnz,
+                            num, sums, err, hist, st_t, st_d);
+}
+
+// A pointer read from a job table, marked as global memory (a generic pointer would
+// turn every access into a flat one).
+template <typename P>
+__device__ __forceinline__ P* global_ptr(P* p) {
+    using G = __attribute__((address_space(1))) P*;
+    return (P*)reinterpret_cast<G>(reinterpret_cast<uintptr_t>(p));
+}
+
+// Several jobs in one launch: work item w belongs to job w.pad (padding items: bin < 0);
+// lag windows past a job's frame count return at once.
+template <typename T, int NW>
+__global__ __launch_bounds__(64 * NW) void event_pairs_jobs_kernel(
+    const EventJob* __restrict__ jobs, const EventWork* __restrict__ work) {
+    __shared__ typename EvHist<T>::type hist[EvHist<T>::lags_per_wave * NW];
+    __shared__ int st_t[NW][ECAP];
+    __shared__ int st_d[NW][ECAP];
+    const EventWork w = work[blockIdx.x];
+    if (w.bin < 0) return;
+    const EventJob j = jobs[w.pad];
+    event_pairs_item<T, NW>(global_ptr(j.times), global_ptr(reinterpret_cast<const T*>(j.data)),
+                            global_ptr(j.offsets), global_ptr(j.pix), global_ptr(j.bins), w,
+                            j.nframes, j.nnz, global_ptr(j.num), global_ptr(j.sums),
+                            global_ptr(j.err), hist, st_t, st_d);
+}
+
+// Waves per workgroup of the pair kernels: 16 (one workgroup per CU, 32,768-lag windows
+// for 8-bit counts: the fewest windows, so the least per-window work per pixel), or
+// DXG2_EVENT_WAVES = 8 (two per CU, 16,384 lags) or 4 (four per CU, 8,192 lags).
+// Measured on the events workload (1030x1065 x 100,000 frames): pairs 8.4 ms at 16
+// waves, 9.8 ms at 8, 13.2 ms at 4 (largest work items each).
+int event_waves() {
+    static const int nw = [] {
+        const char* v = getenv("DXG2_EVENT_WAVES");
+        const int x = v ? atoi(v) : 16;
+        return (x == 4 || x == 8) ? x : 16;
+    }();
+    return nw;
+}
+
+int event_lag_window(int dtype) {
+    return (dtype == DXG2_U16 ? EvHist<uint16_t>::lags_per_wave : EvHist<uint8_t>::lags_per_wave) *
+           event_waves();
+}
+
+template <typename T, int NW>
+void pairs_jobs_launch(const EventJob* jobs, const EventWork* work, int n_work, int max_frames,
+                       hipStream_t s) {
+    constexpr int EW = EvHist<T>::lags_per_wave * NW;
+    event_pairs_jobs_kernel<T, NW><<<dim3((unsigned)n_work, (unsigned)((max_frames + EW - 1) / EW)),
+                                     64 * NW, 0, s>>>(jobs, work);
+}
+
+template <typename T>
+void pairs_jobs_typed(const EventJob* jobs, const EventWork* work, int n_work, int max_frames,
+                      hipStream_t s) {
+    if (event_waves() == 4)
+        pairs_jobs_launch<T, 4>(jobs, work, n_work, max_frames, s);
+    else if (event_waves() == 16)
+        pairs_jobs_launch<T, 16>(jobs, work, n_work, max_frames, s);
+    else
+        pairs_jobs_launch<T, 8>(jobs, work, n_work, max_frames, s);
+}
+
+void launch_event_pairs_jobs(int dtype, const EventJob* jobs, const EventWork* work, int n_work,
+                             int max_frames, hipStream_t s) {
+    if (n_work <= 0 || max_frames <= 0) return;
+    if (dtype == DXG2_U16)
+        pairs_jobs_typed<uint16_t>(jobs, work, n_work, max_frames, s);
+    else if (dtype == DXG2_I8)
+        pairs_jobs_typed<int8_t>(jobs, work, n_work, max_frames, s);
+    else
+        pairs_jobs_typed<uint8_t>(jobs, work, n_work, max_frames, s);
+}
+
+template <typename T, int NW>
+void pairs_launch(const int32_t* times, const void* data, const uint32_t* offsets,
+                  const int32_t* pix_idx, const BinDev* bins, const EventWork* work, int n_work,
+                  int nframes, int64_t nnz, int64_t* num, int64_t* sums, int* err, hipStream_t s) {
+    constexpr int EW = EvHist<T>::lags_per_wave * NW;
+    event_pairs_kernel<T, NW><<<dim3((unsigned)n_work, (unsigned)((nframes + EW - 1) / EW)),
+                                64 * NW, 0, s>>>(times, (const T*)data, offsets, pix_idx, bins,
+                                                 work, nframes, nnz, num, sums, err);
+}
+
+template <typename T>
+void pairs_typed(const int32_t* times, const void* data, const uint32_t* offsets,
+                 const int32_t* pix_idx, const BinDev* bins, const EventWork* work, int n_work,
+                 int nframes, int64_t nnz, int64_t* num, int64_t* sums, int* err, hipStream_t s) {
+    if (event_waves() == 4)
+        pairs_launch<T, 4>(times, data, offsets, pix_idx, bins, work, n_work, nframes, nnz, num,
+                           sums, err, s);
+    else if (event_waves() == 16)
+        pairs_launch<T, 16>(times, data, offsets, pix_idx, bins, work, n_work, nframes, nnz, num,
+                            sums, err, s);
+    else
+        pairs_launch<T, 8>(times, data, offsets, pix_idx, bins, work, n_work, nframes, nnz, num,
+                           sums, err, s);
 }
 
 void launch_event_pairs(int dtype, const int32_t* times, const void* data,
@@ -275,19 +436,51 @@ void launch_event_pairs(int dtype, const int32_t* times, const void* data,
                         const EventWork* work, int n_work, int nframes, int64_t nnz,
                         int64_t* num, int64_t* sums, int* err, hipStream_t s) {
     if (n_work <= 0 || nframes <= 0) return;
-    auto grid = [&](int ew) { return dim3((unsigned)n_work, (unsigned)((nframes + ew - 1) / ew)); };
     if (dtype == DXG2_U16)
-        event_pairs_kernel<uint16_t><<<grid(EvHist<uint16_t>::ew), 256, 0, s>>>(times, (const uint16_t*)data, offsets,
-                                                          pix_idx, bins, work, nframes, nnz,
-                                                          num, sums, err);
+        pairs_typed<uint16_t>(times, data, offsets, pix_idx, bins, work, n_work, nframes, nnz,
+                              num, sums, err, s);
     else if (dtype == DXG2_I8)
-        event_pairs_kernel<int8_t><<<grid(EvHist<int8_t>::ew), 256, 0, s>>>(times, (const int8_t*)data, offsets,
-                                                        pix_idx, bins, work, nframes, nnz, num,
-                                                        sums, err);
+        pairs_typed<int8_t>(times, data, offsets, pix_idx, bins, work, n_work, nframes, nnz, num,
+                            sums, err, s);
     else
-        event_pairs_kernel<uint8_t><<<grid(EvHist<uint8_t>::ew), 256, 0, s>>>(times, (const uint8_t*)data, offsets,
-                                                         pix_idx, bins, work, nframes, nnz,
-                                                         num, sums, err);
+        pairs_typed<uint8_t>(times, data, offsets, pix_idx, bins, work, n_work, nframes, nnz,
+                             num, sums, err, s);
+}
+
+// LDS atomic-add throughput, the ceiling the pair kernel's histogram updates are measured
+// against: workgroups of 256 threads with that kernel's LDS footprint (8192 32-bit
+// counters and 8 KiB of staging, four workgroups per CU), each lane adding 1 `iters`
+// times at pseudo-random counters (mode 0; a linear congruential sequence per lane) or
+// at lane-consecutive ones (mode 1).
+template <int MODE>
+__global__ __launch_bounds__(256) void lds_atomic_rate_kernel(uint32_t* __restrict__ out,
+                                                              int iters) {
+    __shared__ uint32_t h[(EvHist<uint8_t>::lags_per_wave * 4)];
+    __shared__ int stage_share[2 * 4 * ECAP];  // the staging arrays of four waves
+    for (int i = threadIdx.x; i < (EvHist<uint8_t>::lags_per_wave * 4); i += 256) h[i] = 0;
+    if (threadIdx.x < 8) stage_share[threadIdx.x] = 0;
+    __syncthreads();
+    uint32_t x = blockIdx.x * 2654435761u + threadIdx.x * 40503u + 1u;
+    for (int i = 0; i < iters; ++i) {
+        uint32_t a;
+        if (MODE == 0) {
+            x = x * 1664525u + 1013904223u;
+            a = x >> 19;  // 0 .. 8191
+        } else {
+            a = (threadIdx.x + 64u * (uint32_t)i) & ((EvHist<uint8_t>::lags_per_wave * 4) - 1);
+        }
+        atomicAdd(&h[a], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+        out[blockIdx.x] = h[blockIdx.x & ((EvHist<uint8_t>::lags_per_wave * 4) - 1)] + (uint32_t)stage_share[0];
+}
+
+void launch_lds_atomic_rate(int mode, int blocks, int iters, uint32_t* out, hipStream_t s) {
+    if (mode == 0)
+        lds_atomic_rate_kernel<0><<<blocks, 256, 0, s>>>(out, iters);
+    else
+        lds_atomic_rate_kernel<1><<<blocks, 256, 0, s>>>(out, iters);
 }
 
 // ----------------------------------------------------------------- COO -> CSR

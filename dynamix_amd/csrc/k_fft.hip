@@ -200,8 +200,13 @@ struct Elem<uint16_t> {
 };
 
 // --------------------------------------------------------------- pixel-major gather
-// grid (bin k-blocks, frame tiles of 256).  Lane = pixel j of the k-block; each wave
-// stages 64 frames, the LDS tile is [64 pixels][256 frames + pad].
+// grid (k-blocks in gather order, frame tiles of 256).  Lane = pixel of the k-block: each
+// load instruction reads one frame of the 64 pixels (the k-block's stretch of detector
+// row).  A lane packs 4 / sizeof(E) consecutive frames into one 32-bit word and writes
+// it to its pixel's row of an LDS tile whose row stride is an odd number of words, so
+// the 64 lanes' column writes fall in distinct banks; then each thread reads 16
+// contiguous bytes of one row and stores them to xs, so consecutive threads write
+// consecutive 16-byte pieces of a pixel's series.
 template <typename E>
 __global__ __launch_bounds__(256) void gather_pm_kernel(const uint8_t* __restrict__ frames,
                                                         int64_t hw, int nframes, int nrows,
@@ -210,34 +215,40 @@ __global__ __launch_bounds__(256) void gather_pm_kernel(const uint8_t* __restric
                                                         const KbWork* __restrict__ work,
                                                         uint8_t* __restrict__ xs) {
     constexpr int ES = sizeof(E);
-    __shared__ E tile[64][256 + 16 / ES];
+    constexpr int FPW = 4 / ES;        // frames per word
+    constexpr int WPR = 256 / FPW;     // words per pixel row of the tile
+    constexpr int STRIDE = WPR + 1;    // odd
+    __shared__ uint32_t tile[KB * STRIDE];
     const KbWork w = work[blockIdx.x];
+    if (w.kb < 0) return;  // padding of the gather order
     const BinDev bd = bins[w.bin];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int t0 = blockIdx.y * 256;
     const int p = pix_idx[bd.idx_off + (int64_t)w.kb * KB + lane];
-    const int pc = p >= 0 ? p : 0;
-    const E* fr = reinterpret_cast<const E*>(frames);
-    for (int f = wave; f < 256; f += 4) {
-        const int t = t0 + f;
-        tile[lane][f] = (t < nframes && p >= 0) ? fr[(int64_t)t * hw + pc] : (E)0;
+    const E* fr = reinterpret_cast<const E*>(frames) + (p >= 0 ? p : 0);
+    const int nt = p >= 0 ? max(0, min(256, nframes - t0)) : 0;  // frames present
+    constexpr uint32_t MASK = ES == 1 ? 0xFFu : 0xFFFFu;
+#pragma unroll
+    for (int i = 0; i < WPR / 4; ++i) {
+        const int k = wave + 4 * i;  // word k: frames t0 + FPW k ..
+        uint32_t word = 0u;
+#pragma unroll
+        for (int q = 0; q < FPW; ++q) {
+            const int f = FPW * k + q;
+            const uint32_t v = f < nt ? (uint32_t)fr[(int64_t)(t0 + f) * hw] : 0u;
+            word |= (v & MASK) << (8 * ES * q);
+        }
+        tile[lane * STRIDE + k] = word;
     }
     __syncthreads();
-    // pixel row j gets 256 * ES contiguous bytes, written 16 bytes per thread
-    constexpr int CHUNKS = 16 * ES;            // 16-byte chunks per row
-    for (int c = threadIdx.x; c < 64 * CHUNKS; c += 256) {
+    constexpr int CHUNKS = WPR / 4;  // 16-byte pieces per pixel row
+    for (int c = threadIdx.x; c < KB * CHUNKS; c += 256) {
         const int j = c / CHUNKS;
-        const int o = (c % CHUNKS) * (16 / ES);    // first element of the chunk
-        uint4 v;
-        uint8_t* vb = reinterpret_cast<uint8_t*>(&v);
-#pragma unroll
-        for (int q = 0; q < 16 / ES; ++q) {
-            const E e = tile[j][o + q];
-            __builtin_memcpy(vb + q * ES, &e, ES);
-        }
-        uint8_t* dst = xs + bd.slab_off + (((int64_t)w.kb * KB + j) * nrows + t0 + o) * ES;
-        *reinterpret_cast<uint4*>(dst) = v;
+        const int o = (c % CHUNKS) * 4;
+        const uint32_t* r = tile + j * STRIDE + o;
+        uint8_t* dst = xs + bd.slab_off + (((int64_t)w.kb * KB + j) * nrows + t0) * ES + 4 * o;
+        *reinterpret_cast<uint4*>(dst) = make_uint4(r[0], r[1], r[2], r[3]);
     }
 }
 

@@ -6,9 +6,13 @@
 // pixel walks the frames in order, so writes of a frame are coalesced across threads.
 //
 // Random numbers: Philox-4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers:
-// as easy as 1, 2, 3", SC 2011), counter = (pixel, frame + 1, 0, 0), key = seed.
+// as easy as 1, 2, 3", SC 2011), counter = (pixel, frame + 1, 0, 0), key = seed; each
+// round's two 32x32 -> 64-bit products are single v_mad_u64_u32 instructions.
 // Normals by the Box-Muller transform; Poisson by sequential inversion for lambda < 32
-// and by a rounded normal approximation above (only the hot-pixel population).
+// and by a rounded normal approximation above (only the hot-pixel population).  The
+// transcendentals are the hardware's (v_log_f32, v_exp_f32, v_sqrt_f32 and
+// v_sin_f32 / v_cos_f32, whose argument is in turns): a test-data generator needs no
+// more than their few-ulp accuracy, and they cut its instruction count.
 #include "dxg2_internal.h"
 
 namespace dxg2 {
@@ -22,8 +26,9 @@ __device__ __forceinline__ U4 philox4x32_10(U4 c, uint32_t k0, uint32_t k1) {
     const uint32_t W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
 #pragma unroll
     for (int r = 0; r < 10; ++r) {
-        const uint32_t hi0 = __umulhi(M0, c.x), lo0 = M0 * c.x;
-        const uint32_t hi1 = __umulhi(M1, c.z), lo1 = M1 * c.z;
+        const uint64_t p0 = (uint64_t)M0 * c.x, p1 = (uint64_t)M1 * c.z;
+        const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
         c = U4{hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0};
         k0 += W0;
         k1 += W1;
@@ -33,6 +38,12 @@ __device__ __forceinline__ U4 philox4x32_10(U4 c, uint32_t k0, uint32_t k1) {
 
 __device__ __forceinline__ float u01(uint32_t x) {  // (0, 1)
     return ((float)(x >> 8) + 0.5f) * (1.0f / 16777216.0f);
+}
+
+// sqrt(-ln u) for u in (0, 1): sqrt(-2 ln u) * sqrt(1/2), the Box-Muller radius of a
+// complex normal of unit variance
+__device__ __forceinline__ float bm_radius(float u) {
+    return __builtin_amdgcn_sqrtf(-0.69314718f * __builtin_amdgcn_logf(u));
 }
 
 // Per-pixel speckle state: field E and the count draw of frame t.
@@ -50,30 +61,27 @@ struct SpeckleWalk {
         k0 = k0_;
         k1 = k1_;
         const U4 r = philox4x32_10(U4{(uint32_t)p, 0u, (uint32_t)(p >> 32), 0u}, k0, k1);
-        const float rad = sqrtf(-logf(u01(r.x)));  // sqrt(-2 ln u) * sqrt(1/2)
-        float sn, cs;
-        sincosf(6.283185307179586f * u01(r.y), &sn, &cs);
-        er = rad * cs;
-        ei = rad * sn;
+        const float rad = bm_radius(u01(r.x));
+        const float turn = u01(r.y);  // angle 2 pi turn
+        er = rad * __builtin_amdgcn_cosf(turn);
+        ei = rad * __builtin_amdgcn_sinf(turn);
     }
 
     // advance to frame t (t = 0 keeps E_0) and draw its count
     __device__ __forceinline__ uint32_t draw(int t, uint32_t maxval) {
-        const float two_pi = 6.283185307179586f;
         const U4 r = philox4x32_10(U4{(uint32_t)p, (uint32_t)(t + 1), (uint32_t)(p >> 32), 0u},
                                    k0, k1);
         if (t > 0) {
-            const float rad = sqrtf(-logf(u01(r.x)));
-            float sn, cs;
-            sincosf(two_pi * u01(r.y), &sn, &cs);
-            er = rho * er + inov * rad * cs;
-            ei = rho * ei + inov * rad * sn;
+            const float rad = bm_radius(u01(r.x));
+            const float turn = u01(r.y);
+            er = rho * er + inov * rad * __builtin_amdgcn_cosf(turn);
+            ei = rho * ei + inov * rad * __builtin_amdgcn_sinf(turn);
         }
         const float lam = mu * (er * er + ei * ei);
         uint32_t k = 0;
         if (lam < 32.f) {
             const float u = u01(r.z);
-            float pk = expf(-lam);
+            float pk = __builtin_amdgcn_exp2f(-1.44269504f * lam);  // e^-lam
             float cdf = pk;
             while (u > cdf && k < 200u) {
                 ++k;
@@ -81,8 +89,7 @@ struct SpeckleWalk {
                 cdf += pk;
             }
         } else {
-            const float rad = sqrtf(-2.f * logf(u01(r.z)));
-            const float z = rad * cosf(two_pi * u01(r.w));
+            const float z = 1.41421356f * bm_radius(u01(r.z)) * __builtin_amdgcn_cosf(u01(r.w));
             const float v = floorf(lam + sqrtf(lam) * z + 0.5f);
             k = v > 0.f ? (uint32_t)v : 0u;
         }

@@ -12,16 +12,18 @@ The loop is split in two:
     budget next to the jobs already resident and a slot is free, keeps a ready queue,
     and dispatches to `n_streams` compute streams with at most `depth` dispatches in
     flight each.  The dense jobs at the head of the queue (similar sizes, up to
-    `max_batch`, under `batch_ops` Gram work) go out as ONE grouped launch; an event
-    job goes out alone.  Completions are polled and retired in stream order.
+    `max_batch`, under `batch_ops` Gram work) go out as ONE grouped launch, and so do
+    up to `max_batch` event jobs (one pair launch over all of them).  Completions are
+    polled and retired in stream order.
     A job that fails -- in preparation, over the budget, at admission, at dispatch or
     at completion -- fails alone: it is released and recorded with its error, and
     every other job carries on.  A grouped dispatch that fails is retried job by job,
     so only the job that fails by itself is lost.
   * `GpuExecutor` -- the device side: plans (dxg2_plan_create), job inputs generated on
     an admission stream, grouped dense launches (dxg2_batch_correlate_dense: per-job
-    gathers, one Gram launch over every job's tiles, per-job finalise), event launches
-    (dxg2_correlate_events_csr), completion events, and stream-ordered release of plans
+    gathers, one Gram launch over every job's tiles, per-job finalise), grouped event
+    launches (dxg2_batch_correlate_events: one pair launch over every job's pixels,
+    per-job finalise), completion events, and stream-ordered release of plans
     into torch's caching allocator (dxg2_set_allocator), so nothing waits on hipFree.
 
 `Server` is the two together.  Tests drive the Scheduler with a CPU executor whose
@@ -211,17 +213,20 @@ class Scheduler(object):
         return job
 
     def _group(self, ready):
-        """Dense jobs for one grouped launch: the head of the queue, then later dense jobs
-        no larger than twice the head, while the total Gram work stays under batch_ops
-        (small jobs share a launch; a large job runs alone)."""
+        """Jobs for one grouped launch: the head of the queue, then later jobs of its
+        layout.  Dense: no larger than twice the head, while the total Gram work stays
+        under batch_ops (small jobs share a launch; a large job runs alone).  Events:
+        up to max_batch jobs in one pair launch."""
         head = ready[0]
+        dense = head.spec.layout == "dense"
         group, total = [head], head.ops
         if not head.solo:
             for j in list(ready)[1:]:
                 if len(group) >= self.max_batch:
                     break
-                if (j.spec.layout != "dense" or j.solo or j.ops > 2 * head.ops or
-                        total + j.ops > self.batch_ops):
+                if j.spec.layout != head.spec.layout or j.solo:
+                    continue
+                if dense and (j.ops > 2 * head.ops or total + j.ops > self.batch_ops):
                     continue
                 group.append(j)
                 total += j.ops
@@ -322,8 +327,7 @@ class Scheduler(object):
                 k = min(range(self.n_streams), key=lambda i: len(busy[i]))
                 if len(busy[k]) >= self.depth:
                     break
-                group = self._group(ready) if ready[0].spec.layout == "dense" \
-                    else [ready.popleft()]
+                group = self._group(ready)
                 t = clock()
                 try:
                     handle = ex.dispatch(k, group)
@@ -526,12 +530,28 @@ class GpuExecutor(object):
             except Exception:
                 self.free_batches[k].append(batch)   # the slot is not in flight
                 raise
-        else:
+        elif len(jobs) == 1:
             j = jobs[0]
             _lib.check(self.lib.dxg2_correlate_events_csr(
                 j.plan.handle, _lib.ptr(j.inputs["times"]), _lib.ptr(j.inputs["data"]),
                 _lib.ptr(j.inputs["offsets"]), j.inputs["nnz"],
                 *[_lib.ptr(x) for x in j.outputs], st), "dxg2_correlate_events_csr")
+        else:
+            n = len(jobs)
+            arr = lambda vals: (ctypes.c_void_p * n)(*vals)
+            batch = self.free_batches[k].popleft()
+            try:
+                _lib.check(self.lib.dxg2_batch_correlate_events(
+                    batch, n, arr([j.plan.handle.value for j in jobs]),
+                    arr([_lib.ptr(j.inputs["times"]) for j in jobs]),
+                    arr([_lib.ptr(j.inputs["data"]) for j in jobs]),
+                    arr([_lib.ptr(j.inputs["offsets"]) for j in jobs]),
+                    (ctypes.c_int64 * n)(*[j.inputs["nnz"] for j in jobs]),
+                    *[arr([_lib.ptr(j.outputs[i]) for j in jobs]) for i in range(4)], st),
+                    "dxg2_batch_correlate_events")
+            except Exception:
+                self.free_batches[k].append(batch)   # the slot is not in flight
+                raise
         for j in jobs:
             for t in list(j.outputs) + [v for v in j.inputs.values() if hasattr(v, "record_stream")]:
                 t.record_stream(s)

@@ -184,6 +184,19 @@ int dxg2_batch_correlate_dense(dxg2_batch* batch, int32_t n, dxg2_plan* const* p
                                const void* const* d_frames, int64_t* const* d_num,
                                int64_t* const* d_sums, double* const* d_den,
                                float* const* d_g2, void* stream);
+/* Grouped event launch over several plans (continuous batching of event jobs): ONE pair
+ * launch over the work items of every job (each item tagged with its job, lag windows
+ * up to the longest series), then each plan's denominator and finalise, on `stream`.
+ * Per job: pixel-major CSR input as for dxg2_correlate_events_csr (d_times, d_data,
+ * d_offsets, nnz[i] events on the host).  Plans must share the count type and appear
+ * once each; d_num, d_sums, d_den may be NULL arrays.  Results, and each plan's
+ * dxg2_plan_event_status, equal those of dxg2_correlate_events_csr on each job alone.
+ * No reference counterpart: the serving loop's grouped submission. */
+int dxg2_batch_correlate_events(dxg2_batch* batch, int32_t n, dxg2_plan* const* plans,
+                                const int32_t* const* d_times, const void* const* d_data,
+                                const uint32_t* const* d_offsets, const int64_t* nnz,
+                                int64_t* const* d_num, int64_t* const* d_sums,
+                                double* const* d_den, float* const* d_g2, void* stream);
 
 /* Plain comparator: one bin per launch, FP32 gather, FP32 full-square Gram, FP32
  * diagonal sums and FP32 mean outer-product denominator.
@@ -237,10 +250,18 @@ int dxg2_plan_event_status(dxg2_plan* plan, int32_t* flags, void* stream);
  * dxg2_plan_stage_times waits for the last recorded call and returns the averages over
  * every call since profiling was switched on or last read, in ms: times[0] gather,
  * [1] gram, [2] denominator+finalise, [3] total, [4] number of calls averaged; then
- * it resets the record.  Counterpart of the reference's OpenCL event profiling
+ * it resets the record.  Event calls: [0] COO -> CSR conversion (0 for CSR input),
+ * [1] pairs and frame sums, [2] denominator+finalise.  Counterpart of the reference's OpenCL event profiling
  * (profile=True, profile_add in correlator/dense.py:244,261,272). */
 int dxg2_plan_set_profiling(dxg2_plan* plan, int32_t on);
 int dxg2_plan_stage_times(dxg2_plan* plan, float* times, int32_t n);
+
+/* LDS atomic-add throughput of this GPU (32-bit adds per second, summed over the
+ * chip), measured with workgroups of the event pair kernel's LDS footprint: mode 0
+ * random counters, mode 1 lane-consecutive counters, `iters` adds per lane.  The
+ * ceiling the event path's histogram updates are compared with.  Waits for the
+ * device.  No reference counterpart. */
+int dxg2_measure_lds_atomics(int32_t mode, int32_t iters, double* adds_per_s, void* stream);
 
 /* Synthetic speckle frames on the device: per-pixel AR(1) complex Gaussian field with
  * correlation time tau_c frames, counts ~ Poisson(mean_map[p] * |E|^2) clipped to the

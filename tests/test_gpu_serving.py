@@ -90,6 +90,8 @@ def test_mix_equals_alone():
         srv.close()
     assert stats["jobs"] == len(specs) and stats["failed"] == 0, stats["failures"]
     assert stats["mean_batch_size"] > 1.0, "no grouped launch happened"
+    assert any(getattr(j, "batch_size", 1) > 1 for j in stats["records"]
+               if j.spec.layout == "events"), "no grouped event launch happened"
     assert check_mix_equals_alone(stats["records"]) == {"dense", "events"}
 
 

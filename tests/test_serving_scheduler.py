@@ -97,6 +97,11 @@ def test_all_jobs_complete_and_batch():
     assert sorted(ex.retired) == list(range(40))
     assert ex.resident == 0
     assert any(len(d) > 1 for d in ex.dispatches), "no grouped dispatch"
+    layout = {s.id: s.layout for s in _specs()}
+    for d in ex.dispatches:  # a launch holds one layout
+        assert len({layout[i] for i in d}) == 1, d
+    assert any(len(d) > 1 and layout[d[0]] == "events" for d in ex.dispatches), \
+        "no grouped event dispatch"
 
 
 def test_realtime_arrivals_latency():
