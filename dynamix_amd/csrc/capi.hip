@@ -210,8 +210,10 @@ struct dxg2_plan {
     std::vector<int32_t> pix_idx;
     std::vector<BinDev> bins;
     std::vector<GatherWork> gather_work;
+    // general gather slots: every k-block in detector order, 16 per item (kb -1: none)
+    std::vector<KbWork> gather_slots;
     // band gather (uint8/int8): bands of detector rows, their (bin, k-block) list, the
-    // k-blocks left to gather_i8_kernel, and the LDS bytes staged per frame
+    // k-blocks left to the general gather, and the LDS bytes staged per frame
     std::vector<BandWork> gband;
     std::vector<int2> gband_kb;
     std::vector<GatherWork> gfall;
@@ -232,6 +234,7 @@ struct dxg2_plan {
     int32_t* d_pix = nullptr;
     BinDev* d_bins = nullptr;
     GatherWork* d_gwork = nullptr;
+    KbWork* d_gslots = nullptr;
     GramWork* d_mwork = nullptr;
     GramWork* d_xwork = nullptr;
     // persistent launch: per-XCD queues of the plain and cross items, their offsets
@@ -249,7 +252,8 @@ struct dxg2_plan {
     int64_t* d_psums = nullptr;   // digit planes: S_H, S_L, then S_Lw rows (n_bins each)
     uint8_t* d_wflag = nullptr;   // uint16: per detector pixel, a count > 255 in some frame
     int32_t* d_pixw = nullptr;    // uint16: the pixel lists reordered wide-first
-    int* d_nwkb = nullptr;        // uint16: per bin, k-blocks holding the wide pixels
+    int* d_nwkb = nullptr;        // uint16: per bin, k-blocks holding the wide pixels; then
+                                  // their maximum over the bins
     int64_t* d_prefix = nullptr;  // per bin and plane: prefix sums of S(t), N + 1 values
     int64_t* d_smax = nullptr;    // per bin: max |S(t)|
     double* d_dpre = nullptr;     // per bin and lag: FP64 Den(tau) before the finalise
@@ -305,6 +309,37 @@ namespace {
 int env_int(const char* name, int dflt) {
     const char* v = getenv(name);
     return (v && *v) ? atoi(v) : dflt;
+}
+
+// Slots of the general gather (gather_slots_kernel): every k-block of every bin sorted by
+// the detector position of its first pixel, 16 per workgroup item (four per wave; a slot
+// past the item's last k-block repeats its wave's last one, a wave with none gets kb -1),
+// items dealt to the XCDs in chunks of GATHER_XCD_GROUP.
+void build_gather_slots(dxg2_plan* p) {
+    std::vector<std::pair<int64_t, KbWork>> all;
+    for (int b = 0; b < p->n_bins; ++b)
+        for (int kb = 0; kb < p->bins[b].nkb; ++kb)
+            all.push_back({(int64_t)p->pix_idx[p->bins[b].idx_off + (int64_t)kb * KB], KbWork{b, kb}});
+    std::stable_sort(all.begin(), all.end(),
+                     [](const std::pair<int64_t, KbWork>& a, const std::pair<int64_t, KbWork>& b) {
+                         return a.first < b.first;
+                     });
+    using Item = std::array<KbWork, 16>;
+    std::vector<Item> items;
+    for (size_t i = 0; i < all.size(); i += 16) {
+        const int c = (int)std::min<size_t>(16, all.size() - i);
+        Item it;
+        for (int w = 0; w < 4; ++w)
+            for (int j = 0; j < 4; ++j)
+                it[4 * w + j] = 4 * w < c ? all[i + std::min(4 * w + j, c - 1)].second
+                                          : KbWork{0, -1};
+        items.push_back(it);
+    }
+    Item pad;
+    pad.fill(KbWork{0, -1});
+    items = remap_for_xcds(items, GATHER_XCD_GROUP, pad);
+    p->gather_slots.clear();
+    for (const Item& it : items) p->gather_slots.insert(p->gather_slots.end(), it.begin(), it.end());
 }
 
 // Band gather lists (uint8/int8 frames): rows per band RB such that 4 frames x (RB + 2)
@@ -791,9 +826,9 @@ int ensure_dense_buffers(dxg2_plan* p, hipStream_t s) {
 // Clear the accumulators of a dense call and gather the frames into the slab.  Returns
 // the frame-sum rows the Gram's diagonal tiles add into: the caller's sums (uint8/int8)
 // or the plan's digit-plane sums S_H, S_L (uint16).
-// wide_first (uint16 plans; the plain dense calls): the bins' pixels go into the slab
-// wide-first (k_wide.hip), for a Gram clipped to the wide k-blocks (dense_gram and
-// dense_finish with the same flag).
+// wide_first (uint16 plans; the plain dense calls): the low plane in list order with the
+// wide pixels flagged in the same pass, then the wide pixels' own k-blocks (k_wide.hip),
+// for a Gram clipped to them (dense_gram and dense_finish with the same flag).
 int64_t* dense_gather(dxg2_plan* p, const void* d_frames, int64_t* d_sums, hipStream_t s,
                       bool wide_first = false) {
     int64_t* sums = d_sums ? d_sums : p->d_sums;
@@ -803,24 +838,26 @@ int64_t* dense_gather(dxg2_plan* p, const void* d_frames, int64_t* d_sums, hipSt
     launch_fill_i64(gsums, (wide ? 2 : 1) * nbn, s);
     launch_fill_i64(p->d_ugram, (wide ? 3 : 1) * nbn, s);
     const int64_t hw = (int64_t)p->H * p->W;
-    const int32_t* pix = p->d_pix;
-    if (wide && wide_first) {
-        launch_wide_first((const uint16_t*)d_frames, hw, p->N, p->d_pix, p->d_bins, p->n_bins,
-                          p->d_wflag, p->d_pixw, p->d_nwkb, s);
-        pix = p->d_pixw;
-    }
+    uint32_t* flagw = wide && wide_first ? reinterpret_cast<uint32_t*>(p->d_wflag) : nullptr;
+    if (flagw) launch_fill_u32(flagw, (hw + 3) / 4, s);
     if (!p->gband.empty() && reinterpret_cast<uintptr_t>(d_frames) % 16 == 0) {
         launch_gather_band((const uint8_t*)d_frames, p->dtype, hw, p->W, p->N, p->nrows,
-                           p->gband_srow, pix, p->d_bins, p->d_gband, (int)p->gband.size(),
+                           p->gband_srow, p->d_pix, p->d_bins, p->d_gband, (int)p->gband.size(),
                            p->d_gband_kb, p->d_gfall, (int)p->gfall.size(), p->d_slab, s);
     } else {
-        launch_gather_i8((const uint8_t*)d_frames, p->dtype, hw, p->N, p->nrows, pix,
-                         p->d_bins, p->d_gwork, (int)p->gather_work.size(), p->d_slab, s,
-                         wide && wide_first ? p->d_nwkb : nullptr);
+        launch_gather_slots((const uint8_t*)d_frames, p->dtype, hw, p->N, p->nrows, p->d_pix,
+                            p->d_bins, p->d_gslots, (int)(p->gather_slots.size() / 16),
+                            p->d_slab, s, flagw);
     }
-    if (wide && wide_first)
+    if (flagw) {
+        launch_fill_i32(p->d_nwkb + p->n_bins, 1, s);
+        launch_wide_partition(p->d_pix, p->d_bins, p->n_bins, p->d_wflag, p->d_pixw, p->d_nwkb,
+                              s);
+        launch_wide_gather((const uint16_t*)d_frames, hw, p->N, p->nrows, p->d_pix, p->d_pixw,
+                           p->d_bins, p->n_bins, p->d_nwkb, p->d_slab, s);
         launch_wide_low_sums(p->d_slab, p->d_bins, p->d_nwkb, p->n_bins, p->N, p->nrows,
                              gsums + 2 * nbn, s);
+    }
     return gsums;
 }
 
@@ -983,6 +1020,7 @@ dxg2_plan* build_plan_host(const dxg2_plan_desc* desc, int* rc) {
             return (int64_t)p->pix_idx[p->bins[g.bin].idx_off + (int64_t)g.kb0 * KB];
         },
         GATHER_XCD_GROUP, GatherWork{0, 0, 0, 0});
+    build_gather_slots(p);
     build_band_work(p, lists);
     build_gram_work(p);
     build_event_work(p);
@@ -1041,6 +1079,7 @@ int dxg2_plan_create(const dxg2_plan_desc* desc, dxg2_plan** out) {
     hipStream_t s0 = sc.stream;
     const size_t staged = p->pix_idx.size() * 4 + p->bins.size() * sizeof(BinDev) +
                           (p->gather_work.size() + p->gfall.size() + 2) * sizeof(GatherWork) +
+                          (p->gather_slots.size() + 1) * sizeof(KbWork) +
                           (p->gband.size() + 1) * sizeof(BandWork) +
                           (p->gband_kb.size() + 1) * sizeof(int2) +
                           (p->gram_work.size() + p->gram_xwork.size() + 2) * sizeof(GramWork) +
@@ -1080,6 +1119,9 @@ int dxg2_plan_create(const dxg2_plan_desc* desc, dxg2_plan** out) {
     if (!rc)
         rc = upload((void**)&p->d_gwork, p->gather_work.data(),
                     p->gather_work.size() * sizeof(GatherWork));
+    if (!rc)
+        rc = upload((void**)&p->d_gslots, p->gather_slots.data(),
+                    p->gather_slots.size() * sizeof(KbWork));
     if (!rc) rc = upload((void**)&p->d_gband, p->gband.data(), p->gband.size() * sizeof(BandWork));
     if (!rc)
         rc = upload((void**)&p->d_gband_kb, p->gband_kb.data(), p->gband_kb.size() * sizeof(int2));
@@ -1108,7 +1150,7 @@ int dxg2_plan_create(const dxg2_plan_desc* desc, dxg2_plan** out) {
         rc = set_error(DXG2_ENOMEM, "cannot allocate digit-plane sums workspace");
     if (!rc && planes == 2 &&
         (dev_alloc(p->alloc, (void**)&p->d_wflag, (size_t)p->H * p->W + 16, s0) != hipSuccess ||
-         dev_alloc(p->alloc, (void**)&p->d_nwkb, (size_t)std::max(p->n_bins, 1) * sizeof(int),
+         dev_alloc(p->alloc, (void**)&p->d_nwkb, (size_t)(p->n_bins + 1) * sizeof(int),
                    s0) != hipSuccess))
         rc = set_error(DXG2_ENOMEM, "cannot allocate wide-pixel workspace");
     if (!rc && planes == 2) rc = upload((void**)&p->d_pixw, p->pix_idx.data(), p->pix_idx.size() * 4);
@@ -1152,7 +1194,7 @@ static void drop_std(dxg2_plan* p, hipStream_t s) {
 static void plan_release(dxg2_plan* p, hipStream_t s) {
     drop_graph(p);
     if (p->cap_stream) (void)hipStreamDestroy(p->cap_stream);
-    void* bufs[] = {p->d_pix,   p->d_bins, p->d_gwork, p->d_mwork, p->d_xwork, p->d_slab,
+    void* bufs[] = {p->d_pix,   p->d_bins, p->d_gwork, p->d_gslots, p->d_mwork, p->d_xwork, p->d_slab,
                     p->d_ugram, p->d_sums, p->d_psums, p->d_prefix, p->d_smax, p->d_dpre, p->d_pa,
                     p->d_pg, p->d_pm,
                     p->d_ework, p->d_err,  p->d_cnt,   p->d_scan,  p->d_off,
@@ -1206,6 +1248,7 @@ int dxg2_plan_footprint(const dxg2_plan* p, int32_t path, size_t* bytes) {
     switch (path) {
         case 0:
             *bytes = common + p->slab_bytes + p->gather_work.size() * sizeof(GatherWork) +
+                     p->gather_slots.size() * sizeof(KbWork) +
                      (p->gram_work.size() + p->gram_xwork.size() + p->qm_work.size() +
                       p->qx_work.size()) * sizeof(GramWork) + sizeof(p->q_off) + 16 * sizeof(int);
             return DXG2_OK;

@@ -139,9 +139,12 @@ struct KbWork {          // one k-block (64 pixels) of one bin
 // ------------------------------------------------------------------ launchers
 void launch_lds_atomic_rate(int mode, int blocks, int iters, uint32_t* out, hipStream_t s);
 void launch_fill_i64(int64_t* p, int64_t n, hipStream_t s);
-void launch_gather_i8(const uint8_t* frames, int dtype, int64_t hw, int nframes, int nrows,
-                      const int32_t* pix_idx, const BinDev* bins, const GatherWork* work,
-                      int n_work, int8_t* slab, hipStream_t s, const int* nwkb = nullptr);
+void launch_gather_slots(const uint8_t* frames, int dtype, int64_t hw, int nframes, int nrows,
+                         const int32_t* pix_idx, const BinDev* bins, const KbWork* kbl,
+                         int n_items, int8_t* slab, hipStream_t s, uint32_t* flagw);
+void launch_wide_gather(const uint16_t* frames, int64_t hw, int nframes, int nrows,
+                        const int32_t* pix, const int32_t* pix_w, const BinDev* bins, int n_bins,
+                        const int* nwkb, int8_t* slab, hipStream_t s);
 // Band gather of uint8/int8 frames: stage `rows` detector rows of 4 frames in LDS
 // (srow bytes per frame), write every k-block of the band from there.
 void launch_gather_band(const uint8_t* frames, int dtype, int64_t hw, int width, int nframes,
@@ -162,9 +165,8 @@ void launch_gram_i8_queue(int tile, bool cross, const int8_t* slab, const BinDev
                           int64_t* sums, hipStream_t s);
 // uint16 wide pixels first (k_wide.hip): flags of the pixels with a count > 255 in any
 // frame, each bin's pixel list reordered wide-first into pix_w, and nwkb per bin.
-void launch_wide_first(const uint16_t* frames, int64_t hw, int nframes, const int32_t* pix,
-                       const BinDev* bins, int n_bins, uint8_t* flag, int32_t* pix_w, int* nwkb,
-                       hipStream_t s);
+void launch_wide_partition(const int32_t* pix, const BinDev* bins, int n_bins,
+                           const uint8_t* flag, int32_t* pix_w, int* nwkb, hipStream_t s);
 // Low-byte frame sums over each bin's first nwkb k-blocks of the low plane.
 void launch_wide_low_sums(const int8_t* slab, const BinDev* bins, const int* nwkb, int n_bins,
                           int nframes, int nrows, int64_t* lsums, hipStream_t s);

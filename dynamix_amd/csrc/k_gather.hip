@@ -3,7 +3,9 @@
 //
 // The reference gathers one bin at a time on the host (correlator/dense.py:48-49,
 // dense.py:317-319).  Here one launch covers every bin.  A wave owns four k-blocks (64
-// consecutive entries each of a bin's raster-ordered pixel list) and a frame tile:
+// consecutive entries each of a bin's raster-ordered pixel list; the k-blocks of all
+// bins are taken in detector order, so a workgroup's sixteen read neighbouring stretches
+// of the same rows) and a frame tile:
 //   * each load instruction reads entry `lane` of one frame, so the 64 lanes read 64
 //     consecutive list entries -- runs of neighbouring pixels, few cache lines;
 //   * after eight frames each lane holds 8 frames x 1 pixel per k-block; a 4x4 byte
@@ -65,8 +67,88 @@ __device__ __forceinline__ uint32_t quad_transpose(uint32_t x, uint32_t sel_lo, 
     return __builtin_amdgcn_perm(y1, y0, sel_lo) | __builtin_amdgcn_perm(y3, y2, sel_hi);
 }
 
+// One wave's step over four slots: slot j is k-block kbj[j] of the bin whose slab starts
+// at dst[j] (low plane `plane[j]` bytes further; high bytes written when hi[j]), its 64
+// pixels read through src[j] (keep[j] = 0 for padded list entries).  Frames [t0, tend)
+// FG at a time: each step issues all 4 FG loads before its first wait (one basic block,
+// so the waits count down through them).  A slot that repeats another stores the same
+// bytes to the same addresses.  Whole-line stores (an exchange across the wave halves so
+// that each store writes two 128-byte slab lines) measured slower here than these
+// 64-byte runs.
+// FLAG (uint16): the high bytes are not stored; instead the lane ORs its pixel's counts
+// over the frames and sets the pixel's byte of flagw (pixel 4j+k is byte k of word j)
+// when any count exceeds 255 (pix[j] = the slot's pixel index, < 0 for padding).
+template <typename T, bool BIAS, bool FLAG = false>
+__device__ __forceinline__ void gather_wave4(const T* const (&src)[4], const uint32_t (&keep)[4],
+                                             int8_t* const (&dst)[4], const int (&kbj)[4],
+                                             const int64_t (&plane)[4], const bool (&hi)[4],
+                                             int64_t hw, int nframes, int nrows, int t0,
+                                             int tend, const int (&pix)[4] = {-1, -1, -1, -1},
+                                             uint32_t* __restrict__ flagw = nullptr) {
+    constexpr bool WIDE = sizeof(T) == 2;
+    uint32_t seen[4] = {0u, 0u, 0u, 0u};
+    constexpr int FG = 8;  // frames per step (a multiple of 4 dividing 64; 16 measured slower)
+    const int lane = threadIdx.x & 63;
+    const int qi = lane & 3;        // frame offset this lane stores after the transpose
+    const int q4 = (lane >> 2) * 4; // first pixel of this lane's stored dword
+    // byte selectors for v_perm: lo = (y0[qi], y1[qi], 0, 0), hi = (0, 0, y2[qi], y3[qi])
+    const uint32_t sel_lo = (uint32_t)qi | ((uint32_t)(4 + qi) << 8) | 0x0c0c0000u;
+    const uint32_t sel_hi = 0x00000c0cu | ((uint32_t)qi << 16) | ((uint32_t)(4 + qi) << 24);
+    for (int t = t0; t < tend; t += FG) {
+        uint32_t v[4][FG];
+#pragma unroll
+        for (int i = 0; i < FG; ++i) {
+            const int64_t ro = (int64_t)(t + i < nframes ? t + i : nframes - 1) * hw;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                v[j][i] = (uint32_t)src[j][ro];
+                if (FLAG) seen[j] |= v[j][i];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            // dwords of frames t + 4 g + qi, low and high digits; the transposes run with
+            // every lane active (DPP reads the quad's other lanes, including those past
+            // the last frame)
+            uint32_t d[FG / 4][2];
+#pragma unroll
+            for (int g = 0; g < FG / 4; ++g) {
+                uint32_t xl = 0, xh = 0;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const uint32_t x = v[j][4 * g + i] & keep[j];
+                    xl |= (x & 0xFFu) << (8 * i);
+                    if (WIDE) xh |= (x >> 8) << (8 * i);
+                }
+                d[g][0] = quad_transpose(xl, sel_lo, sel_hi) ^ (BIAS ? 0x80808080u : 0u);
+                d[g][1] = WIDE ? quad_transpose(xh, sel_lo, sel_hi) ^ 0x80808080u : 0u;
+            }
+#pragma unroll
+            for (int g = 0; g < FG / 4; ++g) {
+                const int tq = t + 4 * g + qi;
+                const int64_t o = slab_offset(kbj[j], tq, q4, nrows);
+                if (tq < nframes) {
+                    if (WIDE) {
+                        if (!FLAG && hi[j]) *reinterpret_cast<uint32_t*>(dst[j] + o) = d[g][1];
+                        *reinterpret_cast<uint32_t*>(dst[j] + plane[j] + o) = d[g][0];
+                    } else {
+                        *reinterpret_cast<uint32_t*>(dst[j] + o) = d[g][0];
+                    }
+                }
+            }
+        }
+    }
+    if (FLAG) {
+        // slots repeating a k-block flag the same pixels again: harmless
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            if (pix[j] >= 0 && (seen[j] & 0xFF00u))
+                atomicOr(flagw + (pix[j] >> 2), 1u << (8 * (pix[j] & 3)));
+    }
+}
+
 // Work layout: blockIdx.x -> GatherWork (bin, first k-block, k-block count <= 16),
-// blockIdx.y -> 64-frame tile.  Wave w handles k-blocks kb0 + 4 w .. kb0 + 4 w + 3.
+// blockIdx.y -> frame tile.  Wave w handles k-blocks kb0 + 4 w .. kb0 + 4 w + 3.
 // BIAS (uint8 and uint16 input): bytes are stored as v ^ 0x80 = v - 128 (i8 MFMA
 // operands); padded list entries (index -1) read as 0 and become -128 the same way.
 // T = uint16_t writes the high bytes to k-block kb and the low bytes to k-block
@@ -81,93 +163,159 @@ __device__ __forceinline__ void gather_general(const T* __restrict__ frames, int
                                                int t0, int tend, int8_t* __restrict__ slab,
                                                const int* __restrict__ nwkb = nullptr) {
     constexpr bool WIDE = sizeof(T) == 2;
-    constexpr int FG = 8;  // frames per step (a multiple of 4 dividing 64; 16 measured slower)
     const BinDev bd = bins[w.bin];
     const int hi_end = WIDE && nwkb ? nwkb[w.bin] : bd.nkb;  // k-blocks whose high bytes go out
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const int qi = lane & 3;        // frame offset this lane stores after the transpose
-    const int q4 = (lane >> 2) * 4; // first pixel of this lane's stored dword
-    // byte selectors for v_perm: lo = (y0[qi], y1[qi], 0, 0), hi = (0, 0, y2[qi], y3[qi])
-    const uint32_t sel_lo = (uint32_t)qi | ((uint32_t)(4 + qi) << 8) | 0x0c0c0000u;
-    const uint32_t sel_hi = 0x00000c0cu | ((uint32_t)qi << 16) | ((uint32_t)(4 + qi) << 24);
     const int64_t plane = (int64_t)bd.nkb * nrows * KB;  // bytes between digit planes
-
-    // Wave w takes the 4 consecutive k-blocks kb0 + 4 w .. + 3 of each 16 and walks them
-    // FG frames at a time together, so that at each frame the workgroup reads up to 16
-    // consecutive k-blocks' pixels (up to 2 KiB of contiguous frame row for uint16).
-    // Slots past the item's last k-block repeat that k-block: every step then issues
-    // all 4 FG = 32 loads before its first wait (one basic block, so the waits count
-    // down through them), and a repeat stores the same bytes to the same addresses.
-    // Whole-line stores (an exchange across the wave halves so that each store writes
-    // two 128-byte slab lines) measured slower here than these 64-byte runs.
+    // Wave w takes the 4 consecutive k-blocks kb0 + 4 w .. + 3 of each 16, so that at each
+    // frame the workgroup reads up to 16 consecutive k-blocks' pixels.  Slots past the
+    // item's last k-block repeat that k-block.
     const int kend = w.kb0 + w.nkb;
-    int8_t* dst = slab + bd.slab_off;
     for (int kbase = w.kb0 + 4 * wave; kbase < kend; kbase += 16) {
         const int nk = min(4, kend - kbase);
         int kbj[4];
         const T* src[4];
         uint32_t keep[4];
+        int8_t* dst[4];
+        int64_t pl[4];
+        bool hi[4];
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             kbj[j] = kbase + min(j, nk - 1);
             const int p = pix_idx[bd.idx_off + (int64_t)kbj[j] * KB + lane];
             src[j] = frames + (p >= 0 ? p : 0);
             keep[j] = p >= 0 ? 0xFFFFu : 0u;
+            dst[j] = slab + bd.slab_off;
+            pl[j] = plane;
+            hi[j] = kbj[j] < hi_end;
         }
-        for (int t = t0; t < tend; t += FG) {
-            uint32_t v[4][FG];
-#pragma unroll
-            for (int i = 0; i < FG; ++i) {
-                const int64_t ro = (int64_t)(t + i < nframes ? t + i : nframes - 1) * hw;
-#pragma unroll
-                for (int j = 0; j < 4; ++j) v[j][i] = (uint32_t)src[j][ro];
-            }
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                // dwords of frames t + 4 g + qi, low and high digits;
-                // the transposes run with every lane active (DPP reads the quad's other
-                // lanes, including those past the last frame)
-                uint32_t d[FG / 4][2];
-#pragma unroll
-                for (int g = 0; g < FG / 4; ++g) {
-                    uint32_t xl = 0, xh = 0;
-#pragma unroll
-                    for (int i = 0; i < 4; ++i) {
-                        const uint32_t x = v[j][4 * g + i] & keep[j];
-                        xl |= (x & 0xFFu) << (8 * i);
-                        if (WIDE) xh |= (x >> 8) << (8 * i);
-                    }
-                    d[g][0] = quad_transpose(xl, sel_lo, sel_hi) ^ (BIAS ? 0x80808080u : 0u);
-                    d[g][1] = WIDE ? quad_transpose(xh, sel_lo, sel_hi) ^ 0x80808080u : 0u;
-                }
-#pragma unroll
-                for (int g = 0; g < FG / 4; ++g) {
-                    const int tq = t + 4 * g + qi;
-                    const int64_t o = slab_offset(kbj[j], tq, q4, nrows);
-                    if (tq < nframes) {
-                        if (WIDE) {
-                            if (kbj[j] < hi_end) *reinterpret_cast<uint32_t*>(dst + o) = d[g][1];
-                            *reinterpret_cast<uint32_t*>(dst + plane + o) = d[g][0];
-                        } else {
-                            *reinterpret_cast<uint32_t*>(dst + o) = d[g][0];
-                        }
-                    }
-                }
-            }
-        }
+        gather_wave4<T, BIAS>(src, keep, dst, kbj, pl, hi, hw, nframes, nrows, t0, tend);
     }
 }
 
-template <typename T, bool BIAS>
-__global__ __launch_bounds__(256) void gather_i8_kernel(
+// The general gather over k-blocks of any bins: blockIdx.x -> 16 slots kbl[16 x ..],
+// k-blocks (bin, kb) in detector order (a wave's four slots; a slot past the item's
+// last k-block repeats the wave's last one, a wave with nothing to do has kb < 0 in
+// its first slot), blockIdx.y -> frame tile.  Mixing bins lets the k-blocks that read
+// the same stretch of detector rows share one workgroup and its cache lines.
+// FLAG (uint16 with wide pixels handled apart): only the low plane is written, and the
+// pixels with a count above 255 are flagged in flagw (zeroed beforehand).
+template <typename T, bool BIAS, bool FLAG>
+__global__ __launch_bounds__(256) void gather_slots_kernel(
     const T* __restrict__ frames, int64_t hw, int nframes, int nrows,
     const int32_t* __restrict__ pix_idx, const BinDev* __restrict__ bins,
-    const GatherWork* __restrict__ work, int8_t* __restrict__ slab,
-    const int* __restrict__ nwkb) {
+    const KbWork* __restrict__ kbl, int8_t* __restrict__ slab, uint32_t* __restrict__ flagw) {
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const KbWork* sl = kbl + (int64_t)blockIdx.x * 16 + 4 * wave;
+    if (sl[0].kb < 0) return;
     const int t0 = blockIdx.y * GATHER_TF;
-    gather_general<T, BIAS>(frames, hw, nframes, nrows, pix_idx, bins, work[blockIdx.x], t0,
-                            min(nframes, t0 + GATHER_TF), slab, nwkb);
+    const int tend = min(nframes, t0 + GATHER_TF);
+    int kbj[4], pix[4];
+    const T* src[4];
+    uint32_t keep[4];
+    int8_t* dst[4];
+    int64_t pl[4];
+    bool hi[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const KbWork e = sl[j];
+        const BinDev bd = bins[e.bin];
+        kbj[j] = e.kb;
+        const int p = pix_idx[bd.idx_off + (int64_t)e.kb * KB + lane];
+        pix[j] = p;
+        src[j] = frames + (p >= 0 ? p : 0);
+        keep[j] = p >= 0 ? 0xFFFFu : 0u;
+        dst[j] = slab + bd.slab_off;
+        pl[j] = (int64_t)bd.nkb * nrows * KB;
+        hi[j] = true;
+    }
+    gather_wave4<T, BIAS, FLAG>(src, keep, dst, kbj, pl, hi, hw, nframes, nrows, t0, tend, pix,
+                                flagw);
+}
+
+// uint16 wide pixels (after gather_slots_kernel<FLAG> and wide_partition_kernel).  For a
+// bin with clip = nwkb[b] wide k-blocks:
+//  * 2 clip <= nkb: k-blocks kb < clip of the wide-first list go to the high plane's
+//    k-block kb (high bytes) and k-block clip + kb (low bytes);
+//  * otherwise (clip = nkb): every k-block of the list in its own order, high bytes to
+//    k-block kb (and the low bytes again to the low plane, unchanged).
+// Work units (16 k-blocks of a bin from kb0 = 16 level, one frame tile) up to the level
+// of the largest clip (nwkb[n_bins]) are taken by a fixed grid in turn: the wide
+// k-blocks are a small share of the slab, so a grid over every k-block would be mostly
+// empty workgroups.
+__global__ __launch_bounds__(256) void wide_gather_kernel(
+    const uint16_t* __restrict__ frames, int64_t hw, int nframes, int nrows,
+    const int32_t* __restrict__ pix, const int32_t* __restrict__ pix_w,
+    const BinDev* __restrict__ bins, int n_bins, const int* __restrict__ nwkb,
+    int8_t* __restrict__ slab) {
+    const int n_tiles = (nframes + GATHER_TF - 1) / GATHER_TF;
+    const int levels = (nwkb[n_bins] + 15) / 16;
+    const int64_t units = (int64_t)levels * n_bins * n_tiles;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    for (int64_t u = blockIdx.x; u < units; u += gridDim.x) {
+        const int tile = (int)(u % n_tiles);
+        const int64_t rest = u / n_tiles;
+        const int b = (int)(rest % n_bins);
+        const int kb0 = 16 * (int)(rest / n_bins);
+        const BinDev bd = bins[b];
+        const int clip = nwkb[b];
+        const int kend = min(kb0 + 16, clip);
+        const int kbase = kb0 + 4 * wave;
+        if (kbase >= kend) continue;  // per wave: no barrier in this kernel
+        const bool compact = 2 * clip <= bd.nkb;
+        const int32_t* list = compact ? pix_w : pix;
+        const int t0 = tile * GATHER_TF;
+        const int tend = min(nframes, t0 + GATHER_TF);
+        const int nk = min(4, kend - kbase);
+        int kbj[4];
+        const uint16_t* src[4];
+        uint32_t keep[4];
+        int8_t* dst[4];
+        int64_t pl[4];
+        bool hi[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            kbj[j] = kbase + min(j, nk - 1);
+            const int p = list[bd.idx_off + (int64_t)kbj[j] * KB + lane];
+            src[j] = frames + (p >= 0 ? p : 0);
+            keep[j] = p >= 0 ? 0xFFFFu : 0u;
+            dst[j] = slab + bd.slab_off;
+            pl[j] = (int64_t)(compact ? clip : bd.nkb) * nrows * KB;
+            hi[j] = true;
+        }
+        gather_wave4<uint16_t, true>(src, keep, dst, kbj, pl, hi, hw, nframes, nrows, t0, tend);
+    }
+}
+
+void launch_wide_gather(const uint16_t* frames, int64_t hw, int nframes, int nrows,
+                        const int32_t* pix, const int32_t* pix_w, const BinDev* bins, int n_bins,
+                        const int* nwkb, int8_t* slab, hipStream_t s) {
+    if (n_bins <= 0 || nframes <= 0) return;
+    wide_gather_kernel<<<dim3(2048), dim3(256), 0, s>>>(frames, hw, nframes, nrows, pix, pix_w,
+                                                        bins, n_bins, nwkb, slab);
+}
+
+void launch_gather_slots(const uint8_t* frames, int dtype, int64_t hw, int nframes, int nrows,
+                         const int32_t* pix_idx, const BinDev* bins, const KbWork* kbl,
+                         int n_items, int8_t* slab, hipStream_t s, uint32_t* flagw) {
+    if (n_items <= 0 || nframes <= 0) return;
+    dim3 grid((unsigned)n_items, (unsigned)((nframes + GATHER_TF - 1) / GATHER_TF));
+    const uint16_t* f16 = reinterpret_cast<const uint16_t*>(frames);
+    if (dtype == DXG2_U16 && flagw)
+        gather_slots_kernel<uint16_t, true, true><<<grid, dim3(256), 0, s>>>(
+            f16, hw, nframes, nrows, pix_idx, bins, kbl, slab, flagw);
+    else if (dtype == DXG2_U16)
+        gather_slots_kernel<uint16_t, true, false><<<grid, dim3(256), 0, s>>>(
+            f16, hw, nframes, nrows, pix_idx, bins, kbl, slab, nullptr);
+    else if (dtype == DXG2_U8)
+        gather_slots_kernel<uint8_t, true, false><<<grid, dim3(256), 0, s>>>(
+            frames, hw, nframes, nrows, pix_idx, bins, kbl, slab, nullptr);
+    else
+        gather_slots_kernel<uint8_t, false, false><<<grid, dim3(256), 0, s>>>(
+            frames, hw, nframes, nrows, pix_idx, bins, kbl, slab, nullptr);
 }
 
 // Band gather (uint8 / int8 frames).  A workgroup takes one band of detector rows and up
@@ -183,7 +331,7 @@ __global__ __launch_bounds__(256) void gather_i8_kernel(
 // part (slab_offset(kb, t, kk) = slab_offset(kb, 0, kk) + (t/32) 2048 + (t%32) 16).  The
 // work list gives each k-block to the band of its first row, with one row of overlap
 // staged below the band, so k-blocks spanning two rows are covered; the others, and
-// every bin's partial last k-block, go to gather_i8_kernel.  Two workgroups share a CU:
+// every bin's partial last k-block, go to the general gather.  Two workgroups share a CU:
 // one copies while the other assembles.
 constexpr int BAND_FRAMES = 128;    // frames per workgroup (a multiple of BAND_STAGE)
 constexpr int BAND_GROUP = 4;       // k-blocks whose LDS reads are issued together
@@ -325,25 +473,6 @@ void launch_gather_band(const uint8_t* frames, int dtype, int64_t hw, int width,
         gather_band_kernel<false><<<grid, dim3(256), lds, s>>>(frames, hw, width, nframes, nrows,
                                                                srow, pix_idx, bins, bands, n_bands,
                                                                band_kb, fall, slab);
-}
-
-void launch_gather_i8(const uint8_t* frames, int dtype, int64_t hw, int nframes, int nrows,
-                      const int32_t* pix_idx, const BinDev* bins, const GatherWork* work,
-                      int n_work, int8_t* slab, hipStream_t s, const int* nwkb) {
-    if (n_work <= 0 || nframes <= 0) return;
-    dim3 grid((unsigned)n_work, (unsigned)((nframes + GATHER_TF - 1) / GATHER_TF));
-    if (dtype == DXG2_U16)
-        gather_i8_kernel<uint16_t, true><<<grid, dim3(256), 0, s>>>(
-            reinterpret_cast<const uint16_t*>(frames), hw, nframes, nrows, pix_idx, bins, work,
-            slab, nwkb);
-    else if (dtype == DXG2_U8)
-        gather_i8_kernel<uint8_t, true><<<grid, dim3(256), 0, s>>>(frames, hw, nframes, nrows,
-                                                                  pix_idx, bins, work, slab,
-                                                                  nullptr);
-    else
-        gather_i8_kernel<uint8_t, false><<<grid, dim3(256), 0, s>>>(frames, hw, nframes, nrows,
-                                                                   pix_idx, bins, work, slab,
-                                                                   nullptr);
 }
 
 }  // namespace dxg2

@@ -243,36 +243,48 @@ struct GramTile {
     }
 };
 
-// k-block end of item `w` once clipped to the bin's wide k-blocks (uint16 plans with the
-// pixels reordered wide-first; nwkb null: no clipping).  HH items (plain, k-blocks below
-// nkb) and cross items stop after the high plane's nonzero k-blocks; LL items are whole.
+// K-range of item `w` and the CROSS kernel's plane distance.  nwkb (uint16 plans whose
+// wide pixels -- those with a count above 255 -- have k-blocks of their own; null: every
+// item whole, distance nkb): per bin, `clip` k-blocks at the start of the high plane
+// hold the wide pixels' high bytes, and
+//  * HH items (plain, k-blocks below nkb) stop at clip; LL items are whole;
+//  * cross items: the first half [0, nkb) stops at clip and reads its B operand `off`
+//    k-blocks further; the second half [nkb, 2 nkb) moves to [off, off + clip) and reads
+//    B `off` k-blocks back.  off = clip when the wide pixels' low bytes follow their high
+//    bytes in the high plane (2 clip <= nkb), else nkb (clip = nkb: every pixel, the high
+//    and low planes in list order).
+struct ItemRange {
+    int kb0, kb1, nkbp;
+};
 template <bool CROSS>
-__device__ __forceinline__ int clipped_kb1(const GramWork& w, int nkb, const int* nwkb) {
-    if (!nwkb) return w.kb1;
-    const int nw = nwkb[w.bin];
-    if (w.kb0 < nkb) return min(w.kb1, nw);
-    return CROSS ? min(w.kb1, nkb + nw) : w.kb1;
+__device__ __forceinline__ ItemRange item_range(const GramWork& w, int nkb, const int* nwkb) {
+    if (!nwkb) return ItemRange{w.kb0, w.kb1, nkb};
+    const int clip = nwkb[w.bin];
+    const int off = 2 * clip <= nkb ? clip : nkb;
+    if (w.kb0 < nkb) return ItemRange{w.kb0, min(w.kb1, clip), off};
+    if (!CROSS) return ItemRange{w.kb0, w.kb1, nkb};
+    return ItemRange{off + (w.kb0 - nkb), off + min(w.kb1 - nkb, clip), off};
 }
 
 // One work item: the tile of `w` in a bin whose slab starts at slab_bin, over k-blocks
-// [w.kb0, kb1).
+// [r.kb0, r.kb1) (a CROSS item's B operand r.nkbp k-blocks away).
 template <int WM, int WN, bool CROSS, bool STORE>
-__device__ __forceinline__ void gram_item(const GramWork& w, int kb1, const int8_t* slab_bin,
-                                          int nkb, int nframes, int nrows, int biased,
+__device__ __forceinline__ void gram_item(const GramWork& w, ItemRange r, const int8_t* slab_bin,
+                                          int nframes, int nrows, int biased,
                                           int64_t* __restrict__ out, int64_t* __restrict__ sums,
                                           int* __restrict__ tile, unsigned int* sDiag) {
     using S = GramShape<WM, WN>;
     const int tid = threadIdx.x;
     const int wave = tid >> 6;
     if (!CROSS && w.ti == w.tj) {
-        GramTile<WM, WN, true, false, STORE> t{slab_bin, w.kb0, kb1, w.ti * S::TM,
+        GramTile<WM, WN, true, false, STORE> t{slab_bin, r.kb0, r.kb1, w.ti * S::TM,
                                                w.tj * S::TN, nframes, nrows, tid, tid & 63,
                                                wave >> 1, wave & 1, 0, sDiag};
         t.run(biased, out, sums, tile);
     } else {
-        GramTile<WM, WN, false, CROSS, STORE> t{slab_bin, w.kb0, kb1, w.ti * S::TM,
+        GramTile<WM, WN, false, CROSS, STORE> t{slab_bin, r.kb0, r.kb1, w.ti * S::TM,
                                                 w.tj * S::TN, nframes, nrows, tid, tid & 63,
-                                                wave >> 1, wave & 1, nkb, sDiag};
+                                                wave >> 1, wave & 1, r.nkbp, sDiag};
         t.run(biased, out, sums, tile);
     }
 }
@@ -291,9 +303,9 @@ __global__ __launch_bounds__(256) void gram_i8_kernel(const int8_t* __restrict__
     const GramWork w = work[blockIdx.x];
     if (w.bin < 0) return;  // padding item
     const BinDev bd = bins[w.bin];
-    const int kb1 = clipped_kb1<CROSS>(w, bd.nkb, nwkb);
-    if (kb1 <= w.kb0) return;  // nothing of this item is left
-    gram_item<WM, WN, CROSS, false>(w, kb1, slab + bd.slab_off, bd.nkb, nframes, nrows, biased,
+    const ItemRange r = item_range<CROSS>(w, bd.nkb, nwkb);
+    if (r.kb1 <= r.kb0) return;  // nothing of this item is left
+    gram_item<WM, WN, CROSS, false>(w, r, slab + bd.slab_off, nframes, nrows, biased,
                                     ugram + (int64_t)w.out * nframes,
                                     sums + (int64_t)w.out * nframes, nullptr, sDiag);
 }
@@ -340,10 +352,10 @@ __global__ __launch_bounds__(256) void gram_i8_queue_kernel(
         if (it < 0) break;
         const GramWork w = work[it];
         const BinDev bd = bins[w.bin];
-        const int kb1 = clipped_kb1<CROSS>(w, bd.nkb, nwkb);
-        if (kb1 > w.kb0)  // the same for every thread of the workgroup
-            gram_item<WM, WN, CROSS, false>(w, kb1, slab + bd.slab_off, bd.nkb, nframes, nrows,
-                                            biased, ugram + (int64_t)w.out * nframes,
+        const ItemRange r = item_range<CROSS>(w, bd.nkb, nwkb);
+        if (r.kb1 > r.kb0)  // the same for every thread of the workgroup
+            gram_item<WM, WN, CROSS, false>(w, r, slab + bd.slab_off, nframes, nrows, biased,
+                                            ugram + (int64_t)w.out * nframes,
                                             sums + (int64_t)w.out * nframes, nullptr, sDiag);
         __syncthreads();  // s_item and sDiag are reused by the next item
     }
@@ -417,9 +429,9 @@ __global__ __launch_bounds__(256) void gram_i8_store_kernel(
     const GramWork w = work[blockIdx.x];
     if (w.bin < 0) return;  // padding item
     const BinDev bd = bins[w.bin];
-    gram_item<WM, WN, CROSS, true>(w, w.kb1, slab + bd.slab_off, bd.nkb, nframes, nrows, biased,
-                                   ugram + (int64_t)w.out * nframes, nullptr,
-                                   tiles + (int64_t)w.slot * S::TM * S::TN, sDiag);
+    gram_item<WM, WN, CROSS, true>(w, ItemRange{w.kb0, w.kb1, bd.nkb}, slab + bd.slab_off,
+                                   nframes, nrows, biased, ugram + (int64_t)w.out * nframes,
+                                   nullptr, tiles + (int64_t)w.slot * S::TM * S::TN, sDiag);
 }
 
 // Grouped launch over several plans (continuous batching of dense jobs): the same tile
@@ -434,8 +446,9 @@ __global__ __launch_bounds__(256) void gram_i8_jobs_kernel(const GramJob* __rest
     if (w.bin < 0) return;  // padding item
     const GramJob j = jobs[w.job];
     const BinDev bd = gload_bin(j.bins + w.bin);
-    gram_item<WM, WN, CROSS, false>(w, w.kb1, j.slab + bd.slab_off, bd.nkb, j.nframes, j.nrows,
-                                    j.biased, j.ugram + (int64_t)w.out * j.nframes,
+    gram_item<WM, WN, CROSS, false>(w, ItemRange{w.kb0, w.kb1, bd.nkb}, j.slab + bd.slab_off,
+                                    j.nframes, j.nrows, j.biased,
+                                    j.ugram + (int64_t)w.out * j.nframes,
                                     j.sums + (int64_t)w.out * j.nframes, nullptr, sDiag);
 }
 
