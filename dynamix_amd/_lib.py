@@ -47,7 +47,8 @@ class PlanDesc(ctypes.Structure):
 
 
 class Tuning(ctypes.Structure):
-    _fields_ = [("tile", c_int), ("band", c_int), ("group", c_int), ("queue", c_int)]
+    _fields_ = [("tile", c_int), ("band", c_int), ("group", c_int), ("queue", c_int),
+                ("kchunk", c_int)]
 
 
 ALLOC_FN = ctypes.CFUNCTYPE(c_void_p, c_size_t, c_void_p, c_void_p)

@@ -1,8 +1,9 @@
 """Autotuner of the grouped Gram launch.
 
 The Gram's speed depends on the tile size (128 or 256 frames), on the raster band and
-XCD chunk of its work list, and on the launch form (persistent workgroups over per-XCD
-work queues, or one workgroup per tile).  The best choice depends on the number of
+XCD chunk of its work list, on the launch form (persistent workgroups over per-XCD
+work queues, or one workgroup per tile) and on the k-chunk (k-blocks per work item:
+shorter items let the tiles that run together share a chunk of frame panel in L2).  The best choice depends on the number of
 frames and on how the binned pixels split into bins, so results are cached under a key
 built from (frame dtype, nframes, histogram of bin sizes in power-of-two k-block
 buckets).  `autotune()` times candidates on real frames with the library's own HIP event
@@ -27,6 +28,7 @@ TILES = (256, 128)
 QUEUE = (1, 0)
 BANDS = (2, 4, 8)
 GROUPS = (16, 32, 64)
+KCHUNKS = (2047, 1024, 512, 256)
 
 
 def cache_path(path=None):
@@ -81,7 +83,8 @@ def apply_cached(plan, nframes, path=None):
     """Apply the recorded tuning of this plan's key; returns the entry or None."""
     entry = load_cache(path).get(plan_key(plan, nframes))
     if entry:
-        plan.set_tuning(entry["tile"], entry["band"], entry["group"], entry["queue"])
+        plan.set_tuning(entry["tile"], entry["band"], entry["group"], entry["queue"],
+                        entry.get("kchunk", 0))
     return entry
 
 
@@ -105,8 +108,8 @@ def autotune(plan, d_frames, nframes, reps=2, path=None, record=True):
     """Time the Gram over candidate tunings on `d_frames` (a GPU tensor of the plan's
     shape and dtype), apply the fastest to `plan` and (record=True) cache it.
 
-    Two stages: tile size x launch form at the default band and chunk, then band x
-    chunk for the best of those.  Returns the cache entry: the chosen tuning, its Gram
+    Three stages: tile size x launch form at the default band, chunk and k-chunk; then
+    band x chunk for the best of those; then the k-chunk.  Returns the cache entry: the chosen tuning, its Gram
     time, the default's time and every timing measured.
     """
     import torch
@@ -120,33 +123,41 @@ def autotune(plan, d_frames, nframes, reps=2, path=None, record=True):
     default = plan.get_tuning()
     timings = []
 
-    def trial(tile, band, group, queue):
-        plan.set_tuning(tile, band, group, queue)
+    def trial(tile, band, group, queue, kchunk):
+        plan.set_tuning(tile, band, group, queue, kchunk)
         ms = _time_gram(plan, d_frames, bufs, stream, reps)
         timings.append({"tile": tile, "band": band, "group": group, "queue": queue,
-                        "gram_ms": ms})
+                        "kchunk": kchunk, "gram_ms": ms})
         return ms
 
-    default_ms = trial(default["tile"], default["band"], default["group"], default["queue"])
+    kc0 = default["kchunk"]
+    default_ms = trial(default["tile"], default["band"], default["group"], default["queue"],
+                       kc0)
     best = dict(default, gram_ms=default_ms)
     for tile in TILES:
         for queue in QUEUE:
             if (tile, queue) == (default["tile"], default["queue"]):
                 continue
-            ms = trial(tile, default["band"], default["group"], queue)
+            ms = trial(tile, default["band"], default["group"], queue, kc0)
             if ms < best["gram_ms"]:
                 best = {"tile": tile, "band": default["band"], "group": default["group"],
-                        "queue": queue, "gram_ms": ms}
+                        "queue": queue, "kchunk": kc0, "gram_ms": ms}
     tile, queue = best["tile"], best["queue"]
     for band in BANDS:
         for group in GROUPS:
             if (band, group) == (best["band"], best["group"]):
                 continue
-            ms = trial(tile, band, group, queue)
+            ms = trial(tile, band, group, queue, kc0)
             if ms < best["gram_ms"]:
                 best = {"tile": tile, "band": band, "group": group, "queue": queue,
-                        "gram_ms": ms}
-    plan.set_tuning(best["tile"], best["band"], best["group"], best["queue"])
+                        "kchunk": kc0, "gram_ms": ms}
+    for kchunk in KCHUNKS:
+        if kchunk == best["kchunk"]:
+            continue
+        ms = trial(best["tile"], best["band"], best["group"], best["queue"], kchunk)
+        if ms < best["gram_ms"]:
+            best = dict(best, kchunk=kchunk, gram_ms=ms)
+    plan.set_tuning(best["tile"], best["band"], best["group"], best["queue"], best["kchunk"])
     entry = dict(best, default_ms=default_ms, trials=timings)
     if record:
         save_entry(plan_key(plan, nframes), entry, path)

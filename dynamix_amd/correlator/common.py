@@ -137,14 +137,16 @@ class Plan(object):
                                             self.scale.ctypes.data))
 
     def get_tuning(self):
-        """The plan's Gram launch tuning: dict(tile, band, group, queue)."""
+        """The plan's Gram launch tuning: dict(tile, band, group, queue, kchunk)."""
         t = _lib.Tuning()
         _lib.check(self.lib.dxg2_plan_get_tuning(self.handle, ctypes.byref(t)))
-        return {"tile": t.tile, "band": t.band, "group": t.group, "queue": t.queue}
+        return {"tile": t.tile, "band": t.band, "group": t.group, "queue": t.queue,
+                "kchunk": t.kchunk}
 
-    def set_tuning(self, tile, band, group, queue):
-        """Rebuild the Gram work lists for a tuning (waits for the device)."""
-        t = _lib.Tuning(int(tile), int(band), int(group), int(queue))
+    def set_tuning(self, tile, band, group, queue, kchunk=0):
+        """Rebuild the Gram work lists for a tuning (waits for the device); kchunk 0 is
+        the dtype's default."""
+        t = _lib.Tuning(int(tile), int(band), int(group), int(queue), int(kchunk))
         _lib.check(self.lib.dxg2_plan_set_tuning(self.handle, ctypes.byref(t)),
                    "dxg2_plan_set_tuning")
 

@@ -201,6 +201,7 @@ struct dxg2_plan {
     int tune_band = 4;    // tile rows per raster band of the Gram work list
     int tune_group = 32;  // consecutive items per XCD chunk (0: no XCD permutation)
     int tune_kchunk = MAX_KBLOCKS_PER_ITEM;
+    int gather_frames = 64;  // frames per workgroup of the general gather (a multiple of 8)
     // graph replay of the dense chain (dxg2_correlate_dense_graph)
     hipStream_t cap_stream = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -251,9 +252,10 @@ struct dxg2_plan {
     int64_t* d_sums = nullptr;
     int64_t* d_psums = nullptr;   // digit planes: S_H, S_L, then S_Lw rows (n_bins each)
     uint8_t* d_wflag = nullptr;   // uint16: per detector pixel, a count > 255 in some frame
-    int32_t* d_pixw = nullptr;    // uint16: the pixel lists reordered wide-first
+    int32_t* d_pixw = nullptr;    // uint16: per bin, its wide pixels (count[b] of them)
     int* d_nwkb = nullptr;        // uint16: per bin, k-blocks holding the wide pixels; then
-                                  // their maximum over the bins
+                                  // their maximum over the bins; then per bin the number
+                                  // of wide pixels
     int64_t* d_prefix = nullptr;  // per bin and plane: prefix sums of S(t), N + 1 values
     int64_t* d_smax = nullptr;    // per bin: max |S(t)|
     double* d_dpre = nullptr;     // per bin and lag: FP64 Den(tau) before the finalise
@@ -311,6 +313,13 @@ int env_int(const char* name, int dflt) {
     return (v && *v) ? atoi(v) : dflt;
 }
 
+// K-blocks per Gram work item by default.  uint16 plans split the k-range into chunks of
+// 512: with N = 5,000 a tile row's frame panel of a large bin (256 frames x 3,000
+// k-blocks x 64 bytes, 49 MB) outgrows an XCD's L2, and tiles that walk the same chunk
+// at the same time share it there (cfg3 Gram 49.1 -> 44.3 ms); 8-bit plans keep whole
+// items (at most 2,047 k-blocks: the int32 accumulator bound).
+int default_kchunk(int dtype) { return dtype == DXG2_U16 ? 512 : MAX_KBLOCKS_PER_ITEM; }
+
 // Slots of the general gather (gather_slots_kernel): every k-block of every bin sorted by
 // the detector position of its first pixel, 16 per workgroup item (four per wave; a slot
 // past the item's last k-block repeats its wave's last one, a wave with none gets kb -1),
@@ -337,7 +346,7 @@ void build_gather_slots(dxg2_plan* p) {
     }
     Item pad;
     pad.fill(KbWork{0, -1});
-    items = remap_for_xcds(items, GATHER_XCD_GROUP, pad);
+    items = remap_for_xcds(items, std::max(1, env_int("DXG2_GATHER_GROUP", GATHER_XCD_GROUP)), pad);
     p->gather_slots.clear();
     for (const Item& it : items) p->gather_slots.insert(p->gather_slots.end(), it.begin(), it.end());
 }
@@ -847,12 +856,13 @@ int64_t* dense_gather(dxg2_plan* p, const void* d_frames, int64_t* d_sums, hipSt
     } else {
         launch_gather_slots((const uint8_t*)d_frames, p->dtype, hw, p->N, p->nrows, p->d_pix,
                             p->d_bins, p->d_gslots, (int)(p->gather_slots.size() / 16),
-                            p->d_slab, s, flagw);
+                            p->gather_frames, p->d_slab, s, flagw);
     }
     if (flagw) {
-        launch_fill_i32(p->d_nwkb + p->n_bins, 1, s);
-        launch_wide_partition(p->d_pix, p->d_bins, p->n_bins, p->d_wflag, p->d_pixw, p->d_nwkb,
-                              s);
+        int64_t max_npix = 0;
+        for (auto n : p->npix) max_npix = std::max(max_npix, n);
+        launch_wide_partition(p->d_pix, p->d_bins, p->n_bins, (int)max_npix, p->d_wflag,
+                              p->d_pixw, p->d_nwkb, s);
         launch_wide_gather((const uint16_t*)d_frames, hw, p->N, p->nrows, p->d_pix, p->d_pixw,
                            p->d_bins, p->n_bins, p->d_nwkb, p->d_slab, s);
         launch_wide_low_sums(p->d_slab, p->d_bins, p->d_nwkb, p->n_bins, p->N, p->nrows,
@@ -965,8 +975,9 @@ dxg2_plan* build_plan_host(const dxg2_plan_desc* desc, int* rc) {
     // dxg2_plan_set_tuning replaces them.
     p->tune_band = std::max(1, env_int("DXG2_GRAM_BAND", 4));
     p->tune_group = std::max(0, env_int("DXG2_GRAM_GROUP", 32));
+    p->gather_frames = std::max(8, env_int("DXG2_GATHER_FRAMES", 64) / 8 * 8);
     p->tune_kchunk = std::min(MAX_KBLOCKS_PER_ITEM,
-                              std::max(1, env_int("DXG2_GRAM_KCHUNK", MAX_KBLOCKS_PER_ITEM)));
+                              std::max(1, env_int("DXG2_GRAM_KCHUNK", default_kchunk(p->dtype))));
     p->use_queue = env_int("DXG2_GRAM_QUEUE", 1) != 0 && !(desc->flags & DXG2_FLAG_NO_QUEUE);
 
     // qmask -> bins (common.py:36-46)
@@ -1150,7 +1161,7 @@ int dxg2_plan_create(const dxg2_plan_desc* desc, dxg2_plan** out) {
         rc = set_error(DXG2_ENOMEM, "cannot allocate digit-plane sums workspace");
     if (!rc && planes == 2 &&
         (dev_alloc(p->alloc, (void**)&p->d_wflag, (size_t)p->H * p->W + 16, s0) != hipSuccess ||
-         dev_alloc(p->alloc, (void**)&p->d_nwkb, (size_t)(p->n_bins + 1) * sizeof(int),
+         dev_alloc(p->alloc, (void**)&p->d_nwkb, (size_t)(2 * p->n_bins + 1) * sizeof(int),
                    s0) != hipSuccess))
         rc = set_error(DXG2_ENOMEM, "cannot allocate wide-pixel workspace");
     if (!rc && planes == 2) rc = upload((void**)&p->d_pixw, p->pix_idx.data(), p->pix_idx.size() * 4);
@@ -1344,28 +1355,30 @@ int dxg2_plan_get_tuning(const dxg2_plan* p, dxg2_tuning* t) {
     t->band = p->tune_band;
     t->group = p->tune_group;
     t->queue = p->use_queue ? 1 : 0;
+    t->kchunk = p->tune_kchunk;
     return DXG2_OK;
 }
 
 int dxg2_plan_set_tuning(dxg2_plan* p, const dxg2_tuning* t) {
     if (!p || !t) return set_error(DXG2_EINVAL, "null argument");
     if ((t->tile != 128 && t->tile != 256) || t->band < 1 || t->group < 0 ||
-        (t->queue != 0 && t->queue != 1))
-        return set_error(DXG2_EINVAL, "bad tuning (tile %d, band %d, group %d, queue %d)",
-                         t->tile, t->band, t->group, t->queue);
+        (t->queue != 0 && t->queue != 1) || t->kchunk < 0 || t->kchunk > MAX_KBLOCKS_PER_ITEM)
+        return set_error(DXG2_EINVAL,
+                         "bad tuning (tile %d, band %d, group %d, queue %d, kchunk %d)", t->tile,
+                         t->band, t->group, t->queue, t->kchunk);
     DXG2_HIP(hipDeviceSynchronize());  // calls on the plan may still read its work lists
     // The new work lists are built on a copy of the plan's tuning state and uploaded into
     // new buffers; the plan switches to them only when every upload succeeded, so a
     // failure leaves it whole, with its previous tuning.
     struct TuningState {
-        int tile, band, group;
+        int tile, band, group, kchunk;
         bool queue;
         std::vector<GramWork> mwork, xwork, raster, xraster, qm, qx;
         int q_off[18];
         int gram_group;
     };
     auto save = [&]() {
-        TuningState st{p->gram_tile, p->tune_band, p->tune_group, p->use_queue,
+        TuningState st{p->gram_tile, p->tune_band, p->tune_group, p->tune_kchunk, p->use_queue,
                        p->gram_work, p->gram_xwork, p->gram_raster, p->gram_xraster,
                        p->qm_work, p->qx_work, {}, p->gram_group};
         memcpy(st.q_off, p->q_off, sizeof(p->q_off));
@@ -1375,6 +1388,7 @@ int dxg2_plan_set_tuning(dxg2_plan* p, const dxg2_tuning* t) {
         p->gram_tile = st.tile;
         p->tune_band = st.band;
         p->tune_group = st.group;
+        p->tune_kchunk = st.kchunk;
         p->use_queue = st.queue;
         p->gram_work.swap(st.mwork);
         p->gram_xwork.swap(st.xwork);
@@ -1389,6 +1403,7 @@ int dxg2_plan_set_tuning(dxg2_plan* p, const dxg2_tuning* t) {
     p->gram_tile = t->tile;
     p->tune_band = t->band;
     p->tune_group = t->group;
+    p->tune_kchunk = t->kchunk > 0 ? t->kchunk : default_kchunk(p->dtype);
     p->use_queue = t->queue != 0;
     p->gram_work.clear();
     p->gram_xwork.clear();

@@ -141,7 +141,7 @@ void launch_lds_atomic_rate(int mode, int blocks, int iters, uint32_t* out, hipS
 void launch_fill_i64(int64_t* p, int64_t n, hipStream_t s);
 void launch_gather_slots(const uint8_t* frames, int dtype, int64_t hw, int nframes, int nrows,
                          const int32_t* pix_idx, const BinDev* bins, const KbWork* kbl,
-                         int n_items, int8_t* slab, hipStream_t s, uint32_t* flagw);
+                         int n_items, int tf, int8_t* slab, hipStream_t s, uint32_t* flagw);
 void launch_wide_gather(const uint16_t* frames, int64_t hw, int nframes, int nrows,
                         const int32_t* pix, const int32_t* pix_w, const BinDev* bins, int n_bins,
                         const int* nwkb, int8_t* slab, hipStream_t s);
@@ -152,8 +152,8 @@ void launch_gather_band(const uint8_t* frames, int dtype, int64_t hw, int width,
                         const BandWork* bands, int n_bands, const int2* band_kb,
                         const GatherWork* fall, int n_fall, int8_t* slab, hipStream_t s);
 // cross: the work items are cross-plane ([H|L] [L|H]^T) items of uint16 frames.
-// nwkb (uint16 slabs filled wide-first, else null): per bin, the k-blocks the HH and
-// cross items are clipped to.
+// nwkb (uint16 slabs with the wide pixels apart, else null): per bin, the k-blocks the
+// HH and cross items are clipped to (item_range in k_gram.hip).
 void launch_gram_i8(int tile, bool cross, const int8_t* slab, const BinDev* bins,
                     const GramWork* work, int n_work, int nframes, int nrows, int biased,
                     const int* nwkb, int64_t* ugram, int64_t* sums, hipStream_t s);
@@ -163,11 +163,12 @@ void launch_gram_i8_queue(int tile, bool cross, const int8_t* slab, const BinDev
                           const GramWork* work, int n_work, const int* qoff, int* heads,
                           int nframes, int nrows, int biased, const int* nwkb, int64_t* ugram,
                           int64_t* sums, hipStream_t s);
-// uint16 wide pixels first (k_wide.hip): flags of the pixels with a count > 255 in any
-// frame, each bin's pixel list reordered wide-first into pix_w, and nwkb per bin.
-void launch_wide_partition(const int32_t* pix, const BinDev* bins, int n_bins,
+// uint16 wide pixels apart (k_wide.hip): from the flags of the pixels with a count > 255
+// in some frame, each bin's wide pixels listed in pix_w, their count and clip = nwkb per
+// bin (nwkb[n_bins] the largest clip, nwkb[n_bins + 1 + b] the counts).
+void launch_wide_partition(const int32_t* pix, const BinDev* bins, int n_bins, int max_npix,
                            const uint8_t* flag, int32_t* pix_w, int* nwkb, hipStream_t s);
-// Low-byte frame sums over each bin's first nwkb k-blocks of the low plane.
+// Low-byte frame sums over the nwkb k-blocks each bin's cross term reads them from.
 void launch_wide_low_sums(const int8_t* slab, const BinDev* bins, const int* nwkb, int n_bins,
                           int nframes, int nrows, int64_t* lsums, hipStream_t s);
 // Device queries of the persistent launch's grid size, done once per device (outside

@@ -318,8 +318,8 @@ __device__ __forceinline__ int64_t edge_sums(const int64_t* P, int nframes, int 
 // numerators of the Gram and event paths) or double (FFT path).  mode 0: numerators as
 // accumulated; 1: uint8 bias correction; 2: uint16 digit planes (ugram rows b, n_bins + b,
 // 2 n_bins + b hold the biased HH, LL and cross sums); 3: as 2, with the HH and cross
-// sums taken over the first nwkb[b] k-blocks only (pixels ordered wide-first, so the high
-// bytes past them are zero):
+// sums taken over nwkb[b] k-blocks of the bin's wide pixels only (the high bytes of every
+// other pixel are zero):
 //   HH = U_HH + 128 E_H - 2^14 K_w (N-tau),   X = U_X + 128 (E_H + E_Lw) - 2^15 K_w (N-tau)
 // with K_w = 64 nwkb[b] and E_Lw from the low-byte sums over those k-blocks (prefix
 // plane 2).

@@ -152,19 +152,14 @@ __device__ __forceinline__ void gather_wave4(const T* const (&src)[4], const uin
 // BIAS (uint8 and uint16 input): bytes are stored as v ^ 0x80 = v - 128 (i8 MFMA
 // operands); padded list entries (index -1) read as 0 and become -128 the same way.
 // T = uint16_t writes the high bytes to k-block kb and the low bytes to k-block
-// kb + nkb of the bin (two digit planes); with nwkb (pixel lists ordered wide-first) the
-// high bytes of k-blocks past nwkb[bin], all zero and never read by the clipped Gram, are
-// not written.
+// kb + nkb of the bin (two digit planes).
 template <typename T, bool BIAS>
 __device__ __forceinline__ void gather_general(const T* __restrict__ frames, int64_t hw,
                                                int nframes, int nrows,
                                                const int32_t* __restrict__ pix_idx,
                                                const BinDev* __restrict__ bins, const GatherWork w,
-                                               int t0, int tend, int8_t* __restrict__ slab,
-                                               const int* __restrict__ nwkb = nullptr) {
-    constexpr bool WIDE = sizeof(T) == 2;
+                                               int t0, int tend, int8_t* __restrict__ slab) {
     const BinDev bd = bins[w.bin];
-    const int hi_end = WIDE && nwkb ? nwkb[w.bin] : bd.nkb;  // k-blocks whose high bytes go out
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int64_t plane = (int64_t)bd.nkb * nrows * KB;  // bytes between digit planes
@@ -188,7 +183,7 @@ __device__ __forceinline__ void gather_general(const T* __restrict__ frames, int
             keep[j] = p >= 0 ? 0xFFFFu : 0u;
             dst[j] = slab + bd.slab_off;
             pl[j] = plane;
-            hi[j] = kbj[j] < hi_end;
+            hi[j] = true;
         }
         gather_wave4<T, BIAS>(src, keep, dst, kbj, pl, hi, hw, nframes, nrows, t0, tend);
     }
@@ -205,13 +200,14 @@ template <typename T, bool BIAS, bool FLAG>
 __global__ __launch_bounds__(256) void gather_slots_kernel(
     const T* __restrict__ frames, int64_t hw, int nframes, int nrows,
     const int32_t* __restrict__ pix_idx, const BinDev* __restrict__ bins,
-    const KbWork* __restrict__ kbl, int8_t* __restrict__ slab, uint32_t* __restrict__ flagw) {
+    const KbWork* __restrict__ kbl, int tf, int8_t* __restrict__ slab,
+    uint32_t* __restrict__ flagw) {
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const KbWork* sl = kbl + (int64_t)blockIdx.x * 16 + 4 * wave;
     if (sl[0].kb < 0) return;
-    const int t0 = blockIdx.y * GATHER_TF;
-    const int tend = min(nframes, t0 + GATHER_TF);
+    const int t0 = blockIdx.y * tf;
+    const int tend = min(nframes, t0 + tf);
     int kbj[4], pix[4];
     const T* src[4];
     uint32_t keep[4];
@@ -235,23 +231,26 @@ __global__ __launch_bounds__(256) void gather_slots_kernel(
                                 flagw);
 }
 
-// uint16 wide pixels (after gather_slots_kernel<FLAG> and wide_partition_kernel).  For a
+// uint16 wide pixels (after gather_slots_kernel<FLAG> and launch_wide_partition).  For a
 // bin with clip = nwkb[b] wide k-blocks:
-//  * 2 clip <= nkb: k-blocks kb < clip of the wide-first list go to the high plane's
-//    k-block kb (high bytes) and k-block clip + kb (low bytes);
+//  * 2 clip <= nkb: k-blocks kb < clip of the bin's wide list (entries past its
+//    n_wide = count[b] read as padding) go to the high plane's k-block kb (high bytes)
+//    and k-block clip + kb (low bytes);
 //  * otherwise (clip = nkb): every k-block of the list in its own order, high bytes to
 //    k-block kb (and the low bytes again to the low plane, unchanged).
-// Work units (16 k-blocks of a bin from kb0 = 16 level, one frame tile) up to the level
-// of the largest clip (nwkb[n_bins]) are taken by a fixed grid in turn: the wide
-// k-blocks are a small share of the slab, so a grid over every k-block would be mostly
-// empty workgroups.
+// Work units (4 k-blocks of a bin from kb0 = 4 level, one frame tile; each wave a quarter
+// of the tile's frames) up to the level of the largest clip (nwkb[n_bins]) are taken by
+// a fixed grid in turn: the wide k-blocks are a small share of the slab, so a grid over
+// every k-block would be mostly empty workgroups, and scattered hot pixels make these
+// loads latency-bound, so every wave of a unit has frames to load.
 __global__ __launch_bounds__(256) void wide_gather_kernel(
     const uint16_t* __restrict__ frames, int64_t hw, int nframes, int nrows,
     const int32_t* __restrict__ pix, const int32_t* __restrict__ pix_w,
     const BinDev* __restrict__ bins, int n_bins, const int* __restrict__ nwkb,
     int8_t* __restrict__ slab) {
+    constexpr int WF = GATHER_TF / 4;  // frames per wave of a unit
     const int n_tiles = (nframes + GATHER_TF - 1) / GATHER_TF;
-    const int levels = (nwkb[n_bins] + 15) / 16;
+    const int levels = (nwkb[n_bins] + 3) / 4;
     const int64_t units = (int64_t)levels * n_bins * n_tiles;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
@@ -259,17 +258,17 @@ __global__ __launch_bounds__(256) void wide_gather_kernel(
         const int tile = (int)(u % n_tiles);
         const int64_t rest = u / n_tiles;
         const int b = (int)(rest % n_bins);
-        const int kb0 = 16 * (int)(rest / n_bins);
+        const int kbase = 4 * (int)(rest / n_bins);
         const BinDev bd = bins[b];
         const int clip = nwkb[b];
-        const int kend = min(kb0 + 16, clip);
-        const int kbase = kb0 + 4 * wave;
-        if (kbase >= kend) continue;  // per wave: no barrier in this kernel
+        if (kbase >= clip) continue;  // the same for the whole workgroup; no barrier here
         const bool compact = 2 * clip <= bd.nkb;
         const int32_t* list = compact ? pix_w : pix;
-        const int t0 = tile * GATHER_TF;
-        const int tend = min(nframes, t0 + GATHER_TF);
-        const int nk = min(4, kend - kbase);
+        const int n_list = compact ? nwkb[n_bins + 1 + b] : bd.npix;
+        const int t0 = tile * GATHER_TF + wave * WF;
+        const int tend = min(nframes, t0 + WF);
+        if (t0 >= tend) continue;
+        const int nk = min(4, clip - kbase);
         int kbj[4];
         const uint16_t* src[4];
         uint32_t keep[4];
@@ -279,7 +278,8 @@ __global__ __launch_bounds__(256) void wide_gather_kernel(
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             kbj[j] = kbase + min(j, nk - 1);
-            const int p = list[bd.idx_off + (int64_t)kbj[j] * KB + lane];
+            const int e = kbj[j] * KB + lane;
+            const int p = e < n_list ? list[bd.idx_off + e] : -1;
             src[j] = frames + (p >= 0 ? p : 0);
             keep[j] = p >= 0 ? 0xFFFFu : 0u;
             dst[j] = slab + bd.slab_off;
@@ -300,22 +300,22 @@ void launch_wide_gather(const uint16_t* frames, int64_t hw, int nframes, int nro
 
 void launch_gather_slots(const uint8_t* frames, int dtype, int64_t hw, int nframes, int nrows,
                          const int32_t* pix_idx, const BinDev* bins, const KbWork* kbl,
-                         int n_items, int8_t* slab, hipStream_t s, uint32_t* flagw) {
-    if (n_items <= 0 || nframes <= 0) return;
-    dim3 grid((unsigned)n_items, (unsigned)((nframes + GATHER_TF - 1) / GATHER_TF));
+                         int n_items, int tf, int8_t* slab, hipStream_t s, uint32_t* flagw) {
+    if (n_items <= 0 || nframes <= 0 || tf <= 0) return;
+    dim3 grid((unsigned)n_items, (unsigned)((nframes + tf - 1) / tf));
     const uint16_t* f16 = reinterpret_cast<const uint16_t*>(frames);
     if (dtype == DXG2_U16 && flagw)
         gather_slots_kernel<uint16_t, true, true><<<grid, dim3(256), 0, s>>>(
-            f16, hw, nframes, nrows, pix_idx, bins, kbl, slab, flagw);
+            f16, hw, nframes, nrows, pix_idx, bins, kbl, tf, slab, flagw);
     else if (dtype == DXG2_U16)
         gather_slots_kernel<uint16_t, true, false><<<grid, dim3(256), 0, s>>>(
-            f16, hw, nframes, nrows, pix_idx, bins, kbl, slab, nullptr);
+            f16, hw, nframes, nrows, pix_idx, bins, kbl, tf, slab, nullptr);
     else if (dtype == DXG2_U8)
         gather_slots_kernel<uint8_t, true, false><<<grid, dim3(256), 0, s>>>(
-            frames, hw, nframes, nrows, pix_idx, bins, kbl, slab, nullptr);
+            frames, hw, nframes, nrows, pix_idx, bins, kbl, tf, slab, nullptr);
     else
         gather_slots_kernel<uint8_t, false, false><<<grid, dim3(256), 0, s>>>(
-            frames, hw, nframes, nrows, pix_idx, bins, kbl, slab, nullptr);
+            frames, hw, nframes, nrows, pix_idx, bins, kbl, tf, slab, nullptr);
 }
 
 // Band gather (uint8 / int8 frames).  A workgroup takes one band of detector rows and up

@@ -7,12 +7,12 @@
 //  * the general gather (gather_slots_kernel<FLAG>) writes every pixel's low bytes to the
 //    low plane in list order and flags the pixels with h != 0 in some frame, in the
 //    same pass over the stack;
-//  * wide_partition_kernel reorders each bin's pixel list stably, wide pixels first, and
-//    sets clip = nwkb[b] = ceil(n_wide / 64), or nkb when 2 clip > nkb (then every
-//    pixel goes through HH and the cross Gram, in list order);
-//  * wide_gather_kernel writes the first clip k-blocks of the wide-first list to the
-//    high plane: high bytes to k-blocks [0, clip), low bytes to [clip, 2 clip) (with
-//    clip = nkb: every k-block's high bytes, list order);
+//  * wide_compact_kernel lists each bin's wide pixels (n_wide of them) and
+//    wide_clip_kernel sets clip = nwkb[b] = ceil(n_wide / 64), or nkb when 2 clip > nkb
+//    (then every pixel goes through HH and the cross Gram, in list order);
+//  * wide_gather_kernel writes the first clip k-blocks of the wide list (padded past
+//    n_wide) to the high plane: high bytes to k-blocks [0, clip), low bytes to
+//    [clip, 2 clip) (with clip = nkb: every k-block's high bytes, list order);
 //  * HH items stop after clip k-blocks and the cross items read [0, clip) against its
 //    low copy (item_range in k_gram.hip); LL runs over the whole low plane;
 //  * wide_low_sums_kernel gives the low-byte frame sums over the k-blocks the cross Gram
@@ -24,62 +24,37 @@
 
 namespace dxg2 {
 
-// Exclusive scan of one value per thread over a 1024-thread block; adds the block total
-// to `carry` and returns the value's exclusive prefix.
-__device__ __forceinline__ int wide_block_scan(int v, int* wave_tot, int& carry) {
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    int x = v;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const int y = __shfl_up(x, d, 64);
-        if (lane >= d) x += y;
+// The wide pixels of each bin, compacted: one thread per list entry; a wide pixel takes
+// the next free entry of its bin's wide list in pix_w (in no particular order: no sum
+// depends on it), counted in count[b] (zeroed beforehand).  Grid (entry chunks of 256,
+// bins).
+__global__ __launch_bounds__(256) void wide_compact_kernel(const int32_t* __restrict__ pix,
+                                                           const BinDev* __restrict__ bins,
+                                                           const uint8_t* __restrict__ flag,
+                                                           int32_t* __restrict__ pix_w,
+                                                           int* __restrict__ count) {
+    const int b = blockIdx.y;
+    const BinDev bd = bins[b];
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= bd.npix) return;
+    const int p = pix[bd.idx_off + k];
+    if (flag[p]) {
+        const int pos = atomicAdd(count + b, 1);
+        pix_w[bd.idx_off + pos] = p;
     }
-    if (lane == 63) wave_tot[wave] = x;
-    __syncthreads();
-    int before = carry, total = carry;
-    for (int w = 0; w < 16; ++w) {
-        if (w < wave) before += wave_tot[w];
-        total += wave_tot[w];
-    }
-    __syncthreads();
-    carry = total;
-    return before + x - v;
 }
 
-// One 1024-thread block per bin: the bin's pixel list, stably reordered wide-first, into
-// pix_w (same layout as pix; the padding entries past n_b are left as they are), and
-// nwkb[b] = ceil(n_wide / 64), or nkb when twice that exceeds nkb; nwkb[n_bins] gets the
-// largest of them.  Two passes over the list: count, then place.
-__global__ __launch_bounds__(1024) void wide_partition_kernel(const int32_t* __restrict__ pix,
-                                                              const BinDev* __restrict__ bins,
-                                                              const uint8_t* __restrict__ flag,
-                                                              int32_t* __restrict__ pix_w,
-                                                              int* __restrict__ nwkb) {
-    __shared__ int wave_tot[16];
-    const BinDev bd = bins[blockIdx.x];
-    const int32_t* src = pix + bd.idx_off;
-    int32_t* dst = pix_w + bd.idx_off;
-    const int n = bd.npix;
-    int nw = 0;
-    for (int k0 = 0; k0 < n; k0 += 1024) {
-        const int k = k0 + threadIdx.x;
-        const int w = k < n ? flag[src[k]] : 0;
-        (void)wide_block_scan(w, wave_tot, nw);
-    }
-    int cw = 0, cn = 0;
-    for (int k0 = 0; k0 < n; k0 += 1024) {
-        const int k = k0 + threadIdx.x;
-        const int p = k < n ? src[k] : -1;
-        const int w = k < n ? flag[p] : 0;
-        const int rw = wide_block_scan(w, wave_tot, cw);
-        const int rn = wide_block_scan(k < n ? 1 - w : 0, wave_tot, cn);
-        if (k < n) dst[w ? rw : nw + rn] = p;
-    }
-    if (threadIdx.x == 0) {
-        const int clip = (nw + KB - 1) / KB;
-        const int c = 2 * clip <= bd.nkb ? clip : bd.nkb;
-        nwkb[blockIdx.x] = c;
-        atomicMax(nwkb + gridDim.x, c);  // the largest over the bins (zeroed beforehand)
+// clip = nwkb[b] = ceil(count[b] / 64), or nkb when twice that exceeds nkb; nwkb[n_bins]
+// gets the largest clip (zeroed beforehand).
+__global__ __launch_bounds__(256) void wide_clip_kernel(const BinDev* __restrict__ bins,
+                                                        int n_bins, const int* __restrict__ count,
+                                                        int* __restrict__ nwkb) {
+    for (int b = threadIdx.x; b < n_bins; b += 256) {
+        const int nkb = bins[b].nkb;
+        const int c = (count[b] + KB - 1) / KB;
+        const int clip = 2 * c <= nkb ? c : nkb;
+        nwkb[b] = clip;
+        atomicMax(nwkb + n_bins, clip);
     }
 }
 
@@ -114,10 +89,15 @@ __global__ __launch_bounds__(256) void wide_low_sums_kernel(const int8_t* __rest
     lsums[(int64_t)b * nframes + t] = total + (int64_t)128 * KB * nw;
 }
 
-void launch_wide_partition(const int32_t* pix, const BinDev* bins, int n_bins,
+void launch_wide_partition(const int32_t* pix, const BinDev* bins, int n_bins, int max_npix,
                            const uint8_t* flag, int32_t* pix_w, int* nwkb, hipStream_t s) {
     if (n_bins <= 0) return;
-    wide_partition_kernel<<<dim3((unsigned)n_bins), 1024, 0, s>>>(pix, bins, flag, pix_w, nwkb);
+    int* count = nwkb + n_bins + 1;
+    launch_fill_i32(nwkb + n_bins, n_bins + 1, s);  // the maximum and the counts
+    if (max_npix > 0)
+        wide_compact_kernel<<<dim3((unsigned)((max_npix + 255) / 256), (unsigned)n_bins), 256, 0,
+                              s>>>(pix, bins, flag, pix_w, count);
+    wide_clip_kernel<<<1, 256, 0, s>>>(bins, n_bins, count, nwkb);
 }
 
 void launch_wide_low_sums(const int8_t* slab, const BinDev* bins, const int* nwkb, int n_bins,

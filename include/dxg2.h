@@ -155,16 +155,19 @@ int dxg2_correlate_dense_graph(dxg2_plan* plan, const void* d_frames, int64_t* d
                                int64_t* d_sums, double* d_den, float* d_g2, void* stream);
 
 /* Gram launch tuning of a plan: tile size (128 or 256 frames), raster band (tile rows),
- * XCD chunk (consecutive items per XCD, 0 = none) and launch form (1 = persistent
- * workgroups over per-XCD work queues, 0 = one workgroup per tile).  set_tuning waits
- * for the device, rebuilds the work lists and drops the plan's graph and calc_std
- * workspaces.  Used by the autotuner (dynamix_amd.autotune).  No reference counterpart:
- * the reference takes block_size for its OpenCL kernels (dense.py:153,163). */
+ * XCD chunk (consecutive items per XCD, 0 = none), launch form (1 = persistent
+ * workgroups over per-XCD work queues, 0 = one workgroup per tile) and k-chunk (k-blocks
+ * of 64 pixels per work item, 1..2047; 0 on set = the dtype's default: 512 for uint16,
+ * 2047 otherwise).  set_tuning waits for the device, rebuilds the work lists and drops
+ * the plan's graph and calc_std workspaces.  Used by the autotuner
+ * (dynamix_amd.autotune).  No reference counterpart: the reference takes block_size for
+ * its OpenCL kernels (dense.py:153,163). */
 typedef struct dxg2_tuning {
     int32_t tile;
     int32_t band;
     int32_t group;
     int32_t queue;
+    int32_t kchunk;
 } dxg2_tuning;
 int dxg2_plan_get_tuning(const dxg2_plan* plan, dxg2_tuning* out);
 int dxg2_plan_set_tuning(dxg2_plan* plan, const dxg2_tuning* tuning);

@@ -81,20 +81,21 @@ def test_graph_recaptures_for_new_buffers_and_uint16():
     plan.close()
 
 
-@pytest.mark.parametrize("tile,band,group,queue", [
-    (256, 4, 32, 1), (256, 4, 32, 0), (128, 4, 32, 1), (128, 1, 0, 0), (256, 3, 8, 1),
-    (128, 16, 64, 1)])
-def test_every_tuning_gives_exact_numerators(tile, band, group, queue):
+@pytest.mark.parametrize("tile,band,group,queue,kchunk", [
+    (256, 4, 32, 1, 2047), (256, 4, 32, 0, 2047), (128, 4, 32, 1, 2047), (128, 1, 0, 0, 2047),
+    (256, 3, 8, 1, 2047), (128, 16, 64, 1, 2047), (256, 4, 32, 1, 1), (128, 2, 16, 0, 2)])
+def test_every_tuning_gives_exact_numerators(tile, band, group, queue, kchunk):
     import torch
     from dynamix_amd import _lib
     from dynamix_amd.correlator.common import Plan
-    _announce("tuning %s" % ((tile, band, group, queue),))
+    _announce("tuning %s" % ((tile, band, group, queue, kchunk),))
     f = _stack(12, n=700, shape=(30, 32), mean=0.4)
     q = _qmask(12, (30, 32), 5)
     ref = og.dense_g2(f, q)
     plan = Plan((30, 32), 700, q, np.uint8)
-    plan.set_tuning(tile, band, group, queue)
-    assert plan.get_tuning() == {"tile": tile, "band": band, "group": group, "queue": queue}
+    plan.set_tuning(tile, band, group, queue, kchunk)
+    assert plan.get_tuning() == {"tile": tile, "band": band, "group": group, "queue": queue,
+                                 "kchunk": kchunk}
     d = torch.from_numpy(f).cuda()
     nb = plan.n_bins
     num = torch.empty((nb, 700), dtype=torch.int64, device="cuda")
@@ -111,10 +112,45 @@ def test_every_tuning_gives_exact_numerators(tile, band, group, queue):
     plan.close()
 
 
+@pytest.mark.parametrize("kchunk", [1, 2, 0])
+def test_uint16_kchunks_give_exact_numerators(kchunk):
+    """uint16 with wide pixels handled apart, its HH and cross items split into k-chunks
+    (0: the uint16 default) and clipped to the wide k-blocks: exact against the oracle."""
+    import torch
+    from dynamix_amd import _lib
+    from dynamix_amd.correlator.common import Plan
+    _announce("uint16 kchunk %d" % kchunk)
+    rng = np.random.default_rng(31)
+    shape, n = (40, 48), 330
+    f = rng.poisson(0.7, size=(n,) + shape).astype(np.uint16)
+    hot = rng.choice(shape[0] * shape[1], size=300, replace=False)  # several wide k-blocks
+    f.reshape(n, -1)[:, hot] += rng.integers(0, 900, size=(n, hot.size)).astype(np.uint16)
+    q = _qmask(31, shape, 3)
+    ref = og.dense_g2(f, q)
+    plan = Plan(shape, n, q, np.uint16)
+    t = plan.get_tuning()
+    plan.set_tuning(t["tile"], t["band"], t["group"], t["queue"], kchunk)
+    assert plan.get_tuning()["kchunk"] == (kchunk or 512)
+    d = torch.from_numpy(f.astype(np.int32)).to(torch.int16).cuda()
+    nb = plan.n_bins
+    num = torch.empty((nb, n), dtype=torch.int64, device="cuda")
+    sums = torch.empty_like(num)
+    den = torch.empty((nb, n), dtype=torch.float64, device="cuda")
+    g2 = torch.empty((nb, n), dtype=torch.float32, device="cuda")
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    _lib.check(plan.lib.dxg2_correlate_dense(plan.handle, _lib.ptr(d), _lib.ptr(num),
+                                             _lib.ptr(sums), _lib.ptr(den), _lib.ptr(g2), st))
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(num.cpu().numpy(), ref["num"])
+    np.testing.assert_array_equal(sums.cpu().numpy(), ref["sums"])
+    plan.close()
+
+
 def test_bad_tuning_is_rejected():
     from dynamix_amd.correlator.common import Plan
     plan = Plan((8, 8), 40, None, np.uint8)
-    for bad in [(64, 4, 32, 1), (256, 0, 32, 1), (256, 4, -1, 1), (256, 4, 32, 2)]:
+    for bad in [(64, 4, 32, 1), (256, 0, 32, 1), (256, 4, -1, 1), (256, 4, 32, 2),
+                (256, 4, 32, 1, -1), (256, 4, 32, 1, 2048)]:
         with pytest.raises(ValueError):
             plan.set_tuning(*bad)
     plan.close()
@@ -138,7 +174,8 @@ def test_autotune_records_and_reapplies(tmp_path, monkeypatch):
     assert len(entry["trials"]) >= 10
     plan = c._plans[np.dtype(np.uint8)]
     assert key == autotune.plan_key(plan, 520)
-    assert plan.get_tuning() == {k: entry[k] for k in ("tile", "band", "group", "queue")}
+    assert plan.get_tuning() == {k: entry[k] for k in ("tile", "band", "group", "queue",
+                                                        "kchunk")}
     c2 = MatMulCorrelator((24, 30), 520, q, extra_options={"autotune": True})
     g2 = c2.correlate(f)
     p2 = c2._plans[np.dtype(np.uint8)]
@@ -189,7 +226,7 @@ def test_set_tuning_allocation_failure_keeps_plan_whole():
         num0, g20 = run()
         np.testing.assert_array_equal(num0, ref["num"])
         before = plan.get_tuning()
-        new = {"tile": 128, "band": 2, "group": 16, "queue": 1 - before["queue"]}
+        new = {"tile": 128, "band": 2, "group": 16, "queue": 1 - before["queue"], "kchunk": 3}
         state["fail"] = True
         with pytest.raises(MemoryError):
             plan.set_tuning(**new)
