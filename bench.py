@@ -428,16 +428,19 @@ def bench_serving(seed, n_jobs=200, rates=(20.0, 40.0, 80.0)):
                    "jobs": "%d jobs: shapes %s, 1k-20k frames, 10-100 bins, 70%% dense uint8 / "
                            "30%% events, tau_c 10-500 frames per job"
                            % (n_jobs, list(serving.SERVE_SHAPES))})
-        log("serving at %.0f/s: %.1f jobs/s, p50 %.0f ms, p99 %.0f ms, batch %.2f, failed %d"
+        st["failures"] = [(i, str(e)[:200]) for i, e in st["failures"][:5]]
+        log("serving at %.0f/s: %.1f jobs/s, p50 %.0f ms, p99 %.0f ms, batch %.2f, failed %d%s"
             % (rate, st["jobs_per_s"] or 0, st["latency_p50_ms"], st["latency_p99_ms"],
-               st["mean_batch_size"] or 0, st["failed"]))
+               st["mean_batch_size"] or 0, st["failed"],
+               " (first: %s)" % st["failures"][0][1] if st["failures"] else ""))
         sweep.append(st)
         del srv, recs
         torch.cuda.empty_cache()
     out = dict(sweep[0])
     out["rate_sweep"] = [{k: st[k] for k in ("arrival_rate_per_s", "jobs_per_s", "latency_p50_ms",
                                              "latency_p99_ms", "mean_batch_size",
-                                             "pixel_frames_per_s", "failed")} for st in sweep]
+                                             "pixel_frames_per_s", "failed", "failures")}
+                         for st in sweep]
     return out
 
 

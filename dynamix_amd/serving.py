@@ -7,14 +7,17 @@ frames or events generated on the GPU from the job's seed and correlation time.
 
 The loop is split in two:
 
-  * `Scheduler` -- the policy, in plain Python: it prepares arriving jobs on a small
-    host thread pool, admits a prepared job when its device footprint fits the memory
-    budget next to the jobs already resident and a slot is free, keeps a ready queue,
+  * `Scheduler` -- the policy, in plain Python: it stages the inputs of the next few
+    jobs while their reserved footprint fits the memory budget, prepares arriving jobs
+    on a small host thread pool, admits a prepared job when its device footprint fits
+    the budget next to the jobs already resident and a slot is free, keeps a ready queue,
     and dispatches to `n_streams` compute streams with at most `depth` dispatches in
     flight each.  The dense jobs at the head of the queue (similar sizes, up to
     `max_batch`, under `batch_ops` Gram work) go out as ONE grouped launch, and so do
     up to `max_batch` event jobs (one pair launch over all of them).  Completions are
-    polled and retired in stream order.
+    polled and retired in stream order.  Jobs that arrive faster than they are served
+    wait on the host, holding no device memory, so an overload raises latency rather
+    than failing jobs.
     A job that fails -- in preparation, over the budget, at admission, at dispatch or
     at completion -- fails alone: it is released and recorded with its error, and
     every other job carries on.  A grouped dispatch that fails is retried job by job,
@@ -179,7 +182,11 @@ class Scheduler(object):
     before the job arrives, or None), admit(job)
     (device memory and input, may raise), dispatch(k, jobs) -> handle (may raise),
     poll(handle) -> bool, complete(handle) -> [error or None per job], retire(job) and
-    release(job) (after success / failure; never raise), and `budget` in bytes.
+    release(job) (after success / failure; never raise), and `budget` in bytes; with
+    stage(), the optional footprint_bound(spec) bounds a job's footprint before it is
+    prepared, and that much of the budget (at most all of it) is held from the moment
+    its input is staged; a job whose input_bytes(spec) (optional) exceed the budget
+    fails on arrival, unstaged.
     """
 
     def __init__(self, executor, max_batch=8, n_streams=2, depth=2, max_resident=32,
@@ -187,7 +194,8 @@ class Scheduler(object):
                  sleep=time.sleep):
         self.ex = executor
         # Inputs of the next `lookahead` jobs are staged before they arrive (the job's
-        # frames are resident when it is submitted) by the executor's optional stage().
+        # frames are resident when it is submitted) by the executor's optional stage(),
+        # counted from the last job admitted, so that an overload queues on the host.
         self.lookahead = lookahead
         self.producer = ThreadPoolExecutor(max_workers=1)
         self.max_batch = max_batch
@@ -198,6 +206,7 @@ class Scheduler(object):
         self.prep = ThreadPoolExecutor(max_workers=prep_threads)
         self.clock = clock
         self.sleep = sleep
+        self._t0 = 0.0
 
     def close(self):
         self.prep.shutdown(wait=True)
@@ -209,7 +218,7 @@ class Scheduler(object):
         except Exception as e:  # the job fails alone
             job = Job(spec)
             job.error = "prepare: %s: %s" % (type(e).__name__, e)
-        job.t_prepared = self.clock()
+        job.t_prepared = self.clock() - self._t0
         return job
 
     def _group(self, ready):
@@ -241,17 +250,36 @@ class Scheduler(object):
         order = sorted(specs, key=lambda s: s.arrival)
         pending = collections.deque(order)
         stage = getattr(ex, "stage", None)
+        footprint_bound = getattr(ex, "footprint_bound", None)
+        input_bytes = getattr(ex, "input_bytes", None)
         staged = {}
         n_staged = 0
-        waiting = collections.deque()    # arrived, in preparation or prepared
+        # Inputs are staged, and jobs prepared, at most `lookahead` jobs beyond the last
+        # one that left preparation (admitted or failed), and a job's reservation (the
+        # executor's footprint_bound) counts against the memory budget from the moment
+        # its input is staged: when jobs arrive faster than they are served, the backlog
+        # waits on the host, holding no device memory.
+        n_passed = 0
+        staged_bytes = 0
+        holds = {}                       # job id -> bytes reserved when it was staged
+        too_big = set()                  # jobs whose input alone exceeds the budget
+        arrived = collections.deque()    # arrived, not yet in preparation
+        waiting = collections.deque()    # in preparation or prepared
         ready = collections.deque()      # admitted, not dispatched
         busy = [collections.deque() for _ in range(self.n_streams)]
         done, failed = [], []
         resident_bytes, n_resident = 0, 0
-        t0 = self.clock()
+        t0 = self._t0 = self.clock()
         clock = lambda: self.clock() - t0
 
+        def leave_waiting(j):
+            nonlocal staged_bytes, n_passed
+            staged_bytes -= holds.pop(j.spec.id, 0)
+            n_passed += 1
+
         def fail(jobs, err, held):
+            """held: the jobs were admitted (their footprint is resident); otherwise they
+            fail on their way out of preparation."""
             nonlocal resident_bytes, n_resident
             t = clock()
             for j in jobs:
@@ -261,19 +289,45 @@ class Scheduler(object):
                 if held:
                     resident_bytes -= j.footprint
                     n_resident -= 1
+                else:
+                    leave_waiting(j)
                 failed.append(j)
 
-        while pending or waiting or ready or any(busy):
+        while pending or arrived or waiting or ready or any(busy):
             now = clock()
             progressed = False
-            n_arrived = len(order) - len(pending)
-            while stage and n_staged < len(order) and n_staged - n_arrived < self.lookahead:
+            while stage and n_staged < len(order) and n_staged - n_passed < self.lookahead:
                 spec = order[n_staged]
-                staged[spec.id] = self.producer.submit(stage, spec)
+                est = min(int(footprint_bound(spec)), ex.budget) if footprint_bound else 0
+                if input_bytes and int(input_bytes(spec)) > ex.budget:
+                    too_big.add(spec.id)
+                elif resident_bytes + staged_bytes + est > ex.budget and \
+                        (n_staged > n_passed or n_resident > 0):
+                    break    # staged when memory frees up
+                else:
+                    staged[spec.id] = self.producer.submit(stage, spec)
+                    holds[spec.id] = est
+                    staged_bytes += est
                 n_staged += 1
             while pending and (not realtime or pending[0].arrival <= now):
-                spec = pending.popleft()
+                arrived.append(pending.popleft())
+            while arrived:
+                spec = arrived[0]
+                if spec.id in too_big:
+                    arrived.popleft()
+                    job = Job(spec)
+                    job.t_prepared = clock()
+                    fail([job], "MemoryError: input needs %d bytes, budget %d"
+                         % (int(input_bytes(spec)), ex.budget), False)
+                    progressed = True
+                    continue
+                if stage and spec.id not in staged:
+                    break    # its input is not staged yet
+                if not stage and len(waiting) >= self.lookahead:
+                    break
+                arrived.popleft()
                 waiting.append(self.prep.submit(self._prepare, spec, staged.pop(spec.id, None)))
+                progressed = True
             # completions, in stream order
             for q in busy:
                 while q and ex.poll(q[0][0]):
@@ -296,7 +350,6 @@ class Scheduler(object):
             # admission by footprint and slots
             while waiting and n_resident < self.max_resident and waiting[0].done():
                 job = waiting[0].result()
-                job.t_prepared -= t0
                 if job.error:
                     waiting.popleft()
                     fail([job], job.error, False)
@@ -308,7 +361,9 @@ class Scheduler(object):
                          % (job.footprint, ex.budget), False)
                     progressed = True
                     continue
-                if resident_bytes + job.footprint > ex.budget:
+                # the job's reservation is replaced by its footprint
+                others = staged_bytes - holds.get(job.spec.id, 0)
+                if n_resident > 0 and resident_bytes + others + job.footprint > ex.budget:
                     break
                 waiting.popleft()
                 job.t_admit = clock()
@@ -318,6 +373,7 @@ class Scheduler(object):
                     fail([job], "admit: %s: %s" % (type(e).__name__, e), False)
                     progressed = True
                     continue
+                leave_waiting(job)
                 resident_bytes += job.footprint
                 n_resident += 1
                 ready.append(job)
@@ -449,6 +505,33 @@ class GpuExecutor(object):
                 ev = self.torch.cuda.Event()
                 ev.record(self.input_stream)
         return inp, ev
+
+    def input_bytes(self, spec):
+        """Device bytes of a job's input before it is made: the frames of a dense job;
+        for an event job the offsets alone (its event lists are sized when made)."""
+        h, w = spec.shape
+        if spec.layout == "dense":
+            return spec.nframes * h * w
+        return (h * w + 1) * 4
+
+    def footprint_bound(self, spec):
+        """Device bytes a job may hold, bounded before its plan exists (the scheduler
+        reserves them when the job's input is staged): the input (dense frames; event
+        CSR lists at 1.25 times the expected number of events, as a pixel-frame holds an
+        event with probability below EVENT_MEAN), the outputs and per-bin frame rows, the
+        pixel lists, for dense jobs the slab (every pixel binned, each bin padded to whole
+        64-pixel k-blocks, slab rows padded by up to 256) and the Gram work lists (one
+        item per 256-frame tile pair per bin and k-chunk, in two lists)."""
+        h, w = spec.shape
+        hw, n, nb = h * w, spec.nframes, max(int(spec.n_bins), 1)
+        rows = nb * (n + 2) * 8 * 4 + nb * n * (8 + 8 + 8 + 4) + hw * 4 + (1 << 16)
+        if spec.layout == "dense":
+            nkb = hw // 64 + nb
+            nt = (n + 255) // 256
+            items = nt * (nt + 1) // 2 * (nb + nkb // 2047 + 1)
+            lists = 2 * 40 * items + 32 * nkb
+            return n * hw + (hw + 64 * nb) * (n + 256) + rows + lists
+        return int(1.25 * EVENT_MEAN * n * hw) * 5 + 4 * (hw + 1) * 4 + rows + hw
 
     def prepare(self, spec, staged=None):
         """Plan and footprint of a job; its input comes from stage() (waited for if it

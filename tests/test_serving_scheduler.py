@@ -169,3 +169,71 @@ def test_slot_limit_is_respected():
         sch.close()
     assert st["jobs"] == 25
     assert max(seen) <= 3
+
+
+class StagingExecutor(FakeExecutor):
+    """Inputs staged ahead of admission: `held` counts the staged inputs of jobs not yet
+    admitted, and `peak` the most device memory ever in use (staged + resident).  The
+    reservation of a job is its footprint."""
+
+    def __init__(self, budget, input_bytes, **kw):
+        FakeExecutor.__init__(self, budget, **kw)
+        self.inbytes = input_bytes
+        self.held = 0
+        self.staged_ids = []
+
+    def input_bytes(self, spec):
+        return self.inbytes(spec)
+
+    def footprint_bound(self, spec):
+        return self.footprint(spec)
+
+    def stage(self, spec):
+        with self.lock:
+            self.held += self.inbytes(spec)
+            self.peak = max(self.peak, self.held + self.resident)
+            self.staged_ids.append(spec.id)
+        return spec.id
+
+    def prepare(self, spec, staged=None):
+        job = FakeExecutor.prepare(self, spec, staged)
+        job.staged = staged is not None
+        return job
+
+    def _unstage(self, job):
+        if getattr(job, "staged", False):
+            with self.lock:
+                self.held -= self.inbytes(job.spec)
+            job.staged = False
+
+    def admit(self, job):
+        FakeExecutor.admit(self, job)
+        self._unstage(job)
+        with self.lock:
+            self.peak = max(self.peak, self.held + self.resident)
+
+    def release(self, job):
+        self._unstage(job)
+        FakeExecutor.release(self, job)
+
+
+def test_overload_keeps_staged_inputs_within_budget():
+    """Every job arrives at once, far more than fits: inputs are staged only as memory
+    frees up (staged + resident bytes never exceed the budget), the backlog waits on the
+    host, every job completes, and a job whose input alone exceeds the budget fails
+    alone without being staged."""
+    big = 11
+    inb = lambda s: (5 << 30) if s.id == big else (200 << 20)
+    fp = lambda s: inb(s) + (50 << 20)
+    ex = StagingExecutor(budget=1 << 30, input_bytes=inb, footprint=fp)
+    sch = Scheduler(ex, max_batch=4, n_streams=2, depth=2, lookahead=8)
+    try:
+        st = sch.run(_specs(40), realtime=False)
+    finally:
+        sch.close()
+    assert st["failed"] == 1 and st["failures"][0][0] == big, st["failures"]
+    assert "MemoryError" in st["failures"][0][1]
+    assert st["jobs"] == 39
+    assert big not in ex.staged_ids and len(ex.staged_ids) == 39
+    assert ex.peak <= ex.budget, (ex.peak, ex.budget)
+    assert ex.held == 0 and ex.resident == 0
