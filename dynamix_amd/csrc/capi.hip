@@ -259,6 +259,7 @@ struct dxg2_plan {
     int64_t* d_prefix = nullptr;  // per bin and plane: prefix sums of S(t), N + 1 values
     int64_t* d_smax = nullptr;    // per bin: max |S(t)|
     double* d_dpre = nullptr;     // per bin and lag: FP64 Den(tau) before the finalise
+    DenMfma dmf;                  // i8-MFMA denominator workspace (k_den.hip)
     float* d_pa = nullptr;
     float* d_pg = nullptr;
     float* d_pm = nullptr;
@@ -604,6 +605,12 @@ int ensure_ttcf_buffers(dxg2_plan* p, int b, hipStream_t s) {
     return DXG2_OK;
 }
 
+// Bytes of the i8-MFMA denominator workspace: per bin, two digit planes of the frame
+// sums, each 64 rows of dmf.up bytes.
+size_t den_mfma_bytes(const dxg2_plan* p) {
+    return (size_t)std::max(p->n_bins, 1) * 2 * 64 * (size_t)p->dmf.up;
+}
+
 size_t std_footprint(const dxg2_plan* p) {
     const size_t T = p->gram_tile;
     const size_t nT = (p->N + T - 1) / T;
@@ -902,7 +909,8 @@ void dense_finish(dxg2_plan* p, int64_t* gsums, int64_t* d_num, int64_t* d_sums,
     const bool wide = p->dtype == DXG2_U16;
     const int mode = wide ? (wide_first ? 3 : 2) : (p->dtype == DXG2_U8 ? 1 : 0);
     launch_finalize(gsums, sums, p->d_ugram, p->d_bins, p->n_bins, p->N, mode, p->d_prefix,
-                    p->d_smax, p->d_dpre, wide ? p->d_nwkb : nullptr, d_num, d_den, d_g2, s);
+                    p->d_smax, p->d_dpre, wide ? p->d_nwkb : nullptr, d_num, d_den, d_g2,
+                    p->dmf, s);
 }
 
 }  // namespace
@@ -979,6 +987,10 @@ dxg2_plan* build_plan_host(const dxg2_plan_desc* desc, int* rc) {
     p->tune_kchunk = std::min(MAX_KBLOCKS_PER_ITEM,
                               std::max(1, env_int("DXG2_GRAM_KCHUNK", default_kchunk(p->dtype))));
     p->use_queue = env_int("DXG2_GRAM_QUEUE", 1) != 0 && !(desc->flags & DXG2_FLAG_NO_QUEUE);
+    // i8-MFMA denominators for the bins whose frame sums fit two digits; DXG2_DEN_MFMA=0
+    // leaves every bin to the VALU kernels
+    p->dmf.up = env_int("DXG2_DEN_MFMA", 1) ? den_mfma_columns(p->N) : 0;
+    p->dmf.umax = p->dtype == DXG2_I8 ? 127 : 255;
 
     // qmask -> bins (common.py:36-46)
     std::vector<std::vector<int32_t>> lists;
@@ -1174,6 +1186,11 @@ int dxg2_plan_create(const dxg2_plan_desc* desc, dxg2_plan** out) {
                 dev_alloc(p->alloc, (void**)&p->d_dpre,
                           (size_t)std::max(p->n_bins, 1) * p->N * sizeof(double), s0) != hipSuccess))
         rc = set_error(DXG2_ENOMEM, "cannot allocate denominator workspace");
+    if (!rc && p->dmf.up > 0 &&
+        dev_alloc(p->alloc, (void**)&p->dmf.v, den_mfma_bytes(p), s0) != hipSuccess) {
+        p->dmf.v = nullptr;
+        rc = set_error(DXG2_ENOMEM, "cannot allocate denominator workspace");
+    }
     if (!rc && hipStreamSynchronize(s0) != hipSuccess)
         rc = set_error(DXG2_EHIP, "plan upload failed");
     if (rc) {
@@ -1206,7 +1223,8 @@ static void plan_release(dxg2_plan* p, hipStream_t s) {
     drop_graph(p);
     if (p->cap_stream) (void)hipStreamDestroy(p->cap_stream);
     void* bufs[] = {p->d_pix,   p->d_bins, p->d_gwork, p->d_gslots, p->d_mwork, p->d_xwork, p->d_slab,
-                    p->d_ugram, p->d_sums, p->d_psums, p->d_prefix, p->d_smax, p->d_dpre, p->d_pa,
+                    p->d_ugram, p->d_sums, p->d_psums, p->d_prefix, p->d_smax, p->d_dpre,
+                    p->dmf.v, p->d_pa,
                     p->d_pg, p->d_pm,
                     p->d_ework, p->d_err,  p->d_cnt,   p->d_scan,  p->d_off,
                     p->d_etimes, p->d_edata, p->d_kbwork, p->d_kbgather, p->d_xs, p->d_tw,
@@ -1250,7 +1268,7 @@ int dxg2_plan_footprint(const dxg2_plan* p, int32_t path, size_t* bytes) {
     const size_t nb_n = (size_t)std::max(p->n_bins, 1) * p->N * sizeof(int64_t);
     const int planes = slab_planes(p->dtype);
     size_t common = p->pix_idx.size() * 4 + p->bins.size() * sizeof(BinDev) +
-                    (planes == 2 ? 8 : 3) * nb_n +
+                    (p->dmf.up > 0 ? den_mfma_bytes(p) : 0) + (planes == 2 ? 8 : 3) * nb_n +
                     (size_t)std::max(p->n_bins, 1) * ((planes == 2 ? 3 : 1) * (p->N + 1) + 1) *
                         sizeof(int64_t) +
                     (planes == 2 ? p->pix_idx.size() * 4 + (size_t)p->H * p->W +
@@ -1793,7 +1811,7 @@ int dxg2_batch_correlate_events(dxg2_batch* b, int32_t n, dxg2_plan* const* plan
         if (p->n_bins == 0) continue;
         launch_finalize(jobs[i].sums, jobs[i].sums, p->d_ugram, p->d_bins, p->n_bins, p->N, 0,
                         p->d_prefix, p->d_smax, p->d_dpre, nullptr, d_num ? d_num[i] : nullptr,
-                        d_den ? d_den[i] : nullptr, d_g2[i], s);
+                        d_den ? d_den[i] : nullptr, d_g2[i], p->dmf, s);
     }
     DXG2_HIP(hipGetLastError());
     return DXG2_OK;
@@ -1851,7 +1869,7 @@ static int events_from_csr(dxg2_plan* p, const int32_t* times, const void* data,
                        (int)p->event_work.size(), p->N, nnz, p->d_ugram, sums, p->d_err, s);
     if (ev) DXG2_HIP(hipEventRecord(ev[2], s));
     launch_finalize(sums, sums, p->d_ugram, p->d_bins, p->n_bins, p->N, 0, p->d_prefix,
-                    p->d_smax, p->d_dpre, nullptr, d_num, d_den, d_g2, s);
+                    p->d_smax, p->d_dpre, nullptr, d_num, d_den, d_g2, p->dmf, s);
     if (ev) DXG2_HIP(hipEventRecord(ev[3], s));
     DXG2_HIP(hipGetLastError());
     return DXG2_OK;
@@ -1953,7 +1971,7 @@ int dxg2_correlate_fft(dxg2_plan* p, const void* d_frames, double* d_num, int64_
     launch_fft_inverse(p->d_power, p->fft_L, p->n_bins, p->N, p->d_fnum, s);
     if (ev) DXG2_HIP(hipEventRecord(ev[2], s));
     launch_finalize_f64(sums, p->d_fnum, p->d_bins, p->n_bins, p->N, p->d_prefix, p->d_smax,
-                        p->d_dpre, d_num, d_den, d_g2, s);
+                        p->d_dpre, d_num, d_den, d_g2, p->dmf, s);
     if (ev) DXG2_HIP(hipEventRecord(ev[3], s));
     DXG2_HIP(hipGetLastError());
     return DXG2_OK;

@@ -177,13 +177,28 @@ void gram_queue_prepare(int tile);
 // Grouped launch: work items of several plans, each naming its job descriptor.
 void launch_gram_i8_jobs(int tile, bool cross, const GramJob* jobs, const GramWork* work,
                          int n_work, hipStream_t s);
+// Workspace of the i8-MFMA denominator (k_den.hip): v holds n_bins x 2 x 64 x up bytes
+// (two digit planes per bin); umax is the largest max |S| taken as one biased digit (255
+// for unsigned frames, 127 for signed: none); 0 switches the path off.
+struct DenMfma {
+    int8_t* v = nullptr;
+    int64_t up = 0;
+    int umax = 0;
+};
+// Columns of V per row for N frames (0: more than 32,768 rows of 64 frames, not taken).
+int64_t den_mfma_columns(int nframes);
+// Largest max |S| of a bin the i8-MFMA path takes (two digits).
+int den_mfma_max();
+void launch_den_mfma(const int64_t* sums, int n_bins, int nframes, const int64_t* smax,
+                     const int64_t* prefix, int planes, const DenMfma& dm, double* den,
+                     hipStream_t s);
 // mode 0: numerators as accumulated; 1: uint8 bias correction; 2: uint16 digit planes
 // (ugram holds HH, LL, cross rows; plane_sums the per-plane S_H, S_L rows, and `sums`
 // receives S = 256 S_H + S_L).
 void launch_finalize(const int64_t* plane_sums, int64_t* sums, const int64_t* ugram,
                      const BinDev* bins, int n_bins, int nframes, int mode, int64_t* prefix,
                      int64_t* smax, double* dpre, const int* nwkb, int64_t* num, double* den,
-                     float* g2, hipStream_t s);
+                     float* g2, const DenMfma& dm, hipStream_t s);
 void launch_plain_bin(const uint8_t* frames, int dtype, int64_t hw, int nframes,
                       const int32_t* pix, int npix, float* a, float* gram, float* means,
                       float* g2_row, hipStream_t s);
@@ -213,7 +228,7 @@ hipError_t launch_coo_to_csr(int dtype, const uint32_t* pix, const uint32_t* fra
 void launch_fill_i32(int* p, int64_t n, hipStream_t s);
 void launch_finalize_f64(const int64_t* sums, const double* fnum, const BinDev* bins, int n_bins,
                          int nframes, int64_t* prefix, int64_t* smax, double* dpre, double* num,
-                         double* den, float* g2, hipStream_t s);
+                         double* den, float* g2, const DenMfma& dm, hipStream_t s);
 void launch_gather_pm(const uint8_t* frames, int dtype, int64_t hw, int nframes, int nrows,
                       const int32_t* pix_idx, const BinDev* bins, const KbWork* work, int n_work,
                       uint8_t* xs, hipStream_t s);

@@ -68,7 +68,9 @@ __device__ __forceinline__ int64_t block_scan(int64_t v, int64_t* wave_tot, int6
 // planes): rows b (S_H) and n_bins + b (S_L), combined into sums[b][t] = 256 S_H + S_L;
 // PLANES = 3 adds rows 2 n_bins + b (S_Lw, the low bytes over the wide k-blocks, which
 // only the bias correction uses).  Writes the exclusive prefix sums of each plane,
-// prefix[plane * n_bins + b][t] for t = 0..N, and smax[b] = max |S|.
+// prefix[plane * n_bins + b][t] for t = 0..N, and smax[b] = max |S|.  Each pass takes
+// PREP_ITEMS consecutive frames per thread (fewer with more planes, to stay in registers):
+// a serial prefix in registers, then one block scan of the thread totals.
 template <int PLANES>
 __global__ __launch_bounds__(PREP_THREADS) void sums_prep_kernel(
     const int64_t* __restrict__ plane_sums, int64_t* __restrict__ sums, int nframes,
@@ -84,28 +86,46 @@ __global__ __launch_bounds__(PREP_THREADS) void sums_prep_kernel(
     int64_t* P0 = prefix + (int64_t)b * (nframes + 1);
     int64_t* P1 = prefix + (int64_t)(nb + b) * (nframes + 1);
     int64_t* P2 = prefix + (int64_t)(2 * nb + b) * (nframes + 1);
+    constexpr int PREP_ITEMS = PLANES == 1 ? 8 : (PLANES == 2 ? 4 : 1);
     int64_t c0 = 0, c1 = 0, c2 = 0, mx = 0;
-    for (int t0 = 0; t0 < nframes; t0 += PREP_THREADS) {
-        const int t = t0 + tid;
-        const int64_t v0 = t < nframes ? S0[t] : 0;
-        const int64_t e0 = block_scan(v0, wave_tot, c0);
-        if (t < nframes) P0[t] = e0;
-        int64_t v = v0;
-        if (WIDE) {
-            const int64_t v1 = t < nframes ? S1[t] : 0;
-            const int64_t e1 = block_scan(v1, wave_tot, c1);
+    // exclusive prefixes of this thread's PREP_ITEMS frames of one plane, in place
+    auto scan = [&](int64_t (&v)[PREP_ITEMS], int64_t& carry) {
+        int64_t run = 0;
+#pragma unroll
+        for (int k = 0; k < PREP_ITEMS; ++k) {
+            const int64_t x = v[k];
+            v[k] = run;
+            run += x;
+        }
+        const int64_t base = block_scan(run, wave_tot, carry);
+#pragma unroll
+        for (int k = 0; k < PREP_ITEMS; ++k) v[k] += base;
+    };
+    for (int t0 = 0; t0 < nframes; t0 += PREP_THREADS * PREP_ITEMS) {
+        const int tb = t0 + PREP_ITEMS * tid;
+        int64_t v0[PREP_ITEMS], v1[PREP_ITEMS], v2[PREP_ITEMS];
+#pragma unroll
+        for (int k = 0; k < PREP_ITEMS; ++k) {
+            const int t = tb + k;
+            v0[k] = t < nframes ? S0[t] : 0;
+            if (WIDE) v1[k] = t < nframes ? S1[t] : 0;
+            if (PLANES == 3) v2[k] = t < nframes ? S2[t] : 0;
+            const int64_t v = WIDE ? 256 * v0[k] + v1[k] : v0[k];
+            if (WIDE && t < nframes) sums[(int64_t)b * nframes + t] = v;
+            mx = max(mx, v < 0 ? -v : v);
+        }
+        scan(v0, c0);
+        if (WIDE) scan(v1, c1);
+        if (PLANES == 3) scan(v2, c2);
+#pragma unroll
+        for (int k = 0; k < PREP_ITEMS; ++k) {
+            const int t = tb + k;
             if (t < nframes) {
-                P1[t] = e1;
-                v = 256 * v0 + v1;
-                sums[(int64_t)b * nframes + t] = v;
+                P0[t] = v0[k];
+                if (WIDE) P1[t] = v1[k];
+                if (PLANES == 3) P2[t] = v2[k];
             }
         }
-        if (PLANES == 3) {
-            const int64_t v2 = t < nframes ? S2[t] : 0;
-            const int64_t e2 = block_scan(v2, wave_tot, c2);
-            if (t < nframes) P2[t] = e2;
-        }
-        mx = max(mx, v < 0 ? -v : v);
     }
     if (tid == 0) {
         P0[nframes] = c0;
@@ -197,7 +217,7 @@ __device__ __forceinline__ uint32_t pack_s16(int64_t a, int64_t b) {
 __global__ __launch_bounds__(FIN_THREADS) void den_i16_kernel(const int64_t* __restrict__ sums,
                                                               int nframes, int cpb,
                                                               const int64_t* __restrict__ smax,
-                                                              double* __restrict__ den) {
+                                                              int skip, double* __restrict__ den) {
     __shared__ uint32_t sa[DEN_CH / 2];
     __shared__ uint32_t sb[DEN_SB + DEN_SB / 16 + 1];
     const int b = blockIdx.y;
@@ -206,6 +226,7 @@ __global__ __launch_bounds__(FIN_THREADS) void den_i16_kernel(const int64_t* __r
     const double m = (double)smax[b];
     if (!((double)nframes * m * m < 4503599627370496.0)) return;  // 128-bit path
     if (smax[b] > DEN_I16_MAX) return;                            // den_fp64_kernel's bin
+    if (smax[b] <= skip) return;                                  // den_mfma_kernel's bin
     const int tb = blockIdx.z * cpb * DEN_CH;
     const int te = min(nframes - tau0, tb + cpb * DEN_CH);
     if (tb >= te) return;
@@ -259,6 +280,7 @@ __global__ __launch_bounds__(FIN_THREADS) void den_i16_kernel(const int64_t* __r
 __global__ __launch_bounds__(FIN_THREADS) void den_fp64_kernel(const int64_t* __restrict__ sums,
                                                                int nframes, int cpb,
                                                                const int64_t* __restrict__ smax,
+                                                               int skip,
                                                                double* __restrict__ den) {
     __shared__ double sa[DEN_CH];
     __shared__ double sb[DEN_SB + DEN_SB / 16 + 1];
@@ -268,6 +290,7 @@ __global__ __launch_bounds__(FIN_THREADS) void den_fp64_kernel(const int64_t* __
     const double m = (double)smax[b];
     if (!((double)nframes * m * m < 4503599627370496.0)) return;  // 128-bit path
     if (smax[b] <= DEN_I16_MAX) return;                           // den_i16_kernel's bin
+    if (smax[b] <= skip) return;                                  // den_mfma_kernel's bin
     const int tb = blockIdx.z * cpb * DEN_CH;
     const int te = min(nframes - tau0, tb + cpb * DEN_CH);
     if (tb >= te) return;
@@ -392,9 +415,15 @@ __global__ __launch_bounds__(FIN_THREADS) void finalize_kernel(
     }
 }
 
+// Den for every bin with N M^2 < 2^52: i8 MFMA for the bins whose |S| fits two digits
+// (k_den.hip), packed 16-bit dot products for |S| <= DEN_I16_MAX, FP64 above.
 static void launch_den_fp64(const int64_t* sums, int n_bins, int nframes, const int64_t* smax,
-                            double* dpre, hipStream_t s) {
+                            const int64_t* prefix, int planes, const DenMfma& dm, double* dpre,
+                            hipStream_t s) {
+    // bins with max |S| <= skip are den_mfma_kernel's
+    const int skip = dm.v && dm.umax > 0 ? den_mfma_max() : -1;
     launch_fill_i64(reinterpret_cast<int64_t*>(dpre), (int64_t)n_bins * nframes, s);  // +0.0
+    launch_den_mfma(sums, n_bins, nframes, smax, prefix, planes, dm, dpre, s);
     const int64_t lag_blocks = (nframes + DEN_SPAN - 1) / DEN_SPAN;
     const int64_t chunks = (nframes + DEN_CH - 1) / DEN_CH;
     // frame passes per block: about DEN_BLOCKS blocks in all (half of them, above the
@@ -403,15 +432,15 @@ static void launch_den_fp64(const int64_t* sums, int n_bins, int nframes, const 
     if (cpb < 1) cpb = 1;
     const int64_t zb = (chunks + cpb - 1) / cpb;
     den_fp64_kernel<<<dim3((unsigned)lag_blocks, (unsigned)n_bins, (unsigned)zb),
-                      dim3(FIN_THREADS), 0, s>>>(sums, nframes, (int)cpb, smax, dpre);
+                      dim3(FIN_THREADS), 0, s>>>(sums, nframes, (int)cpb, smax, skip, dpre);
     den_i16_kernel<<<dim3((unsigned)lag_blocks, (unsigned)n_bins, (unsigned)zb),
-                     dim3(FIN_THREADS), 0, s>>>(sums, nframes, (int)cpb, smax, dpre);
+                     dim3(FIN_THREADS), 0, s>>>(sums, nframes, (int)cpb, smax, skip, dpre);
 }
 
 void launch_finalize(const int64_t* plane_sums, int64_t* sums, const int64_t* ugram,
                      const BinDev* bins, int n_bins, int nframes, int mode, int64_t* prefix,
                      int64_t* smax, double* dpre, const int* nwkb, int64_t* num, double* den,
-                     float* g2, hipStream_t s) {
+                     float* g2, const DenMfma& dm, hipStream_t s) {
     if (n_bins <= 0 || nframes <= 0) return;
     if (mode == 3)
         sums_prep_kernel<3><<<dim3((unsigned)n_bins), dim3(PREP_THREADS), 0, s>>>(
@@ -422,7 +451,7 @@ void launch_finalize(const int64_t* plane_sums, int64_t* sums, const int64_t* ug
     else
         sums_prep_kernel<1><<<dim3((unsigned)n_bins), dim3(PREP_THREADS), 0, s>>>(
             sums, sums, nframes, prefix, smax);
-    launch_den_fp64(sums, n_bins, nframes, smax, dpre, s);
+    launch_den_fp64(sums, n_bins, nframes, smax, prefix, mode >= 2 ? 2 : 1, dm, dpre, s);
     dim3 grid((unsigned)((nframes + FIN_SPAN - 1) / FIN_SPAN), (unsigned)n_bins);
     finalize_kernel<int64_t><<<grid, dim3(FIN_THREADS), 0, s>>>(
         sums, ugram, bins, nframes, mode, prefix, smax, dpre, nwkb, num, den, g2);
@@ -430,11 +459,11 @@ void launch_finalize(const int64_t* plane_sums, int64_t* sums, const int64_t* ug
 
 void launch_finalize_f64(const int64_t* sums, const double* fnum, const BinDev* bins, int n_bins,
                          int nframes, int64_t* prefix, int64_t* smax, double* dpre, double* num,
-                         double* den, float* g2, hipStream_t s) {
+                         double* den, float* g2, const DenMfma& dm, hipStream_t s) {
     if (n_bins <= 0 || nframes <= 0) return;
     sums_prep_kernel<1><<<dim3((unsigned)n_bins), dim3(PREP_THREADS), 0, s>>>(
         sums, const_cast<int64_t*>(sums), nframes, prefix, smax);
-    launch_den_fp64(sums, n_bins, nframes, smax, dpre, s);
+    launch_den_fp64(sums, n_bins, nframes, smax, prefix, 1, dm, dpre, s);
     dim3 grid((unsigned)((nframes + FIN_SPAN - 1) / FIN_SPAN), (unsigned)n_bins);
     finalize_kernel<double><<<grid, dim3(FIN_THREADS), 0, s>>>(
         sums, fnum, bins, nframes, 0, prefix, smax, dpre, nullptr, num, den, g2);
