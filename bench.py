@@ -311,6 +311,15 @@ def bench_events(seed):
     ms_coo = _timed(coo, 2, torch)
     stages_coo = _stage_times(plan, coo, 2, torch)
     same = bool((num == num_csr).all().item())
+    # the i8-MFMA denominator's work: per bin 1 digit pass (max |S| <= 255) or 4 (two
+    # digits, max |S| <= 16383); per pass and workgroup of 16 lag blocks c0.., 4 waves x
+    # 24 MFMA 32x32x32 per 32 rows over K = rows - c0 rounded up to 96 rows
+    smax = sums.abs().amax(dim=1).cpu().numpy()
+    passes = np.where(smax <= 255, 1, np.where(smax <= 16383, 4, 0))
+    rows = (n + 63) // 64
+    steps = sum(-(-(rows - c0) // 96) * 3 for c0 in range(0, rows, 16))
+    den_exec_ops = float(passes.sum()) * steps * 4 * 24 * 65536
+    den_useful_ops = float(passes.sum()) * n * (n + 1)   # 2 ops per product, tau >= 0
     # LDS atomic-add ceiling of this GPU (random counters, the pair kernel's LDS
     # footprint): every pair costs at least one such add
     lds_rate = ctypes.c_double()
@@ -332,6 +341,12 @@ def bench_events(seed):
             "stage_ms_csr": {"pairs_and_sums": stages_csr[1], "denominator_and_finalise": stages_csr[2]},
             "stage_ms_coo": {"coo_to_csr": stages_coo[0], "pairs_and_sums": stages_coo[1],
                              "denominator_and_finalise": stages_coo[2]},
+            "den_kernel": {"kernel": "den_mfma_kernel", "bins_one_digit": int((passes == 1).sum()),
+                           "bins_two_digits": int((passes == 4).sum()),
+                           "bins_valu": int((passes == 0).sum()),
+                           "executed_ops": den_exec_ops, "useful_ops": den_useful_ops,
+                           "note": "the kernel's time is in the kernel trace (rocprofv3); the "
+                                   "stage time above adds the prefix sums and the finaliser"},
             "pairs_kernel": {"kernel": "event_pairs_kernel", "pairs_per_s": pairs / (pairs_ms * 1e-3),
                              "lds_atomic_ceiling_per_s": lds_rate.value,
                              "frac_of_lds_atomic_ceiling": pairs / (pairs_ms * 1e-3) / lds_rate.value,

@@ -18,10 +18,10 @@
 // operand of rows s is V[s][r ..] (aligned); the B operand of lag block c and column s' is
 // V[s' mod 64][r + c (+1 for s' >= 64) ..], 16 bytes at any byte offset.  A workgroup
 // takes 16 consecutive lag blocks c0 .. c0 + 15, four per wave (c0 + 4w + i): a wave's B
-// operands start at c0 + 4w + o (o = 0..4), so per row two aligned 16-byte loads at
-// c0 + r (+16) hold every window, cut out with v_alignbyte at shifts 4w + o that are
-// compile-time constants (the wave index is a template argument).  Per 32 rows of K a
-// wave issues 6 loads for 24 MFMAs into 12 accumulators.
+// operands start at c0 + 4w + o (o = 0..4), so per row a 16-byte and a 4-byte load at
+// c0 + 4w + r (4-byte aligned) hold every window, cut out with v_alignbyte at shifts o
+// that are compile-time constants (the wave index is a template argument).  Per 32 rows
+// of K a wave issues 6 loads for 24 MFMAs into 12 accumulators.
 //
 // Digits, by the bin's M = max |S|:
 //   M <= 127: one pass over x = S;
@@ -102,11 +102,10 @@ __global__ __launch_bounds__(256) void den_mfma_prep_kernel(const int64_t* __res
     }
 }
 
-// 16 bytes of the 32 in (lo, hi) from byte OFF (0 <= OFF <= 16).
+// 16 bytes of the 20 in (lo, hi) from byte OFF (0 <= OFF <= 4).
 template <int OFF>
-__device__ __forceinline__ v4i window(const v4i& lo, const v4i& hi) {
-    const uint32_t w[8] = {(uint32_t)lo.x, (uint32_t)lo.y, (uint32_t)lo.z, (uint32_t)lo.w,
-                           (uint32_t)hi.x, (uint32_t)hi.y, (uint32_t)hi.z, (uint32_t)hi.w};
+__device__ __forceinline__ v4i window(const v4i& lo, uint32_t hi) {
+    const uint32_t w[5] = {(uint32_t)lo.x, (uint32_t)lo.y, (uint32_t)lo.z, (uint32_t)lo.w, hi};
     constexpr int A = OFF >> 2, B = OFF & 3;
     v4i r;
     if constexpr (B == 0) {
@@ -123,6 +122,18 @@ __device__ __forceinline__ v4i window(const v4i& lo, const v4i& hi) {
     return r;
 }
 
+// 16 and 4 bytes at a 4-byte aligned global address.
+typedef int v4i_a4 __attribute__((ext_vector_type(4), aligned(4)));
+typedef __attribute__((address_space(1))) const v4i_a4 global_v4i_a4;
+typedef __attribute__((address_space(1))) const uint32_t global_u32;
+__device__ __forceinline__ v4i load16_a4(const int8_t* p) {
+    const v4i_a4 v = *reinterpret_cast<global_v4i_a4*>(reinterpret_cast<uintptr_t>(p));
+    return v4i{v.x, v.y, v.z, v.w};
+}
+__device__ __forceinline__ uint32_t load4(const int8_t* p) {
+    return *reinterpret_cast<global_u32*>(reinterpret_cast<uintptr_t>(p));
+}
+
 #define DEN_MFMA(a, b, c) __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b, c, 0, 0, 0)
 
 // Lag blocks c = c0 + 4 W + i (i < 4) over K = rows - c0 rows (later rows meet zeros):
@@ -135,7 +146,8 @@ __device__ __forceinline__ void den_wave(const int8_t* __restrict__ vx,
     const int lane = threadIdx.x & 63, l32 = lane & 31, hl = lane >> 5;
     const int8_t* pa0 = vx + (int64_t)l32 * up + 16 * hl;
     const int8_t* pa1 = pa0 + 32 * (int64_t)up;
-    const int8_t* pb0 = vy + (int64_t)l32 * up + c0 + 16 * hl;
+    // B rows from byte 4 W past c0: the windows of this wave start at offsets 0 .. 4
+    const int8_t* pb0 = vy + (int64_t)l32 * up + c0 + 16 * hl + 4 * W;
     const int8_t* pb1 = pb0 + 32 * (int64_t)up;
     // m[i][d]: diagonal group d (j = 32 d + column - row) of lag block c0 + 4 W + i; the
     // two blocks of a group, (0, d) and (1, d + 1), share one accumulator
@@ -146,35 +158,62 @@ __device__ __forceinline__ void den_wave(const int8_t* __restrict__ vx,
         for (int k = 0; k < 3; ++k)
 #pragma unroll
             for (int e = 0; e < 16; ++e) m[i][k][e] = 0;
-    v4i a0 = load16(pa0), a1 = load16(pa1), b0l = load16(pb0), b0h = load16(pb0 + 16),
-        b1l = load16(pb1), b1h = load16(pb1 + 16);
-    for (int r0 = 0; r0 < K; r0 += 32) {
-        // the next 32 rows load while these multiply (the last iteration's loads stay
-        // inside V's zero columns: c0 + r0 + 63 + 16 + 15 < rows + 96 <= up)
-        const int rn = r0 + 32;
-        const v4i na0 = load16(pa0 + rn), na1 = load16(pa1 + rn), nb0l = load16(pb0 + rn),
-                  nb0h = load16(pb0 + rn + 16), nb1l = load16(pb1 + rn),
-                  nb1h = load16(pb1 + rn + 16);
-        // B(c0 + 4W + i): blocks 0, 1 (s' < 64) at offset 4W + i of rows l32 and l32 + 32,
-        // blocks 2, 3 (s' >= 64) at offset 4W + i + 1
+    // Three register sets, each the operands of 32 rows, rotate through an unrolled
+    // loop: a set is reloaded (two steps ahead) right after its MFMAs, so every load has
+    // two steps of MFMAs (1,536 cycles) to arrive, and no register is copied.  The last
+    // iteration may run up to two steps past K and load two more ahead; those rows meet
+    // zero B columns (r + c >= rows) and stay inside V (c0 + K + 96 + 63 + 32 + 15 <
+    // rows + 192 <= up).
+    struct Ops {
+        v4i a0, a1, b0l, b1l;
+        uint32_t b0h, b1h;
+    };
+    auto load = [&](Ops& o, int r) {
+        o.a0 = load16(pa0 + r);
+        o.a1 = load16(pa1 + r);
+        o.b0l = load16_a4(pb0 + r);
+        o.b0h = load4(pb0 + r + 16);
+        o.b1l = load16_a4(pb1 + r);
+        o.b1h = load4(pb1 + r + 16);
+    };
+    // B(c0 + 4W + i): blocks 0, 1 (s' < 64) at offset 4W + i of rows l32 and l32 + 32,
+    // blocks 2, 3 (s' >= 64) at offset 4W + i + 1 (window offsets i and i + 1)
+    auto step = [&](const Ops& o) {
 #define DEN_STEP(I)                                                                          \
         {                                                                                    \
-            const v4i f0 = window<4 * W + I>(b0l, b0h), f1 = window<4 * W + I>(b1l, b1h);      \
-            const v4i g0 = window<4 * W + I + 1>(b0l, b0h),                                  \
-                      g1 = window<4 * W + I + 1>(b1l, b1h);                                  \
-            m[I][0] = DEN_MFMA(a0, f0, m[I][0]);                                             \
-            m[I][1] = DEN_MFMA(a0, f1, m[I][1]);                                             \
-            m[I][2] = DEN_MFMA(a0, g0, m[I][2]);                                             \
-            m[I][0] = DEN_MFMA(a1, f1, m[I][0]);                                             \
-            m[I][1] = DEN_MFMA(a1, g0, m[I][1]);                                             \
-            m[I][2] = DEN_MFMA(a1, g1, m[I][2]);                                             \
+            const v4i f0 = window<I>(o.b0l, o.b0h), f1 = window<I>(o.b1l, o.b1h);            \
+            const v4i g0 = window<I + 1>(o.b0l, o.b0h), g1 = window<I + 1>(o.b1l, o.b1h);    \
+            m[I][0] = DEN_MFMA(o.a0, f0, m[I][0]);                                           \
+            m[I][1] = DEN_MFMA(o.a0, f1, m[I][1]);                                           \
+            m[I][2] = DEN_MFMA(o.a0, g0, m[I][2]);                                           \
+            m[I][0] = DEN_MFMA(o.a1, f1, m[I][0]);                                           \
+            m[I][1] = DEN_MFMA(o.a1, g0, m[I][1]);                                           \
+            m[I][2] = DEN_MFMA(o.a1, g1, m[I][2]);                                           \
         }
         DEN_STEP(0)
         DEN_STEP(1)
         DEN_STEP(2)
         DEN_STEP(3)
 #undef DEN_STEP
-        a0 = na0; a1 = na1; b0l = nb0l; b0h = nb0h; b1l = nb1l; b1h = nb1h;
+    };
+    Ops s0, s1, s2;
+    load(s0, 0);
+    load(s1, 32);
+    // the scheduler would sink each load group to its first use, exposing its latency:
+    // fences keep every group above the MFMAs it overlaps
+    for (int r0 = 0; r0 < K; r0 += 96) {
+        load(s2, r0 + 64);
+        __builtin_amdgcn_sched_barrier(0);
+        step(s0);
+        __builtin_amdgcn_sched_barrier(0);
+        load(s0, r0 + 96);
+        __builtin_amdgcn_sched_barrier(0);
+        step(s1);
+        __builtin_amdgcn_sched_barrier(0);
+        load(s1, r0 + 128);
+        __builtin_amdgcn_sched_barrier(0);
+        step(s2);
+        __builtin_amdgcn_sched_barrier(0);
     }
     // 32 x 32 accumulator layout: column l32, rows 8 (e / 4) + 4 hl + e % 4.  Block
     // (sh, sb) holds s = 32 sh + row, s' = 32 sb + l32, diagonal j = 32 (sb - sh) + l32 -
@@ -250,7 +289,7 @@ int den_mfma_max() { return DEN_MFMA_MAX; }
 int64_t den_mfma_columns(int nframes) {
     const int64_t rows = ((int64_t)nframes + 63) / 64;
     if (nframes <= 0 || rows > 32768) return 0;
-    return (rows + 15) / 16 * 16 + 96;
+    return (rows + 15) / 16 * 16 + 192;
 }
 
 void launch_den_mfma(const int64_t* sums, int n_bins, int nframes, const int64_t* smax,

@@ -42,6 +42,12 @@ __device__ __forceinline__ double i128_to_double(__int128 x) {
     return (double)hi * 18446744073709551616.0 + (double)lo;
 }
 
+__device__ __forceinline__ int64_t wave_sum64(int64_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
 // Exclusive prefix scan of one value per thread over the block; returns the prefix and
 // adds the block total to `carry`.  wave_tot: PREP_THREADS / 64 words of LDS.
 __device__ __forceinline__ int64_t block_scan(int64_t v, int64_t* wave_tot, int64_t& carry) {
@@ -64,20 +70,26 @@ __device__ __forceinline__ int64_t block_scan(int64_t v, int64_t* wave_tot, int6
     return before + x - v;
 }
 
-// One block per bin.  S(t) are the rows of plane_sums; PLANES >= 2 (uint16 digit
+// Block (bin x, chunk y).  S(t) are the rows of plane_sums; PLANES >= 2 (uint16 digit
 // planes): rows b (S_H) and n_bins + b (S_L), combined into sums[b][t] = 256 S_H + S_L;
 // PLANES = 3 adds rows 2 n_bins + b (S_Lw, the low bytes over the wide k-blocks, which
 // only the bias correction uses).  Writes the exclusive prefix sums of each plane,
-// prefix[plane * n_bins + b][t] for t = 0..N, and smax[b] = max |S|.  Each pass takes
-// PREP_ITEMS consecutive frames per thread (fewer with more planes, to stay in registers):
-// a serial prefix in registers, then one block scan of the thread totals.
+// prefix[plane * n_bins + b][t] for t = 0..N, and smax[b] = max |S| (by atomic max into
+// a row zeroed first).  A block scans PREP_CHUNK frames: its starting offsets are the sums
+// of every earlier frame (a block reduction over them, so no pass waits for another
+// block), then each pass takes PREP_ITEMS consecutive frames per thread -- fewer with more
+// planes, to stay in registers -- with a serial prefix in registers and one block scan of
+// the thread totals.
+constexpr int PREP_CHUNK = 16384;
 template <int PLANES>
 __global__ __launch_bounds__(PREP_THREADS) void sums_prep_kernel(
     const int64_t* __restrict__ plane_sums, int64_t* __restrict__ sums, int nframes,
     int64_t* __restrict__ prefix, int64_t* __restrict__ smax) {
     __shared__ int64_t wave_tot[PREP_THREADS / 64];
     __shared__ int64_t wave_max[PREP_THREADS / 64];
+    __shared__ int64_t red[3][PREP_THREADS / 64];
     constexpr bool WIDE = PLANES >= 2;
+    constexpr int PREP_ITEMS = PLANES == 1 ? 8 : (PLANES == 2 ? 4 : 1);
     const int b = blockIdx.x, nb = gridDim.x;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int64_t* S0 = plane_sums + (int64_t)b * nframes;
@@ -86,8 +98,30 @@ __global__ __launch_bounds__(PREP_THREADS) void sums_prep_kernel(
     int64_t* P0 = prefix + (int64_t)b * (nframes + 1);
     int64_t* P1 = prefix + (int64_t)(nb + b) * (nframes + 1);
     int64_t* P2 = prefix + (int64_t)(2 * nb + b) * (nframes + 1);
-    constexpr int PREP_ITEMS = PLANES == 1 ? 8 : (PLANES == 2 ? 4 : 1);
+    const int cb = blockIdx.y * PREP_CHUNK;
+    const int ce = min(nframes, cb + PREP_CHUNK);
+    // offsets: the planes' sums over frames [0, cb)
     int64_t c0 = 0, c1 = 0, c2 = 0, mx = 0;
+    for (int t = tid; t < cb; t += PREP_THREADS) {
+        c0 += S0[t];
+        if (WIDE) c1 += S1[t];
+        if (PLANES == 3) c2 += S2[t];
+    }
+    c0 = wave_sum64(c0);
+    if (WIDE) c1 = wave_sum64(c1);
+    if (PLANES == 3) c2 = wave_sum64(c2);
+    if (lane == 0) {
+        red[0][wave] = c0;
+        red[1][wave] = c1;
+        red[2][wave] = c2;
+    }
+    __syncthreads();
+    c0 = c1 = c2 = 0;
+    for (int w = 0; w < PREP_THREADS / 64; ++w) {
+        c0 += red[0][w];
+        c1 += red[1][w];
+        c2 += red[2][w];
+    }
     // exclusive prefixes of this thread's PREP_ITEMS frames of one plane, in place
     auto scan = [&](int64_t (&v)[PREP_ITEMS], int64_t& carry) {
         int64_t run = 0;
@@ -101,17 +135,17 @@ __global__ __launch_bounds__(PREP_THREADS) void sums_prep_kernel(
 #pragma unroll
         for (int k = 0; k < PREP_ITEMS; ++k) v[k] += base;
     };
-    for (int t0 = 0; t0 < nframes; t0 += PREP_THREADS * PREP_ITEMS) {
+    for (int t0 = cb; t0 < ce; t0 += PREP_THREADS * PREP_ITEMS) {
         const int tb = t0 + PREP_ITEMS * tid;
         int64_t v0[PREP_ITEMS], v1[PREP_ITEMS], v2[PREP_ITEMS];
 #pragma unroll
         for (int k = 0; k < PREP_ITEMS; ++k) {
             const int t = tb + k;
-            v0[k] = t < nframes ? S0[t] : 0;
-            if (WIDE) v1[k] = t < nframes ? S1[t] : 0;
-            if (PLANES == 3) v2[k] = t < nframes ? S2[t] : 0;
+            v0[k] = t < ce ? S0[t] : 0;
+            if (WIDE) v1[k] = t < ce ? S1[t] : 0;
+            if (PLANES == 3) v2[k] = t < ce ? S2[t] : 0;
             const int64_t v = WIDE ? 256 * v0[k] + v1[k] : v0[k];
-            if (WIDE && t < nframes) sums[(int64_t)b * nframes + t] = v;
+            if (WIDE && t < ce) sums[(int64_t)b * nframes + t] = v;
             mx = max(mx, v < 0 ? -v : v);
         }
         scan(v0, c0);
@@ -120,14 +154,14 @@ __global__ __launch_bounds__(PREP_THREADS) void sums_prep_kernel(
 #pragma unroll
         for (int k = 0; k < PREP_ITEMS; ++k) {
             const int t = tb + k;
-            if (t < nframes) {
+            if (t < ce) {
                 P0[t] = v0[k];
                 if (WIDE) P1[t] = v1[k];
                 if (PLANES == 3) P2[t] = v2[k];
             }
         }
     }
-    if (tid == 0) {
+    if (tid == 0 && ce == nframes) {
         P0[nframes] = c0;
         if (WIDE) P1[nframes] = c1;
         if (PLANES == 3) P2[nframes] = c2;
@@ -139,8 +173,20 @@ __global__ __launch_bounds__(PREP_THREADS) void sums_prep_kernel(
     if (tid == 0) {
         int64_t m = 0;
         for (int w = 0; w < PREP_THREADS / 64; ++w) m = max(m, wave_max[w]);
-        smax[b] = m;
+        __hip_atomic_fetch_max(reinterpret_cast<unsigned long long*>(smax + b),
+                               (unsigned long long)m, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
     }
+}
+
+// smax zeroed, then the chunks of every bin.
+template <int PLANES>
+static void launch_sums_prep(const int64_t* plane_sums, int64_t* sums, int n_bins, int nframes,
+                             int64_t* prefix, int64_t* smax, hipStream_t s) {
+    launch_fill_i64(smax, n_bins, s);
+    const dim3 grid((unsigned)n_bins, (unsigned)((nframes + PREP_CHUNK - 1) / PREP_CHUNK));
+    sums_prep_kernel<PLANES><<<grid, dim3(PREP_THREADS), 0, s>>>(plane_sums, sums, nframes,
+                                                                 prefix, smax);
 }
 
 // Accumulate Den for the FIN_LAGS lags of this thread over the frames of the block.
@@ -443,14 +489,11 @@ void launch_finalize(const int64_t* plane_sums, int64_t* sums, const int64_t* ug
                      float* g2, const DenMfma& dm, hipStream_t s) {
     if (n_bins <= 0 || nframes <= 0) return;
     if (mode == 3)
-        sums_prep_kernel<3><<<dim3((unsigned)n_bins), dim3(PREP_THREADS), 0, s>>>(
-            plane_sums, sums, nframes, prefix, smax);
+        launch_sums_prep<3>(plane_sums, sums, n_bins, nframes, prefix, smax, s);
     else if (mode == 2)
-        sums_prep_kernel<2><<<dim3((unsigned)n_bins), dim3(PREP_THREADS), 0, s>>>(
-            plane_sums, sums, nframes, prefix, smax);
+        launch_sums_prep<2>(plane_sums, sums, n_bins, nframes, prefix, smax, s);
     else
-        sums_prep_kernel<1><<<dim3((unsigned)n_bins), dim3(PREP_THREADS), 0, s>>>(
-            sums, sums, nframes, prefix, smax);
+        launch_sums_prep<1>(sums, sums, n_bins, nframes, prefix, smax, s);
     launch_den_fp64(sums, n_bins, nframes, smax, prefix, mode >= 2 ? 2 : 1, dm, dpre, s);
     dim3 grid((unsigned)((nframes + FIN_SPAN - 1) / FIN_SPAN), (unsigned)n_bins);
     finalize_kernel<int64_t><<<grid, dim3(FIN_THREADS), 0, s>>>(
@@ -461,8 +504,7 @@ void launch_finalize_f64(const int64_t* sums, const double* fnum, const BinDev* 
                          int nframes, int64_t* prefix, int64_t* smax, double* dpre, double* num,
                          double* den, float* g2, const DenMfma& dm, hipStream_t s) {
     if (n_bins <= 0 || nframes <= 0) return;
-    sums_prep_kernel<1><<<dim3((unsigned)n_bins), dim3(PREP_THREADS), 0, s>>>(
-        sums, const_cast<int64_t*>(sums), nframes, prefix, smax);
+    launch_sums_prep<1>(sums, const_cast<int64_t*>(sums), n_bins, nframes, prefix, smax, s);
     launch_den_fp64(sums, n_bins, nframes, smax, prefix, 1, dm, dpre, s);
     dim3 grid((unsigned)((nframes + FIN_SPAN - 1) / FIN_SPAN), (unsigned)n_bins);
     finalize_kernel<double><<<grid, dim3(FIN_THREADS), 0, s>>>(
