@@ -166,6 +166,9 @@ __device__ __forceinline__ void event_pairs_item(
                     st_t[wave][k] = ct[q];
                     st_d[wave][k] = cd[q];
                     sq += (long long)cd[q] * cd[q];
+                } else if (k < n + 4 && k < ECAP) {
+                    st_t[wave][k] = 0x7fffffff;  // sentinel time: no partner lies there
+                    st_d[wave][k] = 0;
                 }
             }
         }
@@ -242,6 +245,41 @@ __device__ __forceinline__ void event_pairs_item(
             // compare, which also rejects a negative lag.
             auto partners = [&](int ti, C di, int j) {
                 const int base = ti + w0;
+                if constexpr (S && LDS && sizeof(T) == 1) {
+                    // Staged 8-bit counts into the LDS histogram: the four partners of a
+                    // step come from one aligned 16-byte read of each staged array, from
+                    // j & ~3 (j < n <= ECAP, so the reads stay inside the wave's ECAP
+                    // entries).  One unsigned compare of t - (t_i + lo) against the
+                    // window, lo = max(w0, 1), rejects everything that is not a partner:
+                    // entries before j lie below t_i + lo (times strictly increase; in the
+                    // first window entry i itself sits at lag 0 < lo), and the entries
+                    // from n on hold the staging's sentinel time INT_MAX.  Every slot
+                    // adds -- a rejected one adds 0 at lag `lane` -- so the adds take no
+                    // branch.  Products of two 8-bit counts fit 24 bits.
+                    const int lo = w0 > 0 ? w0 : 1;
+                    const int base1 = ti + lo;
+                    const unsigned span1 = span - (unsigned)(lo - w0);
+                    H* const h1 = hist + (lo - w0);
+                    const int4* t4p = reinterpret_cast<const int4*>(st_t[wave]);
+                    const int4* d4p = reinterpret_cast<const int4*>(st_d[wave]);
+                    for (int j4 = j & ~3; j4 < n; j4 += 4) {
+                        const int4 t4 = t4p[j4 >> 2];
+                        const int4 d4 = d4p[j4 >> 2];
+                        const int tt[4] = {t4.x, t4.y, t4.z, t4.w};
+                        const int dd[4] = {d4.x, d4.y, d4.z, d4.w};
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            const unsigned lag = (unsigned)(tt[u] - base1);
+                            const bool ok = lag < span1;
+                            const int prod = std::is_signed<T>::value
+                                                 ? __mul24((int)di, dd[u])
+                                                 : (int)__umul24((unsigned)di, (unsigned)dd[u]);
+                            atomicAdd(&h1[ok ? (int)lag : lane], (H)(ok ? prod : 0));
+                        }
+                        if (tt[3] - base1 >= (int)span1) break;  // INT_MAX past n
+                    }
+                    return;
+                }
                 for (; j < n; j += 4) {
                     int tj[4];
                     C dj[4];
@@ -326,8 +364,8 @@ __global__ __launch_bounds__(64 * NW) void event_pairs_kernel(
     const BinDev* __restrict__ bins, const EventWork* __restrict__ work, int nframes,
     int64_t nnz, int64_t* __restrict__ num, int64_t* __restrict__ sums, int* __restrict__ err) {
     __shared__ typename EvHist<T>::type hist[EvHist<T>::lags_per_wave * NW];
-    __shared__ int st_t[NW][ECAP];
-    __shared__ int st_d[NW][ECAP];
+    __shared__ __attribute__((aligned(16))) int st_t[NW][ECAP];
+    __shared__ __attribute__((aligned(16))) int st_d[NW][ECAP];
     event_pairs_item<T, NW>(times, data, offsets, pix_idx, bins, work[blockIdx.x], nframes, nnz,
                             num, sums, err, hist, st_t, st_d);
 }
@@ -346,8 +384,8 @@ template <typename T, int NW>
 __global__ __launch_bounds__(64 * NW) void event_pairs_jobs_kernel(
     const EventJob* __restrict__ jobs, const EventWork* __restrict__ work) {
     __shared__ typename EvHist<T>::type hist[EvHist<T>::lags_per_wave * NW];
-    __shared__ int st_t[NW][ECAP];
-    __shared__ int st_d[NW][ECAP];
+    __shared__ __attribute__((aligned(16))) int st_t[NW][ECAP];
+    __shared__ __attribute__((aligned(16))) int st_d[NW][ECAP];
     const EventWork w = work[blockIdx.x];
     if (w.bin < 0) return;
     const EventJob j = jobs[w.pad];
