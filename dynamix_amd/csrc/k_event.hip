@@ -45,6 +45,9 @@ constexpr int EPF = ECAP / 64;  // event rounds of the next pixel prefetched in 
 template <typename T> struct EvHist {
     using type = uint32_t;
     static constexpr int lags_per_wave = 2048;
+    // lags of one window: the last 64 counters stay outside every window, so the staged
+    // partner loop can send each rejected partner's product to its own lane's counter there
+    static constexpr int window(int nw) { return lags_per_wave * nw - 64; }
     static constexpr long long budget(int nw) {
         return (std::is_signed<T>::value ? (1ll << 31) : (1ll << 32)) / nw;
     }
@@ -56,6 +59,7 @@ template <typename T> struct EvHist {
 template <> struct EvHist<uint16_t> {
     using type = unsigned long long;
     static constexpr int lags_per_wave = 1024;
+    static constexpr int window(int nw) { return lags_per_wave * nw; }
     static constexpr long long budget(int) { return 0; }  // no bound needed
     static __device__ __forceinline__ unsigned long long widen(unsigned long long v) { return v; }
 };
@@ -88,13 +92,14 @@ __device__ __forceinline__ void event_pairs_item(
     typename EvHist<T>::type* __restrict__ hist, int (*__restrict__ st_t)[ECAP],
     int (*__restrict__ st_d)[ECAP]) {
     using H = typename EvHist<T>::type;
-    constexpr int EW = EvHist<T>::lags_per_wave * NW;
+    constexpr int EW = EvHist<T>::lags_per_wave * NW;  // counters
+    constexpr int WS = EvHist<T>::window(NW);          // lags of a window
     constexpr int NT = 64 * NW;
     constexpr bool BOUNDED = sizeof(H) == 4;
     const BinDev bd = bins[w.bin];
-    const int w0 = blockIdx.y * EW;
+    const int w0 = blockIdx.y * WS;
     if (w0 >= nframes) return;
-    const int w1 = min(nframes, w0 + EW);
+    const int w1 = min(nframes, w0 + WS);
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     for (int k = threadIdx.x; k < EW; k += NT) hist[k] = 0;
@@ -249,16 +254,19 @@ __device__ __forceinline__ void event_pairs_item(
                     // Staged 8-bit counts into the LDS histogram: the four partners of a
                     // step come from one aligned 16-byte read of each staged array, from
                     // j & ~3 (j < n <= ECAP, so the reads stay inside the wave's ECAP
-                    // entries).  One unsigned compare of t - (t_i + lo) against the
-                    // window, lo = max(w0, 1), rejects everything that is not a partner:
+                    // entries).  With lo = max(w0, 1), a partner has t - (t_i + lo) in
+                    // [0, span1); everything else lies outside as an unsigned value:
                     // entries before j lie below t_i + lo (times strictly increase; in the
                     // first window entry i itself sits at lag 0 < lo), and the entries
-                    // from n on hold the staging's sentinel time INT_MAX.  Every slot
-                    // adds -- a rejected one adds 0 at lag `lane` -- so the adds take no
-                    // branch.  Products of two 8-bit counts fit 24 bits.
+                    // from n on hold the staging's sentinel time INT_MAX.  The counter
+                    // index min(t - (t_i + lo), span1 + lane) sends every non-partner to
+                    // a counter past the window (the 64 that no window covers), so every
+                    // slot adds without a test or a branch.  Products of two 8-bit counts
+                    // fit 24 bits.
                     const int lo = w0 > 0 ? w0 : 1;
                     const int base1 = ti + lo;
                     const unsigned span1 = span - (unsigned)(lo - w0);
+                    const unsigned park = span1 + (unsigned)lane;
                     H* const h1 = hist + (lo - w0);
                     const int4* t4p = reinterpret_cast<const int4*>(st_t[wave]);
                     const int4* d4p = reinterpret_cast<const int4*>(st_d[wave]);
@@ -269,12 +277,11 @@ __device__ __forceinline__ void event_pairs_item(
                         const int dd[4] = {d4.x, d4.y, d4.z, d4.w};
 #pragma unroll
                         for (int u = 0; u < 4; ++u) {
-                            const unsigned lag = (unsigned)(tt[u] - base1);
-                            const bool ok = lag < span1;
+                            const unsigned lag = min((unsigned)(tt[u] - base1), park);
                             const int prod = std::is_signed<T>::value
                                                  ? __mul24((int)di, dd[u])
                                                  : (int)__umul24((unsigned)di, (unsigned)dd[u]);
-                            atomicAdd(&h1[ok ? (int)lag : lane], (H)(ok ? prod : 0));
+                            atomicAdd(&h1[lag], (H)prod);
                         }
                         if (tt[3] - base1 >= (int)span1) break;  // INT_MAX past n
                     }
@@ -395,9 +402,10 @@ __global__ __launch_bounds__(64 * NW) void event_pairs_jobs_kernel(
                             global_ptr(j.err), hist, st_t, st_d);
 }
 
-// Waves per workgroup of the pair kernels: 16 (one workgroup per CU, 32,768-lag windows
-// for 8-bit counts: the fewest windows, so the least per-window work per pixel), or
-// DXG2_EVENT_WAVES = 8 (two per CU, 16,384 lags) or 4 (four per CU, 8,192 lags).
+// Waves per workgroup of the pair kernels: 16 (one workgroup per CU, 32,768 counters and
+// windows of 32,704 lags for 8-bit counts: the fewest windows, so the least per-window
+// work per pixel), or DXG2_EVENT_WAVES = 8 (two per CU, 16,384 counters) or 4 (four per
+// CU, 8,192 counters).
 // Measured on the events workload (1030x1065 x 100,000 frames): pairs 8.4 ms at 16
 // waves, 9.8 ms at 8, 13.2 ms at 4 (largest work items each).
 int event_waves() {
@@ -410,15 +418,15 @@ int event_waves() {
 }
 
 int event_lag_window(int dtype) {
-    return (dtype == DXG2_U16 ? EvHist<uint16_t>::lags_per_wave : EvHist<uint8_t>::lags_per_wave) *
-           event_waves();
+    return dtype == DXG2_U16 ? EvHist<uint16_t>::window(event_waves())
+                             : EvHist<uint8_t>::window(event_waves());
 }
 
 template <typename T, int NW>
 void pairs_jobs_launch(const EventJob* jobs, const EventWork* work, int n_work, int max_frames,
                        hipStream_t s) {
-    constexpr int EW = EvHist<T>::lags_per_wave * NW;
-    event_pairs_jobs_kernel<T, NW><<<dim3((unsigned)n_work, (unsigned)((max_frames + EW - 1) / EW)),
+    constexpr int WS = EvHist<T>::window(NW);
+    event_pairs_jobs_kernel<T, NW><<<dim3((unsigned)n_work, (unsigned)((max_frames + WS - 1) / WS)),
                                      64 * NW, 0, s>>>(jobs, work);
 }
 
@@ -448,8 +456,8 @@ template <typename T, int NW>
 void pairs_launch(const int32_t* times, const void* data, const uint32_t* offsets,
                   const int32_t* pix_idx, const BinDev* bins, const EventWork* work, int n_work,
                   int nframes, int64_t nnz, int64_t* num, int64_t* sums, int* err, hipStream_t s) {
-    constexpr int EW = EvHist<T>::lags_per_wave * NW;
-    event_pairs_kernel<T, NW><<<dim3((unsigned)n_work, (unsigned)((nframes + EW - 1) / EW)),
+    constexpr int WS = EvHist<T>::window(NW);
+    event_pairs_kernel<T, NW><<<dim3((unsigned)n_work, (unsigned)((nframes + WS - 1) / WS)),
                                 64 * NW, 0, s>>>(times, (const T*)data, offsets, pix_idx, bins,
                                                  work, nframes, nnz, num, sums, err);
 }

@@ -225,3 +225,36 @@ def test_device_event_generator_matches_dense_compaction():
     assert (fr[1:] >= fr[:-1]).all().item()
     ce.correlate_coo(pix, fr, cnt)
     assert (cd.last_numerators == ce.last_numerators).all().item()
+
+
+@pytest.mark.parametrize("dtype", [np.uint8, np.int8])
+def test_events_lags_at_window_edges(dtype):
+    """Pairs whose lags sit on and around the lag-window boundaries (multiples of the
+    window length, for 4-, 8- and 16-wave windows of 2048 lags per wave less 64), one
+    event apart in every window, over 70,000 frames: exact against the Python pair
+    loop."""
+    _announce("events window edges %s" % np.dtype(dtype).name)
+    from dynamix_amd.correlator.event import EventCorrelator
+    n, shape = 70000, (3, 4)
+    edges = sorted({k * w + d for w in (8192 - 64, 16384 - 64, 32768 - 64, 8192, 32768)
+                    for k in (1, 2) for d in (-2, -1, 0, 1, 2) if 0 < k * w + d < n - 1})
+    rng = np.random.default_rng(9)
+    lo = -3 if dtype == np.int8 else 1
+    times, data, offsets = [], [], [0]
+    for p in range(shape[0] * shape[1]):
+        t0 = int(rng.integers(0, 200))
+        ts = sorted({t0} | {t0 + e for e in edges if t0 + e < n} |
+                    set(int(x) for x in rng.integers(0, n, size=40)))
+        times += ts
+        data += [int(v) if v != 0 else 1 for v in rng.integers(lo, 4, size=len(ts))]
+        offsets.append(len(times))
+    times = np.array(times, np.int32)
+    data = np.array(data, dtype)
+    offsets = np.array(offsets, np.uint32)
+    q = np.array([[1, 1, 2, 2], [1, 2, 2, 0], [2, 1, 1, 2]], np.int32)
+    c = EventCorrelator(shape, n, max_events_count=int(np.diff(offsets).max()), qmask=q,
+                        dtype=dtype)
+    c.correlate(times, data, offsets)
+    num, sums = ev.event_brute_force(times, data, offsets, q, n)
+    np.testing.assert_array_equal(c.last_numerators.cpu().numpy(), num.astype(np.int64))
+    np.testing.assert_array_equal(c.last_sums.cpu().numpy(), sums.astype(np.int64))
