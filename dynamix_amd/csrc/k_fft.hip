@@ -27,7 +27,8 @@
 //    loop issues no other global load.  The last pass works in place (each butterfly
 //    writes the points it read).
 //  * fft_inverse_kernel: Num(tau) = (P_0 + (-1)^tau P_L + 2 sum_{k=1}^{L-1} P_k
-//    cos(2 pi k tau / Nf)) / Nf in FP64, the cosine by an FP64 rotation recurrence.
+//    cos(2 pi k tau / Nf)) / Nf in FP64, the cosines by four interleaved FP64 rotation
+//    recurrences.
 // Twiddle tables are computed on the host in FP64 and rounded to FP32.
 #include "dxg2_internal.h"
 
@@ -677,29 +678,65 @@ __global__ __launch_bounds__(Plan::T) void four_step_rows_kernel(
 
 // Num(tau) from the power spectrum, FP64; grid (ceil(N / 256), n_bins).  Every
 // thread reads the same ps[k] in step, so the spectrum is read as uniform loads.
+// One block per (256 lags, group of INV_G bins): the group's power rows are staged in LDS
+// in chunks of INV_CH values (every thread reads the same P_k, a broadcast), and each
+// cosine serves the INV_G bins.  The cosines run as four independent rotation
+// recurrences (k = k0 + 4 m + r, each turning by 4 theta, theta = 2 pi tau / Nf), so four
+// dependent chains overlap; each chain is re-anchored with sincospi at every chunk start
+// (512 rotations at most), and the four partial sums are joined in a fixed order.
+constexpr int INV_CH = 2048;
+constexpr int INV_G = 4;
 __global__ __launch_bounds__(256) void fft_inverse_kernel(const double* __restrict__ power,
-                                                          int L, int nframes,
+                                                          int L, int n_bins, int nframes,
                                                           double* __restrict__ num) {
-    const int b = blockIdx.y;
-    const double* ps = power + (int64_t)b * (L + 1);
-    const int tau = blockIdx.x * 256 + threadIdx.x;
-    if (tau >= nframes) return;
+    __shared__ double pk[INV_G][INV_CH];
+    const int b0 = blockIdx.y * INV_G;
+    const int ng = min(INV_G, n_bins - b0);
+    const int tau0 = blockIdx.x * 256 + threadIdx.x;
+    const int64_t tau = tau0 < nframes ? tau0 : 0;
     const double nf = 2.0 * L;
-    double sn, cs;
-    sincospi(2.0 * tau / nf, &sn, &cs);      // rotation by 2 pi tau / Nf
-    double c = cs, s = sn;                   // cos, sin of 2 pi k tau / Nf at k = 1
-    double acc = 0.0;
-    for (int k = 1; k < L; ++k) {
-        acc += ps[k] * c;
-        const double cn = c * cs - s * sn;
-        s = s * cs + c * sn;
-        c = cn;
-        if ((k & 1023) == 0) {               // re-anchor the recurrence
-            sincospi(2.0 * (double)(((int64_t)(k + 1) * tau) % (int64_t)(2 * L)) / nf, &s, &c);
+    const int64_t period = 2 * (int64_t)L;  // cos(2 pi k tau / Nf) depends on k tau mod Nf
+    double s4, c4;
+    sincospi(2.0 * (double)((4 * tau) % period) / nf, &s4, &c4);  // rotation by 4 theta
+    double acc[INV_G][4];
+#pragma unroll
+    for (int g = 0; g < INV_G; ++g)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc[g][r] = 0.0;
+    for (int k0 = 1; k0 < L; k0 += INV_CH) {
+        const int k1 = min(L, k0 + INV_CH);
+        const int n = k1 - k0;
+        __syncthreads();                     // the previous chunk has been read
+        for (int i = threadIdx.x; i < INV_G * INV_CH; i += 256) {
+            const int g = i / INV_CH, k = i % INV_CH;
+            pk[g][k] = (g < ng && k < n) ? power[(int64_t)(b0 + g) * (L + 1) + k0 + k] : 0.0;
+        }
+        __syncthreads();
+        double c[4], s[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+            sincospi(2.0 * (double)(((int64_t)(k0 + r) * tau) % period) / nf, &s[r], &c[r]);
+        // the staged rows are zero past n, so whole steps of four cover the chunk
+        for (int i = 0; i < n; i += 4) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+#pragma unroll
+                for (int g = 0; g < INV_G; ++g) acc[g][r] += pk[g][i + r] * c[r];
+                const double cn = c[r] * c4 - s[r] * s4;
+                s[r] = s[r] * c4 + c[r] * s4;
+                c[r] = cn;
+            }
         }
     }
-    const double last = (tau & 1) ? -ps[L] : ps[L];
-    num[(int64_t)b * nframes + tau] = (ps[0] + last + 2.0 * acc) / nf;
+    if (tau0 >= nframes) return;
+#pragma unroll
+    for (int g = 0; g < INV_G; ++g) {
+        if (g >= ng) break;
+        const double* ps = power + (int64_t)(b0 + g) * (L + 1);
+        const double last = (tau & 1) ? -ps[L] : ps[L];
+        num[(int64_t)(b0 + g) * nframes + tau] =
+            (ps[0] + last + 2.0 * ((acc[g][0] + acc[g][1]) + (acc[g][2] + acc[g][3]))) / nf;
+    }
 }
 
 // ------------------------------------------------------------------ launchers
@@ -883,8 +920,9 @@ int launch_fft_four_step(int L1, int L2, int dtype, const uint8_t* xs, const Bin
 void launch_fft_inverse(const double* power, int L, int n_bins, int nframes, double* num,
                         hipStream_t s) {
     if (n_bins <= 0) return;
-    fft_inverse_kernel<<<dim3((unsigned)((nframes + 255) / 256), (unsigned)n_bins), 256, 0, s>>>(
-        power, L, nframes, num);
+    fft_inverse_kernel<<<dim3((unsigned)((nframes + 255) / 256),
+                              (unsigned)((n_bins + INV_G - 1) / INV_G)),
+                         256, 0, s>>>(power, L, n_bins, nframes, num);
 }
 
 }  // namespace dxg2
