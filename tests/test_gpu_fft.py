@@ -124,3 +124,22 @@ def test_fft_long_series_four_step(n, L, dtype, monkeypatch):
     q[:, :8] = 1          # bin 1 spans several k-blocks
     c = FFTCorrelator(shape, n, q)
     _check(c, c.correlate(f), f, q, _sparse_ref(f, q))
+
+
+@pytest.mark.parametrize("shape", [(7, 13), (5, 17), (16, 16)])
+def test_fft_gather_quads_odd_widths(shape):
+    """8-bit frames whose size is not a multiple of 4 bytes: the gather's dword loads of
+    consecutive pixel quads sit at every byte alignment, quads break at row and bin
+    boundaries, and the last quads of the stack take the byte loads; every bin's series
+    must arrive intact (exact sums, numerators within the FFT tolerance)."""
+    _announce("fft gather quads shape=%s" % (shape,))
+    from dynamix_amd.correlator.dense import FFTCorrelator
+    from dynamix_amd.synth import speckle_frames
+    n = 300
+    f = speckle_frames(shape, n, 2.0, 20.0, 5, np.uint8)
+    q = np.ones(shape, np.int32)
+    q.flat[::5] = 2                 # runs of four interrupted by another bin
+    q.flat[3] = 0                   # a masked pixel inside the first quad
+    c = FFTCorrelator(shape, n, q)
+    g2 = c.correlate(f)
+    _check(c, g2, f, q)
