@@ -218,6 +218,7 @@ struct dxg2_plan {
     std::vector<BandWork> gband;
     std::vector<int2> gband_kb;
     std::vector<GatherWork> gfall;
+    int64_t gfall_kb = 0;  // k-blocks on the fallback
     int gband_srow = 0;
     BandWork* d_gband = nullptr;
     int2* d_gband_kb = nullptr;
@@ -359,6 +360,7 @@ void build_band_work(dxg2_plan* p, const std::vector<std::vector<int32_t>>& list
     p->gband.clear();
     p->gband_kb.clear();
     p->gfall.clear();
+    p->gfall_kb = 0;
     p->gband_srow = 0;
     const int W = p->W, H = p->H;
     // bands of RB rows, staged with one row of overlap below: RB + 1 rows of a frame fit
@@ -378,6 +380,7 @@ void build_band_work(dxg2_plan* p, const std::vector<std::vector<int32_t>>& list
         auto flush = [&](int kb_end) {
             for (int k = fall0; k < kb_end; k += 16)
                 p->gfall.push_back(GatherWork{b, k, std::min(16, kb_end - k), 0});
+            p->gfall_kb += kb_end - fall0;
             fall0 = -1;
         };
         const int nkb = (n + KB - 1) / KB;
@@ -409,15 +412,13 @@ void build_band_work(dxg2_plan* p, const std::vector<std::vector<int32_t>>& list
         }
     }
     p->gband_srow = ((RB + 1) * W + 16 + 15) / 16 * 16;
-    // the band kernel stages BAND_STAGE frames of rows in LDS
-    int dev = 0, lds_max = 0;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) !=
-            hipSuccess ||
-        (size_t)BAND_STAGE * p->gband_srow + 64 > (size_t)lds_max) {
+    // the band kernel stages BAND_STAGE frames of rows in LDS, two workgroups per CU
+    // (gfx950: 160 KiB of LDS per CU)
+    if ((size_t)BAND_STAGE * p->gband_srow + 64 > (size_t)GFX950_LDS_BYTES / 2) {
         p->gband.clear();
         p->gband_kb.clear();
         p->gfall.clear();
+        p->gfall_kb = 0;
         p->gband_srow = 0;
     }
 }
@@ -1067,7 +1068,7 @@ int dxg2_plan_dry_run(const dxg2_plan_desc* desc, int64_t* summary, int32_t n) {
                          (int64_t)p->gather_work.size(),
                          (int64_t)p->gband.size(),
                          (int64_t)p->gband_kb.size(),
-                         (int64_t)p->gfall.size(),
+                         p->gfall_kb,
                          (int64_t)(p->gram_work.size() + p->gram_xwork.size()),
                          (int64_t)(p->qm_work.size() + p->qx_work.size()),
                          (int64_t)p->event_work.size(),

@@ -56,6 +56,7 @@ constexpr int PM_XCD_GROUP = 128;
 constexpr int BAND_ROW_BYTES = 9728;
 constexpr int BAND_KMAX = 36;
 constexpr int BAND_ITEM_KMAX = 4 * BAND_KMAX;
+constexpr int GFX950_LDS_BYTES = 160 * 1024;  // LDS per CU
 
 struct BandWork {        // one band of detector rows of the band gather (x dimension)
     int32_t y0;          // first staged row

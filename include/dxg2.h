@@ -83,9 +83,9 @@ typedef struct dxg2_plan_desc {
 int dxg2_plan_create(const dxg2_plan_desc* desc, dxg2_plan** out);
 /* Runs the host half of dxg2_plan_create (validation, bins, pixel lists, slab layout,
  * every work list) with no device call, and writes up to n of: n_bins, padded pixel
- * count, slab bytes, gather items, band items, band k-blocks, fallback gather items,
- * Gram items, Gram items of the queue schedule, event items, calc_std tiles, calc_std
- * chunk items.  Returns the error dxg2_plan_create would return for the description.
+ * count, slab bytes, gather items, band items, band k-blocks, k-blocks on the band
+ * gather's fallback, Gram items, Gram items of the queue schedule, event items, calc_std
+ * tiles, calc_std chunk items.  Returns the error dxg2_plan_create would return for the description.
  * For checking the plan builder off the GPU (sanitizers, fuzzing).  No reference
  * counterpart. */
 int dxg2_plan_dry_run(const dxg2_plan_desc* desc, int64_t* summary, int32_t n);

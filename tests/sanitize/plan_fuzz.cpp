@@ -80,6 +80,10 @@ int check_case(int c, std::mt19937_64& rng) {
         if (s[i] < 0) return fail(c, "work list size", s[i], 0);
     if (padded > 0 && s[3] == 0) return fail(c, "gather items", s[3], 1);
     if (padded > 0 && s[7] == 0) return fail(c, "Gram items", s[7], 1);
+    // the band gather, when it has bands, covers every k-block once: its own list plus the
+    // fallback's
+    if (s[4] > 0 && s[5] + s[6] != padded / kBlock)
+        return fail(c, "band + fallback k-blocks", s[5] + s[6], padded / kBlock);
 
     // the same description with a wrong number of scale factors is refused
     if (n_bins > 1) {
