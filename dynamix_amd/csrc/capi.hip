@@ -1949,11 +1949,8 @@ int dxg2_correlate_fft(dxg2_plan* p, const void* d_frames, double* d_num, int64_
     if (ev) DXG2_HIP(hipEventRecord(ev[0], s));
     launch_fill_i64(reinterpret_cast<int64_t*>(p->d_power), (int64_t)p->n_bins * (p->fft_L + 1), s);
     launch_gather_pm((const uint8_t*)d_frames, p->dtype, (int64_t)p->H * p->W, p->N, p->nrows,
-                     p->d_pix, p->d_bins, p->d_kbgather, (int)p->kb_gather.size(), p->d_xs, s);
-    int64_t max_npix = 0;
-    for (auto n : p->npix) max_npix = std::max(max_npix, n);
-    launch_pm_sums(p->d_xs, p->dtype, p->d_bins, p->n_bins, (int)max_npix, p->N, p->nrows, sums,
-                   s);
+                     p->d_pix, p->d_bins, p->n_bins, p->d_kbgather, (int)p->kb_gather.size(),
+                     p->d_xs, sums, s);
     if (ev) DXG2_HIP(hipEventRecord(ev[1], s));
     if (p->fft_L1) {
         const int total = (int)p->kb_work.size();

@@ -13,8 +13,9 @@
 // set of mixed-radix lengths (2^a 5^b), so a linear autocorrelation needs no more.
 //  * gather_pm_kernel<E>: frames -> pixel-major series xs[k][t] of the frame element
 //    type E (uint8, int8 or uint16; k in the bin's raster list order, t < nrows, zero
-//    past the last frame), 64 pixels x 256 frames per workgroup through an LDS transpose.
-//  * pm_sums_kernel<E>: exact S_b(t) from xs.
+//    past the last frame), 64 pixels x 256 frames per workgroup through an LDS transpose;
+//    the same workgroup adds its 64 pixels per frame from the LDS tile and sends each
+//    frame's total to the exact S_b(t) with one 64-bit atomic.
 //  * fft_spectra_kernel<E, L, T, R...>: one workgroup per (bin, k-block of 64 series);
 //    each series is packed as z[n] = x[2n] + i x[2n+1] (L complex points) and
 //    transformed by an FP32 mixed-radix Stockham FFT (Govindaraju et al., "High
@@ -207,19 +208,29 @@ struct Elem<uint16_t> {
 // it to its pixel's row of an LDS tile whose row stride is an odd number of words, so
 // the 64 lanes' column writes fall in distinct banks; then each thread reads 16
 // contiguous bytes of one row and stores them to xs, so consecutive threads write
-// consecutive 16-byte pieces of a pixel's series.
-template <typename E>
+// consecutive 16-byte pieces of a pixel's series.  The frame sums S_b(t) (zeroed first)
+// come from the same tile: a thread adds one word (4 frames of 8-bit values in 16-bit
+// halves, or 2 of 16-bit values in 32-bit registers) down 16 or 32 pixel rows, the row
+// groups meet in LDS, and each frame's total over the 64 pixels (below 2^31) goes to
+// sums with one 64-bit atomic (the row groups' partials reuse the tile's LDS once every
+// thread has read it).  SGN (int8 frames): bytes are added as v ^ 0x80 = v + 128
+// and 64 x 128 is taken off the total (padded pixels and frames hold 0, so they add 128
+// each and are taken off with the rest).
+template <typename E, bool SGN>
 __global__ __launch_bounds__(256) void gather_pm_kernel(const uint8_t* __restrict__ frames,
                                                         int64_t hw, int nframes, int nrows,
                                                         const int32_t* __restrict__ pix_idx,
                                                         const BinDev* __restrict__ bins,
                                                         const KbWork* __restrict__ work,
-                                                        uint8_t* __restrict__ xs) {
+                                                        uint8_t* __restrict__ xs,
+                                                        int64_t* __restrict__ sums) {
     constexpr int ES = sizeof(E);
     constexpr int FPW = 4 / ES;        // frames per word
     constexpr int WPR = 256 / FPW;     // words per pixel row of the tile
     constexpr int STRIDE = WPR + 1;    // odd
-    __shared__ uint32_t tile[KB * STRIDE];
+    constexpr int RG = 256 / WPR;      // row groups of the sums (4 or 2)
+    __shared__ __attribute__((aligned(16))) uint32_t tile[KB * STRIDE];
+    static_assert(RG * 256 <= KB * STRIDE, "partials fit the tile");
     const KbWork w = work[blockIdx.x];
     if (w.kb < 0) return;  // padding of the gather order
     const BinDev bd = bins[w.bin];
@@ -251,50 +262,43 @@ __global__ __launch_bounds__(256) void gather_pm_kernel(const uint8_t* __restric
         uint8_t* dst = xs + bd.slab_off + (((int64_t)w.kb * KB + j) * nrows + t0) * ES + 4 * o;
         *reinterpret_cast<uint4*>(dst) = make_uint4(r[0], r[1], r[2], r[3]);
     }
-}
-
-// S_b(t) = sum over the bin's pixels of xs[k][t].
-// grid (frame tiles of 256, bins, pixel chunks of PM_CH).  Thread (r, g) = (tid / 16,
-// tid % 16) adds frames 16 g .. 16 g + 15 of pixel rows chunk + r, r + 16, ... in 32-bit
-// lanes (at most PM_CH / 16 rows of |value| <= 65535: below 2^31); the 16 row groups
-// are reduced in LDS and each frame's chunk total goes to sums (zeroed first) with one
-// 64-bit atomic add.
-constexpr int PM_CH = 1024;
-
-template <typename E>
-__global__ __launch_bounds__(256) void pm_sums_kernel(const uint8_t* __restrict__ xs,
-                                                      const BinDev* __restrict__ bins,
-                                                      int nframes, int nrows,
-                                                      int64_t* __restrict__ sums) {
-    __shared__ int32_t part[16][256 + 1];
-    const int b = blockIdx.y;
-    const BinDev bd = bins[b];
-    const int k0 = blockIdx.z * PM_CH;
-    if (k0 >= bd.npix) return;
-    const int k1 = min(bd.npix, k0 + PM_CH);
-    const int t0 = blockIdx.x * 256;
-    const int r = threadIdx.x >> 4, g = threadIdx.x & 15;
-    int32_t acc[16];
+    const int k = threadIdx.x % WPR, g = threadIdx.x / WPR;  // word k, rows KB/RG g ..
+    const uint32_t* col = tile + (KB / RG) * g * STRIDE + k;
+    uint32_t s0 = 0u, s1 = 0u;
+    if constexpr (ES == 1) {
+        // s0: frames 4k, 4k + 2 and s1: 4k + 1, 4k + 3 in 16-bit halves (16 rows x 255)
 #pragma unroll
-    for (int q = 0; q < 16; ++q) acc[q] = 0;
-    const E* base = reinterpret_cast<const E*>(xs + bd.slab_off) + t0 + 16 * g;
-    for (int k = k0 + r; k < k1; k += 16) {
-        const E* row = base + (int64_t)k * nrows;
-        E v[16];
-        __builtin_memcpy(v, row, sizeof(v));    // 16 or 32 aligned bytes
-#pragma unroll
-        for (int q = 0; q < 16; ++q) acc[q] += (int32_t)v[q];
+        for (int j = 0; j < KB / RG; ++j) {
+            const uint32_t x = col[j * STRIDE] ^ (SGN ? 0x80808080u : 0u);
+            s0 += x & 0x00FF00FFu;
+            s1 += (x >> 8) & 0x00FF00FFu;
+        }
+    } else {
+        // s0: frame 2k, s1: 2k + 1 (32 rows x 65535)
+#pragma unroll 8
+        for (int j = 0; j < KB / RG; ++j) {
+            const uint32_t x = col[j * STRIDE];
+            s0 += x & 0xFFFFu;
+            s1 += x >> 16;
+        }
     }
-#pragma unroll
-    for (int q = 0; q < 16; ++q) part[r][16 * g + q] = acc[q];
+    __syncthreads();  // every thread has read the tile: its words now hold the partials
+    int32_t* part = reinterpret_cast<int32_t*>(tile);  // [RG][256]
+    if constexpr (ES == 1)
+        *reinterpret_cast<int4*>(part + 256 * g + 4 * k) =
+            make_int4((int)(s0 & 0xFFFFu), (int)(s1 & 0xFFFFu), (int)(s0 >> 16),
+                      (int)(s1 >> 16));
+    else
+        *reinterpret_cast<int2*>(part + 256 * g + 2 * k) = make_int2((int)s0, (int)s1);
     __syncthreads();
     const int t = t0 + threadIdx.x;
     if (t < nframes) {
-        int64_t tot = 0;
+        int64_t tot = SGN ? -(int64_t)KB * 128 : 0;
 #pragma unroll
-        for (int i = 0; i < 16; ++i) tot += part[i][threadIdx.x];
-        if (tot) atomicAdd(reinterpret_cast<unsigned long long*>(sums + (int64_t)b * nframes + t),
-                           (unsigned long long)tot);
+        for (int q = 0; q < RG; ++q) tot += part[256 * q + threadIdx.x];
+        if (tot)
+            atomicAdd(reinterpret_cast<unsigned long long*>(sums + (int64_t)w.bin * nframes + t),
+                      (unsigned long long)tot);
     }
 }
 
@@ -740,36 +744,30 @@ __global__ __launch_bounds__(256) void fft_inverse_kernel(const double* __restri
 }
 
 // ------------------------------------------------------------------ launchers
-template <typename E>
+template <typename E, bool SGN>
 static void gather_pm_typed(const uint8_t* frames, int64_t hw, int nframes, int nrows,
                             const int32_t* pix_idx, const BinDev* bins, const KbWork* work,
-                            int n_work, uint8_t* xs, hipStream_t s) {
-    gather_pm_kernel<E><<<dim3((unsigned)n_work, (unsigned)(nrows / 256)), 256, 0, s>>>(
-        frames, hw, nframes, nrows, pix_idx, bins, work, xs);
+                            int n_work, uint8_t* xs, int64_t* sums, hipStream_t s) {
+    gather_pm_kernel<E, SGN><<<dim3((unsigned)n_work, (unsigned)(nrows / 256)), 256, 0, s>>>(
+        frames, hw, nframes, nrows, pix_idx, bins, work, xs, sums);
 }
 
 void launch_gather_pm(const uint8_t* frames, int dtype, int64_t hw, int nframes, int nrows,
-                      const int32_t* pix_idx, const BinDev* bins, const KbWork* work, int n_work,
-                      uint8_t* xs, hipStream_t s) {
+                      const int32_t* pix_idx, const BinDev* bins, int n_bins,
+                      const KbWork* work, int n_work, uint8_t* xs, int64_t* sums,
+                      hipStream_t s) {
+    if (n_bins <= 0) return;
+    launch_fill_i64(sums, (int64_t)n_bins * nframes, s);
     if (n_work <= 0) return;
     if (dtype == DXG2_U16)
-        gather_pm_typed<uint16_t>(frames, hw, nframes, nrows, pix_idx, bins, work, n_work, xs, s);
-    else
-        gather_pm_typed<uint8_t>(frames, hw, nframes, nrows, pix_idx, bins, work, n_work, xs, s);
-}
-
-void launch_pm_sums(const uint8_t* xs, int dtype, const BinDev* bins, int n_bins, int max_npix,
-                    int nframes, int nrows, int64_t* sums, hipStream_t s) {
-    if (n_bins <= 0 || max_npix <= 0) return;
-    launch_fill_i64(sums, (int64_t)n_bins * nframes, s);
-    const dim3 grid((unsigned)((nframes + 255) / 256), (unsigned)n_bins,
-                    (unsigned)((max_npix + PM_CH - 1) / PM_CH));
-    if (dtype == DXG2_U16)
-        pm_sums_kernel<uint16_t><<<grid, 256, 0, s>>>(xs, bins, nframes, nrows, sums);
+        gather_pm_typed<uint16_t, false>(frames, hw, nframes, nrows, pix_idx, bins, work, n_work,
+                                         xs, sums, s);
     else if (dtype == DXG2_I8)
-        pm_sums_kernel<int8_t><<<grid, 256, 0, s>>>(xs, bins, nframes, nrows, sums);
+        gather_pm_typed<uint8_t, true>(frames, hw, nframes, nrows, pix_idx, bins, work, n_work,
+                                       xs, sums, s);
     else
-        pm_sums_kernel<uint8_t><<<grid, 256, 0, s>>>(xs, bins, nframes, nrows, sums);
+        gather_pm_typed<uint8_t, false>(frames, hw, nframes, nrows, pix_idx, bins, work, n_work,
+                                        xs, sums, s);
 }
 
 // Lengths with a plan: L (complex points), threads, radices.
