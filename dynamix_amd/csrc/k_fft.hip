@@ -164,6 +164,13 @@ struct Dft<16> {
 // LDS index of point i: the low 4 bits are xor-ed with the next 4, so the stride-16
 // accesses of a pass hit 16 different 8-byte slots and the array needs no padding.
 __device__ __forceinline__ int sw(int i) { return i ^ ((i >> 4) & 15); }
+// sw(base + r S): for S a multiple of 256, r S leaves bits 0..7 of base + r S as they are
+// in base, so the swizzle is sw(base) + r S (one address per butterfly, constant offsets)
+template <int S>
+__device__ __forceinline__ int sw_step(int base, int r) {
+    if constexpr (S % 256 == 0) return sw(base) + r * S;
+    else return sw(base + r * S);
+}
 
 // ----------------------------------------------------------- series layout
 // Raw load of complex point i of a series: elements 2i and 2i + 1 (one 16-bit word for
@@ -330,7 +337,7 @@ __device__ __forceinline__ void mid_pass(c32* z, const c32* tab, int tid) {
             const int k = b % NS;
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                c32 a = z[sw(b + r * NB)];
+                c32 a = z[sw_step<NB>(b, r)];
                 if (r) a = cmul(a, tab[r * k]);      // exp(-2 pi i r k / (R NS)), PT = R NS
                 v[m][r] = a;
             }
@@ -345,7 +352,7 @@ __device__ __forceinline__ void mid_pass(c32* z, const c32* tab, int tid) {
             const int k = b % NS;
             const int o = (b - k) * R + k;
 #pragma unroll
-            for (int q = 0; q < R; ++q) z[sw(o + q * NS)] = v[m][q];
+            for (int q = 0; q < R; ++q) z[sw_step<NS>(o, q)] = v[m][q];
         }
     }
     __syncthreads();
@@ -361,6 +368,7 @@ __global__ __launch_bounds__(Plan::T) void fft_spectra_kernel(
     constexpr int NB1 = L / R1, M1 = (NB1 + T - 1) / T;       // first pass
     constexpr int NBL = L / RL, ML = (NBL + T - 1) / T;       // last pass
     constexpr int MU = (L + 1 + T - 1) / T;                   // unpacked points
+    static_assert((MU - 1) * T <= L, "unpacked points");
     __shared__ c32 z[L];
     __shared__ c32 tab2[Plan::P2];
     __shared__ c32 tab3[Plan::P3];
@@ -434,23 +442,25 @@ __global__ __launch_bounds__(Plan::T) void fft_spectra_kernel(
                 c32 v[RL];
 #pragma unroll
                 for (int r = 0; r < RL; ++r) {
-                    const c32 a = z[sw(b + r * NBL)];
+                    const c32 a = z[sw_step<NBL>(b, r)];
                     v[r] = r ? cmul(a, c_of(tw[r * b])) : a;
                 }
                 Dft<RL>::run(v);
 #pragma unroll
-                for (int q = 0; q < RL; ++q) z[sw(b + q * NBL)] = v[q];
+                for (int q = 0; q < RL; ++q) z[sw_step<NBL>(b, q)] = v[q];
             }
         }
         __syncthreads();
         // unpack X[k] = (Z[k] + conj Z[L-k]) / 2 - i W^k (Z[k] - conj Z[L-k]) / 2,
-        // W = exp(-pi i / L); accumulate |X[k]|^2
+        // W = exp(-pi i / L); accumulate |X[k]|^2.  With t < T and (MU - 1) T <= L, only
+        // m = 0 can meet k = 0 and only m = MU - 1 can meet or pass k = L: the other
+        // points need no test, so their reads (factors from L1/L2 included) issue together.
 #pragma unroll
         for (int m = 0; m < MU; ++m) {
             const int k = t + m * T;
-            if (k <= L) {
-                const c32 a = z[sw(k == L ? 0 : k)];
-                const c32 c = z[sw(k == 0 ? 0 : L - k)];
+            if (m < MU - 1 || k <= L) {
+                const c32 a = z[sw(m == MU - 1 && k == L ? 0 : k)];
+                const c32 c = z[sw(m == 0 && k == 0 ? 0 : L - k)];
                 const c32 s2 = c32{a.x + c.x, a.y - c.y};
                 const c32 d = c32{a.x - c.x, a.y + c.y};
                 const c32 wd = cmul(c_of(uw[k]), d);
