@@ -164,11 +164,15 @@ struct Dft<16> {
 // LDS index of point i: the low 4 bits are xor-ed with the next 4, so the stride-16
 // accesses of a pass hit 16 different 8-byte slots and the array needs no padding.
 __device__ __forceinline__ int sw(int i) { return i ^ ((i >> 4) & 15); }
-// sw(base + r S): for S a multiple of 256, r S leaves bits 0..7 of base + r S as they are
-// in base, so the swizzle is sw(base) + r S (one address per butterfly, constant offsets)
+// sw(base + r S) with r known at compile time (r may be negative): for S a multiple of
+// 256, r S leaves bits 0..7 of base + r S as they are in base, so the swizzle is
+// sw(base) + r S; for S an odd multiple of 128 it leaves bits 0..6 and flips bit 7 when
+// r is odd, which flips bit 3 of the xor term: (sw(base) ^ 8) + r S.  Either way one
+// address per butterfly and constant offsets.
 template <int S>
 __device__ __forceinline__ int sw_step(int base, int r) {
     if constexpr (S % 256 == 0) return sw(base) + r * S;
+    else if constexpr (S % 128 == 0) return (sw(base) ^ ((r & 1) << 3)) + r * S;
     else return sw(base + r * S);
 }
 
@@ -459,8 +463,8 @@ __global__ __launch_bounds__(Plan::T) void fft_spectra_kernel(
         for (int m = 0; m < MU; ++m) {
             const int k = t + m * T;
             if (m < MU - 1 || k <= L) {
-                const c32 a = z[sw(m == MU - 1 && k == L ? 0 : k)];
-                const c32 c = z[sw(m == 0 && k == 0 ? 0 : L - k)];
+                const c32 a = z[m == MU - 1 ? sw(k == L ? 0 : k) : sw_step<T>(t, m)];
+                const c32 c = z[m == 0 ? sw(k == 0 ? 0 : L - k) : sw_step<T>(L - t, -m)];
                 const c32 s2 = c32{a.x + c.x, a.y - c.y};
                 const c32 d = c32{a.x - c.x, a.y + c.y};
                 const c32 wd = cmul(c_of(uw[k]), d);
