@@ -149,6 +149,25 @@ struct Dft<16> {
             for (int m = 0; m < 4; ++m) e[p][m] = a[4 * m + p];
             Dft<4>::run(e[p]);
         }
+        twist(e, a);
+    }
+    // a[8..15] zero (the zero-padded half of a series): the first DFT4s see
+    // (a[p], a[4 + p], 0, 0), and Dft<4> on them gives these sums and differences (its
+    // terms with the zeros are exact), so the result is the same as run()'s.
+    __device__ __forceinline__ static void run_lo8(c32* a) {
+        c32 e[4][4];
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+            const c32 x0 = a[p], x1 = a[4 + p], y1 = negi(x1);
+            e[p][0] = cadd(x0, x1);
+            e[p][1] = cadd(x0, y1);
+            e[p][2] = csub(x0, x1);
+            e[p][3] = csub(x0, y1);
+        }
+        twist(e, a);
+    }
+    // second stage: a[k + 4q] = DFT4 over p of (w16^(p k) E_p[k]), output q
+    __device__ __forceinline__ static void twist(c32 (&e)[4][4], c32* a) {
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             c32 v[4];
@@ -390,11 +409,16 @@ __global__ __launch_bounds__(Plan::T) void fft_spectra_kernel(
     const int kend = min(KB, bd.npix - w.kb * KB);
     const uint8_t* rows = xs + bd.slab_off + (int64_t)w.kb * KB * nrows * Elem<E>::SIZE;
     const int64_t row_bytes = (int64_t)nrows * Elem<E>::SIZE;
-    uint32_t raw[M1][R1];
+    // Points from L / 2 on are frames from L on, past every row (the launcher checks
+    // nrows <= L): the zero padding.  With R1 = 16 they are the first pass's inputs
+    // r >= 8, which are neither loaded nor added.
+    constexpr int RLD = R1 == 16 ? 8 : R1;   // inputs loaded per first-pass butterfly
+    static_assert(R1 != 16 || 8 * NB1 == L / 2, "zero half");
+    uint32_t raw[M1][RLD];
 #pragma unroll
     for (int m = 0; m < M1; ++m)
 #pragma unroll
-        for (int r = 0; r < R1; ++r) {
+        for (int r = 0; r < RLD; ++r) {
             const int b = tid + m * T;
             raw[m][r] = (NB1 % T == 0 || b < NB1) ? Elem<E>::load(rows, b + r * NB1, nrows) : 0u;
         }
@@ -416,8 +440,11 @@ __global__ __launch_bounds__(Plan::T) void fft_spectra_kernel(
             if (NB1 % T == 0 || b < NB1) {
                 c32 v[R1];
 #pragma unroll
-                for (int r = 0; r < R1; ++r) v[r] = Elem<E>::value(raw[m][r]);
-                Dft<R1>::run(v);
+                for (int r = 0; r < RLD; ++r) v[r] = Elem<E>::value(raw[m][r]);
+                if constexpr (R1 == 16)
+                    Dft<16>::run_lo8(v);
+                else
+                    Dft<R1>::run(v);
 #pragma unroll
                 for (int q = 0; q < R1; ++q) z[sw(b * R1 + q)] = v[q];
             }
@@ -428,7 +455,7 @@ __global__ __launch_bounds__(Plan::T) void fft_spectra_kernel(
 #pragma unroll
             for (int m = 0; m < M1; ++m)
 #pragma unroll
-                for (int r = 0; r < R1; ++r) {
+                for (int r = 0; r < RLD; ++r) {
                     const int b = t + m * T;
                     if (NB1 % T == 0 || b < NB1)
                         raw[m][r] = Elem<E>::load(nxt, b + r * NB1, nrows);
@@ -827,6 +854,7 @@ int launch_fft_power(int L, int dtype, const uint8_t* xs, const BinDev* bins, co
                      int n_work, int nrows, const float2* tw, const float2* uw, double* power,
                      hipStream_t s) {
     if (n_work <= 0) return 0;
+    if (nrows > L) return -1;  // the spectra kernels take frames from L on as zero
 #define DXG2_FFT_CASE(LL)                                                                  \
     case LL:                                                                               \
         spectra_dtype<Plan##LL>(dtype, xs, bins, work, n_work, nrows, tw, uw, power, s);   \
