@@ -390,8 +390,8 @@ __global__ __launch_bounds__(Plan::T) void fft_spectra_kernel(
     constexpr int R1 = Plan::R1, R2 = Plan::R2, R3 = Plan::R3, RL = Plan::RL;
     constexpr int NB1 = L / R1, M1 = (NB1 + T - 1) / T;       // first pass
     constexpr int NBL = L / RL, ML = (NBL + T - 1) / T;       // last pass
-    constexpr int MU = (L + 1 + T - 1) / T;                   // unpacked points
-    static_assert((MU - 1) * T <= L, "unpacked points");
+    constexpr int HP = L / 2;                                 // k = HP pairs with itself
+    constexpr int MH = (HP + T - 1) / T;                      // pairs (k, L - k), k < HP
     __shared__ c32 z[L];
     __shared__ c32 tab2[Plan::P2];
     __shared__ c32 tab3[Plan::P3];
@@ -402,9 +402,9 @@ __global__ __launch_bounds__(Plan::T) void fft_spectra_kernel(
     for (int m = tid; m < Plan::P2; m += T) tab2[m] = c_of(tw[m * (L / Plan::P2)]);
     if (Plan::NP == 4)
         for (int m = tid; m < Plan::P3; m += T) tab3[m] = c_of(tw[m * (L / Plan::P3)]);
-    double acc[MU];
+    double acc_lo[MH], acc_hi[MH], acc_mid = 0.0;  // |X|^2 at k, L - k, HP
 #pragma unroll
-    for (int m = 0; m < MU; ++m) acc[m] = 0.0;
+    for (int m = 0; m < MH; ++m) acc_lo[m] = acc_hi[m] = 0.0;
 
     const int kend = min(KB, bd.npix - w.kb * KB);
     const uint8_t* rows = xs + bd.slab_off + (int64_t)w.kb * KB * nrows * Elem<E>::SIZE;
@@ -482,32 +482,48 @@ __global__ __launch_bounds__(Plan::T) void fft_spectra_kernel(
             }
         }
         __syncthreads();
-        // unpack X[k] = (Z[k] + conj Z[L-k]) / 2 - i W^k (Z[k] - conj Z[L-k]) / 2,
-        // W = exp(-pi i / L); accumulate |X[k]|^2.  With t < T and (MU - 1) T <= L, only
-        // m = 0 can meet k = 0 and only m = MU - 1 can meet or pass k = L: the other
-        // points need no test, so their reads (factors from L1/L2 included) issue together.
-#pragma unroll
-        for (int m = 0; m < MU; ++m) {
-            const int k = t + m * T;
-            if (m < MU - 1 || k <= L) {
-                const c32 a = z[m == MU - 1 ? sw(k == L ? 0 : k) : sw_step<T>(t, m)];
-                const c32 c = z[m == 0 ? sw(k == 0 ? 0 : L - k) : sw_step<T>(L - t, -m)];
-                const c32 s2 = c32{a.x + c.x, a.y - c.y};
-                const c32 d = c32{a.x - c.x, a.y + c.y};
-                const c32 wd = cmul(c_of(uw[k]), d);
-                const float xr = 0.5f * (s2.x + wd.y);
-                const float xi = 0.5f * (s2.y - wd.x);
-                acc[m] += (double)xr * xr + (double)xi * xi;
+        // unpack X[k] = S + D with S = (Z[k] + conj Z[L-k]) / 2 and
+        // D = -i W^k (Z[k] - conj Z[L-k]) / 2, W = exp(-pi i / L), Z[L] = Z[0].  As
+        // W^(L-k) = -conj W^k, the same S and D give conj X[L-k] = S - D, so each pair
+        // (k, L - k), k < L/2, is read and unpacked once (k = L/2 pairs with itself), and
+        // |X|^2 accumulates at both ends.  With t < T only m = 0 meets k = 0, and only
+        // the last m can pass L/2: the other points need no test.
+        auto unpack = [&](c32 a, c32 c, int k, double& lo, double* hi) {
+            const c32 s2 = c32{a.x + c.x, a.y - c.y};
+            const c32 d = c32{a.x - c.x, a.y + c.y};
+            const c32 wd = cmul(c_of(uw[k]), d);
+            const float xr = 0.5f * (s2.x + wd.y), xi = 0.5f * (s2.y - wd.x);
+            lo += (double)xr * xr + (double)xi * xi;
+            if (hi) {
+                const float yr = 0.5f * (s2.x - wd.y), yi = 0.5f * (s2.y + wd.x);
+                *hi += (double)yr * yr + (double)yi * yi;
             }
+        };
+#pragma unroll
+        for (int m = 0; m < MH; ++m) {
+            const int k = t + m * T;
+            if (HP % T == 0 || m < MH - 1 || k < HP) {
+                const c32 a = z[sw_step<T>(t, m)];
+                const c32 c = z[m == 0 ? sw(k == 0 ? 0 : L - k) : sw_step<T>(L - t, -m)];
+                unpack(a, c, k, acc_lo[m], &acc_hi[m]);
+            }
+        }
+        if (t == 0) {
+            const c32 a = z[sw(HP)];
+            unpack(a, a, HP, acc_mid, nullptr);
         }
         __syncthreads();
     }
     double* prow = power + (int64_t)w.bin * (L + 1);
 #pragma unroll
-    for (int m = 0; m < MU; ++m) {
+    for (int m = 0; m < MH; ++m) {
         const int k = tid + m * T;
-        if (k <= L && acc[m] != 0.0) atomicAdd(prow + k, acc[m]);
+        if (k < HP) {
+            if (acc_lo[m] != 0.0) atomicAdd(prow + k, acc_lo[m]);
+            if (acc_hi[m] != 0.0) atomicAdd(prow + L - k, acc_hi[m]);
+        }
     }
+    if (tid == 0 && acc_mid != 0.0) atomicAdd(prow + HP, acc_mid);
 }
 
 // ------------------------------------------------- long series: four-step through HBM
