@@ -292,6 +292,13 @@ struct dxg2_plan {
     KbWork* d_kbwork = nullptr;
     std::vector<KbWork> kb_gather; // the same in gather order (kb -1: padding)
     KbWork* d_kbgather = nullptr;
+    // the FFT path's pixel order within each bin (its sums do not depend on it): aligned
+    // quads of detector pixels 4a .. 4a + 3 that lie wholly in the bin first, then the
+    // rest, in raster order; fft_qkb[b] leading k-blocks of bin b are all such quads
+    std::vector<int32_t> pix_fft;
+    std::vector<int32_t> fft_qkb;
+    int32_t* d_pixfft = nullptr;
+    int32_t* d_fftqkb = nullptr;
     uint8_t* d_xs = nullptr;       // pixel-major bytes, same size as the slab
     float2* d_tw = nullptr;        // exp(-2 pi i m / L), m < L; then exp(-pi i k / L), k <= L
     double* d_power = nullptr;     // (n_bins, L + 1)
@@ -676,10 +683,32 @@ int ensure_fft_buffers(dxg2_plan* p, hipStream_t s) {
     p->fft_L2 = L2;
     for (int b = 0; b < p->n_bins; ++b)
         for (int kb = 0; kb < p->bins[b].nkb; ++kb) p->kb_work.push_back(KbWork{b, kb});
+    // quads are loaded as aligned 4-byte words, which needs 4 | H W (and a 4-aligned
+    // frame pointer, checked per call)
+    const bool quads = ((int64_t)p->H * p->W) % 4 == 0;
+    p->pix_fft = p->pix_idx;
+    p->fft_qkb.assign((size_t)std::max(p->n_bins, 1), 0);
+    for (int b = 0; b < p->n_bins && quads; ++b) {
+        const int32_t* list = p->pix_idx.data() + p->bins[b].idx_off;
+        const int n = p->bins[b].npix;
+        std::vector<int32_t> quad, rest;
+        for (int i = 0; i < n;) {
+            if (list[i] % 4 == 0 && i + 3 < n && list[i + 3] == list[i] + 3) {
+                quad.insert(quad.end(), list + i, list + i + 4);  // sorted, distinct
+                i += 4;
+            } else {
+                rest.push_back(list[i++]);
+            }
+        }
+        int32_t* out = p->pix_fft.data() + p->bins[b].idx_off;
+        std::copy(quad.begin(), quad.end(), out);
+        std::copy(rest.begin(), rest.end(), out + quad.size());
+        p->fft_qkb[b] = (int32_t)(quad.size() / KB);
+    }
     p->kb_gather = raster_order_for_xcds(
         p->kb_work,
         [&](const KbWork& w) {
-            return (int64_t)p->pix_idx[p->bins[w.bin].idx_off + (int64_t)w.kb * KB];
+            return (int64_t)p->pix_fft[p->bins[w.bin].idx_off + (int64_t)w.kb * KB];
         },
         PM_XCD_GROUP, KbWork{0, -1});
     p->fft_chunk = L1 ? fft_chunk_kblocks(L, (int)p->kb_work.size()) : 0;
@@ -702,8 +731,14 @@ int ensure_fft_buffers(dxg2_plan* p, hipStream_t s) {
     KbWork* kbw = nullptr;
     KbWork* kbg = nullptr;
     float2* fy = nullptr;
+    int32_t* pixf = nullptr;
+    int32_t* qkb = nullptr;
     if ((L1 && dev_alloc(p->alloc, (void**)&fy, (size_t)p->fft_chunk * KB * L * sizeof(float2),
                          s) != hipSuccess) ||
+        (!p->pix_fft.empty() &&
+         dev_alloc(p->alloc, (void**)&pixf, p->pix_fft.size() * sizeof(int32_t), s) !=
+             hipSuccess) ||
+        dev_alloc(p->alloc, (void**)&qkb, p->fft_qkb.size() * sizeof(int32_t), s) != hipSuccess ||
         dev_alloc(p->alloc, (void**)&xs, p->slab_bytes, s) != hipSuccess ||
         dev_alloc(p->alloc, (void**)&tw, p->twiddles.size() * sizeof(float2), s) != hipSuccess ||
         dev_alloc(p->alloc, (void**)&fnum, (size_t)std::max(p->n_bins, 1) * p->N * sizeof(double), s) !=
@@ -715,11 +750,13 @@ int ensure_fft_buffers(dxg2_plan* p, hipStream_t s) {
         dev_alloc(p->alloc, (void**)&power, (size_t)std::max(p->n_bins, 1) * (L + 1) * sizeof(double), s) !=
             hipSuccess) {
         for (void* b : {(void*)xs, (void*)tw, (void*)fnum, (void*)kbw, (void*)kbg, (void*)power,
-                        (void*)fy})
+                        (void*)fy, (void*)pixf, (void*)qkb})
             dev_free(p->alloc, b, s);
         return set_error(DXG2_ENOMEM, "cannot allocate FFT workspace");
     }
     p->d_fy = fy;
+    p->d_pixfft = pixf;
+    p->d_fftqkb = qkb;
     p->d_xs = xs;
     p->d_tw = tw;
     p->d_fnum = fnum;
@@ -727,6 +764,11 @@ int ensure_fft_buffers(dxg2_plan* p, hipStream_t s) {
     p->d_kbgather = kbg;
     p->d_power = power;
     DXG2_HIP(hipMemcpyAsync(p->d_tw, p->twiddles.data(), p->twiddles.size() * sizeof(float2),
+                            hipMemcpyHostToDevice, s));
+    if (!p->pix_fft.empty())
+        DXG2_HIP(hipMemcpyAsync(p->d_pixfft, p->pix_fft.data(),
+                                p->pix_fft.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
+    DXG2_HIP(hipMemcpyAsync(p->d_fftqkb, p->fft_qkb.data(), p->fft_qkb.size() * sizeof(int32_t),
                             hipMemcpyHostToDevice, s));
     if (!p->kb_work.empty()) {
         DXG2_HIP(hipMemcpyAsync(p->d_kbwork, p->kb_work.data(),
@@ -1229,6 +1271,7 @@ static void plan_release(dxg2_plan* p, hipStream_t s) {
                     p->d_pg, p->d_pm,
                     p->d_ework, p->d_err,  p->d_cnt,   p->d_scan,  p->d_off,
                     p->d_etimes, p->d_edata, p->d_kbwork, p->d_kbgather, p->d_xs, p->d_tw,
+                    p->d_pixfft, p->d_fftqkb,
                     p->d_power, p->d_fnum, p->d_fy, p->d_qmwork, p->d_qxwork, p->d_qoff,
                     p->d_heads, p->d_gband, p->d_gband_kb, p->d_gfall,
                     p->coo_sort.keys, p->coo_sort.vals0, p->coo_sort.vals1, p->coo_sort.tmp,
@@ -1297,7 +1340,8 @@ int dxg2_plan_footprint(const dxg2_plan* p, int32_t path, size_t* bytes) {
                 L1 ? (size_t)fft_chunk_kblocks(L, (int)nkb) * KB * L * sizeof(float2) : 0;
             *bytes = common + p->slab_bytes + (2 * (size_t)L + 1) * sizeof(float2) + scratch +
                      (size_t)std::max(p->n_bins, 1) * ((L + 1) + p->N) * sizeof(double) +
-                     (2 * (size_t)nkb + N_XCD) * sizeof(KbWork);
+                     (2 * (size_t)nkb + N_XCD) * sizeof(KbWork) +
+                     (p->pix_idx.size() + (size_t)std::max(p->n_bins, 1)) * sizeof(int32_t);
             return DXG2_OK;
         }
         case 3: {
@@ -1948,9 +1992,10 @@ int dxg2_correlate_fft(dxg2_plan* p, const void* d_frames, double* d_num, int64_
     if (p->profiling && (rc = next_events(p, &ev))) return rc;
     if (ev) DXG2_HIP(hipEventRecord(ev[0], s));
     launch_fill_i64(reinterpret_cast<int64_t*>(p->d_power), (int64_t)p->n_bins * (p->fft_L + 1), s);
+    const bool quads = reinterpret_cast<uintptr_t>(d_frames) % 4 == 0;
     launch_gather_pm((const uint8_t*)d_frames, p->dtype, (int64_t)p->H * p->W, p->N, p->nrows,
-                     p->d_pix, p->d_bins, p->n_bins, p->d_kbgather, (int)p->kb_gather.size(),
-                     p->d_xs, sums, s);
+                     p->d_pixfft, quads ? p->d_fftqkb : nullptr, p->d_bins, p->n_bins,
+                     p->d_kbgather, (int)p->kb_gather.size(), p->d_xs, sums, s);
     if (ev) DXG2_HIP(hipEventRecord(ev[1], s));
     if (p->fft_L1) {
         const int total = (int)p->kb_work.size();

@@ -230,11 +230,12 @@ void launch_fill_i32(int* p, int64_t n, hipStream_t s);
 void launch_finalize_f64(const int64_t* sums, const double* fnum, const BinDev* bins, int n_bins,
                          int nframes, int64_t* prefix, int64_t* smax, double* dpre, double* num,
                          double* den, float* g2, const DenMfma& dm, hipStream_t s);
-// Pixel-major copy xs of every bin's series and the exact frame sums (sums[b][t] for
-// t < nframes, zeroed here first).
+// Pixel-major copy xs of every bin's series (in the order of pix_idx) and the exact frame
+// sums (sums[b][t] for t < nframes, zeroed here first).  quad_kb (8-bit frames; may be
+// null): per bin, the leading k-blocks whose entries are 16 aligned pixel quads.
 void launch_gather_pm(const uint8_t* frames, int dtype, int64_t hw, int nframes, int nrows,
-                      const int32_t* pix_idx, const BinDev* bins, int n_bins,
-                      const KbWork* work, int n_work, uint8_t* xs, int64_t* sums,
+                      const int32_t* pix_idx, const int32_t* quad_kb, const BinDev* bins,
+                      int n_bins, const KbWork* work, int n_work, uint8_t* xs, int64_t* sums,
                       hipStream_t s);
 int fft_lds_length(int n);
 int fft_four_step_length(int n, int* L1, int* L2);

@@ -246,10 +246,17 @@ struct Elem<uint16_t> {
 // thread has read it).  SGN (int8 frames): bytes are added as v ^ 0x80 = v + 128
 // and 64 x 128 is taken off the total (padded pixels and frames hold 0, so they add 128
 // each and are taken off with the rest).
+// quad_kb (8-bit frames, 4 | hw, frames 4-aligned; else null): per bin, the leading
+// k-blocks whose list entries are 16 aligned quads of detector pixels 4a .. 4a + 3.  In
+// those, lane (g, kk) = (lane % 16, lane / 16) reads quad g of frame 4k + f (f < 4) of word
+// k = 4 (wave + 4 i) + kk as one aligned 4-byte load (a quarter of the load instructions),
+// and a 4 x 4 byte transpose in registers turns the four frames' quads into the four
+// pixels' words of the tile.
 template <typename E, bool SGN>
 __global__ __launch_bounds__(256) void gather_pm_kernel(const uint8_t* __restrict__ frames,
                                                         int64_t hw, int nframes, int nrows,
                                                         const int32_t* __restrict__ pix_idx,
+                                                        const int32_t* __restrict__ quad_kb,
                                                         const BinDev* __restrict__ bins,
                                                         const KbWork* __restrict__ work,
                                                         uint8_t* __restrict__ xs,
@@ -271,17 +278,45 @@ __global__ __launch_bounds__(256) void gather_pm_kernel(const uint8_t* __restric
     const E* fr = reinterpret_cast<const E*>(frames) + (p >= 0 ? p : 0);
     const int nt = p >= 0 ? max(0, min(256, nframes - t0)) : 0;  // frames present
     constexpr uint32_t MASK = ES == 1 ? 0xFFu : 0xFFFFu;
+    if (ES == 1 && quad_kb && w.kb < quad_kb[w.bin]) {  // the same for the whole workgroup
+        const int g = lane & 15, kk = lane >> 4;
+        const int q0 = pix_idx[bd.idx_off + (int64_t)w.kb * KB + 4 * g];  // 4 | q0
+        const int ntq = max(0, min(256, nframes - t0));
 #pragma unroll
-    for (int i = 0; i < WPR / 4; ++i) {
-        const int k = wave + 4 * i;  // word k: frames t0 + FPW k ..
-        uint32_t word = 0u;
+        for (int i = 0; i < WPR / 16; ++i) {
+            const int k = 4 * (wave + 4 * i) + kk;
+            uint32_t v[4];
 #pragma unroll
-        for (int q = 0; q < FPW; ++q) {
-            const int f = FPW * k + q;
-            const uint32_t v = f < nt ? (uint32_t)fr[(int64_t)(t0 + f) * hw] : 0u;
-            word |= (v & MASK) << (8 * ES * q);
+            for (int f4 = 0; f4 < 4; ++f4) {
+                const int f = 4 * k + f4;
+                v[f4] = f < ntq ? *reinterpret_cast<const uint32_t*>(
+                                      frames + (int64_t)(t0 + f) * hw + q0)
+                                : 0u;
+            }
+            // v[f] byte b = pixel 4g + b at frame 4k + f  ->  word k of pixel 4g + b
+            const uint32_t l01 = __builtin_amdgcn_perm(v[1], v[0], 0x05010400u);
+            const uint32_t h01 = __builtin_amdgcn_perm(v[1], v[0], 0x07030602u);
+            const uint32_t l23 = __builtin_amdgcn_perm(v[3], v[2], 0x05010400u);
+            const uint32_t h23 = __builtin_amdgcn_perm(v[3], v[2], 0x07030602u);
+            uint32_t* row = tile + (4 * g) * STRIDE + k;
+            row[0] = __builtin_amdgcn_perm(l23, l01, 0x05040100u);
+            row[STRIDE] = __builtin_amdgcn_perm(l23, l01, 0x07060302u);
+            row[2 * STRIDE] = __builtin_amdgcn_perm(h23, h01, 0x05040100u);
+            row[3 * STRIDE] = __builtin_amdgcn_perm(h23, h01, 0x07060302u);
         }
-        tile[lane * STRIDE + k] = word;
+    } else {
+#pragma unroll
+        for (int i = 0; i < WPR / 4; ++i) {
+            const int k = wave + 4 * i;  // word k: frames t0 + FPW k ..
+            uint32_t word = 0u;
+#pragma unroll
+            for (int q = 0; q < FPW; ++q) {
+                const int f = FPW * k + q;
+                const uint32_t v = f < nt ? (uint32_t)fr[(int64_t)(t0 + f) * hw] : 0u;
+                word |= (v & MASK) << (8 * ES * q);
+            }
+            tile[lane * STRIDE + k] = word;
+        }
     }
     __syncthreads();
     constexpr int CHUNKS = WPR / 4;  // 16-byte pieces per pixel row
@@ -803,28 +838,31 @@ __global__ __launch_bounds__(256) void fft_inverse_kernel(const double* __restri
 // ------------------------------------------------------------------ launchers
 template <typename E, bool SGN>
 static void gather_pm_typed(const uint8_t* frames, int64_t hw, int nframes, int nrows,
-                            const int32_t* pix_idx, const BinDev* bins, const KbWork* work,
-                            int n_work, uint8_t* xs, int64_t* sums, hipStream_t s) {
+                            const int32_t* pix_idx, const int32_t* quad_kb, const BinDev* bins,
+                            const KbWork* work, int n_work, uint8_t* xs, int64_t* sums,
+                            hipStream_t s) {
     gather_pm_kernel<E, SGN><<<dim3((unsigned)n_work, (unsigned)(nrows / 256)), 256, 0, s>>>(
-        frames, hw, nframes, nrows, pix_idx, bins, work, xs, sums);
+        frames, hw, nframes, nrows, pix_idx, quad_kb, bins, work, xs, sums);
 }
 
 void launch_gather_pm(const uint8_t* frames, int dtype, int64_t hw, int nframes, int nrows,
-                      const int32_t* pix_idx, const BinDev* bins, int n_bins,
-                      const KbWork* work, int n_work, uint8_t* xs, int64_t* sums,
+                      const int32_t* pix_idx, const int32_t* quad_kb, const BinDev* bins,
+                      int n_bins, const KbWork* work, int n_work, uint8_t* xs, int64_t* sums,
                       hipStream_t s) {
     if (n_bins <= 0) return;
     launch_fill_i64(sums, (int64_t)n_bins * nframes, s);
     if (n_work <= 0) return;
+    // aligned quad loads need 4 | hw and a 4-aligned frame pointer
+    if (hw % 4 != 0 || reinterpret_cast<uintptr_t>(frames) % 4 != 0) quad_kb = nullptr;
     if (dtype == DXG2_U16)
-        gather_pm_typed<uint16_t, false>(frames, hw, nframes, nrows, pix_idx, bins, work, n_work,
-                                         xs, sums, s);
+        gather_pm_typed<uint16_t, false>(frames, hw, nframes, nrows, pix_idx, nullptr, bins,
+                                         work, n_work, xs, sums, s);
     else if (dtype == DXG2_I8)
-        gather_pm_typed<uint8_t, true>(frames, hw, nframes, nrows, pix_idx, bins, work, n_work,
-                                       xs, sums, s);
+        gather_pm_typed<uint8_t, true>(frames, hw, nframes, nrows, pix_idx, quad_kb, bins, work,
+                                       n_work, xs, sums, s);
     else
-        gather_pm_typed<uint8_t, false>(frames, hw, nframes, nrows, pix_idx, bins, work, n_work,
-                                        xs, sums, s);
+        gather_pm_typed<uint8_t, false>(frames, hw, nframes, nrows, pix_idx, quad_kb, bins,
+                                        work, n_work, xs, sums, s);
 }
 
 // Lengths with a plan: L (complex points), threads, radices.

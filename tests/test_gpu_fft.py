@@ -143,3 +143,29 @@ def test_fft_gather_quads_odd_widths(shape):
     c = FFTCorrelator(shape, n, q)
     g2 = c.correlate(f)
     _check(c, g2, f, q)
+
+
+@pytest.mark.parametrize("dtype", [np.uint8, np.int8])
+@pytest.mark.parametrize("offset", [0, 1])
+def test_fft_aligned_quads_ring_bins(dtype, offset):
+    """8-bit frames with 4 | H W and annular bins: the FFT path orders each bin's pixels as
+    aligned detector quads first (one 4-byte load per quad and frame in the gather) and
+    the rest after.  Sums and denominators stay exact and numerators within the FFT
+    tolerance, with the frames 4-aligned (quad loads) and one byte past it (byte loads)."""
+    import torch
+    _announce("fft aligned quads dtype=%s offset=%d" % (np.dtype(dtype).name, offset))
+    from dynamix_amd import synth
+    from dynamix_amd.correlator.dense import FFTCorrelator
+    shape, n = (40, 64), 700
+    f = synth.speckle_frames(shape, n, 2.0, 20.0, 9, np.uint8)
+    if dtype == np.int8:
+        f = (f.astype(np.int16) - 3).clip(-128, 127).astype(np.int8)
+    q = synth.annular_qmask(shape, 6)
+    q[5, 7:30] = 0                    # a masked run breaks quads inside a ring
+    buf = torch.zeros(f.size + offset, dtype=torch.uint8, device="cuda")
+    buf[offset:] = torch.from_numpy(f.ravel().view(np.uint8)).cuda()
+    view = buf[offset:].view(torch.int8 if dtype == np.int8 else torch.uint8).view(n, *shape)
+    assert (view.data_ptr() % 4 == 0) == (offset == 0)
+    c = FFTCorrelator(shape, n, q)
+    g2 = c.correlate(view)
+    _check(c, g2, f, q)
