@@ -27,7 +27,8 @@
 
 namespace dxg2 {
 
-constexpr int ECAP = 256;       // events of one pixel staged in LDS per wave
+constexpr int ECAP = 256;       // staging entries of one wave in LDS
+constexpr int EMAX = ECAP - 4;  // events of a staged pixel: its 4 sentinels always fit
 constexpr int EPF = ECAP / 64;  // event rounds of the next pixel prefetched in registers
 
 // LDS lag histogram of one work item, shared by the NW waves of its workgroup: 8 KiB of
@@ -82,15 +83,17 @@ struct EvPixel {
 
 // The pairs of one work item (a run of one bin's pixels) in lag window blockIdx.y, with
 // the workgroup's LDS: the lag histogram and each wave's staged events (uint8, int8 and
-// uint16 counts fit 32 bits).
+// uint16 counts fit 32 bits), times and counts in one array st[2][NW][ECAP] so that the
+// two reads of a step share one address.
 template <typename T, int NW>
 __device__ __forceinline__ void event_pairs_item(
     const int32_t* __restrict__ times, const T* __restrict__ data,
     const uint32_t* __restrict__ offsets, const int32_t* __restrict__ pix_idx,
     const BinDev* __restrict__ bins, const EventWork w, int nframes, int64_t nnz,
     int64_t* __restrict__ num, int64_t* __restrict__ sums, int* __restrict__ err,
-    typename EvHist<T>::type* __restrict__ hist, int (*__restrict__ st_t)[ECAP],
-    int (*__restrict__ st_d)[ECAP]) {
+    typename EvHist<T>::type* __restrict__ hist, int (*__restrict__ st)[NW][ECAP]) {
+    int (*const st_t)[ECAP] = st[0];
+    int (*const st_d)[ECAP] = st[1];
     using H = typename EvHist<T>::type;
     constexpr int EW = EvHist<T>::lags_per_wave * NW;  // counters
     constexpr int WS = EvHist<T>::window(NW);          // lags of a window
@@ -144,7 +147,7 @@ __device__ __forceinline__ void event_pairs_item(
 #pragma unroll
         for (int q = 0; q < EPF; ++q) {
             const int k = lane + 64 * q;
-            if (k < x.n && x.n <= ECAP) {
+            if (k < x.n && x.n <= EMAX) {
                 ct[q] = times[x.beg + k];
                 cd[q] = (int)data[x.beg + k];
             }
@@ -161,7 +164,7 @@ __device__ __forceinline__ void event_pairs_item(
         // wave-uniform (the compiler keeps them, and the loops they bound, scalar)
         const int n = __builtin_amdgcn_readfirstlane(cur.n);
         const uint32_t beg = (uint32_t)__builtin_amdgcn_readfirstlane((int)cur.beg);
-        const bool staged = n <= ECAP;
+        const bool staged = n <= EMAX;
         long long sq = 0;  // this lane's share of the pixel's sum of d^2
         if (n > 0 && staged) {
 #pragma unroll
@@ -171,7 +174,7 @@ __device__ __forceinline__ void event_pairs_item(
                     st_t[wave][k] = ct[q];
                     st_d[wave][k] = cd[q];
                     sq += (long long)cd[q] * cd[q];
-                } else if (k < n + 4 && k < ECAP) {
+                } else if (k < n + 4) {
                     st_t[wave][k] = 0x7fffffff;  // sentinel time: no partner lies there
                     st_d[wave][k] = 0;
                 }
@@ -253,8 +256,9 @@ __device__ __forceinline__ void event_pairs_item(
                 if constexpr (S && LDS && sizeof(T) == 1) {
                     // Staged 8-bit counts into the LDS histogram: the four partners of a
                     // step come from one aligned 16-byte read of each staged array, from
-                    // j & ~3 (j < n <= ECAP, so the reads stay inside the wave's ECAP
-                    // entries).  With lo = max(w0, 1), a partner has t - (t_i + lo) in
+                    // j & ~3 (j <= n <= EMAX, so the reads stay inside the wave's n + 4
+                    // staged entries, the sentinels included, and a step that reaches
+                    // them ends the loop: no test of j against n).  With lo = max(w0, 1), a partner has t - (t_i + lo) in
                     // [0, span1); everything else lies outside as an unsigned value:
                     // entries before j lie below t_i + lo (times strictly increase; in the
                     // first window entry i itself sits at lag 0 < lo), and the entries
@@ -270,7 +274,7 @@ __device__ __forceinline__ void event_pairs_item(
                     H* const h1 = hist + (lo - w0);
                     const int4* t4p = reinterpret_cast<const int4*>(st_t[wave]);
                     const int4* d4p = reinterpret_cast<const int4*>(st_d[wave]);
-                    for (int j4 = j & ~3; j4 < n; j4 += 4) {
+                    for (int j4 = j & ~3;; j4 += 4) {
                         const int4 t4 = t4p[j4 >> 2];
                         const int4 d4 = d4p[j4 >> 2];
                         const int tt[4] = {t4.x, t4.y, t4.z, t4.w};
@@ -371,10 +375,9 @@ __global__ __launch_bounds__(64 * NW) void event_pairs_kernel(
     const BinDev* __restrict__ bins, const EventWork* __restrict__ work, int nframes,
     int64_t nnz, int64_t* __restrict__ num, int64_t* __restrict__ sums, int* __restrict__ err) {
     __shared__ typename EvHist<T>::type hist[EvHist<T>::lags_per_wave * NW];
-    __shared__ __attribute__((aligned(16))) int st_t[NW][ECAP];
-    __shared__ __attribute__((aligned(16))) int st_d[NW][ECAP];
+    __shared__ __attribute__((aligned(16))) int st[2][NW][ECAP];
     event_pairs_item<T, NW>(times, data, offsets, pix_idx, bins, work[blockIdx.x], nframes, nnz,
-                            num, sums, err, hist, st_t, st_d);
+                            num, sums, err, hist, st);
 }
 
 // A pointer read from a job table, marked as global memory (a generic pointer would
@@ -391,15 +394,14 @@ template <typename T, int NW>
 __global__ __launch_bounds__(64 * NW) void event_pairs_jobs_kernel(
     const EventJob* __restrict__ jobs, const EventWork* __restrict__ work) {
     __shared__ typename EvHist<T>::type hist[EvHist<T>::lags_per_wave * NW];
-    __shared__ __attribute__((aligned(16))) int st_t[NW][ECAP];
-    __shared__ __attribute__((aligned(16))) int st_d[NW][ECAP];
+    __shared__ __attribute__((aligned(16))) int st[2][NW][ECAP];
     const EventWork w = work[blockIdx.x];
     if (w.bin < 0) return;
     const EventJob j = jobs[w.pad];
     event_pairs_item<T, NW>(global_ptr(j.times), global_ptr(reinterpret_cast<const T*>(j.data)),
                             global_ptr(j.offsets), global_ptr(j.pix), global_ptr(j.bins), w,
                             j.nframes, j.nnz, global_ptr(j.num), global_ptr(j.sums),
-                            global_ptr(j.err), hist, st_t, st_d);
+                            global_ptr(j.err), hist, st);
 }
 
 // Waves per workgroup of the pair kernels: 16 (one workgroup per CU, 32,768 counters and

@@ -127,22 +127,25 @@ def test_events_scale_factor_and_int8_and_goldens(golden):
 def test_events_saturated_pixels_stay_exact(case):
     """Saturated counts on every pixel: 8-bit counts use 32-bit LDS lag counters while a
     wave's sum of d^2 stays within its bound, and send the pairs of later pixels straight
-    to the 64-bit numerators.  Pixels with 256 staged events (uint8 from a dense stack,
-    against the dense oracle; int8 event lists of both signs, which the compressor
-    would not produce, against the event oracle) and pixels with 4000 unstaged events
-    (uint8) pass that bound within one wave."""
+    to the 64-bit numerators.  Pixels of 252 events (the most a wave stages in LDS), 253
+    and 256 (read from HBM) in the same waves (uint8 from a dense stack, against the dense
+    oracle), pixels of 250, 251 and 252 staged events, so that the staged lists end at
+    every alignment of their four-entry steps (int8 event lists of both signs, which the
+    compressor would not produce, against the event oracle), and pixels with 4000
+    unstaged events (uint8) pass that bound within one wave."""
     _announce("events saturated %s" % case)
     from dynamix_amd.correlator.event import EventCorrelator
     rng = np.random.default_rng(31)
     if case == "i8_staged":
         shape, n = (32, 32), 300
         npx = shape[0] * shape[1]
-        times = np.concatenate([np.sort(rng.choice(n, size=256, replace=False))
-                                for _ in range(npx)]).astype(np.int32)
+        counts = np.array([250, 251, 252])[np.arange(npx) % 3]
+        times = np.concatenate([np.sort(rng.choice(n, size=k, replace=False))
+                                for k in counts]).astype(np.int32)
         data = rng.choice(np.array([-128, 127], dtype=np.int8), size=times.size)
-        offsets = (256 * np.arange(npx + 1)).astype(np.uint32)
+        offsets = np.concatenate([[0], np.cumsum(counts)]).astype(np.uint32)
         q = np.ones(shape, np.int32)
-        c = EventCorrelator(shape, n, 256, qmask=q, dtype=np.int8)
+        c = EventCorrelator(shape, n, 252, qmask=q, dtype=np.int8)
         g2 = c.correlate(times, data, offsets)
         _check(c, g2, ev.event_g2(times, data, offsets, q, n))
         return
@@ -153,7 +156,8 @@ def test_events_saturated_pixels_stay_exact(case):
         shape, n = (32, 32), 300
         f = np.zeros((n,) + shape, dtype=np.uint8)
         for p in range(shape[0] * shape[1]):
-            f[rng.choice(n, size=256, replace=False), p // shape[1], p % shape[1]] = 255
+            k = (252, 253, 256)[p % 3]
+            f[rng.choice(n, size=k, replace=False), p // shape[1], p % shape[1]] = 255
     q = np.ones(shape, np.int32)
     data, times, offsets = ev.compress_all_stack(f)
     c = EventCorrelator(shape, n, int(np.diff(offsets).max()), qmask=q, dtype=np.uint8)
