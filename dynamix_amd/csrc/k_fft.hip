@@ -522,16 +522,18 @@ __global__ __launch_bounds__(Plan::T) void fft_spectra_kernel(
         // W^(L-k) = -conj W^k, the same S and D give conj X[L-k] = S - D, so each pair
         // (k, L - k), k < L/2, is read and unpacked once (k = L/2 pairs with itself), and
         // |X|^2 accumulates at both ends.  With t < T only m = 0 meets k = 0, and only
-        // the last m can pass L/2: the other points need no test.
+        // the last m can pass L/2: the other points need no test.  The halves of S and D
+        // are left out here, and the accumulated |2X|^2 is scaled by 1/4 (exact) when it
+        // is flushed.
         auto unpack = [&](c32 a, c32 c, int k, double& lo, double* hi) {
             const c32 s2 = c32{a.x + c.x, a.y - c.y};
             const c32 d = c32{a.x - c.x, a.y + c.y};
             const c32 wd = cmul(c_of(uw[k]), d);
-            const float xr = 0.5f * (s2.x + wd.y), xi = 0.5f * (s2.y - wd.x);
-            lo += (double)xr * xr + (double)xi * xi;
+            const double xr = s2.x + wd.y, xi = s2.y - wd.x;  // 2 X[k], rounded in FP32
+            lo = fma(xi, xi, fma(xr, xr, lo));
             if (hi) {
-                const float yr = 0.5f * (s2.x - wd.y), yi = 0.5f * (s2.y + wd.x);
-                *hi += (double)yr * yr + (double)yi * yi;
+                const double yr = s2.x - wd.y, yi = s2.y + wd.x;  // 2 conj X[L-k]
+                *hi = fma(yi, yi, fma(yr, yr, *hi));
             }
         };
 #pragma unroll
@@ -554,11 +556,11 @@ __global__ __launch_bounds__(Plan::T) void fft_spectra_kernel(
     for (int m = 0; m < MH; ++m) {
         const int k = tid + m * T;
         if (k < HP) {
-            if (acc_lo[m] != 0.0) atomicAdd(prow + k, acc_lo[m]);
-            if (acc_hi[m] != 0.0) atomicAdd(prow + L - k, acc_hi[m]);
+            if (acc_lo[m] != 0.0) atomicAdd(prow + k, 0.25 * acc_lo[m]);
+            if (acc_hi[m] != 0.0) atomicAdd(prow + L - k, 0.25 * acc_hi[m]);
         }
     }
-    if (tid == 0 && acc_mid != 0.0) atomicAdd(prow + HP, acc_mid);
+    if (tid == 0 && acc_mid != 0.0) atomicAdd(prow + HP, 0.25 * acc_mid);
 }
 
 // ------------------------------------------------- long series: four-step through HBM
