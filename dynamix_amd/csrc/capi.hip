@@ -705,12 +705,29 @@ int ensure_fft_buffers(dxg2_plan* p, hipStream_t s) {
         std::copy(rest.begin(), rest.end(), out + quad.size());
         p->fft_qkb[b] = (int32_t)(quad.size() / KB);
     }
-    p->kb_gather = raster_order_for_xcds(
-        p->kb_work,
-        [&](const KbWork& w) {
-            return (int64_t)p->pix_fft[p->bins[w.bin].idx_off + (int64_t)w.kb * KB];
-        },
-        PM_XCD_GROUP, KbWork{0, -1});
+    // gather items: PM_KPB k-blocks consecutive in detector order (kb -1 past the end),
+    // dealt to the XCDs as units
+    {
+        std::vector<std::pair<int64_t, KbWork>> order;
+        for (const KbWork& w : p->kb_work)
+            order.push_back({(int64_t)p->pix_fft[p->bins[w.bin].idx_off + (int64_t)w.kb * KB], w});
+        std::stable_sort(order.begin(), order.end(),
+                         [](const std::pair<int64_t, KbWork>& a,
+                            const std::pair<int64_t, KbWork>& b) { return a.first < b.first; });
+        using Item = std::array<KbWork, PM_KPB>;
+        std::vector<Item> items;
+        for (size_t i = 0; i < order.size(); i += PM_KPB) {
+            Item it;
+            for (int j = 0; j < PM_KPB; ++j)
+                it[j] = i + j < order.size() ? order[i + j].second : KbWork{0, -1};
+            items.push_back(it);
+        }
+        Item pad;
+        pad.fill(KbWork{0, -1});
+        items = remap_for_xcds(items, PM_XCD_GROUP / PM_KPB, pad);
+        p->kb_gather.clear();
+        for (const Item& it : items) p->kb_gather.insert(p->kb_gather.end(), it.begin(), it.end());
+    }
     p->fft_chunk = L1 ? fft_chunk_kblocks(L, (int)p->kb_work.size()) : 0;
     // [0, L): exp(-2 pi i m / L), the transform's twiddles; [L, 2L]: exp(-pi i k / L),
     // k = 0..L, the factor that unpacks the half-length complex transform
@@ -1340,7 +1357,7 @@ int dxg2_plan_footprint(const dxg2_plan* p, int32_t path, size_t* bytes) {
                 L1 ? (size_t)fft_chunk_kblocks(L, (int)nkb) * KB * L * sizeof(float2) : 0;
             *bytes = common + p->slab_bytes + (2 * (size_t)L + 1) * sizeof(float2) + scratch +
                      (size_t)std::max(p->n_bins, 1) * ((L + 1) + p->N) * sizeof(double) +
-                     (2 * (size_t)nkb + N_XCD) * sizeof(KbWork) +
+                     (2 * (size_t)nkb + (size_t)N_XCD * PM_XCD_GROUP) * sizeof(KbWork) +
                      (p->pix_idx.size() + (size_t)std::max(p->n_bins, 1)) * sizeof(int32_t);
             return DXG2_OK;
         }
@@ -1995,7 +2012,7 @@ int dxg2_correlate_fft(dxg2_plan* p, const void* d_frames, double* d_num, int64_
     const bool quads = reinterpret_cast<uintptr_t>(d_frames) % 4 == 0;
     launch_gather_pm((const uint8_t*)d_frames, p->dtype, (int64_t)p->H * p->W, p->N, p->nrows,
                      p->d_pixfft, quads ? p->d_fftqkb : nullptr, p->d_bins, p->n_bins,
-                     p->d_kbgather, (int)p->kb_gather.size(), p->d_xs, sums, s);
+                     p->d_kbgather, (int)(p->kb_gather.size() / PM_KPB), p->d_xs, sums, s);
     if (ev) DXG2_HIP(hipEventRecord(ev[1], s));
     if (p->fft_L1) {
         const int total = (int)p->kb_work.size();

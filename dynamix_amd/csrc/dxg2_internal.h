@@ -53,6 +53,7 @@ constexpr int BAND_STAGE = 8;
 // dense gather items (up to 16 k-blocks each) and pixel-major gather k-blocks.
 constexpr int GATHER_XCD_GROUP = 16;
 constexpr int PM_XCD_GROUP = 128;
+constexpr int PM_KPB = 2;  // k-blocks per workgroup of the pixel-major gather
 constexpr int BAND_ROW_BYTES = 9728;
 constexpr int BAND_KMAX = 36;
 constexpr int BAND_ITEM_KMAX = 4 * BAND_KMAX;
@@ -231,8 +232,9 @@ void launch_finalize_f64(const int64_t* sums, const double* fnum, const BinDev* 
                          int nframes, int64_t* prefix, int64_t* smax, double* dpre, double* num,
                          double* den, float* g2, const DenMfma& dm, hipStream_t s);
 // Pixel-major copy xs of every bin's series (in the order of pix_idx) and the exact frame
-// sums (sums[b][t] for t < nframes, zeroed here first).  quad_kb (8-bit frames; may be
-// null): per bin, the leading k-blocks whose entries are 16 aligned pixel quads.
+// sums (sums[b][t] for t < nframes, zeroed here first).  work: n_work items of PM_KPB
+// k-blocks (kb -1: none).  quad_kb (8-bit frames; may be null): per bin, the leading
+// k-blocks whose entries are 16 aligned pixel quads.
 void launch_gather_pm(const uint8_t* frames, int dtype, int64_t hw, int nframes, int nrows,
                       const int32_t* pix_idx, const int32_t* quad_kb, const BinDev* bins,
                       int n_bins, const KbWork* work, int n_work, uint8_t* xs, int64_t* sums,

@@ -252,7 +252,10 @@ struct Elem<uint16_t> {
 // k = 4 (wave + 4 i) + kk as one aligned 4-byte load (a quarter of the load instructions),
 // and a 4 x 4 byte transpose in registers turns the four frames' quads into the four
 // pixels' words of the tile.
-template <typename E, bool SGN>
+// A workgroup takes KPB k-blocks that are neighbours in detector order (work item
+// blockIdx.x; kb -1: none) over 256 / KPB frames, so that the workgroups in flight read
+// a larger share of each frame row, and still stores whole 128-byte lines of each series.
+template <typename E, bool SGN, int KPB>
 __global__ __launch_bounds__(256) void gather_pm_kernel(const uint8_t* __restrict__ frames,
                                                         int64_t hw, int nframes, int nrows,
                                                         const int32_t* __restrict__ pix_idx,
@@ -261,79 +264,92 @@ __global__ __launch_bounds__(256) void gather_pm_kernel(const uint8_t* __restric
                                                         const KbWork* __restrict__ work,
                                                         uint8_t* __restrict__ xs,
                                                         int64_t* __restrict__ sums) {
+    constexpr int FT = 256 / KPB;      // frames per workgroup
     constexpr int ES = sizeof(E);
     constexpr int FPW = 4 / ES;        // frames per word
-    constexpr int WPR = 256 / FPW;     // words per pixel row of the tile
+    constexpr int WPR = FT / FPW;      // words per pixel row of the tile
     constexpr int STRIDE = WPR + 1;    // odd
-    constexpr int RG = 256 / WPR;      // row groups of the sums (4 or 2)
-    __shared__ __attribute__((aligned(16))) uint32_t tile[KB * STRIDE];
-    static_assert(RG * 256 <= KB * STRIDE, "partials fit the tile");
-    const KbWork w = work[blockIdx.x];
-    if (w.kb < 0) return;  // padding of the gather order
-    const BinDev bd = bins[w.bin];
+    constexpr int ROWS = KPB * KB;     // pixel rows of the tile
+    constexpr int RG = 256 / WPR;      // row groups of the sums (threads = WPR x RG)
+    constexpr int RPG = ROWS / RG;     // rows per group: within one k-block
+    static_assert(KB % RPG == 0 && FT * KPB == 256 && WPR % 16 == 0, "gather shape");
+    __shared__ __attribute__((aligned(16))) uint32_t tile[ROWS * STRIDE];
+    static_assert(RG * FT <= ROWS * STRIDE, "partials fit the tile");
+    const KbWork* item = work + (int64_t)blockIdx.x * KPB;
+    if (item[0].kb < 0) return;  // padding of the gather order (the first slot is filled first)
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const int t0 = blockIdx.y * 256;
-    const int p = pix_idx[bd.idx_off + (int64_t)w.kb * KB + lane];
-    const E* fr = reinterpret_cast<const E*>(frames) + (p >= 0 ? p : 0);
-    const int nt = p >= 0 ? max(0, min(256, nframes - t0)) : 0;  // frames present
+    const int t0 = blockIdx.y * FT;
+    const int ntq = max(0, min(FT, nframes - t0));  // frames present
     constexpr uint32_t MASK = ES == 1 ? 0xFFu : 0xFFFFu;
-    if (ES == 1 && quad_kb && w.kb < quad_kb[w.bin]) {  // the same for the whole workgroup
-        const int g = lane & 15, kk = lane >> 4;
-        const int q0 = pix_idx[bd.idx_off + (int64_t)w.kb * KB + 4 * g];  // 4 | q0
-        const int ntq = max(0, min(256, nframes - t0));
 #pragma unroll
-        for (int i = 0; i < WPR / 16; ++i) {
-            const int k = 4 * (wave + 4 * i) + kk;
-            uint32_t v[4];
+    for (int j = 0; j < KPB; ++j) {
+        const KbWork w = item[j];
+        if (w.kb < 0) continue;  // the same for the whole workgroup
+        const BinDev bd = bins[w.bin];
+        uint32_t* rows = tile + j * KB * STRIDE;
+        if (ES == 1 && quad_kb && w.kb < quad_kb[w.bin]) {
+            const int g = lane & 15, kk = lane >> 4;
+            const int q0 = pix_idx[bd.idx_off + (int64_t)w.kb * KB + 4 * g];  // 4 | q0
 #pragma unroll
-            for (int f4 = 0; f4 < 4; ++f4) {
-                const int f = 4 * k + f4;
-                v[f4] = f < ntq ? *reinterpret_cast<const uint32_t*>(
-                                      frames + (int64_t)(t0 + f) * hw + q0)
-                                : 0u;
+            for (int i = 0; i < WPR / 16; ++i) {
+                const int k = 4 * (wave + 4 * i) + kk;
+                uint32_t v[4];
+#pragma unroll
+                for (int f4 = 0; f4 < 4; ++f4) {
+                    const int f = 4 * k + f4;
+                    v[f4] = f < ntq ? *reinterpret_cast<const uint32_t*>(
+                                          frames + (int64_t)(t0 + f) * hw + q0)
+                                    : 0u;
+                }
+                // v[f] byte b = pixel 4g + b at frame 4k + f  ->  word k of pixel 4g + b
+                const uint32_t l01 = __builtin_amdgcn_perm(v[1], v[0], 0x05010400u);
+                const uint32_t h01 = __builtin_amdgcn_perm(v[1], v[0], 0x07030602u);
+                const uint32_t l23 = __builtin_amdgcn_perm(v[3], v[2], 0x05010400u);
+                const uint32_t h23 = __builtin_amdgcn_perm(v[3], v[2], 0x07030602u);
+                uint32_t* row = rows + (4 * g) * STRIDE + k;
+                row[0] = __builtin_amdgcn_perm(l23, l01, 0x05040100u);
+                row[STRIDE] = __builtin_amdgcn_perm(l23, l01, 0x07060302u);
+                row[2 * STRIDE] = __builtin_amdgcn_perm(h23, h01, 0x05040100u);
+                row[3 * STRIDE] = __builtin_amdgcn_perm(h23, h01, 0x07060302u);
             }
-            // v[f] byte b = pixel 4g + b at frame 4k + f  ->  word k of pixel 4g + b
-            const uint32_t l01 = __builtin_amdgcn_perm(v[1], v[0], 0x05010400u);
-            const uint32_t h01 = __builtin_amdgcn_perm(v[1], v[0], 0x07030602u);
-            const uint32_t l23 = __builtin_amdgcn_perm(v[3], v[2], 0x05010400u);
-            const uint32_t h23 = __builtin_amdgcn_perm(v[3], v[2], 0x07030602u);
-            uint32_t* row = tile + (4 * g) * STRIDE + k;
-            row[0] = __builtin_amdgcn_perm(l23, l01, 0x05040100u);
-            row[STRIDE] = __builtin_amdgcn_perm(l23, l01, 0x07060302u);
-            row[2 * STRIDE] = __builtin_amdgcn_perm(h23, h01, 0x05040100u);
-            row[3 * STRIDE] = __builtin_amdgcn_perm(h23, h01, 0x07060302u);
-        }
-    } else {
+        } else {
+            const int p = pix_idx[bd.idx_off + (int64_t)w.kb * KB + lane];
+            const E* fr = reinterpret_cast<const E*>(frames) + (p >= 0 ? p : 0);
+            const int nt = p >= 0 ? ntq : 0;
 #pragma unroll
-        for (int i = 0; i < WPR / 4; ++i) {
-            const int k = wave + 4 * i;  // word k: frames t0 + FPW k ..
-            uint32_t word = 0u;
+            for (int i = 0; i < WPR / 4; ++i) {
+                const int k = wave + 4 * i;  // word k: frames t0 + FPW k ..
+                uint32_t word = 0u;
 #pragma unroll
-            for (int q = 0; q < FPW; ++q) {
-                const int f = FPW * k + q;
-                const uint32_t v = f < nt ? (uint32_t)fr[(int64_t)(t0 + f) * hw] : 0u;
-                word |= (v & MASK) << (8 * ES * q);
+                for (int q = 0; q < FPW; ++q) {
+                    const int f = FPW * k + q;
+                    const uint32_t v = f < nt ? (uint32_t)fr[(int64_t)(t0 + f) * hw] : 0u;
+                    word |= (v & MASK) << (8 * ES * q);
+                }
+                rows[lane * STRIDE + k] = word;
             }
-            tile[lane * STRIDE + k] = word;
         }
     }
     __syncthreads();
     constexpr int CHUNKS = WPR / 4;  // 16-byte pieces per pixel row
-    for (int c = threadIdx.x; c < KB * CHUNKS; c += 256) {
-        const int j = c / CHUNKS;
+    for (int c = threadIdx.x; c < ROWS * CHUNKS; c += 256) {
+        const int r = c / CHUNKS;
+        const KbWork w = item[r / KB];
+        if (w.kb < 0) continue;
         const int o = (c % CHUNKS) * 4;
-        const uint32_t* r = tile + j * STRIDE + o;
-        uint8_t* dst = xs + bd.slab_off + (((int64_t)w.kb * KB + j) * nrows + t0) * ES + 4 * o;
-        *reinterpret_cast<uint4*>(dst) = make_uint4(r[0], r[1], r[2], r[3]);
+        const uint32_t* src = tile + r * STRIDE + o;
+        uint8_t* dst = xs + bins[w.bin].slab_off +
+                       (((int64_t)w.kb * KB + r % KB) * nrows + t0) * ES + 4 * o;
+        *reinterpret_cast<uint4*>(dst) = make_uint4(src[0], src[1], src[2], src[3]);
     }
-    const int k = threadIdx.x % WPR, g = threadIdx.x / WPR;  // word k, rows KB/RG g ..
-    const uint32_t* col = tile + (KB / RG) * g * STRIDE + k;
+    const int k = threadIdx.x % WPR, g = threadIdx.x / WPR;  // word k, rows RPG g ..
+    const uint32_t* col = tile + RPG * g * STRIDE + k;
     uint32_t s0 = 0u, s1 = 0u;
     if constexpr (ES == 1) {
         // s0: frames 4k, 4k + 2 and s1: 4k + 1, 4k + 3 in 16-bit halves (16 rows x 255)
 #pragma unroll
-        for (int j = 0; j < KB / RG; ++j) {
+        for (int j = 0; j < RPG; ++j) {
             const uint32_t x = col[j * STRIDE] ^ (SGN ? 0x80808080u : 0u);
             s0 += x & 0x00FF00FFu;
             s1 += (x >> 8) & 0x00FF00FFu;
@@ -341,26 +357,31 @@ __global__ __launch_bounds__(256) void gather_pm_kernel(const uint8_t* __restric
     } else {
         // s0: frame 2k, s1: 2k + 1 (32 rows x 65535)
 #pragma unroll 8
-        for (int j = 0; j < KB / RG; ++j) {
+        for (int j = 0; j < RPG; ++j) {
             const uint32_t x = col[j * STRIDE];
             s0 += x & 0xFFFFu;
             s1 += x >> 16;
         }
     }
+    static_assert(ES != 1 || RPG <= 256, "16-bit halves");
     __syncthreads();  // every thread has read the tile: its words now hold the partials
-    int32_t* part = reinterpret_cast<int32_t*>(tile);  // [RG][256]
+    int32_t* part = reinterpret_cast<int32_t*>(tile);  // [RG][FT]
     if constexpr (ES == 1)
-        *reinterpret_cast<int4*>(part + 256 * g + 4 * k) =
+        *reinterpret_cast<int4*>(part + FT * g + 4 * k) =
             make_int4((int)(s0 & 0xFFFFu), (int)(s1 & 0xFFFFu), (int)(s0 >> 16),
                       (int)(s1 >> 16));
     else
-        *reinterpret_cast<int2*>(part + 256 * g + 2 * k) = make_int2((int)s0, (int)s1);
+        *reinterpret_cast<int2*>(part + FT * g + 2 * k) = make_int2((int)s0, (int)s1);
     __syncthreads();
-    const int t = t0 + threadIdx.x;
-    if (t < nframes) {
+    // thread (j, f): frame t0 + f of k-block j, from the RG / KPB row groups of that k-block
+    const int jb = threadIdx.x / FT, f = threadIdx.x % FT;
+    const KbWork w = item[jb];
+    const int t = t0 + f;
+    if (w.kb >= 0 && t < nframes) {
+        constexpr int GPK = RG / KPB;  // row groups per k-block
         int64_t tot = SGN ? -(int64_t)KB * 128 : 0;
 #pragma unroll
-        for (int q = 0; q < RG; ++q) tot += part[256 * q + threadIdx.x];
+        for (int q = 0; q < GPK; ++q) tot += part[FT * (GPK * jb + q) + f];
         if (tot)
             atomicAdd(reinterpret_cast<unsigned long long*>(sums + (int64_t)w.bin * nframes + t),
                       (unsigned long long)tot);
@@ -843,8 +864,10 @@ static void gather_pm_typed(const uint8_t* frames, int64_t hw, int nframes, int 
                             const int32_t* pix_idx, const int32_t* quad_kb, const BinDev* bins,
                             const KbWork* work, int n_work, uint8_t* xs, int64_t* sums,
                             hipStream_t s) {
-    gather_pm_kernel<E, SGN><<<dim3((unsigned)n_work, (unsigned)(nrows / 256)), 256, 0, s>>>(
-        frames, hw, nframes, nrows, pix_idx, quad_kb, bins, work, xs, sums);
+    constexpr int FT = 256 / PM_KPB;  // nrows is a multiple of 256
+    gather_pm_kernel<E, SGN, PM_KPB>
+        <<<dim3((unsigned)n_work, (unsigned)(nrows / FT)), 256, 0, s>>>(
+            frames, hw, nframes, nrows, pix_idx, quad_kb, bins, work, xs, sums);
 }
 
 void launch_gather_pm(const uint8_t* frames, int dtype, int64_t hw, int nframes, int nrows,
